@@ -1,0 +1,128 @@
+/*
+ * hip_diag.h -- C ABI of the MI355X-native rlm/hlm solver (librlm_hip.so).
+ *
+ * Drop-in boundary for qiongz/Random-Landau-Model's diag_wrappers plugin layer
+ * and the rlm/hlm hot path. Every entry point takes plain pointers and sizes.
+ * Complex arrays are interleaved (re, im) doubles, i.e. std::complex<double> /
+ * hipDoubleComplex layout. "d_" pointers are device (HBM) memory; the others
+ * are host memory.
+ *
+ * Status: 0 = OK; > 0 = solver info (as LAPACK's info); < 0 = error
+ * (-22 invalid argument, -12 out of memory, other negatives = -hipError_t).
+ * Every entry point is synchronous unless it names a stream.
+ */
+#ifndef HIP_DIAG_H
+#define HIP_DIAG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision modes of the batched solver. */
+#define HIP_DIAG_FP64 0
+
+/* ------------------------------------------------------------------------ */
+/* diag_wrappers-level entry points                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces cusolver_diag_hamil (diag_wrappers/cusolver_diag.h:6,
+ * diag_wrappers/cusolver_diag.cu:3-22) and magma_diag_hamil
+ * (diag_wrappers/magma_diag.h:9): Hermitian eigen-decomposition in place of a
+ * device n x n complex128 column-major matrix (lda = n), lower triangle read,
+ * eigenvalues ascending into device d_w, eigenvectors overwrite d_A (column j
+ * = state j). FP64 (the reference wrappers are FP32). */
+int hip_diag_hamil(double *d_A, double *d_w, int n);
+
+/* The same with the reference's void signature, for call sites that expect it
+ * (rlm/main_rlm_gpu.cu:176,216). Errors are printed to stderr. */
+void hip_diag_hamil_v(double *d_A, double *d_w, int n);
+
+/* Replaces mkl_heevd (diag_wrappers/mkl_diag.h:13, mkl_diag.cpp:2-9):
+ * LAPACKE_zheevd(COL_MAJOR, 'V', 'U', n, A, n, w) on host memory, computed on
+ * the GPU. */
+int hip_heevd(double *A, double *w, int n);
+
+/* Batched form of the two above (no reference counterpart: the reference calls
+ * its wrappers once per k-point, rlm/main_rlm_gpu.cu:198-229). `batch`
+ * matrices at d_A + 2*b*strideA (lda >= n), uplo 'L' or 'U' names the triangle
+ * read; eigenvectors overwrite the matrices, eigenvalues go to
+ * d_W + b*strideW. stream: a hipStream_t or NULL for the default stream. */
+int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA, double *d_W, long strideW, int batch,
+                           char uplo, int precision, void *stream);
+
+/* Replaces cal_Chern (diag_wrappers/chern.h:29, chern.cpp:3-35): for one
+ * theta_1 line, chern_theta[m + n*n_mesh] = -arg(prod of the 4 link overlaps
+ * of plaquette m for state n)/(2 pi), reading the 2-slot host ring buffer
+ * wfs_full[(slot*(n_mesh+1) + theta_2)*n_phi*dim_vec + state*dim_vec + i]. */
+int hip_cal_chern(const double *wfs_full, double *chern_theta, int n_phi, int dim_vec, int n_mesh, int theta_1);
+
+/* ------------------------------------------------------------------------ */
+/* rlm model entry points                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces prepare_coeff (rlm/matrix_coefficients.h:139-140,
+ * matrix_coefficients.cpp:3-25), double-precision build: host tables
+ * coeff_mn[dim_m*dim_n], coeff_m_theta[(n_mesh+1)*dim_m], coeff_jm[n_phi*dim_m]
+ * (complex), including the single-precision expf and 1.0f/n_phi of :11. */
+int rlm_prepare_coeff(double *coeff_mn, double *coeff_m_theta, double *coeff_jm, int n_phi, int off_head, int dim_m,
+                      int dim_n, int n_mesh, double L1, double L2);
+
+/* Replaces generate_disorder_potential (rlm/disorder_potential.h:9,
+ * disorder_potential.cpp:3-21): MKL-compatible MT19937 stream of `seed`
+ * (init_by_array), skip 3*impurity_num*sample_offset words, x and y uniform
+ * on [0,L1) and [0,L2), intensity +1 for an even word and -1 for an odd one. */
+int rlm_generate_disorder_potential(double *x, double *y, double *intensity, long impurity_num, double L1, double L2,
+                                    unsigned long seed, long sample_offset);
+
+/* Replaces prepare_potential_coeff (rlm/potential_coeff.h,
+ * potential_coeff.cu:21-26) in FP64 with the CPU build's sign convention
+ * (get_potential_coeff, matrix_coefficients.cpp:27-44):
+ * d_Vmn[m*dim_n+n] = coeff_mn * sum_i s_i e^{-i 2pi((m-o)x_i + (n-o)y_i)}. */
+int hip_prepare_potential_coeff(const double *d_x, const double *d_y, const double *d_intensity, double *d_Vmn,
+                                const double *d_coeff_mn, int dim_m, int dim_n, int off_head, long impurity_num);
+
+/* Replaces set_hamil_matrix (rlm/hamiltonian_rlm_gpu.h,
+ * hamiltonian_rlm_gpu.cu:43-50) in FP64 with the CPU build's conventions
+ * (hamiltonian_rlm_cpu.cpp:14-48): d_wfs receives the full Hermitian n_phi x
+ * n_phi matrix, column-major, exactly the matrix LAPACK sees in the CPU build
+ * (A(r,c) = conj(h_rc) for c >= r). Rows of coeff_m_theta are the theta_1 and
+ * theta_2 rows; d_coeff_jm is the n_phi x dim_n table. */
+int hip_set_hamil_matrix(const double *d_coeff_m_theta_1, const double *d_coeff_m_theta_2, const double *d_coeff_jm,
+                         const double *d_Vmn, double *d_wfs, int n_phi, int dim_n, int off_head);
+
+/* ------------------------------------------------------------------------ */
+/* Batched pipelines                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces the sample / theta_1 / theta_2 loops of rlm/main_rlm_cpu.cpp:109-243
+ * (and rlm/main_rlm_gpu.cu:138-300): for each disorder sample, assemble,
+ * diagonalise and Chern-reduce the whole (n_mesh+1)^2 twist mesh on the GPU.
+ * off_override < 0 keeps the reference's off_head = (long)(sqrt(N_imp)/2+1). */
+typedef struct rlm_plan_s *rlm_plan_t;
+int rlm_plan_create(rlm_plan_t *plan, long n_phi, double quanta_concentration, double impurity_concentration,
+                    long n_mesh, long off_override, unsigned long seed, int samples_per_batch, int precision);
+/* Generates samples first..first+count-1 (count <= samples_per_batch) on the
+ * host and uploads them. */
+int rlm_plan_load(rlm_plan_t plan, long first_sample, int count);
+/* Runs every GPU stage for the loaded samples on the plan's stream. */
+int rlm_plan_execute(rlm_plan_t plan);
+/* Copies energies / Chern values / plaquette margins ([count][n_phi], host)
+ * after synchronising the stream. margin may be NULL. */
+int rlm_plan_fetch(rlm_plan_t plan, double *energy, double *chern, double *margin);
+/* load + execute + fetch over any number of samples, batch by batch. */
+int rlm_plan_run(rlm_plan_t plan, long first_sample, long n_samples, double *energy, double *chern, double *margin);
+/* The plan's hipStream_t. */
+void *rlm_plan_stream(rlm_plan_t plan);
+/* Sizes derived as rlm/main_rlm_cpu.cpp:29-34: out[0]=n_phi, [1]=impurity_num,
+ * [2]=off_head, [3]=dim_m, [4]=n_mesh, [5]=samples_per_batch. */
+int rlm_plan_info(rlm_plan_t plan, long *out6);
+/* Device pointer of the eigenvalues [kp][n_phi] / eigenvectors
+ * [kp][component][state] of the last execute (kp = (s*(g+1)+t1)*(g+1)+t2). */
+const double *rlm_plan_eigenvalues(rlm_plan_t plan);
+const double *rlm_plan_eigenvectors(rlm_plan_t plan);
+int rlm_plan_destroy(rlm_plan_t plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIP_DIAG_H */

@@ -1,0 +1,453 @@
+// C ABI of librlm_hip.so: diag_wrappers-level entry points, rlm model entry
+// points and the batched rlm pipeline. Declarations and reference citations are
+// in include/hip_diag.h.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/hip_diag.h"
+#include "chern.hpp"
+#include "common.hpp"
+#include "eig.hpp"
+#include "host_rng.hpp"
+#include "rlm_kernels.hpp"
+
+using namespace rlm;
+
+namespace {
+
+constexpr int kGridTrd = 1024;   // persistent workgroups for the tridiagonalisation
+constexpr int kGridTail = 512;   // persistent workgroups for D&C + back-transform
+
+// ---------------------------------------------------------------- small kernels
+// Fill the triangle opposite to `uplo` by conjugate mirroring; real diagonal.
+__global__ void hermitize_kernel(double *A, long lda, long strideA, int n, int from_lower) {
+    double *M = A + 2 * blockIdx.y * strideA;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * n) return;
+    const int c = (int)(idx / n), r = (int)(idx % n);
+    if (r == c) {
+        M[2 * ((long)c * lda + r) + 1] = 0.0;
+        return;
+    }
+    const bool dst = from_lower ? (r < c) : (r > c);
+    if (!dst) return;
+    const cd v = ld(M, (long)r * lda + c);  // (c, r) element
+    st(M, (long)c * lda + r, cconj(v));
+}
+
+// A(r, j) = U[r*n + j] (component-major -> column-major eigenvectors)
+__global__ void transpose_out_kernel(const double *U, long strideU, int n, double *A, long lda, long strideA) {
+    const double *Ub = U + 2 * blockIdx.y * strideU;
+    double *M = A + 2 * blockIdx.y * strideA;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * n) return;
+    const int j = (int)(idx / n), r = (int)(idx % n);
+    st(M, (long)j * lda + r, ld(Ub, (long)r * n + j));
+}
+
+// Full-matrix copy with lda (column-major) -> packed lda = n
+__global__ void pack_kernel(const double *A, long lda, long strideA, int n, double *P) {
+    const double *M = A + 2 * blockIdx.y * strideA;
+    double *Q = P + 2 * (long)blockIdx.y * n * n;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * n) return;
+    const int c = (int)(idx / n), r = (int)(idx % n);
+    st(Q, idx, ld(M, (long)c * lda + r));
+}
+
+// ---------------------------------------------------------------- device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b) {
+        if (b <= bytes) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipMalloc(&p, b) != hipSuccess) return -12;
+        bytes = b;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    double *d() { return static_cast<double *>(p); }
+};
+
+// Workspace of the batched eigensolver.
+struct EigWork {
+    DevBuf dev, U, work;
+    int ensure(int n, long batch, bool with_u = true) {
+        int rc;
+        if ((rc = dev.ensure(sizeof(double) * 4 * (size_t)n * batch))) return rc;
+        if (with_u && (rc = U.ensure(sizeof(double) * 2 * (size_t)n * n * batch))) return rc;
+        const long g = batch < kGridTail ? batch : kGridTail;
+        if ((rc = work.ensure(sizeof(double) * 3 * (size_t)n * n * g))) return rc;
+        return 0;
+    }
+    void release() {
+        dev.release();
+        U.release();
+        work.release();
+    }
+};
+
+// Eigen-decomposition of `batch` packed (lda = n, stride n*n) full Hermitian
+// matrices in A. Eigenvalues -> W (stride n), eigenvectors component-major -> Uout.
+int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &ws, hipStream_t s) {
+    int rc;
+    const long strideV = 4L * n;   // d | e | tau(re,im) per matrix, interleaved blocks
+    double *base = ws.dev.d();
+    // layout per matrix: D at base + b*strideV, E at +n, TAU at +2n (complex, 2n doubles)
+    // launch_* take separate arrays with a common stride:
+    double *D = base, *E = base + n, *TAU = base + 2 * n;
+    // TAU is read with a complex stride of strideV doubles -> pass complex-stride strideV/2
+    const int gt = (int)(batch < kGridTrd ? batch : kGridTrd);
+    if ((rc = launch_zhetrd(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, gt, s))) return rc;
+    const int gl = (int)(batch < kGridTail ? batch : kGridTail);
+    if ((rc = launch_tail(A, (long)n * n, n, n, D, E, TAU, strideV, W, n, Uout, (long)n * n, ws.work.d(), (int)batch,
+                          gl, s)))
+        return rc;
+    return 0;
+}
+
+std::mutex g_single_mu;
+EigWork g_single_ws;
+DevBuf g_single_pack;
+
+}  // namespace
+
+// ============================================================================
+// diag_wrappers-level
+// ============================================================================
+extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA, double *d_W, long strideW,
+                                      int batch, char uplo, int precision, void *stream) {
+    if (n < 1 || n > 512 || lda < n || batch < 0 || strideW < n || strideA < lda * (long)n) return -22;
+    if (precision != HIP_DIAG_FP64) return -22;
+    if (batch == 0) return 0;
+    if (uplo != 'L' && uplo != 'U' && uplo != 'l' && uplo != 'u') return -22;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_single_mu);
+    int rc;
+    if ((rc = g_single_ws.ensure(n, batch))) return rc;
+    if ((rc = g_single_pack.ensure(sizeof(double) * 2 * (size_t)n * n * batch + sizeof(double) * (size_t)n * batch)))
+        return rc;
+    double *P = g_single_pack.d();
+    double *Wp = P + 2L * n * n * batch;
+    const long nn = (long)n * n;
+    dim3 grid((unsigned)((nn + 255) / 256), (unsigned)batch);
+    hipLaunchKernelGGL(pack_kernel, grid, dim3(256), 0, s, d_A, lda, strideA, n, P);
+    hipLaunchKernelGGL(hermitize_kernel, grid, dim3(256), 0, s, P, (long)n, nn, n,
+                       (uplo == 'L' || uplo == 'l') ? 1 : 0);
+    RLM_HIP_CHECK(hipGetLastError());
+    if ((rc = eig_packed(P, n, batch, Wp, g_single_ws.U.d(), g_single_ws, s))) return rc;
+    hipLaunchKernelGGL(transpose_out_kernel, grid, dim3(256), 0, s, g_single_ws.U.d(), nn, n, d_A, lda, strideA);
+    RLM_HIP_CHECK(hipGetLastError());
+    RLM_HIP_CHECK(hipMemcpy2DAsync(d_W, sizeof(double) * strideW, Wp, sizeof(double) * n, sizeof(double) * n, batch,
+                                   hipMemcpyDeviceToDevice, s));
+    RLM_HIP_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int hip_diag_hamil(double *d_A, double *d_w, int n) {
+    return hip_diag_heevd_batched(n, d_A, n, (long)n * n, d_w, n, 1, 'L', HIP_DIAG_FP64, nullptr);
+}
+
+extern "C" void hip_diag_hamil_v(double *d_A, double *d_w, int n) {
+    int rc = hip_diag_hamil(d_A, d_w, n);
+    if (rc) fprintf(stderr, "[rlm_hip] hip_diag_hamil failed with status %d\n", rc);
+}
+
+extern "C" int hip_heevd(double *A, double *w, int n) {
+    if (n < 1 || !A || !w) return -22;
+    double *dA = nullptr, *dw = nullptr;
+    const size_t bA = sizeof(double) * 2 * (size_t)n * n;
+    if (hipMalloc(&dA, bA) != hipSuccess) return -12;
+    if (hipMalloc(&dw, sizeof(double) * n) != hipSuccess) {
+        (void)hipFree(dA);
+        return -12;
+    }
+    int rc = 0;
+    if (hipMemcpy(dA, A, bA, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    if (!rc) rc = hip_diag_heevd_batched(n, dA, n, (long)n * n, dw, n, 1, 'U', HIP_DIAG_FP64, nullptr);
+    if (!rc && hipMemcpy(A, dA, bA, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+    if (!rc && hipMemcpy(w, dw, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+    (void)hipFree(dA);
+    (void)hipFree(dw);
+    return rc;
+}
+
+extern "C" int hip_cal_chern(const double *wfs_full, double *chern_theta, int n_phi, int dim_vec, int n_mesh,
+                             int theta_1) {
+    if (!wfs_full || !chern_theta || n_phi < 1 || dim_vec < 1 || n_mesh < 1 || theta_1 < 1) return -22;
+    // Ring slots: slot (theta_1-1)%2 holds line theta_1-1, slot theta_1%2 holds line theta_1.
+    // Re-pack the two lines as component-major twist points kp = t*(g+1)+t2 (t = 0: line theta_1-1).
+    const int g1 = n_mesh + 1;
+    const long per = (long)n_phi * dim_vec;
+    std::vector<double> host(2 * 2 * g1 * per);
+    const int slot_prev = (theta_1 - 1) % 2, slot_cur = theta_1 % 2;
+    for (int t = 0; t < 2; t++) {
+        const int slot = t == 0 ? slot_prev : slot_cur;
+        for (int t2 = 0; t2 < g1; t2++) {
+            const double *src = wfs_full + 2 * ((long)slot * g1 + t2) * per;
+            double *dst = host.data() + 2 * ((long)t * g1 + t2) * per;
+            for (int st_ = 0; st_ < n_phi; st_++)
+                for (int i = 0; i < dim_vec; i++) {
+                    dst[2 * ((long)i * n_phi + st_)] = src[2 * ((long)st_ * dim_vec + i)];
+                    dst[2 * ((long)i * n_phi + st_) + 1] = src[2 * ((long)st_ * dim_vec + i) + 1];
+                }
+        }
+    }
+    double *dU = nullptr, *dL = nullptr;
+    const long nkp = 2L * g1;
+    int rc = 0;
+    if (hipMalloc(&dU, sizeof(double) * host.size()) != hipSuccess) return -12;
+    if (hipMalloc(&dL, sizeof(double) * 4 * nkp * n_phi) != hipSuccess) rc = -12;
+    if (!rc && hipMemcpy(dU, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    if (!rc) rc = launch_links(dU, per, dim_vec, n_phi, 2, g1, n_mesh, 0, nkp, dL, dL + 2 * nkp * n_phi, nullptr);
+    // The per-plaquette phases of one line are 4 products per (m, state): finish on the host.
+    std::vector<double> L(4 * nkp * n_phi);
+    if (!rc && hipMemcpy(L.data(), dL, sizeof(double) * L.size(), hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+    if (!rc) {
+        const double *L1 = L.data(), *L2 = L.data() + 2 * nkp * n_phi;
+        for (int m = 0; m < n_mesh; m++) {
+            const long k00 = m, k10 = g1 + m, k01 = m + 1;
+            for (int j = 0; j < n_phi; j++) {
+                cd p{1.0, 0.0};
+                p = cmul(p, cd{L1[2 * (k00 * n_phi + j)], L1[2 * (k00 * n_phi + j) + 1]});
+                p = cmul(p, cd{L2[2 * (k10 * n_phi + j)], L2[2 * (k10 * n_phi + j) + 1]});
+                p = cmul(p, cconj(cd{L1[2 * (k01 * n_phi + j)], L1[2 * (k01 * n_phi + j) + 1]}));
+                p = cmul(p, cconj(cd{L2[2 * (k00 * n_phi + j)], L2[2 * (k00 * n_phi + j) + 1]}));
+                chern_theta[m + (long)j * n_mesh] = -atan2(p.y, p.x) / (M_PI * 2.0);
+            }
+        }
+    }
+    (void)hipFree(dU);
+    (void)hipFree(dL);
+    return rc;
+}
+
+// ============================================================================
+// rlm model entry points
+// ============================================================================
+extern "C" int rlm_prepare_coeff(double *coeff_mn, double *coeff_m_theta, double *coeff_jm, int n_phi, int off_head,
+                                 int dim_m, int dim_n, int n_mesh, double L1, double L2) {
+    if (!coeff_mn || !coeff_m_theta || !coeff_jm || n_phi < 1 || n_mesh < 1 || dim_m != dim_n) return -22;
+    host_prepare_coeff(coeff_mn, coeff_m_theta, coeff_jm, n_phi, off_head, dim_m, n_mesh, L1, L2);
+    return 0;
+}
+
+extern "C" int rlm_generate_disorder_potential(double *x, double *y, double *intensity, long impurity_num, double L1,
+                                               double L2, unsigned long seed, long sample_offset) {
+    if (!x || !y || !intensity || impurity_num < 0 || sample_offset < 0) return -22;
+    host_rlm_disorder(x, y, intensity, impurity_num, L1, L2, (uint32_t)seed, sample_offset);
+    return 0;
+}
+
+extern "C" int hip_prepare_potential_coeff(const double *d_x, const double *d_y, const double *d_intensity,
+                                           double *d_Vmn, const double *d_coeff_mn, int dim_m, int dim_n,
+                                           int off_head, long impurity_num) {
+    if (dim_m != dim_n || dim_m < 1 || impurity_num < 0) return -22;
+    int rc = launch_potential(d_x, d_y, d_intensity, impurity_num, d_coeff_mn, dim_m, off_head, d_Vmn, 1, nullptr);
+    if (rc) return rc;
+    RLM_HIP_CHECK(hipDeviceSynchronize());
+    return 0;
+}
+
+extern "C" int hip_set_hamil_matrix(const double *d_coeff_m_theta_1, const double *d_coeff_m_theta_2,
+                                    const double *d_coeff_jm, const double *d_Vmn, double *d_wfs, int n_phi,
+                                    int dim_n, int off_head) {
+    if (n_phi < 1 || dim_n != 2 * off_head + 1) return -22;
+    // Build a 2-row coeff_m_theta table (rows: theta_1, theta_2) and reuse the batched kernels
+    // with n_mesh = 1, theta_1 = 0 row, theta_2 = 1 row.
+    double *cmt = nullptr, *E = nullptr, *Wt = nullptr;
+    const int d = dim_n;
+    int rc = 0;
+    if (hipMalloc(&cmt, sizeof(double) * 4 * d) != hipSuccess) return -12;
+    if (hipMalloc(&E, sizeof(double) * 2 * 2L * d * n_phi) != hipSuccess) rc = -12;
+    if (!rc && hipMalloc(&Wt, sizeof(double) * 2 * 2L * d * n_phi) != hipSuccess) rc = -12;
+    if (!rc && hipMemcpy(cmt, d_coeff_m_theta_1, sizeof(double) * 2 * d, hipMemcpyDeviceToDevice) != hipSuccess)
+        rc = -1;
+    if (!rc && hipMemcpy(cmt + 2 * d, d_coeff_m_theta_2, sizeof(double) * 2 * d, hipMemcpyDeviceToDevice) !=
+                   hipSuccess)
+        rc = -1;
+    if (!rc) rc = launch_etable(cmt, d_coeff_jm, n_phi, d, 1, E, nullptr);
+    if (!rc) rc = launch_wtable(d_Vmn, cmt, n_phi, d, off_head, 1, 1, Wt, nullptr);
+    // k-point (t1 = 0, t2 = 1) of a 2x2 mesh is kp = 0*2 + 1
+    if (!rc) rc = launch_assemble(E, Wt, n_phi, d, off_head, 1, 1, 1, d_wfs, (long)n_phi * n_phi, nullptr);
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = -1;
+    (void)hipFree(cmt);
+    (void)hipFree(E);
+    (void)hipFree(Wt);
+    return rc;
+}
+
+// ============================================================================
+// rlm batched pipeline
+// ============================================================================
+struct rlm_plan_s {
+    long n_phi, n_imp, off, d, g, spb;
+    double L1, L2;
+    uint32_t seed;
+    int precision;
+    hipStream_t stream = nullptr;
+    // host
+    std::vector<double> cmn, cmt, cjm, hx, hy, hs;
+    // device
+    DevBuf dcmn, dcmt, dE, dX, dY, dS, dV, dW, dA, dU, dEig, dL, dOut;
+    EigWork ws;
+    int loaded = 0;
+    long first = 0;
+};
+
+extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double imp, long n_mesh, long off_override,
+                               unsigned long seed, int samples_per_batch, int precision) {
+    if (!plan || n_phi < 1 || n_phi > 512 || n_mesh < 1 || samples_per_batch < 1 || q <= 0.0) return -22;
+    if (precision != HIP_DIAG_FP64) return -22;
+    auto *p = new rlm_plan_s();
+    p->n_phi = n_phi;
+    p->L1 = p->L2 = std::sqrt((double)n_phi * q);                     // rlm/main_rlm_cpu.cpp:29-30
+    p->n_imp = (long)(imp * (double)n_phi / q);                       // :31
+    p->off = off_override >= 0 ? off_override : (long)(std::sqrt((double)p->n_imp) / 2 + 1);  // :32
+    p->d = 2 * p->off + 1;                                            // :33
+    p->g = n_mesh;
+    p->spb = samples_per_batch;
+    p->seed = (uint32_t)seed;
+    p->precision = precision;
+    const long N = n_phi, d = p->d, g1 = n_mesh + 1, S = samples_per_batch;
+    const long nkp = S * g1 * g1;
+    p->cmn.resize(2 * d * d);
+    p->cmt.resize(2 * g1 * d);
+    p->cjm.resize(2 * N * d);
+    host_prepare_coeff(p->cmn.data(), p->cmt.data(), p->cjm.data(), (int)N, (int)p->off, (int)d, (int)n_mesh, p->L1,
+                       p->L2);
+    int rc = 0;
+    auto fail = [&](int r) {
+        rlm_plan_destroy(p);
+        return r;
+    };
+    if (hipStreamCreate(&p->stream) != hipSuccess) return fail(-1);
+    if ((rc = p->dcmn.ensure(8 * p->cmn.size())) || (rc = p->dcmt.ensure(8 * p->cmt.size())) ||
+        (rc = p->dE.ensure(16 * g1 * d * N)) || (rc = p->dX.ensure(8 * S * p->n_imp + 8)) ||
+        (rc = p->dY.ensure(8 * S * p->n_imp + 8)) || (rc = p->dS.ensure(8 * S * p->n_imp + 8)) ||
+        (rc = p->dV.ensure(16 * S * d * d)) || (rc = p->dW.ensure(16 * S * g1 * d * N)) ||
+        (rc = p->dA.ensure(16 * nkp * N * N)) || (rc = p->dU.ensure(16 * nkp * N * N)) ||
+        (rc = p->dEig.ensure(8 * nkp * N)) || (rc = p->dL.ensure(32 * nkp * N)) ||
+        (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)))
+        return fail(rc);
+    if (hipMemcpy(p->dcmn.p, p->cmn.data(), 8 * p->cmn.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->dcmt.p, p->cmt.data(), 8 * p->cmt.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(-1);
+    {
+        DevBuf dcjm;
+        if ((rc = dcjm.ensure(8 * p->cjm.size()))) return fail(rc);
+        if (hipMemcpy(dcjm.p, p->cjm.data(), 8 * p->cjm.size(), hipMemcpyHostToDevice) != hipSuccess) return fail(-1);
+        if ((rc = launch_etable(p->dcmt.d(), dcjm.d(), (int)N, (int)d, (int)n_mesh, p->dE.d(), p->stream)))
+            return fail(rc);
+        if (hipStreamSynchronize(p->stream) != hipSuccess) return fail(-1);
+        dcjm.release();
+    }
+    *plan = p;
+    return 0;
+}
+
+extern "C" int rlm_plan_load(rlm_plan_t p, long first_sample, int count) {
+    if (!p || count < 1 || count > p->spb || first_sample < 0) return -22;
+    const long ni = p->n_imp;
+    p->hx.resize(count * ni);
+    p->hy.resize(count * ni);
+    p->hs.resize(count * ni);
+    for (int s = 0; s < count; s++)
+        host_rlm_disorder(p->hx.data() + s * ni, p->hy.data() + s * ni, p->hs.data() + s * ni, ni, p->L1, p->L2,
+                          p->seed, first_sample + s);
+    RLM_HIP_CHECK(hipMemcpyAsync(p->dX.p, p->hx.data(), 8 * count * ni, hipMemcpyHostToDevice, p->stream));
+    RLM_HIP_CHECK(hipMemcpyAsync(p->dY.p, p->hy.data(), 8 * count * ni, hipMemcpyHostToDevice, p->stream));
+    RLM_HIP_CHECK(hipMemcpyAsync(p->dS.p, p->hs.data(), 8 * count * ni, hipMemcpyHostToDevice, p->stream));
+    RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+    p->loaded = count;
+    p->first = first_sample;
+    return 0;
+}
+
+extern "C" int rlm_plan_execute(rlm_plan_t p) {
+    if (!p || p->loaded < 1) return -22;
+    const int S = p->loaded;
+    const int N = (int)p->n_phi, d = (int)p->d, off = (int)p->off, g = (int)p->g;
+    const long g1 = g + 1, nkp = (long)S * g1 * g1;
+    hipStream_t s = p->stream;
+    int rc;
+    if ((rc = launch_potential(p->dX.d(), p->dY.d(), p->dS.d(), p->n_imp, p->dcmn.d(), d, off, p->dV.d(), S, s)))
+        return rc;
+    if ((rc = launch_wtable(p->dV.d(), p->dcmt.d(), N, d, off, g, S, p->dW.d(), s))) return rc;
+    if ((rc = launch_assemble(p->dE.d(), p->dW.d(), N, d, off, g, 0, nkp, p->dA.d(), (long)N * N, s))) return rc;
+    if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s))) return rc;
+    double *L1 = p->dL.d(), *L2 = L1 + 2 * nkp * N;
+    if ((rc = launch_links(p->dU.d(), (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
+    double *out = p->dOut.d();
+    if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, (int)g1, 0, (double)N, S, out, out + (long)S * N,
+                            out + 2L * S * N, s)))
+        return rc;
+    return 0;
+}
+
+extern "C" int rlm_plan_fetch(rlm_plan_t p, double *energy, double *chern, double *margin) {
+    if (!p || p->loaded < 1) return -22;
+    const long SN = (long)p->loaded * p->n_phi;
+    RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+    std::vector<double> h(3 * SN);
+    RLM_HIP_CHECK(hipMemcpy(h.data(), p->dOut.p, 8 * 3 * SN, hipMemcpyDeviceToHost));
+    if (energy) memcpy(energy, h.data(), 8 * SN);
+    if (chern) memcpy(chern, h.data() + SN, 8 * SN);
+    if (margin) memcpy(margin, h.data() + 2 * SN, 8 * SN);
+    return 0;
+}
+
+extern "C" int rlm_plan_run(rlm_plan_t p, long first_sample, long n_samples, double *energy, double *chern,
+                            double *margin) {
+    if (!p || n_samples < 0) return -22;
+    for (long s0 = 0; s0 < n_samples; s0 += p->spb) {
+        const int cnt = (int)((n_samples - s0) < p->spb ? (n_samples - s0) : p->spb);
+        int rc;
+        if ((rc = rlm_plan_load(p, first_sample + s0, cnt))) return rc;
+        if ((rc = rlm_plan_execute(p))) return rc;
+        const long off = s0 * p->n_phi;
+        if ((rc = rlm_plan_fetch(p, energy ? energy + off : nullptr, chern ? chern + off : nullptr,
+                                 margin ? margin + off : nullptr)))
+            return rc;
+    }
+    return 0;
+}
+
+extern "C" void *rlm_plan_stream(rlm_plan_t p) { return p ? (void *)p->stream : nullptr; }
+
+extern "C" int rlm_plan_info(rlm_plan_t p, long *o) {
+    if (!p || !o) return -22;
+    o[0] = p->n_phi;
+    o[1] = p->n_imp;
+    o[2] = p->off;
+    o[3] = p->d;
+    o[4] = p->g;
+    o[5] = p->spb;
+    return 0;
+}
+
+extern "C" const double *rlm_plan_eigenvalues(rlm_plan_t p) { return p ? p->dEig.d() : nullptr; }
+extern "C" const double *rlm_plan_eigenvectors(rlm_plan_t p) { return p ? p->dU.d() : nullptr; }
+
+extern "C" int rlm_plan_destroy(rlm_plan_t p) {
+    if (!p) return 0;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    DevBuf *bufs[] = {&p->dcmn, &p->dcmt, &p->dE, &p->dX, &p->dY, &p->dS, &p->dV,
+                      &p->dW,   &p->dA,   &p->dU, &p->dEig, &p->dL, &p->dOut};
+    for (DevBuf *b : bufs) b->release();
+    p->ws.release();
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return 0;
+}

@@ -1,0 +1,123 @@
+// Chern stage and mesh-averaged energies (FP64).
+//
+// Reference: diag_wrappers/chern.cpp:37-75 (peer_cal_Chern: per plaquette and
+// state, prod of the 4 link overlaps zdotc(u_l, u_r) around
+// (t1-1,m)->(t1,m)->(t1,m+1)->(t1-1,m+1), chern_theta = -arg(prod)/2pi),
+// rlm/main_rlm_cpu.cpp:155-158,187-190,230-239 (energy sums over the
+// (g+1)^2 mesh, Chern sums line by line, +1/n_phi) and
+// hlm/main_hlm_cpu.cpp:111-118,143-150,184-191 (theta_2 = g copied from
+// theta_2 = 0, energies sum theta_2 < g plus theta_2 = 0, Chern = -sum).
+//
+// Instead of walking 4 links per plaquette, every link is computed once
+// (two per twist point), then a per-state kernel walks the plaquettes in the
+// reference's order and accumulates in FP64.
+//
+// Eigenvectors are component-major: U[kp][i*ns + j] is component i of state j,
+// so one thread per state reads coalesced rows.
+#include "common.hpp"
+#include "chern.hpp"
+
+namespace rlm {
+
+// links: grid (nkp), block 256 (threads over states).
+// Twist point kp = (s*lines + t1)*L2n + t2 with L2n stored theta_2 per line
+// (g+1 for rlm, g for hlm whose theta_2 = g is theta_2 = 0) and nlink2 links
+// along theta_2 per line (g in both models).
+// L1[kp][j] = <u(t1,t2)|u(t1+1,t2)> (t1 < lines-1); L2[kp][j] = <u(t1,t2)|u(t1,t2+1)>.
+__global__ __launch_bounds__(256) void links_kernel(const double *__restrict__ U, long strideU, int dim_vec, int ns,
+                                                    int lines, int L2n, int nlink2, long kp0,
+                                                    double *__restrict__ Lk1, double *__restrict__ Lk2) {
+    const long kp = kp0 + blockIdx.x;
+    const int t2 = (int)(kp % L2n);
+    const int t1 = (int)((kp / L2n) % lines);
+    const long base_line = kp - t2;         // kp of (t1, 0)
+    const bool has1 = t1 < lines - 1;
+    const bool has2 = t2 < nlink2;
+    const long kp_1 = kp + L2n;             // (t1+1, t2)
+    const long kp_2 = base_line + ((t2 + 1) % L2n);  // (t1, t2+1), wraps to 0 for hlm
+    const double *u0 = U + 2 * (kp - kp0) * strideU;
+    const double *u1 = U + 2 * (kp_1 - kp0) * strideU;
+    const double *u2 = U + 2 * (kp_2 - kp0) * strideU;
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+        cd a1{0.0, 0.0}, a2{0.0, 0.0};
+        for (int i = 0; i < dim_vec; i++) {
+            const cd x = ld(u0, (long)i * ns + j);
+            if (has1) cfmac(a1, x, ld(u1, (long)i * ns + j));
+            if (has2) cfmac(a2, x, ld(u2, (long)i * ns + j));
+        }
+        st(Lk1, kp * ns + j, a1);
+        st(Lk2, kp * ns + j, a2);
+    }
+}
+
+// plaquettes + energies: grid (S), block 256 (states).
+// mode 0 = rlm (+1/n_phi, chern += t), mode 1 = hlm (chern -= t).
+__global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ Lk1, const double *__restrict__ Lk2,
+                                                     const double *__restrict__ Wv, int ns, int n_mesh, int L2n,
+                                                     int mode, double n_phi_d, double *__restrict__ energy,
+                                                     double *__restrict__ chern, double *__restrict__ margin) {
+    const int s = blockIdx.x;
+    const int g = n_mesh, g1 = n_mesh + 1;
+    const long kbase = (long)s * g1 * L2n;
+    const double TWO_PI = 6.283185307179586232;   // (M_PI*2.0)
+    const double PI = 3.141592653589793116;
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+        // energies (reference order: line by line, theta_2 ascending)
+        double e = 0.0;
+        for (int t1 = 0; t1 <= g; t1++) {
+            const long lb = kbase + (long)t1 * L2n;
+            if (mode == 0) {
+                for (int t2 = 0; t2 <= g; t2++) e += Wv[(lb + t2) * ns + j];
+            } else {
+                for (int t2 = 0; t2 < g; t2++) e += Wv[(lb + t2) * ns + j];
+                e += Wv[lb * ns + j];
+            }
+        }
+        e /= (double)(g1 * g1);
+        // plaquettes (a = theta_1 - 1, b = m)
+        double c = 0.0, mg = 4.0;
+        for (int a = 0; a < g; a++) {
+            for (int b = 0; b < g; b++) {
+                const long k00 = kbase + (long)a * L2n + (b % L2n);
+                const long k10 = k00 + L2n;
+                const long k01 = kbase + (long)a * L2n + ((b + 1) % L2n);
+                const cd l1 = ld(Lk1, k00 * ns + j);     // <00|10>
+                const cd l2 = ld(Lk2, k10 * ns + j);     // <10|11>
+                const cd l3 = cconj(ld(Lk1, k01 * ns + j));  // <11|01>
+                const cd l4 = cconj(ld(Lk2, k00 * ns + j));  // <01|00>
+                cd prod{1.0, 0.0};
+                prod = cmul(prod, l1);
+                prod = cmul(prod, l2);
+                prod = cmul(prod, l3);
+                prod = cmul(prod, l4);
+                const double ph = atan2(prod.y, prod.x);
+                const double tth = -ph / TWO_PI;
+                if (mode == 0) c += tth; else c += -tth;
+                mg = fmin(mg, PI - fabs(ph));
+            }
+        }
+        if (mode == 0) c += 1.0 / n_phi_d;
+        energy[(long)s * ns + j] = e;
+        chern[(long)s * ns + j] = c;
+        if (margin) margin[(long)s * ns + j] = mg;
+    }
+}
+
+int launch_links(const double *U, long strideU, int dim_vec, int ns, int lines, int L2n, int nlink2, long kp0,
+                 long nkp, double *Lk1, double *Lk2, hipStream_t stream) {
+    if (nkp <= 0) return 0;
+    hipLaunchKernelGGL(links_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, U, strideU, dim_vec, ns, lines, L2n,
+                       nlink2, kp0, Lk1, Lk2);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_reduce(const double *Lk1, const double *Lk2, const double *Wv, int ns, int n_mesh, int L2n, int mode,
+                  double n_phi_d, int n_samples, double *energy, double *chern, double *margin, hipStream_t stream) {
+    hipLaunchKernelGGL(reduce_kernel, dim3(n_samples), dim3(256), 0, stream, Lk1, Lk2, Wv, ns, n_mesh, L2n, mode,
+                       n_phi_d, energy, chern, margin);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace rlm

@@ -1,0 +1,106 @@
+// Shared device helpers for the rlm/hlm HIP library (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+#define RLM_HIP_CHECK(expr)                                                        \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            fprintf(stderr, "[rlm_hip] %s failed: %s (%s:%d)\n", #expr,            \
+                    hipGetErrorString(_e), __FILE__, __LINE__);                    \
+            return -(int)_e;                                                       \
+        }                                                                          \
+    } while (0)
+
+namespace rlm {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- complex (double2)
+struct cd {
+    double x, y;
+};
+
+__host__ __device__ inline cd mk(double a, double b) { return cd{a, b}; }
+__host__ __device__ inline cd cadd(cd a, cd b) { return cd{a.x + b.x, a.y + b.y}; }
+__host__ __device__ inline cd csub(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}; }
+__host__ __device__ inline cd cmul(cd a, cd b) { return cd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// conj(a) * b
+__host__ __device__ inline cd cmulc(cd a, cd b) { return cd{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
+__host__ __device__ inline cd cconj(cd a) { return cd{a.x, -a.y}; }
+__host__ __device__ inline cd cscale(cd a, double s) { return cd{a.x * s, a.y * s}; }
+__host__ __device__ inline double cabs2(cd a) { return a.x * a.x + a.y * a.y; }
+// acc += a*b
+__host__ __device__ inline void cfma(cd &acc, cd a, cd b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+// acc += conj(a)*b
+__host__ __device__ inline void cfmac(cd &acc, cd a, cd b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+}
+// 1/a
+__host__ __device__ inline cd cinv(cd a) {
+    double s = 1.0 / (a.x * a.x + a.y * a.y);
+    return cd{a.x * s, -a.y * s};
+}
+
+__device__ inline cd ld(const double *p, long i) {
+    const double2 v = reinterpret_cast<const double2 *>(p)[i];
+    return cd{v.x, v.y};
+}
+__device__ inline void st(double *p, long i, cd v) {
+    reinterpret_cast<double2 *>(p)[i] = make_double2(v.x, v.y);
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Block-wide sum of one double; `red` is LDS scratch of >= blockDim/64 doubles.
+// All threads receive the total. Contains two barriers.
+__device__ inline double block_sum(double v, double *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < nw; i++) t += red[i];
+    __syncthreads();
+    return t;
+}
+
+// Block-wide sum of a complex value.
+__device__ inline cd block_sum_c(cd v, double *red) {
+    v.x = wave_sum(v.x);
+    v.y = wave_sum(v.y);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (lane == 0) {
+        red[2 * wid] = v.x;
+        red[2 * wid + 1] = v.y;
+    }
+    __syncthreads();
+    cd t{0.0, 0.0};
+    for (int i = 0; i < nw; i++) {
+        t.x += red[2 * i];
+        t.y += red[2 * i + 1];
+    }
+    __syncthreads();
+    return t;
+}
+
+}  // namespace rlm

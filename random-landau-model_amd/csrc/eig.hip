@@ -1,0 +1,662 @@
+// Batched Hermitian eigensolver (FP64) for gfx950.
+//
+// Replaces the per-k-point LAPACK zheevd('V') of diag_wrappers/mkl_diag.cpp:2-9
+// (and the cusolverDnCheevd / magma_cheevd_gpu calls of
+// diag_wrappers/cusolver_diag.cu:3-22, diag_wrappers/magma_diag.cu:2-49) with
+// three persistent kernels that each take one workgroup per matrix:
+//
+//   1. zhetrd_kernel: Householder reduction A = Q T Q^H (LAPACK zhetd2
+//      convention, lower storage of the reflectors), with the rank-2 update of
+//      step i fused with the Hermitian matrix-vector product of step i+1 so the
+//      trailing block is read and written once per column.
+//   2. tail_kernel, stage 1: divide & conquer on the real tridiagonal T
+//      (Cuppen tear with leaf size 1, LAPACK dlaed2-style deflation,
+//      rational-model secular solver, Gu-Eisenstat eigenvector recomputation),
+//      level-synchronous so every tree level runs its nodes in parallel.
+//   3. tail_kernel, stage 2: back-transform U = Q Z with compact-WY blocks of
+//      reflectors (I - V T V^H), one thread per eigenvector column.
+//
+// Matrices are complex128, column-major with full Hermitian storage on input.
+// Eigenvectors are returned component-major ("row-major": U[r*n + j] is
+// component r of state j) so that later per-state kernels read coalesced.
+#include "common.hpp"
+#include "eig.hpp"
+
+namespace rlm {
+
+// ============================================================================
+// 1. Householder tridiagonalisation
+// ============================================================================
+template <int NT, int MAXR>
+__global__ __launch_bounds__(NT) void zhetrd_kernel(double *__restrict__ Abase, long strideA, int lda, int n,
+                                                    double *__restrict__ Dout, double *__restrict__ Eout,
+                                                    double *__restrict__ TAUout, long strideV, int batch) {
+    extern __shared__ double smem[];
+    // LDS carve: v[2][n], w[2][n] (complex), part[NT/64][n] complex, red[2*NT/64], scal[8]
+    cd *vbuf = reinterpret_cast<cd *>(smem);          // 2*n
+    cd *wbuf = vbuf + 2 * n;                          // 2*n
+    cd *part = wbuf + 2 * n;                          // (NT/64)*n
+    double *red = reinterpret_cast<double *>(part + (NT / kWave) * n);
+    double *sc = red + 2 * (NT / kWave);              // 8 scalars
+    const int t = threadIdx.x;
+
+    for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+        double *A = Abase + 2 * b * strideA;           // complex index = 2*...
+        double *Dv = Dout + b * strideV;
+        double *Ev = Eout + b * strideV;
+        double *Tv = TAUout + b * strideV;      // tau: complex, strideV doubles per matrix
+        auto Aidx = [&](int r, int c) -> long { return (long)c * lda + r; };
+
+        int cur = 0;          // vbuf/wbuf slot holding the pending update
+        bool pending = false;
+        for (int i = 0; i < n - 1; i++) {
+            cd *vold = vbuf + cur * n, *wold = wbuf + cur * n;   // indexed from row i
+            cd *vnew = vbuf + (cur ^ 1) * n;                     // indexed from row i+1
+            cd *wnew = wbuf + (cur ^ 1) * n;
+            // (a) bring column i (rows >= i) up to date, collect ||x||^2, alpha, d_i
+            double nrm2 = 0.0;
+            for (int r = i + t; r < n; r += NT) {
+                cd a = ld(A, Aidx(r, i));
+                if (pending) {
+                    cd vr = vold[r - i], wr = wold[r - i];
+                    cd vc = vold[0], wc = wold[0];
+                    // a -= v_r conj(w_c) + w_r conj(v_c)
+                    cd u1 = cmul(vr, cconj(wc)), u2 = cmul(wr, cconj(vc));
+                    a = csub(a, cadd(u1, u2));
+                    st(A, Aidx(r, i), a);
+                }
+                if (r == i) sc[0] = a.x;
+                else if (r == i + 1) { sc[1] = a.x; sc[2] = a.y; }
+                else nrm2 += cabs2(a);
+            }
+            __syncthreads();
+            nrm2 = block_sum(nrm2, red);
+            if (t == 0) {
+                Dv[i] = sc[0];
+                double ar = sc[1], ai = sc[2];
+                double beta, taur, taui, sr, si;
+                if (nrm2 == 0.0 && ai == 0.0) {
+                    beta = ar; taur = 0.0; taui = 0.0; sr = 0.0; si = 0.0;
+                } else {
+                    double nrm = sqrt(ar * ar + ai * ai + nrm2);
+                    beta = (ar >= 0.0) ? -nrm : nrm;
+                    taur = (beta - ar) / beta;
+                    taui = -ai / beta;
+                    cd s = cinv(cd{ar - beta, ai});
+                    sr = s.x; si = s.y;
+                }
+                Ev[i] = beta;
+                Tv[2 * i] = taur;
+                Tv[2 * i + 1] = taui;
+                sc[3] = taur; sc[4] = taui; sc[5] = sr; sc[6] = si; sc[7] = beta;
+            }
+            __syncthreads();
+            const cd tau{sc[3], sc[4]};
+            const cd scal{sc[5], sc[6]};
+            // (b) v = [1; x*scal], stored back into A(i+2:n, i); A(i+1,i) = beta
+            for (int r = i + 1 + t; r < n; r += NT) {
+                cd v;
+                if (r == i + 1) {
+                    v = cd{1.0, 0.0};
+                    st(A, Aidx(r, i), cd{sc[7], 0.0});
+                } else {
+                    v = cmul(ld(A, Aidx(r, i)), scal);
+                    st(A, Aidx(r, i), v);
+                }
+                vnew[r - (i + 1)] = v;
+            }
+            __syncthreads();
+            // (c) fused pass over the trailing block rows/cols >= i+1:
+            //     apply the pending update of step i-1, accumulate acc = A22 * vnew
+            const int nr = n - (i + 1);
+            const int R = (nr >= NT) ? NT : ((nr + kWave - 1) / kWave) * kWave;
+            const int G = NT / R;
+            const int rl = t % R, cg = t / R;
+            cd acc[MAXR];
+#pragma unroll
+            for (int k = 0; k < MAXR; k++) acc[k] = cd{0.0, 0.0};
+            if (cg < G) {
+                for (int c = i + 1 + cg; c < n; c += G) {
+                    const cd vn = vnew[c - (i + 1)];
+                    cd voc{0, 0}, woc{0, 0};
+                    if (pending) { voc = cconj(vold[c - i]); woc = cconj(wold[c - i]); }
+#pragma unroll
+                    for (int k = 0; k < MAXR; k++) {
+                        const int r = i + 1 + rl + k * R;
+                        if (r < n) {
+                            cd a = ld(A, Aidx(r, c));
+                            if (pending) {
+                                const cd vr = vold[r - i], wr = wold[r - i];
+                                a = csub(a, cadd(cmul(vr, woc), cmul(wr, voc)));
+                                if (r == c) a.y = 0.0;   // Hermitian diagonal stays real (zher2)
+                                st(A, Aidx(r, c), a);
+                            }
+                            cfma(acc[k], a, vn);
+                        }
+                    }
+                }
+            }
+            // reduce the G column groups
+#pragma unroll
+            for (int k = 0; k < MAXR; k++) {
+                const int r = i + 1 + rl + k * R;
+                if (r < n) part[cg * n + (r - (i + 1))] = acc[k];
+            }
+            __syncthreads();
+            cd dotl{0.0, 0.0};
+            for (int r = i + 1 + t; r < n; r += NT) {
+                cd p{0.0, 0.0};
+                for (int g = 0; g < G; g++) p = cadd(p, part[g * n + (r - (i + 1))]);
+                p = cmul(tau, p);
+                wnew[r - (i + 1)] = p;                        // p for now
+                dotl = cadd(dotl, cmulc(p, vnew[r - (i + 1)])); // conj(p) v
+            }
+            cd dot = block_sum_c(dotl, red);
+            // alpha2 = -1/2 tau (p^H v); w = p + alpha2 v
+            const cd alpha2 = cscale(cmul(tau, dot), -0.5);
+            for (int r = i + 1 + t; r < n; r += NT) {
+                cd p = wnew[r - (i + 1)];
+                wnew[r - (i + 1)] = cadd(p, cmul(alpha2, vnew[r - (i + 1)]));
+            }
+            __syncthreads();
+            pending = (tau.x != 0.0 || tau.y != 0.0);
+            cur ^= 1;
+        }
+        // last diagonal element
+        if (t == 0) {
+            cd a = ld(A, Aidx(n - 1, n - 1));
+            if (pending && n >= 2) {
+                cd v = vbuf[cur * n], w = wbuf[cur * n];
+                a = csub(a, cadd(cmul(v, cconj(w)), cmul(w, cconj(v))));
+            }
+            Dv[n - 1] = a.x;
+        }
+        __syncthreads();
+    }
+}
+
+// ============================================================================
+// 2. Divide & conquer on the symmetric tridiagonal + 3. back-transform
+// ============================================================================
+struct DcShared {
+    double *dq, *dqn, *ds, *zs, *dl, *wz, *tau, *vals, *rc, *rs, *what, *inrm;
+    int *perm, *orig, *rank, *rpj, *rnj, *ndx, *dfx;
+    int *cnt;  // per node: K, ndefl, nrot  (3 ints per node slot)
+};
+
+__device__ inline double dc_eps() { return 1.1102230246251565e-16; }  // 2^-53
+
+// Root j of 1 + rho * sum_i w_i^2 / (dl_i - lambda) = 0, poles dl ascending.
+// Returns origin index o and tau with lambda = dl[o] + tau.
+__device__ void secular_root(int j, int K, const double *dl, const double *w, double rho, int &o_out,
+                             double &tau_out) {
+    int o;
+    double lo, hi, tau;
+    if (j < K - 1) {
+        const double Delta = dl[j + 1] - dl[j];
+        // f at the midpoint, origin dl[j]
+        double f = 1.0;
+        const double tm = 0.5 * Delta;
+        for (int i = 0; i < K; i++) {
+            const double del = (dl[i] - dl[j]) - tm;
+            f += rho * w[i] * w[i] / del;
+        }
+        if (f >= 0.0) { o = j; lo = 0.0; hi = tm; tau = tm; }
+        else { o = j + 1; lo = -tm; hi = 0.0; tau = -tm; }
+    } else {
+        o = j; lo = 0.0; hi = rho; tau = rho;
+    }
+    const double dlo = dl[o];
+    const double Dj = dl[j] - dlo;
+    const double Dj1 = (j < K - 1) ? (dl[j + 1] - dlo) : 0.0;
+    for (int it = 0; it < 120; it++) {
+        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, err = 0.0;
+        double delj = 0.0, delj1 = 0.0;
+        for (int i = 0; i < K; i++) {
+            const double del = (dl[i] - dlo) - tau;
+            const double tt = rho * w[i] * w[i] / del;
+            const double t2 = tt / del;
+            if (i <= j) { psi += tt; dpsi += t2; } else { phi += tt; dphi += t2; }
+            err += fabs(tt);
+            if (i == j) delj = del;
+            if (i == j + 1) delj1 = del;
+        }
+        const double f = 1.0 + psi + phi;
+        if (f > 0.0) hi = fmin(hi, tau);
+        else if (f < 0.0) lo = fmax(lo, tau);
+        if (f == 0.0 || fabs(f) <= 8.0 * dc_eps() * (1.0 + err)) break;
+        if (hi - lo <= 4.0 * dc_eps() * fmax(fabs(lo), fabs(hi))) break;
+        const double bq = dpsi * delj * delj;
+        const double aq = psi - dpsi * delj;
+        double nt;
+        if (j < K - 1) {
+            const double Bq = dphi * delj1 * delj1;
+            const double Aq = phi - dphi * delj1;
+            const double c = 1.0 + aq + Aq;
+            if (o == j) {
+                const double beta = c * Dj1 + bq + Bq;
+                nt = 2.0 * bq * Dj1 / (beta + sqrt(fmax(beta * beta - 4.0 * c * bq * Dj1, 0.0)));
+            } else {
+                const double beta = c * Dj + bq + Bq;
+                nt = 2.0 * Bq * Dj / (beta + sqrt(fmax(beta * beta - 4.0 * c * Bq * Dj, 0.0)));
+            }
+        } else {
+            const double c = 1.0 + aq + phi;
+            nt = (c > 0.0) ? bq / c : hi + 1.0;  // force bisection when the model fails
+        }
+        if (!(nt > lo && nt < hi)) nt = 0.5 * (lo + hi);
+        tau = nt;
+    }
+    o_out = o;
+    tau_out = tau;
+}
+
+// Number of elements of the ascending array a[0..len) that are < v (strict) or <= v.
+__device__ inline int count_below(const double *a, int len, double v, bool inclusive) {
+    int lo = 0, hi = len;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        bool go = inclusive ? (a[mid] <= v) : (a[mid] < v);
+        if (go) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Stage 1: D&C. d (n), e (n-1) in; eigenvalues ascending in sh.dq (returned via
+// pointer swap) and eigenvectors of T in the returned buffer (row-major n x n).
+template <int NT>
+__device__ double *stedc_block(int n, const double *dIn, const double *eIn, double *Qa, double *Qb, double *S,
+                               DcShared &sh, double **dq_final) {
+    const int t = threadIdx.x;
+    // init: torn diagonal, identity / zero
+    for (int k = t; k < n; k += NT) {
+        double v = dIn[k];
+        if (k > 0) v -= fabs(eIn[k - 1]);
+        if (k < n - 1) v -= fabs(eIn[k]);
+        sh.dq[k] = v;
+    }
+    for (long idx = t; idx < (long)n * n; idx += NT) {
+        const long r = idx / n, c = idx % n;
+        Qa[idx] = (r == c) ? 1.0 : 0.0;
+        Qb[idx] = 0.0;
+    }
+    __syncthreads();
+    double *dq = sh.dq, *dqn = sh.dqn;
+    for (int L = 1; (1 << (L - 1)) < n; L++) {
+        const int half = 1 << (L - 1), blk = half << 1;
+        const int nnodes = (n + blk - 1) / blk;
+        int G = 1;
+        while (G * 2 <= NT / nnodes && G * 2 <= blk) G *= 2;
+        if (G > NT) G = NT;
+        const int per = NT / G;  // nodes handled per round
+        for (int nb0 = 0; nb0 < nnodes; nb0 += per) {
+            const int k = nb0 + t / G;
+            const int lane = t % G;
+            const bool active = k < nnodes;
+            const int lo = k * blk, mid = lo + half, hi = min(lo + blk, n);
+            const bool merging = active && mid < n;
+            const int m = hi - lo, n1 = mid - lo;
+            const double beta = merging ? eIn[mid - 1] : 0.0;
+            const double sgn = (beta >= 0.0) ? 1.0 : -1.0;
+            const double rho = 2.0 * fabs(beta);
+            const double isq2 = 0.70710678118654752440;
+            // P1: merge the two sorted halves; gather z
+            if (merging) {
+                for (int i = lane; i < m; i += G) {
+                    double val, zi;
+                    int pos;
+                    if (i < n1) {
+                        val = dq[lo + i];
+                        pos = i + count_below(dq + mid, hi - mid, val, false);
+                        zi = Qa[(long)(mid - 1) * n + lo + i];
+                    } else {
+                        const int j = i - n1;
+                        val = dq[mid + j];
+                        pos = j + count_below(dq + lo, n1, val, true);
+                        zi = sgn * Qa[(long)mid * n + mid + j];
+                    }
+                    sh.ds[lo + pos] = val;
+                    sh.zs[lo + pos] = zi * isq2;
+                    sh.perm[lo + pos] = i;
+                }
+            }
+            __syncthreads();
+            // P2: deflation scan (one lane per node)
+            if (merging && lane == 0) {
+                double dmax = 0.0, zmax = 0.0;
+                for (int i = 0; i < m; i++) {
+                    dmax = fmax(dmax, fabs(sh.ds[lo + i]));
+                    zmax = fmax(zmax, fabs(sh.zs[lo + i]));
+                }
+                const double tol = 8.0 * dc_eps() * fmax(dmax, zmax);
+                int K = 0, nd = 0, nr = 0, pj = -1;
+                for (int nj = 0; nj < m; nj++) {
+                    if (rho * fabs(sh.zs[lo + nj]) <= tol) {
+                        sh.dfx[lo + nd++] = nj;
+                        continue;
+                    }
+                    if (pj < 0) { pj = nj; continue; }
+                    double S_ = sh.zs[lo + pj], C_ = sh.zs[lo + nj];
+                    const double tz = hypot(C_, S_);
+                    const double tt = sh.ds[lo + nj] - sh.ds[lo + pj];
+                    C_ /= tz;
+                    S_ = -S_ / tz;
+                    if (fabs(tt * C_ * S_) <= tol) {
+                        sh.zs[lo + nj] = tz;
+                        sh.zs[lo + pj] = 0.0;
+                        sh.rpj[lo + nr] = pj;
+                        sh.rnj[lo + nr] = nj;
+                        sh.rc[lo + nr] = C_;
+                        sh.rs[lo + nr] = S_;
+                        nr++;
+                        const double dp = sh.ds[lo + pj], dn = sh.ds[lo + nj];
+                        const double t1 = dp * C_ * C_ + dn * S_ * S_;
+                        sh.ds[lo + nj] = dp * S_ * S_ + dn * C_ * C_;
+                        sh.ds[lo + pj] = t1;
+                        sh.dfx[lo + nd++] = pj;
+                        pj = nj;
+                    } else {
+                        sh.ndx[lo + K++] = pj;
+                        pj = nj;
+                    }
+                }
+                if (pj >= 0) sh.ndx[lo + K++] = pj;
+                sh.cnt[3 * k + 0] = K;
+                sh.cnt[3 * k + 1] = nd;
+                sh.cnt[3 * k + 2] = nr;
+            }
+            __syncthreads();
+            // P3: apply deflation rotations to Qa's node block; secular inputs
+            int K = 0, nd = 0, nr = 0;
+            if (merging) { K = sh.cnt[3 * k]; nd = sh.cnt[3 * k + 1]; nr = sh.cnt[3 * k + 2]; }
+            if (merging) {
+                if (nr > 0) {
+                    for (int r = lo + lane; r < hi; r += G) {
+                        double *row = Qa + (long)r * n + lo;
+                        for (int q = 0; q < nr; q++) {
+                            const int cp = sh.perm[lo + sh.rpj[lo + q]], cn = sh.perm[lo + sh.rnj[lo + q]];
+                            const double c = sh.rc[lo + q], s = sh.rs[lo + q];
+                            const double x = row[cp], y = row[cn];
+                            row[cp] = c * x + s * y;
+                            row[cn] = c * y - s * x;
+                        }
+                    }
+                }
+                for (int q = lane; q < K; q += G) {
+                    sh.dl[lo + q] = sh.ds[lo + sh.ndx[lo + q]];
+                    sh.wz[lo + q] = sh.zs[lo + sh.ndx[lo + q]];
+                }
+            }
+            __syncthreads();
+            // P4: secular roots
+            if (merging) {
+                for (int j = lane; j < K; j += G) {
+                    int o;
+                    double tu;
+                    secular_root(j, K, sh.dl + lo, sh.wz + lo, rho, o, tu);
+                    sh.orig[lo + j] = o;
+                    sh.tau[lo + j] = tu;
+                }
+            }
+            __syncthreads();
+            // P5: Gu-Eisenstat recomputed weights
+            if (merging) {
+                const double *dl = sh.dl + lo;
+                for (int i = lane; i < K; i += G) {
+                    double p = (dl[i] - dl[sh.orig[lo + i]]) - sh.tau[lo + i];
+                    for (int j = 0; j < K; j++) {
+                        if (j == i) continue;
+                        const double del = (dl[i] - dl[sh.orig[lo + j]]) - sh.tau[lo + j];
+                        p *= del / (dl[i] - dl[j]);
+                    }
+                    sh.what[lo + i] = copysign(sqrt(fmax(-p, 0.0)), sh.wz[lo + i]);
+                }
+            }
+            __syncthreads();
+            // P6: eigenvector norms; output values and ranks
+            if (merging) {
+                const double *dl = sh.dl + lo;
+                for (int j = lane; j < K; j += G) {
+                    const int oj = sh.orig[lo + j];
+                    const double tj = sh.tau[lo + j];
+                    double s2 = 0.0;
+                    for (int i = 0; i < K; i++) {
+                        const double u = sh.what[lo + i] / ((dl[i] - dl[oj]) - tj);
+                        s2 += u * u;
+                    }
+                    sh.inrm[lo + j] = 1.0 / sqrt(s2);
+                    sh.vals[lo + j] = dl[oj] + tj;
+                }
+                for (int q = lane; q < nd; q += G) sh.vals[lo + K + q] = sh.ds[lo + sh.dfx[lo + q]];
+            }
+            __syncthreads();
+            if (merging) {
+                for (int q = lane; q < m; q += G) {
+                    const double v = sh.vals[lo + q];
+                    int rk = 0;
+                    for (int p = 0; p < m; p++) {
+                        const double u = sh.vals[lo + p];
+                        rk += (u < v) || (u == v && p < q);
+                    }
+                    sh.rank[lo + q] = rk;
+                    dqn[lo + rk] = v;
+                }
+            }
+            __syncthreads();
+            // P7: U (K x K) into S rows lo.., scaled by the inverse norms
+            if (merging) {
+                const double *dl = sh.dl + lo;
+                for (int idx = lane; idx < K * K; idx += G) {
+                    const int i = idx / K, q = idx % K;
+                    const double del = (dl[i] - dl[sh.orig[lo + q]]) - sh.tau[lo + q];
+                    S[(long)(lo + i) * n + q] = sh.what[lo + i] / del * sh.inrm[lo + q];
+                }
+            }
+            __syncthreads();
+            // P8: Qb block = Qa block (rotated, permuted) * U ; deflated columns copied
+            if (merging) {
+                for (int idx = lane; idx < m * m; idx += G) {
+                    const int r = lo + idx / m, q = idx % m;
+                    const double *row = Qa + (long)r * n + lo;
+                    double v;
+                    if (q < K) {
+                        v = 0.0;
+                        for (int i = 0; i < K; i++)
+                            v = fma(row[sh.perm[lo + sh.ndx[lo + i]]], S[(long)(lo + i) * n + q], v);
+                    } else {
+                        v = row[sh.perm[lo + sh.dfx[lo + q - K]]];
+                    }
+                    Qb[(long)r * n + lo + sh.rank[lo + q]] = v;
+                }
+            } else if (active) {
+                for (int idx = lane; idx < m * m; idx += G) {
+                    const int r = lo + idx / m, c = lo + idx % m;
+                    Qb[(long)r * n + c] = Qa[(long)r * n + c];
+                }
+                for (int q = lane; q < m; q += G) dqn[lo + q] = dq[lo + q];
+            }
+            __syncthreads();
+        }
+        double *tq = Qa; Qa = Qb; Qb = tq;
+        double *td = dq; dq = dqn; dqn = td;
+    }
+    *dq_final = dq;
+    return Qa;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void tail_kernel(const double *__restrict__ Abase, long strideA, int lda, int n,
+                                                  const double *__restrict__ Din, const double *__restrict__ Ein,
+                                                  const double *__restrict__ TAUin, long strideV,
+                                                  double *__restrict__ Wout, long strideW,
+                                                  double *__restrict__ Uout, long strideU,
+                                                  double *__restrict__ work, int nb_wy, int batch) {
+    extern __shared__ double smem[];
+    const int t = threadIdx.x;
+    // per-workgroup global workspace: Qa, Qb, S (n*n doubles each)
+    double *Qa0 = work + (long)blockIdx.x * 3 * n * n;
+    double *Qb0 = Qa0 + (long)n * n;
+    double *S = Qb0 + (long)n * n;
+    DcShared sh;
+    {
+        double *p = smem;
+        sh.dq = p; p += n; sh.dqn = p; p += n; sh.ds = p; p += n; sh.zs = p; p += n;
+        sh.dl = p; p += n; sh.wz = p; p += n; sh.tau = p; p += n; sh.vals = p; p += n;
+        sh.rc = p; p += n; sh.rs = p; p += n; sh.what = p; p += n; sh.inrm = p; p += n;
+        int *ip = reinterpret_cast<int *>(p);
+        sh.perm = ip; ip += n; sh.orig = ip; ip += n; sh.rank = ip; ip += n; sh.rpj = ip; ip += n;
+        sh.rnj = ip; ip += n; sh.ndx = ip; ip += n; sh.dfx = ip; ip += n; sh.cnt = ip; ip += 3 * n;
+    }
+    for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+        const double *A = Abase + 2 * b * strideA;
+        const double *dIn = Din + b * strideV;
+        const double *eIn = Ein + b * strideV;
+        const double *tauv = TAUin + b * strideV;
+        double *W = Wout + b * strideW;
+        double *U = Uout + 2 * b * strideU;
+        double *dqf;
+        double *Z = stedc_block<NT>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf);
+        for (int k = t; k < n; k += NT) W[k] = dqf[k];
+        // U = Z (complex, row-major)
+        for (long idx = t; idx < (long)n * n; idx += NT) st(U, idx, cd{Z[idx], 0.0});
+        __syncthreads();
+        // back-transform with blocks of reflectors, last block first
+        cd *Vl = reinterpret_cast<cd *>(smem);        // rows x nb (reuses the D&C LDS)
+        const int nref = n - 1;
+        const int nblk = (nref + nb_wy - 1) / nb_wy;
+        cd *Tl = Vl + (long)n * nb_wy;                // nb x nb
+        for (int bk = nblk - 1; bk >= 0; bk--) {
+            const int j0 = bk * nb_wy;
+            const int nbk = min(nb_wy, nref - j0);
+            const int r0 = j0 + 1;
+            const int nrow = n - r0;
+            // V block: V[(r - r0)*nb + k] for reflector j0+k
+            for (int idx = t; idx < nrow * nb_wy; idx += NT) {
+                const int rr = idx / nb_wy, k = idx % nb_wy;
+                const int r = r0 + rr, j = j0 + k;
+                cd v{0.0, 0.0};
+                if (k < nbk) {
+                    if (r == j + 1) v = cd{1.0, 0.0};
+                    else if (r > j + 1) v = ld(A, (long)j * lda + r);
+                }
+                Vl[idx] = v;
+            }
+            __syncthreads();
+            // T (upper triangular): T[i][k] stored Tl[i*nb+k]; first Gram products V_i^H V_k
+            for (int idx = t; idx < nb_wy * nb_wy; idx += NT) {
+                const int i = idx / nb_wy, k = idx % nb_wy;
+                cd g{0.0, 0.0};
+                if (i < k && k < nbk) {
+                    for (int rr = 0; rr < nrow; rr++) cfmac(g, Vl[rr * nb_wy + i], Vl[rr * nb_wy + k]);
+                }
+                Tl[idx] = g;
+            }
+            __syncthreads();
+            if (t == 0) {
+                // T[0:k,k] = -tau_k * T[0:k,0:k] * G[0:k,k]; T[k][k] = tau_k
+                cd *tmp = Tl + nb_wy * nb_wy;   // nb scratch entries in LDS
+                for (int k = 0; k < nbk; k++) {
+                    const cd tk{tauv[2 * (j0 + k)], tauv[2 * (j0 + k) + 1]};
+                    for (int i = 0; i < k; i++) {
+                        cd s{0.0, 0.0};
+                        for (int l = i; l < k; l++) {
+                            const cd Til = (l == i) ? Tl[i * nb_wy + i] : Tl[i * nb_wy + l];
+                            cfma(s, Til, Tl[l * nb_wy + k]);
+                        }
+                        tmp[i] = s;
+                    }
+                    for (int i = 0; i < k; i++) {
+                        cd v = cmul(tk, tmp[i]);
+                        Tl[i * nb_wy + k] = cd{-v.x, -v.y};
+                    }
+                    Tl[k * nb_wy + k] = tk;
+                }
+                for (int k = nbk; k < nb_wy; k++)
+                    for (int i = 0; i < nb_wy; i++) Tl[i * nb_wy + k] = cd{0.0, 0.0};
+            }
+            __syncthreads();
+            // apply to every column: w = V^H u ; w = T w ; u -= V w
+            for (int c = t; c < n; c += NT) {
+                cd w[16];
+#pragma unroll
+                for (int k = 0; k < 16; k++) w[k] = cd{0.0, 0.0};
+                for (int rr = 0; rr < nrow; rr++) {
+                    const cd u = ld(U, (long)(r0 + rr) * n + c);
+#pragma unroll
+                    for (int k = 0; k < 16; k++)
+                        if (k < nb_wy) cfmac(w[k], Vl[rr * nb_wy + k], u);
+                }
+                cd w2[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    cd s{0.0, 0.0};
+#pragma unroll
+                    for (int k = 0; k < 16; k++)
+                        if (i < nb_wy && k >= i && k < nb_wy) cfma(s, Tl[i * nb_wy + k], w[k]);
+                    w2[i] = s;
+                }
+                for (int rr = 0; rr < nrow; rr++) {
+                    cd u = ld(U, (long)(r0 + rr) * n + c);
+#pragma unroll
+                    for (int k = 0; k < 16; k++)
+                        if (k < nb_wy) {
+                            const cd vv = Vl[rr * nb_wy + k];
+                            u.x -= vv.x * w2[k].x - vv.y * w2[k].y;
+                            u.y -= vv.x * w2[k].y + vv.y * w2[k].x;
+                        }
+                    st(U, (long)(r0 + rr) * n + c, u);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ============================================================================
+// host launchers
+// ============================================================================
+size_t heevd_dc_lds_bytes(int n, int nb_wy) {
+    size_t dc = (size_t)n * (12 * sizeof(double) + 10 * sizeof(int));
+    size_t wy = (size_t)n * nb_wy * 16 + (size_t)nb_wy * (nb_wy + 1) * 16;
+    return dc > wy ? dc : wy;
+}
+
+int heevd_dc_workspace_doubles(int n, int grid_tail, long *per_matrix, long *per_wg) {
+    *per_matrix = 4L * n;         // d, e, tau(complex)
+    *per_wg = 3L * n * n;
+    (void)grid_tail;
+    return 0;
+}
+
+int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,
+                  int grid, hipStream_t stream) {
+    if (n < 1 || n > 512) return -22;
+    constexpr int NT = 256;
+    size_t lds = (size_t)(4 * n + (NT / kWave) * n) * 16 + (2 * (NT / kWave) + 8) * sizeof(double);
+    if (grid > batch) grid = batch;
+    if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)zhetrd_kernel<NT, 2>,
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((zhetrd_kernel<NT, 2>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU,
+                       strideV, batch);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_tail(const double *A, long strideA, int lda, int n, const double *D, const double *E, const double *TAU,
+                long strideV, double *W, long strideW, double *U, long strideU, double *work, int batch, int grid,
+                hipStream_t stream) {
+    if (n < 1 || n > 512) return -22;
+    constexpr int NT = 256;
+    const int nb_wy = 16;
+    size_t lds = heevd_dc_lds_bytes(n, nb_wy);
+    if (lds > 160 * 1024) return -22;
+    if (grid > batch) grid = batch;
+    if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)tail_kernel<NT>,
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((tail_kernel<NT>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU, strideV,
+                       W, strideW, U, strideU, work, nb_wy, batch);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace rlm

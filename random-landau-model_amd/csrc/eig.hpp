@@ -1,0 +1,24 @@
+// Host launchers of the batched FP64 Hermitian eigensolver (eig.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace rlm {
+
+// Householder tridiagonalisation of `batch` n x n complex matrices (column-major,
+// lda, full Hermitian storage, real diagonal). Writes d (n), e (n-1) and the
+// complex tau (n-1) at stride strideV per matrix; reflectors stay in A.
+int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,
+                  int grid, hipStream_t stream);
+
+// Divide & conquer on (d, e) + back-transform with the reflectors in A.
+// W: eigenvalues ascending (strideW per matrix). U: eigenvectors, component-major
+// complex (U[r*n + j] = component r of state j), strideU complex elements per matrix.
+// work: grid * 3*n*n doubles.
+int launch_tail(const double *A, long strideA, int lda, int n, const double *D, const double *E, const double *TAU,
+                long strideV, double *W, long strideW, double *U, long strideU, double *work, int batch, int grid,
+                hipStream_t stream);
+
+size_t heevd_dc_lds_bytes(int n, int nb_wy);
+
+}  // namespace rlm

@@ -1,0 +1,189 @@
+"""Python plumbing over librlm_hip.so (the C ABI declared in include/hip_diag.h).
+
+Tests and bench.py drive the HIP library through these ctypes bindings; torch is
+used only to own device memory. The library must be built
+(``make -C random-landau-model_amd``); nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "librlm_hip.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "hip_diag.h")
+
+HIP_DIAG_FP64 = 0
+
+_lib = None
+
+
+class HipDiagError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise HipDiagError(f"{what} failed with status {rc}")
+
+
+def lib() -> ctypes.CDLL:
+    """Load librlm_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HipDiagError(f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR}`")
+    L = ctypes.CDLL(LIB_PATH)
+    dp = ctypes.c_void_p
+    i, l, d, vp = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
+    sig = {
+        "hip_diag_hamil": ([dp, dp, i], i),
+        "hip_diag_hamil_v": ([dp, dp, i], None),
+        "hip_heevd": ([dp, dp, i], i),
+        "hip_diag_heevd_batched": ([i, dp, l, l, dp, l, i, ctypes.c_char, i, vp], i),
+        "hip_cal_chern": ([dp, dp, i, i, i, i], i),
+        "rlm_prepare_coeff": ([dp, dp, dp, i, i, i, i, i, d, d], i),
+        "rlm_generate_disorder_potential": ([dp, dp, dp, l, d, d, ctypes.c_ulong, l], i),
+        "hip_prepare_potential_coeff": ([dp, dp, dp, dp, dp, i, i, i, l], i),
+        "hip_set_hamil_matrix": ([dp, dp, dp, dp, dp, i, i, i], i),
+        "rlm_plan_create": ([ctypes.POINTER(vp), l, d, d, l, l, ctypes.c_ulong, i, i], i),
+        "rlm_plan_load": ([vp, l, i], i),
+        "rlm_plan_execute": ([vp], i),
+        "rlm_plan_fetch": ([vp, dp, dp, dp], i),
+        "rlm_plan_run": ([vp, l, l, dp, dp, dp], i),
+        "rlm_plan_stream": ([vp], vp),
+        "rlm_plan_info": ([vp, ctypes.POINTER(ctypes.c_long)], i),
+        "rlm_plan_eigenvalues": ([vp], vp),
+        "rlm_plan_eigenvectors": ([vp], vp),
+        "rlm_plan_destroy": ([vp], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/hip_diag.h."""
+    text = open(HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("defined",)))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------- host helpers
+def prepare_coeff(n_phi: int, off: int, d: int, n_mesh: int, L1: float, L2: float):
+    """rlm_prepare_coeff -> (coeff_mn[d,d], coeff_m_theta[g+1,d], coeff_jm[N,d]) complex128."""
+    cmn = np.zeros((d, d), np.complex128)
+    cmt = np.zeros((n_mesh + 1, d), np.complex128)
+    cjm = np.zeros((n_phi, d), np.complex128)
+    _check(lib().rlm_prepare_coeff(_ptr(cmn), _ptr(cmt), _ptr(cjm), n_phi, off, d, d, n_mesh, L1, L2),
+           "rlm_prepare_coeff")
+    return cmn, cmt, cjm
+
+
+def generate_disorder(n_imp: int, L1: float, L2: float, seed: int, sample: int):
+    x = np.zeros(n_imp)
+    y = np.zeros(n_imp)
+    s = np.zeros(n_imp)
+    _check(lib().rlm_generate_disorder_potential(_ptr(x), _ptr(y), _ptr(s), n_imp, L1, L2, seed, sample),
+           "rlm_generate_disorder_potential")
+    return x, y, s
+
+
+# ---------------------------------------------------------------- device helpers (torch-owned memory)
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise HipDiagError("no GPU visible to torch")
+    return torch
+
+
+def heevd_batched(H, precision: int = HIP_DIAG_FP64, uplo: str = "L"):
+    """Eigen-decomposition of a stack of Hermitian matrices H[b] (torch complex128, cuda).
+
+    Returns (w[b, n], U[b, n, n]) with U[b][:, j] the eigenvector of w[b, j] (H U = U diag(w)).
+    """
+    torch = _torch()
+    squeeze = H.dim() == 2
+    if squeeze:
+        H = H[None]
+    b, n, _ = H.shape
+    # the C ABI is column-major: hand it H^T's row-major buffer
+    A = H.transpose(-1, -2).contiguous().to(torch.complex128)
+    W = torch.empty((b, n), dtype=torch.float64, device=A.device)
+    torch.cuda.synchronize()
+    _check(lib().hip_diag_heevd_batched(n, A.data_ptr(), n, n * n, W.data_ptr(), n, b, uplo.encode(), precision,
+                                        None), "hip_diag_heevd_batched")
+    U = A.transpose(-1, -2)
+    if squeeze:
+        return W[0], U[0]
+    return W, U
+
+
+class RlmPlan:
+    """rlm_plan_* (batched rlm pipeline)."""
+
+    def __init__(self, n_phi, q=1.0, imp=16.0, n_mesh=32, seed=1, samples_per_batch=1, off_override=-1,
+                 precision=HIP_DIAG_FP64):
+        L = lib()
+        self._h = ctypes.c_void_p()
+        _check(L.rlm_plan_create(ctypes.byref(self._h), n_phi, q, imp, n_mesh, off_override, seed,
+                                 samples_per_batch, precision), "rlm_plan_create")
+        info = (ctypes.c_long * 6)()
+        _check(L.rlm_plan_info(self._h, info), "rlm_plan_info")
+        self.n_phi, self.n_imp, self.off, self.d, self.n_mesh, self.spb = list(info)
+        self.loaded = 0
+
+    def load(self, first_sample: int, count: int):
+        _check(lib().rlm_plan_load(self._h, first_sample, count), "rlm_plan_load")
+        self.loaded = count
+
+    def execute(self):
+        _check(lib().rlm_plan_execute(self._h), "rlm_plan_execute")
+
+    def fetch(self):
+        n = self.loaded * self.n_phi
+        e, c, m = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().rlm_plan_fetch(self._h, _ptr(e), _ptr(c), _ptr(m)), "rlm_plan_fetch")
+        shp = (self.loaded, self.n_phi)
+        return e.reshape(shp), c.reshape(shp), m.reshape(shp)
+
+    def run(self, first_sample: int, n_samples: int):
+        n = n_samples * self.n_phi
+        e, c, m = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().rlm_plan_run(self._h, first_sample, n_samples, _ptr(e), _ptr(c), _ptr(m)), "rlm_plan_run")
+        shp = (n_samples, self.n_phi)
+        return e.reshape(shp), c.reshape(shp), m.reshape(shp)
+
+    @property
+    def stream(self) -> int:
+        return lib().rlm_plan_stream(self._h) or 0
+
+    def eigenvalues_ptr(self) -> int:
+        return lib().rlm_plan_eigenvalues(self._h) or 0
+
+    def eigenvectors_ptr(self) -> int:
+        return lib().rlm_plan_eigenvectors(self._h) or 0
+
+    def close(self):
+        if self._h:
+            lib().rlm_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

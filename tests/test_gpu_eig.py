@@ -1,0 +1,114 @@
+"""GPU parity of the batched Hermitian eigensolver (librlm_hip.so) against LAPACK zheevd
+(numpy.linalg.eigh), the routine the reference calls in diag_wrappers/mkl_diag.cpp:8.
+
+Tolerances (FP64): eigenvalues |w - w_ref| <= 1e-12 * ||A||, residual
+||A U - U diag(w)|| <= 1e-12 * ||A||, orthogonality ||U^H U - I|| <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _rand_herm(rng, b, n, scale=1.0):
+    X = rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
+    return scale * (X + X.conj().transpose(0, 2, 1)) / 2
+
+
+def _check(H, w, U):
+    b, n, _ = H.shape
+    for k in range(b):
+        A = H[k]
+        nrm = max(np.linalg.norm(A, 2), 1e-300)
+        wr = np.linalg.eigvalsh(A)
+        assert np.all(np.diff(w[k]) >= 0), "eigenvalues not ascending"
+        assert np.abs(w[k] - wr).max() <= TOL * nrm * max(1, n / 64), (k, np.abs(w[k] - wr).max() / nrm)
+        R = A @ U[k] - U[k] * w[k][None, :]
+        assert np.abs(R).max() <= TOL * nrm * max(1, n / 64), (k, np.abs(R).max() / nrm)
+        O = U[k].conj().T @ U[k] - np.eye(n)
+        assert np.abs(O).max() <= TOL * max(1, n / 64), (k, np.abs(O).max())
+
+
+@pytest.mark.parametrize("n,b", [(1, 3), (2, 5), (3, 4), (7, 3), (16, 8), (33, 4), (64, 3), (72, 6), (128, 2),
+                                 (255, 2), (256, 3)])
+def test_random_hermitian(n, b):
+    import torch
+    from rlm_amd import heevd_batched
+    print(f"[test_random_hermitian] n={n} b={b}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    H = _rand_herm(rng, b, n)
+    w, U = heevd_batched(torch.from_numpy(H).cuda())
+    _check(H, w.cpu().numpy(), U.cpu().numpy())
+
+
+@pytest.mark.parametrize("n", [384, 512])
+def test_large_sizes(n):
+    import torch
+    from rlm_amd import heevd_batched
+    print(f"[test_large_sizes] n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    H = _rand_herm(rng, 2, n)
+    w, U = heevd_batched(torch.from_numpy(H).cuda())
+    _check(H, w.cpu().numpy(), U.cpu().numpy())
+
+
+def test_degenerate_and_structured():
+    import torch
+    from rlm_amd import heevd_batched
+    print("[test_degenerate_and_structured]", flush=True)
+    n = 40
+    rng = np.random.default_rng(7)
+    mats = []
+    mats.append(np.eye(n, dtype=np.complex128))                      # fully degenerate
+    mats.append(np.zeros((n, n), dtype=np.complex128))               # zero matrix
+    mats.append(np.diag(np.arange(n, dtype=float)).astype(np.complex128))   # already diagonal
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    ev = np.repeat([-1.0, 0.5, 2.0, 2.0 + 1e-13], n // 4)
+    mats.append(Q @ np.diag(ev) @ Q.conj().T)                        # clusters
+    T = np.diag(np.abs(np.arange(n) - n // 2).astype(float)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    mats.append(T.astype(np.complex128))                             # Wilkinson-like (real)
+    mats.append(1e-100 * _rand_herm(rng, 1, n)[0])                   # tiny scale
+    mats.append(1e100 * _rand_herm(rng, 1, n)[0])                    # huge scale
+    H = np.stack([(M + M.conj().T) / 2 for M in mats])
+    w, U = heevd_batched(torch.from_numpy(H).cuda())
+    _check(H, w.cpu().numpy(), U.cpu().numpy())
+
+
+def test_uplo_and_hamil_entry():
+    """hip_diag_hamil reads only the lower triangle (cusolver_diag.cu:8); hip_heevd reads 'U' (mkl_diag.cpp:6)."""
+    import ctypes
+    import torch
+    from rlm_amd import lib
+    print("[test_uplo_and_hamil_entry]", flush=True)
+    n = 48
+    rng = np.random.default_rng(3)
+    H = _rand_herm(rng, 1, n)[0]
+    junk = H.copy()
+    iu = np.triu_indices(n, 1)
+    junk[iu] = 99.0 + 7j                       # garbage in the strict upper triangle
+    A = torch.from_numpy(junk.T.copy()).cuda()  # column-major buffer of `junk`
+    w = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert lib().hip_diag_hamil(A.data_ptr(), w.data_ptr(), n) == 0
+    wr = np.linalg.eigvalsh(H)
+    assert np.abs(w.cpu().numpy() - wr).max() <= 1e-12 * np.linalg.norm(H, 2)
+    U = A.cpu().numpy().T
+    assert np.abs(H @ U - U * w.cpu().numpy()[None, :]).max() <= 1e-12 * np.linalg.norm(H, 2)
+    # host entry, upper triangle read
+    junk2 = H.copy()
+    il = np.tril_indices(n, -1)
+    junk2[il] = -5.0
+    buf = np.ascontiguousarray(junk2.T)        # column-major host buffer
+    wh = np.zeros(n)
+    assert lib().hip_heevd(buf.ctypes.data_as(ctypes.c_void_p), wh.ctypes.data_as(ctypes.c_void_p), n) == 0
+    assert np.abs(wh - wr).max() <= 1e-12 * np.linalg.norm(H, 2)
+
+
+def test_invalid_arguments():
+    from rlm_amd import lib
+    print("[test_invalid_arguments]", flush=True)
+    assert lib().hip_diag_heevd_batched(0, None, 1, 1, None, 1, 1, b"L", 0, None) == -22
+    assert lib().hip_diag_heevd_batched(600, None, 600, 600 * 600, None, 600, 1, b"L", 0, None) == -22
+    assert lib().hip_diag_heevd_batched(4, None, 4, 16, None, 4, 1, b"X", 0, None) == -22
+    assert lib().hip_diag_heevd_batched(4, None, 4, 16, None, 4, 0, b"L", 0, None) == 0
