@@ -100,7 +100,8 @@ struct EigWork {
 
 // Eigen-decomposition of `batch` packed (lda = n, stride n*n) full Hermitian
 // matrices in A. Eigenvalues -> W (stride n), eigenvectors component-major -> Uout.
-int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &ws, hipStream_t s) {
+int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &ws, hipStream_t s,
+               hipEvent_t mid = nullptr) {
     int rc;
     const long strideV = 4L * n;   // d | e | tau(re,im) per matrix, interleaved blocks
     double *base = ws.dev.d();
@@ -110,6 +111,7 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
     // TAU is read with a complex stride of strideV doubles -> pass complex-stride strideV/2
     const int gt = (int)(batch < kGridTrd ? batch : kGridTrd);
     if ((rc = launch_zhetrd(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, gt, s))) return rc;
+    if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
     const int gl = (int)(batch < kGridTail ? batch : kGridTail);
     if ((rc = launch_tail(A, (long)n * n, n, n, D, E, TAU, strideV, W, n, Uout, (long)n * n, ws.work.d(), (int)batch,
                           gl, s)))
@@ -304,6 +306,8 @@ struct rlm_plan_s {
     EigWork ws;
     int loaded = 0;
     long first = 0;
+    static constexpr int kStages = 6;
+    hipEvent_t ev[kStages + 1] = {};
 };
 
 extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double imp, long n_mesh, long off_override,
@@ -333,6 +337,8 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
         return r;
     };
     if (hipStreamCreate(&p->stream) != hipSuccess) return fail(-1);
+    for (int k = 0; k <= rlm_plan_s::kStages; k++)
+        if (hipEventCreate(&p->ev[k]) != hipSuccess) return fail(-1);
     if ((rc = p->dcmn.ensure(8 * p->cmn.size())) || (rc = p->dcmt.ensure(8 * p->cmt.size())) ||
         (rc = p->dE.ensure(16 * g1 * d * N)) || (rc = p->dX.ensure(8 * S * p->n_imp + 8)) ||
         (rc = p->dY.ensure(8 * S * p->n_imp + 8)) || (rc = p->dS.ensure(8 * S * p->n_imp + 8)) ||
@@ -382,17 +388,23 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     const long g1 = g + 1, nkp = (long)S * g1 * g1;
     hipStream_t s = p->stream;
     int rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
     if ((rc = launch_potential(p->dX.d(), p->dY.d(), p->dS.d(), p->n_imp, p->dcmn.d(), d, off, p->dV.d(), S, s)))
         return rc;
     if ((rc = launch_wtable(p->dV.d(), p->dcmt.d(), N, d, off, g, S, p->dW.d(), s))) return rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[1], s));
     if ((rc = launch_assemble(p->dE.d(), p->dW.d(), N, d, off, g, 0, nkp, p->dA.d(), (long)N * N, s))) return rc;
-    if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s))) return rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[2], s));
+    if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[3]))) return rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
     double *L1 = p->dL.d(), *L2 = L1 + 2 * nkp * N;
     if ((rc = launch_links(p->dU.d(), (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
     double *out = p->dOut.d();
     if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, (int)g1, 0, (double)N, S, out, out + (long)S * N,
                             out + 2L * S * N, s)))
         return rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
     return 0;
 }
 
@@ -437,6 +449,17 @@ extern "C" int rlm_plan_info(rlm_plan_t p, long *o) {
     return 0;
 }
 
+extern "C" int rlm_plan_stage_times(rlm_plan_t p, double *ms) {
+    if (!p || !ms) return -22;
+    RLM_HIP_CHECK(hipEventSynchronize(p->ev[rlm_plan_s::kStages]));
+    for (int k = 0; k < rlm_plan_s::kStages; k++) {
+        float t = 0.f;
+        RLM_HIP_CHECK(hipEventElapsedTime(&t, p->ev[k], p->ev[k + 1]));
+        ms[k] = t;
+    }
+    return rlm_plan_s::kStages;
+}
+
 extern "C" const double *rlm_plan_eigenvalues(rlm_plan_t p) { return p ? p->dEig.d() : nullptr; }
 extern "C" const double *rlm_plan_eigenvectors(rlm_plan_t p) { return p ? p->dU.d() : nullptr; }
 
@@ -447,6 +470,8 @@ extern "C" int rlm_plan_destroy(rlm_plan_t p) {
                       &p->dW,   &p->dA,   &p->dU, &p->dEig, &p->dL, &p->dOut};
     for (DevBuf *b : bufs) b->release();
     p->ws.release();
+    for (int k = 0; k <= rlm_plan_s::kStages; k++)
+        if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return 0;

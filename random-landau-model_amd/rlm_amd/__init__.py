@@ -57,6 +57,7 @@ def lib() -> ctypes.CDLL:
         "rlm_plan_fetch": ([vp, dp, dp, dp], i),
         "rlm_plan_run": ([vp, l, l, dp, dp, dp], i),
         "rlm_plan_stream": ([vp], vp),
+        "rlm_plan_stage_times": ([vp, dp], i),
         "rlm_plan_info": ([vp, ctypes.POINTER(ctypes.c_long)], i),
         "rlm_plan_eigenvalues": ([vp], vp),
         "rlm_plan_eigenvectors": ([vp], vp),
@@ -166,6 +167,15 @@ class RlmPlan:
         _check(lib().rlm_plan_run(self._h, first_sample, n_samples, _ptr(e), _ptr(c), _ptr(m)), "rlm_plan_run")
         shp = (n_samples, self.n_phi)
         return e.reshape(shp), c.reshape(shp), m.reshape(shp)
+
+    STAGES = ("potential+tables", "assembly", "tridiagonalisation", "dc+backtransform", "links", "plaquettes")
+
+    def stage_times(self) -> dict:
+        ms = np.zeros(6)
+        rc = lib().rlm_plan_stage_times(self._h, _ptr(ms))
+        if rc != 6:
+            raise HipDiagError(f"rlm_plan_stage_times failed with status {rc}")
+        return dict(zip(self.STAGES, ms.tolist()))
 
     @property
     def stream(self) -> int:
