@@ -18,8 +18,15 @@
 extern "C" {
 #endif
 
-/* Precision modes of the batched solver. */
+/* Precision modes of the batched solver.
+ * HIP_DIAG_FP64: every stage in FP64.
+ * HIP_DIAG_MIXED (rlm plans): FP32 tridiagonalisation and back-transform with FP32
+ * accumulation, FP64 divide & conquer, FP64 Rayleigh quotients of the FP32
+ * eigenvectors against the FP64 matrix (the reported energies), and link overlaps
+ * of the FP32 vectors accumulated in FP64 (each product of two floats is exact in
+ * FP64). */
 #define HIP_DIAG_FP64 0
+#define HIP_DIAG_MIXED 1
 
 /* ------------------------------------------------------------------------ */
 /* diag_wrappers-level entry points                                          */
@@ -121,11 +128,38 @@ void *rlm_plan_stream(rlm_plan_t plan);
 /* Sizes derived as rlm/main_rlm_cpu.cpp:29-34: out[0]=n_phi, [1]=impurity_num,
  * [2]=off_head, [3]=dim_m, [4]=n_mesh, [5]=samples_per_batch. */
 int rlm_plan_info(rlm_plan_t plan, long *out6);
+/* Mixed precision: the largest ||A u - lambda u|| / ||u|| (FP64, against the FP64
+ * matrix) over every state and twist point of the last execute; 0 for FP64 plans. */
+int rlm_plan_max_residual(rlm_plan_t plan, double *out);
 /* Device pointer of the eigenvalues [kp][n_phi] / eigenvectors
- * [kp][component][state] of the last execute (kp = (s*(g+1)+t1)*(g+1)+t2). */
+ * [kp][component][state] of the last execute (kp = (s*(g+1)+t1)*(g+1)+t2);
+ * eigenvectors are complex128 (FP64 plans) or complex64 (mixed plans). */
 const double *rlm_plan_eigenvalues(rlm_plan_t plan);
 const double *rlm_plan_eigenvectors(rlm_plan_t plan);
 int rlm_plan_destroy(rlm_plan_t plan);
+
+/* Replaces the hlm sample / theta loops of hlm/main_hlm_cpu.cpp:75-196 (with
+ * hlm/hamiltonian_hlm_cpu.cpp:2-56,149-184): lx x ly lattice, flux 1/Q, the
+ * Hamiltonian projected on Bloch band `band` (n_phi = lx*ly/Q states), twist
+ * mesh theta_1 = 0..g, theta_2 = 0..g-1 (theta_2 = g is theta_2 = 0). The
+ * on-site disorder of sample s is the reference's single MT19937 stream,
+ * V = (W/2) * U[-1,1) (W = 2 is the reference, hlm/main_hlm_cpu.cpp:79).
+ * Requires lx % Q == 0 (the reference's k-space basis assumes it). */
+typedef struct hlm_plan_s *hlm_plan_t;
+int hlm_plan_create(hlm_plan_t *plan, int lx, int ly, int Q, int band, int n_mesh, unsigned long seed, double W,
+                    int samples_per_batch, int precision);
+int hlm_plan_load(hlm_plan_t plan, long first_sample, int count);
+int hlm_plan_execute(hlm_plan_t plan);
+int hlm_plan_fetch(hlm_plan_t plan, double *energy, double *chern, double *margin);
+int hlm_plan_run(hlm_plan_t plan, long first_sample, long n_samples, double *energy, double *chern, double *margin);
+/* [0] projection Phi^H diag(V) Phi, [1] tridiagonalisation, [2] divide & conquer
+ * + back-transform, [3] link overlaps, [4] plaquettes + energies, [5] unused. */
+int hlm_plan_stage_times(hlm_plan_t plan, double *ms);
+void *hlm_plan_stream(hlm_plan_t plan);
+int hlm_plan_destroy(hlm_plan_t plan);
+/* hlm/main_hlm_cpu.cpp:66-67,79: samples first..first+n_samples-1 of the run's
+ * single stream, n_sites values each, scaled to [-W/2, W/2). Host memory. */
+int hlm_generate_disorder(double *V, long n_sites, long n_samples, unsigned long seed, long first_sample, double W);
 
 #ifdef __cplusplus
 }

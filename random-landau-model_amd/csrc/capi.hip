@@ -12,6 +12,7 @@
 #include "chern.hpp"
 #include "common.hpp"
 #include "eig.hpp"
+#include "hlm_kernels.hpp"
 #include "host_rng.hpp"
 #include "rlm_kernels.hpp"
 
@@ -24,39 +25,40 @@ constexpr int kGridTail = 512;   // persistent workgroups for D&C + back-transfo
 
 // ---------------------------------------------------------------- small kernels
 // Fill the triangle opposite to `uplo` by conjugate mirroring; real diagonal.
+// grid.x = batch (y/z grid dims are limited to 65535), threads over elements.
 __global__ void hermitize_kernel(double *A, long lda, long strideA, int n, int from_lower) {
-    double *M = A + 2 * blockIdx.y * strideA;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)n * n) return;
-    const int c = (int)(idx / n), r = (int)(idx % n);
-    if (r == c) {
-        M[2 * ((long)c * lda + r) + 1] = 0.0;
-        return;
+    double *M = A + 2 * blockIdx.x * strideA;
+    for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+        const int c = (int)(idx / n), r = (int)(idx % n);
+        if (r == c) {
+            M[2 * ((long)c * lda + r) + 1] = 0.0;
+            continue;
+        }
+        const bool dst = from_lower ? (r < c) : (r > c);
+        if (!dst) continue;
+        const cd v = ld(M, (long)r * lda + c);  // (c, r) element
+        st(M, (long)c * lda + r, cconj(v));
     }
-    const bool dst = from_lower ? (r < c) : (r > c);
-    if (!dst) return;
-    const cd v = ld(M, (long)r * lda + c);  // (c, r) element
-    st(M, (long)c * lda + r, cconj(v));
 }
 
 // A(r, j) = U[r*n + j] (component-major -> column-major eigenvectors)
 __global__ void transpose_out_kernel(const double *U, long strideU, int n, double *A, long lda, long strideA) {
-    const double *Ub = U + 2 * blockIdx.y * strideU;
-    double *M = A + 2 * blockIdx.y * strideA;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)n * n) return;
-    const int j = (int)(idx / n), r = (int)(idx % n);
-    st(M, (long)j * lda + r, ld(Ub, (long)r * n + j));
+    const double *Ub = U + 2 * blockIdx.x * strideU;
+    double *M = A + 2 * blockIdx.x * strideA;
+    for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+        const int j = (int)(idx / n), r = (int)(idx % n);
+        st(M, (long)j * lda + r, ld(Ub, (long)r * n + j));
+    }
 }
 
 // Full-matrix copy with lda (column-major) -> packed lda = n
 __global__ void pack_kernel(const double *A, long lda, long strideA, int n, double *P) {
-    const double *M = A + 2 * blockIdx.y * strideA;
-    double *Q = P + 2 * (long)blockIdx.y * n * n;
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)n * n) return;
-    const int c = (int)(idx / n), r = (int)(idx % n);
-    st(Q, idx, ld(M, (long)c * lda + r));
+    const double *M = A + 2 * blockIdx.x * strideA;
+    double *Q = P + 2 * (long)blockIdx.x * n * n;
+    for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+        const int c = (int)(idx / n), r = (int)(idx % n);
+        st(Q, idx, ld(M, (long)c * lda + r));
+    }
 }
 
 // ---------------------------------------------------------------- device buffer
@@ -119,6 +121,25 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
     return 0;
 }
 
+// FP32 tridiagonalisation and back-transform (mixed precision); D&C in FP64.
+int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork &ws, hipStream_t s,
+                   hipEvent_t mid = nullptr) {
+    int rc;
+    const long strideV = 4L * n;
+    double *base = ws.dev.d();
+    double *D = base, *E = base + n;
+    float *TAU = reinterpret_cast<float *>(base + 2 * n);   // 2n floats inside the 2n-double slot
+    const int gt = (int)(batch < kGridTrd ? batch : kGridTrd);
+    // TAU uses a stride of 2*strideV floats (= strideV doubles) per matrix
+    if ((rc = launch_chetrd(A, (long)n * n, n, n, D, E, TAU, 2 * strideV, (int)batch, gt, s))) return rc;
+    if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
+    const int gl = (int)(batch < kGridTail ? batch : kGridTail);
+    if ((rc = launch_tail_f32(A, (long)n * n, n, n, D, E, TAU, 2 * strideV, W, n, Uout, (long)n * n, ws.work.d(),
+                              (int)batch, gl, s)))
+        return rc;
+    return 0;
+}
+
 std::mutex g_single_mu;
 EigWork g_single_ws;
 DevBuf g_single_pack;
@@ -143,7 +164,7 @@ extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA
     double *P = g_single_pack.d();
     double *Wp = P + 2L * n * n * batch;
     const long nn = (long)n * n;
-    dim3 grid((unsigned)((nn + 255) / 256), (unsigned)batch);
+    dim3 grid((unsigned)batch);
     hipLaunchKernelGGL(pack_kernel, grid, dim3(256), 0, s, d_A, lda, strideA, n, P);
     hipLaunchKernelGGL(hermitize_kernel, grid, dim3(256), 0, s, P, (long)n, nn, n,
                        (uplo == 'L' || uplo == 'l') ? 1 : 0);
@@ -302,7 +323,7 @@ struct rlm_plan_s {
     // host
     std::vector<double> cmn, cmt, cjm, hx, hy, hs;
     // device
-    DevBuf dcmn, dcmt, dE, dX, dY, dS, dV, dW, dA, dU, dEig, dL, dOut;
+    DevBuf dcmn, dcmt, dE, dX, dY, dS, dV, dW, dA, dU, dEig, dL, dOut, dA32, dResid;
     EigWork ws;
     int loaded = 0;
     long first = 0;
@@ -313,7 +334,7 @@ struct rlm_plan_s {
 extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double imp, long n_mesh, long off_override,
                                unsigned long seed, int samples_per_batch, int precision) {
     if (!plan || n_phi < 1 || n_phi > 512 || n_mesh < 1 || samples_per_batch < 1 || q <= 0.0) return -22;
-    if (precision != HIP_DIAG_FP64) return -22;
+    if (precision != HIP_DIAG_FP64 && precision != HIP_DIAG_MIXED) return -22;
     auto *p = new rlm_plan_s();
     p->n_phi = n_phi;
     p->L1 = p->L2 = std::sqrt((double)n_phi * q);                     // rlm/main_rlm_cpu.cpp:29-30
@@ -343,7 +364,10 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
         (rc = p->dE.ensure(16 * g1 * d * N)) || (rc = p->dX.ensure(8 * S * p->n_imp + 8)) ||
         (rc = p->dY.ensure(8 * S * p->n_imp + 8)) || (rc = p->dS.ensure(8 * S * p->n_imp + 8)) ||
         (rc = p->dV.ensure(16 * S * d * d)) || (rc = p->dW.ensure(16 * S * g1 * d * N)) ||
-        (rc = p->dA.ensure(16 * nkp * N * N)) || (rc = p->dU.ensure(16 * nkp * N * N)) ||
+        (rc = p->dA.ensure(16 * nkp * N * N)) ||
+        (rc = p->dU.ensure((precision == HIP_DIAG_MIXED ? 8 : 16) * nkp * N * N)) ||
+        (precision == HIP_DIAG_MIXED && (rc = p->dA32.ensure(8 * nkp * N * N))) ||
+        (rc = p->dResid.ensure(8 * nkp)) ||
         (rc = p->dEig.ensure(8 * nkp * N)) || (rc = p->dL.ensure(32 * nkp * N)) ||
         (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)))
         return fail(rc);
@@ -393,12 +417,24 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
         return rc;
     if ((rc = launch_wtable(p->dV.d(), p->dcmt.d(), N, d, off, g, S, p->dW.d(), s))) return rc;
     RLM_HIP_CHECK(hipEventRecord(p->ev[1], s));
-    if ((rc = launch_assemble(p->dE.d(), p->dW.d(), N, d, off, g, 0, nkp, p->dA.d(), (long)N * N, s))) return rc;
+    const bool mixed = p->precision == HIP_DIAG_MIXED;
+    float *A32 = mixed ? static_cast<float *>(p->dA32.p) : nullptr;
+    if ((rc = launch_assemble(p->dE.d(), p->dW.d(), N, d, off, g, 0, nkp, p->dA.d(), (long)N * N, s, A32)))
+        return rc;
     RLM_HIP_CHECK(hipEventRecord(p->ev[2], s));
-    if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[3]))) return rc;
-    RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
     double *L1 = p->dL.d(), *L2 = L1 + 2 * nkp * N;
-    if ((rc = launch_links(p->dU.d(), (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
+    if (!mixed) {
+        if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[3]))) return rc;
+        RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
+        if ((rc = launch_links(p->dU.d(), (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
+    } else {
+        float *U32 = static_cast<float *>(p->dU.p);
+        if ((rc = eig_packed_f32(A32, N, nkp, p->dEig.d(), U32, p->ws, s, p->ev[3]))) return rc;
+        // FP64 Rayleigh quotients against the FP64 matrix (band |c-r| <= off_head)
+        if ((rc = launch_rayleigh(p->dA.d(), U32, N, off, nkp, p->dEig.d(), p->dResid.d(), s))) return rc;
+        RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
+        if ((rc = launch_links_f32(U32, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
+    }
     RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
     double *out = p->dOut.d();
     if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, (int)g1, 0, (double)N, S, out, out + (long)S * N,
@@ -460,17 +496,228 @@ extern "C" int rlm_plan_stage_times(rlm_plan_t p, double *ms) {
     return rlm_plan_s::kStages;
 }
 
+extern "C" int rlm_plan_max_residual(rlm_plan_t p, double *out) {
+    if (!p || !out || p->loaded < 1) return -22;
+    if (p->precision != HIP_DIAG_MIXED) {
+        *out = 0.0;
+        return 0;
+    }
+    const long nkp = (long)p->loaded * (p->g + 1) * (p->g + 1);
+    std::vector<double> h(nkp);
+    RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+    RLM_HIP_CHECK(hipMemcpy(h.data(), p->dResid.p, 8 * nkp, hipMemcpyDeviceToHost));
+    double m = 0.0;
+    for (double v : h) m = v > m ? v : m;
+    *out = m;
+    return 0;
+}
+
 extern "C" const double *rlm_plan_eigenvalues(rlm_plan_t p) { return p ? p->dEig.d() : nullptr; }
 extern "C" const double *rlm_plan_eigenvectors(rlm_plan_t p) { return p ? p->dU.d() : nullptr; }
 
 extern "C" int rlm_plan_destroy(rlm_plan_t p) {
     if (!p) return 0;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    DevBuf *bufs[] = {&p->dcmn, &p->dcmt, &p->dE, &p->dX, &p->dY, &p->dS, &p->dV,
-                      &p->dW,   &p->dA,   &p->dU, &p->dEig, &p->dL, &p->dOut};
+    DevBuf *bufs[] = {&p->dcmn, &p->dcmt, &p->dE, &p->dX,   &p->dY, &p->dS,   &p->dV,    &p->dW,
+                      &p->dA,   &p->dU,   &p->dEig, &p->dL, &p->dOut, &p->dA32, &p->dResid};
     for (DevBuf *b : bufs) b->release();
     p->ws.release();
     for (int k = 0; k <= rlm_plan_s::kStages; k++)
+        if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return 0;
+}
+
+// ============================================================================
+// hlm batched pipeline
+// ============================================================================
+extern "C" int hlm_generate_disorder(double *V, long n_sites, long n_samples, unsigned long seed, long first_sample,
+                                     double W) {
+    if (!V || n_sites < 1 || n_samples < 0 || first_sample < 0) return -22;
+    Mt19937 gen((uint32_t)seed);
+    gen.discard((uint64_t)n_sites * (uint64_t)first_sample);
+    const double half = 0.5 * W;
+    for (long i = 0; i < n_sites * n_samples; i++) {
+        const double u = gen.uniform(-1.0, 1.0);   // hlm/main_hlm_cpu.cpp:79
+        V[i] = (W == 2.0) ? u : half * u;
+    }
+    return 0;
+}
+
+struct hlm_plan_s {
+    int lx, ly, Q, band, g, n_sites, n_phi, spb;
+    uint32_t seed;
+    double W;
+    int precision;
+    long ntw;                       // (g+1)*g twist points per sample
+    hipStream_t stream = nullptr;
+    std::vector<double> hV;
+    DevBuf dPhi, dM1, dM2, dV, dH, dU, dEig, dL, dOut;
+    EigWork ws;
+    int loaded = 0;
+    static constexpr int kStages = 6;
+    hipEvent_t ev[kStages + 1] = {};
+};
+
+extern "C" int hlm_plan_destroy(hlm_plan_t p);
+
+extern "C" int hlm_plan_create(hlm_plan_t *plan, int lx, int ly, int Q, int band, int n_mesh, unsigned long seed,
+                               double W, int samples_per_batch, int precision) {
+    if (!plan || lx < 1 || ly < 1 || Q < 1 || lx % Q != 0 || band < 0 || band >= Q || n_mesh < 1 ||
+        samples_per_batch < 1 || precision != HIP_DIAG_FP64 || !(W > 0.0))
+        return -22;
+    const int n_sites = lx * ly, n_phi = n_sites / Q;
+    if (n_phi > 512 || Q > 512) return -22;
+    auto *p = new hlm_plan_s();
+    p->lx = lx; p->ly = ly; p->Q = Q; p->band = band; p->g = n_mesh;
+    p->n_sites = n_sites; p->n_phi = n_phi; p->spb = samples_per_batch;
+    p->seed = (uint32_t)seed; p->W = W; p->precision = precision;
+    p->ntw = (long)(n_mesh + 1) * n_mesh;
+    auto fail = [&](int r) {
+        hlm_plan_destroy(p);
+        return r;
+    };
+    if (hipStreamCreate(&p->stream) != hipSuccess) return fail(-1);
+    for (int k = 0; k <= hlm_plan_s::kStages; k++)
+        if (hipEventCreate(&p->ev[k]) != hipSuccess) return fail(-1);
+    const long N = n_phi, S = samples_per_batch, ntw = p->ntw, nkp = S * ntw;
+    int rc;
+    if ((rc = p->dPhi.ensure(16 * ntw * n_sites * N)) || (rc = p->dM1.ensure(16 * ntw * N * N)) ||
+        (rc = p->dM2.ensure(16 * ntw * N * N)) || (rc = p->dV.ensure(8 * S * n_sites)) ||
+        (rc = p->dH.ensure(16 * nkp * N * N)) || (rc = p->dU.ensure(16 * nkp * N * N)) ||
+        (rc = p->dEig.ensure(8 * nkp * N)) || (rc = p->dL.ensure(32 * nkp * N)) ||
+        (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)))
+        return fail(rc);
+    // Bloch basis for every twist (disorder independent)
+    {
+        const long nmat = ntw * N;
+        DevBuf bH, bW, bU;
+        EigWork bws;
+        if ((rc = bH.ensure(16 * nmat * Q * Q)) || (rc = bW.ensure(8 * nmat * Q)) ||
+            (rc = bU.ensure(16 * nmat * Q * Q)) || (rc = bws.ensure(Q, nmat, false)))
+            return fail(rc);
+        if ((rc = launch_bloch_build(lx, ly, Q, n_mesh, nmat, bH.d(), p->stream))) return fail(rc);
+        if ((rc = eig_packed(bH.d(), Q, nmat, bW.d(), bU.d(), bws, p->stream))) return fail(rc);
+        if ((rc = launch_phi_build(lx, ly, Q, band, n_mesh, nmat, bU.d(), p->dPhi.d(), p->stream))) return fail(rc);
+        if (hipStreamSynchronize(p->stream) != hipSuccess) return fail(-1);
+    }
+    // link matrices M1(t1,t2) = Phi(t1,t2)^H Phi(t1+1,t2), M2(t1,t2) = Phi(t1,t2)^H Phi(t1,(t2+1)%g)
+    {
+        const long sz = (long)n_sites * N;        // complex elements per Phi
+        const long nn = N * N;
+        const double *Phi = p->dPhi.d();
+        const int g = n_mesh;
+        // M1: batch over tw in [0, g*g)
+        if ((rc = launch_zgemm((int)N, (int)N, n_sites, Phi, 1, n_sites, sz, 0, 1, Phi + 2 * g * sz, 1, n_sites, sz,
+                               0, nullptr, 0, 0, p->dM1.d(), 1, N, nn, 0, g * g, 1, p->stream)))
+            return fail(rc);
+        // M2 for t2 < g-1: b1 = t1 (stride g), b2 = t2
+        if (g > 1 &&
+            (rc = launch_zgemm((int)N, (int)N, n_sites, Phi, 1, n_sites, g * sz, sz, 1, Phi + 2 * sz, 1, n_sites,
+                               g * sz, sz, nullptr, 0, 0, p->dM2.d(), 1, N, g * nn, nn, g + 1, g - 1, p->stream)))
+            return fail(rc);
+        // M2 for t2 = g-1 -> t2 = 0 of the same line
+        if ((rc = launch_zgemm((int)N, (int)N, n_sites, Phi + 2 * (g - 1) * sz, 1, n_sites, g * sz, 0, 1, Phi, 1,
+                               n_sites, g * sz, 0, nullptr, 0, 0, p->dM2.d() + 2 * (g - 1) * nn, 1, N, g * nn, 0,
+                               g + 1, 1, p->stream)))
+            return fail(rc);
+        if (hipStreamSynchronize(p->stream) != hipSuccess) return fail(-1);
+    }
+    *plan = p;
+    return 0;
+}
+
+extern "C" int hlm_plan_load(hlm_plan_t p, long first_sample, int count) {
+    if (!p || count < 1 || count > p->spb || first_sample < 0) return -22;
+    p->hV.resize((size_t)count * p->n_sites);
+    int rc = hlm_generate_disorder(p->hV.data(), p->n_sites, count, p->seed, first_sample, p->W);
+    if (rc) return rc;
+    RLM_HIP_CHECK(hipMemcpyAsync(p->dV.p, p->hV.data(), 8 * p->hV.size(), hipMemcpyHostToDevice, p->stream));
+    RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+    p->loaded = count;
+    return 0;
+}
+
+extern "C" int hlm_plan_execute(hlm_plan_t p) {
+    if (!p || p->loaded < 1) return -22;
+    const int S = p->loaded, N = p->n_phi, ns = p->n_sites, g = p->g;
+    const long ntw = p->ntw, nkp = S * ntw, nn = (long)N * N, sz = (long)ns * N;
+    hipStream_t s = p->stream;
+    int rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
+    // Htr[(s*ntw + tw)] = Phi(tw)^H diag(V_s) Phi(tw): b1 = sample, b2 = twist
+    if ((rc = launch_zgemm(N, N, ns, p->dPhi.d(), 1, ns, 0, sz, 1, p->dPhi.d(), 1, ns, 0, sz, p->dV.d(), ns, 0,
+                           p->dH.d(), 1, N, ntw * nn, nn, S, (int)ntw, s)))
+        return rc;
+    // the reference's zheevd reads the upper triangle of Htr (mkl_diag.cpp:6)
+    {
+        hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)nkp), dim3(256), 0, s, p->dH.d(), (long)N, nn, N, 0);
+        RLM_HIP_CHECK(hipGetLastError());
+    }
+    RLM_HIP_CHECK(hipEventRecord(p->ev[1], s));
+    if ((rc = eig_packed(p->dH.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[2]))) return rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[3], s));
+    double *L1 = p->dL.d(), *L2 = L1 + 2 * nkp * N;
+    if ((rc = launch_hlm_links(p->dU.d(), N, g, nkp, p->dM1.d(), p->dM2.d(), L1, L2, s))) return rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
+    double *out = p->dOut.d();
+    if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, g, 1, (double)N, S, out, out + (long)S * N, out + 2L * S * N,
+                            s)))
+        return rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
+    RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
+    return 0;
+}
+
+extern "C" int hlm_plan_fetch(hlm_plan_t p, double *energy, double *chern, double *margin) {
+    if (!p || p->loaded < 1) return -22;
+    const long SN = (long)p->loaded * p->n_phi;
+    RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+    std::vector<double> h(3 * SN);
+    RLM_HIP_CHECK(hipMemcpy(h.data(), p->dOut.p, 8 * 3 * SN, hipMemcpyDeviceToHost));
+    if (energy) memcpy(energy, h.data(), 8 * SN);
+    if (chern) memcpy(chern, h.data() + SN, 8 * SN);
+    if (margin) memcpy(margin, h.data() + 2 * SN, 8 * SN);
+    return 0;
+}
+
+extern "C" int hlm_plan_run(hlm_plan_t p, long first_sample, long n_samples, double *energy, double *chern,
+                            double *margin) {
+    if (!p || n_samples < 0) return -22;
+    for (long s0 = 0; s0 < n_samples; s0 += p->spb) {
+        const int cnt = (int)((n_samples - s0) < p->spb ? (n_samples - s0) : p->spb);
+        int rc;
+        if ((rc = hlm_plan_load(p, first_sample + s0, cnt))) return rc;
+        if ((rc = hlm_plan_execute(p))) return rc;
+        const long off = s0 * p->n_phi;
+        if ((rc = hlm_plan_fetch(p, energy ? energy + off : nullptr, chern ? chern + off : nullptr,
+                                 margin ? margin + off : nullptr)))
+            return rc;
+    }
+    return 0;
+}
+
+extern "C" int hlm_plan_stage_times(hlm_plan_t p, double *ms) {
+    if (!p || !ms) return -22;
+    RLM_HIP_CHECK(hipEventSynchronize(p->ev[hlm_plan_s::kStages]));
+    for (int k = 0; k < hlm_plan_s::kStages; k++) {
+        float t = 0.f;
+        RLM_HIP_CHECK(hipEventElapsedTime(&t, p->ev[k], p->ev[k + 1]));
+        ms[k] = t;
+    }
+    return hlm_plan_s::kStages;
+}
+
+extern "C" void *hlm_plan_stream(hlm_plan_t p) { return p ? (void *)p->stream : nullptr; }
+
+extern "C" int hlm_plan_destroy(hlm_plan_t p) {
+    if (!p) return 0;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    DevBuf *bufs[] = {&p->dPhi, &p->dM1, &p->dM2, &p->dV, &p->dH, &p->dU, &p->dEig, &p->dL, &p->dOut};
+    for (DevBuf *b : bufs) b->release();
+    p->ws.release();
+    for (int k = 0; k <= hlm_plan_s::kStages; k++)
         if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;

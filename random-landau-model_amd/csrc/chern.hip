@@ -24,7 +24,10 @@ namespace rlm {
 // (g+1 for rlm, g for hlm whose theta_2 = g is theta_2 = 0) and nlink2 links
 // along theta_2 per line (g in both models).
 // L1[kp][j] = <u(t1,t2)|u(t1+1,t2)> (t1 < lines-1); L2[kp][j] = <u(t1,t2)|u(t1,t2+1)>.
-__global__ __launch_bounds__(256) void links_kernel(const double *__restrict__ U, long strideU, int dim_vec, int ns,
+// TU = double (FP64 path) or float (mixed path: products of two floats are exact in
+// double, so the FP64 accumulation is a compensated dot product of the FP32 vectors).
+template <typename TU>
+__global__ __launch_bounds__(256) void links_kernel(const TU *__restrict__ U, long strideU, int dim_vec, int ns,
                                                     int lines, int L2n, int nlink2, long kp0,
                                                     double *__restrict__ Lk1, double *__restrict__ Lk2) {
     const long kp = kp0 + blockIdx.x;
@@ -35,15 +38,16 @@ __global__ __launch_bounds__(256) void links_kernel(const double *__restrict__ U
     const bool has2 = t2 < nlink2;
     const long kp_1 = kp + L2n;             // (t1+1, t2)
     const long kp_2 = base_line + ((t2 + 1) % L2n);  // (t1, t2+1), wraps to 0 for hlm
-    const double *u0 = U + 2 * (kp - kp0) * strideU;
-    const double *u1 = U + 2 * (kp_1 - kp0) * strideU;
-    const double *u2 = U + 2 * (kp_2 - kp0) * strideU;
+    const TU *u0 = U + 2 * (kp - kp0) * strideU;
+    const TU *u1 = U + 2 * (kp_1 - kp0) * strideU;
+    const TU *u2 = U + 2 * (kp_2 - kp0) * strideU;
+    auto up = [](cx<TU> v) { return cd{(double)v.x, (double)v.y}; };
     for (int j = threadIdx.x; j < ns; j += blockDim.x) {
         cd a1{0.0, 0.0}, a2{0.0, 0.0};
         for (int i = 0; i < dim_vec; i++) {
-            const cd x = ld(u0, (long)i * ns + j);
-            if (has1) cfmac(a1, x, ld(u1, (long)i * ns + j));
-            if (has2) cfmac(a2, x, ld(u2, (long)i * ns + j));
+            const cd x = up(ldx(u0, (long)i * ns + j));
+            if (has1) cfmac(a1, x, up(ldx(u1, (long)i * ns + j)));
+            if (has2) cfmac(a2, x, up(ldx(u2, (long)i * ns + j)));
         }
         st(Lk1, kp * ns + j, a1);
         st(Lk2, kp * ns + j, a2);
@@ -106,8 +110,17 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ 
 int launch_links(const double *U, long strideU, int dim_vec, int ns, int lines, int L2n, int nlink2, long kp0,
                  long nkp, double *Lk1, double *Lk2, hipStream_t stream) {
     if (nkp <= 0) return 0;
-    hipLaunchKernelGGL(links_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, U, strideU, dim_vec, ns, lines, L2n,
-                       nlink2, kp0, Lk1, Lk2);
+    hipLaunchKernelGGL(links_kernel<double>, dim3((unsigned)nkp), dim3(256), 0, stream, U, strideU, dim_vec, ns, lines,
+                       L2n, nlink2, kp0, Lk1, Lk2);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_links_f32(const float *U, long strideU, int dim_vec, int ns, int lines, int L2n, int nlink2, long kp0,
+                     long nkp, double *Lk1, double *Lk2, hipStream_t stream) {
+    if (nkp <= 0) return 0;
+    hipLaunchKernelGGL(links_kernel<float>, dim3((unsigned)nkp), dim3(256), 0, stream, U, strideU, dim_vec, ns, lines,
+                       L2n, nlink2, kp0, Lk1, Lk2);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -104,3 +104,91 @@ __device__ inline cd block_sum_c(cd v, double *red) {
 }
 
 }  // namespace rlm
+
+// ---------------------------------------------------------------- precision-generic complex
+namespace rlm {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+    T x, y;
+};
+
+template <typename T>
+__host__ __device__ inline cx<T> operator+(cx<T> a, cx<T> b) { return cx<T>{a.x + b.x, a.y + b.y}; }
+template <typename T>
+__host__ __device__ inline cx<T> operator-(cx<T> a, cx<T> b) { return cx<T>{a.x - b.x, a.y - b.y}; }
+template <typename T>
+__host__ __device__ inline cx<T> operator*(cx<T> a, cx<T> b) {
+    return cx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__host__ __device__ inline cx<T> operator*(cx<T> a, T s) { return cx<T>{a.x * s, a.y * s}; }
+template <typename T>
+__host__ __device__ inline cx<T> conjx(cx<T> a) { return cx<T>{a.x, -a.y}; }
+template <typename T>
+__host__ __device__ inline T abs2x(cx<T> a) { return a.x * a.x + a.y * a.y; }
+// acc += a*b
+template <typename T>
+__host__ __device__ inline void fmax_(cx<T> &acc, cx<T> a, cx<T> b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+// acc += conj(a)*b
+template <typename T>
+__host__ __device__ inline void fmacx(cx<T> &acc, cx<T> a, cx<T> b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+}
+template <typename T>
+__device__ inline cx<T> ldx(const T *p, long i) { return reinterpret_cast<const cx<T> *>(p)[i]; }
+template <typename T>
+__device__ inline void stx(T *p, long i, cx<T> v) { reinterpret_cast<cx<T> *>(p)[i] = v; }
+
+template <typename T>
+__device__ inline T wave_sum_t(T v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Block-wide sum; `red` is LDS scratch of >= 2*blockDim/64 elements of T. Two barriers.
+template <typename T>
+__device__ inline T block_sum_t(T v, T *red) {
+    v = wave_sum_t(v);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    T t = 0;
+    for (int i = 0; i < nw; i++) t += red[i];
+    __syncthreads();
+    return t;
+}
+
+template <typename T>
+__device__ inline cx<T> block_sum_cx(cx<T> v, T *red) {
+    v.x = wave_sum_t(v.x);
+    v.y = wave_sum_t(v.y);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (lane == 0) {
+        red[2 * wid] = v.x;
+        red[2 * wid + 1] = v.y;
+    }
+    __syncthreads();
+    cx<T> t{0, 0};
+    for (int i = 0; i < nw; i++) {
+        t.x += red[2 * i];
+        t.y += red[2 * i + 1];
+    }
+    __syncthreads();
+    return t;
+}
+
+}  // namespace rlm

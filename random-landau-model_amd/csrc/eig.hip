@@ -27,81 +27,83 @@ namespace rlm {
 // ============================================================================
 // 1. Householder tridiagonalisation
 // ============================================================================
-template <int NT, int MAXR>
-__global__ __launch_bounds__(NT) void zhetrd_kernel(double *__restrict__ Abase, long strideA, int lda, int n,
+template <typename T, int NT, int MAXR>
+__global__ __launch_bounds__(NT) void zhetrd_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
                                                     double *__restrict__ Dout, double *__restrict__ Eout,
-                                                    double *__restrict__ TAUout, long strideV, int batch) {
-    extern __shared__ double smem[];
+                                                    T *__restrict__ TAUout, long strideV, int batch) {
+    using C = cx<T>;
+    extern __shared__ double smem_d[];
     // LDS carve: v[2][n], w[2][n] (complex), part[NT/64][n] complex, red[2*NT/64], scal[8]
-    cd *vbuf = reinterpret_cast<cd *>(smem);          // 2*n
-    cd *wbuf = vbuf + 2 * n;                          // 2*n
-    cd *part = wbuf + 2 * n;                          // (NT/64)*n
-    double *red = reinterpret_cast<double *>(part + (NT / kWave) * n);
-    double *sc = red + 2 * (NT / kWave);              // 8 scalars
+    C *vbuf = reinterpret_cast<C *>(smem_d);          // 2*n
+    C *wbuf = vbuf + 2 * n;                           // 2*n
+    C *part = wbuf + 2 * n;                           // (NT/64)*n
+    T *red = reinterpret_cast<T *>(part + (NT / kWave) * n);
+    T *sc = red + 2 * (NT / kWave);                   // 8 scalars
     const int t = threadIdx.x;
 
     for (long b = blockIdx.x; b < batch; b += gridDim.x) {
-        double *A = Abase + 2 * b * strideA;           // complex index = 2*...
+        T *A = Abase + 2 * b * strideA;                // complex index = 2*...
         double *Dv = Dout + b * strideV;
         double *Ev = Eout + b * strideV;
-        double *Tv = TAUout + b * strideV;      // tau: complex, strideV doubles per matrix
+        T *Tv = TAUout + b * strideV;                  // tau: complex, strideV reals per matrix
         auto Aidx = [&](int r, int c) -> long { return (long)c * lda + r; };
 
         int cur = 0;          // vbuf/wbuf slot holding the pending update
         bool pending = false;
         for (int i = 0; i < n - 1; i++) {
-            cd *vold = vbuf + cur * n, *wold = wbuf + cur * n;   // indexed from row i
-            cd *vnew = vbuf + (cur ^ 1) * n;                     // indexed from row i+1
-            cd *wnew = wbuf + (cur ^ 1) * n;
+            C *vold = vbuf + cur * n, *wold = wbuf + cur * n;    // indexed from row i
+            C *vnew = vbuf + (cur ^ 1) * n;                      // indexed from row i+1
+            C *wnew = wbuf + (cur ^ 1) * n;
             // (a) bring column i (rows >= i) up to date, collect ||x||^2, alpha, d_i
-            double nrm2 = 0.0;
+            T nrm2 = 0;
             for (int r = i + t; r < n; r += NT) {
-                cd a = ld(A, Aidx(r, i));
+                C a = ldx(A, Aidx(r, i));
                 if (pending) {
-                    cd vr = vold[r - i], wr = wold[r - i];
-                    cd vc = vold[0], wc = wold[0];
+                    const C vr = vold[r - i], wr = wold[r - i];
+                    const C vc = vold[0], wc = wold[0];
                     // a -= v_r conj(w_c) + w_r conj(v_c)
-                    cd u1 = cmul(vr, cconj(wc)), u2 = cmul(wr, cconj(vc));
-                    a = csub(a, cadd(u1, u2));
-                    st(A, Aidx(r, i), a);
+                    a = a - (vr * conjx(wc) + wr * conjx(vc));
+                    stx(A, Aidx(r, i), a);
                 }
                 if (r == i) sc[0] = a.x;
                 else if (r == i + 1) { sc[1] = a.x; sc[2] = a.y; }
-                else nrm2 += cabs2(a);
+                else nrm2 += abs2x(a);
             }
             __syncthreads();
-            nrm2 = block_sum(nrm2, red);
+            nrm2 = block_sum_t(nrm2, red);
             if (t == 0) {
-                Dv[i] = sc[0];
-                double ar = sc[1], ai = sc[2];
-                double beta, taur, taui, sr, si;
-                if (nrm2 == 0.0 && ai == 0.0) {
-                    beta = ar; taur = 0.0; taui = 0.0; sr = 0.0; si = 0.0;
+                Dv[i] = (double)sc[0];
+                const T ar = sc[1], ai = sc[2];
+                T beta, taur, taui, sr, si;
+                if (nrm2 == (T)0 && ai == (T)0) {
+                    beta = ar; taur = 0; taui = 0; sr = 0; si = 0;
                 } else {
-                    double nrm = sqrt(ar * ar + ai * ai + nrm2);
-                    beta = (ar >= 0.0) ? -nrm : nrm;
+                    // zlarfg: beta = -sign(alphr) * ||(alpha, x)||, tau = (beta-alpha)/beta, x *= 1/(alpha-beta)
+                    const T nrm = sqrt(ar * ar + ai * ai + nrm2);
+                    beta = (ar >= (T)0) ? -nrm : nrm;
                     taur = (beta - ar) / beta;
                     taui = -ai / beta;
-                    cd s = cinv(cd{ar - beta, ai});
-                    sr = s.x; si = s.y;
+                    const T dr = ar - beta, di = ai;
+                    const T inv = (T)1 / (dr * dr + di * di);
+                    sr = dr * inv; si = -di * inv;
                 }
-                Ev[i] = beta;
+                Ev[i] = (double)beta;
                 Tv[2 * i] = taur;
                 Tv[2 * i + 1] = taui;
                 sc[3] = taur; sc[4] = taui; sc[5] = sr; sc[6] = si; sc[7] = beta;
             }
             __syncthreads();
-            const cd tau{sc[3], sc[4]};
-            const cd scal{sc[5], sc[6]};
+            const C tau{sc[3], sc[4]};
+            const C scal{sc[5], sc[6]};
             // (b) v = [1; x*scal], stored back into A(i+2:n, i); A(i+1,i) = beta
             for (int r = i + 1 + t; r < n; r += NT) {
-                cd v;
+                C v;
                 if (r == i + 1) {
-                    v = cd{1.0, 0.0};
-                    st(A, Aidx(r, i), cd{sc[7], 0.0});
+                    v = C{(T)1, (T)0};
+                    stx(A, Aidx(r, i), C{sc[7], (T)0});
                 } else {
-                    v = cmul(ld(A, Aidx(r, i)), scal);
-                    st(A, Aidx(r, i), v);
+                    v = ldx(A, Aidx(r, i)) * scal;
+                    stx(A, Aidx(r, i), v);
                 }
                 vnew[r - (i + 1)] = v;
             }
@@ -112,26 +114,26 @@ __global__ __launch_bounds__(NT) void zhetrd_kernel(double *__restrict__ Abase, 
             const int R = (nr >= NT) ? NT : ((nr + kWave - 1) / kWave) * kWave;
             const int G = NT / R;
             const int rl = t % R, cg = t / R;
-            cd acc[MAXR];
+            C acc[MAXR];
 #pragma unroll
-            for (int k = 0; k < MAXR; k++) acc[k] = cd{0.0, 0.0};
+            for (int k = 0; k < MAXR; k++) acc[k] = C{(T)0, (T)0};
             if (cg < G) {
                 for (int c = i + 1 + cg; c < n; c += G) {
-                    const cd vn = vnew[c - (i + 1)];
-                    cd voc{0, 0}, woc{0, 0};
-                    if (pending) { voc = cconj(vold[c - i]); woc = cconj(wold[c - i]); }
+                    const C vn = vnew[c - (i + 1)];
+                    C voc{(T)0, (T)0}, woc{(T)0, (T)0};
+                    if (pending) { voc = conjx(vold[c - i]); woc = conjx(wold[c - i]); }
 #pragma unroll
                     for (int k = 0; k < MAXR; k++) {
                         const int r = i + 1 + rl + k * R;
                         if (r < n) {
-                            cd a = ld(A, Aidx(r, c));
+                            C a = ldx(A, Aidx(r, c));
                             if (pending) {
-                                const cd vr = vold[r - i], wr = wold[r - i];
-                                a = csub(a, cadd(cmul(vr, woc), cmul(wr, voc)));
-                                if (r == c) a.y = 0.0;   // Hermitian diagonal stays real (zher2)
-                                st(A, Aidx(r, c), a);
+                                const C vr = vold[r - i], wr = wold[r - i];
+                                a = a - (vr * woc + wr * voc);
+                                if (r == c) a.y = (T)0;   // Hermitian diagonal stays real (zher2)
+                                stx(A, Aidx(r, c), a);
                             }
-                            cfma(acc[k], a, vn);
+                            fmax_(acc[k], a, vn);
                         }
                     }
                 }
@@ -143,33 +145,33 @@ __global__ __launch_bounds__(NT) void zhetrd_kernel(double *__restrict__ Abase, 
                 if (r < n) part[cg * n + (r - (i + 1))] = acc[k];
             }
             __syncthreads();
-            cd dotl{0.0, 0.0};
+            C dotl{(T)0, (T)0};
             for (int r = i + 1 + t; r < n; r += NT) {
-                cd p{0.0, 0.0};
-                for (int g = 0; g < G; g++) p = cadd(p, part[g * n + (r - (i + 1))]);
-                p = cmul(tau, p);
+                C p{(T)0, (T)0};
+                for (int g = 0; g < G; g++) p = p + part[g * n + (r - (i + 1))];
+                p = tau * p;
                 wnew[r - (i + 1)] = p;                        // p for now
-                dotl = cadd(dotl, cmulc(p, vnew[r - (i + 1)])); // conj(p) v
+                fmacx(dotl, p, vnew[r - (i + 1)]);             // conj(p) v
             }
-            cd dot = block_sum_c(dotl, red);
+            const C dot = block_sum_cx(dotl, red);
             // alpha2 = -1/2 tau (p^H v); w = p + alpha2 v
-            const cd alpha2 = cscale(cmul(tau, dot), -0.5);
+            const C alpha2 = (tau * dot) * (T)(-0.5);
             for (int r = i + 1 + t; r < n; r += NT) {
-                cd p = wnew[r - (i + 1)];
-                wnew[r - (i + 1)] = cadd(p, cmul(alpha2, vnew[r - (i + 1)]));
+                const C p = wnew[r - (i + 1)];
+                wnew[r - (i + 1)] = p + alpha2 * vnew[r - (i + 1)];
             }
             __syncthreads();
-            pending = (tau.x != 0.0 || tau.y != 0.0);
+            pending = (tau.x != (T)0 || tau.y != (T)0);
             cur ^= 1;
         }
         // last diagonal element
         if (t == 0) {
-            cd a = ld(A, Aidx(n - 1, n - 1));
+            C a = ldx(A, Aidx(n - 1, n - 1));
             if (pending && n >= 2) {
-                cd v = vbuf[cur * n], w = wbuf[cur * n];
-                a = csub(a, cadd(cmul(v, cconj(w)), cmul(w, cconj(v))));
+                const C v = vbuf[cur * n], w = wbuf[cur * n];
+                a = a - (v * conjx(w) + w * conjx(v));
             }
-            Dv[n - 1] = a.x;
+            Dv[n - 1] = (double)a.x;
         }
         __syncthreads();
     }
@@ -484,13 +486,14 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
     return Qa;
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void tail_kernel(const double *__restrict__ Abase, long strideA, int lda, int n,
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, long strideA, int lda, int n,
                                                   const double *__restrict__ Din, const double *__restrict__ Ein,
-                                                  const double *__restrict__ TAUin, long strideV,
+                                                  const T *__restrict__ TAUin, long strideV,
                                                   double *__restrict__ Wout, long strideW,
-                                                  double *__restrict__ Uout, long strideU,
+                                                  T *__restrict__ Uout, long strideU,
                                                   double *__restrict__ work, int nb_wy, int batch) {
+    using C = cx<T>;
     extern __shared__ double smem[];
     const int t = threadIdx.x;
     // per-workgroup global workspace: Qa, Qb, S (n*n doubles each)
@@ -508,23 +511,23 @@ __global__ __launch_bounds__(NT) void tail_kernel(const double *__restrict__ Aba
         sh.rnj = ip; ip += n; sh.ndx = ip; ip += n; sh.dfx = ip; ip += n; sh.cnt = ip; ip += 3 * n;
     }
     for (long b = blockIdx.x; b < batch; b += gridDim.x) {
-        const double *A = Abase + 2 * b * strideA;
+        const T *A = Abase + 2 * b * strideA;
         const double *dIn = Din + b * strideV;
         const double *eIn = Ein + b * strideV;
-        const double *tauv = TAUin + b * strideV;
+        const T *tauv = TAUin + b * strideV;
         double *W = Wout + b * strideW;
-        double *U = Uout + 2 * b * strideU;
+        T *U = Uout + 2 * b * strideU;
         double *dqf;
         double *Z = stedc_block<NT>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf);
         for (int k = t; k < n; k += NT) W[k] = dqf[k];
         // U = Z (complex, row-major)
-        for (long idx = t; idx < (long)n * n; idx += NT) st(U, idx, cd{Z[idx], 0.0});
+        for (long idx = t; idx < (long)n * n; idx += NT) stx(U, idx, C{(T)Z[idx], (T)0});
         __syncthreads();
         // back-transform with blocks of reflectors, last block first
-        cd *Vl = reinterpret_cast<cd *>(smem);        // rows x nb (reuses the D&C LDS)
+        C *Vl = reinterpret_cast<C *>(smem);          // rows x nb (reuses the D&C LDS)
         const int nref = n - 1;
         const int nblk = (nref + nb_wy - 1) / nb_wy;
-        cd *Tl = Vl + (long)n * nb_wy;                // nb x nb
+        C *Tl = Vl + (long)n * nb_wy;                 // nb x nb
         for (int bk = nblk - 1; bk >= 0; bk--) {
             const int j0 = bk * nb_wy;
             const int nbk = min(nb_wy, nref - j0);
@@ -534,10 +537,10 @@ __global__ __launch_bounds__(NT) void tail_kernel(const double *__restrict__ Aba
             for (int idx = t; idx < nrow * nb_wy; idx += NT) {
                 const int rr = idx / nb_wy, k = idx % nb_wy;
                 const int r = r0 + rr, j = j0 + k;
-                cd v{0.0, 0.0};
+                C v{(T)0, (T)0};
                 if (k < nbk) {
-                    if (r == j + 1) v = cd{1.0, 0.0};
-                    else if (r > j + 1) v = ld(A, (long)j * lda + r);
+                    if (r == j + 1) v = C{(T)1, (T)0};
+                    else if (r > j + 1) v = ldx(A, (long)j * lda + r);
                 }
                 Vl[idx] = v;
             }
@@ -545,66 +548,56 @@ __global__ __launch_bounds__(NT) void tail_kernel(const double *__restrict__ Aba
             // T (upper triangular): T[i][k] stored Tl[i*nb+k]; first Gram products V_i^H V_k
             for (int idx = t; idx < nb_wy * nb_wy; idx += NT) {
                 const int i = idx / nb_wy, k = idx % nb_wy;
-                cd g{0.0, 0.0};
+                C g{(T)0, (T)0};
                 if (i < k && k < nbk) {
-                    for (int rr = 0; rr < nrow; rr++) cfmac(g, Vl[rr * nb_wy + i], Vl[rr * nb_wy + k]);
+                    for (int rr = 0; rr < nrow; rr++) fmacx(g, Vl[rr * nb_wy + i], Vl[rr * nb_wy + k]);
                 }
                 Tl[idx] = g;
             }
             __syncthreads();
             if (t == 0) {
-                // T[0:k,k] = -tau_k * T[0:k,0:k] * G[0:k,k]; T[k][k] = tau_k
-                cd *tmp = Tl + nb_wy * nb_wy;   // nb scratch entries in LDS
+                // T[0:k,k] = -tau_k * T[0:k,0:k] * G[0:k,k]; T[k][k] = tau_k (zlarft, forward, columnwise)
+                C *tmp = Tl + nb_wy * nb_wy;   // nb scratch entries in LDS
                 for (int k = 0; k < nbk; k++) {
-                    const cd tk{tauv[2 * (j0 + k)], tauv[2 * (j0 + k) + 1]};
+                    const C tk{tauv[2 * (j0 + k)], tauv[2 * (j0 + k) + 1]};
                     for (int i = 0; i < k; i++) {
-                        cd s{0.0, 0.0};
-                        for (int l = i; l < k; l++) {
-                            const cd Til = (l == i) ? Tl[i * nb_wy + i] : Tl[i * nb_wy + l];
-                            cfma(s, Til, Tl[l * nb_wy + k]);
-                        }
+                        C s{(T)0, (T)0};
+                        for (int l = i; l < k; l++) fmax_(s, Tl[i * nb_wy + l], Tl[l * nb_wy + k]);
                         tmp[i] = s;
                     }
-                    for (int i = 0; i < k; i++) {
-                        cd v = cmul(tk, tmp[i]);
-                        Tl[i * nb_wy + k] = cd{-v.x, -v.y};
-                    }
+                    for (int i = 0; i < k; i++) Tl[i * nb_wy + k] = (tk * tmp[i]) * (T)(-1);
                     Tl[k * nb_wy + k] = tk;
                 }
                 for (int k = nbk; k < nb_wy; k++)
-                    for (int i = 0; i < nb_wy; i++) Tl[i * nb_wy + k] = cd{0.0, 0.0};
+                    for (int i = 0; i < nb_wy; i++) Tl[i * nb_wy + k] = C{(T)0, (T)0};
             }
             __syncthreads();
             // apply to every column: w = V^H u ; w = T w ; u -= V w
             for (int c = t; c < n; c += NT) {
-                cd w[16];
+                C w[16];
 #pragma unroll
-                for (int k = 0; k < 16; k++) w[k] = cd{0.0, 0.0};
+                for (int k = 0; k < 16; k++) w[k] = C{(T)0, (T)0};
                 for (int rr = 0; rr < nrow; rr++) {
-                    const cd u = ld(U, (long)(r0 + rr) * n + c);
+                    const C u = ldx(U, (long)(r0 + rr) * n + c);
 #pragma unroll
                     for (int k = 0; k < 16; k++)
-                        if (k < nb_wy) cfmac(w[k], Vl[rr * nb_wy + k], u);
+                        if (k < nb_wy) fmacx(w[k], Vl[rr * nb_wy + k], u);
                 }
-                cd w2[16];
+                C w2[16];
 #pragma unroll
                 for (int i = 0; i < 16; i++) {
-                    cd s{0.0, 0.0};
+                    C s{(T)0, (T)0};
 #pragma unroll
                     for (int k = 0; k < 16; k++)
-                        if (i < nb_wy && k >= i && k < nb_wy) cfma(s, Tl[i * nb_wy + k], w[k]);
-                    w2[i] = s;
+                        if (i < nb_wy && k >= i && k < nb_wy) fmax_(s, Tl[i * nb_wy + k], w[k]);
+                    w2[i] = s * (T)(-1);
                 }
                 for (int rr = 0; rr < nrow; rr++) {
-                    cd u = ld(U, (long)(r0 + rr) * n + c);
+                    C u = ldx(U, (long)(r0 + rr) * n + c);
 #pragma unroll
                     for (int k = 0; k < 16; k++)
-                        if (k < nb_wy) {
-                            const cd vv = Vl[rr * nb_wy + k];
-                            u.x -= vv.x * w2[k].x - vv.y * w2[k].y;
-                            u.y -= vv.x * w2[k].y + vv.y * w2[k].x;
-                        }
-                    st(U, (long)(r0 + rr) * n + c, u);
+                        if (k < nb_wy) fmax_(u, Vl[rr * nb_wy + k], w2[k]);
+                    stx(U, (long)(r0 + rr) * n + c, u);
                 }
             }
             __syncthreads();
@@ -628,35 +621,59 @@ int heevd_dc_workspace_doubles(int n, int grid_tail, long *per_matrix, long *per
     return 0;
 }
 
-int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,
-                  int grid, hipStream_t stream) {
+template <typename T>
+int launch_zhetrd_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
+                    int grid, hipStream_t stream) {
     if (n < 1 || n > 512) return -22;
     constexpr int NT = 256;
-    size_t lds = (size_t)(4 * n + (NT / kWave) * n) * 16 + (2 * (NT / kWave) + 8) * sizeof(double);
+    const size_t cs = 2 * sizeof(T);
+    size_t lds = (size_t)(4 * n + (NT / kWave) * n) * cs + (2 * (NT / kWave) + 8) * sizeof(T);
     if (grid > batch) grid = batch;
-    if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)zhetrd_kernel<NT, 2>,
+    if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)zhetrd_kernel<T, NT, 2>,
                                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((zhetrd_kernel<NT, 2>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU,
+    hipLaunchKernelGGL((zhetrd_kernel<T, NT, 2>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU,
                        strideV, batch);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_tail(const double *A, long strideA, int lda, int n, const double *D, const double *E, const double *TAU,
-                long strideV, double *W, long strideW, double *U, long strideU, double *work, int batch, int grid,
-                hipStream_t stream) {
+template <typename T>
+int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, const double *E, const T *TAU,
+                  long strideV, double *W, long strideW, T *U, long strideU, double *work, int batch, int grid,
+                  hipStream_t stream) {
     if (n < 1 || n > 512) return -22;
     constexpr int NT = 256;
     const int nb_wy = 16;
     size_t lds = heevd_dc_lds_bytes(n, nb_wy);
     if (lds > 160 * 1024) return -22;
     if (grid > batch) grid = batch;
-    if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)tail_kernel<NT>,
+    if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)tail_kernel<T, NT>,
                                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((tail_kernel<NT>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU, strideV,
-                       W, strideW, U, strideU, work, nb_wy, batch);
+    hipLaunchKernelGGL((tail_kernel<T, NT>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU,
+                       strideV, W, strideW, U, strideU, work, nb_wy, batch);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
+}
+
+int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,
+                  int grid, hipStream_t stream) {
+    return launch_zhetrd_t<double>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+}
+int launch_chetrd(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV, int batch,
+                  int grid, hipStream_t stream) {
+    return launch_zhetrd_t<float>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+}
+int launch_tail(const double *A, long strideA, int lda, int n, const double *D, const double *E, const double *TAU,
+                long strideV, double *W, long strideW, double *U, long strideU, double *work, int batch, int grid,
+                hipStream_t stream) {
+    return launch_tail_t<double>(A, strideA, lda, n, D, E, TAU, strideV, W, strideW, U, strideU, work, batch, grid,
+                                 stream);
+}
+int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *D, const double *E, const float *TAU,
+                    long strideV, double *W, long strideW, float *U, long strideU, double *work, int batch, int grid,
+                    hipStream_t stream) {
+    return launch_tail_t<float>(A, strideA, lda, n, D, E, TAU, strideV, W, strideW, U, strideU, work, batch, grid,
+                                stream);
 }
 
 }  // namespace rlm

@@ -19,6 +19,14 @@ int launch_tail(const double *A, long strideA, int lda, int n, const double *D, 
                 long strideV, double *W, long strideW, double *U, long strideU, double *work, int batch, int grid,
                 hipStream_t stream);
 
+// FP32 variants (mixed precision): A, TAU, U single-precision complex; D, E, W and the
+// divide & conquer stay FP64.
+int launch_chetrd(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV, int batch,
+                  int grid, hipStream_t stream);
+int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *D, const double *E, const float *TAU,
+                    long strideV, double *W, long strideW, float *U, long strideU, double *work, int batch, int grid,
+                    hipStream_t stream);
+
 size_t heevd_dc_lds_bytes(int n, int nb_wy);
 
 }  // namespace rlm

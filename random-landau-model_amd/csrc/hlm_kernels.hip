@@ -1,0 +1,218 @@
+// hlm (Hofstadter lattice model) kernels, FP64.
+//
+// Reference: hlm/hamiltonian_hlm_cpu.cpp:2-56 (get_wfs_ktheta: per twist, the
+// lx/Q x ly Bloch QxQ problems, band `band` spread over the lattice with plane-wave
+// phases, normalised), :149-184 (peer_solve_projected: W = diag(V) Phi,
+// Htr = Phi^H W, heevd(Htr), wfs = Phi U) and hlm/main_hlm_cpu.cpp:117-118,149-150
+// (theta_2 = g copies theta_2 = 0).
+//
+// MI355X design: the Bloch basis Phi(theta) is disorder independent, so it is
+// built once per run for every twist; link overlaps of the projected states are
+// <Phi u | Phi' u'> = u^H (Phi^H Phi') u', so the link matrices M = Phi^H Phi'
+// are also built once, and the n_sites-long real-space vectors Phi U of the
+// reference are never formed.
+#include "common.hpp"
+#include "hlm_kernels.hpp"
+
+namespace rlm {
+
+// Bloch QxQ matrices, column-major as hlm/hamiltonian_hlm_cpu.cpp:26-32 writes
+// them. Matrix index b = tw*n_phi + kx*ly + ky with tw = theta_1*g + theta_2.
+__global__ void bloch_build_kernel(int lx, int ly, int Q, int n_mesh, long nmat, double *__restrict__ H) {
+    const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nmat) return;
+    const int n_phi = (lx / Q) * ly;
+    const long tw = b / n_phi;
+    const int col = (int)(b % n_phi);
+    const int kx = col / ly, ky = col % ly;
+    const int t1 = (int)(tw / n_mesh), t2 = (int)(tw % n_mesh);
+    const double PI2 = 6.283185307179586232;
+    const double inv_n_mesh = 1.0 / n_mesh, inv_Q = 1.0 / Q;
+    const double pi2_inv_lx = PI2 / lx, pi2_inv_ly = PI2 / ly;
+    const double kxs = kx + t1 * inv_n_mesh, kys = ky + t2 * inv_n_mesh;
+    double *M = H + 2 * b * Q * Q;
+    for (int i = 0; i < Q * Q; i++) st(M, i, cd{0.0, 0.0});
+    for (int p = 0; p < Q; p++) {
+        st(M, p * Q + p, cd{-2.0 * cos(kys * pi2_inv_ly + PI2 * p * inv_Q), 0.0});
+        if (p + 1 < Q) {
+            st(M, p * Q + p + 1, cd{-1.0, 0.0});
+            st(M, (p + 1) * Q + p, cd{-1.0, 0.0});
+        }
+    }
+    double sn, cs;
+    sincos(kxs * Q * pi2_inv_lx, &sn, &cs);
+    st(M, Q * (Q - 1), cd{-cs, -sn});
+    st(M, Q - 1, cd{-cs, sn});
+}
+
+// Phi[tw][col*n_sites + r], r = j*lx + m + q (m = 0, Q, 2Q, ...): e^{-i phase} * u_band[q],
+// normalised. One workgroup per (tw, col). Ub: component-major Bloch eigenvectors
+// (Ub[b][q*Q + state]).
+__global__ __launch_bounds__(256) void phi_build_kernel(int lx, int ly, int Q, int band, int n_mesh,
+                                                        const double *__restrict__ Ub, double *__restrict__ Phi) {
+    __shared__ double red[8];
+    const long b = blockIdx.x;                 // tw*n_phi + col
+    const int n_phi = (lx / Q) * ly;
+    const int n_sites = lx * ly;
+    const long tw = b / n_phi;
+    const int col = (int)(b % n_phi);
+    const int kx = col / ly, ky = col % ly;
+    const int t1 = (int)(tw / n_mesh), t2 = (int)(tw % n_mesh);
+    const double PI2 = 6.283185307179586232;
+    const double inv_n_mesh = 1.0 / n_mesh;
+    const double pi2_inv_lx = PI2 / lx, pi2_inv_ly = PI2 / ly;
+    const double kxs = kx + t1 * inv_n_mesh, kys = ky + t2 * inv_n_mesh;
+    const double *u = Ub + 2 * b * Q * Q;
+    double *out = Phi + 2 * (tw * (long)n_phi + col) * n_sites;
+    double nrm = 0.0;
+    for (int r = threadIdx.x; r < n_sites; r += blockDim.x) {
+        const int j = r / lx, x = r % lx;
+        const int m = x - x % Q, q = x % Q;
+        const double phase = kxs * pi2_inv_lx * m + kys * pi2_inv_ly * j;
+        double sn, cs;
+        sincos(phase, &sn, &cs);
+        const cd w = cmul(cd{cs, -sn}, ld(u, q * Q + band));
+        nrm += cabs2(w);
+    }
+    nrm = block_sum(nrm, red);
+    const double inv = 1.0 / sqrt(nrm);
+    for (int r = threadIdx.x; r < n_sites; r += blockDim.x) {
+        const int j = r / lx, x = r % lx;
+        const int m = x - x % Q, q = x % Q;
+        const double phase = kxs * pi2_inv_lx * m + kys * pi2_inv_ly * j;
+        double sn, cs;
+        sincos(phase, &sn, &cs);
+        const cd w = cmul(cd{cs, -sn}, ld(u, q * Q + band));
+        st(out, r, cscale(w, inv));
+    }
+}
+
+// Batched complex GEMM with general strides and an optional real weight on k:
+//   C(i,j) = sum_k opA(A)(i,k) * w(k) * B(k,j)
+// opA: conj-transpose when conjA (A(i,k) read at k*rsA + i*csA and conjugated).
+// Batch b = b1*nb2 + b2; offsets = b1*s1 + b2*s2 (complex elements for A, B, C;
+// doubles for w). 32x32 output tiles, 256 threads (2x2 per thread), K chunks of 32.
+__global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, const double *__restrict__ A, long rsA,
+                                                    long csA, long sA1, long sA2, int conjA,
+                                                    const double *__restrict__ B, long rsB, long csB, long sB1,
+                                                    long sB2, const double *__restrict__ w, long sw1, long sw2,
+                                                    double *__restrict__ C, long rsC, long csC, long sC1, long sC2,
+                                                    int nb2) {
+    __shared__ cd As[32][33];
+    __shared__ cd Bs[32][33];
+    const long b = blockIdx.x;                 // batch in x: y/z grid dims are limited to 65535
+    const long b1 = b / nb2, b2 = b % nb2;
+    const double *Ab = A + 2 * (b1 * sA1 + b2 * sA2);
+    const double *Bb = B + 2 * (b1 * sB1 + b2 * sB2);
+    const double *wb = w ? w + b1 * sw1 + b2 * sw2 : nullptr;
+    double *Cb = C + 2 * (b1 * sC1 + b2 * sC2);
+    const int i0 = blockIdx.y * 32, j0 = blockIdx.z * 32;
+    const int t = threadIdx.x;
+    const int ti = (t % 16) * 2, tj = (t / 16) * 2;
+    cd acc[2][2] = {{{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        for (int e = t; e < 32 * 32; e += 256) {
+            const int ii = e % 32, kk = e / 32;      // A tile: As[ii][kk]
+            const int gi = i0 + ii, gk = k0 + kk;
+            cd a{0.0, 0.0};
+            if (gi < M && gk < K) {
+                if (conjA) a = cconj(ld(Ab, gk * rsA + gi * csA));
+                else a = ld(Ab, gi * rsA + gk * csA);
+                if (wb) a = cscale(a, wb[gk]);
+            }
+            As[ii][kk] = a;
+            const int kb = e % 32, jj = e / 32;      // B tile: Bs[kb][jj]
+            const int gk2 = k0 + kb, gj = j0 + jj;
+            cd bv{0.0, 0.0};
+            if (gk2 < K && gj < N) bv = ld(Bb, gk2 * rsB + gj * csB);
+            Bs[kb][jj] = bv;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < 32; kk++) {
+            const cd a0 = As[ti][kk], a1 = As[ti + 1][kk];
+            const cd bb0 = Bs[kk][tj], bb1 = Bs[kk][tj + 1];
+            cfma(acc[0][0], a0, bb0);
+            cfma(acc[0][1], a0, bb1);
+            cfma(acc[1][0], a1, bb0);
+            cfma(acc[1][1], a1, bb1);
+        }
+        __syncthreads();
+    }
+    for (int di = 0; di < 2; di++)
+        for (int dj = 0; dj < 2; dj++) {
+            const int gi = i0 + ti + di, gj = j0 + tj + dj;
+            if (gi < M && gj < N) st(Cb, gi * rsC + gj * csC, acc[di][dj]);
+        }
+}
+
+// hlm links: L[kp][j] = u_j(k)^H M(k,k') u_j(k'). Component-major eigenvectors
+// U[kp][i*ns + j]. M column-major N x N per link. One workgroup per kp (threads = states).
+// Twist points kp = (s*(g+1) + t1)*g + t2 (theta_2 = g is theta_2 = 0).
+// Link matrices: M1[t1*g + t2] (t1 < g), M2[t1*g + t2].
+__global__ __launch_bounds__(128) void hlm_links_kernel(const double *__restrict__ U, int ns, int n_mesh,
+                                                        const double *__restrict__ M1, const double *__restrict__ M2,
+                                                        double *__restrict__ Lk1, double *__restrict__ Lk2) {
+    const long kp = blockIdx.x;
+    const int g = n_mesh;
+    const int t2 = (int)(kp % g);
+    const int t1 = (int)((kp / g) % (g + 1));
+    const long base_line = kp - t2;
+    const long nn = (long)ns * ns;
+    const double *u0 = U + 2 * kp * nn;
+    const bool has1 = t1 < g;
+    const double *u1 = U + 2 * (kp + g) * nn;
+    const double *u2 = U + 2 * (base_line + (t2 + 1) % g) * nn;
+    const double *m1 = M1 + 2 * ((long)t1 * g + t2) * nn;
+    const double *m2 = M2 + 2 * ((long)t1 * g + t2) * nn;
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+        cd a1{0.0, 0.0}, a2{0.0, 0.0};
+        for (int i = 0; i < ns; i++) {
+            cd y1{0.0, 0.0}, y2{0.0, 0.0};
+            for (int l = 0; l < ns; l++) {
+                if (has1) cfma(y1, ld(m1, (long)l * ns + i), ld(u1, (long)l * ns + j));
+                cfma(y2, ld(m2, (long)l * ns + i), ld(u2, (long)l * ns + j));
+            }
+            const cd x = ld(u0, (long)i * ns + j);
+            if (has1) cfmac(a1, x, y1);
+            cfmac(a2, x, y2);
+        }
+        st(Lk1, kp * ns + j, a1);
+        st(Lk2, kp * ns + j, a2);
+    }
+}
+
+int launch_bloch_build(int lx, int ly, int Q, int n_mesh, long nmat, double *H, hipStream_t s) {
+    hipLaunchKernelGGL(bloch_build_kernel, dim3((unsigned)((nmat + 127) / 128)), dim3(128), 0, s, lx, ly, Q, n_mesh,
+                       nmat, H);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_phi_build(int lx, int ly, int Q, int band, int n_mesh, long nmat, const double *Ub, double *Phi,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(phi_build_kernel, dim3((unsigned)nmat), dim3(256), 0, s, lx, ly, Q, band, n_mesh, Ub, Phi);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_zgemm(int M, int N, int K, const double *A, long rsA, long csA, long sA1, long sA2, int conjA,
+                 const double *B, long rsB, long csB, long sB1, long sB2, const double *w, long sw1, long sw2,
+                 double *C, long rsC, long csC, long sC1, long sC2, int nb1, int nb2, hipStream_t s) {
+    if (nb1 * (long)nb2 <= 0) return 0;
+    dim3 grid((unsigned)(nb1 * (long)nb2), (M + 31) / 32, (N + 31) / 32);
+    hipLaunchKernelGGL(zgemm_kernel, grid, dim3(256), 0, s, M, N, K, A, rsA, csA, sA1, sA2, conjA, B, rsB, csB, sB1,
+                       sB2, w, sw1, sw2, C, rsC, csC, sC1, sC2, nb2);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_hlm_links(const double *U, int ns, int n_mesh, long nkp, const double *M1, const double *M2, double *Lk1,
+                     double *Lk2, hipStream_t s) {
+    if (nkp <= 0) return 0;
+    hipLaunchKernelGGL(hlm_links_kernel, dim3((unsigned)nkp), dim3(128), 0, s, U, ns, n_mesh, M1, M2, Lk1, Lk2);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace rlm

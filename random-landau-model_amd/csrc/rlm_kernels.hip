@@ -90,7 +90,8 @@ __global__ void wtable_kernel(const double *__restrict__ V, const double *__rest
 // zero outside the band D <= o || D >= N-o. Column-major, lda = N.
 __global__ __launch_bounds__(256) void assemble_kernel(const double *__restrict__ E, const double *__restrict__ Wt,
                                                        int n_phi, int d, int off, int n_mesh, long kp0,
-                                                       double *__restrict__ Aout, long strideA) {
+                                                       double *__restrict__ Aout, long strideA,
+                                                       float *__restrict__ A32) {
     const long kp = kp0 + blockIdx.x;
     const int g1 = n_mesh + 1;
     const long s = kp / ((long)g1 * g1);
@@ -99,6 +100,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double *__restrict_
     const double *Et = E + 2L * t2 * d * n_phi;
     const double *Wl = Wt + 2L * (s * g1 + t1) * d * n_phi;
     double *A = Aout + 2L * blockIdx.x * strideA;
+    float *Af = A32 ? A32 + 2L * blockIdx.x * strideA : nullptr;
     const int N = n_phi;
     for (int c = 0; c < N; c++) {
         for (int r = threadIdx.x; r < N; r += blockDim.x) {
@@ -116,6 +118,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double *__restrict_
             cd a = upper ? cconj(h) : h;
             if (r == c) a.y = 0.0;
             st(A, (long)c * N + r, a);
+            if (Af) stx(Af, (long)c * N + r, cx<float>{(float)a.x, (float)a.y});
         }
     }
 }
@@ -144,10 +147,65 @@ int launch_wtable(const double *V, const double *cmt, int n_phi, int d, int off,
 }
 
 int launch_assemble(const double *E, const double *Wt, int n_phi, int d, int off, int n_mesh, long kp0, long nkp,
-                    double *A, long strideA, hipStream_t stream) {
+                    double *A, long strideA, hipStream_t stream, float *A32) {
     if (nkp <= 0) return 0;
     hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, E, Wt, n_phi, d, off, n_mesh, kp0,
-                       A, strideA);
+                       A, strideA, A32);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Mixed-precision refinement: FP64 Rayleigh quotients of the FP32 eigenvectors
+// against the FP64 matrix, lambda_j = Re(u^H A u) / (u^H u), replacing the
+// eigenvalues of the FP32-reduced tridiagonal. One workgroup per matrix, one
+// thread per state; A(r,c) is zero outside the circular band |c-r| <= bw (bw >= n/2:
+// dense). resid[kp] = max_j ||A u - lambda u|| / ||u|| (diagnostic).
+__global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict__ A, const float *__restrict__ U,
+                                                       int n, int bw, double *__restrict__ W,
+                                                       double *__restrict__ resid) {
+    __shared__ double red[8];
+    const long kp = blockIdx.x;
+    const double *Ak = A + 2L * kp * n * n;
+    const float *Uk = U + 2L * kp * n * n;
+    const bool dense = 2 * bw + 1 >= n;
+    double worst = 0.0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        cd q{0.0, 0.0};
+        double uu = 0.0, yy = 0.0;
+        for (int r = 0; r < n; r++) {
+            cd y{0.0, 0.0};
+            const int c0 = dense ? 0 : r - bw, c1 = dense ? n - 1 : r + bw;
+            for (int cc = c0; cc <= c1; cc++) {
+                const int c = cc < 0 ? cc + n : (cc >= n ? cc - n : cc);
+                const cx<float> u = ldx(Uk, (long)c * n + j);
+                cfma(y, ld(Ak, (long)c * n + r), cd{(double)u.x, (double)u.y});
+            }
+            const cx<float> ur = ldx(Uk, (long)r * n + j);
+            const cd u{(double)ur.x, (double)ur.y};
+            cfmac(q, u, y);
+            uu += cabs2(u);
+            yy += cabs2(y);
+        }
+        const double lam = q.x / uu;
+        W[kp * n + j] = lam;
+        const double r2 = fmax(yy / uu - lam * lam, 0.0);
+        worst = fmax(worst, sqrt(r2));
+    }
+    // max over states
+    for (int off = kWave / 2; off > 0; off >>= 1) worst = fmax(worst, __shfl_xor(worst, off, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = worst;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); w++) m = fmax(m, red[w]);
+        if (resid) resid[kp] = m;
+    }
+}
+
+int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, double *W, double *resid,
+                    hipStream_t stream) {
+    if (nkp <= 0) return 0;
+    hipLaunchKernelGGL(rayleigh_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, A, U, n, bw, W, resid);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -9,5 +9,7 @@ int launch_etable(const double *cmt, const double *cjm, int n_phi, int d, int n_
 int launch_wtable(const double *V, const double *cmt, int n_phi, int d, int off, int n_mesh, int n_samples, double *Wt,
                   hipStream_t stream);
 int launch_assemble(const double *E, const double *Wt, int n_phi, int d, int off, int n_mesh, long kp0, long nkp,
-                    double *A, long strideA, hipStream_t stream);
+                    double *A, long strideA, hipStream_t stream, float *A32 = nullptr);
+int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, double *W, double *resid,
+                    hipStream_t stream);
 }  // namespace rlm

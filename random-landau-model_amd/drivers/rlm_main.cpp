@@ -1,0 +1,132 @@
+// rlm: drop-in for the reference's rlm_cpu_dp / rlm_gpu_* executables
+// (rlm/main_rlm_cpu.cpp, rlm/main_rlm_gpu.cu, rlm/init_rlm.cpp): same flags,
+// log_rlm.txt, energy_Chern_No.txt rows and stderr timing report, with every
+// sample's (g+1)^2 twist mesh assembled, diagonalised and Chern-reduced on the GPU
+// through librlm_hip.so.
+#include <unistd.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <vector>
+
+#include "../../include/hip_diag.h"
+
+using namespace std;
+
+static void usage(const char *target) {
+    cout << "Usage: " << target << " -[OPTIONS]" << endl;
+    cout << "Options:\n";
+    cout << "  -n  n_phi\n";
+    cout << "  -q  quanta concentration \n";
+    cout << "  -i  impurity concentration\n";
+    cout << "  -g  theta mesh griding size (n_mesh)\n";
+    cout << "  -d  disorder potential K-space cut (offhead)\n";
+    cout << "  -c  # of disorder configurations(n_sample)\n";
+    cout << "  -s  seed to generate seeds\n";
+    cout << "  -t  # of threads\n";
+}
+
+int main(int argc, char *argv[]) {
+    // defaults of rlm/init_rlm.cpp:20-28
+    long n_phi = 32, n_mesh = 32, n_sample = 1, num_threads = 1, off_override = -1;
+    double q = 1, imp = 16;
+    unsigned long seed = std::chrono::high_resolution_clock::now().time_since_epoch().count();
+    int ch, err_flag = 0;
+    // the reference's optstring is "n:q:i:g:c:s:h:t:"; -d (documented there) is accepted here
+    while ((ch = getopt(argc, argv, "n:q:i:g:d:c:s:h:t:")) != -1) {
+        switch (ch) {
+            case 'n': n_phi = atoi(optarg); break;
+            case 'q': q = atof(optarg); break;
+            case 'i': imp = atof(optarg); break;
+            case 'g': n_mesh = atoi(optarg); break;
+            case 'd': off_override = atoi(optarg); break;
+            case 'c': n_sample = atoi(optarg); break;
+            case 's': seed = atoi(optarg); break;
+            case 't': num_threads = atoi(optarg); break;
+            default: err_flag++; break;
+        }
+    }
+    if (err_flag) {
+        usage(argv[0]);
+        exit(2);
+    }
+    (void)num_threads;  // the GPU path has no host worker threads
+    {
+        ofstream flog("log_rlm.txt");
+        flog << "########################################################" << endl;
+        flog << setw(25) << "seed: " << seed << endl;
+        flog << setw(25) << "n_phi: " << n_phi << endl;
+        flog << setw(25) << "n_mesh: " << n_mesh << endl;
+        flog << setw(25) << "n_sample: " << n_sample << endl;
+        flog << setw(25) << "quanta_concentration: " << q << endl;
+        flog << setw(25) << "impurity_concentration: " << imp << endl;
+        flog << "########################################################" << endl;
+    }
+    // batch as many samples as fit a 32 GiB budget (eigenvectors + matrices dominate)
+    const double per_sample = 32.0 * (n_mesh + 1) * (n_mesh + 1) * (double)n_phi * n_phi;
+    long spb = (long)(34359738368.0 / per_sample);
+    if (spb < 1) spb = 1;
+    if (spb > n_sample) spb = n_sample;
+
+    auto t1 = chrono::high_resolution_clock::now();
+    rlm_plan_t plan = nullptr;
+    int rc = rlm_plan_create(&plan, n_phi, q, imp, n_mesh, off_override, seed, (int)spb, HIP_DIAG_FP64);
+    if (rc) {
+        cerr << "rlm_plan_create failed with status " << rc << endl;
+        return 1;
+    }
+    auto t2 = chrono::high_resolution_clock::now();
+    const double coeff_time = chrono::duration_cast<chrono::microseconds>(t2 - t1).count() / 1.0E6;
+    cerr << std::left << setw(40) << "coefficients initialization time: " << coeff_time << " s\n";
+
+    ofstream fchern("energy_Chern_No.txt");
+    vector<double> E(spb * n_phi), C(spb * n_phi);
+    const double nkp = (double)(n_mesh + 1) * (n_mesh + 1);
+    for (long s0 = 0; s0 < n_sample; s0 += spb) {
+        const int cnt = (int)min(spb, n_sample - s0);
+        t1 = chrono::high_resolution_clock::now();
+        if ((rc = rlm_plan_load(plan, s0, cnt))) break;
+        t2 = chrono::high_resolution_clock::now();
+        const double load_time = chrono::duration_cast<chrono::microseconds>(t2 - t1).count() / 1.0E6;
+        if ((rc = rlm_plan_execute(plan))) break;
+        if ((rc = rlm_plan_fetch(plan, E.data(), C.data(), nullptr))) break;
+        if (s0 == 0) {
+            double ms[6];
+            rlm_plan_stage_times(plan, ms);
+            const double pot = (load_time + ms[0] / 1e3) / cnt;
+            const double diag = (ms[1] + ms[2] + ms[3]) / 1e3 / (cnt * nkp);
+            const double chern = (ms[4] + ms[5]) / 1e3 / (cnt * nkp);
+            const double total = coeff_time + (pot + chern * n_mesh * (n_mesh + 1) + diag * nkp) * n_sample;
+            cerr << setw(40) << "potential initialization time: " << pot << " s\n";
+            cerr << setw(40) << "diagonalization time/k-point: " << diag << " s\n";
+            cerr << setw(40) << "Chern No. calculation time/k-point: " << chern << " s\n";
+            cerr << setw(40) << "estimated total time:" << std::right << setw(3) << setfill('0')
+                 << (int(total)) / 3600 << setw(1) << ":";
+            cerr << setw(2) << setfill('0') << ((int(total)) % 3600) / 60 << setw(1) << ":";
+            cerr << setw(2) << setfill('0') << (int(total)) % 60 << endl << setfill(' ');
+            cerr << std::left << setw(40) << "estimated # of samples/hour:" << n_sample / (total / 3600.0) << endl;
+            cerr << setw(40) << "# module" << "%(time)" << endl;
+            cerr << setw(40) << "coefficients" << coeff_time / total * 100.0 << "\n";
+            cerr << setw(40) << "potential" << pot * n_sample / total * 100.0 << "\n";
+            cerr << setw(40) << "diagonalization" << diag * nkp * n_sample / total * 100.0 << endl;
+            cerr << setw(40) << "Chern No. calculation" << chern * n_mesh * (n_mesh + 1) * n_sample / total * 100.0
+                 << endl;
+        }
+        // rlm/main_rlm_cpu.cpp:241-242
+        for (int s = 0; s < cnt; s++)
+            for (long i = 0; i < n_phi; i++)
+                fchern << std::left << setw(8) << s0 + s << " " << setw(15) << setprecision(8)
+                       << E[s * n_phi + i] << " " << C[s * n_phi + i] << endl;
+    }
+    rlm_plan_destroy(plan);
+    if (rc) {
+        cerr << "rlm pipeline failed with status " << rc << endl;
+        return 1;
+    }
+    return 0;
+}

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(PKG_DIR, "librlm_hip.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "hip_diag.h")
 
 HIP_DIAG_FP64 = 0
+HIP_DIAG_MIXED = 1
 
 _lib = None
 
@@ -60,8 +61,18 @@ def lib() -> ctypes.CDLL:
         "rlm_plan_stage_times": ([vp, dp], i),
         "rlm_plan_info": ([vp, ctypes.POINTER(ctypes.c_long)], i),
         "rlm_plan_eigenvalues": ([vp], vp),
+        "rlm_plan_max_residual": ([vp, dp], i),
         "rlm_plan_eigenvectors": ([vp], vp),
         "rlm_plan_destroy": ([vp], i),
+        "hlm_plan_create": ([ctypes.POINTER(vp), i, i, i, i, i, ctypes.c_ulong, d, i, i], i),
+        "hlm_plan_load": ([vp, l, i], i),
+        "hlm_plan_execute": ([vp], i),
+        "hlm_plan_fetch": ([vp, dp, dp, dp], i),
+        "hlm_plan_run": ([vp, l, l, dp, dp, dp], i),
+        "hlm_plan_stage_times": ([vp, dp], i),
+        "hlm_plan_stream": ([vp], vp),
+        "hlm_plan_destroy": ([vp], i),
+        "hlm_generate_disorder": ([dp, l, l, ctypes.c_ulong, l, d], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -181,6 +192,11 @@ class RlmPlan:
     def stream(self) -> int:
         return lib().rlm_plan_stream(self._h) or 0
 
+    def max_residual(self) -> float:
+        out = np.zeros(1)
+        _check(lib().rlm_plan_max_residual(self._h, _ptr(out)), "rlm_plan_max_residual")
+        return float(out[0])
+
     def eigenvalues_ptr(self) -> int:
         return lib().rlm_plan_eigenvalues(self._h) or 0
 
@@ -190,6 +206,64 @@ class RlmPlan:
     def close(self):
         if self._h:
             lib().rlm_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hlm_disorder(n_sites: int, n_samples: int, seed: int, first_sample: int = 0, W: float = 2.0):
+    V = np.zeros(n_sites * n_samples)
+    _check(lib().hlm_generate_disorder(_ptr(V), n_sites, n_samples, seed, first_sample, W), "hlm_generate_disorder")
+    return V.reshape(n_samples, n_sites)
+
+
+class HlmPlan:
+    """hlm_plan_* (batched hlm pipeline)."""
+
+    STAGES = ("projection", "tridiagonalisation", "dc+backtransform", "links", "plaquettes", "unused")
+
+    def __init__(self, lx, ly, Q, band=0, n_mesh=32, seed=1, W=2.0, samples_per_batch=1, precision=HIP_DIAG_FP64):
+        self._h = ctypes.c_void_p()
+        _check(lib().hlm_plan_create(ctypes.byref(self._h), lx, ly, Q, band, n_mesh, seed, W, samples_per_batch,
+                                     precision), "hlm_plan_create")
+        self.n_phi = lx * ly // Q
+        self.loaded = 0
+
+    def load(self, first_sample: int, count: int):
+        _check(lib().hlm_plan_load(self._h, first_sample, count), "hlm_plan_load")
+        self.loaded = count
+
+    def execute(self):
+        _check(lib().hlm_plan_execute(self._h), "hlm_plan_execute")
+
+    def fetch(self):
+        n = self.loaded * self.n_phi
+        e, c, m = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().hlm_plan_fetch(self._h, _ptr(e), _ptr(c), _ptr(m)), "hlm_plan_fetch")
+        shp = (self.loaded, self.n_phi)
+        return e.reshape(shp), c.reshape(shp), m.reshape(shp)
+
+    def run(self, first_sample: int, n_samples: int):
+        n = n_samples * self.n_phi
+        e, c, m = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().hlm_plan_run(self._h, first_sample, n_samples, _ptr(e), _ptr(c), _ptr(m)), "hlm_plan_run")
+        shp = (n_samples, self.n_phi)
+        return e.reshape(shp), c.reshape(shp), m.reshape(shp)
+
+    def stage_times(self) -> dict:
+        ms = np.zeros(6)
+        rc = lib().hlm_plan_stage_times(self._h, _ptr(ms))
+        if rc != 6:
+            raise HipDiagError(f"hlm_plan_stage_times failed with status {rc}")
+        return dict(zip(self.STAGES, ms.tolist()))
+
+    def close(self):
+        if self._h:
+            lib().hlm_plan_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

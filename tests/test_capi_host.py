@@ -33,3 +33,12 @@ def test_disorder_matches_oracle(oracle, n_phi, seed, sample):
     b = generate_disorder(sz.n_imp, sz.L1, sz.L2, seed, sample)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_hlm_disorder_matches_oracle(oracle):
+    """hlm/main_hlm_cpu.cpp:66-67,79 single stream; W=1 is the W=2 stream halved exactly."""
+    from rlm_amd import hlm_disorder
+    ref = oracle.hlm_disorder(1, 576, 5)
+    assert np.array_equal(hlm_disorder(576, 5, 1, 0, 2.0), ref)
+    assert np.array_equal(hlm_disorder(576, 2, 1, 3, 2.0), ref[3:5])
+    assert np.array_equal(hlm_disorder(576, 5, 1, 0, 1.0), 0.5 * ref)

@@ -90,3 +90,24 @@ def test_full_size_properties():
     assert np.all(np.rint(c).sum(axis=1) == 1)
     assert np.all(m > 0)
     plan.close()
+
+
+@pytest.mark.parametrize("name", ["rlm_n16_g64_s1", "rlm_n64_g16_s11", "rlm_n128_g16_s11", "rlm_n256_g32_s7"])
+def test_mixed_precision_fixtures(fixtures, name):
+    """Mixed path (FP32 tridiagonalisation/back-transform, FP64 D&C, FP64 Rayleigh
+    quotients, FP64-accumulated links): same bar as FP64 -- Chern integers identical,
+    energies within 1e-10 relative of the FP64 oracle."""
+    from rlm_amd import RlmPlan, HIP_DIAG_MIXED
+    fx = fixtures[name]
+    p = fx["params"]
+    samples = [r["sample"] for r in fx["samples"]]
+    print(f"[test_mixed_precision_fixtures] {name}", flush=True)
+    plan = RlmPlan(p["n_phi"], p["q"], p["imp"], p["n_mesh"], p["seed"], samples_per_batch=len(samples),
+                   precision=HIP_DIAG_MIXED)
+    e, c, m = plan.run(samples[0], len(samples))
+    res = plan.max_residual()
+    print(f"  max FP64 residual of FP32 eigenvectors: {res:.3e}; min plaquette margin {m.min():.3e}", flush=True)
+    for k, row in enumerate(fx["samples"]):
+        _cmp_sample(e[k], c[k], row["energy"], row["chern"], (name, "mixed", row["sample"]))
+    assert res < 1e-3
+    plan.close()
