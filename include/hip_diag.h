@@ -120,8 +120,9 @@ int rlm_plan_fetch(rlm_plan_t plan, double *energy, double *chern, double *margi
 int rlm_plan_run(rlm_plan_t plan, long first_sample, long n_samples, double *energy, double *chern, double *margin);
 /* GPU time of each stage of the last rlm_plan_execute, in ms, from hipEvents
  * recorded on the plan's stream: [0] V_mn + band tables, [1] assembly,
- * [2] tridiagonalisation, [3] divide & conquer + back-transform, [4] link
- * overlaps, [5] plaquettes + energies. Returns 6 (the number of stages). */
+ * [2] tridiagonalisation, [3] divide & conquer + back-transform, [4] FP64
+ * refinement (mixed plans; 0 otherwise), [5] link overlaps, [6] plaquettes +
+ * energies. Returns 7 (the number of stages). */
 int rlm_plan_stage_times(rlm_plan_t plan, double *ms);
 /* The plan's hipStream_t. */
 void *rlm_plan_stream(rlm_plan_t plan);

@@ -327,7 +327,7 @@ struct rlm_plan_s {
     EigWork ws;
     int loaded = 0;
     long first = 0;
-    static constexpr int kStages = 6;
+    static constexpr int kStages = 7;
     hipEvent_t ev[kStages + 1] = {};
 };
 
@@ -426,21 +426,23 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     if (!mixed) {
         if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[3]))) return rc;
         RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
+        RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
         if ((rc = launch_links(p->dU.d(), (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
     } else {
         float *U32 = static_cast<float *>(p->dU.p);
         if ((rc = eig_packed_f32(A32, N, nkp, p->dEig.d(), U32, p->ws, s, p->ev[3]))) return rc;
+        RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
         // FP64 Rayleigh quotients against the FP64 matrix (band |c-r| <= off_head)
         if ((rc = launch_rayleigh(p->dA.d(), U32, N, off, nkp, p->dEig.d(), p->dResid.d(), s))) return rc;
-        RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
+        RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
         if ((rc = launch_links_f32(U32, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
     }
-    RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
+    RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
     double *out = p->dOut.d();
     if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, (int)g1, 0, (double)N, S, out, out + (long)S * N,
                             out + 2L * S * N, s)))
         return rc;
-    RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
+    RLM_HIP_CHECK(hipEventRecord(p->ev[7], s));
     return 0;
 }
 

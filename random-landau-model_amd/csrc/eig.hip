@@ -19,6 +19,9 @@
 // Matrices are complex128, column-major with full Hermitian storage on input.
 // Eigenvectors are returned component-major ("row-major": U[r*n + j] is
 // component r of state j) so that later per-state kernels read coalesced.
+#include <cstdlib>
+#include <cstring>
+
 #include "common.hpp"
 #include "eig.hpp"
 
@@ -177,6 +180,203 @@ __global__ __launch_bounds__(NT) void zhetrd_kernel(T *__restrict__ Abase, long 
     }
 }
 
+// ----------------------------------------------------------------------------
+// Blocked variant (LAPACK zlatrd/zhetrd structure): panels of NB columns. Inside a
+// panel the trailing block is only read (Hermitian matrix-vector product against
+// the panel-start matrix, corrected with the panel's V and W); after the panel
+// one rank-2*NB update A22 -= V W^H + W V^H writes it once. Per column the
+// trailing block is read once instead of read and written.
+// V(r,l) lives in A (column k0+l, unit at row k0+l+1); W is kept in LDS.
+template <typename T, int NT, int NB, int MAXR>
+__global__ __launch_bounds__(NT) void zhetrd_blocked_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
+                                                            double *__restrict__ Dout, double *__restrict__ Eout,
+                                                            T *__restrict__ TAUout, long strideV, int batch) {
+    using C = cx<T>;
+    constexpr int NQ = 2 * NB;              // dot products per column (W^H v and V^H v)
+    constexpr int RG = NT / NQ;             // row groups for those dot products
+    extern __shared__ double smem_d[];
+    C *Wp = reinterpret_cast<C *>(smem_d);           // n x NB, row-major
+    C *vb = Wp + (long)n * NB;                       // n (v of the current column, from row i+1)
+    C *part = vb + n;                                // (NT/64) x n
+    C *dpart = part + (NT / kWave) * n;              // RG x NQ
+    C *dots = dpart + RG * NQ;                       // NQ
+    T *red = reinterpret_cast<T *>(dots + NQ);
+    T *sc = red + 2 * (NT / kWave);
+    const int t = threadIdx.x;
+
+    for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+        T *A = Abase + 2 * b * strideA;
+        double *Dv = Dout + b * strideV;
+        double *Ev = Eout + b * strideV;
+        T *Tv = TAUout + b * strideV;
+        auto Aidx = [&](int r, int c) -> long { return (long)c * lda + r; };
+        // V(r, l) of the current panel (reflector of column k0 + l)
+        auto Vget = [&](int k0, int r, int l) -> C {
+            const int j = k0 + l;
+            if (r < j + 1) return C{(T)0, (T)0};
+            if (r == j + 1) return C{(T)1, (T)0};
+            return ldx(A, Aidx(r, j));
+        };
+        for (int k0 = 0; k0 < n - 1; k0 += NB) {
+            const int nbk = min(NB, n - 1 - k0);
+            for (int j = 0; j < nbk; j++) {
+                const int i = k0 + j;
+                // (1) column i, rows >= i: apply the panel's earlier updates
+                T nrm2 = 0;
+                for (int r = i + t; r < n; r += NT) {
+                    C a = ldx(A, Aidx(r, i));
+                    if (j > 0) {
+                        for (int l = 0; l < j; l++) {
+                            const C vr = Vget(k0, r, l), wr = Wp[(long)r * NB + l];
+                            const C vi = Vget(k0, i, l), wi = Wp[(long)i * NB + l];
+                            a = a - (vr * conjx(wi) + wr * conjx(vi));
+                        }
+                        if (r == i) a.y = (T)0;
+                        stx(A, Aidx(r, i), a);
+                    }
+                    if (r == i) sc[0] = a.x;
+                    else if (r == i + 1) { sc[1] = a.x; sc[2] = a.y; }
+                    else nrm2 += abs2x(a);
+                }
+                __syncthreads();
+                nrm2 = block_sum_t(nrm2, red);
+                if (t == 0) {
+                    Dv[i] = (double)sc[0];
+                    const T ar = sc[1], ai = sc[2];
+                    T beta, taur, taui, sr, si;
+                    if (nrm2 == (T)0 && ai == (T)0) {
+                        beta = ar; taur = 0; taui = 0; sr = 0; si = 0;
+                    } else {
+                        const T nrm = sqrt(ar * ar + ai * ai + nrm2);
+                        beta = (ar >= (T)0) ? -nrm : nrm;
+                        taur = (beta - ar) / beta;
+                        taui = -ai / beta;
+                        const T dr = ar - beta, di = ai;
+                        const T inv = (T)1 / (dr * dr + di * di);
+                        sr = dr * inv; si = -di * inv;
+                    }
+                    Ev[i] = (double)beta;
+                    Tv[2 * i] = taur;
+                    Tv[2 * i + 1] = taui;
+                    sc[3] = taur; sc[4] = taui; sc[5] = sr; sc[6] = si; sc[7] = beta;
+                }
+                __syncthreads();
+                const C tau{sc[3], sc[4]};
+                const C scal{sc[5], sc[6]};
+                // (2) v = [1; x*scal] into A(i+1:, i) (A(i+1,i) keeps beta) and LDS
+                for (int r = i + 1 + t; r < n; r += NT) {
+                    C v;
+                    if (r == i + 1) {
+                        v = C{(T)1, (T)0};
+                        stx(A, Aidx(r, i), C{sc[7], (T)0});
+                    } else {
+                        v = ldx(A, Aidx(r, i)) * scal;
+                        stx(A, Aidx(r, i), v);
+                    }
+                    vb[r - (i + 1)] = v;
+                }
+                __syncthreads();
+                // (3) p = A22 v over rows/cols >= i+1 (A22 as at the panel start)
+                const int nr = n - (i + 1);
+                const int R = (nr >= NT) ? NT : ((nr + kWave - 1) / kWave) * kWave;
+                const int G = NT / R;
+                const int rl = t % R, cg = t / R;
+                C acc[MAXR];
+#pragma unroll
+                for (int k = 0; k < MAXR; k++) acc[k] = C{(T)0, (T)0};
+                if (cg < G) {
+                    for (int c = i + 1 + cg; c < n; c += G) {
+                        const C vn = vb[c - (i + 1)];
+#pragma unroll
+                        for (int k = 0; k < MAXR; k++) {
+                            const int r = i + 1 + rl + k * R;
+                            if (r < n) fmax_(acc[k], ldx(A, Aidx(r, c)), vn);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < MAXR; k++) {
+                    const int r = i + 1 + rl + k * R;
+                    if (r < n) part[cg * n + (r - (i + 1))] = acc[k];
+                }
+                // (4) dot products x_l = W(:,l)^H v, y_l = V(:,l)^H v (rows >= i+1, l < j)
+                {
+                    const int q = t % NQ, rg = t / NQ;
+                    const int l = q >> 1, kind = q & 1;
+                    C s{(T)0, (T)0};
+                    if (l < j) {
+                        for (int r = i + 1 + rg; r < n; r += RG) {
+                            const C x = kind == 0 ? Wp[(long)r * NB + l] : Vget(k0, r, l);
+                            fmacx(s, x, vb[r - (i + 1)]);
+                        }
+                    }
+                    dpart[rg * NQ + q] = s;
+                }
+                __syncthreads();
+                if (t < NQ) {
+                    C s{(T)0, (T)0};
+                    for (int g = 0; g < RG; g++) s = s + dpart[g * NQ + t];
+                    dots[t] = s;
+                }
+                __syncthreads();
+                // (5) p -= V x + W y; p *= tau; w = p - 1/2 tau (p^H v) v
+                C dotl{(T)0, (T)0};
+                for (int r = i + 1 + t; r < n; r += NT) {
+                    C p{(T)0, (T)0};
+                    for (int g = 0; g < G; g++) p = p + part[g * n + (r - (i + 1))];
+                    for (int l = 0; l < j; l++) {
+                        p = p - (Vget(k0, r, l) * dots[2 * l] + Wp[(long)r * NB + l] * dots[2 * l + 1]);
+                    }
+                    p = tau * p;
+                    part[r - (i + 1)] = p;               // row slot of group 0, read back by this thread only
+                    fmacx(dotl, p, vb[r - (i + 1)]);
+                }
+                const C dot = block_sum_cx(dotl, red);
+                const C alpha2 = (tau * dot) * (T)(-0.5);
+                for (int r = i + 1 + t; r < n; r += NT)
+                    Wp[(long)r * NB + j] = part[r - (i + 1)] + alpha2 * vb[r - (i + 1)];
+                __syncthreads();
+            }
+            // rank-2*nbk update of the trailing block rows/cols >= k1 = k0 + nbk
+            const int k1 = k0 + nbk;
+            const int nr = n - k1;
+            if (nr > 0) {
+                const int R = (nr >= NT) ? NT : ((nr + kWave - 1) / kWave) * kWave;
+                const int G = NT / R;
+                const int rl = t % R, cg = t / R;
+                if (cg < G) {
+#pragma unroll
+                    for (int k = 0; k < MAXR; k++) {
+                        const int r = k1 + rl + k * R;
+                        if (r >= n) continue;
+                        C vr[NB], wr[NB];
+#pragma unroll
+                        for (int l = 0; l < NB; l++) {
+                            vr[l] = l < nbk ? Vget(k0, r, l) : C{(T)0, (T)0};
+                            wr[l] = l < nbk ? Wp[(long)r * NB + l] : C{(T)0, (T)0};
+                        }
+                        for (int c = k1 + cg; c < n; c += G) {
+                            C a = ldx(A, Aidx(r, c));
+#pragma unroll
+                            for (int l = 0; l < NB; l++) {
+                                if (l < nbk) {
+                                    const C vc = Vget(k0, c, l), wc = Wp[(long)c * NB + l];
+                                    a = a - (vr[l] * conjx(wc) + wr[l] * conjx(vc));
+                                }
+                            }
+                            if (r == c) a.y = (T)0;
+                            stx(A, Aidx(r, c), a);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) Dv[n - 1] = (double)ldx(A, Aidx(n - 1, n - 1)).x;
+        __syncthreads();
+    }
+}
+
 // ============================================================================
 // 2. Divide & conquer on the symmetric tridiagonal + 3. back-transform
 // ============================================================================
@@ -270,11 +470,26 @@ template <int NT>
 __device__ double *stedc_block(int n, const double *dIn, const double *eIn, double *Qa, double *Qb, double *S,
                                DcShared &sh, double **dq_final) {
     const int t = threadIdx.x;
-    // init: torn diagonal, identity / zero
+    // scale T to unit max-norm (as LAPACK dstedc does) so that the deflation
+    // tolerance is relative to the matrix, whatever its magnitude
+    double amax = 0.0;
     for (int k = t; k < n; k += NT) {
-        double v = dIn[k];
-        if (k > 0) v -= fabs(eIn[k - 1]);
-        if (k < n - 1) v -= fabs(eIn[k]);
+        amax = fmax(amax, fabs(dIn[k]));
+        if (k < n - 1) amax = fmax(amax, fabs(eIn[k]));
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, kWave));
+    if ((t & (kWave - 1)) == 0) sh.dqn[t / kWave] = amax;
+    __syncthreads();
+    amax = 0.0;
+    for (int w = 0; w < NT / kWave; w++) amax = fmax(amax, sh.dqn[w]);
+    __syncthreads();
+    const double scale = (amax > 0.0) ? amax : 1.0;
+    const double iscale = 1.0 / scale;
+    // init: torn diagonal (scaled), identity / zero
+    for (int k = t; k < n; k += NT) {
+        double v = dIn[k] * iscale;
+        if (k > 0) v -= fabs(eIn[k - 1] * iscale);
+        if (k < n - 1) v -= fabs(eIn[k] * iscale);
         sh.dq[k] = v;
     }
     for (long idx = t; idx < (long)n * n; idx += NT) {
@@ -298,7 +513,7 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
             const int lo = k * blk, mid = lo + half, hi = min(lo + blk, n);
             const bool merging = active && mid < n;
             const int m = hi - lo, n1 = mid - lo;
-            const double beta = merging ? eIn[mid - 1] : 0.0;
+            const double beta = merging ? eIn[mid - 1] * iscale : 0.0;
             const double sgn = (beta >= 0.0) ? 1.0 : -1.0;
             const double rho = 2.0 * fabs(beta);
             const double isq2 = 0.70710678118654752440;
@@ -482,11 +697,13 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
         double *tq = Qa; Qa = Qb; Qb = tq;
         double *td = dq; dq = dqn; dqn = td;
     }
+    for (int k = t; k < n; k += NT) dq[k] *= scale;
+    __syncthreads();
     *dq_final = dq;
     return Qa;
 }
 
-template <typename T, int NT>
+template <typename T, int NT, int NBW>
 __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, long strideA, int lda, int n,
                                                   const double *__restrict__ Din, const double *__restrict__ Ein,
                                                   const T *__restrict__ TAUin, long strideV,
@@ -574,28 +791,28 @@ __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, l
             __syncthreads();
             // apply to every column: w = V^H u ; w = T w ; u -= V w
             for (int c = t; c < n; c += NT) {
-                C w[16];
+                C w[NBW];
 #pragma unroll
-                for (int k = 0; k < 16; k++) w[k] = C{(T)0, (T)0};
+                for (int k = 0; k < NBW; k++) w[k] = C{(T)0, (T)0};
                 for (int rr = 0; rr < nrow; rr++) {
                     const C u = ldx(U, (long)(r0 + rr) * n + c);
 #pragma unroll
-                    for (int k = 0; k < 16; k++)
+                    for (int k = 0; k < NBW; k++)
                         if (k < nb_wy) fmacx(w[k], Vl[rr * nb_wy + k], u);
                 }
-                C w2[16];
+                C w2[NBW];
 #pragma unroll
-                for (int i = 0; i < 16; i++) {
+                for (int i = 0; i < NBW; i++) {
                     C s{(T)0, (T)0};
 #pragma unroll
-                    for (int k = 0; k < 16; k++)
+                    for (int k = 0; k < NBW; k++)
                         if (i < nb_wy && k >= i && k < nb_wy) fmax_(s, Tl[i * nb_wy + k], w[k]);
                     w2[i] = s * (T)(-1);
                 }
                 for (int rr = 0; rr < nrow; rr++) {
                     C u = ldx(U, (long)(r0 + rr) * n + c);
 #pragma unroll
-                    for (int k = 0; k < 16; k++)
+                    for (int k = 0; k < NBW; k++)
                         if (k < nb_wy) fmax_(u, Vl[rr * nb_wy + k], w2[k]);
                     stx(U, (long)(r0 + rr) * n + c, u);
                 }
@@ -608,9 +825,9 @@ __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, l
 // ============================================================================
 // host launchers
 // ============================================================================
-size_t heevd_dc_lds_bytes(int n, int nb_wy) {
+size_t heevd_dc_lds_bytes(int n, int nb_wy, size_t real) {
     size_t dc = (size_t)n * (12 * sizeof(double) + 10 * sizeof(int));
-    size_t wy = (size_t)n * nb_wy * 16 + (size_t)nb_wy * (nb_wy + 1) * 16;
+    size_t wy = ((size_t)n * nb_wy + (size_t)nb_wy * (nb_wy + 1)) * 2 * real;
     return dc > wy ? dc : wy;
 }
 
@@ -637,30 +854,66 @@ int launch_zhetrd_t(T *A, long strideA, int lda, int n, double *D, double *E, T 
     return 0;
 }
 
+static size_t blocked_lds_bytes(int n, int NB, size_t real) {
+    constexpr int NT = 256;
+    return ((size_t)n * NB + n + (NT / kWave) * n + (NT / (2 * NB)) * 2 * NB + 2 * NB) * 2 * real +
+           (2 * (NT / kWave) + 8) * real;
+}
+
+template <typename T, int NB>
+int launch_zhetrd_blocked_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
+                            int grid, hipStream_t stream) {
+    if (n < 1 || n > 512) return -22;
+    constexpr int NT = 256;
+    const size_t lds = blocked_lds_bytes(n, NB, sizeof(T));
+    if (lds > 160 * 1024) return -22;
+    if (grid > batch) grid = batch;
+    if (lds > 64 * 1024)
+        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)zhetrd_blocked_kernel<T, NT, NB, 2>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((zhetrd_blocked_kernel<T, NT, NB, 2>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D,
+                       E, TAU, strideV, batch);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, const double *E, const T *TAU,
                   long strideV, double *W, long strideW, T *U, long strideU, double *work, int batch, int grid,
                   hipStream_t stream) {
     if (n < 1 || n > 512) return -22;
     constexpr int NT = 256;
-    const int nb_wy = 16;
-    size_t lds = heevd_dc_lds_bytes(n, nb_wy);
+    // reflectors per compact-WY block: the V block (n x NBW complex) lives in LDS
+    constexpr int NBW = sizeof(T) == 4 ? 32 : 16;
+    size_t lds = heevd_dc_lds_bytes(n, NBW, sizeof(T));
     if (lds > 160 * 1024) return -22;
     if (grid > batch) grid = batch;
-    if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)tail_kernel<T, NT>,
+    if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)tail_kernel<T, NT, NBW>,
                                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((tail_kernel<T, NT>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU,
-                       strideV, W, strideW, U, strideU, work, nb_wy, batch);
+    hipLaunchKernelGGL((tail_kernel<T, NT, NBW>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU,
+                       strideV, W, strideW, U, strideU, work, NBW, batch);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
+// RLM_HETRD=fused selects the unblocked fused kernel; the blocked kernel is the default.
+// Read at every launch so a process can compare both.
+static bool use_blocked() {
+    const char *e = getenv("RLM_HETRD");
+    return !(e && strcmp(e, "fused") == 0);
+}
+
 int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,
                   int grid, hipStream_t stream) {
+    // the blocked kernel keeps W (n x 16) in LDS: beyond 160 KB fall back to the fused kernel
+    if (use_blocked() && blocked_lds_bytes(n, 16, sizeof(double)) <= 160 * 1024)
+        return launch_zhetrd_blocked_t<double, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
     return launch_zhetrd_t<double>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
 }
 int launch_chetrd(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV, int batch,
                   int grid, hipStream_t stream) {
+    if (use_blocked() && blocked_lds_bytes(n, 16, sizeof(float)) <= 160 * 1024)
+        return launch_zhetrd_blocked_t<float, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
     return launch_zhetrd_t<float>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
 }
 int launch_tail(const double *A, long strideA, int lda, int n, const double *D, const double *E, const double *TAU,

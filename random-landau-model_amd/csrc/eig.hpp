@@ -27,6 +27,6 @@ int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *
                     long strideV, double *W, long strideW, float *U, long strideU, double *work, int batch, int grid,
                     hipStream_t stream);
 
-size_t heevd_dc_lds_bytes(int n, int nb_wy);
+size_t heevd_dc_lds_bytes(int n, int nb_wy, size_t real);
 
 }  // namespace rlm

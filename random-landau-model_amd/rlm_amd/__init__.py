@@ -179,12 +179,13 @@ class RlmPlan:
         shp = (n_samples, self.n_phi)
         return e.reshape(shp), c.reshape(shp), m.reshape(shp)
 
-    STAGES = ("potential+tables", "assembly", "tridiagonalisation", "dc+backtransform", "links", "plaquettes")
+    STAGES = ("potential+tables", "assembly", "tridiagonalisation", "dc+backtransform", "refinement", "links",
+              "plaquettes")
 
     def stage_times(self) -> dict:
-        ms = np.zeros(6)
+        ms = np.zeros(7)
         rc = lib().rlm_plan_stage_times(self._h, _ptr(ms))
-        if rc != 6:
+        if rc != 7:
             raise HipDiagError(f"rlm_plan_stage_times failed with status {rc}")
         return dict(zip(self.STAGES, ms.tolist()))
 

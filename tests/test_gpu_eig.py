@@ -43,6 +43,20 @@ def test_random_hermitian(n, b):
     _check(H, w.cpu().numpy(), U.cpu().numpy())
 
 
+@pytest.mark.parametrize("variant", ["fused", "blocked"])
+@pytest.mark.parametrize("n,b", [(5, 3), (17, 4), (100, 3), (256, 2)])
+def test_tridiagonalisation_variants(monkeypatch, variant, n, b):
+    """Both tridiagonalisation kernels (RLM_HETRD=fused | blocked) against LAPACK."""
+    import torch
+    from rlm_amd import heevd_batched
+    print(f"[test_tridiagonalisation_variants] {variant} n={n}", flush=True)
+    monkeypatch.setenv("RLM_HETRD", variant)
+    rng = np.random.default_rng(7 * n + 1)
+    H = _rand_herm(rng, b, n)
+    w, U = heevd_batched(torch.from_numpy(H).cuda())
+    _check(H, w.cpu().numpy(), U.cpu().numpy())
+
+
 @pytest.mark.parametrize("n", [384, 512])
 def test_large_sizes(n):
     import torch

@@ -19,7 +19,9 @@ def _cmp_sample(e, c, ref_e, ref_c, tag):
     scale = np.abs(ref_e).max()
     assert np.abs(e - ref_e).max() <= E_RTOL * scale, (tag, np.abs(e - ref_e).max() / scale)
     assert np.array_equal(np.rint(c), np.rint(ref_c)), (tag, np.rint(c), np.rint(ref_c))
-    assert np.rint(c).sum() == 1, tag
+    # the per-sample sum is 1 once the mesh resolves every plaquette phase; on coarse
+    # meshes (n_phi=8, g=6) the reference's own sum can differ, and must be matched
+    assert np.rint(c).sum() == np.rint(ref_c).sum(), tag
     assert np.abs(c - np.rint(c)).max() < 1e-6, tag
 
 
@@ -46,6 +48,7 @@ def test_config_fixtures(fixtures, name):
     e, c, m = plan.run(samples[0], len(samples))
     for k, row in enumerate(fx["samples"]):
         _cmp_sample(e[k], c[k], row["energy"], row["chern"], (name, row["sample"]))
+        assert np.rint(c[k]).sum() == 1
     plan.close()
 
 
