@@ -99,7 +99,21 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--cpu-lines", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--path", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calibration_mib:
+        import torch
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -149,8 +163,13 @@ def main():
         return {"elapsed": elapsed, "stage": stage_avg, "e": e, "c": c, "m": m, "resid": res,
                 "value": 3600.0 * S * args.steps / elapsed}
 
-    fp64 = run_path(HIP_DIAG_FP64, "fp64")
-    mixed = run_path(HIP_DIAG_MIXED, "mixed")
+    fp64 = run_path(HIP_DIAG_FP64, "fp64") if args.path in ("both", "fp64") else None
+    mixed = run_path(HIP_DIAG_MIXED, "mixed") if args.path in ("both", "mixed") else None
+    if fp64 is None or mixed is None:          # single-path run (profiling): report it alone
+        r = fp64 or mixed
+        print(json.dumps({"path": args.path, "samples_per_hour": r["value"],
+                          "stage_ms": {a: round(v, 3) for a, v in r["stage"].items()}}), flush=True)
+        return
 
     def chern_ok(r):
         ci = np.rint(r["c"])

@@ -106,6 +106,7 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
                hipEvent_t mid = nullptr) {
     int rc;
     const long strideV = 4L * n;   // d | e | tau(re,im) per matrix, interleaved blocks
+    if ((size_t)batch * strideV * sizeof(double) > ws.dev.bytes) return -22;
     double *base = ws.dev.d();
     // layout per matrix: D at base + b*strideV, E at +n, TAU at +2n (complex, 2n doubles)
     // launch_* take separate arrays with a common stride:
@@ -115,6 +116,7 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
     if ((rc = launch_zhetrd(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, gt, s))) return rc;
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
     const int gl = (int)(batch < kGridTail ? batch : kGridTail);
+    if ((size_t)gl * 3 * n * n * sizeof(double) > ws.work.bytes) return -22;
     if ((rc = launch_tail(A, (long)n * n, n, n, D, E, TAU, strideV, W, n, Uout, (long)n * n, ws.work.d(), (int)batch,
                           gl, s)))
         return rc;
@@ -122,19 +124,22 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
 }
 
 // FP32 tridiagonalisation and back-transform (mixed precision); D&C in FP64.
+// Same per-matrix layout of the d | e | tau slot as eig_packed (4n doubles); tau holds
+// 2(n-1) floats inside its 2n-double region.
 int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork &ws, hipStream_t s,
                    hipEvent_t mid = nullptr) {
     int rc;
-    const long strideV = 4L * n;
+    const long strideV = 4L * n;   // doubles per matrix, for D, E and TAU alike
+    if ((size_t)batch * strideV * sizeof(double) > ws.dev.bytes) return -22;
     double *base = ws.dev.d();
     double *D = base, *E = base + n;
-    float *TAU = reinterpret_cast<float *>(base + 2 * n);   // 2n floats inside the 2n-double slot
+    float *TAU = reinterpret_cast<float *>(base + 2 * n);
     const int gt = (int)(batch < kGridTrd ? batch : kGridTrd);
-    // TAU uses a stride of 2*strideV floats (= strideV doubles) per matrix
-    if ((rc = launch_chetrd(A, (long)n * n, n, n, D, E, TAU, 2 * strideV, (int)batch, gt, s))) return rc;
+    if ((rc = launch_chetrd(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, gt, s))) return rc;
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
     const int gl = (int)(batch < kGridTail ? batch : kGridTail);
-    if ((rc = launch_tail_f32(A, (long)n * n, n, n, D, E, TAU, 2 * strideV, W, n, Uout, (long)n * n, ws.work.d(),
+    if ((size_t)gl * 3 * n * n * sizeof(double) > ws.work.bytes) return -22;
+    if ((rc = launch_tail_f32(A, (long)n * n, n, n, D, E, TAU, strideV, W, n, Uout, (long)n * n, ws.work.d(),
                               (int)batch, gl, s)))
         return rc;
     return 0;

@@ -48,7 +48,8 @@ __global__ __launch_bounds__(NT) void zhetrd_kernel(T *__restrict__ Abase, long 
         T *A = Abase + 2 * b * strideA;                // complex index = 2*...
         double *Dv = Dout + b * strideV;
         double *Ev = Eout + b * strideV;
-        T *Tv = TAUout + b * strideV;                  // tau: complex, strideV reals per matrix
+        // strideV: per-matrix stride in doubles of D, E and TAU alike (TAU holds T values)
+        T *Tv = TAUout + b * strideV * (long)(sizeof(double) / sizeof(T));
         auto Aidx = [&](int r, int c) -> long { return (long)c * lda + r; };
 
         int cur = 0;          // vbuf/wbuf slot holding the pending update
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(NT) void zhetrd_blocked_kernel(T *__restrict__ Abas
         T *A = Abase + 2 * b * strideA;
         double *Dv = Dout + b * strideV;
         double *Ev = Eout + b * strideV;
-        T *Tv = TAUout + b * strideV;
+        T *Tv = TAUout + b * strideV * (long)(sizeof(double) / sizeof(T));
         auto Aidx = [&](int r, int c) -> long { return (long)c * lda + r; };
         // V(r, l) of the current panel (reflector of column k0 + l)
         auto Vget = [&](int k0, int r, int l) -> C {
@@ -377,11 +378,230 @@ __global__ __launch_bounds__(NT) void zhetrd_blocked_kernel(T *__restrict__ Abas
     }
 }
 
+// ----------------------------------------------------------------------------
+// Lower-storage blocked variant: as zhetrd_blocked_kernel, but only the lower
+// triangle of A is ever read or written. The Hermitian product A22*v walks the
+// lower triangle in 32x32 tiles, one wave per tile: each element read once feeds
+// both y_r += A(r,c) v_c and y_c += conj(A(r,c)) v_r (column sums by wave
+// shuffles). Each wave accumulates into its own LDS vector; the 4 vectors are
+// summed in a fixed order, so results do not depend on scheduling. Halves the
+// trailing-matrix traffic of the full-storage kernels.
+template <typename T, int NT, int NB, int MAXR>
+__global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
+                                                          double *__restrict__ Dout, double *__restrict__ Eout,
+                                                          T *__restrict__ TAUout, long strideV, int batch) {
+    using C = cx<T>;
+    constexpr int NW = NT / kWave;
+    constexpr int NQ = 2 * NB;
+    constexpr int RG = NT / NQ;
+    constexpr int TS = 32;
+    extern __shared__ double smem_d[];
+    C *Wp = reinterpret_cast<C *>(smem_d);           // n x NB, row-major
+    C *vb = Wp + (long)n * NB;                       // n
+    C *ypart = vb + n;                               // NW x n (one vector per wave)
+    C *dpart = ypart + NW * n;                       // RG x NQ
+    C *dots = dpart + RG * NQ;                       // NQ
+    T *red = reinterpret_cast<T *>(dots + NQ);
+    T *sc = red + 2 * NW;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1), wid = t / kWave;
+
+    for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+        T *A = Abase + 2 * b * strideA;
+        double *Dv = Dout + b * strideV;
+        double *Ev = Eout + b * strideV;
+        T *Tv = TAUout + b * strideV * (long)(sizeof(double) / sizeof(T));
+        auto Aidx = [&](int r, int c) -> long { return (long)c * lda + r; };
+        auto Vget = [&](int k0, int r, int l) -> C {
+            const int j = k0 + l;
+            if (r < j + 1) return C{(T)0, (T)0};
+            if (r == j + 1) return C{(T)1, (T)0};
+            return ldx(A, Aidx(r, j));
+        };
+        for (int k0 = 0; k0 < n - 1; k0 += NB) {
+            const int nbk = min(NB, n - 1 - k0);
+            for (int j = 0; j < nbk; j++) {
+                const int i = k0 + j;
+                // (1) column i, rows >= i
+                T nrm2 = 0;
+                for (int r = i + t; r < n; r += NT) {
+                    C a = ldx(A, Aidx(r, i));
+                    if (j > 0) {
+                        for (int l = 0; l < j; l++) {
+                            const C vr = Vget(k0, r, l), wr = Wp[(long)r * NB + l];
+                            const C vi = Vget(k0, i, l), wi = Wp[(long)i * NB + l];
+                            a = a - (vr * conjx(wi) + wr * conjx(vi));
+                        }
+                        if (r == i) a.y = (T)0;
+                        stx(A, Aidx(r, i), a);
+                    }
+                    if (r == i) sc[0] = a.x;
+                    else if (r == i + 1) { sc[1] = a.x; sc[2] = a.y; }
+                    else nrm2 += abs2x(a);
+                }
+                __syncthreads();
+                nrm2 = block_sum_t(nrm2, red);
+                if (t == 0) {
+                    Dv[i] = (double)sc[0];
+                    const T ar = sc[1], ai = sc[2];
+                    T beta, taur, taui, sr, si;
+                    if (nrm2 == (T)0 && ai == (T)0) {
+                        beta = ar; taur = 0; taui = 0; sr = 0; si = 0;
+                    } else {
+                        const T nrm = sqrt(ar * ar + ai * ai + nrm2);
+                        beta = (ar >= (T)0) ? -nrm : nrm;
+                        taur = (beta - ar) / beta;
+                        taui = -ai / beta;
+                        const T dr = ar - beta, di = ai;
+                        const T inv = (T)1 / (dr * dr + di * di);
+                        sr = dr * inv; si = -di * inv;
+                    }
+                    Ev[i] = (double)beta;
+                    Tv[2 * i] = taur;
+                    Tv[2 * i + 1] = taui;
+                    sc[3] = taur; sc[4] = taui; sc[5] = sr; sc[6] = si; sc[7] = beta;
+                }
+                __syncthreads();
+                const C tau{sc[3], sc[4]};
+                const C scal{sc[5], sc[6]};
+                // (2) v, and clear the per-wave accumulators
+                for (int r = i + 1 + t; r < n; r += NT) {
+                    C v;
+                    if (r == i + 1) {
+                        v = C{(T)1, (T)0};
+                        stx(A, Aidx(r, i), C{sc[7], (T)0});
+                    } else {
+                        v = ldx(A, Aidx(r, i)) * scal;
+                        stx(A, Aidx(r, i), v);
+                    }
+                    vb[r - (i + 1)] = v;
+                }
+                for (int e = t; e < NW * n; e += NT) ypart[e] = C{(T)0, (T)0};
+                __syncthreads();
+                // (3) y = A22 v from the lower triangle, 32x32 tiles, one wave per tile
+                const int m = n - (i + 1);
+                const int Tn = (m + TS - 1) / TS;
+                const int ntiles = Tn * (Tn + 1) / 2;
+                C *yw = ypart + wid * n;
+                const int rl = lane & 31, ch = lane >> 5;
+                for (int tk = wid; tk < ntiles; tk += NW) {
+                    // tile index tk -> (R, Cc) with R >= Cc, row-major over the lower triangle of tiles
+                    int R = 0;
+                    while ((R + 1) * (R + 2) / 2 <= tk) R++;
+                    const int Cc = tk - R * (R + 1) / 2;
+                    const int rloc = R * TS + rl;                 // local row in A22
+                    const int r = i + 1 + rloc;
+                    const bool rv = rloc < m;
+                    const C vr = rv ? vb[rloc] : C{(T)0, (T)0};
+                    C rowacc{(T)0, (T)0};
+#pragma unroll 4
+                    for (int k = 0; k < TS / 2; k++) {
+                        const int cloc = Cc * TS + ch + 2 * k;
+                        const int c = i + 1 + cloc;
+                        bool use = rv && cloc < m && (R > Cc || rloc >= cloc);
+                        C a{(T)0, (T)0};
+                        if (use) a = ldx(A, Aidx(r, c));
+                        if (use) fmax_(rowacc, a, vb[cloc]);
+                        // transposed contribution to y_c (strictly lower elements only)
+                        C tcon{(T)0, (T)0};
+                        if (use && rloc != cloc) fmacx(tcon, a, vr);
+                        // sum over the 32 rows of this column (lanes with the same ch)
+#pragma unroll
+                        for (int off = 16; off > 0; off >>= 1) {
+                            tcon.x += __shfl_xor(tcon.x, off, kWave);
+                            tcon.y += __shfl_xor(tcon.y, off, kWave);
+                        }
+                        if (rl == 0 && cloc < m) yw[cloc] = yw[cloc] + tcon;
+                    }
+                    // combine the two column halves of each row
+                    rowacc.x += __shfl_xor(rowacc.x, 32, kWave);
+                    rowacc.y += __shfl_xor(rowacc.y, 32, kWave);
+                    if (ch == 0 && rv) yw[rloc] = yw[rloc] + rowacc;
+                }
+                // (4) dot products x_l = W(:,l)^H v, y_l = V(:,l)^H v (rows >= i+1, l < j)
+                {
+                    const int q = t % NQ, rg = t / NQ;
+                    const int l = q >> 1, kind = q & 1;
+                    C s2{(T)0, (T)0};
+                    if (l < j) {
+                        for (int r = i + 1 + rg; r < n; r += RG) {
+                            const C x = kind == 0 ? Wp[(long)r * NB + l] : Vget(k0, r, l);
+                            fmacx(s2, x, vb[r - (i + 1)]);
+                        }
+                    }
+                    dpart[rg * NQ + q] = s2;
+                }
+                __syncthreads();
+                if (t < NQ) {
+                    C s2{(T)0, (T)0};
+                    for (int g = 0; g < RG; g++) s2 = s2 + dpart[g * NQ + t];
+                    dots[t] = s2;
+                }
+                __syncthreads();
+                // (5) p = sum of the wave vectors - V x - W y; p *= tau; w = p - 1/2 tau (p^H v) v
+                C dotl{(T)0, (T)0};
+                for (int r = i + 1 + t; r < n; r += NT) {
+                    const int rr = r - (i + 1);
+                    C p{(T)0, (T)0};
+                    for (int w = 0; w < NW; w++) p = p + ypart[w * n + rr];
+                    for (int l = 0; l < j; l++)
+                        p = p - (Vget(k0, r, l) * dots[2 * l] + Wp[(long)r * NB + l] * dots[2 * l + 1]);
+                    p = tau * p;
+                    ypart[rr] = p;                    // slot of wave 0, read back by this thread only
+                    fmacx(dotl, p, vb[rr]);
+                }
+                const C dot = block_sum_cx(dotl, red);
+                const C alpha2 = (tau * dot) * (T)(-0.5);
+                for (int r = i + 1 + t; r < n; r += NT)
+                    Wp[(long)r * NB + j] = ypart[r - (i + 1)] + alpha2 * vb[r - (i + 1)];
+                __syncthreads();
+            }
+            // rank-2*nbk update of the lower triangle of the trailing block (rows >= cols >= k1)
+            const int k1 = k0 + nbk;
+            const int nr = n - k1;
+            if (nr > 0) {
+                const int R = (nr >= NT) ? NT : ((nr + kWave - 1) / kWave) * kWave;
+                const int G = NT / R;
+                const int rl = t % R, cg = t / R;
+                if (cg < G) {
+#pragma unroll
+                    for (int k = 0; k < MAXR; k++) {
+                        const int r = k1 + rl + k * R;
+                        if (r >= n) continue;
+                        C vr[NB], wr[NB];
+#pragma unroll
+                        for (int l = 0; l < NB; l++) {
+                            vr[l] = l < nbk ? Vget(k0, r, l) : C{(T)0, (T)0};
+                            wr[l] = l < nbk ? Wp[(long)r * NB + l] : C{(T)0, (T)0};
+                        }
+                        for (int c = k1 + cg; c <= r; c += G) {
+                            C a = ldx(A, Aidx(r, c));
+#pragma unroll
+                            for (int l = 0; l < NB; l++) {
+                                if (l < nbk) {
+                                    const C vc = Vget(k0, c, l), wc = Wp[(long)c * NB + l];
+                                    a = a - (vr[l] * conjx(wc) + wr[l] * conjx(vc));
+                                }
+                            }
+                            if (r == c) a.y = (T)0;
+                            stx(A, Aidx(r, c), a);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) Dv[n - 1] = (double)ldx(A, Aidx(n - 1, n - 1)).x;
+        __syncthreads();
+    }
+}
+
 // ============================================================================
 // 2. Divide & conquer on the symmetric tridiagonal + 3. back-transform
 // ============================================================================
 struct DcShared {
     double *dq, *dqn, *ds, *zs, *dl, *wz, *tau, *vals, *rc, *rs, *what, *inrm;
+    double *red;   // 8 doubles of reduction scratch
     int *perm, *orig, *rank, *rpj, *rnj, *ndx, *dfx;
     int *cnt;  // per node: K, ndefl, nrot  (3 ints per node slot)
 };
@@ -478,10 +698,10 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
         if (k < n - 1) amax = fmax(amax, fabs(eIn[k]));
     }
     for (int off = kWave / 2; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, kWave));
-    if ((t & (kWave - 1)) == 0) sh.dqn[t / kWave] = amax;
+    if ((t & (kWave - 1)) == 0) sh.red[t / kWave] = amax;
     __syncthreads();
     amax = 0.0;
-    for (int w = 0; w < NT / kWave; w++) amax = fmax(amax, sh.dqn[w]);
+    for (int w = 0; w < NT / kWave; w++) amax = fmax(amax, sh.red[w]);
     __syncthreads();
     const double scale = (amax > 0.0) ? amax : 1.0;
     const double iscale = 1.0 / scale;
@@ -723,6 +943,7 @@ __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, l
         sh.dq = p; p += n; sh.dqn = p; p += n; sh.ds = p; p += n; sh.zs = p; p += n;
         sh.dl = p; p += n; sh.wz = p; p += n; sh.tau = p; p += n; sh.vals = p; p += n;
         sh.rc = p; p += n; sh.rs = p; p += n; sh.what = p; p += n; sh.inrm = p; p += n;
+        sh.red = p; p += 8;
         int *ip = reinterpret_cast<int *>(p);
         sh.perm = ip; ip += n; sh.orig = ip; ip += n; sh.rank = ip; ip += n; sh.rpj = ip; ip += n;
         sh.rnj = ip; ip += n; sh.ndx = ip; ip += n; sh.dfx = ip; ip += n; sh.cnt = ip; ip += 3 * n;
@@ -731,7 +952,7 @@ __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, l
         const T *A = Abase + 2 * b * strideA;
         const double *dIn = Din + b * strideV;
         const double *eIn = Ein + b * strideV;
-        const T *tauv = TAUin + b * strideV;
+        const T *tauv = TAUin + b * strideV * (long)(sizeof(double) / sizeof(T));
         double *W = Wout + b * strideW;
         T *U = Uout + 2 * b * strideU;
         double *dqf;
@@ -826,7 +1047,7 @@ __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, l
 // host launchers
 // ============================================================================
 size_t heevd_dc_lds_bytes(int n, int nb_wy, size_t real) {
-    size_t dc = (size_t)n * (12 * sizeof(double) + 10 * sizeof(int));
+    size_t dc = (size_t)n * (12 * sizeof(double) + 10 * sizeof(int)) + 8 * sizeof(double);
     size_t wy = ((size_t)n * nb_wy + (size_t)nb_wy * (nb_wy + 1)) * 2 * real;
     return dc > wy ? dc : wy;
 }
@@ -877,6 +1098,29 @@ int launch_zhetrd_blocked_t(T *A, long strideA, int lda, int n, double *D, doubl
     return 0;
 }
 
+static size_t lower_lds_bytes(int n, int NB, size_t real) {
+    constexpr int NT = 256;
+    return ((size_t)n * NB + n + (NT / kWave) * n + (NT / (2 * NB)) * 2 * NB + 2 * NB) * 2 * real +
+           (2 * (NT / kWave) + 8) * real;
+}
+
+template <typename T, int NB>
+int launch_zhetrd_lower_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
+                          int grid, hipStream_t stream) {
+    if (n < 1 || n > 512) return -22;
+    constexpr int NT = 256;
+    const size_t lds = lower_lds_bytes(n, NB, sizeof(T));
+    if (lds > 160 * 1024) return -22;
+    if (grid > batch) grid = batch;
+    if (lds > 64 * 1024)
+        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)zhetrd_lower_kernel<T, NT, NB, 2>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((zhetrd_lower_kernel<T, NT, NB, 2>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D,
+                       E, TAU, strideV, batch);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, const double *E, const T *TAU,
                   long strideV, double *W, long strideW, T *U, long strideU, double *work, int batch, int grid,
@@ -896,23 +1140,32 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
     return 0;
 }
 
-// RLM_HETRD=fused selects the unblocked fused kernel; the blocked kernel is the default.
-// Read at every launch so a process can compare both.
-static bool use_blocked() {
+// RLM_HETRD selects the tridiagonalisation kernel: "fused" (unblocked, full storage),
+// "blocked" (panels, full storage; default) or "lower" (panels, lower storage, tiled
+// symv). Read at every launch so one process can compare them.
+static int hetrd_variant() {
     const char *e = getenv("RLM_HETRD");
-    return !(e && strcmp(e, "fused") == 0);
+    if (e && strcmp(e, "fused") == 0) return 0;
+    if (e && strcmp(e, "lower") == 0) return 2;
+    return 1;
 }
 
 int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,
                   int grid, hipStream_t stream) {
-    // the blocked kernel keeps W (n x 16) in LDS: beyond 160 KB fall back to the fused kernel
-    if (use_blocked() && blocked_lds_bytes(n, 16, sizeof(double)) <= 160 * 1024)
+    // the panel kernels keep W (n x 16) in LDS: beyond 160 KB fall back to the fused kernel
+    const int v = hetrd_variant();
+    if (v == 2 && lower_lds_bytes(n, 16, sizeof(double)) <= 160 * 1024)
+        return launch_zhetrd_lower_t<double, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+    if (v >= 1 && blocked_lds_bytes(n, 16, sizeof(double)) <= 160 * 1024)
         return launch_zhetrd_blocked_t<double, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
     return launch_zhetrd_t<double>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
 }
 int launch_chetrd(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV, int batch,
                   int grid, hipStream_t stream) {
-    if (use_blocked() && blocked_lds_bytes(n, 16, sizeof(float)) <= 160 * 1024)
+    const int v = hetrd_variant();
+    if (v == 2 && lower_lds_bytes(n, 16, sizeof(float)) <= 160 * 1024)
+        return launch_zhetrd_lower_t<float, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+    if (v >= 1 && blocked_lds_bytes(n, 16, sizeof(float)) <= 160 * 1024)
         return launch_zhetrd_blocked_t<float, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
     return launch_zhetrd_t<float>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
 }

@@ -29,16 +29,18 @@ static void usage(const char *target) {
     cout << "  -c  # of disorder configurations(n_sample)\n";
     cout << "  -s  seed to generate seeds\n";
     cout << "  -t  # of threads\n";
+    cout << "  -p  precision: 0 = FP64 (default), 1 = mixed (FP32 reduction + FP64 refinement)\n";
 }
 
 int main(int argc, char *argv[]) {
     // defaults of rlm/init_rlm.cpp:20-28
     long n_phi = 32, n_mesh = 32, n_sample = 1, num_threads = 1, off_override = -1;
+    int precision = HIP_DIAG_FP64;
     double q = 1, imp = 16;
     unsigned long seed = std::chrono::high_resolution_clock::now().time_since_epoch().count();
     int ch, err_flag = 0;
     // the reference's optstring is "n:q:i:g:c:s:h:t:"; -d (documented there) is accepted here
-    while ((ch = getopt(argc, argv, "n:q:i:g:d:c:s:h:t:")) != -1) {
+    while ((ch = getopt(argc, argv, "n:q:i:g:d:c:s:h:t:p:")) != -1) {
         switch (ch) {
             case 'n': n_phi = atoi(optarg); break;
             case 'q': q = atof(optarg); break;
@@ -48,6 +50,7 @@ int main(int argc, char *argv[]) {
             case 'c': n_sample = atoi(optarg); break;
             case 's': seed = atoi(optarg); break;
             case 't': num_threads = atoi(optarg); break;
+            case 'p': precision = atoi(optarg) == 1 ? HIP_DIAG_MIXED : HIP_DIAG_FP64; break;
             default: err_flag++; break;
         }
     }
@@ -75,7 +78,7 @@ int main(int argc, char *argv[]) {
 
     auto t1 = chrono::high_resolution_clock::now();
     rlm_plan_t plan = nullptr;
-    int rc = rlm_plan_create(&plan, n_phi, q, imp, n_mesh, off_override, seed, (int)spb, HIP_DIAG_FP64);
+    int rc = rlm_plan_create(&plan, n_phi, q, imp, n_mesh, off_override, seed, (int)spb, precision);
     if (rc) {
         cerr << "rlm_plan_create failed with status " << rc << endl;
         return 1;

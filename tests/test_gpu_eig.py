@@ -43,7 +43,7 @@ def test_random_hermitian(n, b):
     _check(H, w.cpu().numpy(), U.cpu().numpy())
 
 
-@pytest.mark.parametrize("variant", ["fused", "blocked"])
+@pytest.mark.parametrize("variant", ["fused", "blocked", "lower"])
 @pytest.mark.parametrize("n,b", [(5, 3), (17, 4), (100, 3), (256, 2)])
 def test_tridiagonalisation_variants(monkeypatch, variant, n, b):
     """Both tridiagonalisation kernels (RLM_HETRD=fused | blocked) against LAPACK."""
