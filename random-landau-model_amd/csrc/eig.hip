@@ -478,45 +478,65 @@ __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase,
                 }
                 for (int e = t; e < NW * n; e += NT) ypart[e] = C{(T)0, (T)0};
                 __syncthreads();
-                // (3) y = A22 v from the lower triangle, 32x32 tiles, one wave per tile
+                // (3) y = A22 v from the lower triangle, 32x32 tiles, one wave per tile.
+                // Lane = (rg, cg) owns the 4x4 sub-block rows 4rg.., cols 4cg.. of the tile:
+                // row sums reduce over cg (lane bits 3-5), column sums over rg (lane bits 0-2).
                 const int m = n - (i + 1);
                 const int Tn = (m + TS - 1) / TS;
                 const int ntiles = Tn * (Tn + 1) / 2;
                 C *yw = ypart + wid * n;
-                const int rl = lane & 31, ch = lane >> 5;
+                const int lrg = lane & 7, lcg = lane >> 3;
                 for (int tk = wid; tk < ntiles; tk += NW) {
-                    // tile index tk -> (R, Cc) with R >= Cc, row-major over the lower triangle of tiles
                     int R = 0;
                     while ((R + 1) * (R + 2) / 2 <= tk) R++;
                     const int Cc = tk - R * (R + 1) / 2;
-                    const int rloc = R * TS + rl;                 // local row in A22
-                    const int r = i + 1 + rloc;
-                    const bool rv = rloc < m;
-                    const C vr = rv ? vb[rloc] : C{(T)0, (T)0};
-                    C rowacc{(T)0, (T)0};
-#pragma unroll 4
-                    for (int k = 0; k < TS / 2; k++) {
-                        const int cloc = Cc * TS + ch + 2 * k;
-                        const int c = i + 1 + cloc;
-                        bool use = rv && cloc < m && (R > Cc || rloc >= cloc);
-                        C a{(T)0, (T)0};
-                        if (use) a = ldx(A, Aidx(r, c));
-                        if (use) fmax_(rowacc, a, vb[cloc]);
-                        // transposed contribution to y_c (strictly lower elements only)
-                        C tcon{(T)0, (T)0};
-                        if (use && rloc != cloc) fmacx(tcon, a, vr);
-                        // sum over the 32 rows of this column (lanes with the same ch)
+                    const int rb = R * TS + 4 * lrg;          // first local row of the sub-block
+                    const int cb = Cc * TS + 4 * lcg;         // first local column
+                    C vrow[4], vcol[4], rs[4], cs[4];
 #pragma unroll
-                        for (int off = 16; off > 0; off >>= 1) {
-                            tcon.x += __shfl_xor(tcon.x, off, kWave);
-                            tcon.y += __shfl_xor(tcon.y, off, kWave);
-                        }
-                        if (rl == 0 && cloc < m) yw[cloc] = yw[cloc] + tcon;
+                    for (int u = 0; u < 4; u++) {
+                        vrow[u] = (rb + u < m) ? vb[rb + u] : C{(T)0, (T)0};
+                        vcol[u] = (cb + u < m) ? vb[cb + u] : C{(T)0, (T)0};
+                        rs[u] = C{(T)0, (T)0};
+                        cs[u] = C{(T)0, (T)0};
                     }
-                    // combine the two column halves of each row
-                    rowacc.x += __shfl_xor(rowacc.x, 32, kWave);
-                    rowacc.y += __shfl_xor(rowacc.y, 32, kWave);
-                    if (ch == 0 && rv) yw[rloc] = yw[rloc] + rowacc;
+#pragma unroll
+                    for (int cu = 0; cu < 4; cu++) {
+                        const int cloc = cb + cu;
+#pragma unroll
+                        for (int ru = 0; ru < 4; ru++) {
+                            const int rloc = rb + ru;
+                            const bool use = rloc < m && cloc < m && (R > Cc || rloc >= cloc);
+                            if (use) {
+                                const C a = ldx(A, Aidx(i + 1 + rloc, i + 1 + cloc));
+                                fmax_(rs[ru], a, vcol[cu]);                      // y_r += A(r,c) v_c
+                                if (rloc != cloc) fmacx(cs[cu], a, vrow[ru]);   // y_c += conj(A(r,c)) v_r
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+#pragma unroll
+                        for (int off = 8; off < 64; off <<= 1) {     // sum over the 8 column groups
+                            rs[u].x += __shfl_xor(rs[u].x, off, kWave);
+                            rs[u].y += __shfl_xor(rs[u].y, off, kWave);
+                        }
+#pragma unroll
+                        for (int off = 1; off < 8; off <<= 1) {      // sum over the 8 row groups
+                            cs[u].x += __shfl_xor(cs[u].x, off, kWave);
+                            cs[u].y += __shfl_xor(cs[u].y, off, kWave);
+                        }
+                    }
+                    if (lcg == 0) {
+#pragma unroll
+                        for (int u = 0; u < 4; u++)
+                            if (rb + u < m) yw[rb + u] = yw[rb + u] + rs[u];
+                    }
+                    if (lrg == 0) {
+#pragma unroll
+                        for (int u = 0; u < 4; u++)
+                            if (cb + u < m) yw[cb + u] = yw[cb + u] + cs[u];
+                    }
                 }
                 // (4) dot products x_l = W(:,l)^H v, y_l = V(:,l)^H v (rows >= i+1, l < j)
                 {
@@ -603,7 +623,8 @@ struct DcShared {
     double *dq, *dqn, *ds, *zs, *dl, *wz, *tau, *vals, *rc, *rs, *what, *inrm;
     double *red;   // 8 doubles of reduction scratch
     int *perm, *orig, *rank, *rpj, *rnj, *ndx, *dfx;
-    int *cnt;  // per node: K, ndefl, nrot  (3 ints per node slot)
+    int *pcol, *lstL, *lstR, *rflag;  // eigenvector-update gather index, per-half column lists, rotated flags
+    int *cnt;  // per node: K, ndefl, nrot, |lstL|, |lstR|  (5 ints per node slot)
 };
 
 __device__ inline double dc_eps() { return 1.1102230246251565e-16; }  // 2^-53
@@ -755,6 +776,7 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                     sh.ds[lo + pos] = val;
                     sh.zs[lo + pos] = zi * isq2;
                     sh.perm[lo + pos] = i;
+                    sh.rflag[lo + pos] = 0;
                 }
             }
             __syncthreads();
@@ -766,7 +788,17 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                     zmax = fmax(zmax, fabs(sh.zs[lo + i]));
                 }
                 const double tol = 8.0 * dc_eps() * fmax(dmax, zmax);
-                int K = 0, nd = 0, nr = 0, pj = -1;
+                int K = 0, nd = 0, nr = 0, pj = -1, KL = 0, KR = 0;
+                // a non-deflated column q: record its node-local source column and which
+                // halves of the node's rows it can be non-zero in (a rotation mixes halves)
+                auto append = [&](int sj) {
+                    const int pc = sh.perm[lo + sj];
+                    sh.ndx[lo + K] = sj;
+                    sh.pcol[lo + K] = pc;
+                    if (sh.rflag[lo + sj] || pc < n1) sh.lstL[lo + KL++] = K;
+                    if (sh.rflag[lo + sj] || pc >= n1) sh.lstR[lo + KR++] = K;
+                    K++;
+                };
                 for (int nj = 0; nj < m; nj++) {
                     if (rho * fabs(sh.zs[lo + nj]) <= tol) {
                         sh.dfx[lo + nd++] = nj;
@@ -783,6 +815,8 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                         sh.zs[lo + pj] = 0.0;
                         sh.rpj[lo + nr] = pj;
                         sh.rnj[lo + nr] = nj;
+                        sh.rflag[lo + pj] = 1;
+                        sh.rflag[lo + nj] = 1;
                         sh.rc[lo + nr] = C_;
                         sh.rs[lo + nr] = S_;
                         nr++;
@@ -793,19 +827,25 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                         sh.dfx[lo + nd++] = pj;
                         pj = nj;
                     } else {
-                        sh.ndx[lo + K++] = pj;
+                        append(pj);
                         pj = nj;
                     }
                 }
-                if (pj >= 0) sh.ndx[lo + K++] = pj;
-                sh.cnt[3 * k + 0] = K;
-                sh.cnt[3 * k + 1] = nd;
-                sh.cnt[3 * k + 2] = nr;
+                if (pj >= 0) append(pj);
+                sh.cnt[5 * k + 0] = K;
+                sh.cnt[5 * k + 1] = nd;
+                sh.cnt[5 * k + 2] = nr;
+                sh.cnt[5 * k + 3] = KL;
+                sh.cnt[5 * k + 4] = KR;
             }
             __syncthreads();
             // P3: apply deflation rotations to Qa's node block; secular inputs
             int K = 0, nd = 0, nr = 0;
-            if (merging) { K = sh.cnt[3 * k]; nd = sh.cnt[3 * k + 1]; nr = sh.cnt[3 * k + 2]; }
+            int KL = 0, KR = 0;
+            if (merging) {
+                K = sh.cnt[5 * k]; nd = sh.cnt[5 * k + 1]; nr = sh.cnt[5 * k + 2];
+                KL = sh.cnt[5 * k + 3]; KR = sh.cnt[5 * k + 4];
+            }
             if (merging) {
                 if (nr > 0) {
                     for (int r = lo + lane; r < hi; r += G) {
@@ -890,20 +930,41 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                 }
             }
             __syncthreads();
-            // P8: Qb block = Qa block (rotated, permuted) * U ; deflated columns copied
+            // P8: Qb block = Qa block (rotated, gathered) * U; deflated columns copied.
+            // Rows of the left half only meet columns of lstL (left child or rotated), rows
+            // of the right half only lstR (the block-diagonal structure dlaed3 exploits).
+            // Each task is one row and 4 consecutive output columns.
             if (merging) {
-                for (int idx = lane; idx < m * m; idx += G) {
-                    const int r = lo + idx / m, q = idx % m;
+                const int nq4 = (K + 3) >> 2;
+                for (int task = lane; task < m * nq4; task += G) {
+                    const int rloc = task / nq4, q0 = (task - rloc * nq4) * 4;
+                    const int r = lo + rloc;
                     const double *row = Qa + (long)r * n + lo;
-                    double v;
-                    if (q < K) {
-                        v = 0.0;
-                        for (int i = 0; i < K; i++)
-                            v = fma(row[sh.perm[lo + sh.ndx[lo + i]]], S[(long)(lo + i) * n + q], v);
-                    } else {
-                        v = row[sh.perm[lo + sh.dfx[lo + q - K]]];
+                    const bool left = rloc < n1;
+                    const int *lst = left ? sh.lstL + lo : sh.lstR + lo;
+                    const int cntl = left ? KL : KR;
+                    const int qa = q0, qb = min(q0 + 1, K - 1), qc = min(q0 + 2, K - 1), qd = min(q0 + 3, K - 1);
+                    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll 4
+                    for (int idx = 0; idx < cntl; idx++) {
+                        const int i = lst[idx];
+                        const double x = row[sh.pcol[lo + i]];
+                        const double *srow = S + (long)(lo + i) * n;
+                        a0 = fma(x, srow[qa], a0);
+                        a1 = fma(x, srow[qb], a1);
+                        a2 = fma(x, srow[qc], a2);
+                        a3 = fma(x, srow[qd], a3);
                     }
-                    Qb[(long)r * n + lo + sh.rank[lo + q]] = v;
+                    double *orow = Qb + (long)r * n + lo;
+                    orow[sh.rank[lo + q0]] = a0;
+                    if (q0 + 1 < K) orow[sh.rank[lo + q0 + 1]] = a1;
+                    if (q0 + 2 < K) orow[sh.rank[lo + q0 + 2]] = a2;
+                    if (q0 + 3 < K) orow[sh.rank[lo + q0 + 3]] = a3;
+                }
+                for (int idx = lane; idx < m * nd; idx += G) {
+                    const int rloc = idx / nd, qq = idx - rloc * nd;
+                    const int r = lo + rloc;
+                    Qb[(long)r * n + lo + sh.rank[lo + K + qq]] = Qa[(long)r * n + lo + sh.perm[lo + sh.dfx[lo + qq]]];
                 }
             } else if (active) {
                 for (int idx = lane; idx < m * m; idx += G) {
@@ -946,7 +1007,9 @@ __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, l
         sh.red = p; p += 8;
         int *ip = reinterpret_cast<int *>(p);
         sh.perm = ip; ip += n; sh.orig = ip; ip += n; sh.rank = ip; ip += n; sh.rpj = ip; ip += n;
-        sh.rnj = ip; ip += n; sh.ndx = ip; ip += n; sh.dfx = ip; ip += n; sh.cnt = ip; ip += 3 * n;
+        sh.rnj = ip; ip += n; sh.ndx = ip; ip += n; sh.dfx = ip; ip += n;
+        sh.pcol = ip; ip += n; sh.lstL = ip; ip += n; sh.lstR = ip; ip += n; sh.rflag = ip; ip += n;
+        sh.cnt = ip; ip += 5 * n;
     }
     for (long b = blockIdx.x; b < batch; b += gridDim.x) {
         const T *A = Abase + 2 * b * strideA;
@@ -1047,7 +1110,7 @@ __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, l
 // host launchers
 // ============================================================================
 size_t heevd_dc_lds_bytes(int n, int nb_wy, size_t real) {
-    size_t dc = (size_t)n * (12 * sizeof(double) + 10 * sizeof(int)) + 8 * sizeof(double);
+    size_t dc = (size_t)n * (12 * sizeof(double) + 16 * sizeof(int)) + 8 * sizeof(double);
     size_t wy = ((size_t)n * nb_wy + (size_t)nb_wy * (nb_wy + 1)) * 2 * real;
     return dc > wy ? dc : wy;
 }
@@ -1099,7 +1162,7 @@ int launch_zhetrd_blocked_t(T *A, long strideA, int lda, int n, double *D, doubl
 }
 
 static size_t lower_lds_bytes(int n, int NB, size_t real) {
-    constexpr int NT = 256;
+    constexpr int NT = 512;
     return ((size_t)n * NB + n + (NT / kWave) * n + (NT / (2 * NB)) * 2 * NB + 2 * NB) * 2 * real +
            (2 * (NT / kWave) + 8) * real;
 }
@@ -1108,14 +1171,14 @@ template <typename T, int NB>
 int launch_zhetrd_lower_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
                           int grid, hipStream_t stream) {
     if (n < 1 || n > 512) return -22;
-    constexpr int NT = 256;
+    constexpr int NT = 512;
     const size_t lds = lower_lds_bytes(n, NB, sizeof(T));
     if (lds > 160 * 1024) return -22;
     if (grid > batch) grid = batch;
     if (lds > 64 * 1024)
-        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)zhetrd_lower_kernel<T, NT, NB, 2>,
+        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)zhetrd_lower_kernel<T, NT, NB, 1>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((zhetrd_lower_kernel<T, NT, NB, 2>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D,
+    hipLaunchKernelGGL((zhetrd_lower_kernel<T, NT, NB, 1>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D,
                        E, TAU, strideV, batch);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
@@ -1140,14 +1203,14 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
     return 0;
 }
 
-// RLM_HETRD selects the tridiagonalisation kernel: "fused" (unblocked, full storage),
-// "blocked" (panels, full storage; default) or "lower" (panels, lower storage, tiled
+// RLM_HETRD selects the tridiagonalisation kernel: "fused" (unblocked, full storage;
+// default), "blocked" (panels, full storage) or "lower" (panels, lower storage, tiled
 // symv). Read at every launch so one process can compare them.
 static int hetrd_variant() {
     const char *e = getenv("RLM_HETRD");
-    if (e && strcmp(e, "fused") == 0) return 0;
+    if (e && strcmp(e, "blocked") == 0) return 1;
     if (e && strcmp(e, "lower") == 0) return 2;
-    return 1;
+    return 0;
 }
 
 int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,
