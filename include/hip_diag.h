@@ -53,7 +53,11 @@ int hip_heevd(double *A, double *w, int n);
  * its wrappers once per k-point, rlm/main_rlm_gpu.cu:198-229). `batch`
  * matrices at d_A + 2*b*strideA (lda >= n), uplo 'L' or 'U' names the triangle
  * read; eigenvectors overwrite the matrices, eigenvalues go to
- * d_W + b*strideW. stream: a hipStream_t or NULL for the default stream. */
+ * d_W + b*strideW. stream: a hipStream_t or NULL for the default stream.
+ * precision HIP_DIAG_FP64: FP64 throughout. HIP_DIAG_MIXED: FP32 Householder
+ * reduction and back-transform, FP64 divide & conquer, eigenvalues = FP64
+ * Rayleigh quotients of the FP32 eigenvectors against the FP64 matrix
+ * (eigenvectors carry FP32 accuracy, ~1e-6 relative residual). */
 int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA, double *d_W, long strideW, int batch,
                            char uplo, int precision, void *stream);
 

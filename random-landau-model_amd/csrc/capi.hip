@@ -61,6 +61,38 @@ __global__ void pack_kernel(const double *A, long lda, long strideA, int n, doub
     }
 }
 
+// FP64 -> FP32 copy of packed complex matrices (mixed-precision solver input), each
+// matrix scaled by a power of two to max |entry| in [1/2, 1) so that the FP32
+// reduction neither overflows nor underflows; eigenvectors do not depend on the
+// scale, and the eigenvalues are taken from the FP64 matrix afterwards.
+// grid.x = batch, 256 threads.
+__global__ __launch_bounds__(256) void to_f32_scaled_kernel(const double *P, long nn, float *Q) {
+    __shared__ double red[256 / kWave];
+    const double *M = P + 2 * blockIdx.x * nn;
+    float *R = Q + 2 * blockIdx.x * nn;
+    double mx = 0.0;
+    for (long i = threadIdx.x; i < 2 * nn; i += blockDim.x) mx = fmax(mx, fabs(M[i]));
+    for (int off = kWave / 2; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    for (int w = 0; w < (int)(blockDim.x / kWave); w++) mx = fmax(mx, red[w]);
+    int e = 0;
+    if (mx > 0.0) (void)frexp(mx, &e);
+    const double sc = ldexp(1.0, -e);
+    for (long i = threadIdx.x; i < 2 * nn; i += blockDim.x) R[i] = (float)(M[i] * sc);
+}
+
+// A(r, j) = U[r*n + j] from FP32 eigenvectors (component-major -> column-major FP64)
+__global__ void transpose_out_f32_kernel(const float *U, long strideU, int n, double *A, long lda, long strideA) {
+    const float *Ub = U + 2 * blockIdx.x * strideU;
+    double *M = A + 2 * blockIdx.x * strideA;
+    for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+        const int j = (int)(idx / n), r = (int)(idx % n);
+        const cx<float> u = ldx(Ub, (long)r * n + j);
+        st(M, (long)j * lda + r, cd{(double)u.x, (double)u.y});
+    }
+}
+
 // ---------------------------------------------------------------- device buffer
 struct DevBuf {
     void *p = nullptr;
@@ -147,7 +179,7 @@ int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork 
 
 std::mutex g_single_mu;
 EigWork g_single_ws;
-DevBuf g_single_pack;
+DevBuf g_single_pack, g_single_a32;
 
 }  // namespace
 
@@ -157,7 +189,7 @@ DevBuf g_single_pack;
 extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA, double *d_W, long strideW,
                                       int batch, char uplo, int precision, void *stream) {
     if (n < 1 || n > 512 || lda < n || batch < 0 || strideW < n || strideA < lda * (long)n) return -22;
-    if (precision != HIP_DIAG_FP64) return -22;
+    if (precision != HIP_DIAG_FP64 && precision != HIP_DIAG_MIXED) return -22;
     if (batch == 0) return 0;
     if (uplo != 'L' && uplo != 'U' && uplo != 'l' && uplo != 'u') return -22;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -174,8 +206,21 @@ extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA
     hipLaunchKernelGGL(hermitize_kernel, grid, dim3(256), 0, s, P, (long)n, nn, n,
                        (uplo == 'L' || uplo == 'l') ? 1 : 0);
     RLM_HIP_CHECK(hipGetLastError());
-    if ((rc = eig_packed(P, n, batch, Wp, g_single_ws.U.d(), g_single_ws, s))) return rc;
-    hipLaunchKernelGGL(transpose_out_kernel, grid, dim3(256), 0, s, g_single_ws.U.d(), nn, n, d_A, lda, strideA);
+    if (precision == HIP_DIAG_MIXED) {
+        // FP32 reduction and back-transform, FP64 divide & conquer, then FP64
+        // Rayleigh quotients of the FP32 vectors against the FP64 matrix
+        if ((rc = g_single_a32.ensure(sizeof(float) * 2 * (size_t)nn * batch))) return rc;
+        float *A32 = static_cast<float *>(g_single_a32.p);
+        float *U32 = reinterpret_cast<float *>(g_single_ws.U.d());   // holds 2*nn*batch doubles >= floats
+        hipLaunchKernelGGL(to_f32_scaled_kernel, grid, dim3(256), 0, s, P, nn, A32);
+        RLM_HIP_CHECK(hipGetLastError());
+        if ((rc = eig_packed_f32(A32, n, batch, Wp, U32, g_single_ws, s))) return rc;
+        if ((rc = launch_rayleigh(P, U32, n, n, batch, Wp, nullptr, s))) return rc;
+        hipLaunchKernelGGL(transpose_out_f32_kernel, grid, dim3(256), 0, s, U32, nn, n, d_A, lda, strideA);
+    } else {
+        if ((rc = eig_packed(P, n, batch, Wp, g_single_ws.U.d(), g_single_ws, s))) return rc;
+        hipLaunchKernelGGL(transpose_out_kernel, grid, dim3(256), 0, s, g_single_ws.U.d(), nn, n, d_A, lda, strideA);
+    }
     RLM_HIP_CHECK(hipGetLastError());
     RLM_HIP_CHECK(hipMemcpy2DAsync(d_W, sizeof(double) * strideW, Wp, sizeof(double) * n, sizeof(double) * n, batch,
                                    hipMemcpyDeviceToDevice, s));

@@ -616,6 +616,363 @@ __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase,
     }
 }
 
+// ----------------------------------------------------------------------------
+// Register-resident variant: the lower triangle of A is read into VGPRs once and
+// stays there for the whole reduction, so a matrix costs one read of its lower
+// triangle plus the writes of d, e, tau and the reflectors (the kernels above
+// stream the trailing block through memory once per column). One workgroup of
+// 512 threads (8 waves) per matrix. The lower triangle is cut into 32x32 tiles
+// dealt to the waves in column-major order (tile k -> wave k % 8, slot k / 8),
+// and lane (lrg, lcg) = (lane & 7, lane >> 3) of a wave holds the 4x4 sub-block
+// at tile rows 4*lrg.., columns 4*lcg.. of each of its tiles. Step i (reflector
+// v_i known):
+//   all waves: apply the pending rank-2 update of step i-1 to the trailing block,
+//              accumulate y = A22 v_i from the lower triangle (row sums and
+//              conjugate column sums, one LDS vector per wave) and copy column
+//              i+1 to LDS;
+//   wave 0:    w_i = tau_i y - 1/2 tau_i^2 (y^H v_i) v_i; column i+1 -= v_i w_i(0)^*
+//              + w_i v_i(0)^*; reflector of step i+1 from that column (zlarfg).
+// Two barriers per column. NS = tiles per wave held in registers; the nl =
+// ntiles - 8*NS tiles left over (0 <= nl <= NL) are the last nl tiles of tile
+// column 0, kept in LDS by waves (TN - nl .. TN - 1) mod 8 (they are live for the
+// first 32 columns only). n <= 256.
+
+// Cross-lane moves for the tile reductions: DPP within a 16-lane row, ds_swizzle
+// within 32 lanes (no LDS traffic), bpermute only across the two wave halves.
+template <int CTRL>
+__device__ inline float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int PAT>
+__device__ inline float swz_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), PAT));
+}
+template <int PAT>
+__device__ inline double swz_mov(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(u & 0xffffffffLL), PAT);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(u >> 32), PAT);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// sum over lane bits 0-2 (xor 1, xor 2 by quad_perm, then the half-row mirror,
+// which pairs quad 0 with quad 1 of each 8-lane group)
+template <typename T>
+__device__ inline T sum_lanes_012(T v) {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    return v;
+}
+// sum over lane bits 3-5 (row rotate by 8 = xor 8, swizzle xor 16, bpermute xor 32)
+template <typename T>
+__device__ inline T sum_lanes_345(T v) {
+    v += dpp_mov<0x128>(v);
+    v += swz_mov<0x401F>(v);
+    v += __shfl_xor(v, 32, kWave);
+    return v;
+}
+
+// One tile of the fused pass of step i (see above). diag (wave-uniform): the tile
+// straddles the diagonal, so entries above it stay zero and diagonal entries real.
+// Storage of one lane's 4x4 sub-block: registers, or LDS (entry k of lane l at
+// p[k * 64], p = tile base + l, conflict-free).
+template <typename T>
+struct RegSub {
+    cx<T> (&a)[16];
+    __device__ cx<T> get(int k) const { return a[k]; }
+    __device__ void set(int k, cx<T> x) { a[k] = x; }
+};
+template <typename T>
+struct LdsSub {
+    cx<T> *p;
+    __device__ cx<T> get(int k) const { return p[k * kWave]; }
+    __device__ void set(int k, cx<T> x) { p[k * kWave] = x; }
+};
+
+template <typename T, typename Sub>
+__device__ __forceinline__ void resident_tile(Sub a, bool diag, int rb, int cb, int i, int n, int lrg, int lcg,
+                                              const cx<T> *vn, const cx<T> *vo, const cx<T> *wo, cx<T> *yw,
+                                              bool own1, int c1u, cx<T> *colb) {
+    using CT = cx<T>;
+    const int i1 = i + 1;
+    const CT z{(T)0, (T)0};
+    const bool lowerSub = !diag || lrg > lcg, lowerEq = !diag || lrg >= lcg, diagSub = diag && lrg == lcg;
+    CT vc[4], poc[4], woc[4], cs[4];
+    bool cin[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int c = cb + u;
+        cin[u] = c >= i1 && c < n;
+        const int cc = cin[u] ? c : i1;                 // in-range LDS address either way
+        const CT x0 = vn[cc - i1], x1 = vo[cc - i], x2 = wo[cc - i];
+        vc[u] = cin[u] ? x0 : z;
+        poc[u] = cin[u] ? conjx(x1) : z;
+        woc[u] = cin[u] ? conjx(x2) : z;
+        cs[u] = z;
+    }
+#pragma unroll
+    for (int ru = 0; ru < 4; ru++) {
+        const int r = rb + ru;
+        const bool rin = r >= i1 && r < n;
+        const int rr = rin ? r : i1;
+        const CT y0 = vn[rr - i1], y1 = vo[rr - i], y2 = wo[rr - i];
+        const CT vr = rin ? y0 : z, por = rin ? y1 : z, wor = rin ? y2 : z;
+        CT rs = z, xc = z;
+#pragma unroll
+        for (int cu = 0; cu < 4; cu++) {
+            CT x = a.get(ru * 4 + cu);
+            const CT xn = x - (por * woc[cu] + wor * poc[cu]);
+            CT vre = vr;
+            x = ((ru >= cu) ? lowerEq : lowerSub) ? xn : x;
+            if (ru == cu) {
+                x.y = diagSub ? (T)0 : x.y;
+                vre = diagSub ? z : vr;                      // the diagonal enters y once
+            }
+            a.set(ru * 4 + cu, x);
+            xc = (cu == c1u) ? x : xc;                       // column i+1 (selected, never indexed)
+            fmax_(rs, x, vc[cu]);                            // y_r += A(r,c) v_c
+            fmacx(cs[cu], x, vre);                           // y_c += conj(A(r,c)) v_r
+        }
+        if (own1 && rin) colb[r] = xc;
+        rs.x = sum_lanes_345(rs.x);
+        rs.y = sum_lanes_345(rs.y);
+        if (lcg == 0 && rin) yw[r] = yw[r] + rs;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        cs[u].x = sum_lanes_012(cs[u].x);
+        cs[u].y = sum_lanes_012(cs[u].y);
+        if (lrg == 0 && cin[u]) yw[cb + u] = yw[cb + u] + cs[u];
+    }
+}
+
+template <typename T, int NS, int NL>
+__global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
+                                                             double *__restrict__ Dout, double *__restrict__ Eout,
+                                                             T *__restrict__ TAUout, long strideV, int batch) {
+    using CT = cx<T>;
+    constexpr int NT = 512, NW = NT / kWave, TS = 32, NMAX = 256, NQ = NMAX / kWave;
+    constexpr int NLS = NL > 0 ? NL : 1;
+    __shared__ CT vbuf[2][NMAX], wbuf[2][NMAX], colb[NMAX], yp[NW][NMAX];
+    __shared__ CT ltile[NLS][16 * kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
+    const int lrg = lane & 7, lcg = lane >> 3;
+    const int TN = (n + TS - 1) / TS;
+    const int ntiles = TN * (TN + 1) / 2;
+    const int nl = ntiles > NW * NS ? ntiles - NW * NS : 0;    // host guarantees nl <= min(NL, TN)
+    // register tiles: column-major order over the lower tile triangle without the LDS
+    // tiles (column 0, rows TN-nl..TN-1); list position j -> wave j % 8, slot j / 8
+    int tR[NS], tC[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int j = wid + NW * s;
+        tR[s] = -1;
+        tC[s] = -1;
+        if (j < ntiles - nl) {
+            int c0 = 0, len = TN - nl;                   // column 0 keeps TN - nl register tiles
+            while (j >= len) { j -= len; c0++; len = TN - c0; }
+            tC[s] = c0;
+            tR[s] = c0 + j;
+        }
+    }
+    // LDS tile of this wave (at most one): tile row lR of column 0, slot lq
+    int lR = -1, lq = 0;
+    for (int q = 0; q < nl; q++)
+        if ((TN - nl + q) % NW == wid) { lR = TN - nl + q; lq = q; }
+    CT *lsub = &ltile[lq][lane];
+
+    {
+        const long b = blockIdx.x;            // one workgroup per matrix (grid = batch)
+        if (b >= batch) return;
+        T *A = Abase + 2 * b * strideA;
+        double *Dv = Dout + b * strideV;
+        double *Ev = Eout + b * strideV;
+        T *Tv = TAUout + b * strideV * (long)(sizeof(double) / sizeof(T));
+        if (n == 1) {
+            if (t == 0) Dv[0] = (double)ldx(A, 0).x;
+            return;
+        }
+        CT a[NS][16];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int rb = tR[s] * TS + 4 * lrg, cb = tC[s] * TS + 4 * lcg;
+#pragma unroll
+            for (int cu = 0; cu < 4; cu++)
+#pragma unroll
+                for (int ru = 0; ru < 4; ru++) {
+                    const int r = rb + ru, c = cb + cu;
+                    a[s][ru * 4 + cu] = (tR[s] >= 0 && r < n && r >= c) ? ldx(A, (long)c * lda + r) : CT{(T)0, (T)0};
+                }
+        }
+        if (NL > 0 && lR >= 0) {
+            const int rb = lR * TS + 4 * lrg, cb = 4 * lcg;
+#pragma unroll
+            for (int cu = 0; cu < 4; cu++)
+#pragma unroll
+                for (int ru = 0; ru < 4; ru++) {
+                    const int r = rb + ru, c = cb + cu;
+                    lsub[(ru * 4 + cu) * kWave] = r < n ? ldx(A, (long)c * lda + r) : CT{(T)0, (T)0};
+                }
+        }
+        for (int e = t; e < NMAX; e += NT) {
+            vbuf[1][e] = CT{(T)0, (T)0};
+            wbuf[1][e] = CT{(T)0, (T)0};
+        }
+        // column 0 to LDS
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            if (tC[s] == 0 && lcg == 0) {
+#pragma unroll
+                for (int ru = 0; ru < 4; ru++) {
+                    const int r = tR[s] * TS + 4 * lrg + ru;
+                    if (r < n) colb[r] = a[s][ru * 4];
+                }
+            }
+        }
+        if (NL > 0 && lR >= 0 && lcg == 0) {
+#pragma unroll
+            for (int ru = 0; ru < 4; ru++) {
+                const int r = lR * TS + 4 * lrg + ru;
+                if (r < n) colb[r] = lsub[(ru * 4) * kWave];
+            }
+        }
+        __syncthreads();
+
+        // wave 0 only: reflector of step i from cq[q] = column i, row i + lane + 64 q
+        // (updated). Writes v_i (element rr <-> row i+1+rr) to vbuf[slot], beta and the
+        // scaled vector into A(i+1:n, i), d_i, e_i and tau_i.
+        auto reflector = [&](int i, const CT *cq, int slot, CT &tau) {
+            const T dii = __shfl(cq[0].x, 0, kWave);
+            const T ar = __shfl(cq[0].x, 1, kWave), ai = __shfl(cq[0].y, 1, kWave);
+            T s2 = 0;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int rr = lane + kWave * q;
+                if (rr >= 2 && i + rr < n) s2 += abs2x(cq[q]);
+            }
+            const T nrm2 = wave_sum_t(s2);
+            T beta, taur, taui, sr, si;
+            if (nrm2 == (T)0 && ai == (T)0) {
+                beta = ar; taur = 0; taui = 0; sr = 0; si = 0;
+            } else {
+                const T nrm = sqrt(ar * ar + ai * ai + nrm2);
+                beta = (ar >= (T)0) ? -nrm : nrm;
+                taur = (beta - ar) / beta;
+                taui = -ai / beta;
+                const T dr = ar - beta, di = ai;
+                const T inv = (T)1 / (dr * dr + di * di);
+                sr = dr * inv; si = -di * inv;
+            }
+            const CT scal{sr, si};
+            CT *vn = vbuf[slot];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int rr = lane + kWave * q, r = i + rr;
+                if (rr >= 1 && r < n) {
+                    const CT v = (rr == 1) ? CT{(T)1, (T)0} : cq[q] * scal;
+                    vn[rr - 1] = v;
+                    stx(A, (long)i * lda + r, (rr == 1) ? CT{beta, (T)0} : v);
+                }
+            }
+            if (lane == 0) {
+                Dv[i] = (double)dii;
+                Ev[i] = (double)beta;
+                Tv[2 * i] = taur;
+                Tv[2 * i + 1] = taui;
+            }
+            tau = CT{taur, taui};
+        };
+
+        CT tau{(T)0, (T)0};   // tau of the current step (wave 0)
+        if (wid == 0) {
+            CT cq[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int r = lane + kWave * q;
+                cq[q] = r < n ? colb[r] : CT{(T)0, (T)0};
+            }
+            reflector(0, cq, 0, tau);
+        }
+        __syncthreads();
+
+        for (int i = 0; i < n - 1; i++) {
+            const int cur = i & 1, i1 = i + 1;
+            const CT *vn = vbuf[cur];                            // v_i: rr <-> row i+1+rr
+            const CT *vo = vbuf[cur ^ 1], *wo = wbuf[cur ^ 1];   // v_{i-1}, w_{i-1}: rr <-> row i+rr
+            CT *yw = yp[wid];
+            for (int e = lane; e < NMAX; e += kWave) yw[e] = CT{(T)0, (T)0};
+            const int c1t = i1 / TS, c1g = (i1 % TS) / 4, c1u = i1 & 3;   // owner of column i+1
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                if (tR[s] < 0 || tC[s] * TS + TS - 1 < i1) continue;   // wave-uniform
+                int rb = tR[s] * TS + 4 * lrg, cb = tC[s] * TS + 4 * lcg;
+                asm volatile("" : "+v"(rb), "+v"(cb));       // keep per-slot indices out of loop-invariant hoisting
+                const bool own1 = tC[s] == c1t && lcg == c1g;
+                resident_tile<T>(RegSub<T>{a[s]}, tR[s] == tC[s], rb, cb, i, n, lrg, lcg, vn, vo, wo, yw, own1, c1u,
+                                 colb);
+            }
+            if (NL > 0 && lR >= 0 && i1 < TS) {
+                const int rb = lR * TS + 4 * lrg, cb = 4 * lcg;
+                resident_tile<T>(LdsSub<T>{lsub}, false, rb, cb, i, n, lrg, lcg, vn, vo, wo, yw,
+                                 c1t == 0 && lcg == c1g, c1u, colb);
+            }
+            __syncthreads();
+            if (wid == 0) {
+                const int m = n - i1;
+                CT p[NQ], cq[NQ];
+                CT dl{(T)0, (T)0};
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const int rr = lane + kWave * q;
+                    p[q] = CT{(T)0, (T)0};
+                    if (rr < m) {
+                        CT y{(T)0, (T)0};
+#pragma unroll
+                        for (int w = 0; w < NW; w++) y = y + yp[w][i1 + rr];
+                        p[q] = tau * y;
+                        fmacx(dl, p[q], vn[rr]);
+                    }
+                }
+                dl.x = wave_sum_t(dl.x);
+                dl.y = wave_sum_t(dl.y);
+                const CT alpha2 = (tau * dl) * (T)(-0.5);
+                CT *wn = wbuf[cur];
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const int rr = lane + kWave * q;
+                    if (rr < m) {
+                        p[q] = p[q] + alpha2 * vn[rr];                  // w_i
+                        wn[rr] = p[q];
+                    }
+                }
+                const CT w0{__shfl(p[0].x, 0, kWave), __shfl(p[0].y, 0, kWave)};
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const int rr = lane + kWave * q;
+                    cq[q] = CT{(T)0, (T)0};
+                    if (rr < m) {
+                        CT x = colb[i1 + rr];
+                        x = x - (vn[rr] * conjx(w0) + p[q]);            // v(0) = 1
+                        if (rr == 0) x.y = (T)0;
+                        cq[q] = x;
+                    }
+                }
+                if (i1 < n - 1) reflector(i1, cq, cur ^ 1, tau);
+                else if (lane == 0) Dv[n - 1] = (double)cq[0].x;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // ============================================================================
 // 2. Divide & conquer on the symmetric tridiagonal + 3. back-transform
 // ============================================================================
@@ -1203,34 +1560,68 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
     return 0;
 }
 
-// RLM_HETRD selects the tridiagonalisation kernel: "fused" (unblocked, full storage;
-// default), "blocked" (panels, full storage) or "lower" (panels, lower storage, tiled
-// symv). Read at every launch so one process can compare them.
+template <typename T, int NS, int NL>
+static int launch_zhetrd_resident_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
+                                    int batch, hipStream_t stream) {
+    const int tn = (n + 31) / 32, ntiles = tn * (tn + 1) / 2;
+    const int nl = ntiles > 8 * NS ? ntiles - 8 * NS : 0;
+    if (n < 1 || n > 256 || nl > NL || nl > tn) return -22;
+    hipLaunchKernelGGL((zhetrd_resident_kernel<T, NS, NL>), dim3(batch), dim3(512), 0, stream, A, strideA, lda, n, D,
+                       E, TAU, strideV, batch);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Largest n of the register-resident kernel: complex float keeps 4 tiles per wave
+// in registers (128 VGPRs) plus up to 4 tiles of tile column 0 in LDS (n <= 256);
+// complex double keeps 1 tile per wave in registers plus up to 4 in LDS (n <= 128).
+static int resident_max_n(size_t real) { return real == sizeof(float) ? 256 : 128; }
+
+// RLM_HETRD selects the tridiagonalisation kernel: "resident" (lower triangle in
+// registers), "fused" (unblocked, full storage), "blocked" (panels, full storage),
+// "lower" (panels, lower storage, tiled symv). Unset or "auto": resident where n
+// fits it, else lower. Read at every launch so one process can compare them.
+enum { kTrdAuto = 0, kTrdFused, kTrdBlocked, kTrdLower, kTrdResident };
 static int hetrd_variant() {
     const char *e = getenv("RLM_HETRD");
-    if (e && strcmp(e, "blocked") == 0) return 1;
-    if (e && strcmp(e, "lower") == 0) return 2;
-    return 0;
+    if (!e) return kTrdAuto;
+    if (strcmp(e, "fused") == 0) return kTrdFused;
+    if (strcmp(e, "blocked") == 0) return kTrdBlocked;
+    if (strcmp(e, "lower") == 0) return kTrdLower;
+    if (strcmp(e, "resident") == 0) return kTrdResident;
+    return kTrdAuto;
+}
+
+template <typename T>
+static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
+                            int grid, hipStream_t stream) {
+    int v = hetrd_variant();
+    if (v == kTrdAuto) v = n <= resident_max_n(sizeof(T)) ? kTrdResident : kTrdLower;
+    if (v == kTrdResident && n <= resident_max_n(sizeof(T))) {
+        const int tn = (n + 31) / 32, ntiles = tn * (tn + 1) / 2;
+        if constexpr (sizeof(T) == sizeof(float)) {
+            if (ntiles <= 10) return launch_zhetrd_resident_t<T, 1, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
+            if (ntiles <= 16) return launch_zhetrd_resident_t<T, 2, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
+            return launch_zhetrd_resident_t<T, 4, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
+        } else {
+            return launch_zhetrd_resident_t<T, 1, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
+        }
+    }
+    // the panel kernels keep W (n x 16) in LDS: beyond 160 KB fall back to the fused kernel
+    if ((v == kTrdLower || v == kTrdResident) && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024)
+        return launch_zhetrd_lower_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+    if (v == kTrdBlocked && blocked_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024)
+        return launch_zhetrd_blocked_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+    return launch_zhetrd_t<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
 }
 
 int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,
                   int grid, hipStream_t stream) {
-    // the panel kernels keep W (n x 16) in LDS: beyond 160 KB fall back to the fused kernel
-    const int v = hetrd_variant();
-    if (v == 2 && lower_lds_bytes(n, 16, sizeof(double)) <= 160 * 1024)
-        return launch_zhetrd_lower_t<double, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
-    if (v >= 1 && blocked_lds_bytes(n, 16, sizeof(double)) <= 160 * 1024)
-        return launch_zhetrd_blocked_t<double, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
-    return launch_zhetrd_t<double>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+    return launch_hetrd_any<double>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
 }
 int launch_chetrd(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV, int batch,
                   int grid, hipStream_t stream) {
-    const int v = hetrd_variant();
-    if (v == 2 && lower_lds_bytes(n, 16, sizeof(float)) <= 160 * 1024)
-        return launch_zhetrd_lower_t<float, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
-    if (v >= 1 && blocked_lds_bytes(n, 16, sizeof(float)) <= 160 * 1024)
-        return launch_zhetrd_blocked_t<float, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
-    return launch_zhetrd_t<float>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+    return launch_hetrd_any<float>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
 }
 int launch_tail(const double *A, long strideA, int lda, int n, const double *D, const double *E, const double *TAU,
                 long strideV, double *W, long strideW, double *U, long strideU, double *work, int batch, int grid,

@@ -39,6 +39,13 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise HipDiagError(f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR}`")
+    # torch ships its own HIP runtime under the same soname as /opt/rocm's: load torch
+    # first so the library binds to that copy (loading ours first leaves torch's HIP
+    # initialisation without devices when the two runtimes differ)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     dp = ctypes.c_void_p
     i, l, d, vp = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p

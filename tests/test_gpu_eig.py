@@ -43,10 +43,11 @@ def test_random_hermitian(n, b):
     _check(H, w.cpu().numpy(), U.cpu().numpy())
 
 
-@pytest.mark.parametrize("variant", ["fused", "blocked", "lower"])
-@pytest.mark.parametrize("n,b", [(5, 3), (17, 4), (100, 3), (256, 2)])
+@pytest.mark.parametrize("variant", ["fused", "blocked", "lower", "resident"])
+@pytest.mark.parametrize("n,b", [(5, 3), (17, 4), (100, 3), (128, 2), (256, 2)])
 def test_tridiagonalisation_variants(monkeypatch, variant, n, b):
-    """Both tridiagonalisation kernels (RLM_HETRD=fused | blocked) against LAPACK."""
+    """Every tridiagonalisation kernel (RLM_HETRD=fused | blocked | lower | resident) against
+    LAPACK; the register-resident FP64 kernel covers n <= 128 (beyond, the lower kernel runs)."""
     import torch
     from rlm_amd import heevd_batched
     print(f"[test_tridiagonalisation_variants] {variant} n={n}", flush=True)
@@ -126,3 +127,59 @@ def test_invalid_arguments():
     assert lib().hip_diag_heevd_batched(600, None, 600, 600 * 600, None, 600, 1, b"L", 0, None) == -22
     assert lib().hip_diag_heevd_batched(4, None, 4, 16, None, 4, 1, b"X", 0, None) == -22
     assert lib().hip_diag_heevd_batched(4, None, 4, 16, None, 4, 0, b"L", 0, None) == 0
+
+
+def _check_mixed(H, w, U, tol_vec=3e-5):
+    """Mixed precision: FP64 Rayleigh-quotient eigenvalues within 1e-9 ||A|| of LAPACK's,
+    FP32-accurate eigenvectors (residual and orthogonality within tol_vec * n/64)."""
+    b, n, _ = H.shape
+    for k in range(b):
+        A = H[k]
+        nrm = max(np.linalg.norm(A, 2), 1e-300)
+        wr = np.linalg.eigvalsh(A)
+        sc = max(1, n / 64)
+        assert np.abs(np.sort(w[k]) - wr).max() <= 1e-9 * nrm, (k, np.abs(np.sort(w[k]) - wr).max() / nrm)
+        R = A @ U[k] - U[k] * w[k][None, :]
+        assert np.abs(R).max() <= tol_vec * nrm * sc, (k, np.abs(R).max() / nrm)
+        O = U[k].conj().T @ U[k] - np.eye(n)
+        assert np.abs(O).max() <= tol_vec * sc, (k, np.abs(O).max())
+
+
+@pytest.mark.parametrize("variant", ["auto", "lower", "fused"])
+@pytest.mark.parametrize("n,b", [(1, 2), (2, 3), (3, 3), (31, 3), (33, 3), (64, 2), (96, 2), (97, 2), (128, 3),
+                                 (129, 2), (160, 2), (161, 2), (200, 2), (224, 2), (225, 2), (256, 4)])
+def test_mixed_precision(monkeypatch, variant, n, b):
+    """HIP_DIAG_MIXED: FP32 tridiagonalisation (register-resident kernel for n <= 256 under
+    "auto", with 1, 2 or 4 register tiles per wave and LDS tiles where 8 waves x slots fall
+    short) and back-transform, FP64 D&C and Rayleigh quotients."""
+    import torch
+    from rlm_amd import heevd_batched, HIP_DIAG_MIXED
+    print(f"[test_mixed_precision] {variant} n={n} b={b}", flush=True)
+    if variant == "auto":
+        monkeypatch.delenv("RLM_HETRD", raising=False)
+    else:
+        monkeypatch.setenv("RLM_HETRD", variant)
+    rng = np.random.default_rng(11 * n + b)
+    H = _rand_herm(rng, b, n)
+    w, U = heevd_batched(torch.from_numpy(H).cuda(), precision=HIP_DIAG_MIXED)
+    _check_mixed(H, w.cpu().numpy(), U.cpu().numpy())
+
+
+def test_mixed_precision_structured():
+    """Mixed precision on zero, diagonal, banded-sparse and widely scaled matrices."""
+    import torch
+    from rlm_amd import heevd_batched, HIP_DIAG_MIXED
+    print("[test_mixed_precision_structured]", flush=True)
+    n = 256
+    rng = np.random.default_rng(5)
+    mats = [np.zeros((n, n), dtype=np.complex128),
+            np.diag(np.linspace(-1, 1, n)).astype(np.complex128)]
+    B = _rand_herm(rng, 1, n)[0]
+    band = np.abs(np.subtract.outer(np.arange(n), np.arange(n)))
+    B[(band > 33) & (band < n - 33)] = 0            # the circular band of the rlm Hamiltonian
+    mats.append(B)
+    mats.append(1e-30 * _rand_herm(rng, 1, n)[0])
+    mats.append(1e30 * _rand_herm(rng, 1, n)[0])
+    H = np.stack([(M + M.conj().T) / 2 for M in mats])
+    w, U = heevd_batched(torch.from_numpy(H).cuda(), precision=HIP_DIAG_MIXED)
+    _check_mixed(H, w.cpu().numpy(), U.cpu().numpy())
