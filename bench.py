@@ -145,7 +145,7 @@ def main():
             return names[variant]
         return names["resident"] if n_phi <= (256 if path == "mixed" else 128) else names["lower"]
 
-    flops = {"tridiagonalisation": 16.0 / 3.0 * N ** 3, "dc+backtransform": 8.0 * N ** 3}
+    flops = {"tridiagonalisation": 16.0 / 3.0 * N ** 3, "backtransform": 8.0 * N ** 3}
 
     def run_path(precision, tag):
         print(f"[bench] {tag}: n_phi={n_phi} g={g} samples={S} seed={seed}: {nkp} twist points", file=sys.stderr,
@@ -201,7 +201,7 @@ def main():
     # dominant kernel of the reported path: algorithmic flops per launch / measured launch time
     dom = max(flops, key=lambda k: main["stage"][k])
     flops_unit = flops[dom]
-    kname = trd_name(path) if dom == "tridiagonalisation" else "tail_kernel"
+    kname = trd_name(path) if dom == "tridiagonalisation" else "backtransform_kernel"
     t_launch = main["stage"][dom] / 1e3
     achieved = flops_unit * nkp / t_launch / 1e12
     peak = FP32_PEAK_TFLOPS if (path == "mixed") else FP64_PEAK_TFLOPS

@@ -124,9 +124,9 @@ int rlm_plan_fetch(rlm_plan_t plan, double *energy, double *chern, double *margi
 int rlm_plan_run(rlm_plan_t plan, long first_sample, long n_samples, double *energy, double *chern, double *margin);
 /* GPU time of each stage of the last rlm_plan_execute, in ms, from hipEvents
  * recorded on the plan's stream: [0] V_mn + band tables, [1] assembly,
- * [2] tridiagonalisation, [3] divide & conquer + back-transform, [4] FP64
- * refinement (mixed plans; 0 otherwise), [5] link overlaps, [6] plaquettes +
- * energies. Returns 7 (the number of stages). */
+ * [2] tridiagonalisation, [3] divide & conquer, [4] back-transform, [5] FP64
+ * refinement (mixed plans; 0 otherwise), [6] link overlaps, [7] plaquettes +
+ * energies. Returns 8 (the number of stages). */
 int rlm_plan_stage_times(rlm_plan_t plan, double *ms);
 /* The plan's hipStream_t. */
 void *rlm_plan_stream(rlm_plan_t plan);
@@ -157,8 +157,9 @@ int hlm_plan_load(hlm_plan_t plan, long first_sample, int count);
 int hlm_plan_execute(hlm_plan_t plan);
 int hlm_plan_fetch(hlm_plan_t plan, double *energy, double *chern, double *margin);
 int hlm_plan_run(hlm_plan_t plan, long first_sample, long n_samples, double *energy, double *chern, double *margin);
-/* [0] projection Phi^H diag(V) Phi, [1] tridiagonalisation, [2] divide & conquer
- * + back-transform, [3] link overlaps, [4] plaquettes + energies, [5] unused. */
+/* [0] projection Phi^H diag(V) Phi, [1] tridiagonalisation, [2] divide & conquer,
+ * [3] back-transform, [4] link overlaps, [5] plaquettes + energies, [6] unused.
+ * Returns 7. */
 int hlm_plan_stage_times(hlm_plan_t plan, double *ms);
 void *hlm_plan_stream(hlm_plan_t plan);
 int hlm_plan_destroy(hlm_plan_t plan);

@@ -116,26 +116,29 @@ struct DevBuf {
 
 // Workspace of the batched eigensolver.
 struct EigWork {
-    DevBuf dev, U, work;
+    DevBuf dev, U, work, tw;
     int ensure(int n, long batch, bool with_u = true) {
         int rc;
         if ((rc = dev.ensure(sizeof(double) * 4 * (size_t)n * batch))) return rc;
         if (with_u && (rc = U.ensure(sizeof(double) * 2 * (size_t)n * n * batch))) return rc;
         const long g = batch < kGridTail ? batch : kGridTail;
         if ((rc = work.ensure(sizeof(double) * 3 * (size_t)n * n * g))) return rc;
+        // triangular factors of the matrix-core back-transform (complex double covers float)
+        if ((rc = tw.ensure(sizeof(double) * 2 * (size_t)backtransform_tws_complex(n) * batch))) return rc;
         return 0;
     }
     void release() {
         dev.release();
         U.release();
         work.release();
+        tw.release();
     }
 };
 
 // Eigen-decomposition of `batch` packed (lda = n, stride n*n) full Hermitian
 // matrices in A. Eigenvalues -> W (stride n), eigenvectors component-major -> Uout.
 int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &ws, hipStream_t s,
-               hipEvent_t mid = nullptr) {
+               hipEvent_t mid = nullptr, hipEvent_t mid2 = nullptr) {
     int rc;
     const long strideV = 4L * n;   // d | e | tau(re,im) per matrix, interleaved blocks
     if ((size_t)batch * strideV * sizeof(double) > ws.dev.bytes) return -22;
@@ -149,8 +152,9 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
     const int gl = (int)(batch < kGridTail ? batch : kGridTail);
     if ((size_t)gl * 3 * n * n * sizeof(double) > ws.work.bytes) return -22;
-    if ((rc = launch_tail(A, (long)n * n, n, n, D, E, TAU, strideV, W, n, Uout, (long)n * n, ws.work.d(), (int)batch,
-                          gl, s)))
+    if ((size_t)batch * backtransform_tws_complex(n) * 2 * sizeof(double) > ws.tw.bytes) return -22;
+    if ((rc = launch_tail(A, (long)n * n, n, n, D, E, TAU, strideV, W, n, Uout, (long)n * n, ws.work.d(), ws.tw.d(),
+                          (int)batch, gl, s, mid2)))
         return rc;
     return 0;
 }
@@ -159,7 +163,7 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
 // Same per-matrix layout of the d | e | tau slot as eig_packed (4n doubles); tau holds
 // 2(n-1) floats inside its 2n-double region.
 int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork &ws, hipStream_t s,
-                   hipEvent_t mid = nullptr) {
+                   hipEvent_t mid = nullptr, hipEvent_t mid2 = nullptr) {
     int rc;
     const long strideV = 4L * n;   // doubles per matrix, for D, E and TAU alike
     if ((size_t)batch * strideV * sizeof(double) > ws.dev.bytes) return -22;
@@ -171,8 +175,9 @@ int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork 
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
     const int gl = (int)(batch < kGridTail ? batch : kGridTail);
     if ((size_t)gl * 3 * n * n * sizeof(double) > ws.work.bytes) return -22;
+    if ((size_t)batch * backtransform_tws_complex(n) * 2 * sizeof(float) > ws.tw.bytes) return -22;
     if ((rc = launch_tail_f32(A, (long)n * n, n, n, D, E, TAU, strideV, W, n, Uout, (long)n * n, ws.work.d(),
-                              (int)batch, gl, s)))
+                              static_cast<float *>(ws.tw.p), (int)batch, gl, s, mid2)))
         return rc;
     return 0;
 }
@@ -377,7 +382,7 @@ struct rlm_plan_s {
     EigWork ws;
     int loaded = 0;
     long first = 0;
-    static constexpr int kStages = 7;
+    static constexpr int kStages = 8;
     hipEvent_t ev[kStages + 1] = {};
 };
 
@@ -474,25 +479,25 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     RLM_HIP_CHECK(hipEventRecord(p->ev[2], s));
     double *L1 = p->dL.d(), *L2 = L1 + 2 * nkp * N;
     if (!mixed) {
-        if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[3]))) return rc;
-        RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
+        if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[3], p->ev[4]))) return rc;
         RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
+        RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
         if ((rc = launch_links(p->dU.d(), (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
     } else {
         float *U32 = static_cast<float *>(p->dU.p);
-        if ((rc = eig_packed_f32(A32, N, nkp, p->dEig.d(), U32, p->ws, s, p->ev[3]))) return rc;
-        RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
+        if ((rc = eig_packed_f32(A32, N, nkp, p->dEig.d(), U32, p->ws, s, p->ev[3], p->ev[4]))) return rc;
+        RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
         // FP64 Rayleigh quotients against the FP64 matrix (band |c-r| <= off_head)
         if ((rc = launch_rayleigh(p->dA.d(), U32, N, off, nkp, p->dEig.d(), p->dResid.d(), s))) return rc;
-        RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
+        RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
         if ((rc = launch_links_f32(U32, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
     }
-    RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
+    RLM_HIP_CHECK(hipEventRecord(p->ev[7], s));
     double *out = p->dOut.d();
     if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, (int)g1, 0, (double)N, S, out, out + (long)S * N,
                             out + 2L * S * N, s)))
         return rc;
-    RLM_HIP_CHECK(hipEventRecord(p->ev[7], s));
+    RLM_HIP_CHECK(hipEventRecord(p->ev[8], s));
     return 0;
 }
 
@@ -608,7 +613,7 @@ struct hlm_plan_s {
     DevBuf dPhi, dM1, dM2, dV, dH, dU, dEig, dL, dOut;
     EigWork ws;
     int loaded = 0;
-    static constexpr int kStages = 6;
+    static constexpr int kStages = 7;
     hipEvent_t ev[kStages + 1] = {};
 };
 
@@ -708,17 +713,17 @@ extern "C" int hlm_plan_execute(hlm_plan_t p) {
         RLM_HIP_CHECK(hipGetLastError());
     }
     RLM_HIP_CHECK(hipEventRecord(p->ev[1], s));
-    if ((rc = eig_packed(p->dH.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[2]))) return rc;
-    RLM_HIP_CHECK(hipEventRecord(p->ev[3], s));
+    if ((rc = eig_packed(p->dH.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[2], p->ev[3]))) return rc;
+    RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
     double *L1 = p->dL.d(), *L2 = L1 + 2 * nkp * N;
     if ((rc = launch_hlm_links(p->dU.d(), N, g, nkp, p->dM1.d(), p->dM2.d(), L1, L2, s))) return rc;
-    RLM_HIP_CHECK(hipEventRecord(p->ev[4], s));
+    RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
     double *out = p->dOut.d();
     if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, g, 1, (double)N, S, out, out + (long)S * N, out + 2L * S * N,
                             s)))
         return rc;
-    RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
     RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
+    RLM_HIP_CHECK(hipEventRecord(p->ev[7], s));
     return 0;
 }
 

@@ -1014,8 +1014,9 @@ __device__ void secular_root(int j, int K, const double *dl, const double *w, do
         double delj = 0.0, delj1 = 0.0;
         for (int i = 0; i < K; i++) {
             const double del = (dl[i] - dlo) - tau;
-            const double tt = rho * w[i] * w[i] / del;
-            const double t2 = tt / del;
+            const double inv = 1.0 / del;                 // one division per term
+            const double tt = rho * w[i] * w[i] * inv;
+            const double t2 = tt * inv;
             if (i <= j) { psi += tt; dpsi += t2; } else { phi += tt; dphi += t2; }
             err += fabs(tt);
             if (i == j) delj = del;
@@ -1341,13 +1342,13 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
     return Qa;
 }
 
-template <typename T, int NT, int NBW>
-__global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, long strideA, int lda, int n,
-                                                  const double *__restrict__ Din, const double *__restrict__ Ein,
-                                                  const T *__restrict__ TAUin, long strideV,
-                                                  double *__restrict__ Wout, long strideW,
-                                                  T *__restrict__ Uout, long strideU,
-                                                  double *__restrict__ work, int nb_wy, int batch) {
+// Stage 1 kernel: divide & conquer per matrix; eigenvalues to W, eigenvectors of
+// the tridiagonal to U (component-major complex, zero imaginary part).
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restrict__ Din,
+                                                   const double *__restrict__ Ein, long strideV,
+                                                   double *__restrict__ Wout, long strideW, T *__restrict__ Uout,
+                                                   long strideU, double *__restrict__ work, int batch) {
     using C = cx<T>;
     extern __shared__ double smem[];
     const int t = threadIdx.x;
@@ -1369,20 +1370,34 @@ __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, l
         sh.cnt = ip; ip += 5 * n;
     }
     for (long b = blockIdx.x; b < batch; b += gridDim.x) {
-        const T *A = Abase + 2 * b * strideA;
         const double *dIn = Din + b * strideV;
         const double *eIn = Ein + b * strideV;
-        const T *tauv = TAUin + b * strideV * (long)(sizeof(double) / sizeof(T));
         double *W = Wout + b * strideW;
         T *U = Uout + 2 * b * strideU;
         double *dqf;
         double *Z = stedc_block<NT>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf);
         for (int k = t; k < n; k += NT) W[k] = dqf[k];
-        // U = Z (complex, row-major)
         for (long idx = t; idx < (long)n * n; idx += NT) stx(U, idx, C{(T)Z[idx], (T)0});
         __syncthreads();
+    }
+}
+
+// Stage 2 kernel: back-transform U = Q U with compact-WY blocks of nb_wy reflectors
+// (I - V T V^H), last block first, one thread per eigenvector column.
+template <typename T, int NT, int NBW>
+__global__ __launch_bounds__(NT) void backtransform_kernel(const T *__restrict__ Abase, long strideA, int lda, int n,
+                                                           const T *__restrict__ TAUin, long strideV,
+                                                           T *__restrict__ Uout, long strideU, int nb_wy,
+                                                           int batch) {
+    using C = cx<T>;
+    extern __shared__ double smem[];
+    const int t = threadIdx.x;
+    for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+        const T *A = Abase + 2 * b * strideA;
+        const T *tauv = TAUin + b * strideV * (long)(sizeof(double) / sizeof(T));
+        T *U = Uout + 2 * b * strideU;
         // back-transform with blocks of reflectors, last block first
-        C *Vl = reinterpret_cast<C *>(smem);          // rows x nb (reuses the D&C LDS)
+        C *Vl = reinterpret_cast<C *>(smem);          // rows x nb
         const int nref = n - 1;
         const int nblk = (nref + nb_wy - 1) / nb_wy;
         C *Tl = Vl + (long)n * nb_wy;                 // nb x nb
@@ -1463,13 +1478,260 @@ __global__ __launch_bounds__(NT) void tail_kernel(const T *__restrict__ Abase, l
     }
 }
 
+// ----------------------------------------------------------------------------
+// Back-transform on the matrix cores (n <= 256). Real 16x16x4 MFMAs, four per
+// complex product. Lane l of an MFMA holds A[l%16][l/16] and B[l/16][l%16]; the
+// 16x16 result D sits in 4 elements per lane, element v of lane l being
+// D[Mf<T>::row(l, v)][l%16]. Element v of an accumulator is therefore directly
+// the B operand of a K-step whose K index l/16 stands for row Mf<T>::row(l, v):
+// U, W = V^H U and W2 = T W never leave the registers between the products.
+template <typename T>
+struct Mf;
+template <>
+struct Mf<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int l, int v) { return (l >> 4) + 4 * v; }
+};
+template <>
+struct Mf<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int l, int v) { return 4 * (l >> 4) + v; }
+};
+
+// Element (row, k) of the reflector block starting at reflector j0 (zhetrd lower
+// storage: unit at row j+1, A(j+2:n, j) below, zero above; zero for j >= n-1).
+template <typename T>
+__device__ inline cx<T> refl_get(const T *A, int lda, int n, int j0, int row, int k) {
+    const int j = j0 + k;
+    if (j >= n - 1 || row <= j || row >= n) return cx<T>{(T)0, (T)0};
+    if (row == j + 1) return cx<T>{(T)1, (T)0};
+    return ldx(A, (long)j * lda + row);
+}
+
+// Triangular factors of the compact-WY blocks (zlarft, forward, columnwise):
+// block bk of matrix b covers reflectors 16bk..16bk+15; T[k][k] = tau_k,
+// T[0:k, k] = -tau_k T[0:k, 0:k] G[0:k, k] with G = V^H V (one MFMA product per
+// block). Stored column-major per block: Tout[b*strideT + (bk*256 + k2*16 + k1)]
+// (complex). One wave per block, 4 waves per workgroup, one workgroup per matrix.
+template <typename T>
+__global__ __launch_bounds__(256) void larft_kernel(const T *__restrict__ Abase, long strideA, int lda, int n,
+                                                    const T *__restrict__ TAUin, long strideV, T *__restrict__ Tout,
+                                                    long strideT, int batch) {
+    using M = Mf<T>;
+    using v4 = typename M::v4;
+    using CT = cx<T>;
+    __shared__ CT gs[4][16 * 17];
+    __shared__ CT ts[4][16 * 17];
+    const long b = blockIdx.x;
+    if (b >= batch) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const T *A = Abase + 2 * b * strideA;
+    const T *tau = TAUin + b * strideV * (long)(sizeof(double) / sizeof(T));
+    CT *Tb = reinterpret_cast<CT *>(Tout + 2 * b * strideT);
+    const int nref = n - 1, nblk = (nref + 15) / 16;
+    for (int bk = w; bk < nblk; bk += 4) {
+        const int j0 = bk * 16;
+        v4 gr = {0, 0, 0, 0}, gi = {0, 0, 0, 0};
+        // G[k1][k2] = sum_r conj(V[r][k1]) V[r][k2]; K-step over rows 4s.. (rows >= j0+1)
+        for (int r4 = j0 + 1 - ((j0 + 1) & 3); r4 < n; r4 += 4) {
+            const int r = r4 + lq;
+            const CT x = refl_get(A, lda, n, j0, r, li);   // A operand: conj(V)[k1=li][row]; B: V[row][k2=li]
+            gr = M::mma(x.x, x.x, gr);
+            gr = M::mma(x.y, x.y, gr);
+            gi = M::mma(x.x, x.y, gi);
+            gi = M::mma(-x.y, x.x, gi);
+        }
+        // G to LDS (D layout: element v of lane l is G[row(l, v)][l%16])
+#pragma unroll
+        for (int v = 0; v < 4; v++) gs[w][M::row(lane, v) * 17 + li] = CT{gr[v], gi[v]};
+        __builtin_amdgcn_wave_barrier();
+        // recurrence over columns k; lane i < 16 owns row i of T
+        for (int k = 0; k < 16; k++) {
+            const int j = j0 + k;
+            const CT tk = (j < nref) ? CT{tau[2 * j], tau[2 * j + 1]} : CT{(T)0, (T)0};
+            CT val{(T)0, (T)0};
+            if (lane < k) {
+                CT sacc{(T)0, (T)0};
+                for (int l2 = lane; l2 < k; l2++) fmax_(sacc, ts[w][lane * 17 + l2], gs[w][l2 * 17 + k]);
+                val = (tk * sacc) * (T)(-1);
+            } else if (lane == k) {
+                val = tk;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane < 16) ts[w][lane * 17 + k] = val;
+            __builtin_amdgcn_wave_barrier();
+        }
+        for (int e = lane; e < 256; e += kWave) {
+            const int k2 = e >> 4, k1 = e & 15;
+            Tb[bk * 256 + e] = ts[w][k1 * 17 + k2];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// U = Q U for a slab of 32 eigenvector columns per workgroup (4 waves; wave w owns
+// the 16-row blocks w, w+4, w+8, w+12). Per reflector block, last first:
+// W = V^H U (each wave over its rows, summed through LDS in a fixed order),
+// W2 = T W, U -= V W2. Workgroups are mapped so that the slabs of one matrix share
+// blockIdx % 8, i.e. one XCD and its L2, which then serves V and T to all of them.
+template <typename T>
+__global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__restrict__ Abase, long strideA, int lda,
+                                                                 int n, const T *__restrict__ Tin, long strideT,
+                                                                 T *__restrict__ Uout, long strideU, int nslab,
+                                                                 int batch) {
+    using M = Mf<T>;
+    using v4 = typename M::v4;
+    using CT = cx<T>;
+    constexpr int CW = 32, NCB = CW / 16, RBW = 4;
+    __shared__ CT vst[4][16 * 17];
+    __shared__ T wred[2][4][2 * NCB * 4 * kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const long q = blockIdx.x >> 3;
+    const long b = (q / nslab) * 8 + (blockIdx.x & 7);
+    const int slab = (int)(q % nslab);
+    if (b >= batch) return;
+    const T *A = Abase + 2 * b * strideA;
+    const CT *Tb = reinterpret_cast<const CT *>(Tin + 2 * b * strideT);
+    T *U = Uout + 2 * b * strideU;
+    const int c0 = slab * CW;
+    v4 ur[RBW][NCB], ui[RBW][NCB];
+#pragma unroll
+    for (int i = 0; i < RBW; i++) {
+        const int R = (w + 4 * i) * 16;
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const int col = c0 + cb * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = R + M::row(lane, v);
+                const CT x = (row < n && col < n) ? ldx(U, (long)row * n + col) : CT{(T)0, (T)0};
+                ur[i][cb][v] = x.x;
+                ui[i][cb][v] = x.y;
+            }
+        }
+    }
+    const int nref = n - 1, nblk = (nref + 15) / 16;
+    for (int bk = nblk - 1; bk >= 0; bk--) {
+        const int j0 = bk * 16, r0 = j0 + 1;
+        v4 wr[NCB], wi[NCB];
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            wr[cb] = v4{0, 0, 0, 0};
+            wi[cb] = v4{0, 0, 0, 0};
+        }
+        // W = V^H U over this wave's rows
+#pragma unroll
+        for (int i = 0; i < RBW; i++) {
+            const int R = (w + 4 * i) * 16;
+            if (R + 15 < r0) continue;                    // wave-uniform: V is zero above row r0
+            // stage V(R..R+15, 0..15) through LDS (coalesced read: lane li = row)
+#pragma unroll
+            for (int kk = lq; kk < 16; kk += 4) vst[w][kk * 17 + li] = refl_get(A, lda, n, j0, R + li, kk);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const CT x = vst[w][li * 17 + M::row(lane, v)];   // V[R + row(l, v)][k = li]
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) {
+                    wr[cb] = M::mma(x.x, ur[i][cb][v], wr[cb]);
+                    wr[cb] = M::mma(x.y, ui[i][cb][v], wr[cb]);
+                    wi[cb] = M::mma(x.x, ui[i][cb][v], wi[cb]);
+                    wi[cb] = M::mma(-x.y, ur[i][cb][v], wi[cb]);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        // sum W over the 4 waves (double-buffered by block parity: one barrier per block)
+        T *wb = wred[bk & 1][w];
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                wb[((0 * NCB + cb) * 4 + v) * kWave + lane] = wr[cb][v];
+                wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = wi[cb][v];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                T sr = 0, si = 0;
+#pragma unroll
+                for (int ww = 0; ww < 4; ww++) {
+                    sr += wred[bk & 1][ww][((0 * NCB + cb) * 4 + v) * kWave + lane];
+                    si += wred[bk & 1][ww][((1 * NCB + cb) * 4 + v) * kWave + lane];
+                }
+                wr[cb][v] = sr;
+                wi[cb][v] = si;
+            }
+        // W2 = T W (K index = reflector k2 = row(l, s))
+        v4 w2r[NCB], w2i[NCB];
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            w2r[cb] = v4{0, 0, 0, 0};
+            w2i[cb] = v4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const CT tv = Tb[bk * 256 + M::row(lane, s) * 16 + li];   // T[k1 = li][k2 = row(l, s)]
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) {
+                w2r[cb] = M::mma(tv.x, wr[cb][s], w2r[cb]);
+                w2r[cb] = M::mma(-tv.y, wi[cb][s], w2r[cb]);
+                w2i[cb] = M::mma(tv.x, wi[cb][s], w2i[cb]);
+                w2i[cb] = M::mma(tv.y, wr[cb][s], w2i[cb]);
+            }
+        }
+        // U -= V W2 on this wave's rows (K index = reflector row(l, s))
+#pragma unroll
+        for (int i = 0; i < RBW; i++) {
+            const int R = (w + 4 * i) * 16;
+            if (R + 15 < r0) continue;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const CT x = refl_get(A, lda, n, j0, R + li, M::row(lane, s));   // V[R + li][k]
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) {
+                    ur[i][cb] = M::mma(-x.x, w2r[cb][s], ur[i][cb]);
+                    ur[i][cb] = M::mma(x.y, w2i[cb][s], ur[i][cb]);
+                    ui[i][cb] = M::mma(-x.x, w2i[cb][s], ui[i][cb]);
+                    ui[i][cb] = M::mma(-x.y, w2r[cb][s], ui[i][cb]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RBW; i++) {
+        const int R = (w + 4 * i) * 16;
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const int col = c0 + cb * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = R + M::row(lane, v);
+                if (row < n && col < n) stx(U, (long)row * n + col, CT{ur[i][cb][v], ui[i][cb][v]});
+            }
+        }
+    }
+}
+
 // ============================================================================
 // host launchers
 // ============================================================================
-size_t heevd_dc_lds_bytes(int n, int nb_wy, size_t real) {
-    size_t dc = (size_t)n * (12 * sizeof(double) + 16 * sizeof(int)) + 8 * sizeof(double);
-    size_t wy = ((size_t)n * nb_wy + (size_t)nb_wy * (nb_wy + 1)) * 2 * real;
-    return dc > wy ? dc : wy;
+size_t heevd_dc_lds_bytes(int n) { return (size_t)n * (12 * sizeof(double) + 16 * sizeof(int)) + 8 * sizeof(double); }
+
+long backtransform_tws_complex(int n) { return (long)((n - 1 + 15) / 16) * 256; }
+
+static size_t backtransform_lds_bytes(int n, int nb_wy, size_t real) {
+    return ((size_t)n * nb_wy + (size_t)nb_wy * (nb_wy + 1)) * 2 * real;
 }
 
 int heevd_dc_workspace_doubles(int n, int grid_tail, long *per_matrix, long *per_wg) {
@@ -1543,19 +1805,43 @@ int launch_zhetrd_lower_t(T *A, long strideA, int lda, int n, double *D, double 
 
 template <typename T>
 int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, const double *E, const T *TAU,
-                  long strideV, double *W, long strideW, T *U, long strideU, double *work, int batch, int grid,
-                  hipStream_t stream) {
+                  long strideV, double *W, long strideW, T *U, long strideU, double *work, T *tws, int batch, int grid,
+                  hipStream_t stream, hipEvent_t mid) {
     if (n < 1 || n > 512) return -22;
     constexpr int NT = 256;
+    const size_t lds_dc = heevd_dc_lds_bytes(n);
+    if (lds_dc > 160 * 1024) return -22;
+    if (grid > batch) grid = batch;
+    if (lds_dc > 64 * 1024)
+        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)stedc_kernel<T, NT>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_dc));
+    hipLaunchKernelGGL((stedc_kernel<T, NT>), dim3(grid), dim3(NT), lds_dc, stream, n, D, E, strideV, W, strideW, U,
+                       strideU, work, batch);
+    RLM_HIP_CHECK(hipGetLastError());
+    if (mid) RLM_HIP_CHECK(hipEventRecord(mid, stream));
+    if (n == 1) return 0;
+    if (n <= 256 && tws) {
+        // matrix-core path: triangular factors, then slabs of 32 eigenvector columns
+        const long strideT = backtransform_tws_complex(n);
+        hipLaunchKernelGGL((larft_kernel<T>), dim3(batch), dim3(256), 0, stream, A, strideA, lda, n, TAU, strideV, tws,
+                           strideT, batch);
+        RLM_HIP_CHECK(hipGetLastError());
+        const int nslab = (n + 31) / 32;
+        const long groups = (batch + 7) / 8;
+        hipLaunchKernelGGL((backtransform_mfma_kernel<T>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0, stream,
+                           A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
+        RLM_HIP_CHECK(hipGetLastError());
+        return 0;
+    }
     // reflectors per compact-WY block: the V block (n x NBW complex) lives in LDS
     constexpr int NBW = sizeof(T) == 4 ? 32 : 16;
-    size_t lds = heevd_dc_lds_bytes(n, NBW, sizeof(T));
-    if (lds > 160 * 1024) return -22;
-    if (grid > batch) grid = batch;
-    if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)tail_kernel<T, NT, NBW>,
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((tail_kernel<T, NT, NBW>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU,
-                       strideV, W, strideW, U, strideU, work, NBW, batch);
+    const size_t lds_bt = backtransform_lds_bytes(n, NBW, sizeof(T));
+    if (lds_bt > 160 * 1024) return -22;
+    if (lds_bt > 64 * 1024)
+        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)backtransform_kernel<T, NT, NBW>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bt));
+    hipLaunchKernelGGL((backtransform_kernel<T, NT, NBW>), dim3(batch), dim3(NT), lds_bt, stream, A, strideA, lda, n,
+                       TAU, strideV, U, strideU, NBW, batch);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -1624,16 +1910,16 @@ int launch_chetrd(float *A, long strideA, int lda, int n, double *D, double *E, 
     return launch_hetrd_any<float>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
 }
 int launch_tail(const double *A, long strideA, int lda, int n, const double *D, const double *E, const double *TAU,
-                long strideV, double *W, long strideW, double *U, long strideU, double *work, int batch, int grid,
-                hipStream_t stream) {
-    return launch_tail_t<double>(A, strideA, lda, n, D, E, TAU, strideV, W, strideW, U, strideU, work, batch, grid,
-                                 stream);
+                long strideV, double *W, long strideW, double *U, long strideU, double *work, double *tws, int batch,
+                int grid, hipStream_t stream, hipEvent_t mid) {
+    return launch_tail_t<double>(A, strideA, lda, n, D, E, TAU, strideV, W, strideW, U, strideU, work, tws, batch,
+                                 grid, stream, mid);
 }
 int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *D, const double *E, const float *TAU,
-                    long strideV, double *W, long strideW, float *U, long strideU, double *work, int batch, int grid,
-                    hipStream_t stream) {
-    return launch_tail_t<float>(A, strideA, lda, n, D, E, TAU, strideV, W, strideW, U, strideU, work, batch, grid,
-                                stream);
+                    long strideV, double *W, long strideW, float *U, long strideU, double *work, float *tws, int batch,
+                    int grid, hipStream_t stream, hipEvent_t mid) {
+    return launch_tail_t<float>(A, strideA, lda, n, D, E, TAU, strideV, W, strideW, U, strideU, work, tws, batch,
+                                grid, stream, mid);
 }
 
 }  // namespace rlm

@@ -11,22 +11,26 @@ namespace rlm {
 int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,
                   int grid, hipStream_t stream);
 
-// Divide & conquer on (d, e) + back-transform with the reflectors in A.
+// Divide & conquer on (d, e) + back-transform with the reflectors in A (the
+// back-transform runs on the matrix cores for n <= 256 when tws is given; `mid`,
+// if given, is recorded between the divide & conquer and the back-transform).
 // W: eigenvalues ascending (strideW per matrix). U: eigenvectors, component-major
 // complex (U[r*n + j] = component r of state j), strideU complex elements per matrix.
-// work: grid * 3*n*n doubles.
+// work: grid * 3*n*n doubles. tws: batch * backtransform_tws_complex(n) complex
+// elements of the precision of A (or null).
 int launch_tail(const double *A, long strideA, int lda, int n, const double *D, const double *E, const double *TAU,
-                long strideV, double *W, long strideW, double *U, long strideU, double *work, int batch, int grid,
-                hipStream_t stream);
+                long strideV, double *W, long strideW, double *U, long strideU, double *work, double *tws, int batch,
+                int grid, hipStream_t stream, hipEvent_t mid = nullptr);
+long backtransform_tws_complex(int n);
 
 // FP32 variants (mixed precision): A, TAU, U single-precision complex; D, E, W and the
 // divide & conquer stay FP64.
 int launch_chetrd(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV, int batch,
                   int grid, hipStream_t stream);
 int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *D, const double *E, const float *TAU,
-                    long strideV, double *W, long strideW, float *U, long strideU, double *work, int batch, int grid,
-                    hipStream_t stream);
+                    long strideV, double *W, long strideW, float *U, long strideU, double *work, float *tws, int batch,
+                    int grid, hipStream_t stream, hipEvent_t mid = nullptr);
 
-size_t heevd_dc_lds_bytes(int n, int nb_wy, size_t real);
+size_t heevd_dc_lds_bytes(int n);
 
 }  // namespace rlm

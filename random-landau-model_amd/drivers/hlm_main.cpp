@@ -87,10 +87,10 @@ int main(int argc, char *argv[]) {
         if ((rc = hlm_plan_execute(plan))) break;
         if ((rc = hlm_plan_fetch(plan, E.data(), C.data(), nullptr))) break;
         if (s0 == 0) {
-            double ms[6];
+            double ms[7];
             hlm_plan_stage_times(plan, ms);
-            const double diag = (ms[0] + ms[1] + ms[2]) / 1e3 / (cnt * nkp);
-            const double chern = (ms[3] + ms[4]) / 1e3 / (cnt * nkp);
+            const double diag = (ms[0] + ms[1] + ms[2] + ms[3]) / 1e3 / (cnt * nkp);
+            const double chern = (ms[4] + ms[5]) / 1e3 / (cnt * nkp);
             const double total = (chern * n_mesh * (n_mesh + 1) + diag * nkp) * n_sample;
             cerr << std::left << setw(40) << "diagonalization time/k-point: " << diag << " s\n";
             cerr << setw(40) << "Chern No. calculation time/k-point: " << chern << " s\n";

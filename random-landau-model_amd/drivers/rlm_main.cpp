@@ -99,11 +99,11 @@ int main(int argc, char *argv[]) {
         if ((rc = rlm_plan_execute(plan))) break;
         if ((rc = rlm_plan_fetch(plan, E.data(), C.data(), nullptr))) break;
         if (s0 == 0) {
-            double ms[7];
+            double ms[8];
             rlm_plan_stage_times(plan, ms);
             const double pot = (load_time + ms[0] / 1e3) / cnt;
-            const double diag = (ms[1] + ms[2] + ms[3] + ms[4]) / 1e3 / (cnt * nkp);
-            const double chern = (ms[5] + ms[6]) / 1e3 / (cnt * nkp);
+            const double diag = (ms[1] + ms[2] + ms[3] + ms[4] + ms[5]) / 1e3 / (cnt * nkp);
+            const double chern = (ms[6] + ms[7]) / 1e3 / (cnt * nkp);
             const double total = coeff_time + (pot + chern * n_mesh * (n_mesh + 1) + diag * nkp) * n_sample;
             cerr << setw(40) << "potential initialization time: " << pot << " s\n";
             cerr << setw(40) << "diagonalization time/k-point: " << diag << " s\n";

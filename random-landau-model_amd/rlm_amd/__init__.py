@@ -186,13 +186,13 @@ class RlmPlan:
         shp = (n_samples, self.n_phi)
         return e.reshape(shp), c.reshape(shp), m.reshape(shp)
 
-    STAGES = ("potential+tables", "assembly", "tridiagonalisation", "dc+backtransform", "refinement", "links",
-              "plaquettes")
+    STAGES = ("potential+tables", "assembly", "tridiagonalisation", "divide_conquer", "backtransform", "refinement",
+              "links", "plaquettes")
 
     def stage_times(self) -> dict:
-        ms = np.zeros(7)
+        ms = np.zeros(len(self.STAGES))
         rc = lib().rlm_plan_stage_times(self._h, _ptr(ms))
-        if rc != 7:
+        if rc != len(self.STAGES):
             raise HipDiagError(f"rlm_plan_stage_times failed with status {rc}")
         return dict(zip(self.STAGES, ms.tolist()))
 
@@ -232,7 +232,7 @@ def hlm_disorder(n_sites: int, n_samples: int, seed: int, first_sample: int = 0,
 class HlmPlan:
     """hlm_plan_* (batched hlm pipeline)."""
 
-    STAGES = ("projection", "tridiagonalisation", "dc+backtransform", "links", "plaquettes", "unused")
+    STAGES = ("projection", "tridiagonalisation", "divide_conquer", "backtransform", "links", "plaquettes", "unused")
 
     def __init__(self, lx, ly, Q, band=0, n_mesh=32, seed=1, W=2.0, samples_per_batch=1, precision=HIP_DIAG_FP64):
         self._h = ctypes.c_void_p()
@@ -263,9 +263,9 @@ class HlmPlan:
         return e.reshape(shp), c.reshape(shp), m.reshape(shp)
 
     def stage_times(self) -> dict:
-        ms = np.zeros(6)
+        ms = np.zeros(len(self.STAGES))
         rc = lib().hlm_plan_stage_times(self._h, _ptr(ms))
-        if rc != 6:
+        if rc != len(self.STAGES):
             raise HipDiagError(f"hlm_plan_stage_times failed with status {rc}")
         return dict(zip(self.STAGES, ms.tolist()))
 

@@ -92,6 +92,8 @@ def test_full_size_properties():
     print("[test_full_size_properties]", flush=True)
     plan = RlmPlan(256, 1.0, 16.0, 32, 7, samples_per_batch=4)
     e, c, m = plan.run(4, 4)
+    st = plan.stage_times()
+    assert list(st) == list(RlmPlan.STAGES) and all(v >= 0 for v in st.values()), st
     assert np.all(np.abs(c - np.rint(c)) < 1e-6)
     assert np.all(np.rint(c).sum(axis=1) == 1)
     assert np.all(m > 0)
