@@ -973,6 +973,30 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
     }
 }
 
+// ----------------------------------------------------------------------------
+// Back-transform on the matrix cores (n <= 256). Real 16x16x4 MFMAs, four per
+// complex product. Lane l of an MFMA holds A[l%16][l/16] and B[l/16][l%16]; the
+// 16x16 result D sits in 4 elements per lane, element v of lane l being
+// D[Mf<T>::row(l, v)][l%16]. Element v of an accumulator is therefore directly
+// the B operand of a K-step whose K index l/16 stands for row Mf<T>::row(l, v):
+// U, W = V^H U and W2 = T W never leave the registers between the products.
+template <typename T>
+struct Mf;
+template <>
+struct Mf<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int l, int v) { return (l >> 4) + 4 * v; }
+};
+template <>
+struct Mf<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int l, int v) { return 4 * (l >> 4) + v; }
+};
+
 // ============================================================================
 // 2. Divide & conquer on the symmetric tridiagonal + 3. back-transform
 // ============================================================================
@@ -1091,8 +1115,10 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
         if (k < n - 1) v -= fabs(eIn[k] * iscale);
         sh.dq[k] = v;
     }
+    // Qa, Qb: eigenvector matrices of the current level, column-major (column c =
+    // eigenvector c, Q[c*n + r])
     for (long idx = t; idx < (long)n * n; idx += NT) {
-        const long r = idx / n, c = idx % n;
+        const long c = idx / n, r = idx % n;
         Qa[idx] = (r == c) ? 1.0 : 0.0;
         Qb[idx] = 0.0;
     }
@@ -1124,12 +1150,12 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                     if (i < n1) {
                         val = dq[lo + i];
                         pos = i + count_below(dq + mid, hi - mid, val, false);
-                        zi = Qa[(long)(mid - 1) * n + lo + i];
+                        zi = Qa[(long)(lo + i) * n + mid - 1];
                     } else {
                         const int j = i - n1;
                         val = dq[mid + j];
                         pos = j + count_below(dq + lo, n1, val, true);
-                        zi = sgn * Qa[(long)mid * n + mid + j];
+                        zi = sgn * Qa[(long)(mid + j) * n + mid];
                     }
                     sh.ds[lo + pos] = val;
                     sh.zs[lo + pos] = zi * isq2;
@@ -1207,13 +1233,13 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
             if (merging) {
                 if (nr > 0) {
                     for (int r = lo + lane; r < hi; r += G) {
-                        double *row = Qa + (long)r * n + lo;
                         for (int q = 0; q < nr; q++) {
                             const int cp = sh.perm[lo + sh.rpj[lo + q]], cn = sh.perm[lo + sh.rnj[lo + q]];
                             const double c = sh.rc[lo + q], s = sh.rs[lo + q];
-                            const double x = row[cp], y = row[cn];
-                            row[cp] = c * x + s * y;
-                            row[cn] = c * y - s * x;
+                            double *xp = Qa + (long)(lo + cp) * n + r, *yp = Qa + (long)(lo + cn) * n + r;
+                            const double x = *xp, y = *yp;
+                            *xp = c * x + s * y;
+                            *yp = c * y - s * x;
                         }
                     }
                 }
@@ -1278,8 +1304,12 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                 }
             }
             __syncthreads();
+            // Matrix-core path for halves of >= 32 rows (whole waves per node): Qb = Qa S
+            // with S[i][q] = what_i / (dl_i - lambda_q) * inrm_q generated per K-step,
+            // S never stored. Below that, S goes through memory and the VALU path.
+            const bool mfma = merging && G >= kWave && half >= 32;
             // P7: U (K x K) into S rows lo.., scaled by the inverse norms
-            if (merging) {
+            if (merging && !mfma) {
                 const double *dl = sh.dl + lo;
                 for (int idx = lane; idx < K * K; idx += G) {
                     const int i = idx / K, q = idx % K;
@@ -1291,13 +1321,67 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
             // P8: Qb block = Qa block (rotated, gathered) * U; deflated columns copied.
             // Rows of the left half only meet columns of lstL (left child or rotated), rows
             // of the right half only lstR (the block-diagonal structure dlaed3 exploits).
-            // Each task is one row and 4 consecutive output columns.
-            if (merging) {
+            if (mfma) {
+                using v4 = Mf<double>::v4;
+                const int wv = lane / kWave, nwv = G / kWave;
+                const int l = lane & (kWave - 1), li = l & 15, lq = l >> 4;
+                const double *dl = sh.dl + lo;
+                const int nqt = (K + 15) / 16;
+                const int nrb = half >= 64 ? 4 : 2;               // 16-row blocks per task
+                const int rows1 = hi - mid;                       // right half (may be short)
+                const int ng0 = (n1 + nrb * 16 - 1) / (nrb * 16), ng1 = (rows1 + nrb * 16 - 1) / (nrb * 16);
+                const int ntask = (ng0 + ng1) * nqt;
+                for (int task = wv; task < ntask; task += nwv) {
+                    const int qt = task % nqt, g = task / nqt;
+                    const bool right = g >= ng0;
+                    const int rbase = right ? mid + (g - ng0) * nrb * 16 : lo + g * nrb * 16;
+                    const int rend = right ? hi : mid;
+                    const int cnt = right ? KR : KL;
+                    const int *lst = (right ? sh.lstR : sh.lstL) + lo;
+                    const int q = qt * 16 + li;
+                    const bool qin = q < K;
+                    const double lq_o = qin ? dl[sh.orig[lo + q]] : 0.0;
+                    const double tq = qin ? sh.tau[lo + q] : 0.0;
+                    const double iq = qin ? sh.inrm[lo + q] : 0.0;
+                    v4 acc[4];
+#pragma unroll
+                    for (int rb = 0; rb < 4; rb++) acc[rb] = v4{0.0, 0.0, 0.0, 0.0};
+                    for (int idx0 = 0; idx0 < cnt; idx0 += 4) {
+                        const int idx = idx0 + lq;
+                        const bool kin = idx < cnt;
+                        const int i = kin ? lst[idx] : 0;
+                        const double bval = (kin && qin) ? sh.what[lo + i] / ((dl[i] - lq_o) - tq) * iq : 0.0;
+                        const double *col = Qa + (long)(lo + sh.pcol[lo + i]) * n;
+#pragma unroll
+                        for (int rb = 0; rb < 4; rb++) {
+                            if (rb < nrb) {
+                                const int r = rbase + rb * 16 + li;
+                                const double av = (kin && r < rend) ? col[r] : 0.0;
+                                acc[rb] = Mf<double>::mma(av, bval, acc[rb]);
+                            }
+                        }
+                    }
+                    if (qin) {
+                        double *ocol = Qb + (long)(lo + sh.rank[lo + q]) * n;
+#pragma unroll
+                        for (int rb = 0; rb < 4; rb++) {
+                            if (rb < nrb) {
+#pragma unroll
+                                for (int v = 0; v < 4; v++) {
+                                    const int r = rbase + rb * 16 + Mf<double>::row(l, v);
+                                    if (r < rend) ocol[r] = acc[rb][v];
+                                }
+                            }
+                        }
+                    }
+                }
+            } else if (merging) {
+                // VALU path: each task is one row and 4 consecutive output columns,
+                // consecutive lanes on consecutive rows
                 const int nq4 = (K + 3) >> 2;
                 for (int task = lane; task < m * nq4; task += G) {
-                    const int rloc = task / nq4, q0 = (task - rloc * nq4) * 4;
+                    const int rloc = task % m, q0 = (task / m) * 4;
                     const int r = lo + rloc;
-                    const double *row = Qa + (long)r * n + lo;
                     const bool left = rloc < n1;
                     const int *lst = left ? sh.lstL + lo : sh.lstR + lo;
                     const int cntl = left ? KL : KR;
@@ -1306,28 +1390,29 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
 #pragma unroll 4
                     for (int idx = 0; idx < cntl; idx++) {
                         const int i = lst[idx];
-                        const double x = row[sh.pcol[lo + i]];
+                        const double x = Qa[(long)(lo + sh.pcol[lo + i]) * n + r];
                         const double *srow = S + (long)(lo + i) * n;
                         a0 = fma(x, srow[qa], a0);
                         a1 = fma(x, srow[qb], a1);
                         a2 = fma(x, srow[qc], a2);
                         a3 = fma(x, srow[qd], a3);
                     }
-                    double *orow = Qb + (long)r * n + lo;
-                    orow[sh.rank[lo + q0]] = a0;
-                    if (q0 + 1 < K) orow[sh.rank[lo + q0 + 1]] = a1;
-                    if (q0 + 2 < K) orow[sh.rank[lo + q0 + 2]] = a2;
-                    if (q0 + 3 < K) orow[sh.rank[lo + q0 + 3]] = a3;
+                    Qb[(long)(lo + sh.rank[lo + q0]) * n + r] = a0;
+                    if (q0 + 1 < K) Qb[(long)(lo + sh.rank[lo + q0 + 1]) * n + r] = a1;
+                    if (q0 + 2 < K) Qb[(long)(lo + sh.rank[lo + q0 + 2]) * n + r] = a2;
+                    if (q0 + 3 < K) Qb[(long)(lo + sh.rank[lo + q0 + 3]) * n + r] = a3;
                 }
+            }
+            if (merging) {
                 for (int idx = lane; idx < m * nd; idx += G) {
-                    const int rloc = idx / nd, qq = idx - rloc * nd;
+                    const int qq = idx / m, rloc = idx - qq * m;
                     const int r = lo + rloc;
-                    Qb[(long)r * n + lo + sh.rank[lo + K + qq]] = Qa[(long)r * n + lo + sh.perm[lo + sh.dfx[lo + qq]]];
+                    Qb[(long)(lo + sh.rank[lo + K + qq]) * n + r] = Qa[(long)(lo + sh.perm[lo + sh.dfx[lo + qq]]) * n + r];
                 }
             } else if (active) {
                 for (int idx = lane; idx < m * m; idx += G) {
-                    const int r = lo + idx / m, c = lo + idx % m;
-                    Qb[(long)r * n + c] = Qa[(long)r * n + c];
+                    const int c = lo + idx / m, r = lo + idx % m;
+                    Qb[(long)c * n + r] = Qa[(long)c * n + r];
                 }
                 for (int q = lane; q < m; q += G) dqn[lo + q] = dq[lo + q];
             }
@@ -1377,8 +1462,23 @@ __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restri
         double *dqf;
         double *Z = stedc_block<NT>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf);
         for (int k = t; k < n; k += NT) W[k] = dqf[k];
-        for (long idx = t; idx < (long)n * n; idx += NT) stx(U, idx, C{(T)Z[idx], (T)0});
         __syncthreads();
+        // U[r*n + j] = Z[j*n + r] (column-major Z -> component-major U) through 32x32
+        // LDS tiles (the D&C arrays are free now; the launch sizes LDS for the tile)
+        double *tile = smem;                       // 32 x 33
+        for (int tr = 0; tr < n; tr += 32)
+            for (int tc = 0; tc < n; tc += 32) {
+                for (int e = t; e < 1024; e += NT) {
+                    const int jj = e >> 5, rr = e & 31;            // read column tc+jj, row tr+rr
+                    if (tc + jj < n && tr + rr < n) tile[rr * 33 + jj] = Z[(long)(tc + jj) * n + tr + rr];
+                }
+                __syncthreads();
+                for (int e = t; e < 1024; e += NT) {
+                    const int rr = e >> 5, jj = e & 31;            // write row tr+rr, column tc+jj
+                    if (tc + jj < n && tr + rr < n) stx(U, (long)(tr + rr) * n + tc + jj, C{(T)tile[rr * 33 + jj], (T)0});
+                }
+                __syncthreads();
+            }
     }
 }
 
@@ -1477,30 +1577,6 @@ __global__ __launch_bounds__(NT) void backtransform_kernel(const T *__restrict__
         }
     }
 }
-
-// ----------------------------------------------------------------------------
-// Back-transform on the matrix cores (n <= 256). Real 16x16x4 MFMAs, four per
-// complex product. Lane l of an MFMA holds A[l%16][l/16] and B[l/16][l%16]; the
-// 16x16 result D sits in 4 elements per lane, element v of lane l being
-// D[Mf<T>::row(l, v)][l%16]. Element v of an accumulator is therefore directly
-// the B operand of a K-step whose K index l/16 stands for row Mf<T>::row(l, v):
-// U, W = V^H U and W2 = T W never leave the registers between the products.
-template <typename T>
-struct Mf;
-template <>
-struct Mf<double> {
-    typedef double v4 __attribute__((ext_vector_type(4)));
-    static __device__ inline v4 mma(double a, double b, v4 c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int l, int v) { return (l >> 4) + 4 * v; }
-};
-template <>
-struct Mf<float> {
-    typedef float v4 __attribute__((ext_vector_type(4)));
-    static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    static __device__ inline int row(int l, int v) { return 4 * (l >> 4) + v; }
-};
 
 // Element (row, k) of the reflector block starting at reflector j0 (zhetrd lower
 // storage: unit at row j+1, A(j+2:n, j) below, zero above; zero for j >= n-1).
@@ -1726,7 +1802,12 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
 // ============================================================================
 // host launchers
 // ============================================================================
-size_t heevd_dc_lds_bytes(int n) { return (size_t)n * (12 * sizeof(double) + 16 * sizeof(int)) + 8 * sizeof(double); }
+// divide & conquer arrays, and at least the 32 x 33 transpose tile of the output copy
+size_t heevd_dc_lds_bytes(int n) {
+    const size_t dc = (size_t)n * (12 * sizeof(double) + 16 * sizeof(int)) + 8 * sizeof(double);
+    const size_t tile = 32 * 33 * sizeof(double);
+    return dc > tile ? dc : tile;
+}
 
 long backtransform_tws_complex(int n) { return (long)((n - 1 + 15) / 16) * 256; }
 
