@@ -422,7 +422,7 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
         (rc = p->dA.ensure(16 * nkp * N * N)) ||
         (rc = p->dU.ensure((precision == HIP_DIAG_MIXED ? 8 : 16) * nkp * N * N)) ||
         (precision == HIP_DIAG_MIXED && (rc = p->dA32.ensure(8 * nkp * N * N))) ||
-        (rc = p->dResid.ensure(8 * nkp)) ||
+        (rc = p->dResid.ensure(8 * nkp * rayleigh_resid_slots((int)N))) ||
         (rc = p->dEig.ensure(8 * nkp * N)) || (rc = p->dL.ensure(32 * nkp * N)) ||
         (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)))
         return fail(rc);
@@ -559,10 +559,10 @@ extern "C" int rlm_plan_max_residual(rlm_plan_t p, double *out) {
         *out = 0.0;
         return 0;
     }
-    const long nkp = (long)p->loaded * (p->g + 1) * (p->g + 1);
-    std::vector<double> h(nkp);
+    const long nres = (long)p->loaded * (p->g + 1) * (p->g + 1) * rayleigh_resid_slots((int)p->n_phi);
+    std::vector<double> h(nres);
     RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
-    RLM_HIP_CHECK(hipMemcpy(h.data(), p->dResid.p, 8 * nkp, hipMemcpyDeviceToHost));
+    RLM_HIP_CHECK(hipMemcpy(h.data(), p->dResid.p, 8 * nres, hipMemcpyDeviceToHost));
     double m = 0.0;
     for (double v : h) m = v > m ? v : m;
     *out = m;

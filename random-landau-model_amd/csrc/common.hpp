@@ -191,4 +191,27 @@ __device__ inline cx<T> block_sum_cx(cx<T> v, T *red) {
     return t;
 }
 
+// ---------------------------------------------------------------- matrix cores
+// Real 16x16x4 MFMAs (four per complex product). Lane l of an MFMA holds
+// A[l%16][l/16] and B[l/16][l%16]; the 16x16 result D sits in 4 elements per lane, element v of lane l being
+// D[Mf<T>::row(l, v)][l%16]. Element v of an accumulator is therefore directly
+// the B operand of a K-step whose K index l/16 stands for row Mf<T>::row(l, v),
+// so chained products need no data movement between them.
+template <typename T>
+struct Mf;
+template <>
+struct Mf<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int l, int v) { return (l >> 4) + 4 * v; }
+};
+template <>
+struct Mf<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int l, int v) { return 4 * (l >> 4) + v; }
+};
+
 }  // namespace rlm

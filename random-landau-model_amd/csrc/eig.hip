@@ -973,30 +973,6 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
     }
 }
 
-// ----------------------------------------------------------------------------
-// Back-transform on the matrix cores (n <= 256). Real 16x16x4 MFMAs, four per
-// complex product. Lane l of an MFMA holds A[l%16][l/16] and B[l/16][l%16]; the
-// 16x16 result D sits in 4 elements per lane, element v of lane l being
-// D[Mf<T>::row(l, v)][l%16]. Element v of an accumulator is therefore directly
-// the B operand of a K-step whose K index l/16 stands for row Mf<T>::row(l, v):
-// U, W = V^H U and W2 = T W never leave the registers between the products.
-template <typename T>
-struct Mf;
-template <>
-struct Mf<double> {
-    typedef double v4 __attribute__((ext_vector_type(4)));
-    static __device__ inline v4 mma(double a, double b, v4 c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int l, int v) { return (l >> 4) + 4 * v; }
-};
-template <>
-struct Mf<float> {
-    typedef float v4 __attribute__((ext_vector_type(4)));
-    static __device__ inline v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    static __device__ inline int row(int l, int v) { return 4 * (l >> 4) + v; }
-};
-
 // ============================================================================
 // 2. Divide & conquer on the symmetric tridiagonal + 3. back-transform
 // ============================================================================
@@ -1578,6 +1554,9 @@ __global__ __launch_bounds__(NT) void backtransform_kernel(const T *__restrict__
     }
 }
 
+// ----------------------------------------------------------------------------
+// Back-transform on the matrix cores (n <= 256), with the MFMA helpers Mf<T> of
+// common.hpp.
 // Element (row, k) of the reflector block starting at reflector j0 (zhetrd lower
 // storage: unit at row j+1, A(j+2:n, j) below, zero above; zero for j >= n-1).
 template <typename T>

@@ -157,55 +157,89 @@ int launch_assemble(const double *E, const double *Wt, int n_phi, int d, int off
 
 // Mixed-precision refinement: FP64 Rayleigh quotients of the FP32 eigenvectors
 // against the FP64 matrix, lambda_j = Re(u^H A u) / (u^H u), replacing the
-// eigenvalues of the FP32-reduced tridiagonal. One workgroup per matrix, one
-// thread per state; A(r,c) is zero outside the circular band |c-r| <= bw (bw >= n/2:
-// dense). resid[kp] = max_j ||A u - lambda u|| / ||u|| (diagnostic).
+// eigenvalues of the FP32-reduced tridiagonal. Y = A U on the FP64 matrix cores:
+// one wave per 16 states, looping over 16-row blocks of Y; for row block r0 only
+// the columns of the circular band window [r0 - bw, r0 + 15 + bw] enter (A is zero
+// outside |c - r| <= bw mod n; the window is the whole row when it would wrap onto
+// itself). The FP32 vectors are widened exactly. Four waves per workgroup, the
+// workgroups of one matrix on one XCD (blockIdx % 8). resid[kp*nsb + sb] = max over
+// the 16 states of block sb of ||A u - lambda u|| / ||u|| (diagnostic).
 __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict__ A, const float *__restrict__ U,
-                                                       int n, int bw, double *__restrict__ W,
-                                                       double *__restrict__ resid) {
-    __shared__ double red[8];
-    const long kp = blockIdx.x;
-    const double *Ak = A + 2L * kp * n * n;
-    const float *Uk = U + 2L * kp * n * n;
-    const bool dense = 2 * bw + 1 >= n;
-    double worst = 0.0;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        cd q{0.0, 0.0};
-        double uu = 0.0, yy = 0.0;
-        for (int r = 0; r < n; r++) {
-            cd y{0.0, 0.0};
-            const int c0 = dense ? 0 : r - bw, c1 = dense ? n - 1 : r + bw;
-            for (int cc = c0; cc <= c1; cc++) {
-                const int c = cc < 0 ? cc + n : (cc >= n ? cc - n : cc);
-                const cx<float> u = ldx(Uk, (long)c * n + j);
-                cfma(y, ld(Ak, (long)c * n + r), cd{(double)u.x, (double)u.y});
-            }
-            const cx<float> ur = ldx(Uk, (long)r * n + j);
-            const cd u{(double)ur.x, (double)ur.y};
-            cfmac(q, u, y);
-            uu += cabs2(u);
-            yy += cabs2(y);
+                                                       int n, int bw, long nkp, int wg_per_kp,
+                                                       double *__restrict__ W, double *__restrict__ resid) {
+    using M = Mf<double>;
+    using v4 = M::v4;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const long q = blockIdx.x >> 3;
+    const long kp = (q / wg_per_kp) * 8 + (blockIdx.x & 7);
+    const int sb = (int)(q % wg_per_kp) * 4 + w;           // state block of this wave
+    const int nsb = (n + 15) / 16;
+    if (kp >= nkp || sb >= nsb) return;                     // no barriers in this kernel
+    const double *Ak = A + 2 * kp * (long)n * n;
+    const float *Uk = U + 2 * kp * (long)n * n;
+    const int j = sb * 16 + li;
+    const bool jin = j < n;
+    const bool dense = 16 + 2 * bw + 3 >= n;
+    const int kw = dense ? n : 16 + 2 * bw;
+    double num = 0.0, yy = 0.0, uu = 0.0;
+    for (int r0 = 0; r0 < n; r0 += 16) {
+        v4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+        const int cs = dense ? 0 : r0 - bw;
+        const int r = r0 + li;
+        for (int k0 = 0; k0 < kw; k0 += 4) {
+            const int kk = k0 + lq;
+            int c = cs + kk;
+            c = c < 0 ? c + n : (c >= n ? c - n : c);
+            const bool kin = kk < kw;
+            const cd a = (kin && r < n) ? ld(Ak, (long)c * n + r) : cd{0.0, 0.0};          // A(r, c)
+            const cx<float> u = (kin && jin) ? ldx(Uk, (long)c * n + j) : cx<float>{0.f, 0.f};   // U[c][j]
+            const double ur = u.x, ui = u.y;
+            yr = M::mma(a.x, ur, yr);
+            yr = M::mma(-a.y, ui, yr);
+            yi = M::mma(a.x, ui, yi);
+            yi = M::mma(a.y, ur, yi);
         }
-        const double lam = q.x / uu;
-        W[kp * n + j] = lam;
-        const double r2 = fmax(yy / uu - lam * lam, 0.0);
-        worst = fmax(worst, sqrt(r2));
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int rr = r0 + M::row(lane, v);
+            if (rr < n && jin) {
+                const cx<float> u = ldx(Uk, (long)rr * n + j);
+                const double ur = u.x, ui = u.y;
+                num += ur * yr[v] + ui * yi[v];
+                yy += yr[v] * yr[v] + yi[v] * yi[v];
+                uu += ur * ur + ui * ui;
+            }
+        }
     }
-    // max over states
-    for (int off = kWave / 2; off > 0; off >>= 1) worst = fmax(worst, __shfl_xor(worst, off, kWave));
-    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = worst;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / kWave); w++) m = fmax(m, red[w]);
-        if (resid) resid[kp] = m;
+    // sum over the four lanes holding state j (lane bits 4-5)
+    for (int off = 16; off < kWave; off <<= 1) {
+        num += __shfl_xor(num, off, kWave);
+        yy += __shfl_xor(yy, off, kWave);
+        uu += __shfl_xor(uu, off, kWave);
     }
+    double worst = 0.0;
+    if (jin && uu > 0.0) {
+        const double lam = num / uu;
+        if (lq == 0) W[kp * n + j] = lam;
+        worst = sqrt(fmax(yy / uu - lam * lam, 0.0));
+    } else if (jin && lq == 0) {
+        W[kp * n + j] = 0.0;
+    }
+    for (int off = 1; off < 16; off <<= 1) worst = fmax(worst, __shfl_xor(worst, off, kWave));
+    if (resid && lane == 0) resid[kp * nsb + sb] = worst;
 }
+
+int rayleigh_resid_slots(int n) { return (n + 15) / 16; }
 
 int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, double *W, double *resid,
                     hipStream_t stream) {
     if (nkp <= 0) return 0;
-    hipLaunchKernelGGL(rayleigh_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, A, U, n, bw, W, resid);
+    const int nsb = (n + 15) / 16, wg = (nsb + 3) / 4;
+    const long groups = (nkp + 7) / 8;
+    hipLaunchKernelGGL(rayleigh_kernel, dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A, U, n, bw, nkp, wg,
+                       W, resid);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
