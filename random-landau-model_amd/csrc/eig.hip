@@ -986,6 +986,16 @@ struct DcShared {
 
 __device__ inline double dc_eps() { return 1.1102230246251565e-16; }  // 2^-53
 
+// 1/x from the hardware reciprocal estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - x r) (each step squares the relative error); x must be finite
+// and non-zero, which holds for every pole distance taken below.
+__device__ inline double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
 // Root j of 1 + rho * sum_i w_i^2 / (dl_i - lambda) = 0, poles dl ascending.
 // Returns origin index o and tau with lambda = dl[o] + tau.
 __device__ void secular_root(int j, int K, const double *dl, const double *w, double rho, int &o_out,
@@ -997,9 +1007,10 @@ __device__ void secular_root(int j, int K, const double *dl, const double *w, do
         // f at the midpoint, origin dl[j]
         double f = 1.0;
         const double tm = 0.5 * Delta;
+#pragma unroll 4
         for (int i = 0; i < K; i++) {
             const double del = (dl[i] - dl[j]) - tm;
-            f += rho * w[i] * w[i] / del;
+            f += rho * w[i] * w[i] * rcp_nr(del);
         }
         if (f >= 0.0) { o = j; lo = 0.0; hi = tm; tau = tm; }
         else { o = j + 1; lo = -tm; hi = 0.0; tau = -tm; }
@@ -1011,17 +1022,25 @@ __device__ void secular_root(int j, int K, const double *dl, const double *w, do
     const double Dj1 = (j < K - 1) ? (dl[j + 1] - dlo) : 0.0;
     for (int it = 0; it < 120; it++) {
         double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, err = 0.0;
-        double delj = 0.0, delj1 = 0.0;
-        for (int i = 0; i < K; i++) {
-            const double del = (dl[i] - dlo) - tau;
-            const double inv = 1.0 / del;                 // one division per term
+        // poles at or below j feed psi, the others phi (all terms of one sign each)
+#pragma unroll 4
+        for (int i = 0; i <= j; i++) {
+            const double inv = rcp_nr((dl[i] - dlo) - tau);
             const double tt = rho * w[i] * w[i] * inv;
-            const double t2 = tt * inv;
-            if (i <= j) { psi += tt; dpsi += t2; } else { phi += tt; dphi += t2; }
+            psi += tt;
+            dpsi += tt * inv;
             err += fabs(tt);
-            if (i == j) delj = del;
-            if (i == j + 1) delj1 = del;
         }
+#pragma unroll 4
+        for (int i = j + 1; i < K; i++) {
+            const double inv = rcp_nr((dl[i] - dlo) - tau);
+            const double tt = rho * w[i] * w[i] * inv;
+            phi += tt;
+            dphi += tt * inv;
+            err += fabs(tt);
+        }
+        const double delj = (dl[j] - dlo) - tau;
+        const double delj1 = (j + 1 < K) ? (dl[j + 1] - dlo) - tau : 0.0;
         const double f = 1.0 + psi + phi;
         if (f > 0.0) hi = fmin(hi, tau);
         else if (f < 0.0) lo = fmax(lo, tau);
@@ -1140,8 +1159,80 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                 }
             }
             __syncthreads();
-            // P2: deflation scan (one lane per node)
-            if (merging && lane == 0) {
+            // P2: deflation scan. Nodes with a whole wave: a parallel pass (ballot
+            // compaction) that is exact whenever no Givens rotation is needed, which it
+            // detects with the same test on the same values as the sequential scan (the
+            // first rotation of that scan always involves untouched entries); the
+            // sequential scan (one lane) runs otherwise, and for smaller nodes.
+            bool seq = merging && lane == 0;
+            if (merging && G >= kWave && lane < kWave) {
+                double dmx = 0.0, zmx = 0.0;
+                for (int e = lane; e < m; e += kWave) {
+                    dmx = fmax(dmx, fabs(sh.ds[lo + e]));
+                    zmx = fmax(zmx, fabs(sh.zs[lo + e]));
+                }
+                for (int off = kWave / 2; off > 0; off >>= 1) {
+                    dmx = fmax(dmx, __shfl_xor(dmx, off, kWave));
+                    zmx = fmax(zmx, __shfl_xor(zmx, off, kWave));
+                }
+                const double tol = 8.0 * dc_eps() * fmax(dmx, zmx);
+                const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+                int K = 0, nd = 0, KL = 0, KR = 0;
+                bool have = false, conflict = false;
+                double pz = 0.0, pd = 0.0;                    // last kept entry of earlier chunks
+                for (int c0 = 0; c0 < m; c0 += kWave) {
+                    const int e = c0 + lane;
+                    const bool valid = e < m;
+                    const double z = valid ? sh.zs[lo + e] : 0.0, d = valid ? sh.ds[lo + e] : 0.0;
+                    const bool defl = valid && rho * fabs(z) <= tol;
+                    const bool keep = valid && !defl;
+                    const unsigned long long km = __ballot(keep);
+                    const unsigned long long lower = km & below;
+                    const int pl = lower ? 63 - __clzll(lower) : 0;
+                    double qz = __shfl(z, pl, kWave), qd = __shfl(d, pl, kWave);
+                    if (!lower) { qz = pz; qd = pd; }
+                    if (keep && (lower != 0 || have)) {
+                        double S_ = qz, C_ = z;
+                        const double tz = hypot(C_, S_);
+                        const double tt = d - qd;
+                        C_ /= tz;
+                        S_ = -S_ / tz;
+                        if (fabs(tt * C_ * S_) <= tol) conflict = true;
+                    }
+                    const int pc = valid ? sh.perm[lo + e] : 0;
+                    const unsigned long long dm = __ballot(defl);
+                    const unsigned long long lm = __ballot(keep && pc < n1);
+                    const unsigned long long rm = __ballot(keep && pc >= n1);
+                    if (keep) {
+                        const int kq = K + __popcll(lower);
+                        sh.ndx[lo + kq] = e;
+                        sh.pcol[lo + kq] = pc;
+                        if (pc < n1) sh.lstL[lo + KL + __popcll(lm & below)] = kq;
+                        else sh.lstR[lo + KR + __popcll(rm & below)] = kq;
+                    }
+                    if (defl) sh.dfx[lo + nd + __popcll(dm & below)] = e;
+                    if (km) {
+                        const int last = 63 - __clzll(km);
+                        pz = __shfl(z, last, kWave);
+                        pd = __shfl(d, last, kWave);
+                        have = true;
+                    }
+                    K += __popcll(km);
+                    nd += __popcll(dm);
+                    KL += __popcll(lm);
+                    KR += __popcll(rm);
+                }
+                const bool any = __ballot(conflict) != 0;
+                if (!any && lane == 0) {
+                    sh.cnt[5 * k + 0] = K;
+                    sh.cnt[5 * k + 1] = nd;
+                    sh.cnt[5 * k + 2] = 0;
+                    sh.cnt[5 * k + 3] = KL;
+                    sh.cnt[5 * k + 4] = KR;
+                }
+                seq = any && lane == 0;
+            }
+            if (seq) {
                 double dmax = 0.0, zmax = 0.0;
                 for (int i = 0; i < m; i++) {
                     dmax = fmax(dmax, fabs(sh.ds[lo + i]));
@@ -1244,7 +1335,7 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                     for (int j = 0; j < K; j++) {
                         if (j == i) continue;
                         const double del = (dl[i] - dl[sh.orig[lo + j]]) - sh.tau[lo + j];
-                        p *= del / (dl[i] - dl[j]);
+                        p *= del * rcp_nr(dl[i] - dl[j]);
                     }
                     sh.what[lo + i] = copysign(sqrt(fmax(-p, 0.0)), sh.wz[lo + i]);
                 }
@@ -1258,7 +1349,7 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                     const double tj = sh.tau[lo + j];
                     double s2 = 0.0;
                     for (int i = 0; i < K; i++) {
-                        const double u = sh.what[lo + i] / ((dl[i] - dl[oj]) - tj);
+                        const double u = sh.what[lo + i] * rcp_nr((dl[i] - dl[oj]) - tj);
                         s2 += u * u;
                     }
                     sh.inrm[lo + j] = 1.0 / sqrt(s2);
@@ -1290,7 +1381,7 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                 for (int idx = lane; idx < K * K; idx += G) {
                     const int i = idx / K, q = idx % K;
                     const double del = (dl[i] - dl[sh.orig[lo + q]]) - sh.tau[lo + q];
-                    S[(long)(lo + i) * n + q] = sh.what[lo + i] / del * sh.inrm[lo + q];
+                    S[(long)(lo + i) * n + q] = sh.what[lo + i] * rcp_nr(del) * sh.inrm[lo + q];
                 }
             }
             __syncthreads();
@@ -1298,56 +1389,73 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
             // Rows of the left half only meet columns of lstL (left child or rotated), rows
             // of the right half only lstR (the block-diagonal structure dlaed3 exploits).
             if (mfma) {
+                // task = 32 rows (2 row blocks) x 64 output columns (4 column tiles):
+                // each gathered Qa value feeds 4 MFMAs, each generated S value 2
                 using v4 = Mf<double>::v4;
+                constexpr int QT = 4, RB = 2;
                 const int wv = lane / kWave, nwv = G / kWave;
                 const int l = lane & (kWave - 1), li = l & 15, lq = l >> 4;
                 const double *dl = sh.dl + lo;
-                const int nqt = (K + 15) / 16;
-                const int nrb = half >= 64 ? 4 : 2;               // 16-row blocks per task
+                const int nqt = (K + 15) / 16, nqr = (nqt + QT - 1) / QT;
                 const int rows1 = hi - mid;                       // right half (may be short)
-                const int ng0 = (n1 + nrb * 16 - 1) / (nrb * 16), ng1 = (rows1 + nrb * 16 - 1) / (nrb * 16);
-                const int ntask = (ng0 + ng1) * nqt;
+                const int ng0 = (n1 + 31) / 32, ng1 = (rows1 + 31) / 32;
+                const int ntask = (ng0 + ng1) * nqr;
                 for (int task = wv; task < ntask; task += nwv) {
-                    const int qt = task % nqt, g = task / nqt;
+                    const int qr = task % nqr, g = task / nqr;
                     const bool right = g >= ng0;
-                    const int rbase = right ? mid + (g - ng0) * nrb * 16 : lo + g * nrb * 16;
+                    const int rbase = right ? mid + (g - ng0) * 32 : lo + g * 32;
                     const int rend = right ? hi : mid;
                     const int cnt = right ? KR : KL;
                     const int *lst = (right ? sh.lstR : sh.lstL) + lo;
-                    const int q = qt * 16 + li;
-                    const bool qin = q < K;
-                    const double lq_o = qin ? dl[sh.orig[lo + q]] : 0.0;
-                    const double tq = qin ? sh.tau[lo + q] : 0.0;
-                    const double iq = qin ? sh.inrm[lo + q] : 0.0;
-                    v4 acc[4];
+                    const int qt0 = qr * QT, nq = min(QT, nqt - qt0);     // wave-uniform
+                    double qo[QT], qtu[QT], qiv[QT];
 #pragma unroll
-                    for (int rb = 0; rb < 4; rb++) acc[rb] = v4{0.0, 0.0, 0.0, 0.0};
+                    for (int u = 0; u < QT; u++) {
+                        const int q = (qt0 + u) * 16 + li;
+                        const bool ok = u < nq && q < K;
+                        qo[u] = ok ? dl[sh.orig[lo + q]] : 0.0;
+                        qtu[u] = ok ? sh.tau[lo + q] : 0.0;
+                        qiv[u] = ok ? sh.inrm[lo + q] : 0.0;          // 0 marks an empty column
+                    }
+                    v4 acc[RB][QT];
+#pragma unroll
+                    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+                        for (int u = 0; u < QT; u++) acc[rb][u] = v4{0.0, 0.0, 0.0, 0.0};
                     for (int idx0 = 0; idx0 < cnt; idx0 += 4) {
                         const int idx = idx0 + lq;
                         const bool kin = idx < cnt;
                         const int i = kin ? lst[idx] : 0;
-                        const double bval = (kin && qin) ? sh.what[lo + i] / ((dl[i] - lq_o) - tq) * iq : 0.0;
+                        const double wi = sh.what[lo + i], di = dl[i];
                         const double *col = Qa + (long)(lo + sh.pcol[lo + i]) * n;
+                        double av[RB];
 #pragma unroll
-                        for (int rb = 0; rb < 4; rb++) {
-                            if (rb < nrb) {
-                                const int r = rbase + rb * 16 + li;
-                                const double av = (kin && r < rend) ? col[r] : 0.0;
-                                acc[rb] = Mf<double>::mma(av, bval, acc[rb]);
+                        for (int rb = 0; rb < RB; rb++) {
+                            const int r = rbase + rb * 16 + li;
+                            av[rb] = (kin && r < rend) ? col[r] : 0.0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < QT; u++) {
+                            if (u < nq) {
+                                const double bv0 = wi * rcp_nr((di - qo[u]) - qtu[u]) * qiv[u];
+                                const double bv = (kin && qiv[u] != 0.0) ? bv0 : 0.0;
+#pragma unroll
+                                for (int rb = 0; rb < RB; rb++) acc[rb][u] = Mf<double>::mma(av[rb], bv, acc[rb][u]);
                             }
                         }
                     }
-                    if (qin) {
-                        double *ocol = Qb + (long)(lo + sh.rank[lo + q]) * n;
 #pragma unroll
-                        for (int rb = 0; rb < 4; rb++) {
-                            if (rb < nrb) {
+                    for (int u = 0; u < QT; u++) {
+                        const int q = (qt0 + u) * 16 + li;
+                        if (u < nq && q < K) {
+                            double *ocol = Qb + (long)(lo + sh.rank[lo + q]) * n;
+#pragma unroll
+                            for (int rb = 0; rb < RB; rb++)
 #pragma unroll
                                 for (int v = 0; v < 4; v++) {
                                     const int r = rbase + rb * 16 + Mf<double>::row(l, v);
-                                    if (r < rend) ocol[r] = acc[rb][v];
+                                    if (r < rend) ocol[r] = acc[rb][u][v];
                                 }
-                            }
                         }
                     }
                 }
