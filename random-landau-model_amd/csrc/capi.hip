@@ -61,6 +61,12 @@ __global__ void pack_kernel(const double *A, long lda, long strideA, int n, doub
     }
 }
 
+// FP64 -> FP32 copy (count values)
+__global__ void to_f32_kernel(const double *P, long count, float *Q) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x)
+        Q[i] = (float)P[i];
+}
+
 // FP64 -> FP32 copy of packed complex matrices (mixed-precision solver input), each
 // matrix scaled by a power of two to max |entry| in [1/2, 1) so that the FP32
 // reduction neither overflows nor underflows; eigenvectors do not depend on the
@@ -162,8 +168,11 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
 // FP32 tridiagonalisation and back-transform (mixed precision); D&C in FP64.
 // Same per-matrix layout of the d | e | tau slot as eig_packed (4n doubles); tau holds
 // 2(n-1) floats inside its 2n-double region.
+// Hb (optional): read the matrices from their band form of offset hb_off instead
+// of A (A then only receives the reflectors).
 int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork &ws, hipStream_t s,
-                   hipEvent_t mid = nullptr, hipEvent_t mid2 = nullptr) {
+                   hipEvent_t mid = nullptr, hipEvent_t mid2 = nullptr, const double *Hb = nullptr,
+                   int hb_off = 0) {
     int rc;
     const long strideV = 4L * n;   // doubles per matrix, for D, E and TAU alike
     if ((size_t)batch * strideV * sizeof(double) > ws.dev.bytes) return -22;
@@ -171,7 +180,11 @@ int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork 
     double *D = base, *E = base + n;
     float *TAU = reinterpret_cast<float *>(base + 2 * n);
     const int gt = (int)(batch < kGridTrd ? batch : kGridTrd);
-    if ((rc = launch_chetrd(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, gt, s))) return rc;
+    if (Hb) {
+        if ((rc = launch_chetrd_band(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, s, Hb, hb_off))) return rc;
+    } else if ((rc = launch_chetrd(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, gt, s))) {
+        return rc;
+    }
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
     const int gl = (int)(batch < kGridTail ? batch : kGridTail);
     if ((size_t)gl * 3 * n * n * sizeof(double) > ws.work.bytes) return -22;
@@ -378,8 +391,10 @@ struct rlm_plan_s {
     // host
     std::vector<double> cmn, cmt, cjm, hx, hy, hs;
     // device
-    DevBuf dcmn, dcmt, dE, dX, dY, dS, dV, dW, dA, dU, dEig, dL, dOut, dA32, dResid;
+    DevBuf dcmn, dcmt, dE, dX, dY, dS, dV, dW, dH, dA, dU, dEig, dL, dOut, dA32, dResid;
     EigWork ws;
+    int nb = 0;              // band width of the band form (dW: S*(g+1) x d x nb, dH: nkp x N x nb)
+    bool band_trd = false;   // mixed plans: the FP32 reduction reads the band form (no dense matrices)
     int loaded = 0;
     long first = 0;
     static constexpr int kStages = 8;
@@ -402,6 +417,9 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
     p->precision = precision;
     const long N = n_phi, d = p->d, g1 = n_mesh + 1, S = samples_per_batch;
     const long nkp = S * g1 * g1;
+    p->nb = band_width((int)N, (int)p->off);
+    p->band_trd = precision == HIP_DIAG_MIXED && chetrd_band_supported((int)N);
+    const long nb = p->nb;
     p->cmn.resize(2 * d * d);
     p->cmt.resize(2 * g1 * d);
     p->cjm.resize(2 * N * d);
@@ -418,8 +436,8 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
     if ((rc = p->dcmn.ensure(8 * p->cmn.size())) || (rc = p->dcmt.ensure(8 * p->cmt.size())) ||
         (rc = p->dE.ensure(16 * g1 * d * N)) || (rc = p->dX.ensure(8 * S * p->n_imp + 8)) ||
         (rc = p->dY.ensure(8 * S * p->n_imp + 8)) || (rc = p->dS.ensure(8 * S * p->n_imp + 8)) ||
-        (rc = p->dV.ensure(16 * S * d * d)) || (rc = p->dW.ensure(16 * S * g1 * d * N)) ||
-        (rc = p->dA.ensure(16 * nkp * N * N)) ||
+        (rc = p->dV.ensure(16 * S * d * d)) || (rc = p->dW.ensure(16 * S * g1 * d * nb)) ||
+        (rc = p->dH.ensure(16 * nkp * N * nb)) || (!p->band_trd && (rc = p->dA.ensure(16 * nkp * N * N))) ||
         (rc = p->dU.ensure((precision == HIP_DIAG_MIXED ? 8 : 16) * nkp * N * N)) ||
         (precision == HIP_DIAG_MIXED && (rc = p->dA32.ensure(8 * nkp * N * N))) ||
         (rc = p->dResid.ensure(8 * nkp * rayleigh_resid_slots((int)N))) ||
@@ -470,12 +488,19 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
     if ((rc = launch_potential(p->dX.d(), p->dY.d(), p->dS.d(), p->n_imp, p->dcmn.d(), d, off, p->dV.d(), S, s)))
         return rc;
-    if ((rc = launch_wtable(p->dV.d(), p->dcmt.d(), N, d, off, g, S, p->dW.d(), s))) return rc;
+    if ((rc = launch_wband(p->dV.d(), p->dcmt.d(), N, d, off, g, S, p->dW.d(), s))) return rc;
     RLM_HIP_CHECK(hipEventRecord(p->ev[1], s));
+    // band form of every k-point's Hamiltonian; dense FP64 matrices where a kernel needs them
     const bool mixed = p->precision == HIP_DIAG_MIXED;
     float *A32 = mixed ? static_cast<float *>(p->dA32.p) : nullptr;
-    if ((rc = launch_assemble(p->dE.d(), p->dW.d(), N, d, off, g, 0, nkp, p->dA.d(), (long)N * N, s, A32)))
-        return rc;
+    if ((rc = launch_hband(p->dE.d(), p->dW.d(), N, d, off, g, 0, nkp, p->dH.d(), s))) return rc;
+    if (!p->band_trd) {
+        if ((rc = launch_expand_band(p->dH.d(), N, off, nkp, p->dA.d(), (long)N * N, s))) return rc;
+        if (mixed) {
+            hipLaunchKernelGGL(to_f32_kernel, dim3(2048), dim3(256), 0, s, p->dA.d(), 2 * nkp * N * (long)N, A32);
+            RLM_HIP_CHECK(hipGetLastError());
+        }
+    }
     RLM_HIP_CHECK(hipEventRecord(p->ev[2], s));
     double *L1 = p->dL.d(), *L2 = L1 + 2 * nkp * N;
     if (!mixed) {
@@ -485,10 +510,14 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
         if ((rc = launch_links(p->dU.d(), (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
     } else {
         float *U32 = static_cast<float *>(p->dU.p);
-        if ((rc = eig_packed_f32(A32, N, nkp, p->dEig.d(), U32, p->ws, s, p->ev[3], p->ev[4]))) return rc;
+        if ((rc = eig_packed_f32(A32, N, nkp, p->dEig.d(), U32, p->ws, s, p->ev[3], p->ev[4],
+                                 p->band_trd ? p->dH.d() : nullptr, off)))
+            return rc;
         RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
         // FP64 Rayleigh quotients against the FP64 matrix (band |c-r| <= off_head)
-        if ((rc = launch_rayleigh(p->dA.d(), U32, N, off, nkp, p->dEig.d(), p->dResid.d(), s))) return rc;
+        if ((rc = launch_rayleigh(p->band_trd ? nullptr : p->dA.d(), U32, N, off, nkp, p->dEig.d(), p->dResid.d(), s,
+                                  p->band_trd ? p->dH.d() : nullptr)))
+            return rc;
         RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
         if ((rc = launch_links_f32(U32, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
     }
@@ -575,8 +604,8 @@ extern "C" const double *rlm_plan_eigenvectors(rlm_plan_t p) { return p ? p->dU.
 extern "C" int rlm_plan_destroy(rlm_plan_t p) {
     if (!p) return 0;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    DevBuf *bufs[] = {&p->dcmn, &p->dcmt, &p->dE, &p->dX,   &p->dY, &p->dS,   &p->dV,    &p->dW,
-                      &p->dA,   &p->dU,   &p->dEig, &p->dL, &p->dOut, &p->dA32, &p->dResid};
+    DevBuf *bufs[] = {&p->dcmn, &p->dcmt, &p->dE,   &p->dX, &p->dY,   &p->dS,   &p->dV,     &p->dW,
+                      &p->dH,   &p->dA,   &p->dU,   &p->dEig, &p->dL, &p->dOut, &p->dA32, &p->dResid};
     for (DevBuf *b : bufs) b->release();
     p->ws.release();
     for (int k = 0; k <= rlm_plan_s::kStages; k++)

@@ -214,4 +214,16 @@ struct Mf<float> {
     static __device__ inline int row(int l, int v) { return 4 * (l >> 4) + v; }
 };
 
+
+// ---------------------------------------------------------------- band form
+// Circular band D = |c - r| in B = {0..o} U {N-o..N-1} (all of [0, N) when
+// 2o+1 >= N), stored as b = 0..nb-1: band_D maps b to D, band_b maps D to b (-1
+// outside the band).
+__host__ __device__ inline int band_nb(int N, int off) { return 2 * off + 1 >= N ? N : 2 * off + 1; }
+__device__ inline int band_D(int b, int N, int off) { return (2 * off + 1 >= N || b <= off) ? b : b + N - 2 * off - 1; }
+__device__ inline int band_b(int D, int N, int off) {
+    if (2 * off + 1 >= N || D <= off) return D;
+    return D >= N - off ? D - N + 2 * off + 1 : -1;
+}
+
 }  // namespace rlm

@@ -753,10 +753,13 @@ __device__ __forceinline__ void resident_tile(Sub a, bool diag, int rb, int cb, 
     }
 }
 
+// Hb (optional): the matrices in the band form of offset hb_off (complex double,
+// n x band_nb(n, hb_off) per matrix) instead of A; A then only receives the reflectors.
 template <typename T, int NS, int NL>
 __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
                                                              double *__restrict__ Dout, double *__restrict__ Eout,
-                                                             T *__restrict__ TAUout, long strideV, int batch) {
+                                                             T *__restrict__ TAUout, long strideV, int batch,
+                                                             const double *__restrict__ Hb, int hb_off) {
     using CT = cx<T>;
     constexpr int NT = 512, NW = NT / kWave, TS = 32, NMAX = 256, NQ = NMAX / kWave;
     constexpr int NLS = NL > 0 ? NL : 1;
@@ -797,9 +800,19 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
         double *Ev = Eout + b * strideV;
         T *Tv = TAUout + b * strideV * (long)(sizeof(double) / sizeof(T));
         if (n == 1) {
-            if (t == 0) Dv[0] = (double)ldx(A, 0).x;
+            if (t == 0) Dv[0] = Hb ? Hb[2 * b] : (double)ldx(A, 0).x;
             return;
         }
+        const int hnb = Hb ? band_nb(n, hb_off) : 0;
+        const double *Hk = Hb ? Hb + 2 * b * (long)n * hnb : nullptr;
+        // A(r, c), r >= c: dense, or H[c][b(r-c)] (real on the diagonal)
+        auto lower_entry = [&](int r, int c) -> CT {
+            if (!Hk) return ldx(A, (long)c * lda + r);
+            const int bb = band_b(r - c, n, hb_off);
+            if (bb < 0) return CT{(T)0, (T)0};
+            const cd h = ld(Hk, (long)c * hnb + bb);
+            return CT{(T)h.x, r == c ? (T)0 : (T)h.y};
+        };
         CT a[NS][16];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
@@ -809,7 +822,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
 #pragma unroll
                 for (int ru = 0; ru < 4; ru++) {
                     const int r = rb + ru, c = cb + cu;
-                    a[s][ru * 4 + cu] = (tR[s] >= 0 && r < n && r >= c) ? ldx(A, (long)c * lda + r) : CT{(T)0, (T)0};
+                    a[s][ru * 4 + cu] = (tR[s] >= 0 && r < n && r >= c) ? lower_entry(r, c) : CT{(T)0, (T)0};
                 }
         }
         if (NL > 0 && lR >= 0) {
@@ -819,7 +832,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
 #pragma unroll
                 for (int ru = 0; ru < 4; ru++) {
                     const int r = rb + ru, c = cb + cu;
-                    lsub[(ru * 4 + cu) * kWave] = r < n ? ldx(A, (long)c * lda + r) : CT{(T)0, (T)0};
+                    lsub[(ru * 4 + cu) * kWave] = r < n ? lower_entry(r, c) : CT{(T)0, (T)0};
                 }
         }
         for (int e = t; e < NMAX; e += NT) {
@@ -2016,12 +2029,12 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
 
 template <typename T, int NS, int NL>
 static int launch_zhetrd_resident_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
-                                    int batch, hipStream_t stream) {
+                                    int batch, hipStream_t stream, const double *Hb, int hb_off) {
     const int tn = (n + 31) / 32, ntiles = tn * (tn + 1) / 2;
     const int nl = ntiles > 8 * NS ? ntiles - 8 * NS : 0;
-    if (n < 1 || n > 256 || nl > NL || nl > tn) return -22;
+    if (n < 1 || n > 256 || nl > NL || (nl > 0 && nl >= tn)) return -22;
     hipLaunchKernelGGL((zhetrd_resident_kernel<T, NS, NL>), dim3(batch), dim3(512), 0, stream, A, strideA, lda, n, D,
-                       E, TAU, strideV, batch);
+                       E, TAU, strideV, batch, Hb, hb_off);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -2046,21 +2059,34 @@ static int hetrd_variant() {
     return kTrdAuto;
 }
 
+// True when launch_chetrd_band can take band-form input (register-resident kernel).
+bool chetrd_band_supported(int n) {
+    const int v = hetrd_variant();
+    return (v == kTrdAuto || v == kTrdResident) && n <= resident_max_n(sizeof(float));
+}
+
+template <typename T>
+static int launch_resident_any(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
+                               int batch, hipStream_t stream, const double *Hb, int hb_off) {
+    const int tn = (n + 31) / 32, ntiles = tn * (tn + 1) / 2;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (ntiles <= 10)
+            return launch_zhetrd_resident_t<T, 1, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb, hb_off);
+        if (ntiles <= 16)
+            return launch_zhetrd_resident_t<T, 2, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb, hb_off);
+        return launch_zhetrd_resident_t<T, 4, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb, hb_off);
+    } else {
+        return launch_zhetrd_resident_t<T, 1, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb, hb_off);
+    }
+}
+
 template <typename T>
 static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
                             int grid, hipStream_t stream) {
     int v = hetrd_variant();
     if (v == kTrdAuto) v = n <= resident_max_n(sizeof(T)) ? kTrdResident : kTrdLower;
-    if (v == kTrdResident && n <= resident_max_n(sizeof(T))) {
-        const int tn = (n + 31) / 32, ntiles = tn * (tn + 1) / 2;
-        if constexpr (sizeof(T) == sizeof(float)) {
-            if (ntiles <= 10) return launch_zhetrd_resident_t<T, 1, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
-            if (ntiles <= 16) return launch_zhetrd_resident_t<T, 2, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
-            return launch_zhetrd_resident_t<T, 4, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
-        } else {
-            return launch_zhetrd_resident_t<T, 1, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
-        }
-    }
+    if (v == kTrdResident && n <= resident_max_n(sizeof(T)))
+        return launch_resident_any<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, nullptr, 0);
     // the panel kernels keep W (n x 16) in LDS: beyond 160 KB fall back to the fused kernel
     if ((v == kTrdLower || v == kTrdResident) && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024)
         return launch_zhetrd_lower_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
@@ -2076,6 +2102,11 @@ int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E,
 int launch_chetrd(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV, int batch,
                   int grid, hipStream_t stream) {
     return launch_hetrd_any<float>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+}
+int launch_chetrd_band(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV,
+                       int batch, hipStream_t stream, const double *Hb, int hb_off) {
+    if (!chetrd_band_supported(n) || !Hb) return -22;
+    return launch_resident_any<float>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb, hb_off);
 }
 int launch_tail(const double *A, long strideA, int lda, int n, const double *D, const double *E, const double *TAU,
                 long strideV, double *W, long strideW, double *U, long strideU, double *work, double *tws, int batch,

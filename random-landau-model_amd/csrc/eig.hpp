@@ -27,6 +27,12 @@ long backtransform_tws_complex(int n);
 // divide & conquer stay FP64.
 int launch_chetrd(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV, int batch,
                   int grid, hipStream_t stream);
+// FP32 tridiagonalisation reading the matrices from their band form (complex
+// double, n x band_nb(n, hb_off) each); A receives the reflectors only. Supported
+// where chetrd_band_supported(n) (register-resident kernel, n <= 256).
+bool chetrd_band_supported(int n);
+int launch_chetrd_band(float *A, long strideA, int lda, int n, double *D, double *E, float *TAU, long strideV,
+                       int batch, hipStream_t stream, const double *Hb, int hb_off);
 int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *D, const double *E, const float *TAU,
                     long strideV, double *W, long strideW, float *U, long strideU, double *work, float *tws, int batch,
                     int grid, hipStream_t stream, hipEvent_t mid = nullptr);

@@ -155,6 +155,121 @@ int launch_assemble(const double *E, const double *Wt, int n_phi, int d, int off
     return 0;
 }
 
+// ---------------------------------------------------------------- band form
+// The projected Hamiltonian of one k-point is non-zero only on the circular band
+// D = |c - r| in B = {0..o} U {N-o..N-1} (all of [0, N) when 2o+1 >= N). The band
+// form keeps H[j][b] = h_{j, j+D(b)} = sum_m E[t2][m][j] W[t1][m][D(b)], b < nb =
+// |B|, from which A(r, c) = conj(H[r][b(c-r)]) for c >= r and H[c][b(r-c)] below
+// (real diagonal), the matrix the dense assembly writes.
+int band_width(int N, int off) { return band_nb(N, off); }
+
+// Wb[z][m][b] = W[z][m][D(b)] (the dense table's non-zero columns); grid
+// ((nb+63)/64, d, S*(g+1)), z = s*(g+1)+t1.
+__global__ void wband_kernel(const double *__restrict__ V, const double *__restrict__ cmt, int n_phi, int d, int off,
+                             int n_mesh, int nb, double *__restrict__ Wb) {
+    const int z = blockIdx.z, m = blockIdx.y;
+    const int s = z / (n_mesh + 1), t1 = z % (n_mesh + 1);
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int D = band_D(b, n_phi, off);
+    const double *Vs = V + 2L * s * d * d;
+    cd sum{0.0, 0.0};
+    if (D <= off) {
+        const int nn = off + D;
+        sum = cadd(sum, cmul(ld(Vs, (long)m * d + nn), ld(cmt, (long)t1 * d + nn)));
+    }
+    if (n_phi - D <= off) {
+        const int nn = off + D - n_phi;
+        sum = cadd(sum, cmul(ld(Vs, (long)m * d + nn), ld(cmt, (long)t1 * d + nn)));
+    }
+    st(Wb, ((long)z * d + m) * nb + b, sum);
+}
+
+// H[kp][j][b] = sum_m E[t2][m][j] Wb[s,t1][m][b] on the FP64 matrix cores: one
+// workgroup per k-point, 16x16 output tiles dealt to the 4 waves, K = m in steps of
+// 4. Output kp is local (blockIdx.x); the k-point index is kp0 + blockIdx.x.
+__global__ __launch_bounds__(256) void hband_kernel(const double *__restrict__ E, const double *__restrict__ Wb, int N,
+                                                    int d, int nb, int n_mesh, long kp0, double *__restrict__ H) {
+    using M = Mf<double>;
+    using v4 = M::v4;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const long kp = kp0 + blockIdx.x;
+    const int g1 = n_mesh + 1;
+    const long s = kp / ((long)g1 * g1);
+    const int rem = (int)(kp % ((long)g1 * g1));
+    const int t1 = rem / g1, t2 = rem % g1;
+    const double *Et = E + 2L * t2 * d * N;
+    const double *Wz = Wb + 2L * (s * g1 + t1) * d * nb;
+    double *Hk = H + 2L * blockIdx.x * N * nb;
+    const int nrb = (N + 15) / 16, ncb = (nb + 15) / 16;
+    for (int tile = w; tile < nrb * ncb; tile += 4) {
+        const int j0 = (tile / ncb) * 16, b0 = (tile % ncb) * 16;
+        v4 hr = {0.0, 0.0, 0.0, 0.0}, hi = {0.0, 0.0, 0.0, 0.0};
+        for (int m0 = 0; m0 < d; m0 += 4) {
+            const int m = m0 + lq;
+            const cd e = (m < d && j0 + li < N) ? ld(Et, (long)m * N + j0 + li) : cd{0.0, 0.0};    // E[m][j]
+            const cd x = (m < d && b0 + li < nb) ? ld(Wz, (long)m * nb + b0 + li) : cd{0.0, 0.0}; // Wb[m][b]
+            hr = M::mma(e.x, x.x, hr);
+            hr = M::mma(-e.y, x.y, hr);
+            hi = M::mma(e.x, x.y, hi);
+            hi = M::mma(e.y, x.x, hi);
+        }
+        const int b = b0 + li;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int j = j0 + M::row(lane, v);
+            if (j < N && b < nb) st(Hk, (long)j * nb + b, cd{hr[v], b == 0 ? 0.0 : hi[v]});
+        }
+    }
+}
+
+// Dense column-major A of each k-point from its band form; grid.x = k-points.
+__global__ __launch_bounds__(256) void expand_band_kernel(const double *__restrict__ H, int N, int off, int nb,
+                                                          double *__restrict__ A, long strideA) {
+    const double *Hk = H + 2L * blockIdx.x * N * nb;
+    double *Ak = A + 2L * blockIdx.x * strideA;
+    for (long idx = threadIdx.x; idx < (long)N * N; idx += blockDim.x) {
+        const int c = (int)(idx / N), r = (int)(idx % N);
+        cd v{0.0, 0.0};
+        if (r <= c) {
+            const int b = band_b(c - r, N, off);
+            if (b >= 0) v = cconj(ld(Hk, (long)r * nb + b));
+        } else {
+            const int b = band_b(r - c, N, off);
+            if (b >= 0) v = ld(Hk, (long)c * nb + b);
+        }
+        st(Ak, (long)c * N + r, v);
+    }
+}
+
+int launch_wband(const double *V, const double *cmt, int n_phi, int d, int off, int n_mesh, int n_samples, double *Wb,
+                 hipStream_t stream) {
+    const int nb = band_width(n_phi, off);
+    dim3 grid((nb + 63) / 64, d, n_samples * (n_mesh + 1));
+    hipLaunchKernelGGL(wband_kernel, grid, dim3(64), 0, stream, V, cmt, n_phi, d, off, n_mesh, nb, Wb);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, int n_mesh, long kp0, long nkp,
+                 double *H, hipStream_t stream) {
+    if (nkp <= 0) return 0;
+    hipLaunchKernelGGL(hband_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, E, Wb, n_phi, d,
+                       band_width(n_phi, off), n_mesh, kp0, H);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_expand_band(const double *H, int n_phi, int off, long nkp, double *A, long strideA, hipStream_t stream) {
+    if (nkp <= 0) return 0;
+    hipLaunchKernelGGL(expand_band_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, H, n_phi, off,
+                       band_width(n_phi, off), A, strideA);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 // Mixed-precision refinement: FP64 Rayleigh quotients of the FP32 eigenvectors
 // against the FP64 matrix, lambda_j = Re(u^H A u) / (u^H u), replacing the
 // eigenvalues of the FP32-reduced tridiagonal. Y = A U on the FP64 matrix cores:
@@ -164,9 +279,11 @@ int launch_assemble(const double *E, const double *Wt, int n_phi, int d, int off
 // itself). The FP32 vectors are widened exactly. Four waves per workgroup, the
 // workgroups of one matrix on one XCD (blockIdx % 8). resid[kp*nsb + sb] = max over
 // the 16 states of block sb of ||A u - lambda u|| / ||u|| (diagnostic).
-__global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict__ A, const float *__restrict__ U,
-                                                       int n, int bw, long nkp, int wg_per_kp,
-                                                       double *__restrict__ W, double *__restrict__ resid) {
+// With Hb (band form, off = bw) the matrix entries come from it instead of A.
+__global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict__ A, const double *__restrict__ Hb,
+                                                       const float *__restrict__ U, int n, int bw, long nkp,
+                                                       int wg_per_kp, double *__restrict__ W,
+                                                       double *__restrict__ resid) {
     using M = Mf<double>;
     using v4 = M::v4;
     const int lane = threadIdx.x & (kWave - 1);
@@ -177,7 +294,9 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
     const int sb = (int)(q % wg_per_kp) * 4 + w;           // state block of this wave
     const int nsb = (n + 15) / 16;
     if (kp >= nkp || sb >= nsb) return;                     // no barriers in this kernel
-    const double *Ak = A + 2 * kp * (long)n * n;
+    const int nb = band_nb(n, bw);
+    const double *Ak = Hb ? nullptr : A + 2 * kp * (long)n * n;
+    const double *Hk = Hb ? Hb + 2 * kp * (long)n * nb : nullptr;
     const float *Uk = U + 2 * kp * (long)n * n;
     const int j = sb * 16 + li;
     const bool jin = j < n;
@@ -193,7 +312,18 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
             int c = cs + kk;
             c = c < 0 ? c + n : (c >= n ? c - n : c);
             const bool kin = kk < kw;
-            const cd a = (kin && r < n) ? ld(Ak, (long)c * n + r) : cd{0.0, 0.0};          // A(r, c)
+            cd a{0.0, 0.0};                                                               // A(r, c)
+            if (kin && r < n) {
+                if (!Hk) {
+                    a = ld(Ak, (long)c * n + r);
+                } else if (r <= c) {
+                    const int b = band_b(c - r, n, bw);
+                    if (b >= 0) a = cconj(ld(Hk, (long)r * nb + b));
+                } else {
+                    const int b = band_b(r - c, n, bw);
+                    if (b >= 0) a = ld(Hk, (long)c * nb + b);
+                }
+            }
             const cx<float> u = (kin && jin) ? ldx(Uk, (long)c * n + j) : cx<float>{0.f, 0.f};   // U[c][j]
             const double ur = u.x, ui = u.y;
             yr = M::mma(a.x, ur, yr);
@@ -234,12 +364,12 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
 int rayleigh_resid_slots(int n) { return (n + 15) / 16; }
 
 int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, double *W, double *resid,
-                    hipStream_t stream) {
+                    hipStream_t stream, const double *Hb) {
     if (nkp <= 0) return 0;
     const int nsb = (n + 15) / 16, wg = (nsb + 3) / 4;
     const long groups = (nkp + 7) / 8;
-    hipLaunchKernelGGL(rayleigh_kernel, dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A, U, n, bw, nkp, wg,
-                       W, resid);
+    hipLaunchKernelGGL(rayleigh_kernel, dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A, Hb, U, n, bw, nkp,
+                       wg, W, resid);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

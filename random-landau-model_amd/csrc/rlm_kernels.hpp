@@ -10,9 +10,18 @@ int launch_wtable(const double *V, const double *cmt, int n_phi, int d, int off,
                   hipStream_t stream);
 int launch_assemble(const double *E, const double *Wt, int n_phi, int d, int off, int n_mesh, long kp0, long nkp,
                     double *A, long strideA, hipStream_t stream, float *A32 = nullptr);
+// Band form of the projected Hamiltonians (see rlm_kernels.hip): nb = band_width
+// columns per row; Wb: S*(g+1) x d x nb; H: nkp x N x nb (complex).
+int band_width(int N, int off);
+int launch_wband(const double *V, const double *cmt, int n_phi, int d, int off, int n_mesh, int n_samples, double *Wb,
+                 hipStream_t stream);
+int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, int n_mesh, long kp0, long nkp,
+                 double *H, hipStream_t stream);
+int launch_expand_band(const double *H, int n_phi, int off, long nkp, double *A, long strideA, hipStream_t stream);
 // FP64 Rayleigh quotients of FP32 eigenvectors against the FP64 matrices (circular
-// band |c - r| <= bw). resid (optional): nkp * rayleigh_resid_slots(n) values.
+// band |c - r| <= bw), dense A or, with Hb, the band form of offset bw.
+// resid (optional): nkp * rayleigh_resid_slots(n) values.
 int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, double *W, double *resid,
-                    hipStream_t stream);
+                    hipStream_t stream, const double *Hb = nullptr);
 int rayleigh_resid_slots(int n);
 }  // namespace rlm
