@@ -97,7 +97,7 @@ def main():
     ap.add_argument("--g", type=int, default=32)
     ap.add_argument("--samples", type=int, default=32)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--cpu-lines", type=int, default=6)
+    ap.add_argument("--cpu-lines", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--path", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
