@@ -486,10 +486,32 @@ __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase,
                 const int ntiles = Tn * (Tn + 1) / 2;
                 C *yw = ypart + wid * n;
                 const int lrg = lane & 7, lcg = lane >> 3;
-                for (int tk = wid; tk < ntiles; tk += NW) {
-                    int R = 0;
+                // The 16 entries of a sub-block load unconditionally (indices clamped into the
+                // trailing block, masked afterwards) and the next tile's loads are issued before
+                // the current tile's arithmetic, so memory latency overlaps the work.
+                auto tile_of = [&](int tk, int &R, int &Cc) {
+                    R = 0;
                     while ((R + 1) * (R + 2) / 2 <= tk) R++;
-                    const int Cc = tk - R * (R + 1) / 2;
+                    Cc = tk - R * (R + 1) / 2;
+                };
+                auto load_tile = [&](int tk, C (&av)[16]) {
+                    int R, Cc;
+                    tile_of(tk, R, Cc);
+                    const int rb = R * TS + 4 * lrg, cb = Cc * TS + 4 * lcg;
+#pragma unroll
+                    for (int cu = 0; cu < 4; cu++)
+#pragma unroll
+                        for (int ru = 0; ru < 4; ru++) {
+                            const int rr = min(rb + ru, m - 1), cc = min(cb + cu, m - 1);
+                            av[ru * 4 + cu] = ldx(A, Aidx(i + 1 + rr, i + 1 + cc));
+                        }
+                };
+                C cur[16], nxt[16];
+                if (wid < ntiles) load_tile(wid, cur);
+                for (int tk = wid; tk < ntiles; tk += NW) {
+                    if (tk + NW < ntiles) load_tile(tk + NW, nxt);
+                    int R, Cc;
+                    tile_of(tk, R, Cc);
                     const int rb = R * TS + 4 * lrg;          // first local row of the sub-block
                     const int cb = Cc * TS + 4 * lcg;         // first local column
                     C vrow[4], vcol[4], rs[4], cs[4];
@@ -507,13 +529,13 @@ __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase,
                         for (int ru = 0; ru < 4; ru++) {
                             const int rloc = rb + ru;
                             const bool use = rloc < m && cloc < m && (R > Cc || rloc >= cloc);
-                            if (use) {
-                                const C a = ldx(A, Aidx(i + 1 + rloc, i + 1 + cloc));
-                                fmax_(rs[ru], a, vcol[cu]);                      // y_r += A(r,c) v_c
-                                if (rloc != cloc) fmacx(cs[cu], a, vrow[ru]);   // y_c += conj(A(r,c)) v_r
-                            }
+                            const C a = use ? cur[ru * 4 + cu] : C{(T)0, (T)0};
+                            fmax_(rs[ru], a, vcol[cu]);                      // y_r += A(r,c) v_c
+                            if (rloc != cloc) fmacx(cs[cu], a, vrow[ru]);   // y_c += conj(A(r,c)) v_r
                         }
                     }
+#pragma unroll
+                    for (int e = 0; e < 16; e++) cur[e] = nxt[e];
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
 #pragma unroll
@@ -577,35 +599,51 @@ __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase,
                 __syncthreads();
             }
             // rank-2*nbk update of the lower triangle of the trailing block (rows >= cols >= k1)
+            // on the matrix cores: A(row, col) -= sum_k X[row][k] conj(Y[col][k]) with
+            // X = [V W], Y = [W V] (K = 2*NB), per 16x16 tile of the lower triangle. The
+            // product is formed transposed, M[col][row], so that element v of lane l is
+            // A(row = R + l%16, col = Cb + Mf::row(l, v)): 16 consecutive rows per column.
             const int k1 = k0 + nbk;
             const int nr = n - k1;
             if (nr > 0) {
-                const int R = (nr >= NT) ? NT : ((nr + kWave - 1) / kWave) * kWave;
-                const int G = NT / R;
-                const int rl = t % R, cg = t / R;
-                if (cg < G) {
+                using M = Mf<T>;
+                using v4 = typename M::v4;
+                const int li = lane & 15, lq = lane >> 4;
+                const int nt16 = (nr + 15) / 16, ntl = nt16 * (nt16 + 1) / 2;
+                for (int tl = wid; tl < ntl; tl += NW) {
+                    int tr = 0;
+                    while ((tr + 1) * (tr + 2) / 2 <= tl) tr++;
+                    const int tc = tl - tr * (tr + 1) / 2;
+                    const int R = k1 + tr * 16, Cb = k1 + tc * 16;
+                    v4 accr, acci;
 #pragma unroll
-                    for (int k = 0; k < MAXR; k++) {
-                        const int r = k1 + rl + k * R;
-                        if (r >= n) continue;
-                        C vr[NB], wr[NB];
+                    for (int v = 0; v < 4; v++) {
+                        const int row = R + li, col = Cb + M::row(lane, v);
+                        const C x = (row < n && col < n && row >= col) ? ldx(A, Aidx(row, col)) : C{(T)0, (T)0};
+                        accr[v] = x.x;
+                        acci[v] = x.y;
+                    }
+                    const int rowq = R + li, colq = Cb + li;      // operand rows of X and Y
 #pragma unroll
-                        for (int l = 0; l < NB; l++) {
-                            vr[l] = l < nbk ? Vget(k0, r, l) : C{(T)0, (T)0};
-                            wr[l] = l < nbk ? Wp[(long)r * NB + l] : C{(T)0, (T)0};
+                    for (int s4 = 0; s4 < 2 * NB / 4; s4++) {
+                        const int kk = 4 * s4 + lq;               // 0..2NB-1: [V | W] for X, [W | V] for Y
+                        const int l = kk < NB ? kk : kk - NB;
+                        C xv{(T)0, (T)0}, yv{(T)0, (T)0};
+                        if (l < nbk) {
+                            if (rowq < n) xv = kk < NB ? Vget(k0, rowq, l) : Wp[(long)rowq * NB + l];
+                            if (colq < n) yv = kk < NB ? Wp[(long)colq * NB + l] : Vget(k0, colq, l);
                         }
-                        for (int c = k1 + cg; c <= r; c += G) {
-                            C a = ldx(A, Aidx(r, c));
+                        // A-operand P[col][k] = conj(Y[col][k]), B-operand Q[k][row] = X[row][k]
+                        accr = M::mma(-yv.x, xv.x, accr);
+                        accr = M::mma(-yv.y, xv.y, accr);
+                        acci = M::mma(-yv.x, xv.y, acci);
+                        acci = M::mma(yv.y, xv.x, acci);
+                    }
 #pragma unroll
-                            for (int l = 0; l < NB; l++) {
-                                if (l < nbk) {
-                                    const C vc = Vget(k0, c, l), wc = Wp[(long)c * NB + l];
-                                    a = a - (vr[l] * conjx(wc) + wr[l] * conjx(vc));
-                                }
-                            }
-                            if (r == c) a.y = (T)0;
-                            stx(A, Aidx(r, c), a);
-                        }
+                    for (int v = 0; v < 4; v++) {
+                        const int row = R + li, col = Cb + M::row(lane, v);
+                        if (row < n && col < n && row >= col)
+                            stx(A, Aidx(row, col), C{accr[v], row == col ? (T)0 : acci[v]});
                     }
                 }
             }
