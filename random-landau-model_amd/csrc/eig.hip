@@ -1802,8 +1802,11 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
     using M = Mf<T>;
     using v4 = typename M::v4;
     using CT = cx<T>;
-    constexpr int CW = 32, NCB = CW / 16, RBW = 4;
-    __shared__ CT vst[4][16 * 17];
+    constexpr int CW = 32, NCB = CW / 16, RBW = 4, VP = 17;   // VP: padded row stride of the V stage
+    // complex float: the next block's V rows are loaded into registers while the
+    // current block computes (complex double keeps them out for register room)
+    constexpr bool PREFETCH = sizeof(T) == sizeof(float);
+    __shared__ CT vst[4][RBW][16 * VP];                          // per wave: V(rows of row block i, k)
     __shared__ T wred[2][4][2 * NCB * 4 * kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -1833,8 +1836,28 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
         }
     }
     const int nref = n - 1, nblk = (nref + 15) / 16;
+    // V(R + li, k) for k = lq + 4kk of this wave's row blocks (coalesced: 16 rows per k)
+    CT vpre[RBW][4];
+    auto load_v = [&](int bk) {
+#pragma unroll
+        for (int i = 0; i < RBW; i++) {
+            const int R = (w + 4 * i) * 16;
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++)
+                vpre[i][kk] = (R + 15 >= bk * 16 + 1) ? refl_get(A, lda, n, bk * 16, R + li, lq + 4 * kk)
+                                                      : CT{(T)0, (T)0};
+        }
+    };
+    if (PREFETCH) load_v(nblk - 1);
     for (int bk = nblk - 1; bk >= 0; bk--) {
         const int j0 = bk * 16, r0 = j0 + 1;
+        if (!PREFETCH) load_v(bk);
+        // stage this wave's V rows (read back transposed by W = V^H U, directly by U -= V W2)
+#pragma unroll
+        for (int i = 0; i < RBW; i++)
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) vst[w][i][(lq + 4 * kk) * VP + li] = vpre[i][kk];
+        __builtin_amdgcn_wave_barrier();
         v4 wr[NCB], wi[NCB];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
@@ -1846,13 +1869,9 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
         for (int i = 0; i < RBW; i++) {
             const int R = (w + 4 * i) * 16;
             if (R + 15 < r0) continue;                    // wave-uniform: V is zero above row r0
-            // stage V(R..R+15, 0..15) through LDS (coalesced read: lane li = row)
-#pragma unroll
-            for (int kk = lq; kk < 16; kk += 4) vst[w][kk * 17 + li] = refl_get(A, lda, n, j0, R + li, kk);
-            __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const CT x = vst[w][li * 17 + M::row(lane, v)];   // V[R + row(l, v)][k = li]
+                const CT x = vst[w][i][li * VP + M::row(lane, v)];   // V[R + row(l, v)][k = li]
 #pragma unroll
                 for (int cb = 0; cb < NCB; cb++) {
                     wr[cb] = M::mma(x.x, ur[i][cb][v], wr[cb]);
@@ -1861,7 +1880,6 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
                     wi[cb] = M::mma(-x.y, ur[i][cb][v], wi[cb]);
                 }
             }
-            __builtin_amdgcn_wave_barrier();
         }
         // sum W over the 4 waves (double-buffered by block parity: one barrier per block)
         T *wb = wred[bk & 1][w];
@@ -1873,6 +1891,7 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
                 wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = wi[cb][v];
             }
         __syncthreads();
+        if (PREFETCH && bk > 0) load_v(bk - 1);           // in flight during T W and the update
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++)
 #pragma unroll
@@ -1911,7 +1930,7 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
             if (R + 15 < r0) continue;
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                const CT x = refl_get(A, lda, n, j0, R + li, M::row(lane, s));   // V[R + li][k]
+                const CT x = vst[w][i][M::row(lane, s) * VP + li];   // V[R + li][k = row(l, s)]
 #pragma unroll
                 for (int cb = 0; cb < NCB; cb++) {
                     ur[i][cb] = M::mma(-x.x, w2r[cb][s], ur[i][cb]);
@@ -1921,6 +1940,7 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
                 }
             }
         }
+        __builtin_amdgcn_wave_barrier();                   // the next block rewrites vst
     }
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
