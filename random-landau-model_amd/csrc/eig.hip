@@ -1028,7 +1028,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
 // 2. Divide & conquer on the symmetric tridiagonal + 3. back-transform
 // ============================================================================
 struct DcShared {
-    double *dq, *dqn, *ds, *zs, *dl, *wz, *tau, *vals, *rc, *rs, *what, *inrm;
+    double *dq, *dqn, *ds, *zs, *dl, *wz, *z2, *tau, *vals, *rc, *rs, *what, *inrm;
     double *red;   // 8 doubles of reduction scratch
     int *perm, *orig, *rank, *rpj, *rnj, *ndx, *dfx;
     int *pcol, *lstL, *lstR, *rflag;  // eigenvector-update gather index, per-half column lists, rotated flags
@@ -1047,9 +1047,10 @@ __device__ inline double rcp_nr(double x) {
     return r;
 }
 
-// Root j of 1 + rho * sum_i w_i^2 / (dl_i - lambda) = 0, poles dl ascending.
+// Root j of 1 + sum_i z2_i / (dl_i - lambda) = 0 with z2_i = rho w_i^2, poles dl
+// ascending (sum_i w_i^2 = 1).
 // Returns origin index o and tau with lambda = dl[o] + tau.
-__device__ void secular_root(int j, int K, const double *dl, const double *w, double rho, int &o_out,
+__device__ void secular_root(int j, int K, const double *dl, const double *z2, double rho, int &o_out,
                              double &tau_out) {
     int o;
     double lo, hi, tau;
@@ -1061,7 +1062,7 @@ __device__ void secular_root(int j, int K, const double *dl, const double *w, do
 #pragma unroll 4
         for (int i = 0; i < K; i++) {
             const double del = (dl[i] - dl[j]) - tm;
-            f += rho * w[i] * w[i] * rcp_nr(del);
+            f += z2[i] * rcp_nr(del);
         }
         if (f >= 0.0) { o = j; lo = 0.0; hi = tm; tau = tm; }
         else { o = j + 1; lo = -tm; hi = 0.0; tau = -tm; }
@@ -1077,7 +1078,7 @@ __device__ void secular_root(int j, int K, const double *dl, const double *w, do
 #pragma unroll 4
         for (int i = 0; i <= j; i++) {
             const double inv = rcp_nr((dl[i] - dlo) - tau);
-            const double tt = rho * w[i] * w[i] * inv;
+            const double tt = z2[i] * inv;
             psi += tt;
             dpsi += tt * inv;
             err += fabs(tt);
@@ -1085,7 +1086,7 @@ __device__ void secular_root(int j, int K, const double *dl, const double *w, do
 #pragma unroll 4
         for (int i = j + 1; i < K; i++) {
             const double inv = rcp_nr((dl[i] - dlo) - tau);
-            const double tt = rho * w[i] * w[i] * inv;
+            const double tt = z2[i] * inv;
             phi += tt;
             dphi += tt * inv;
             err += fabs(tt);
@@ -1362,8 +1363,10 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                     }
                 }
                 for (int q = lane; q < K; q += G) {
+                    const double zq = sh.zs[lo + sh.ndx[lo + q]];
                     sh.dl[lo + q] = sh.ds[lo + sh.ndx[lo + q]];
-                    sh.wz[lo + q] = sh.zs[lo + sh.ndx[lo + q]];
+                    sh.wz[lo + q] = zq;
+                    sh.z2[lo + q] = rho * zq * zq;
                 }
             }
             __syncthreads();
@@ -1372,7 +1375,7 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                 for (int j = lane; j < K; j += G) {
                     int o;
                     double tu;
-                    secular_root(j, K, sh.dl + lo, sh.wz + lo, rho, o, tu);
+                    secular_root(j, K, sh.dl + lo, sh.z2 + lo, rho, o, tu);
                     sh.orig[lo + j] = o;
                     sh.tau[lo + j] = tu;
                 }
@@ -1473,18 +1476,30 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                     for (int rb = 0; rb < RB; rb++)
 #pragma unroll
                         for (int u = 0; u < QT; u++) acc[rb][u] = v4{0.0, 0.0, 0.0, 0.0};
-                    for (int idx0 = 0; idx0 < cnt; idx0 += 4) {
-                        const int idx = idx0 + lq;
-                        const bool kin = idx < cnt;
-                        const int i = kin ? lst[idx] : 0;
-                        const double wi = sh.what[lo + i], di = dl[i];
+                    // K-step operands one step ahead: list entry, weights and the gathered
+                    // Qa values of step idx0 + 4 load while step idx0 runs on the matrix cores
+                    bool kin_n;
+                    double wi_n, di_n, av_n[RB];
+                    auto fetch = [&](int idx) {
+                        kin_n = idx < cnt;
+                        const int i = kin_n ? lst[idx] : 0;
+                        wi_n = sh.what[lo + i];
+                        di_n = dl[i];
                         const double *col = Qa + (long)(lo + sh.pcol[lo + i]) * n;
-                        double av[RB];
 #pragma unroll
                         for (int rb = 0; rb < RB; rb++) {
                             const int r = rbase + rb * 16 + li;
-                            av[rb] = (kin && r < rend) ? col[r] : 0.0;
+                            av_n[rb] = (kin_n && r < rend) ? col[r] : 0.0;
                         }
+                    };
+                    fetch(lq);
+                    for (int idx0 = 0; idx0 < cnt; idx0 += 4) {
+                        const bool kin = kin_n;
+                        const double wi = wi_n, di = di_n;
+                        double av[RB];
+#pragma unroll
+                        for (int rb = 0; rb < RB; rb++) av[rb] = av_n[rb];
+                        if (idx0 + 4 < cnt) fetch(idx0 + 4 + lq);
 #pragma unroll
                         for (int u = 0; u < QT; u++) {
                             if (u < nq) {
@@ -1580,7 +1595,7 @@ __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restri
     {
         double *p = smem;
         sh.dq = p; p += n; sh.dqn = p; p += n; sh.ds = p; p += n; sh.zs = p; p += n;
-        sh.dl = p; p += n; sh.wz = p; p += n; sh.tau = p; p += n; sh.vals = p; p += n;
+        sh.dl = p; p += n; sh.wz = p; p += n; sh.z2 = p; p += n; sh.tau = p; p += n; sh.vals = p; p += n;
         sh.rc = p; p += n; sh.rs = p; p += n; sh.what = p; p += n; sh.inrm = p; p += n;
         sh.red = p; p += 8;
         int *ip = reinterpret_cast<int *>(p);
@@ -1962,7 +1977,7 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
 // ============================================================================
 // divide & conquer arrays, and at least the 32 x 33 transpose tile of the output copy
 size_t heevd_dc_lds_bytes(int n) {
-    const size_t dc = (size_t)n * (12 * sizeof(double) + 16 * sizeof(int)) + 8 * sizeof(double);
+    const size_t dc = (size_t)n * (13 * sizeof(double) + 16 * sizeof(int)) + 8 * sizeof(double);
     const size_t tile = 32 * 33 * sizeof(double);
     return dc > tile ? dc : tile;
 }
