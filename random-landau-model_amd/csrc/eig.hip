@@ -909,7 +909,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                 const int rr = lane + kWave * q;
                 if (rr >= 2 && i + rr < n) s2 += abs2x(cq[q]);
             }
-            const T nrm2 = wave_sum_t(s2);
+            const T nrm2 = sum_lanes_012(sum_lanes_345(s2));   // every lane gets the same sum
             T beta, taur, taui, sr, si;
             if (nrm2 == (T)0 && ai == (T)0) {
                 beta = ar; taur = 0; taui = 0; sr = 0; si = 0;
@@ -992,8 +992,8 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                         fmacx(dl, p[q], vn[rr]);
                     }
                 }
-                dl.x = wave_sum_t(dl.x);
-                dl.y = wave_sum_t(dl.y);
+                dl.x = sum_lanes_012(sum_lanes_345(dl.x));
+                dl.y = sum_lanes_012(sum_lanes_345(dl.y));
                 const CT alpha2 = (tau * dl) * (T)(-0.5);
                 CT *wn = wbuf[cur];
 #pragma unroll
