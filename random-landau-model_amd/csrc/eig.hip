@@ -1809,7 +1809,7 @@ __global__ __launch_bounds__(256) void larft_kernel(const T *__restrict__ Abase,
 // W = V^H U (each wave over its rows, summed through LDS in a fixed order),
 // W2 = T W, U -= V W2. Workgroups are mapped so that the slabs of one matrix share
 // blockIdx % 8, i.e. one XCD and its L2, which then serves V and T to all of them.
-template <typename T>
+template <typename T, int CW>
 __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__restrict__ Abase, long strideA, int lda,
                                                                  int n, const T *__restrict__ Tin, long strideT,
                                                                  T *__restrict__ Uout, long strideU, int nslab,
@@ -1817,11 +1817,13 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
     using M = Mf<T>;
     using v4 = typename M::v4;
     using CT = cx<T>;
-    constexpr int CW = 32, NCB = CW / 16, RBW = 4, VP = 17;   // VP: padded row stride of the V stage
-    // complex float: the next block's V rows are loaded into registers while the
-    // current block computes (complex double keeps them out for register room)
-    constexpr bool PREFETCH = sizeof(T) == sizeof(float);
-    __shared__ CT vst[4][RBW][16 * VP];                          // per wave: V(rows of row block i, k)
+    constexpr int NCB = CW / 16, RBW = 4, VP = 17;            // VP: padded row stride of the V stage
+    // complex float: each wave stages all its V rows once per block and prefetches the
+    // next block's; complex double stages one 16-row block at a time and reads V again
+    // in the update (LDS and registers then allow two workgroups per CU)
+    constexpr bool STAGE_ALL = sizeof(T) == sizeof(float);
+    constexpr bool PREFETCH = STAGE_ALL;
+    __shared__ CT vst[4][STAGE_ALL ? RBW : 1][16 * VP];          // per wave: V(rows of row block i, k)
     __shared__ T wred[2][4][2 * NCB * 4 * kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -1852,8 +1854,9 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
     }
     const int nref = n - 1, nblk = (nref + 15) / 16;
     // V(R + li, k) for k = lq + 4kk of this wave's row blocks (coalesced: 16 rows per k)
-    CT vpre[RBW][4];
+    CT vpre[STAGE_ALL ? RBW : 1][4];
     auto load_v = [&](int bk) {
+        if (!STAGE_ALL) return;
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
             const int R = (w + 4 * i) * 16;
@@ -1866,13 +1869,14 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
     if (PREFETCH) load_v(nblk - 1);
     for (int bk = nblk - 1; bk >= 0; bk--) {
         const int j0 = bk * 16, r0 = j0 + 1;
-        if (!PREFETCH) load_v(bk);
-        // stage this wave's V rows (read back transposed by W = V^H U, directly by U -= V W2)
+        if (STAGE_ALL) {
+            // stage this wave's V rows (read back transposed by W = V^H U, directly by U -= V W2)
 #pragma unroll
-        for (int i = 0; i < RBW; i++)
+            for (int i = 0; i < (STAGE_ALL ? RBW : 1); i++)
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++) vst[w][i][(lq + 4 * kk) * VP + li] = vpre[i][kk];
-        __builtin_amdgcn_wave_barrier();
+                for (int kk = 0; kk < 4; kk++) vst[w][i][(lq + 4 * kk) * VP + li] = vpre[i][kk];
+            __builtin_amdgcn_wave_barrier();
+        }
         v4 wr[NCB], wi[NCB];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
@@ -1884,9 +1888,15 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
         for (int i = 0; i < RBW; i++) {
             const int R = (w + 4 * i) * 16;
             if (R + 15 < r0) continue;                    // wave-uniform: V is zero above row r0
+            const int si = STAGE_ALL ? i : 0;
+            if (!STAGE_ALL) {
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) vst[w][0][(lq + 4 * kk) * VP + li] = refl_get(A, lda, n, j0, R + li, lq + 4 * kk);
+                __builtin_amdgcn_wave_barrier();
+            }
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const CT x = vst[w][i][li * VP + M::row(lane, v)];   // V[R + row(l, v)][k = li]
+                const CT x = vst[w][si][li * VP + M::row(lane, v)];  // V[R + row(l, v)][k = li]
 #pragma unroll
                 for (int cb = 0; cb < NCB; cb++) {
                     wr[cb] = M::mma(x.x, ur[i][cb][v], wr[cb]);
@@ -1895,6 +1905,7 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
                     wi[cb] = M::mma(-x.y, ur[i][cb][v], wi[cb]);
                 }
             }
+            if (!STAGE_ALL) __builtin_amdgcn_wave_barrier();   // the next row block rewrites vst
         }
         // sum W over the 4 waves (double-buffered by block parity: one barrier per block)
         T *wb = wred[bk & 1][w];
@@ -1945,7 +1956,8 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
             if (R + 15 < r0) continue;
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                const CT x = vst[w][i][M::row(lane, s) * VP + li];   // V[R + li][k = row(l, s)]
+                const CT x = STAGE_ALL ? vst[w][STAGE_ALL ? i : 0][M::row(lane, s) * VP + li]
+                                       : refl_get(A, lda, n, j0, R + li, M::row(lane, s));   // V[R + li][k = row(l, s)]
 #pragma unroll
                 for (int cb = 0; cb < NCB; cb++) {
                     ur[i][cb] = M::mma(-x.x, w2r[cb][s], ur[i][cb]);
@@ -2080,10 +2092,13 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
         hipLaunchKernelGGL((larft_kernel<T>), dim3(batch), dim3(256), 0, stream, A, strideA, lda, n, TAU, strideV, tws,
                            strideT, batch);
         RLM_HIP_CHECK(hipGetLastError());
-        const int nslab = (n + 31) / 32;
+        // slab width: 32 columns for complex float; 16 for complex double, which keeps the
+        // accumulators within two waves per SIMD
+        constexpr int CW = sizeof(T) == sizeof(float) ? 32 : 16;
+        const int nslab = (n + CW - 1) / CW;
         const long groups = (batch + 7) / 8;
-        hipLaunchKernelGGL((backtransform_mfma_kernel<T>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0, stream,
-                           A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
+        hipLaunchKernelGGL((backtransform_mfma_kernel<T, CW>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0,
+                           stream, A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
         RLM_HIP_CHECK(hipGetLastError());
         return 0;
     }
