@@ -387,9 +387,12 @@ __global__ __launch_bounds__(NT) void zhetrd_blocked_kernel(T *__restrict__ Abas
 // summed in a fixed order, so results do not depend on scheduling. Halves the
 // trailing-matrix traffic of the full-storage kernels.
 template <typename T, int NT, int NB, int MAXR>
+// ncols: columns to reduce (n - 1 for the whole matrix; fewer leaves the updated
+// trailing block A(ncols:n, ncols:n) for another kernel to finish).
 __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
                                                           double *__restrict__ Dout, double *__restrict__ Eout,
-                                                          T *__restrict__ TAUout, long strideV, int batch) {
+                                                          T *__restrict__ TAUout, long strideV, int batch,
+                                                          int ncols) {
     using C = cx<T>;
     constexpr int NW = NT / kWave;
     constexpr int NQ = 2 * NB;
@@ -418,8 +421,8 @@ __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase,
             if (r == j + 1) return C{(T)1, (T)0};
             return ldx(A, Aidx(r, j));
         };
-        for (int k0 = 0; k0 < n - 1; k0 += NB) {
-            const int nbk = min(NB, n - 1 - k0);
+        for (int k0 = 0; k0 < ncols; k0 += NB) {
+            const int nbk = min(NB, ncols - k0);
             for (int j = 0; j < nbk; j++) {
                 const int i = k0 + j;
                 // (1) column i, rows >= i
@@ -649,7 +652,7 @@ __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase,
             }
             __syncthreads();
         }
-        if (t == 0) Dv[n - 1] = (double)ldx(A, Aidx(n - 1, n - 1)).x;
+        if (t == 0 && ncols == n - 1) Dv[n - 1] = (double)ldx(A, Aidx(n - 1, n - 1)).x;
         __syncthreads();
     }
 }
@@ -2054,7 +2057,8 @@ static size_t lower_lds_bytes(int n, int NB, size_t real) {
 
 template <typename T, int NB>
 int launch_zhetrd_lower_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
-                          int grid, hipStream_t stream) {
+                          int grid, hipStream_t stream, int ncols = -1) {
+    if (ncols < 0) ncols = n - 1;
     if (n < 1 || n > 512) return -22;
     constexpr int NT = 512;
     const size_t lds = lower_lds_bytes(n, NB, sizeof(T));
@@ -2064,7 +2068,7 @@ int launch_zhetrd_lower_t(T *A, long strideA, int lda, int n, double *D, double 
         RLM_HIP_CHECK(hipFuncSetAttribute((const void *)zhetrd_lower_kernel<T, NT, NB, 1>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((zhetrd_lower_kernel<T, NT, NB, 1>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D,
-                       E, TAU, strideV, batch);
+                       E, TAU, strideV, batch, ncols);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -2134,9 +2138,11 @@ static int resident_max_n(size_t real) { return real == sizeof(float) ? 256 : 12
 
 // RLM_HETRD selects the tridiagonalisation kernel: "resident" (lower triangle in
 // registers), "fused" (unblocked, full storage), "blocked" (panels, full storage),
-// "lower" (panels, lower storage, tiled symv). Unset or "auto": resident where n
-// fits it, else lower. Read at every launch so one process can compare them.
-enum { kTrdAuto = 0, kTrdFused, kTrdBlocked, kTrdLower, kTrdResident };
+// "lower" (panels, lower storage, tiled symv), "hybrid" (lower for the leading
+// columns, resident for the trailing block it fits; slower than lower for complex
+// double at n = 256, kept selectable). Unset or "auto": resident where n fits it,
+// else lower. Read at every launch so one process can compare them.
+enum { kTrdAuto = 0, kTrdFused, kTrdBlocked, kTrdLower, kTrdResident, kTrdHybrid };
 static int hetrd_variant() {
     const char *e = getenv("RLM_HETRD");
     if (!e) return kTrdAuto;
@@ -2144,6 +2150,7 @@ static int hetrd_variant() {
     if (strcmp(e, "blocked") == 0) return kTrdBlocked;
     if (strcmp(e, "lower") == 0) return kTrdLower;
     if (strcmp(e, "resident") == 0) return kTrdResident;
+    if (strcmp(e, "hybrid") == 0) return kTrdHybrid;
     return kTrdAuto;
 }
 
@@ -2172,9 +2179,20 @@ template <typename T>
 static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
                             int grid, hipStream_t stream) {
     int v = hetrd_variant();
-    if (v == kTrdAuto) v = n <= resident_max_n(sizeof(T)) ? kTrdResident : kTrdLower;
-    if (v == kTrdResident && n <= resident_max_n(sizeof(T)))
+    const int rmax = resident_max_n(sizeof(T));
+    if (v == kTrdAuto) v = n <= rmax ? kTrdResident : kTrdLower;
+    if ((v == kTrdResident || v == kTrdHybrid) && n <= rmax)
         return launch_resident_any<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, nullptr, 0);
+    if (v == kTrdHybrid && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024) {
+        // columns 0..ncols-1 by panels of 16 through memory; the updated trailing block
+        // A(ncols:n, ncols:n) (at most rmax) is then a Hermitian matrix whose
+        // reduction continues on chip, writing d, e, tau and the reflectors in place
+        const int ncols = ((n - rmax) + 15) / 16 * 16;
+        int rc = launch_zhetrd_lower_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream, ncols);
+        if (rc) return rc;
+        return launch_resident_any<T>(A + 2 * ((long)ncols * lda + ncols), strideA, lda, n - ncols, D + ncols,
+                                      E + ncols, TAU + 2 * ncols, strideV, batch, stream, nullptr, 0);
+    }
     // the panel kernels keep W (n x 16) in LDS: beyond 160 KB fall back to the fused kernel
     if ((v == kTrdLower || v == kTrdResident) && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024)
         return launch_zhetrd_lower_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);

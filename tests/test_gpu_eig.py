@@ -43,10 +43,10 @@ def test_random_hermitian(n, b):
     _check(H, w.cpu().numpy(), U.cpu().numpy())
 
 
-@pytest.mark.parametrize("variant", ["fused", "blocked", "lower", "resident"])
+@pytest.mark.parametrize("variant", ["fused", "blocked", "lower", "resident", "hybrid"])
 @pytest.mark.parametrize("n,b", [(5, 3), (17, 4), (100, 3), (128, 2), (256, 2)])
 def test_tridiagonalisation_variants(monkeypatch, variant, n, b):
-    """Every tridiagonalisation kernel (RLM_HETRD=fused | blocked | lower | resident) against
+    """Every tridiagonalisation kernel (RLM_HETRD=fused | blocked | lower | resident | hybrid) against
     LAPACK; the register-resident FP64 kernel covers n <= 128 (beyond, the lower kernel runs)."""
     import torch
     from rlm_amd import heevd_batched
