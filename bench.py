@@ -128,6 +128,10 @@ def main():
     n_phi, g, S, seed = args.n_phi, args.g, args.samples, args.seed
     g1 = g + 1
     nkp = S * g1 * g1
+    # matrices diagonalised per launch: the plans solve theta_1, theta_2 < g and derive
+    # the theta = 2 pi edges (all (g+1)^2 points with RLM_MESH=full)
+    pl = g1 if os.environ.get("RLM_MESH") == "full" else g
+    nsolved = S * pl * pl
     N = float(n_phi)
     # algorithmic real flops per matrix of the two solver kernels: Householder
     # tridiagonalisation 16/3 n^3, back-transformation of the eigenvectors 8 n^3
@@ -148,7 +152,7 @@ def main():
     flops = {"tridiagonalisation": 16.0 / 3.0 * N ** 3, "backtransform": 8.0 * N ** 3}
 
     def run_path(precision, tag):
-        print(f"[bench] {tag}: n_phi={n_phi} g={g} samples={S} seed={seed}: {nkp} twist points", file=sys.stderr,
+        print(f"[bench] {tag}: n_phi={n_phi} g={g} samples={S} seed={seed}: {nkp} twist points, {nsolved} solved", file=sys.stderr,
               flush=True)
         plan = RlmPlan(n_phi, 1.0, 16.0, g, seed, samples_per_batch=S, precision=precision)
         plan.load(0, S)
@@ -203,7 +207,7 @@ def main():
     flops_unit = flops[dom]
     kname = trd_name(path) if dom == "tridiagonalisation" else "backtransform_kernel"
     t_launch = main["stage"][dom] / 1e3
-    achieved = flops_unit * nkp / t_launch / 1e12
+    achieved = flops_unit * nsolved / t_launch / 1e12
     peak = FP32_PEAK_TFLOPS if (path == "mixed") else FP64_PEAK_TFLOPS
     traffic = _load_traffic(f"{kname}<{'float' if path == 'mixed' else 'double'}>")
 
@@ -224,6 +228,7 @@ def main():
             "workload": f"rlm n_phi={n_phi} q=1 i=16 g={g} ({g1 * g1} twist points) x {S} samples, seed {seed}",
             "path": path,
             "twist_points": nkp,
+            "solved_points": nsolved,
             "time_per_kpoint_us": main["elapsed"] / args.steps / nkp * 1e6,
             "stage_ms": {a: round(v, 3) for a, v in main["stage"].items()},
             "fp64_samples_per_hour": fp64["value"],
@@ -247,7 +252,7 @@ def main():
             "frac": achieved / peak,
             "traffic": traffic,
             "flops_per_matrix": flops_unit,
-            "matrices_per_launch": nkp,
+            "matrices_per_launch": nsolved,
         },
         "cpu_baseline": None,
     }

@@ -107,7 +107,8 @@ int hip_set_hamil_matrix(const double *d_coeff_m_theta_1, const double *d_coeff_
 
 /* Replaces the sample / theta_1 / theta_2 loops of rlm/main_rlm_cpu.cpp:109-243
  * (and rlm/main_rlm_gpu.cu:138-300): for each disorder sample, assemble,
- * diagonalise and Chern-reduce the whole (n_mesh+1)^2 twist mesh on the GPU.
+ * diagonalise and Chern-reduce the whole (n_mesh+1)^2 twist mesh on the GPU
+ * (n_mesh^2 points diagonalised, the theta = 2 pi edges derived from them).
  * off_override < 0 keeps the reference's off_head = (long)(sqrt(N_imp)/2+1). */
 typedef struct rlm_plan_s *rlm_plan_t;
 int rlm_plan_create(rlm_plan_t *plan, long n_phi, double quanta_concentration, double impurity_concentration,
@@ -137,8 +138,13 @@ int rlm_plan_info(rlm_plan_t plan, long *out6);
  * matrix) over every state and twist point of the last execute; 0 for FP64 plans. */
 int rlm_plan_max_residual(rlm_plan_t plan, double *out);
 /* Device pointer of the eigenvalues [kp][n_phi] / eigenvectors
- * [kp][component][state] of the last execute (kp = (s*(g+1)+t1)*(g+1)+t2);
- * eigenvectors are complex128 (FP64 plans) or complex64 (mixed plans). */
+ * [kp][component][state] of the last execute, for the solved twist points
+ * kp = (s*g+t1)*g+t2, t1, t2 < g: the theta = 2 pi points are not solved, their
+ * matrices are unitarily equivalent to the theta = 0 ones (same eigenvalues;
+ * u(t1, g)[i] = u(t1, 0)[(i+1) % n_phi] and u(g, t2)[i] = e^{i 2 pi i/n_phi}
+ * u(0, t2)[i] up to a phase per state). With RLM_MESH=full in the environment at
+ * rlm_plan_create, every point is solved and kp = (s*(g+1)+t1)*(g+1)+t2.
+ * Eigenvectors are complex128 (FP64 plans) or complex64 (mixed plans). */
 const double *rlm_plan_eigenvalues(rlm_plan_t plan);
 const double *rlm_plan_eigenvectors(rlm_plan_t plan);
 int rlm_plan_destroy(rlm_plan_t plan);

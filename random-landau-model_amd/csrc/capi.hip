@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -395,6 +396,10 @@ struct rlm_plan_s {
     EigWork ws;
     int nb = 0;              // band width of the band form (dW: S*(g+1) x d x nb, dH: nkp x N x nb)
     bool band_trd = false;   // mixed plans: the FP32 reduction reads the band form (no dense matrices)
+    // Solve only the twist points theta_1, theta_2 < g (pl = g points per line) and
+    // derive the theta = 2 pi edges in the link kernel; RLM_MESH=full solves all
+    // (g+1)^2 points (pl = g+1).
+    int pl = 0;
     int loaded = 0;
     long first = 0;
     static constexpr int kStages = 8;
@@ -416,7 +421,12 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
     p->seed = (uint32_t)seed;
     p->precision = precision;
     const long N = n_phi, d = p->d, g1 = n_mesh + 1, S = samples_per_batch;
-    const long nkp = S * g1 * g1;
+    {
+        const char *mesh = getenv("RLM_MESH");
+        p->pl = (mesh && !strcmp(mesh, "full")) ? (int)g1 : (int)n_mesh;
+    }
+    const long nkp = S * p->pl * p->pl;   // solved twist points
+    const long nlk = S * g1 * g1;         // twist points with links
     p->nb = band_width((int)N, (int)p->off);
     p->band_trd = precision == HIP_DIAG_MIXED && chetrd_band_supported((int)N);
     const long nb = p->nb;
@@ -441,7 +451,7 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
         (rc = p->dU.ensure((precision == HIP_DIAG_MIXED ? 8 : 16) * nkp * N * N)) ||
         (precision == HIP_DIAG_MIXED && (rc = p->dA32.ensure(8 * nkp * N * N))) ||
         (rc = p->dResid.ensure(8 * nkp * rayleigh_resid_slots((int)N))) ||
-        (rc = p->dEig.ensure(8 * nkp * N)) || (rc = p->dL.ensure(32 * nkp * N)) ||
+        (rc = p->dEig.ensure(8 * nkp * N)) || (rc = p->dL.ensure(32 * nlk * N)) ||
         (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)))
         return fail(rc);
     if (hipMemcpy(p->dcmn.p, p->cmn.data(), 8 * p->cmn.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -482,7 +492,8 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     if (!p || p->loaded < 1) return -22;
     const int S = p->loaded;
     const int N = (int)p->n_phi, d = (int)p->d, off = (int)p->off, g = (int)p->g;
-    const long g1 = g + 1, nkp = (long)S * g1 * g1;
+    const long g1 = g + 1, nkp = (long)S * p->pl * p->pl, nlk = (long)S * g1 * g1;
+    const bool derive = p->pl == g;
     hipStream_t s = p->stream;
     int rc;
     RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
@@ -493,7 +504,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     // band form of every k-point's Hamiltonian; dense FP64 matrices where a kernel needs them
     const bool mixed = p->precision == HIP_DIAG_MIXED;
     float *A32 = mixed ? static_cast<float *>(p->dA32.p) : nullptr;
-    if ((rc = launch_hband(p->dE.d(), p->dW.d(), N, d, off, g, 0, nkp, p->dH.d(), s))) return rc;
+    if ((rc = launch_hband(p->dE.d(), p->dW.d(), N, d, off, g, p->pl, 0, nkp, p->dH.d(), s))) return rc;
     if (!p->band_trd) {
         if ((rc = launch_expand_band(p->dH.d(), N, off, nkp, p->dA.d(), (long)N * N, s))) return rc;
         if (mixed) {
@@ -502,12 +513,16 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
         }
     }
     RLM_HIP_CHECK(hipEventRecord(p->ev[2], s));
-    double *L1 = p->dL.d(), *L2 = L1 + 2 * nkp * N;
+    double *L1 = p->dL.d(), *L2 = L1 + 2 * nlk * N;
     if (!mixed) {
         if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[3], p->ev[4]))) return rc;
         RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
         RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
-        if ((rc = launch_links(p->dU.d(), (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
+        if (derive)
+            rc = launch_links_derived(p->dU.p, false, (long)N * N, N, g, S, L1, L2, s);
+        else
+            rc = launch_links(p->dU.d(), (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s);
+        if (rc) return rc;
     } else {
         float *U32 = static_cast<float *>(p->dU.p);
         if ((rc = eig_packed_f32(A32, N, nkp, p->dEig.d(), U32, p->ws, s, p->ev[3], p->ev[4],
@@ -519,11 +534,15 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
                                   p->band_trd ? p->dH.d() : nullptr)))
             return rc;
         RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
-        if ((rc = launch_links_f32(U32, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s))) return rc;
+        if (derive)
+            rc = launch_links_derived(U32, true, (long)N * N, N, g, S, L1, L2, s);
+        else
+            rc = launch_links_f32(U32, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s);
+        if (rc) return rc;
     }
     RLM_HIP_CHECK(hipEventRecord(p->ev[7], s));
     double *out = p->dOut.d();
-    if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, (int)g1, 0, (double)N, S, out, out + (long)S * N,
+    if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, (int)g1, derive ? 2 : 0, (double)N, S, out, out + (long)S * N,
                             out + 2L * S * N, s)))
         return rc;
     RLM_HIP_CHECK(hipEventRecord(p->ev[8], s));
@@ -588,7 +607,7 @@ extern "C" int rlm_plan_max_residual(rlm_plan_t p, double *out) {
         *out = 0.0;
         return 0;
     }
-    const long nres = (long)p->loaded * (p->g + 1) * (p->g + 1) * rayleigh_resid_slots((int)p->n_phi);
+    const long nres = (long)p->loaded * p->pl * p->pl * rayleigh_resid_slots((int)p->n_phi);
     std::vector<double> h(nres);
     RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
     RLM_HIP_CHECK(hipMemcpy(h.data(), p->dResid.p, 8 * nres, hipMemcpyDeviceToHost));

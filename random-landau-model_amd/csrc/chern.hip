@@ -54,8 +54,70 @@ __global__ __launch_bounds__(256) void links_kernel(const TU *__restrict__ U, lo
     }
 }
 
+// rlm links with the twist-boundary points derived instead of solved. U holds the
+// solved points (theta_1, theta_2 < g), kp_s = (s*g + t1)*g + t2; the others follow
+// exactly from the model's tables (the matrix at theta_2 = 2 pi is the theta_2 = 0
+// matrix with rows and columns shifted by one, the one at theta_1 = 2 pi is
+// diag(e^{i 2 pi j/N}) A diag(e^{-i 2 pi j/N})):
+//   u(t1, g)[i] = u(t1, 0)[(i+1) % N],  u(g, t2)[i] = e^{i 2 pi i/N} u(0, t2)[i].
+// Links are written for the full (g+1)^2 mesh, kp = (s*(g+1) + t1)*(g+1) + t2, as
+// links_kernel does; grid (S*(g+1)^2), block 256.
+template <typename TU>
+__global__ __launch_bounds__(256) void links_derived_kernel(const TU *__restrict__ U, long strideU, int ns, int g,
+                                                            double *__restrict__ Lk1, double *__restrict__ Lk2) {
+    __shared__ cd phs[512];                                  // e^{i 2 pi i/N}, N <= 512
+    const long kp = blockIdx.x;
+    const int g1 = g + 1;
+    const long s = kp / ((long)g1 * g1);
+    const int rem = (int)(kp % ((long)g1 * g1));
+    const int t1 = rem / g1, t2 = rem % g1;
+    const double TWO_PI = 6.283185307179586232;
+    if (t1 >= g - 1) {                                       // blocks that touch theta_1 = 2 pi
+        for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+            double sn, cs;
+            sincos(TWO_PI * (double)i / (double)ns, &sn, &cs);
+            phs[i] = cd{cs, sn};
+        }
+    }
+    __syncthreads();
+    // point (a, b) -> solved base, component shift, component phase
+    auto src = [&](int a, int b, const TU *&base, int &shift, bool &phase) {
+        shift = (b == g) ? 1 : 0;
+        phase = (a == g);
+        const int a0 = (a == g) ? 0 : a, b0 = (b == g) ? 0 : b;
+        base = U + 2 * (((s * g + a0) * g + b0) * strideU);
+    };
+    const TU *u0, *u1, *u2;
+    int sh0, sh1, sh2;
+    bool ph0, ph1, ph2;
+    const bool has1 = t1 < g, has2 = t2 < g;
+    src(t1, t2, u0, sh0, ph0);
+    src(has1 ? t1 + 1 : t1, t2, u1, sh1, ph1);
+    src(t1, has2 ? t2 + 1 : t2, u2, sh2, ph2);
+    auto comp = [&](const TU *u, int sh, bool ph, int i, int j) -> cd {
+        int ii = i + sh;
+        ii = ii >= ns ? ii - ns : ii;
+        const cx<TU> v = ldx(u, (long)ii * ns + j);
+        cd x{(double)v.x, (double)v.y};
+        if (ph) x = cmul(phs[i], x);
+        return x;
+    };
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+        cd a1{0.0, 0.0}, a2{0.0, 0.0};
+        for (int i = 0; i < ns; i++) {
+            const cd x = comp(u0, sh0, ph0, i, j);
+            if (has1) cfmac(a1, x, comp(u1, sh1, ph1, i, j));
+            if (has2) cfmac(a2, x, comp(u2, sh2, ph2, i, j));
+        }
+        st(Lk1, kp * ns + j, a1);
+        st(Lk2, kp * ns + j, a2);
+    }
+}
+
 // plaquettes + energies: grid (S), block 256 (states).
-// mode 0 = rlm (+1/n_phi, chern += t), mode 1 = hlm (chern -= t).
+// mode 0 = rlm (+1/n_phi, chern += t), mode 1 = hlm (chern -= t), mode 2 = rlm with
+// eigenvalues stored for the solved points only (theta < g; the theta = 2 pi points
+// have the spectra of their theta = 0 counterparts), kp_s = (s*g + t1)*g + t2.
 __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ Lk1, const double *__restrict__ Lk2,
                                                      const double *__restrict__ Wv, int ns, int n_mesh, int L2n,
                                                      int mode, double n_phi_d, double *__restrict__ energy,
@@ -72,6 +134,9 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ 
             const long lb = kbase + (long)t1 * L2n;
             if (mode == 0) {
                 for (int t2 = 0; t2 <= g; t2++) e += Wv[(lb + t2) * ns + j];
+            } else if (mode == 2) {
+                const long sb = ((long)s * g + (t1 % g)) * g;
+                for (int t2 = 0; t2 <= g; t2++) e += Wv[(sb + t2 % g) * ns + j];
             } else {
                 for (int t2 = 0; t2 < g; t2++) e += Wv[(lb + t2) * ns + j];
                 e += Wv[lb * ns + j];
@@ -96,11 +161,11 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ 
                 prod = cmul(prod, l4);
                 const double ph = atan2(prod.y, prod.x);
                 const double tth = -ph / TWO_PI;
-                if (mode == 0) c += tth; else c += -tth;
+                if (mode != 1) c += tth; else c += -tth;
                 mg = fmin(mg, PI - fabs(ph));
             }
         }
-        if (mode == 0) c += 1.0 / n_phi_d;
+        if (mode != 1) c += 1.0 / n_phi_d;
         energy[(long)s * ns + j] = e;
         chern[(long)s * ns + j] = c;
         if (margin) margin[(long)s * ns + j] = mg;
@@ -121,6 +186,21 @@ int launch_links_f32(const float *U, long strideU, int dim_vec, int ns, int line
     if (nkp <= 0) return 0;
     hipLaunchKernelGGL(links_kernel<float>, dim3((unsigned)nkp), dim3(256), 0, stream, U, strideU, dim_vec, ns, lines,
                        L2n, nlink2, kp0, Lk1, Lk2);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_links_derived(const void *U, bool fp32, long strideU, int ns, int g, int n_samples, double *Lk1,
+                         double *Lk2, hipStream_t stream) {
+    const long nkp = (long)n_samples * (g + 1) * (g + 1);
+    if (nkp <= 0) return 0;
+    if (ns > 512 || g < 1) return -22;   // phase table size
+    if (fp32)
+        hipLaunchKernelGGL(links_derived_kernel<float>, dim3((unsigned)nkp), dim3(256), 0, stream,
+                           static_cast<const float *>(U), strideU, ns, g, Lk1, Lk2);
+    else
+        hipLaunchKernelGGL(links_derived_kernel<double>, dim3((unsigned)nkp), dim3(256), 0, stream,
+                           static_cast<const double *>(U), strideU, ns, g, Lk1, Lk2);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -187,9 +187,13 @@ __global__ void wband_kernel(const double *__restrict__ V, const double *__restr
 
 // H[kp][j][b] = sum_m E[t2][m][j] Wb[s,t1][m][b] on the FP64 matrix cores: one
 // workgroup per k-point, 16x16 output tiles dealt to the 4 waves, K = m in steps of
-// 4. Output kp is local (blockIdx.x); the k-point index is kp0 + blockIdx.x.
+// 4. Output kp is local (blockIdx.x); the k-point index is kp0 + blockIdx.x over a
+// mesh of pl points per line (theta_1 = 0..pl-1 and theta_2 = 0..pl-1 per sample;
+// pl = g for the solved points, g+1 for the whole mesh). Wb is laid out per
+// (s, theta_1) with g+1 lines per sample.
 __global__ __launch_bounds__(256) void hband_kernel(const double *__restrict__ E, const double *__restrict__ Wb, int N,
-                                                    int d, int nb, int n_mesh, long kp0, double *__restrict__ H) {
+                                                    int d, int nb, int n_mesh, int pl, long kp0,
+                                                    double *__restrict__ H) {
     using M = Mf<double>;
     using v4 = M::v4;
     const int lane = threadIdx.x & (kWave - 1);
@@ -197,9 +201,9 @@ __global__ __launch_bounds__(256) void hband_kernel(const double *__restrict__ E
     const int li = lane & 15, lq = lane >> 4;
     const long kp = kp0 + blockIdx.x;
     const int g1 = n_mesh + 1;
-    const long s = kp / ((long)g1 * g1);
-    const int rem = (int)(kp % ((long)g1 * g1));
-    const int t1 = rem / g1, t2 = rem % g1;
+    const long s = kp / ((long)pl * pl);
+    const int rem = (int)(kp % ((long)pl * pl));
+    const int t1 = rem / pl, t2 = rem % pl;
     const double *Et = E + 2L * t2 * d * N;
     const double *Wz = Wb + 2L * (s * g1 + t1) * d * nb;
     double *Hk = H + 2L * blockIdx.x * N * nb;
@@ -253,11 +257,11 @@ int launch_wband(const double *V, const double *cmt, int n_phi, int d, int off, 
     return 0;
 }
 
-int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, int n_mesh, long kp0, long nkp,
-                 double *H, hipStream_t stream) {
-    if (nkp <= 0) return 0;
+int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, int n_mesh, int pl, long kp0,
+                 long nkp, double *H, hipStream_t stream) {
+    if (nkp <= 0 || pl < 1 || pl > n_mesh + 1) return nkp <= 0 ? 0 : -22;
     hipLaunchKernelGGL(hband_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, E, Wb, n_phi, d,
-                       band_width(n_phi, off), n_mesh, kp0, H);
+                       band_width(n_phi, off), n_mesh, pl, kp0, H);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -15,8 +15,9 @@ int launch_assemble(const double *E, const double *Wt, int n_phi, int d, int off
 int band_width(int N, int off);
 int launch_wband(const double *V, const double *cmt, int n_phi, int d, int off, int n_mesh, int n_samples, double *Wb,
                  hipStream_t stream);
-int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, int n_mesh, long kp0, long nkp,
-                 double *H, hipStream_t stream);
+// pl: points per line of the k-point mesh enumerated (g: solved points, g+1: all)
+int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, int n_mesh, int pl, long kp0,
+                 long nkp, double *H, hipStream_t stream);
 int launch_expand_band(const double *H, int n_phi, int off, long nkp, double *A, long strideA, hipStream_t stream);
 // FP64 Rayleigh quotients of FP32 eigenvectors against the FP64 matrices (circular
 // band |c - r| <= bw), dense A or, with Hb, the band form of offset bw.

@@ -13,10 +13,12 @@ pytestmark = pytest.mark.gpu
 E_RTOL = 1e-10
 
 
-def _cmp_sample(e, c, ref_e, ref_c, tag, int_tol=1e-6):
-    """int_tol bounds |C - round(C)| per state: 1e-6 for FP64 eigenvectors; the mixed path's
-    FP32 eigenvectors leave ~1e-5 (the theta = 2 pi edge of the twist mesh is a separate
-    solve, not an identified copy, so the plaquette sum does not telescope exactly)."""
+def _cmp_sample(e, c, ref_e, ref_c, tag, int_tol=1e-9):
+    """int_tol bounds |C - round(C)| per state. The plans derive the theta = 2 pi edges of
+    the twist mesh from the theta = 0 vectors by the exact boundary relation, so the
+    plaquette sum telescopes to an integer up to the rounding of the atan2 sums (~1e-15)
+    in both precisions; the reference solves the edges separately and leaves ~1e-13
+    (FP64) or ~1e-5 (FP32 vectors)."""
     ref_e = np.asarray(ref_e)
     ref_c = np.asarray(ref_c)
     scale = np.abs(ref_e).max()
@@ -55,8 +57,7 @@ def test_uneven_sizes_vs_live_oracle(oracle, n_phi, g, seed, precision):
     r = oracle.rlm_sample(n_phi, 1.0, 16.0, g, seed, 0, threads=8)
     # the integer comparison is meaningful while every plaquette phase stays clear of +-pi
     assert m.min() > 1e-3, m.min()
-    # mixed: FP32 eigenvector error grows with n; 5e-4 still leaves the rounding unambiguous
-    _cmp_sample(e[0], c[0], r.energy, r.chern, (n_phi, g, precision), int_tol=1e-6 if prec == HIP_DIAG_FP64 else 5e-4)
+    _cmp_sample(e[0], c[0], r.energy, r.chern, (n_phi, g, precision))
     plan.close()
 
 
@@ -114,7 +115,7 @@ def test_full_size_properties():
     e, c, m = plan.run(4, 4)
     st = plan.stage_times()
     assert list(st) == list(RlmPlan.STAGES) and all(v >= 0 for v in st.values()), st
-    assert np.all(np.abs(c - np.rint(c)) < 1e-6)
+    assert np.all(np.abs(c - np.rint(c)) < 1e-9)
     assert np.all(np.rint(c).sum(axis=1) == 1)
     assert np.all(m > 0)
     plan.close()
@@ -136,6 +137,35 @@ def test_mixed_precision_fixtures(fixtures, name):
     res = plan.max_residual()
     print(f"  max FP64 residual of FP32 eigenvectors: {res:.3e}; min plaquette margin {m.min():.3e}", flush=True)
     for k, row in enumerate(fx["samples"]):
-        _cmp_sample(e[k], c[k], row["energy"], row["chern"], (name, "mixed", row["sample"]), int_tol=1e-4)
+        _cmp_sample(e[k], c[k], row["energy"], row["chern"], (name, "mixed", row["sample"]))
     assert res < 1e-3
     plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n_phi,g,seed", [(16, 1, 1), (64, 8, 11), (256, 4, 7)])
+def test_derived_edges_match_full_mesh(monkeypatch, n_phi, g, seed, precision):
+    """Plans diagonalise the g^2 points theta_1, theta_2 < g and derive the theta = 2 pi
+    edges (u(t1, g)[i] = u(t1, 0)[(i+1) % N], u(g, t2)[i] = e^{i 2 pi i/N} u(0, t2)[i]);
+    RLM_MESH=full solves all (g+1)^2 points as the reference does. Same inputs, both
+    ways: energies within 1e-12 relative, identical Chern integers, Chern values within
+    1e-9 (FP64) / 1e-3 (mixed: the separately solved FP32 edges leave ~1e-4 at n = 256)
+    of each other, and the derived-edge values integral to 1e-9 (the plaquette sum
+    telescopes). g = 1 derives three of the four points of each sample."""
+    from rlm_amd import RlmPlan, HIP_DIAG_FP64, HIP_DIAG_MIXED
+    print(f"[test_derived_edges_match_full_mesh] n={n_phi} g={g} seed={seed} {precision}", flush=True)
+    prec = HIP_DIAG_MIXED if precision == "mixed" else HIP_DIAG_FP64
+    plan = RlmPlan(n_phi, 1.0, 16.0, g, seed, samples_per_batch=2, precision=prec)
+    e, c, m = plan.run(0, 2)
+    plan.close()
+    monkeypatch.setenv("RLM_MESH", "full")
+    plan = RlmPlan(n_phi, 1.0, 16.0, g, seed, samples_per_batch=2, precision=prec)
+    ef, cf, mf = plan.run(0, 2)
+    plan.close()
+    assert np.abs(e - ef).max() <= 1e-12 * np.abs(ef).max(), np.abs(e - ef).max()
+    assert np.array_equal(np.rint(c), np.rint(cf))
+    tol = 1e-9 if prec == HIP_DIAG_FP64 else 1e-3
+    assert np.abs(c - cf).max() < tol, np.abs(c - cf).max()
+    assert np.abs(c - np.rint(c)).max() < 1e-9, np.abs(c - np.rint(c)).max()
+    print(f"  max |C - C_full| {np.abs(c - cf).max():.3e}; max |C - rint C| derived "
+          f"{np.abs(c - np.rint(c)).max():.3e}, full {np.abs(cf - np.rint(cf)).max():.3e}", flush=True)
