@@ -720,8 +720,10 @@ __device__ inline T sum_lanes_345(T v) {
     return v;
 }
 
-// One tile of the fused pass of step i (see above). diag (wave-uniform): the tile
-// straddles the diagonal, so entries above it stay zero and diagonal entries real.
+// One tile of the fused pass of step i (see above). The vectors are indexed by
+// absolute row and are zero outside their support (v_i, w_i: rows i+1..n-1), so the
+// pending rank-2 update leaves every entry outside the trailing block unchanged and
+// the stale entries left of it add nothing to y = A v_i: no per-entry masks.
 // Storage of one lane's 4x4 sub-block: registers, or LDS (entry k of lane l at
 // p[k * 64], p = tile base + l, conflict-free).
 template <typename T>
@@ -737,61 +739,119 @@ struct LdsSub {
     __device__ void set(int k, cx<T> x) { p[k * kWave] = x; }
 };
 
+// x -= a * b
+template <typename T>
+__device__ __forceinline__ void fms_(cx<T> &x, cx<T> a, cx<T> b) {
+    x.x = fma(-a.x, b.x, x.x);
+    x.x = fma(a.y, b.y, x.x);
+    x.y = fma(-a.x, b.y, x.y);
+    x.y = fma(-a.y, b.x, x.y);
+}
+
+// Tile strictly below the diagonal: A(r,c) -= v_r w_c^* + w_r v_c^* (step i-1), then
+// y_r += A(r,c) v_c and y_c += A(r,c)^* v_r (step i) into the wave's partial y.
 template <typename T, typename Sub>
-__device__ __forceinline__ void resident_tile(Sub a, bool diag, int rb, int cb, int i, int n, int lrg, int lcg,
-                                              const cx<T> *vn, const cx<T> *vo, const cx<T> *wo, cx<T> *yw,
-                                              bool own1, int c1u, cx<T> *colb) {
+__device__ __forceinline__ void resident_tile_off(Sub a, int rb, int cb, int lrg, int lcg, const cx<T> *vn,
+                                                  const cx<T> *vo, const cx<T> *wo, cx<T> *yw) {
     using CT = cx<T>;
-    const int i1 = i + 1;
     const CT z{(T)0, (T)0};
-    const bool lowerSub = !diag || lrg > lcg, lowerEq = !diag || lrg >= lcg, diagSub = diag && lrg == lcg;
     CT vc[4], poc[4], woc[4], cs[4];
-    bool cin[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-        const int c = cb + u;
-        cin[u] = c >= i1 && c < n;
-        const int cc = cin[u] ? c : i1;                 // in-range LDS address either way
-        const CT x0 = vn[cc - i1], x1 = vo[cc - i], x2 = wo[cc - i];
-        vc[u] = cin[u] ? x0 : z;
-        poc[u] = cin[u] ? conjx(x1) : z;
-        woc[u] = cin[u] ? conjx(x2) : z;
+        vc[u] = vn[cb + u];
+        poc[u] = conjx(vo[cb + u]);
+        woc[u] = conjx(wo[cb + u]);
         cs[u] = z;
     }
 #pragma unroll
     for (int ru = 0; ru < 4; ru++) {
         const int r = rb + ru;
-        const bool rin = r >= i1 && r < n;
-        const int rr = rin ? r : i1;
-        const CT y0 = vn[rr - i1], y1 = vo[rr - i], y2 = wo[rr - i];
-        const CT vr = rin ? y0 : z, por = rin ? y1 : z, wor = rin ? y2 : z;
-        CT rs = z, xc = z;
+        const CT vr = vn[r], por = vo[r], wor = wo[r];
+        CT rs = z;
 #pragma unroll
         for (int cu = 0; cu < 4; cu++) {
             CT x = a.get(ru * 4 + cu);
-            const CT xn = x - (por * woc[cu] + wor * poc[cu]);
-            CT vre = vr;
-            x = ((ru >= cu) ? lowerEq : lowerSub) ? xn : x;
-            if (ru == cu) {
-                x.y = diagSub ? (T)0 : x.y;
-                vre = diagSub ? z : vr;                      // the diagonal enters y once
-            }
+            fms_(x, por, woc[cu]);
+            fms_(x, wor, poc[cu]);
             a.set(ru * 4 + cu, x);
-            xc = (cu == c1u) ? x : xc;                       // column i+1 (selected, never indexed)
             fmax_(rs, x, vc[cu]);                            // y_r += A(r,c) v_c
-            fmacx(cs[cu], x, vre);                           // y_c += conj(A(r,c)) v_r
+            fmacx(cs[cu], x, vr);                            // y_c += conj(A(r,c)) v_r
         }
-        if (own1 && rin) colb[r] = xc;
         rs.x = sum_lanes_345(rs.x);
         rs.y = sum_lanes_345(rs.y);
-        if (lcg == 0 && rin) yw[r] = yw[r] + rs;
+        if (lcg == 0) yw[r] = yw[r] + rs;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         cs[u].x = sum_lanes_012(cs[u].x);
         cs[u].y = sum_lanes_012(cs[u].y);
-        if (lrg == 0 && cin[u]) yw[cb + u] = yw[cb + u] + cs[u];
+        if (lrg == 0) yw[cb + u] = yw[cb + u] + cs[u];
     }
+}
+
+// Diagonal tile, held as the full Hermitian 32x32 block (both triangles, real
+// diagonal): the same update, then y_r += sum_c A(r,c) v_c over the whole block.
+template <typename T>
+__device__ __forceinline__ void resident_tile_diag(cx<T> (&a)[16], int rb, int cb, int lrg, int lcg,
+                                                   const cx<T> *vn, const cx<T> *vo, const cx<T> *wo, cx<T> *yw) {
+    using CT = cx<T>;
+    const CT z{(T)0, (T)0};
+    const bool dsub = lrg == lcg;
+    CT vc[4], poc[4], woc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        vc[u] = vn[cb + u];
+        poc[u] = conjx(vo[cb + u]);
+        woc[u] = conjx(wo[cb + u]);
+    }
+#pragma unroll
+    for (int ru = 0; ru < 4; ru++) {
+        const int r = rb + ru;
+        const CT por = vo[r], wor = wo[r];
+        CT rs = z;
+#pragma unroll
+        for (int cu = 0; cu < 4; cu++) {
+            CT x = a[ru * 4 + cu];
+            fms_(x, por, woc[cu]);
+            fms_(x, wor, poc[cu]);
+            if (ru == cu) x.y = dsub ? (T)0 : x.y;
+            a[ru * 4 + cu] = x;
+            fmax_(rs, x, vc[cu]);
+        }
+        rs.x = sum_lanes_345(rs.x);
+        rs.y = sum_lanes_345(rs.y);
+        if (lcg == 0) yw[r] = yw[r] + rs;
+    }
+}
+
+// column c (lane group lcg == (c % 32) / 4, entry c % 4, wave-uniform) of one tile to colb
+// (0/1 weights rather than branches or selects, which the compiler folds into a
+// dynamically indexed tile and moves the register tiles to scratch)
+template <typename T, typename Sub>
+__device__ __forceinline__ void resident_capture(Sub a, int rb, int cu, cx<T> *colb) {
+    T m[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) m[k] = (cu == k) ? (T)1 : (T)0;
+#pragma unroll
+    for (int ru = 0; ru < 4; ru++) {
+        cx<T> x{(T)0, (T)0};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const cx<T> e = a.get(ru * 4 + k);
+            x.x = fma(m[k], e.x, x.x);
+            x.y = fma(m[k], e.y, x.y);
+        }
+        colb[rb + ru] = x;
+    }
+}
+
+// one of NQ per-lane values by a wave-uniform index (no dynamic register indexing)
+template <typename T, int NQ>
+__device__ __forceinline__ cx<T> pick_q(const cx<T> (&v)[NQ], int q) {
+    cx<T> r = v[0];
+#pragma unroll
+    for (int k = 1; k < NQ; k++) r = (q == k) ? v[k] : r;
+    return r;
 }
 
 // Hb (optional): the matrices in the band form of offset hb_off (complex double,
@@ -854,6 +914,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
             const cd h = ld(Hk, (long)c * hnb + bb);
             return CT{(T)h.x, r == c ? (T)0 : (T)h.y};
         };
+        // diagonal tiles hold both triangles: A(r, c) = conj(A(c, r)) above the diagonal
         CT a[NS][16];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
@@ -863,7 +924,10 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
 #pragma unroll
                 for (int ru = 0; ru < 4; ru++) {
                     const int r = rb + ru, c = cb + cu;
-                    a[s][ru * 4 + cu] = (tR[s] >= 0 && r < n && r >= c) ? lower_entry(r, c) : CT{(T)0, (T)0};
+                    CT x{(T)0, (T)0};
+                    if (tR[s] >= 0 && r < n && c < n) x = r >= c ? lower_entry(r, c) : conjx(lower_entry(c, r));
+                    if (r == c) x.y = (T)0;
+                    a[s][ru * 4 + cu] = x;
                 }
         }
         if (NL > 0 && lR >= 0) {
@@ -877,7 +941,9 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                 }
         }
         for (int e = t; e < NMAX; e += NT) {
+            vbuf[0][e] = CT{(T)0, (T)0};
             vbuf[1][e] = CT{(T)0, (T)0};
+            wbuf[0][e] = CT{(T)0, (T)0};
             wbuf[1][e] = CT{(T)0, (T)0};
         }
         // column 0 to LDS
@@ -900,17 +966,19 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
         }
         __syncthreads();
 
-        // wave 0 only: reflector of step i from cq[q] = column i, row i + lane + 64 q
-        // (updated). Writes v_i (element rr <-> row i+1+rr) to vbuf[slot], beta and the
-        // scaled vector into A(i+1:n, i), d_i, e_i and tau_i.
-        auto reflector = [&](int i, const CT *cq, int slot, CT &tau) {
-            const T dii = __shfl(cq[0].x, 0, kWave);
-            const T ar = __shfl(cq[0].x, 1, kWave), ai = __shfl(cq[0].y, 1, kWave);
+        // wave 0 only: reflector of step i from cq[q] = column i at row lane + 64 q
+        // (updated; rows >= i valid). Writes v_i (absolute rows, zero outside i+1..n-1)
+        // to vbuf[slot], beta and the scaled vector into A(i+1:n, i), d_i, e_i and tau_i.
+        auto reflector = [&](int i, const CT (&cq)[NQ], int slot, CT &tau) {
+            const int j1 = i + 1;
+            const T dii = __shfl(pick_q<T, NQ>(cq, i / kWave).x, i % kWave, kWave);
+            const CT a1v = pick_q<T, NQ>(cq, j1 / kWave);
+            const T ar = __shfl(a1v.x, j1 % kWave, kWave), ai = __shfl(a1v.y, j1 % kWave, kWave);
             T s2 = 0;
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                const int rr = lane + kWave * q;
-                if (rr >= 2 && i + rr < n) s2 += abs2x(cq[q]);
+                const int r = lane + kWave * q;
+                if (r > j1 && r < n) s2 += abs2x(cq[q]);
             }
             const T nrm2 = sum_lanes_012(sum_lanes_345(s2));   // every lane gets the same sum
             T beta, taur, taui, sr, si;
@@ -926,15 +994,15 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                 sr = dr * inv; si = -di * inv;
             }
             const CT scal{sr, si};
-            CT *vn = vbuf[slot];
+            CT *vw = vbuf[slot];
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                const int rr = lane + kWave * q, r = i + rr;
-                if (rr >= 1 && r < n) {
-                    const CT v = (rr == 1) ? CT{(T)1, (T)0} : cq[q] * scal;
-                    vn[rr - 1] = v;
-                    stx(A, (long)i * lda + r, (rr == 1) ? CT{beta, (T)0} : v);
-                }
+                const int r = lane + kWave * q;
+                CT v{(T)0, (T)0};
+                if (r == j1) v = CT{(T)1, (T)0};
+                else if (r > j1 && r < n) v = cq[q] * scal;
+                vw[r] = v;
+                if (r >= j1 && r < n) stx(A, (long)i * lda + r, (r == j1) ? CT{beta, (T)0} : v);
             }
             if (lane == 0) {
                 Dv[i] = (double)dii;
@@ -959,8 +1027,8 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
 
         for (int i = 0; i < n - 1; i++) {
             const int cur = i & 1, i1 = i + 1;
-            const CT *vn = vbuf[cur];                            // v_i: rr <-> row i+1+rr
-            const CT *vo = vbuf[cur ^ 1], *wo = wbuf[cur ^ 1];   // v_{i-1}, w_{i-1}: rr <-> row i+rr
+            const CT *vn = vbuf[cur];                            // v_i (absolute rows)
+            const CT *vo = vbuf[cur ^ 1], *wo = wbuf[cur ^ 1];   // v_{i-1}, w_{i-1}
             CT *yw = yp[wid];
             for (int e = lane; e < NMAX; e += kWave) yw[e] = CT{(T)0, (T)0};
             const int c1t = i1 / TS, c1g = (i1 % TS) / 4, c1u = i1 & 3;   // owner of column i+1
@@ -969,30 +1037,31 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                 if (tR[s] < 0 || tC[s] * TS + TS - 1 < i1) continue;   // wave-uniform
                 int rb = tR[s] * TS + 4 * lrg, cb = tC[s] * TS + 4 * lcg;
                 asm volatile("" : "+v"(rb), "+v"(cb));       // keep per-slot indices out of loop-invariant hoisting
-                const bool own1 = tC[s] == c1t && lcg == c1g;
-                resident_tile<T>(RegSub<T>{a[s]}, tR[s] == tC[s], rb, cb, i, n, lrg, lcg, vn, vo, wo, yw, own1, c1u,
-                                 colb);
+                if (tR[s] == tC[s])
+                    resident_tile_diag<T>(a[s], rb, cb, lrg, lcg, vn, vo, wo, yw);
+                else
+                    resident_tile_off<T>(RegSub<T>{a[s]}, rb, cb, lrg, lcg, vn, vo, wo, yw);
+                if (tC[s] == c1t && lcg == c1g) resident_capture<T>(RegSub<T>{a[s]}, rb, c1u, colb);
             }
             if (NL > 0 && lR >= 0 && i1 < TS) {
                 const int rb = lR * TS + 4 * lrg, cb = 4 * lcg;
-                resident_tile<T>(LdsSub<T>{lsub}, false, rb, cb, i, n, lrg, lcg, vn, vo, wo, yw,
-                                 c1t == 0 && lcg == c1g, c1u, colb);
+                resident_tile_off<T>(LdsSub<T>{lsub}, rb, cb, lrg, lcg, vn, vo, wo, yw);
+                if (c1t == 0 && lcg == c1g) resident_capture<T>(LdsSub<T>{lsub}, rb, c1u, colb);
             }
             __syncthreads();
             if (wid == 0) {
-                const int m = n - i1;
                 CT p[NQ], cq[NQ];
                 CT dl{(T)0, (T)0};
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
-                    const int rr = lane + kWave * q;
+                    const int r = lane + kWave * q;
                     p[q] = CT{(T)0, (T)0};
-                    if (rr < m) {
+                    if (r >= i1 && r < n) {
                         CT y{(T)0, (T)0};
 #pragma unroll
-                        for (int w = 0; w < NW; w++) y = y + yp[w][i1 + rr];
+                        for (int w = 0; w < NW; w++) y = y + yp[w][r];
                         p[q] = tau * y;
-                        fmacx(dl, p[q], vn[rr]);
+                        fmacx(dl, p[q], vn[r]);
                     }
                 }
                 dl.x = sum_lanes_012(sum_lanes_345(dl.x));
@@ -1001,26 +1070,25 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                 CT *wn = wbuf[cur];
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
-                    const int rr = lane + kWave * q;
-                    if (rr < m) {
-                        p[q] = p[q] + alpha2 * vn[rr];                  // w_i
-                        wn[rr] = p[q];
-                    }
+                    const int r = lane + kWave * q;
+                    p[q] = (r >= i1 && r < n) ? p[q] + alpha2 * vn[r] : CT{(T)0, (T)0};   // w_i
+                    wn[r] = p[q];
                 }
-                const CT w0{__shfl(p[0].x, 0, kWave), __shfl(p[0].y, 0, kWave)};
+                const CT w0s = pick_q<T, NQ>(p, i1 / kWave);
+                const CT w0{__shfl(w0s.x, i1 % kWave, kWave), __shfl(w0s.y, i1 % kWave, kWave)};
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
-                    const int rr = lane + kWave * q;
+                    const int r = lane + kWave * q;
                     cq[q] = CT{(T)0, (T)0};
-                    if (rr < m) {
-                        CT x = colb[i1 + rr];
-                        x = x - (vn[rr] * conjx(w0) + p[q]);            // v(0) = 1
-                        if (rr == 0) x.y = (T)0;
+                    if (r >= i1 && r < n) {
+                        CT x = colb[r];
+                        x = x - (vn[r] * conjx(w0) + p[q]);            // v_i(i+1) = 1
+                        if (r == i1) x.y = (T)0;
                         cq[q] = x;
                     }
                 }
                 if (i1 < n - 1) reflector(i1, cq, cur ^ 1, tau);
-                else if (lane == 0) Dv[n - 1] = (double)cq[0].x;
+                else if (lane == (n - 1) % kWave) Dv[n - 1] = (double)pick_q<T, NQ>(cq, (n - 1) / kWave).x;
             }
             __syncthreads();
         }

@@ -148,7 +148,7 @@ def test_derived_edges_match_full_mesh(monkeypatch, n_phi, g, seed, precision):
     """Plans diagonalise the g^2 points theta_1, theta_2 < g and derive the theta = 2 pi
     edges (u(t1, g)[i] = u(t1, 0)[(i+1) % N], u(g, t2)[i] = e^{i 2 pi i/N} u(0, t2)[i]);
     RLM_MESH=full solves all (g+1)^2 points as the reference does. Same inputs, both
-    ways: energies within 1e-12 relative, identical Chern integers, Chern values within
+    ways: energies within 1e-12 (FP64) / 1e-10 (mixed) relative, identical Chern integers, Chern values within
     1e-9 (FP64) / 1e-3 (mixed: the separately solved FP32 edges leave ~1e-4 at n = 256)
     of each other, and the derived-edge values integral to 1e-9 (the plaquette sum
     telescopes). g = 1 derives three of the four points of each sample."""
@@ -162,7 +162,9 @@ def test_derived_edges_match_full_mesh(monkeypatch, n_phi, g, seed, precision):
     plan = RlmPlan(n_phi, 1.0, 16.0, g, seed, samples_per_batch=2, precision=prec)
     ef, cf, mf = plan.run(0, 2)
     plan.close()
-    assert np.abs(e - ef).max() <= 1e-12 * np.abs(ef).max(), np.abs(e - ef).max()
+    # mixed: Rayleigh quotients of two different FP32 vector sets (error ~ residual^2)
+    e_tol = 1e-12 if prec == HIP_DIAG_FP64 else 1e-10
+    assert np.abs(e - ef).max() <= e_tol * np.abs(ef).max(), np.abs(e - ef).max() / np.abs(ef).max()
     assert np.array_equal(np.rint(c), np.rint(cf))
     tol = 1e-9 if prec == HIP_DIAG_FP64 else 1e-3
     assert np.abs(c - cf).max() < tol, np.abs(c - cf).max()
