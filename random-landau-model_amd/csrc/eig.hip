@@ -680,9 +680,11 @@ __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase,
 
 // Cross-lane moves for the tile reductions: DPP within a 16-lane row, ds_swizzle
 // within 32 lanes (no LDS traffic), bpermute only across the two wave halves.
+// (bound_ctrl set: every source lane of the patterns used here lies inside its row,
+// so it changes no value, and it lets the compiler fold the move into the add)
 template <int CTRL>
 __device__ inline float dpp_mov(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
 __device__ inline double dpp_mov(double v) {
@@ -718,6 +720,50 @@ __device__ inline T sum_lanes_345(T v) {
     v += swz_mov<0x401F>(v);
     v += __shfl_xor(v, 32, kWave);
     return v;
+}
+
+// Lane-swap instructions of gfx950 (VALU, no LDS round trip). Swapping a register
+// with itself, v_permlane16_swap returns rows (0,0,2,2) and (1,1,3,3) of the 16-lane
+// rows, v_permlane32_swap the low half twice and the high half twice: the sum of
+// the two results is the xor-16 resp. xor-32 partner sum, in every lane.
+__device__ inline float sum_xor16(float v) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ inline float sum_xor32(float v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ inline double sum_xor16(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(u >> 32), (unsigned)(u >> 32), false, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0])) +
+           __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+__device__ inline double sum_xor32(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(u >> 32), (unsigned)(u >> 32), false, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0])) +
+           __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+// sum over all 64 lanes in VALU only (the latency-bound single-wave phase)
+template <typename T>
+__device__ inline T sum_wave_valu(T v) {
+    v = sum_lanes_012(v);
+    v += dpp_mov<0x128>(v);
+    return sum_xor32(sum_xor16(v));
+}
+// value of lane l (wave-uniform) in every lane, through a scalar register
+__device__ inline float lane_bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline double lane_bcast(double v, int l) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // One tile of the fused pass of step i (see above). The vectors are indexed by
@@ -767,6 +813,7 @@ __device__ __forceinline__ void resident_tile_off(Sub a, int rb, int cb, int lrg
     for (int ru = 0; ru < 4; ru++) {
         const int r = rb + ru;
         const CT vr = vn[r], por = vo[r], wor = wo[r];
+        const T nvrx = -vr.x;
         CT rs = z;
 #pragma unroll
         for (int cu = 0; cu < 4; cu++) {
@@ -775,17 +822,20 @@ __device__ __forceinline__ void resident_tile_off(Sub a, int rb, int cb, int lrg
             fms_(x, wor, poc[cu]);
             a.set(ru * 4 + cu, x);
             fmax_(rs, x, vc[cu]);                            // y_r += A(r,c) v_c
-            fmacx(cs[cu], x, vr);                            // y_c += conj(A(r,c)) v_r
+            cs[cu].x = fma(x.x, vr.x, cs[cu].x);             // y_c += conj(A(r,c)) v_r
+            cs[cu].x = fma(x.y, vr.y, cs[cu].x);
+            cs[cu].y = fma(x.x, vr.y, cs[cu].y);
+            cs[cu].y = fma(x.y, nvrx, cs[cu].y);
         }
         rs.x = sum_lanes_345(rs.x);
         rs.y = sum_lanes_345(rs.y);
-        if (lcg == 0) yw[r] = yw[r] + rs;
+        yw[r] = yw[r] + rs;            // the 8 lanes of a row hold the same sum: same-value writes
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         cs[u].x = sum_lanes_012(cs[u].x);
         cs[u].y = sum_lanes_012(cs[u].y);
-        if (lrg == 0) yw[cb + u] = yw[cb + u] + cs[u];
+        yw[cb + u] = yw[cb + u] + cs[u];
     }
 }
 
@@ -820,7 +870,7 @@ __device__ __forceinline__ void resident_tile_diag(cx<T> (&a)[16], int rb, int c
         }
         rs.x = sum_lanes_345(rs.x);
         rs.y = sum_lanes_345(rs.y);
-        if (lcg == 0) yw[r] = yw[r] + rs;
+        yw[r] = yw[r] + rs;
     }
 }
 
@@ -854,9 +904,17 @@ __device__ __forceinline__ cx<T> pick_q(const cx<T> (&v)[NQ], int q) {
     return r;
 }
 
+// Off-diagonal tiles left over for LDS when every wave holds NS register tiles
+// (TN tiles per side; DIAG0: slot 0 holds diagonal tiles only).
+__host__ __device__ inline int resident_lds_tiles(int TN, int NS, bool diag0) {
+    const int noff = TN * (TN - 1) / 2;
+    const int cap = 8 * (NS - 1) + (diag0 ? 0 : (8 - TN > 0 ? 8 - TN : 0));
+    return noff > cap ? noff - cap : 0;
+}
+
 // Hb (optional): the matrices in the band form of offset hb_off (complex double,
 // n x band_nb(n, hb_off) per matrix) instead of A; A then only receives the reflectors.
-template <typename T, int NS, int NL>
+template <typename T, int NS, int NL, bool DIAG0>
 __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
                                                              double *__restrict__ Dout, double *__restrict__ Eout,
                                                              T *__restrict__ TAUout, long strideV, int batch,
@@ -870,21 +928,31 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
     const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
     const int lrg = lane & 7, lcg = lane >> 3;
     const int TN = (n + TS - 1) / TS;
-    const int ntiles = TN * (TN + 1) / 2;
-    const int nl = ntiles > NW * NS ? ntiles - NW * NS : 0;    // host guarantees nl <= min(NL, TN)
-    // register tiles: column-major order over the lower tile triangle without the LDS
-    // tiles (column 0, rows TN-nl..TN-1); list position j -> wave j % 8, slot j / 8
+    const int noff = TN * (TN - 1) / 2;                        // off-diagonal tiles
+    const int nl = resident_lds_tiles(TN, NS, DIAG0);          // host guarantees nl <= min(NL, TN - 1)
+    // Diagonal tile (w, w) -> slot 0 of wave w (w < TN). Off-diagonal tiles in
+    // column-major order without the LDS tiles (column 0, rows TN-nl..TN-1): DIAG0,
+    // position j -> wave j % 8, slot 1 + j / 8 (slot 0 holds diagonal tiles only, so
+    // its code path is fixed at compile time); otherwise slot 0 of the waves
+    // TN..7 first, then slots 1.. of every wave.
     int tR[NS], tC[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        int j = wid + NW * s;
         tR[s] = -1;
         tC[s] = -1;
-        if (j < ntiles - nl) {
-            int c0 = 0, len = TN - nl;                   // column 0 keeps TN - nl register tiles
-            while (j >= len) { j -= len; c0++; len = TN - c0; }
+        int j;
+        if (s == 0) {
+            if (wid < TN) { tR[0] = tC[0] = wid; continue; }
+            if (DIAG0) continue;
+            j = wid - TN;
+        } else {
+            j = (DIAG0 ? 0 : NW - TN) + wid + NW * (s - 1);
+        }
+        if (j < noff - nl) {
+            int c0 = 0, len = TN - 1 - nl;               // column 0 keeps TN - 1 - nl register tiles
+            while (j >= len) { j -= len; c0++; len = TN - 1 - c0; }
             tC[s] = c0;
-            tR[s] = c0 + j;
+            tR[s] = c0 + 1 + j;
         }
     }
     // LDS tile of this wave (at most one): tile row lR of column 0, slot lq
@@ -971,16 +1039,16 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
         // to vbuf[slot], beta and the scaled vector into A(i+1:n, i), d_i, e_i and tau_i.
         auto reflector = [&](int i, const CT (&cq)[NQ], int slot, CT &tau) {
             const int j1 = i + 1;
-            const T dii = __shfl(pick_q<T, NQ>(cq, i / kWave).x, i % kWave, kWave);
+            const T dii = lane_bcast(pick_q<T, NQ>(cq, i / kWave).x, i % kWave);
             const CT a1v = pick_q<T, NQ>(cq, j1 / kWave);
-            const T ar = __shfl(a1v.x, j1 % kWave, kWave), ai = __shfl(a1v.y, j1 % kWave, kWave);
+            const T ar = lane_bcast(a1v.x, j1 % kWave), ai = lane_bcast(a1v.y, j1 % kWave);
             T s2 = 0;
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
                 const int r = lane + kWave * q;
                 if (r > j1 && r < n) s2 += abs2x(cq[q]);
             }
-            const T nrm2 = sum_lanes_012(sum_lanes_345(s2));   // every lane gets the same sum
+            const T nrm2 = sum_wave_valu(s2);                   // every lane gets the same sum
             T beta, taur, taui, sr, si;
             if (nrm2 == (T)0 && ai == (T)0) {
                 beta = ar; taur = 0; taui = 0; sr = 0; si = 0;
@@ -1037,7 +1105,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                 if (tR[s] < 0 || tC[s] * TS + TS - 1 < i1) continue;   // wave-uniform
                 int rb = tR[s] * TS + 4 * lrg, cb = tC[s] * TS + 4 * lcg;
                 asm volatile("" : "+v"(rb), "+v"(cb));       // keep per-slot indices out of loop-invariant hoisting
-                if (tR[s] == tC[s])
+                if (s == 0 && (DIAG0 || tR[0] == tC[0]))
                     resident_tile_diag<T>(a[s], rb, cb, lrg, lcg, vn, vo, wo, yw);
                 else
                     resident_tile_off<T>(RegSub<T>{a[s]}, rb, cb, lrg, lcg, vn, vo, wo, yw);
@@ -1064,8 +1132,8 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                         fmacx(dl, p[q], vn[r]);
                     }
                 }
-                dl.x = sum_lanes_012(sum_lanes_345(dl.x));
-                dl.y = sum_lanes_012(sum_lanes_345(dl.y));
+                dl.x = sum_wave_valu(dl.x);
+                dl.y = sum_wave_valu(dl.y);
                 const CT alpha2 = (tau * dl) * (T)(-0.5);
                 CT *wn = wbuf[cur];
 #pragma unroll
@@ -1075,7 +1143,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                     wn[r] = p[q];
                 }
                 const CT w0s = pick_q<T, NQ>(p, i1 / kWave);
-                const CT w0{__shfl(w0s.x, i1 % kWave, kWave), __shfl(w0s.y, i1 % kWave, kWave)};
+                const CT w0{lane_bcast(w0s.x, i1 % kWave), lane_bcast(w0s.y, i1 % kWave)};
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
                     const int r = lane + kWave * q;
@@ -2187,14 +2255,14 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
     return 0;
 }
 
-template <typename T, int NS, int NL>
+template <typename T, int NS, int NL, bool DIAG0>
 static int launch_zhetrd_resident_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
                                     int batch, hipStream_t stream, const double *Hb, int hb_off) {
-    const int tn = (n + 31) / 32, ntiles = tn * (tn + 1) / 2;
-    const int nl = ntiles > 8 * NS ? ntiles - 8 * NS : 0;
-    if (n < 1 || n > 256 || nl > NL || (nl > 0 && nl >= tn)) return -22;
-    hipLaunchKernelGGL((zhetrd_resident_kernel<T, NS, NL>), dim3(batch), dim3(512), 0, stream, A, strideA, lda, n, D,
-                       E, TAU, strideV, batch, Hb, hb_off);
+    const int tn = (n + 31) / 32;
+    const int nl = resident_lds_tiles(tn, NS, DIAG0);
+    if (n < 1 || n > 256 || tn > 8 || nl > NL || (nl > 0 && nl > tn - 1)) return -22;
+    hipLaunchKernelGGL((zhetrd_resident_kernel<T, NS, NL, DIAG0>), dim3(batch), dim3(512), 0, stream, A, strideA, lda,
+                       n, D, E, TAU, strideV, batch, Hb, hb_off);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -2231,15 +2299,19 @@ bool chetrd_band_supported(int n) {
 template <typename T>
 static int launch_resident_any(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
                                int batch, hipStream_t stream, const double *Hb, int hb_off) {
-    const int tn = (n + 31) / 32, ntiles = tn * (tn + 1) / 2;
+    const int tn = (n + 31) / 32;
     if constexpr (sizeof(T) == sizeof(float)) {
-        if (ntiles <= 10)
-            return launch_zhetrd_resident_t<T, 1, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb, hb_off);
-        if (ntiles <= 16)
-            return launch_zhetrd_resident_t<T, 2, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb, hb_off);
-        return launch_zhetrd_resident_t<T, 4, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb, hb_off);
+        if (tn <= 4)
+            return launch_zhetrd_resident_t<T, 1, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
+                                                            hb_off);
+        if (tn <= 5)
+            return launch_zhetrd_resident_t<T, 2, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
+                                                            hb_off);
+        return launch_zhetrd_resident_t<T, 4, 4, true>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
+                                                       hb_off);
     } else {
-        return launch_zhetrd_resident_t<T, 1, 4>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb, hb_off);
+        return launch_zhetrd_resident_t<T, 1, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
+                                                        hb_off);
     }
 }
 
