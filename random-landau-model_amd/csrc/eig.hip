@@ -663,20 +663,19 @@ __global__ __launch_bounds__(NT) void zhetrd_lower_kernel(T *__restrict__ Abase,
 // triangle plus the writes of d, e, tau and the reflectors (the kernels above
 // stream the trailing block through memory once per column). One workgroup of
 // 512 threads (8 waves) per matrix. The lower triangle is cut into 32x32 tiles
-// dealt to the waves in column-major order (tile k -> wave k % 8, slot k / 8),
-// and lane (lrg, lcg) = (lane & 7, lane >> 3) of a wave holds the 4x4 sub-block
-// at tile rows 4*lrg.., columns 4*lcg.. of each of its tiles. Step i (reflector
-// v_i known):
+// (diagonal tiles held as full Hermitian blocks) dealt to the waves by
+// resident_deal, and lane (lrg, lcg) = (lane & 7, lane >> 3) of a wave holds the
+// 4x4 sub-block at tile rows 4*lrg.., columns 4*lcg.. of each of its tiles. Step i
+// (reflector v_i known):
 //   all waves: apply the pending rank-2 update of step i-1 to the trailing block,
 //              accumulate y = A22 v_i from the lower triangle (row sums and
 //              conjugate column sums, one LDS vector per wave) and copy column
 //              i+1 to LDS;
 //   wave 0:    w_i = tau_i y - 1/2 tau_i^2 (y^H v_i) v_i; column i+1 -= v_i w_i(0)^*
 //              + w_i v_i(0)^*; reflector of step i+1 from that column (zlarfg).
-// Two barriers per column. NS = tiles per wave held in registers; the nl =
-// ntiles - 8*NS tiles left over (0 <= nl <= NL) are the last nl tiles of tile
-// column 0, kept in LDS by waves (TN - nl .. TN - 1) mod 8 (they are live for the
-// first 32 columns only). n <= 256.
+// Two barriers per column. NS = tiles per wave held in registers; up to NL tiles
+// of tile column 0 that find no register slot are kept in LDS (they are live for
+// the first 32 columns only). n <= 256.
 
 // Cross-lane moves for the tile reductions: DPP within a 16-lane row, ds_swizzle
 // within 32 lanes (no LDS traffic), bpermute only across the two wave halves.
@@ -904,12 +903,59 @@ __device__ __forceinline__ cx<T> pick_q(const cx<T> (&v)[NQ], int q) {
     return r;
 }
 
-// Off-diagonal tiles left over for LDS when every wave holds NS register tiles
-// (TN tiles per side; DIAG0: slot 0 holds diagonal tiles only).
-__host__ __device__ inline int resident_lds_tiles(int TN, int NS, bool diag0) {
-    const int noff = TN * (TN - 1) / 2;
-    const int cap = 8 * (NS - 1) + (diag0 ? 0 : (8 - TN > 0 ? 8 - TN : 0));
-    return noff > cap ? noff - cap : 0;
+// Tile ownership of the register-resident kernel: wave w holds tile (r[w][s], c[w][s])
+// in register slot s (-1: empty) and at most one LDS tile (lr[w], LDS buffer lq[w])
+// of tile column 0.
+struct ResidentDeal {
+    signed char r[8][4], c[8][4], lr[8], lq[8];
+};
+
+// Deals the TN(TN+1)/2 tiles to 8 waves x NS slots. A tile of tile column c is live
+// for the first 32(c+1) columns, so the tiles are placed from the last tile column
+// back, each on the SIMD (waves w and w+4 share one) and then the wave with the
+// least work so far: at every column the live work is spread evenly. Work units:
+// diagonal tile 2 (row sums only), off-diagonal 3. DIAG0: slot 0 holds diagonal
+// tiles only (its code path is fixed at compile time), slots 1.. off-diagonal ones.
+// Off-diagonal tiles of column 0 that find no register slot go to LDS (at most NL,
+// one per wave). Returns false if the tiles do not fit.
+static bool resident_deal(int TN, int NS, bool diag0, int NL, ResidentDeal &d) {
+    memset(&d, -1, sizeof(d));
+    int load[8] = {0, 0, 0, 0, 0, 0, 0, 0}, used[8] = {0, 0, 0, 0, 0, 0, 0, 0}, nlds = 0;
+    bool slot0[8] = {false, false, false, false, false, false, false, false};
+    auto better = [&](int a, int b) {   // is wave a less loaded than wave b
+        if (b < 0) return true;
+        const int sa = load[a % 4] + load[a % 4 + 4], sb = load[b % 4] + load[b % 4 + 4];
+        if (sa != sb) return sa < sb;
+        if (load[a] != load[b]) return load[a] < load[b];
+        return a < b;
+    };
+    for (int c = TN - 1; c >= 0; c--) {
+        for (int r = c; r < TN; r++) {
+            const bool diag = r == c;
+            int best = -1;
+            for (int w = 0; w < 8; w++) {
+                const bool fits = diag0 ? (diag ? !slot0[w] : used[w] < NS - 1) : used[w] < NS;
+                if (fits && better(w, best)) best = w;
+            }
+            if (best >= 0) {
+                int s;
+                if (diag0 && diag) { s = 0; slot0[best] = true; }
+                else s = (diag0 ? 1 : 0) + used[best]++;
+                d.r[best][s] = (signed char)r;
+                d.c[best][s] = (signed char)c;
+                load[best] += diag ? 2 : 3;
+                continue;
+            }
+            if (diag || c != 0 || nlds >= NL) return false;
+            for (int w = 0; w < 8; w++)
+                if (d.lr[w] < 0 && better(w, best)) best = w;
+            if (best < 0) return false;
+            d.lr[best] = (signed char)r;
+            d.lq[best] = (signed char)nlds++;
+            load[best] += 3;
+        }
+    }
+    return true;
 }
 
 // Hb (optional): the matrices in the band form of offset hb_off (complex double,
@@ -918,7 +964,8 @@ template <typename T, int NS, int NL, bool DIAG0>
 __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
                                                              double *__restrict__ Dout, double *__restrict__ Eout,
                                                              T *__restrict__ TAUout, long strideV, int batch,
-                                                             const double *__restrict__ Hb, int hb_off) {
+                                                             const double *__restrict__ Hb, int hb_off,
+                                                             const ResidentDeal deal) {
     using CT = cx<T>;
     constexpr int NT = 512, NW = NT / kWave, TS = 32, NMAX = 256, NQ = NMAX / kWave;
     constexpr int NLS = NL > 0 ? NL : 1;
@@ -927,38 +974,14 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
     const int t = threadIdx.x, lane = t & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
     const int lrg = lane & 7, lcg = lane >> 3;
-    const int TN = (n + TS - 1) / TS;
-    const int noff = TN * (TN - 1) / 2;                        // off-diagonal tiles
-    const int nl = resident_lds_tiles(TN, NS, DIAG0);          // host guarantees nl <= min(NL, TN - 1)
-    // Diagonal tile (w, w) -> slot 0 of wave w (w < TN). Off-diagonal tiles in
-    // column-major order without the LDS tiles (column 0, rows TN-nl..TN-1): DIAG0,
-    // position j -> wave j % 8, slot 1 + j / 8 (slot 0 holds diagonal tiles only, so
-    // its code path is fixed at compile time); otherwise slot 0 of the waves
-    // TN..7 first, then slots 1.. of every wave.
+    // tiles of this wave (host-dealt, see resident_deal)
     int tR[NS], tC[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        tR[s] = -1;
-        tC[s] = -1;
-        int j;
-        if (s == 0) {
-            if (wid < TN) { tR[0] = tC[0] = wid; continue; }
-            if (DIAG0) continue;
-            j = wid - TN;
-        } else {
-            j = (DIAG0 ? 0 : NW - TN) + wid + NW * (s - 1);
-        }
-        if (j < noff - nl) {
-            int c0 = 0, len = TN - 1 - nl;               // column 0 keeps TN - 1 - nl register tiles
-            while (j >= len) { j -= len; c0++; len = TN - 1 - c0; }
-            tC[s] = c0;
-            tR[s] = c0 + 1 + j;
-        }
+        tR[s] = deal.r[wid][s];
+        tC[s] = deal.c[wid][s];
     }
-    // LDS tile of this wave (at most one): tile row lR of column 0, slot lq
-    int lR = -1, lq = 0;
-    for (int q = 0; q < nl; q++)
-        if ((TN - nl + q) % NW == wid) { lR = TN - nl + q; lq = q; }
+    const int lR = NL > 0 ? deal.lr[wid] : -1, lq = lR >= 0 ? deal.lq[wid] : 0;
     CT *lsub = &ltile[lq][lane];
 
     {
@@ -1105,7 +1128,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                 if (tR[s] < 0 || tC[s] * TS + TS - 1 < i1) continue;   // wave-uniform
                 int rb = tR[s] * TS + 4 * lrg, cb = tC[s] * TS + 4 * lcg;
                 asm volatile("" : "+v"(rb), "+v"(cb));       // keep per-slot indices out of loop-invariant hoisting
-                if (s == 0 && (DIAG0 || tR[0] == tC[0]))
+                if (DIAG0 ? s == 0 : tR[s] == tC[s])
                     resident_tile_diag<T>(a[s], rb, cb, lrg, lcg, vn, vo, wo, yw);
                 else
                     resident_tile_off<T>(RegSub<T>{a[s]}, rb, cb, lrg, lcg, vn, vo, wo, yw);
@@ -2259,10 +2282,10 @@ template <typename T, int NS, int NL, bool DIAG0>
 static int launch_zhetrd_resident_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
                                     int batch, hipStream_t stream, const double *Hb, int hb_off) {
     const int tn = (n + 31) / 32;
-    const int nl = resident_lds_tiles(tn, NS, DIAG0);
-    if (n < 1 || n > 256 || tn > 8 || nl > NL || (nl > 0 && nl > tn - 1)) return -22;
+    ResidentDeal deal;
+    if (n < 1 || n > 256 || !resident_deal(tn, NS, DIAG0, NL, deal)) return -22;
     hipLaunchKernelGGL((zhetrd_resident_kernel<T, NS, NL, DIAG0>), dim3(batch), dim3(512), 0, stream, A, strideA, lda,
-                       n, D, E, TAU, strideV, batch, Hb, hb_off);
+                       n, D, E, TAU, strideV, batch, Hb, hb_off, deal);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
