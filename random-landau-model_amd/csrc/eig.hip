@@ -1141,19 +1141,25 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
             }
             __syncthreads();
             if (wid == 0) {
-                CT p[NQ], cq[NQ];
+                // every LDS read up front and unconditional (all rows < NMAX), then
+                // masks by select: one LDS round trip instead of one per branch
+                CT p[NQ], cq[NQ], vv[NQ], cb[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const int r = lane + kWave * q;
+                    CT y = yp[0][r];
+#pragma unroll
+                    for (int w = 1; w < NW; w++) y = y + yp[w][r];
+                    p[q] = y;
+                    vv[q] = vn[r];
+                    cb[q] = colb[r];
+                }
                 CT dl{(T)0, (T)0};
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
                     const int r = lane + kWave * q;
-                    p[q] = CT{(T)0, (T)0};
-                    if (r >= i1 && r < n) {
-                        CT y{(T)0, (T)0};
-#pragma unroll
-                        for (int w = 0; w < NW; w++) y = y + yp[w][r];
-                        p[q] = tau * y;
-                        fmacx(dl, p[q], vn[r]);
-                    }
+                    p[q] = (r >= i1 && r < n) ? tau * p[q] : CT{(T)0, (T)0};
+                    fmacx(dl, p[q], vv[q]);
                 }
                 dl.x = sum_wave_valu(dl.x);
                 dl.y = sum_wave_valu(dl.y);
@@ -1162,7 +1168,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
                     const int r = lane + kWave * q;
-                    p[q] = (r >= i1 && r < n) ? p[q] + alpha2 * vn[r] : CT{(T)0, (T)0};   // w_i
+                    p[q] = (r >= i1 && r < n) ? p[q] + alpha2 * vv[q] : CT{(T)0, (T)0};   // w_i
                     wn[r] = p[q];
                 }
                 const CT w0s = pick_q<T, NQ>(p, i1 / kWave);
@@ -1170,13 +1176,9 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
                     const int r = lane + kWave * q;
-                    cq[q] = CT{(T)0, (T)0};
-                    if (r >= i1 && r < n) {
-                        CT x = colb[r];
-                        x = x - (vn[r] * conjx(w0) + p[q]);            // v_i(i+1) = 1
-                        if (r == i1) x.y = (T)0;
-                        cq[q] = x;
-                    }
+                    CT x = cb[q] - (vv[q] * conjx(w0) + p[q]);          // v_i(i+1) = 1
+                    if (r == i1) x.y = (T)0;
+                    cq[q] = (r >= i1 && r < n) ? x : CT{(T)0, (T)0};
                 }
                 if (i1 < n - 1) reflector(i1, cq, cur ^ 1, tau);
                 else if (lane == (n - 1) % kWave) Dv[n - 1] = (double)pick_q<T, NQ>(cq, (n - 1) / kWave).x;
