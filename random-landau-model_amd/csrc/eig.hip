@@ -1075,6 +1075,17 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
             T beta, taur, taui, sr, si;
             if (nrm2 == (T)0 && ai == (T)0) {
                 beta = ar; taur = 0; taui = 0; sr = 0; si = 0;
+            } else if constexpr (sizeof(T) == sizeof(float)) {
+                // hardware sqrt and reciprocal (about 1 ulp; the FP32 reduction's
+                // backward error is O(n eps) anyway): a short dependent chain
+                const T nrm = __builtin_amdgcn_sqrtf(ar * ar + ai * ai + nrm2);
+                beta = (ar >= (T)0) ? -nrm : nrm;
+                const T rb = __builtin_amdgcn_rcpf(beta);
+                taur = (beta - ar) * rb;
+                taui = -ai * rb;
+                const T dr = ar - beta, di = ai;
+                const T inv = __builtin_amdgcn_rcpf(dr * dr + di * di);
+                sr = dr * inv; si = -di * inv;
             } else {
                 const T nrm = sqrt(ar * ar + ai * ai + nrm2);
                 beta = (ar >= (T)0) ? -nrm : nrm;
