@@ -971,6 +971,8 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
     constexpr int NLS = NL > 0 ? NL : 1;
     __shared__ CT vbuf[2][NMAX], wbuf[2][NMAX], colb[NMAX], yp[NW][NMAX];
     __shared__ CT ltile[NLS][16 * kWave];
+    __shared__ T sd[NMAX], se[NMAX];      // d, e and tau, written to memory at the end
+    __shared__ CT stau[NMAX];
     const int t = threadIdx.x, lane = t & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
     const int lrg = lane & 7, lcg = lane >> 3;
@@ -1104,13 +1106,11 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                 if (r == j1) v = CT{(T)1, (T)0};
                 else if (r > j1 && r < n) v = cq[q] * scal;
                 vw[r] = v;
-                if (r >= j1 && r < n) stx(A, (long)i * lda + r, (r == j1) ? CT{beta, (T)0} : v);
             }
-            if (lane == 0) {
-                Dv[i] = (double)dii;
-                Ev[i] = (double)beta;
-                Tv[2 * i] = taur;
-                Tv[2 * i + 1] = taui;
+            if (lane == 0) {   // the reflector goes to memory from the next pass (off this wave's path)
+                sd[i] = dii;
+                se[i] = beta;
+                stau[i] = CT{taur, taui};
             }
             tau = CT{taur, taui};
         };
@@ -1134,6 +1134,13 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
             CT *yw = yp[wid];
             for (int e = lane; e < NMAX; e += kWave) yw[e] = CT{(T)0, (T)0};
             const int c1t = i1 / TS, c1g = (i1 % TS) / 4, c1u = i1 & 3;   // owner of column i+1
+            if (wid == (i + 1) % NW) {        // reflector i into A(i+1:n, i) (beta at i+1), a wave in turn
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const int r = lane + kWave * q;
+                    if (r > i && r < n) stx(A, (long)i * lda + r, r == i1 ? CT{se[i], (T)0} : vn[r]);
+                }
+            }
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 if (tR[s] < 0 || tC[s] * TS + TS - 1 < i1) continue;   // wave-uniform
@@ -1192,9 +1199,17 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                     cq[q] = (r >= i1 && r < n) ? x : CT{(T)0, (T)0};
                 }
                 if (i1 < n - 1) reflector(i1, cq, cur ^ 1, tau);
-                else if (lane == (n - 1) % kWave) Dv[n - 1] = (double)pick_q<T, NQ>(cq, (n - 1) / kWave).x;
+                else if (lane == (n - 1) % kWave) sd[n - 1] = pick_q<T, NQ>(cq, (n - 1) / kWave).x;
             }
             __syncthreads();
+        }
+        for (int e = t; e < n; e += NT) {
+            Dv[e] = (double)sd[e];
+            if (e < n - 1) {
+                Ev[e] = (double)se[e];
+                Tv[2 * e] = stau[e].x;
+                Tv[2 * e + 1] = stau[e].y;
+            }
         }
     }
 }
