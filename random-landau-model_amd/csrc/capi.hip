@@ -25,11 +25,20 @@ constexpr int kGridTrd = 1024;   // persistent workgroups for the tridiagonalisa
 constexpr int kGridTail = 512;   // persistent workgroups for D&C + back-transform
 
 // ---------------------------------------------------------------- small kernels
+// Elementwise kernels over whole matrices take nch workgroups per matrix
+// (blockIdx.x = b * nch + chunk; the chunks stride over the n*n elements), so a
+// single n = 3000 matrix is spread over the chip; small n use nch = 1.
+static int chunks_per_matrix(int n) {
+    const long nn = (long)n * n;
+    const long c = (nn + 256L * 64 - 1) / (256L * 64);
+    return (int)(c < 1 ? 1 : (c > 256 ? 256 : c));
+}
+
 // Fill the triangle opposite to `uplo` by conjugate mirroring; real diagonal.
-// grid.x = batch (y/z grid dims are limited to 65535), threads over elements.
-__global__ void hermitize_kernel(double *A, long lda, long strideA, int n, int from_lower) {
-    double *M = A + 2 * blockIdx.x * strideA;
-    for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+__global__ void hermitize_kernel(double *A, long lda, long strideA, int n, int from_lower, int nch) {
+    double *M = A + 2 * (blockIdx.x / nch) * strideA;
+    const long step = (long)nch * blockDim.x;
+    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)n * n; idx += step) {
         const int c = (int)(idx / n), r = (int)(idx % n);
         if (r == c) {
             M[2 * ((long)c * lda + r) + 1] = 0.0;
@@ -43,20 +52,25 @@ __global__ void hermitize_kernel(double *A, long lda, long strideA, int n, int f
 }
 
 // A(r, j) = U[r*n + j] (component-major -> column-major eigenvectors)
-__global__ void transpose_out_kernel(const double *U, long strideU, int n, double *A, long lda, long strideA) {
-    const double *Ub = U + 2 * blockIdx.x * strideU;
-    double *M = A + 2 * blockIdx.x * strideA;
-    for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+__global__ void transpose_out_kernel(const double *U, long strideU, int n, double *A, long lda, long strideA,
+                                     int nch) {
+    const long b = blockIdx.x / nch;
+    const double *Ub = U + 2 * b * strideU;
+    double *M = A + 2 * b * strideA;
+    const long step = (long)nch * blockDim.x;
+    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)n * n; idx += step) {
         const int j = (int)(idx / n), r = (int)(idx % n);
         st(M, (long)j * lda + r, ld(Ub, (long)r * n + j));
     }
 }
 
 // Full-matrix copy with lda (column-major) -> packed lda = n
-__global__ void pack_kernel(const double *A, long lda, long strideA, int n, double *P) {
-    const double *M = A + 2 * blockIdx.x * strideA;
-    double *Q = P + 2 * (long)blockIdx.x * n * n;
-    for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+__global__ void pack_kernel(const double *A, long lda, long strideA, int n, double *P, int nch) {
+    const long b = blockIdx.x / nch;
+    const double *M = A + 2 * b * strideA;
+    double *Q = P + 2 * b * n * n;
+    const long step = (long)nch * blockDim.x;
+    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)n * n; idx += step) {
         const int c = (int)(idx / n), r = (int)(idx % n);
         st(Q, idx, ld(M, (long)c * lda + r));
     }
@@ -68,32 +82,52 @@ __global__ void to_f32_kernel(const double *P, long count, float *Q) {
         Q[i] = (float)P[i];
 }
 
-// FP64 -> FP32 copy of packed complex matrices (mixed-precision solver input), each
-// matrix scaled by a power of two to max |entry| in [1/2, 1) so that the FP32
-// reduction neither overflows nor underflows; eigenvectors do not depend on the
-// scale, and the eigenvalues are taken from the FP64 matrix afterwards.
-// grid.x = batch, 256 threads.
-__global__ __launch_bounds__(256) void to_f32_scaled_kernel(const double *P, long nn, float *Q) {
+// Largest |real or imaginary part| of each chunk of the packed complex matrices:
+// part[b * nch + chunk]. 256 threads.
+__global__ __launch_bounds__(256) void maxabs_kernel(const double *P, long nn, int nch, double *part) {
     __shared__ double red[256 / kWave];
-    const double *M = P + 2 * blockIdx.x * nn;
-    float *R = Q + 2 * blockIdx.x * nn;
+    const double *M = P + 2 * (blockIdx.x / nch) * nn;
+    const long step = (long)nch * blockDim.x;
     double mx = 0.0;
-    for (long i = threadIdx.x; i < 2 * nn; i += blockDim.x) mx = fmax(mx, fabs(M[i]));
+    for (long i = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; i < nn; i += step)
+        mx = fmax(mx, fmax(fabs(M[2 * i]), fabs(M[2 * i + 1])));
     for (int off = kWave / 2; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, kWave));
     if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = mx;
     __syncthreads();
-    for (int w = 0; w < (int)(blockDim.x / kWave); w++) mx = fmax(mx, red[w]);
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < (int)(blockDim.x / kWave); w++) mx = fmax(mx, red[w]);
+        part[blockIdx.x] = mx;
+    }
+}
+
+// FP64 -> FP32 copy of packed complex matrices (mixed-precision solver input), each
+// matrix scaled by a power of two to max |entry| in [1/2, 1) (from the maxabs_kernel
+// partials) so that the FP32 reduction neither overflows nor underflows;
+// eigenvectors do not depend on the scale, and the eigenvalues are taken from the
+// FP64 matrix afterwards.
+__global__ __launch_bounds__(256) void to_f32_scaled_kernel(const double *P, long nn, const double *part, int nch,
+                                                            float *Q) {
+    const long b = blockIdx.x / nch;
+    const double *M = P + 2 * b * nn;
+    float *R = Q + 2 * b * nn;
+    double mx = 0.0;
+    for (int k = 0; k < nch; k++) mx = fmax(mx, part[b * nch + k]);
     int e = 0;
     if (mx > 0.0) (void)frexp(mx, &e);
     const double sc = ldexp(1.0, -e);
-    for (long i = threadIdx.x; i < 2 * nn; i += blockDim.x) R[i] = (float)(M[i] * sc);
+    const long step = (long)nch * blockDim.x;
+    for (long i = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; i < 2 * nn; i += step)
+        R[i] = (float)(M[i] * sc);
 }
 
 // A(r, j) = U[r*n + j] from FP32 eigenvectors (component-major -> column-major FP64)
-__global__ void transpose_out_f32_kernel(const float *U, long strideU, int n, double *A, long lda, long strideA) {
-    const float *Ub = U + 2 * blockIdx.x * strideU;
-    double *M = A + 2 * blockIdx.x * strideA;
-    for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+__global__ void transpose_out_f32_kernel(const float *U, long strideU, int n, double *A, long lda, long strideA,
+                                         int nch) {
+    const long b = blockIdx.x / nch;
+    const float *Ub = U + 2 * b * strideU;
+    double *M = A + 2 * b * strideA;
+    const long step = (long)nch * blockDim.x;
+    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)n * n; idx += step) {
         const int j = (int)(idx / n), r = (int)(idx % n);
         const cx<float> u = ldx(Ub, (long)r * n + j);
         st(M, (long)j * lda + r, cd{(double)u.x, (double)u.y});
@@ -121,17 +155,30 @@ struct DevBuf {
     double *d() { return static_cast<double *>(p); }
 };
 
+// n above this goes to the multi-workgroup solver (eig_large.hip)
+constexpr int kSmallMaxN = 512;
+// divide & conquer workgroups (each with its own n*n workspaces) for n > kSmallMaxN
+constexpr int kGridTailLarge = 64;
+
+static int tail_grid(int n, long batch) {
+    const long g = n > kSmallMaxN ? kGridTailLarge : kGridTail;
+    return (int)(batch < g ? batch : g);
+}
+
 // Workspace of the batched eigensolver.
 struct EigWork {
-    DevBuf dev, U, work, tw;
+    DevBuf dev, U, work, tw, aux;
     int ensure(int n, long batch, bool with_u = true) {
         int rc;
         if ((rc = dev.ensure(sizeof(double) * 4 * (size_t)n * batch))) return rc;
         if (with_u && (rc = U.ensure(sizeof(double) * 2 * (size_t)n * n * batch))) return rc;
-        const long g = batch < kGridTail ? batch : kGridTail;
-        if ((rc = work.ensure(sizeof(double) * 3 * (size_t)n * n * g))) return rc;
+        const long g = tail_grid(n, batch);
+        const size_t per_wg = n > kSmallMaxN ? (size_t)stedc_global_work_doubles(n) : 3 * (size_t)n * n;
+        if ((rc = work.ensure(sizeof(double) * per_wg * g))) return rc;
         // triangular factors of the matrix-core back-transform (complex double covers float)
         if ((rc = tw.ensure(sizeof(double) * 2 * (size_t)backtransform_tws_complex(n) * batch))) return rc;
+        if (n > kSmallMaxN && (rc = aux.ensure(sizeof(double) * 2 * (size_t)heevd_large_aux_complex(n) * batch)))
+            return rc;
         return 0;
     }
     void release() {
@@ -139,6 +186,7 @@ struct EigWork {
         U.release();
         work.release();
         tw.release();
+        aux.release();
     }
 };
 
@@ -153,6 +201,15 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
     // layout per matrix: D at base + b*strideV, E at +n, TAU at +2n (complex, 2n doubles)
     // launch_* take separate arrays with a common stride:
     double *D = base, *E = base + n, *TAU = base + 2 * n;
+    if (n > kSmallMaxN) {
+        const int gl = tail_grid(n, batch);
+        if ((size_t)gl * stedc_global_work_doubles(n) * sizeof(double) > ws.work.bytes ||
+            (size_t)batch * heevd_large_aux_complex(n) * 2 * sizeof(double) > ws.aux.bytes ||
+            (size_t)batch * backtransform_tws_complex(n) * 2 * sizeof(double) > ws.tw.bytes)
+            return -22;
+        return heevd_large(A, n, batch, D, E, TAU, strideV, ws.aux.d(), W, Uout, ws.work.d(), gl, ws.tw.d(), s, mid,
+                           mid2);
+    }
     // TAU is read with a complex stride of strideV doubles -> pass complex-stride strideV/2
     const int gt = (int)(batch < kGridTrd ? batch : kGridTrd);
     if ((rc = launch_zhetrd(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, gt, s))) return rc;
@@ -180,6 +237,15 @@ int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork 
     double *base = ws.dev.d();
     double *D = base, *E = base + n;
     float *TAU = reinterpret_cast<float *>(base + 2 * n);
+    if (n > kSmallMaxN) {
+        const int gl = tail_grid(n, batch);
+        if (Hb || (size_t)gl * stedc_global_work_doubles(n) * sizeof(double) > ws.work.bytes ||
+            (size_t)batch * heevd_large_aux_complex(n) * 2 * sizeof(float) > ws.aux.bytes ||
+            (size_t)batch * backtransform_tws_complex(n) * 2 * sizeof(float) > ws.tw.bytes)
+            return -22;
+        return heevd_large(A, n, batch, D, E, TAU, strideV, static_cast<float *>(ws.aux.p), W, Uout, ws.work.d(), gl,
+                           static_cast<float *>(ws.tw.p), s, mid, mid2);
+    }
     const int gt = (int)(batch < kGridTrd ? batch : kGridTrd);
     if (Hb) {
         if ((rc = launch_chetrd_band(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, s, Hb, hb_off))) return rc;
@@ -207,10 +273,12 @@ DevBuf g_single_pack, g_single_a32;
 // ============================================================================
 extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA, double *d_W, long strideW,
                                       int batch, char uplo, int precision, void *stream) {
-    if (n < 1 || n > 512 || lda < n || batch < 0 || strideW < n || strideA < lda * (long)n) return -22;
+    if (n < 1 || n > heevd_large_max_n() || lda < n || batch < 0 || strideW < n || strideA < lda * (long)n)
+        return -22;
     if (precision != HIP_DIAG_FP64 && precision != HIP_DIAG_MIXED) return -22;
     if (batch == 0) return 0;
     if (uplo != 'L' && uplo != 'U' && uplo != 'l' && uplo != 'u') return -22;
+    if (!d_A || !d_W) return -22;
     hipStream_t s = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(g_single_mu);
     int rc;
@@ -220,25 +288,30 @@ extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA
     double *P = g_single_pack.d();
     double *Wp = P + 2L * n * n * batch;
     const long nn = (long)n * n;
-    dim3 grid((unsigned)batch);
-    hipLaunchKernelGGL(pack_kernel, grid, dim3(256), 0, s, d_A, lda, strideA, n, P);
+    const int nch = chunks_per_matrix(n);
+    dim3 grid((unsigned)((long)batch * nch));
+    hipLaunchKernelGGL(pack_kernel, grid, dim3(256), 0, s, d_A, lda, strideA, n, P, nch);
     hipLaunchKernelGGL(hermitize_kernel, grid, dim3(256), 0, s, P, (long)n, nn, n,
-                       (uplo == 'L' || uplo == 'l') ? 1 : 0);
+                       (uplo == 'L' || uplo == 'l') ? 1 : 0, nch);
     RLM_HIP_CHECK(hipGetLastError());
     if (precision == HIP_DIAG_MIXED) {
         // FP32 reduction and back-transform, FP64 divide & conquer, then FP64
         // Rayleigh quotients of the FP32 vectors against the FP64 matrix
-        if ((rc = g_single_a32.ensure(sizeof(float) * 2 * (size_t)nn * batch))) return rc;
+        if ((rc = g_single_a32.ensure(sizeof(float) * 2 * (size_t)nn * batch + sizeof(double) * (size_t)batch * nch)))
+            return rc;
         float *A32 = static_cast<float *>(g_single_a32.p);
+        double *part = reinterpret_cast<double *>(A32 + 2 * (size_t)nn * batch);
         float *U32 = reinterpret_cast<float *>(g_single_ws.U.d());   // holds 2*nn*batch doubles >= floats
-        hipLaunchKernelGGL(to_f32_scaled_kernel, grid, dim3(256), 0, s, P, nn, A32);
+        hipLaunchKernelGGL(maxabs_kernel, grid, dim3(256), 0, s, P, nn, nch, part);
+        hipLaunchKernelGGL(to_f32_scaled_kernel, grid, dim3(256), 0, s, P, nn, part, nch, A32);
         RLM_HIP_CHECK(hipGetLastError());
         if ((rc = eig_packed_f32(A32, n, batch, Wp, U32, g_single_ws, s))) return rc;
         if ((rc = launch_rayleigh(P, U32, n, n, batch, Wp, nullptr, s))) return rc;
-        hipLaunchKernelGGL(transpose_out_f32_kernel, grid, dim3(256), 0, s, U32, nn, n, d_A, lda, strideA);
+        hipLaunchKernelGGL(transpose_out_f32_kernel, grid, dim3(256), 0, s, U32, nn, n, d_A, lda, strideA, nch);
     } else {
         if ((rc = eig_packed(P, n, batch, Wp, g_single_ws.U.d(), g_single_ws, s))) return rc;
-        hipLaunchKernelGGL(transpose_out_kernel, grid, dim3(256), 0, s, g_single_ws.U.d(), nn, n, d_A, lda, strideA);
+        hipLaunchKernelGGL(transpose_out_kernel, grid, dim3(256), 0, s, g_single_ws.U.d(), nn, n, d_A, lda, strideA,
+                           nch);
     }
     RLM_HIP_CHECK(hipGetLastError());
     RLM_HIP_CHECK(hipMemcpy2DAsync(d_W, sizeof(double) * strideW, Wp, sizeof(double) * n, sizeof(double) * n, batch,
@@ -408,7 +481,8 @@ struct rlm_plan_s {
 
 extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double imp, long n_mesh, long off_override,
                                unsigned long seed, int samples_per_batch, int precision) {
-    if (!plan || n_phi < 1 || n_phi > 512 || n_mesh < 1 || samples_per_batch < 1 || q <= 0.0) return -22;
+    if (!plan || n_phi < 1 || n_phi > heevd_large_max_n() || n_mesh < 1 || samples_per_batch < 1 || q <= 0.0)
+        return -22;
     if (precision != HIP_DIAG_FP64 && precision != HIP_DIAG_MIXED) return -22;
     auto *p = new rlm_plan_s();
     p->n_phi = n_phi;
@@ -757,7 +831,7 @@ extern "C" int hlm_plan_execute(hlm_plan_t p) {
         return rc;
     // the reference's zheevd reads the upper triangle of Htr (mkl_diag.cpp:6)
     {
-        hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)nkp), dim3(256), 0, s, p->dH.d(), (long)N, nn, N, 0);
+        hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)nkp), dim3(256), 0, s, p->dH.d(), (long)N, nn, N, 0, 1);
         RLM_HIP_CHECK(hipGetLastError());
     }
     RLM_HIP_CHECK(hipEventRecord(p->ev[1], s));

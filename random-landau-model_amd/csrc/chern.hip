@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void links_kernel(const TU *__restrict__ U, lo
     const TU *u1 = U + 2 * (kp_1 - kp0) * strideU;
     const TU *u2 = U + 2 * (kp_2 - kp0) * strideU;
     auto up = [](cx<TU> v) { return cd{(double)v.x, (double)v.y}; };
-    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    for (int j = blockIdx.y * blockDim.x + threadIdx.x; j < ns; j += gridDim.y * blockDim.x) {
         cd a1{0.0, 0.0}, a2{0.0, 0.0};
         for (int i = 0; i < dim_vec; i++) {
             const cd x = up(ldx(u0, (long)i * ns + j));
@@ -61,11 +61,12 @@ __global__ __launch_bounds__(256) void links_kernel(const TU *__restrict__ U, lo
 // diag(e^{i 2 pi j/N}) A diag(e^{-i 2 pi j/N})):
 //   u(t1, g)[i] = u(t1, 0)[(i+1) % N],  u(g, t2)[i] = e^{i 2 pi i/N} u(0, t2)[i].
 // Links are written for the full (g+1)^2 mesh, kp = (s*(g+1) + t1)*(g+1) + t2, as
-// links_kernel does; grid (S*(g+1)^2), block 256.
+// links_kernel does; grid (S*(g+1)^2, state chunks of 256), block 256, ns complex
+// doubles of dynamic LDS.
 template <typename TU>
 __global__ __launch_bounds__(256) void links_derived_kernel(const TU *__restrict__ U, long strideU, int ns, int g,
                                                             double *__restrict__ Lk1, double *__restrict__ Lk2) {
-    __shared__ cd phs[512];                                  // e^{i 2 pi i/N}, N <= 512
+    extern __shared__ cd phs[];                              // e^{i 2 pi i/N}
     const long kp = blockIdx.x;
     const int g1 = g + 1;
     const long s = kp / ((long)g1 * g1);
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(256) void links_derived_kernel(const TU *__restrict
         if (ph) x = cmul(phs[i], x);
         return x;
     };
-    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+    for (int j = blockIdx.y * blockDim.x + threadIdx.x; j < ns; j += gridDim.y * blockDim.x) {
         cd a1{0.0, 0.0}, a2{0.0, 0.0};
         for (int i = 0; i < ns; i++) {
             const cd x = comp(u0, sh0, ph0, i, j);
@@ -175,8 +176,8 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ 
 int launch_links(const double *U, long strideU, int dim_vec, int ns, int lines, int L2n, int nlink2, long kp0,
                  long nkp, double *Lk1, double *Lk2, hipStream_t stream) {
     if (nkp <= 0) return 0;
-    hipLaunchKernelGGL(links_kernel<double>, dim3((unsigned)nkp), dim3(256), 0, stream, U, strideU, dim_vec, ns, lines,
-                       L2n, nlink2, kp0, Lk1, Lk2);
+    hipLaunchKernelGGL(links_kernel<double>, dim3((unsigned)nkp, (ns + 255) / 256), dim3(256), 0, stream, U, strideU,
+                       dim_vec, ns, lines, L2n, nlink2, kp0, Lk1, Lk2);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -184,8 +185,8 @@ int launch_links(const double *U, long strideU, int dim_vec, int ns, int lines, 
 int launch_links_f32(const float *U, long strideU, int dim_vec, int ns, int lines, int L2n, int nlink2, long kp0,
                      long nkp, double *Lk1, double *Lk2, hipStream_t stream) {
     if (nkp <= 0) return 0;
-    hipLaunchKernelGGL(links_kernel<float>, dim3((unsigned)nkp), dim3(256), 0, stream, U, strideU, dim_vec, ns, lines,
-                       L2n, nlink2, kp0, Lk1, Lk2);
+    hipLaunchKernelGGL(links_kernel<float>, dim3((unsigned)nkp, (ns + 255) / 256), dim3(256), 0, stream, U, strideU,
+                       dim_vec, ns, lines, L2n, nlink2, kp0, Lk1, Lk2);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -194,13 +195,15 @@ int launch_links_derived(const void *U, bool fp32, long strideU, int ns, int g, 
                          double *Lk2, hipStream_t stream) {
     const long nkp = (long)n_samples * (g + 1) * (g + 1);
     if (nkp <= 0) return 0;
-    if (ns > 512 || g < 1) return -22;   // phase table size
+    if (ns > 4096 || g < 1) return -22;   // phase table: ns complex doubles of LDS
+    const dim3 grid((unsigned)nkp, (ns + 255) / 256);
+    const size_t lds = sizeof(cd) * ns;
     if (fp32)
-        hipLaunchKernelGGL(links_derived_kernel<float>, dim3((unsigned)nkp), dim3(256), 0, stream,
-                           static_cast<const float *>(U), strideU, ns, g, Lk1, Lk2);
+        hipLaunchKernelGGL(links_derived_kernel<float>, grid, dim3(256), lds, stream, static_cast<const float *>(U),
+                           strideU, ns, g, Lk1, Lk2);
     else
-        hipLaunchKernelGGL(links_derived_kernel<double>, dim3((unsigned)nkp), dim3(256), 0, stream,
-                           static_cast<const double *>(U), strideU, ns, g, Lk1, Lk2);
+        hipLaunchKernelGGL(links_derived_kernel<double>, grid, dim3(256), lds, stream, static_cast<const double *>(U),
+                           strideU, ns, g, Lk1, Lk2);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -1769,7 +1769,9 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
 
 // Stage 1 kernel: divide & conquer per matrix; eigenvalues to W, eigenvectors of
 // the tridiagonal to U (component-major complex, zero imaginary part).
-template <typename T, int NT>
+// GSH: the per-node arrays live in the workgroup's global workspace after Qa, Qb, S
+// (n > 512, where they exceed the LDS); the LDS then holds only the transpose tile.
+template <typename T, int NT, bool GSH>
 __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restrict__ Din,
                                                    const double *__restrict__ Ein, long strideV,
                                                    double *__restrict__ Wout, long strideW, T *__restrict__ Uout,
@@ -1777,13 +1779,14 @@ __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restri
     using C = cx<T>;
     extern __shared__ double smem[];
     const int t = threadIdx.x;
-    // per-workgroup global workspace: Qa, Qb, S (n*n doubles each)
-    double *Qa0 = work + (long)blockIdx.x * 3 * n * n;
+    // per-workgroup global workspace: Qa, Qb, S (n*n doubles each) [+ the node arrays]
+    const long wsz = GSH ? stedc_global_work_doubles(n) : 3L * n * n;
+    double *Qa0 = work + (long)blockIdx.x * wsz;
     double *Qb0 = Qa0 + (long)n * n;
     double *S = Qb0 + (long)n * n;
     DcShared sh;
     {
-        double *p = smem;
+        double *p = GSH ? S + (long)n * n : smem;
         sh.dq = p; p += n; sh.dqn = p; p += n; sh.ds = p; p += n; sh.zs = p; p += n;
         sh.dl = p; p += n; sh.wz = p; p += n; sh.z2 = p; p += n; sh.tau = p; p += n; sh.vals = p; p += n;
         sh.rc = p; p += n; sh.rs = p; p += n; sh.what = p; p += n; sh.inrm = p; p += n;
@@ -1935,17 +1938,19 @@ __device__ inline cx<T> refl_get(const T *A, int lda, int n, int j0, int row, in
 // block bk of matrix b covers reflectors 16bk..16bk+15; T[k][k] = tau_k,
 // T[0:k, k] = -tau_k T[0:k, 0:k] G[0:k, k] with G = V^H V (one MFMA product per
 // block). Stored column-major per block: Tout[b*strideT + (bk*256 + k2*16 + k1)]
-// (complex). One wave per block, 4 waves per workgroup, one workgroup per matrix.
+// (complex). One wave per block, 4 waves per workgroup, wpm workgroups per matrix
+// (workgroup q of a matrix takes blocks 4q + w, 4q + w + 4 wpm, ...).
 template <typename T>
 __global__ __launch_bounds__(256) void larft_kernel(const T *__restrict__ Abase, long strideA, int lda, int n,
                                                     const T *__restrict__ TAUin, long strideV, T *__restrict__ Tout,
-                                                    long strideT, int batch) {
+                                                    long strideT, int batch, int wpm) {
     using M = Mf<T>;
     using v4 = typename M::v4;
     using CT = cx<T>;
     __shared__ CT gs[4][16 * 17];
     __shared__ CT ts[4][16 * 17];
-    const long b = blockIdx.x;
+    const long b = blockIdx.x / wpm;
+    const int wq = (int)(blockIdx.x % wpm);
     if (b >= batch) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -1954,7 +1959,7 @@ __global__ __launch_bounds__(256) void larft_kernel(const T *__restrict__ Abase,
     const T *tau = TAUin + b * strideV * (long)(sizeof(double) / sizeof(T));
     CT *Tb = reinterpret_cast<CT *>(Tout + 2 * b * strideT);
     const int nref = n - 1, nblk = (nref + 15) / 16;
-    for (int bk = w; bk < nblk; bk += 4) {
+    for (int bk = 4 * wq + w; bk < nblk; bk += 4 * wpm) {
         const int j0 = bk * 16;
         v4 gr = {0, 0, 0, 0}, gi = {0, 0, 0, 0};
         // G[k1][k2] = sum_r conj(V[r][k1]) V[r][k2]; K-step over rows 4s.. (rows >= j0+1)
@@ -2270,10 +2275,10 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
     if (lds_dc > 160 * 1024) return -22;
     if (grid > batch) grid = batch;
     if (lds_dc > 64 * 1024)
-        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)stedc_kernel<T, NT>,
+        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)stedc_kernel<T, NT, false>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_dc));
-    hipLaunchKernelGGL((stedc_kernel<T, NT>), dim3(grid), dim3(NT), lds_dc, stream, n, D, E, strideV, W, strideW, U,
-                       strideU, work, batch);
+    hipLaunchKernelGGL((stedc_kernel<T, NT, false>), dim3(grid), dim3(NT), lds_dc, stream, n, D, E, strideV, W,
+                       strideW, U, strideU, work, batch);
     RLM_HIP_CHECK(hipGetLastError());
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, stream));
     if (n == 1) return 0;
@@ -2281,7 +2286,7 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
         // matrix-core path: triangular factors, then slabs of 32 eigenvector columns
         const long strideT = backtransform_tws_complex(n);
         hipLaunchKernelGGL((larft_kernel<T>), dim3(batch), dim3(256), 0, stream, A, strideA, lda, n, TAU, strideV, tws,
-                           strideT, batch);
+                           strideT, batch, 1);
         RLM_HIP_CHECK(hipGetLastError());
         // slab width: 32 columns for complex float; 16 for complex double, which keeps the
         // accumulators within two waves per SIMD
@@ -2416,6 +2421,49 @@ int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *
                     int grid, hipStream_t stream, hipEvent_t mid) {
     return launch_tail_t<float>(A, strideA, lda, n, D, E, TAU, strideV, W, strideW, U, strideU, work, tws, batch,
                                 grid, stream, mid);
+}
+
+// ---------------------------------------------------------------- n > 512 pieces
+// Divide & conquer with the node arrays in global memory (any n); work holds
+// grid * stedc_global_work_doubles(n) doubles.
+template <typename T>
+static int launch_stedc_global_t(int n, const double *D, const double *E, long strideV, double *W, long strideW, T *U,
+                                 long strideU, double *work, int batch, int grid, hipStream_t stream) {
+    constexpr int NT = 256;
+    if (n < 1 || batch < 1) return batch < 1 ? 0 : -22;
+    if (grid > batch) grid = batch;
+    const size_t lds = 32 * 33 * sizeof(double);
+    hipLaunchKernelGGL((stedc_kernel<T, NT, true>), dim3(grid), dim3(NT), lds, stream, n, D, E, strideV, W, strideW,
+                       U, strideU, work, batch);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+int launch_stedc_global(int n, const double *D, const double *E, long strideV, double *W, long strideW, double *U,
+                        long strideU, double *work, int batch, int grid, hipStream_t stream) {
+    return launch_stedc_global_t<double>(n, D, E, strideV, W, strideW, U, strideU, work, batch, grid, stream);
+}
+int launch_stedc_global(int n, const double *D, const double *E, long strideV, double *W, long strideW, float *U,
+                        long strideU, double *work, int batch, int grid, hipStream_t stream) {
+    return launch_stedc_global_t<float>(n, D, E, strideV, W, strideW, U, strideU, work, batch, grid, stream);
+}
+
+// Triangular factors of every 16-reflector block, wpm workgroups per matrix.
+template <typename T>
+static int launch_larft_t(const T *A, long strideA, int lda, int n, const T *TAU, long strideV, T *tws, long strideT,
+                          int batch, int wpm, hipStream_t stream) {
+    if (batch < 1 || n < 2) return 0;
+    hipLaunchKernelGGL((larft_kernel<T>), dim3((unsigned)((long)batch * wpm)), dim3(256), 0, stream, A, strideA, lda,
+                       n, TAU, strideV, tws, strideT, batch, wpm);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+int launch_larft(const double *A, long strideA, int lda, int n, const double *TAU, long strideV, double *tws,
+                 long strideT, int batch, int wpm, hipStream_t stream) {
+    return launch_larft_t<double>(A, strideA, lda, n, TAU, strideV, tws, strideT, batch, wpm, stream);
+}
+int launch_larft(const float *A, long strideA, int lda, int n, const float *TAU, long strideV, float *tws,
+                 long strideT, int batch, int wpm, hipStream_t stream) {
+    return launch_larft_t<float>(A, strideA, lda, n, TAU, strideV, tws, strideT, batch, wpm, stream);
 }
 
 }  // namespace rlm

@@ -39,4 +39,35 @@ int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *
 
 size_t heevd_dc_lds_bytes(int n);
 
+// ---------------------------------------------------------------- 512 < n <= 4096
+// Doubles of global workspace per workgroup of the divide & conquer with its node
+// arrays in global memory: Qa, Qb, S (n*n each), 13 double and 16 int arrays of n,
+// 8 doubles of reduction scratch (+ 8 of padding).
+__host__ __device__ inline long stedc_global_work_doubles(int n) { return 3L * n * n + 13L * n + 8 + 8L * n + 8; }
+int launch_stedc_global(int n, const double *D, const double *E, long strideV, double *W, long strideW, double *U,
+                        long strideU, double *work, int batch, int grid, hipStream_t stream);
+int launch_stedc_global(int n, const double *D, const double *E, long strideV, double *W, long strideW, float *U,
+                        long strideU, double *work, int batch, int grid, hipStream_t stream);
+// zlarft factors of every 16-reflector block (tws layout of backtransform_tws_complex),
+// wpm workgroups per matrix.
+int launch_larft(const double *A, long strideA, int lda, int n, const double *TAU, long strideV, double *tws,
+                 long strideT, int batch, int wpm, hipStream_t stream);
+int launch_larft(const float *A, long strideA, int lda, int n, const float *TAU, long strideV, float *tws,
+                 long strideT, int batch, int wpm, hipStream_t stream);
+
+// Whole eigensolver for 512 < n <= heevd_large_max_n() (eig_large.hip): A packed
+// (lda = n, stride n*n, full Hermitian storage, overwritten by the reflectors);
+// d | e | tau slots as launch_zhetrd (strideV doubles per matrix); aux: batch *
+// heevd_large_aux_complex(n) complex elements of the precision of A; work: grid_dc *
+// stedc_global_work_doubles(n) doubles; tws: batch * backtransform_tws_complex(n)
+// complex. W: eigenvalues (stride n), U: eigenvectors component-major (stride n*n).
+int heevd_large_max_n();
+long heevd_large_aux_complex(int n);
+int heevd_large(double *A, int n, long batch, double *D, double *E, double *TAU, long strideV, double *aux, double *W,
+                double *U, double *work, int grid_dc, double *tws, hipStream_t s, hipEvent_t mid = nullptr,
+                hipEvent_t mid2 = nullptr);
+int heevd_large(float *A, int n, long batch, double *D, double *E, float *TAU, long strideV, float *aux, double *W,
+                float *U, double *work, int grid_dc, float *tws, hipStream_t s, hipEvent_t mid = nullptr,
+                hipEvent_t mid2 = nullptr);
+
 }  // namespace rlm

@@ -1,0 +1,524 @@
+// Batched Hermitian eigensolver for 512 < n <= 4096: the n_phi = 3000 regime of
+// the random Landau model (rlm/main_rlm_cpu.cpp:29-34 with -n 3000 -q 1 -i 16),
+// where one matrix is 144 MB in complex double and the parallelism has to come
+// from inside each matrix. Same conventions as eig.hip (LAPACK zhetrd lower,
+// zlarfg reflectors kept in A, compact-WY back-transform), so it replaces the
+// per-k-point zheevd of diag_wrappers/mkl_diag.cpp:2-9 at these sizes.
+//
+//   Tridiagonalisation (zhetrd/zlatrd structure, panels of kLgNB columns, full
+//   Hermitian storage so every Hermitian product reads whole rows):
+//     lg_col_kernel    one workgroup per matrix and column: finishes w of the
+//                      previous column (w = tau y - 1/2 tau (w^H v) v), applies
+//                      the panel's pending update to column c, builds reflector c
+//                      (zlarfg) and the dot products W^H v, V^H v;
+//     lg_symv_kernel   y = A22 v - V (W^H v) - W (V^H v), 64 rows per workgroup:
+//                      the trailing block is streamed once per column;
+//     lg_her2k_kernel  after each panel A22 -= V W^H + W V^H on the matrix cores.
+//   Divide & conquer: stedc_kernel with its node arrays in global memory.
+//   Back-transform: larft_kernel with several workgroups per matrix, then
+//     lg_backtransform_kernel: a slab of 32 eigenvector columns per workgroup,
+//     one read and one write of the slab per block of 16 reflectors.
+#include "common.hpp"
+#include "eig.hpp"
+
+#include <algorithm>
+
+namespace rlm {
+
+constexpr int kLgNB = 32;      // reflectors per panel
+constexpr int kLgColNT = 512;  // threads of the column kernel
+constexpr int kLgRPT = 8;      // rows per thread of the column kernel: n <= 4096
+
+int heevd_large_max_n() { return kLgColNT * kLgRPT; }
+
+// Per-matrix panel workspace, complex elements: Vp (kLgNB x n, column l = reflector
+// k0 + l with explicit zeros above and the unit entry), Wp (kLgNB x n), P (2 kLgNB:
+// W^H v then V^H v of the current column).
+long heevd_large_aux_complex(int n) { return 2L * kLgNB * n + 2L * kLgNB; }
+
+// ----------------------------------------------------------------------------
+// Column c = k0 + i of the panel starting at k0 (nbk reflectors). i == nbk: only
+// finish w of the last panel column (and d[n-1] when c == n - 1).
+template <typename T>
+__global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase, long strideA, int n,
+                                                          double *__restrict__ Dout, double *__restrict__ Eout,
+                                                          T *__restrict__ TAUout, long strideV, T *__restrict__ aux,
+                                                          long strideAux, int k0, int i, int nbk) {
+    using C = cx<T>;
+    constexpr int NT = kLgColNT, NW = NT / kWave;
+    __shared__ T red[2 * NW];
+    __shared__ C rowc[2 * kLgNB];
+    __shared__ T sc[8];
+    extern __shared__ double lds_col[];
+    C *vsh = reinterpret_cast<C *>(lds_col);   // v(c+1 .. n-1)
+    const long b = blockIdx.x;
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    T *A = Abase + 2 * b * strideA;
+    C *Vp = reinterpret_cast<C *>(aux + 2 * b * strideAux);
+    C *Wp = Vp + (long)kLgNB * n;
+    C *P = Wp + (long)kLgNB * n;
+    double *Dv = Dout + b * strideV, *Ev = Eout + b * strideV;
+    T *Tv = TAUout + b * strideV * (long)(sizeof(double) / sizeof(T));
+    const int c = k0 + i;
+    // (a) w of panel column i - 1 (reflector c - 1, support rows >= c)
+    if (i > 0) {
+        const C tau{Tv[2 * (c - 1)], Tv[2 * (c - 1) + 1]};
+        C *y = Wp + (long)(i - 1) * n;
+        const C *v = Vp + (long)(i - 1) * n;
+        C wl[kLgRPT];
+        C s{(T)0, (T)0};
+#pragma unroll
+        for (int q = 0; q < kLgRPT; q++) {
+            const int r = c + t + q * NT;
+            wl[q] = C{(T)0, (T)0};
+            if (r < n) {
+                wl[q] = tau * y[r];
+                fmacx(s, wl[q], v[r]);                       // s += conj(w) v
+            }
+        }
+        s = block_sum_cx(s, red);
+        const C alpha = (tau * s) * (T)(-0.5);
+#pragma unroll
+        for (int q = 0; q < kLgRPT; q++) {
+            const int r = c + t + q * NT;
+            if (r < n) y[r] = wl[q] + alpha * v[r];
+        }
+        __syncthreads();
+    }
+    const bool refl = i < nbk && c < n - 1;
+    const bool last = c == n - 1;
+    if (!refl && !last) return;
+    // (b) column c (rows >= c) with the panel's pending rank-2i update
+    if (t < i) {
+        rowc[t] = Wp[(long)t * n + c];
+        rowc[kLgNB + t] = Vp[(long)t * n + c];
+    }
+    __syncthreads();
+    C a[kLgRPT];
+    T nrm2 = 0;
+#pragma unroll
+    for (int q = 0; q < kLgRPT; q++) {
+        const int r = c + t + q * NT;
+        C x{(T)0, (T)0};
+        if (r < n) {
+            x = ldx(A, (long)c * n + r);
+            for (int l = 0; l < i; l++)
+                x = x - (Vp[(long)l * n + r] * conjx(rowc[l]) + Wp[(long)l * n + r] * conjx(rowc[kLgNB + l]));
+            if (r == c) {
+                x.y = (T)0;
+                sc[0] = x.x;
+            } else if (r == c + 1) {
+                sc[1] = x.x;
+                sc[2] = x.y;
+            } else {
+                nrm2 += abs2x(x);
+            }
+        }
+        a[q] = x;
+    }
+    nrm2 = block_sum_t(nrm2, red);                           // its barriers publish sc[0..2]
+    if (last) {
+        if (t == 0) Dv[c] = (double)sc[0];
+        return;
+    }
+    if (t == 0) {
+        Dv[c] = (double)sc[0];
+        const T ar = sc[1], ai = sc[2];
+        T beta, taur, taui, sr, si;
+        if (nrm2 == (T)0 && ai == (T)0) {
+            beta = ar; taur = 0; taui = 0; sr = 0; si = 0;
+        } else {
+            const T nrm = sqrt(ar * ar + ai * ai + nrm2);
+            beta = (ar >= (T)0) ? -nrm : nrm;
+            taur = (beta - ar) / beta;
+            taui = -ai / beta;
+            const T dr = ar - beta, di = ai;
+            const T inv = (T)1 / (dr * dr + di * di);
+            sr = dr * inv; si = -di * inv;
+        }
+        Ev[c] = (double)beta;
+        Tv[2 * c] = taur;
+        Tv[2 * c + 1] = taui;
+        sc[5] = sr; sc[6] = si; sc[7] = beta;
+    }
+    __syncthreads();
+    const C scal{sc[5], sc[6]};
+    // (c) v: reflector storage in A, panel column i of Vp, LDS copy; rows k0..c of
+    //     Vp(:, i) and Wp(:, i) cleared (the product fills Wp below row c)
+#pragma unroll
+    for (int q = 0; q < kLgRPT; q++) {
+        const int r = c + t + q * NT;
+        if (r < n) {
+            C v{(T)0, (T)0};
+            if (r == c + 1) {
+                v = C{(T)1, (T)0};
+                stx(A, (long)c * n + r, C{sc[7], (T)0});
+            } else if (r > c + 1) {
+                v = a[q] * scal;
+                stx(A, (long)c * n + r, v);
+            }
+            Vp[(long)i * n + r] = v;
+            if (r > c) vsh[r - c - 1] = v;
+        }
+    }
+    for (int r = k0 + t; r <= c; r += NT) {
+        if (r < c) Vp[(long)i * n + r] = C{(T)0, (T)0};
+        Wp[(long)i * n + r] = C{(T)0, (T)0};
+    }
+    __syncthreads();
+    // (d) P[l] = W(:, l)^H v, P[kLgNB + l] = V(:, l)^H v over rows > c, one wave per product
+    const int m = n - c - 1;
+    for (int qd = wid; qd < 2 * i; qd += NW) {
+        const C *col = (qd < i) ? Wp + (long)qd * n : Vp + (long)(qd - i) * n;
+        C s{(T)0, (T)0};
+        for (int e = lane; e < m; e += kWave) fmacx(s, col[c + 1 + e], vsh[e]);
+        s.x = wave_sum_t(s.x);
+        s.y = wave_sum_t(s.y);
+        if (lane == 0) P[qd < i ? qd : kLgNB + (qd - i)] = s;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// y = A(c+1:n, c+1:n) v - V (W^H v) - W (V^H v) into Wp(:, i), rows c+1.. of row
+// block rb (64 rows): lane = row, wave w takes the columns c+1+w, +4, ...; the four
+// wave partials are summed in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void lg_symv_kernel(const T *__restrict__ Abase, long strideA, int n,
+                                                      T *__restrict__ aux, long strideAux, int k0, int i, int nrb) {
+    using C = cx<T>;
+    extern __shared__ double lds_sv[];
+    C *vs = reinterpret_cast<C *>(lds_sv);                   // v(c+1 .. n-1)
+    __shared__ C part[4][kWave];
+    const long b = blockIdx.x / nrb;
+    const int rb = (int)(blockIdx.x % nrb);
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    const int c = k0 + i, m = n - c - 1;
+    const T *A = Abase + 2 * b * strideA;
+    C *Vp = reinterpret_cast<C *>(aux + 2 * b * strideAux);
+    C *Wp = Vp + (long)kLgNB * n;
+    const C *P = Wp + (long)kLgNB * n;
+    for (int e = t; e < m; e += 256) vs[e] = Vp[(long)i * n + c + 1 + e];
+    __syncthreads();
+    const int r = c + 1 + rb * 64 + lane;
+    const long rr = r < n ? r : n - 1;                       // clamped; masked at the end
+    const T *Acol = A + 2 * ((long)(c + 1) * n + rr);        // A(rr, c+1)
+    C acc0{(T)0, (T)0}, acc1{(T)0, (T)0};
+    int e = w;
+    for (; e + 12 < m; e += 16) {
+        const C a0 = ldx(Acol, (long)e * n), a1 = ldx(Acol, (long)(e + 4) * n);
+        const C a2 = ldx(Acol, (long)(e + 8) * n), a3 = ldx(Acol, (long)(e + 12) * n);
+        fmax_(acc0, a0, vs[e]);
+        fmax_(acc1, a1, vs[e + 4]);
+        fmax_(acc0, a2, vs[e + 8]);
+        fmax_(acc1, a3, vs[e + 12]);
+    }
+    for (; e < m; e += 4) fmax_(acc0, ldx(Acol, (long)e * n), vs[e]);
+    part[w][lane] = acc0 + acc1;
+    __syncthreads();
+    if (w == 0 && r < n) {
+        C y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+        for (int l = 0; l < i; l++) y = y - (Vp[(long)l * n + r] * P[l] + Wp[(long)l * n + r] * P[kLgNB + l]);
+        Wp[(long)i * n + r] = y;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// A(r, c') -= sum_l V(r,l) conj(W(c',l)) + W(r,l) conj(V(c',l)) for r, c' >= s0,
+// both triangles. With X = [V W], Y = [W V] (n x 2 nbk) the update is X Y^H; the
+// matrix cores compute its transpose D[c'][r] = sum_k (-conj(Y(c',k))) X(r,k) so
+// that the 16 lanes of an accumulator row hold 16 consecutive rows of A
+// (coalesced in column-major storage). 64 x 64 tile per workgroup, 32 x 32 per wave.
+template <typename T>
+__global__ __launch_bounds__(256) void lg_her2k_kernel(T *__restrict__ Abase, long strideA, int n,
+                                                       const T *__restrict__ aux, long strideAux, int s0, int nbk,
+                                                       int nt) {
+    using M = Mf<T>;
+    using v4 = typename M::v4;
+    using C = cx<T>;
+    const int tiles = nt * nt;
+    const long b = blockIdx.x / tiles;
+    const int tile = (int)(blockIdx.x % tiles);
+    const int tr = tile % nt, tc = tile / nt;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int li = lane & 15, lq = lane >> 4;
+    const int R0 = s0 + 64 * tr + 32 * (w & 1), C0 = s0 + 64 * tc + 32 * (w >> 1);
+    T *A = Abase + 2 * b * strideA;
+    const C *Vp = reinterpret_cast<const C *>(aux + 2 * b * strideAux);
+    const C *Wp = Vp + (long)kLgNB * n;
+    v4 dr[2][2], di[2][2];
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) {
+            const int r = R0 + 16 * rb + li;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int cc = C0 + 16 * cb + M::row(lane, v);
+                const C x = (r < n && cc < n) ? ldx(A, (long)cc * n + r) : C{(T)0, (T)0};
+                dr[cb][rb][v] = x.x;
+                di[cb][rb][v] = x.y;
+            }
+        }
+    const int K = 2 * nbk;
+    for (int k4 = 0; k4 < K; k4 += 4) {
+        const int k = k4 + lq;
+        const bool kin = k < K;
+        const C *Xc = k < nbk ? Vp + (long)k * n : Wp + (long)(k - nbk) * n;
+        const C *Yc = k < nbk ? Wp + (long)k * n : Vp + (long)(k - nbk) * n;
+        C ya[2], xb[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int cc = C0 + 16 * u + li, r = R0 + 16 * u + li;
+            ya[u] = (kin && cc < n) ? Yc[cc] : C{(T)0, (T)0};
+            xb[u] = (kin && r < n) ? Xc[r] : C{(T)0, (T)0};
+        }
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++) {
+                dr[cb][rb] = M::mma(-ya[cb].x, xb[rb].x, dr[cb][rb]);
+                dr[cb][rb] = M::mma(-ya[cb].y, xb[rb].y, dr[cb][rb]);
+                di[cb][rb] = M::mma(-ya[cb].x, xb[rb].y, di[cb][rb]);
+                di[cb][rb] = M::mma(ya[cb].y, xb[rb].x, di[cb][rb]);
+            }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) {
+            const int r = R0 + 16 * rb + li;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int cc = C0 + 16 * cb + M::row(lane, v);
+                if (r < n && cc < n) stx(A, (long)cc * n + r, C{dr[cb][rb][v], di[cb][rb][v]});
+            }
+        }
+}
+
+// ----------------------------------------------------------------------------
+// Back-transform U = Q U, Q = H_0 H_1 ... H_{n-2}: blocks of 16 reflectors
+// I - V T V^H (T from larft_kernel), last block first. A workgroup owns a slab of
+// 16 NCB eigenvector columns of one matrix; wave w owns the 16-row blocks w, w+4,
+// ... of it in every phase, so U is only ever re-read by the wave that wrote it.
+// Per block: the wave partials of W = V^H U are summed in LDS (fixed order), W2 =
+// T W, then one pass over the wave's row blocks updates U -= V W2 and, from the
+// updated rows still in registers, accumulates the next block's W partial. The
+// slabs of one matrix share blockIdx % 8 (one XCD, whose L2 serves V and T).
+// Element (row, k) of reflector block j0 (unit at row j+1, A(j+2:n, j) below).
+template <typename T>
+__device__ inline cx<T> lg_refl(const T *A, int n, int j0, int row, int k) {
+    const int j = j0 + k;
+    if (j >= n - 1 || row <= j || row >= n) return cx<T>{(T)0, (T)0};
+    if (row == j + 1) return cx<T>{(T)1, (T)0};
+    return ldx(A, (long)j * n + row);
+}
+
+template <typename T, int NCB>
+__global__ __launch_bounds__(256) void lg_backtransform_kernel(const T *__restrict__ Abase, long strideA, int n,
+                                                               const T *__restrict__ Tin, long strideT,
+                                                               T *__restrict__ Uout, long strideU, int nslab,
+                                                               int batch) {
+    using M = Mf<T>;
+    using v4 = typename M::v4;
+    using CT = cx<T>;
+    __shared__ T wred[2][4][2 * NCB * 4 * kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15;
+    const long q = blockIdx.x >> 3;
+    const long b = (q / nslab) * 8 + (blockIdx.x & 7);
+    const int slab = (int)(q % nslab);
+    if (b >= batch) return;                                  // whole workgroup: no barrier is skipped
+    const T *A = Abase + 2 * b * strideA;
+    const CT *Tb = reinterpret_cast<const CT *>(Tin + 2 * b * strideT);
+    T *U = Uout + 2 * b * strideU;
+    const int c0 = slab * 16 * NCB;
+    const int nblk = (n - 1 + 15) / 16;
+    v4 ur[NCB], ui[NCB];                                     // one 16-row block of the slab
+    auto load_u = [&](int R) {
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            const int col = c0 + 16 * cb + li;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = R + M::row(lane, v);
+                const CT x = (row < n && col < n) ? ldx(U, (long)row * n + col) : CT{(T)0, (T)0};
+                ur[cb][v] = x.x;
+                ui[cb][v] = x.y;
+            }
+        }
+    };
+    // W partial of block j0 from the row block R held in ur/ui: K-step v stands for
+    // rows R + row(lane, v) (the rows of accumulator element v)
+    auto acc_w = [&](int j0, int R, v4 (&wr)[NCB], v4 (&wi)[NCB]) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const CT x = lg_refl(A, n, j0, R + M::row(lane, v), li);   // conj(V)[k = li][row]
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) {
+                wr[cb] = M::mma(x.x, ur[cb][v], wr[cb]);
+                wr[cb] = M::mma(x.y, ui[cb][v], wr[cb]);
+                wi[cb] = M::mma(x.x, ui[cb][v], wi[cb]);
+                wi[cb] = M::mma(-x.y, ur[cb][v], wi[cb]);
+            }
+        }
+    };
+    v4 wr[NCB], wi[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; cb++) {
+        wr[cb] = v4{0, 0, 0, 0};
+        wi[cb] = v4{0, 0, 0, 0};
+    }
+    {
+        const int j0 = 16 * (nblk - 1);
+        for (int rbk = w; 16 * rbk < n; rbk += 4) {
+            const int R = 16 * rbk;
+            if (R + 15 < j0 + 1) continue;
+            load_u(R);
+            acc_w(j0, R, wr, wi);
+        }
+    }
+    for (int bk = nblk - 1; bk >= 0; bk--) {
+        const int j0 = 16 * bk;
+        T *wb = wred[bk & 1][w];
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                wb[((0 * NCB + cb) * 4 + v) * kWave + lane] = wr[cb][v];
+                wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = wi[cb][v];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                T sr = 0, si = 0;
+#pragma unroll
+                for (int ww = 0; ww < 4; ww++) {
+                    sr += wred[bk & 1][ww][((0 * NCB + cb) * 4 + v) * kWave + lane];
+                    si += wred[bk & 1][ww][((1 * NCB + cb) * 4 + v) * kWave + lane];
+                }
+                wr[cb][v] = sr;
+                wi[cb][v] = si;
+            }
+        // W2 = T W (K index = reflector k2 = row(l, s))
+        v4 w2r[NCB], w2i[NCB];
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            w2r[cb] = v4{0, 0, 0, 0};
+            w2i[cb] = v4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const CT tv = Tb[bk * 256 + M::row(lane, s) * 16 + li];   // T[k1 = li][k2 = row(l, s)]
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) {
+                w2r[cb] = M::mma(tv.x, wr[cb][s], w2r[cb]);
+                w2r[cb] = M::mma(-tv.y, wi[cb][s], w2r[cb]);
+                w2i[cb] = M::mma(tv.x, wi[cb][s], w2i[cb]);
+                w2i[cb] = M::mma(tv.y, wr[cb][s], w2i[cb]);
+            }
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            wr[cb] = v4{0, 0, 0, 0};
+            wi[cb] = v4{0, 0, 0, 0};
+        }
+        const int lo = bk > 0 ? j0 - 15 : j0 + 1;            // first row either block needs
+        for (int rbk = w; 16 * rbk < n; rbk += 4) {
+            const int R = 16 * rbk;
+            if (R + 15 < lo) continue;
+            load_u(R);
+            if (R + 15 >= j0 + 1) {
+                // U -= V W2 on rows R.. (K index = reflector row(l, s))
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const CT x = lg_refl(A, n, j0, R + li, M::row(lane, s));
+#pragma unroll
+                    for (int cb = 0; cb < NCB; cb++) {
+                        ur[cb] = M::mma(-x.x, w2r[cb][s], ur[cb]);
+                        ur[cb] = M::mma(x.y, w2i[cb][s], ur[cb]);
+                        ui[cb] = M::mma(-x.x, w2i[cb][s], ui[cb]);
+                        ui[cb] = M::mma(-x.y, w2r[cb][s], ui[cb]);
+                    }
+                }
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) {
+                    const int col = c0 + 16 * cb + li;
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int row = R + M::row(lane, v);
+                        if (row < n && col < n) stx(U, (long)row * n + col, CT{ur[cb][v], ui[cb][v]});
+                    }
+                }
+            }
+            if (bk > 0) acc_w(j0 - 16, R, wr, wi);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// host
+template <typename T>
+static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, long strideV, T *aux, double *W,
+                         T *U, double *work, int grid_dc, T *tws, hipStream_t s, hipEvent_t mid, hipEvent_t mid2) {
+    using C = cx<T>;
+    if (n <= 512 || n > heevd_large_max_n() || batch < 1) return batch < 1 ? 0 : -22;
+    const long strideA = (long)n * n;
+    const long strideAux = heevd_large_aux_complex(n);
+    const int nb = (int)batch;
+    const size_t lds_col = sizeof(C) * n;
+    if (lds_col > 32 * 1024) {                               // dynamic + static beyond the 64 KB default
+        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)lg_col_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds_col));
+        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)lg_symv_kernel<T>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_col));
+    }
+    for (int k0 = 0; k0 < n - 1; k0 += kLgNB) {
+        const int nbk = std::min(kLgNB, n - 1 - k0);
+        for (int i = 0; i <= nbk; i++) {
+            hipLaunchKernelGGL(lg_col_kernel<T>, dim3(nb), dim3(kLgColNT), lds_col, s, A, strideA, n, D, E, TAU,
+                               strideV, aux, strideAux, k0, i, nbk);
+            if (i == nbk) break;
+            const int m = n - (k0 + i) - 1;
+            const int nrb = (m + 63) / 64;
+            hipLaunchKernelGGL(lg_symv_kernel<T>, dim3((unsigned)((long)nb * nrb)), dim3(256), sizeof(C) * m, s, A,
+                               strideA, n, aux, strideAux, k0, i, nrb);
+        }
+        const int s0 = k0 + nbk;
+        if (s0 < n - 1) {
+            const int nt = (n - s0 + 63) / 64;
+            hipLaunchKernelGGL(lg_her2k_kernel<T>, dim3((unsigned)((long)nb * nt * nt)), dim3(256), 0, s, A, strideA,
+                               n, aux, strideAux, s0, nbk, nt);
+        }
+    }
+    RLM_HIP_CHECK(hipGetLastError());
+    if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
+    int rc;
+    if ((rc = launch_stedc_global(n, D, E, strideV, W, n, U, strideA, work, nb, grid_dc, s))) return rc;
+    if (mid2) RLM_HIP_CHECK(hipEventRecord(mid2, s));
+    const long strideT = backtransform_tws_complex(n);
+    const int nblk = (n - 1 + 15) / 16;
+    if ((rc = launch_larft(A, strideA, n, n, TAU, strideV, tws, strideT, nb, (nblk + 3) / 4, s))) return rc;
+    // 32 columns per slab for complex float, 16 for complex double (the W reduction
+    // buffers then stay at 32 KB of LDS)
+    constexpr int NCB = sizeof(T) == sizeof(float) ? 2 : 1;
+    const int nslab = (n + 16 * NCB - 1) / (16 * NCB);
+    const long groups = (batch + 7) / 8;
+    hipLaunchKernelGGL((lg_backtransform_kernel<T, NCB>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0, s, A,
+                       strideA, n, tws, strideT, U, strideA, nslab, nb);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int heevd_large(double *A, int n, long batch, double *D, double *E, double *TAU, long strideV, double *aux, double *W,
+                double *U, double *work, int grid_dc, double *tws, hipStream_t s, hipEvent_t mid, hipEvent_t mid2) {
+    return heevd_large_t<double>(A, n, batch, D, E, TAU, strideV, aux, W, U, work, grid_dc, tws, s, mid, mid2);
+}
+int heevd_large(float *A, int n, long batch, double *D, double *E, float *TAU, long strideV, float *aux, double *W,
+                float *U, double *work, int grid_dc, float *tws, hipStream_t s, hipEvent_t mid, hipEvent_t mid2) {
+    return heevd_large_t<float>(A, n, batch, D, E, TAU, strideV, aux, W, U, work, grid_dc, tws, s, mid, mid2);
+}
+
+}  // namespace rlm

@@ -15,6 +15,8 @@
 #include "common.hpp"
 #include "rlm_kernels.hpp"
 
+#include <algorithm>
+
 namespace rlm {
 
 // V[s][m][n] = coeff_mn[m][n] * sum_i s_i e^{-i 2pi((m-o) x_i + (n-o) y_i)}
@@ -229,12 +231,15 @@ __global__ __launch_bounds__(256) void hband_kernel(const double *__restrict__ E
     }
 }
 
-// Dense column-major A of each k-point from its band form; grid.x = k-points.
+// Dense column-major A of each k-point from its band form; blockIdx.x = kp * nch +
+// chunk, the nch chunks of a k-point striding over its N*N entries.
 __global__ __launch_bounds__(256) void expand_band_kernel(const double *__restrict__ H, int N, int off, int nb,
-                                                          double *__restrict__ A, long strideA) {
-    const double *Hk = H + 2L * blockIdx.x * N * nb;
-    double *Ak = A + 2L * blockIdx.x * strideA;
-    for (long idx = threadIdx.x; idx < (long)N * N; idx += blockDim.x) {
+                                                          double *__restrict__ A, long strideA, int nch) {
+    const long kp = blockIdx.x / nch;
+    const double *Hk = H + 2L * kp * N * nb;
+    double *Ak = A + 2L * kp * strideA;
+    const long step = (long)nch * blockDim.x;
+    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)N * N; idx += step) {
         const int c = (int)(idx / N), r = (int)(idx % N);
         cd v{0.0, 0.0};
         if (r <= c) {
@@ -268,8 +273,10 @@ int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, i
 
 int launch_expand_band(const double *H, int n_phi, int off, long nkp, double *A, long strideA, hipStream_t stream) {
     if (nkp <= 0) return 0;
-    hipLaunchKernelGGL(expand_band_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, H, n_phi, off,
-                       band_width(n_phi, off), A, strideA);
+    const long nn = (long)n_phi * n_phi;
+    const int nch = (int)std::min<long>(256, std::max<long>(1, nn / (256L * 64)));
+    hipLaunchKernelGGL(expand_band_kernel, dim3((unsigned)(nkp * nch)), dim3(256), 0, stream, H, n_phi, off,
+                       band_width(n_phi, off), A, strideA, nch);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

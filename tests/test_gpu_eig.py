@@ -124,6 +124,7 @@ def test_invalid_arguments():
     from rlm_amd import lib
     print("[test_invalid_arguments]", flush=True)
     assert lib().hip_diag_heevd_batched(0, None, 1, 1, None, 1, 1, b"L", 0, None) == -22
+    assert lib().hip_diag_heevd_batched(5000, None, 5000, 5000 * 5000, None, 5000, 1, b"L", 0, None) == -22
     assert lib().hip_diag_heevd_batched(600, None, 600, 600 * 600, None, 600, 1, b"L", 0, None) == -22
     assert lib().hip_diag_heevd_batched(4, None, 4, 16, None, 4, 1, b"X", 0, None) == -22
     assert lib().hip_diag_heevd_batched(4, None, 4, 16, None, 4, 0, b"L", 0, None) == 0
