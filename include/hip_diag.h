@@ -135,7 +135,8 @@ void *rlm_plan_stream(rlm_plan_t plan);
  * [2]=off_head, [3]=dim_m, [4]=n_mesh, [5]=samples_per_batch. */
 int rlm_plan_info(rlm_plan_t plan, long *out6);
 /* Mixed precision: the largest ||A u - lambda u|| / ||u|| (FP64, against the FP64
- * matrix) over every state and twist point of the last execute; 0 for FP64 plans. */
+ * matrix) over every state and twist point of the last execute, computed by this
+ * call (a full FP64 product on the plan's stream); 0 for FP64 plans. */
 int rlm_plan_max_residual(rlm_plan_t plan, double *out);
 /* Device pointer of the eigenvalues [kp][n_phi] / eigenvectors
  * [kp][component][state] of the last execute, for the solved twist points

@@ -604,7 +604,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
             return rc;
         RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
         // FP64 Rayleigh quotients against the FP64 matrix (band |c-r| <= off_head)
-        if ((rc = launch_rayleigh(p->band_trd ? nullptr : p->dA.d(), U32, N, off, nkp, p->dEig.d(), p->dResid.d(), s,
+        if ((rc = launch_rayleigh(p->band_trd ? nullptr : p->dA.d(), U32, N, off, nkp, p->dEig.d(), nullptr, s,
                                   p->band_trd ? p->dH.d() : nullptr)))
             return rc;
         RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
@@ -681,7 +681,15 @@ extern "C" int rlm_plan_max_residual(rlm_plan_t p, double *out) {
         *out = 0.0;
         return 0;
     }
-    const long nres = (long)p->loaded * p->pl * p->pl * rayleigh_resid_slots((int)p->n_phi);
+    // residuals of the last executed batch, from the full FP64 product (the plan's
+    // refinement itself uses the half-band form and computes no residual)
+    const long nkp = (long)p->loaded * p->pl * p->pl;
+    const int N = (int)p->n_phi;
+    int rc;
+    if ((rc = launch_rayleigh(p->band_trd ? nullptr : p->dA.d(), static_cast<const float *>(p->dU.p), N, (int)p->off,
+                              nkp, nullptr, p->dResid.d(), p->stream, p->band_trd ? p->dH.d() : nullptr)))
+        return rc;
+    const long nres = nkp * rayleigh_resid_slots(N);
     std::vector<double> h(nres);
     RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
     RLM_HIP_CHECK(hipMemcpy(h.data(), p->dResid.p, 8 * nres, hipMemcpyDeviceToHost));

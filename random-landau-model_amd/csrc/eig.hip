@@ -1264,7 +1264,9 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
     const double Dj1 = (j < K - 1) ? (dl[j + 1] - dlo) : 0.0;
     for (int it = 0; it < 120; it++) {
         double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, err = 0.0;
-        // poles at or below j feed psi, the others phi (all terms of one sign each)
+        // poles at or below j feed psi, the others phi (all terms of one sign each);
+        // the roots of a wave are consecutive, so its lanes' split points differ by
+        // at most 63 and the two loops together run about K + 64 iterations
 #pragma unroll 4
         for (int i = 0; i <= j; i++) {
             const double inv = rcp_nr((dl[i] - dlo) - tau);
@@ -1324,11 +1326,106 @@ __device__ inline int count_below(const double *a, int len, double v, bool inclu
     return lo;
 }
 
+// Matrix-core merge GEMM of one node, Qb = Qa S (see stedc_block): task = 32 rows
+// (2 row blocks) x 64 output columns (4 column tiles); each gathered Qa value feeds 4
+// MFMAs, each generated S value 2. S[i][q] = what_i / (dl_i - lambda_q) * inrm_q is
+// generated in FP64 per K-step and never stored. QS: storage and matrix-core type of
+// the eigenvector matrices (double; float in the mixed-precision solver, whose
+// eigenvectors carry the FP32 accuracy of its reflectors anyway).
+template <typename QS>
+__device__ __forceinline__ void merge_gemm_mfma(int lane, int G, int lo, int mid, int hi, int n1, int K, int KL, int KR,
+                                                int n, const QS *Qa, QS *Qb, const DcShared &sh) {
+    using MQ = Mf<QS>;
+    using v4 = typename MQ::v4;
+    constexpr int QT = 4, RB = 2;
+    const int wv = lane / kWave, nwv = G / kWave;
+    const int l = lane & (kWave - 1), li = l & 15, lq = l >> 4;
+    const double *dl = sh.dl + lo;
+    const int nqt = (K + 15) / 16, nqr = (nqt + QT - 1) / QT;
+    const int rows1 = hi - mid;                       // right half (may be short)
+    const int ng0 = (n1 + 31) / 32, ng1 = (rows1 + 31) / 32;
+    const int ntask = (ng0 + ng1) * nqr;
+    for (int task = wv; task < ntask; task += nwv) {
+        const int qr = task % nqr, g = task / nqr;
+        const bool right = g >= ng0;
+        const int rbase = right ? mid + (g - ng0) * 32 : lo + g * 32;
+        const int rend = right ? hi : mid;
+        const int cnt = right ? KR : KL;
+        const int *lst = (right ? sh.lstR : sh.lstL) + lo;
+        const int qt0 = qr * QT, nq = min(QT, nqt - qt0);     // wave-uniform
+        double qo[QT], qtu[QT], qiv[QT];
+#pragma unroll
+        for (int u = 0; u < QT; u++) {
+            const int q = (qt0 + u) * 16 + li;
+            const bool ok = u < nq && q < K;
+            qo[u] = ok ? sh.rc[lo + q] : 0.0;
+            qtu[u] = ok ? sh.tau[lo + q] : 0.0;
+            qiv[u] = ok ? sh.inrm[lo + q] : 0.0;          // 0 marks an empty column
+        }
+        v4 acc[RB][QT];
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int u = 0; u < QT; u++) acc[rb][u] = v4{0, 0, 0, 0};
+        // K-step operands one step ahead: list entry, weights and the gathered Qa
+        // values of step idx0 + 4 load while step idx0 runs on the matrix cores
+        bool kin_n;
+        double wi_n, di_n;
+        QS av_n[RB];
+        auto fetch = [&](int idx) {
+            kin_n = idx < cnt;
+            const int i = kin_n ? lst[idx] : 0;
+            wi_n = sh.what[lo + i];
+            di_n = dl[i];
+            const QS *col = Qa + (long)(lo + sh.pcol[lo + i]) * n;
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                const int r = rbase + rb * 16 + li;
+                av_n[rb] = (kin_n && r < rend) ? col[r] : (QS)0;
+            }
+        };
+        fetch(lq);
+        for (int idx0 = 0; idx0 < cnt; idx0 += 4) {
+            const bool kin = kin_n;
+            const double wi = wi_n, di = di_n;
+            QS av[RB];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) av[rb] = av_n[rb];
+            if (idx0 + 4 < cnt) fetch(idx0 + 4 + lq);
+#pragma unroll
+            for (int u = 0; u < QT; u++) {
+                if (u < nq) {
+                    const double bv0 = wi * rcp_nr((di - qo[u]) - qtu[u]) * qiv[u];
+                    const QS bv = (kin && qiv[u] != 0.0) ? (QS)bv0 : (QS)0;
+#pragma unroll
+                    for (int rb = 0; rb < RB; rb++) acc[rb][u] = MQ::mma(av[rb], bv, acc[rb][u]);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < QT; u++) {
+            const int q = (qt0 + u) * 16 + li;
+            if (u < nq && q < K) {
+                QS *ocol = Qb + (long)(lo + sh.rank[lo + q]) * n;
+#pragma unroll
+                for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int r = rbase + rb * 16 + MQ::row(l, v);
+                        if (r < rend) ocol[r] = acc[rb][u][v];
+                    }
+            }
+        }
+    }
+}
+
 // Stage 1: D&C. d (n), e (n-1) in; eigenvalues ascending in sh.dq (returned via
 // pointer swap) and eigenvectors of T in the returned buffer (row-major n x n).
-template <int NT>
-__device__ double *stedc_block(int n, const double *dIn, const double *eIn, double *Qa, double *Qb, double *S,
-                               DcShared &sh, double **dq_final) {
+// QS: storage type of the eigenvector matrices Qa, Qb and of S (see merge_gemm_mfma);
+// every other quantity is FP64.
+template <int NT, typename QS>
+__device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, QS *Qb, QS *S, DcShared &sh,
+                           double **dq_final) {
     const int t = threadIdx.x;
     // scale T to unit max-norm (as LAPACK dstedc does) so that the deflation
     // tolerance is relative to the matrix, whatever its magnitude
@@ -1356,8 +1453,8 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
     // eigenvector c, Q[c*n + r])
     for (long idx = t; idx < (long)n * n; idx += NT) {
         const long c = idx / n, r = idx % n;
-        Qa[idx] = (r == c) ? 1.0 : 0.0;
-        Qb[idx] = 0.0;
+        Qa[idx] = (r == c) ? (QS)1 : (QS)0;
+        Qb[idx] = (QS)0;
     }
     __syncthreads();
     double *dq = sh.dq, *dqn = sh.dqn;
@@ -1545,10 +1642,10 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                         for (int q = 0; q < nr; q++) {
                             const int cp = sh.perm[lo + sh.rpj[lo + q]], cn = sh.perm[lo + sh.rnj[lo + q]];
                             const double c = sh.rc[lo + q], s = sh.rs[lo + q];
-                            double *xp = Qa + (long)(lo + cp) * n + r, *yp = Qa + (long)(lo + cn) * n + r;
+                            QS *xp = Qa + (long)(lo + cp) * n + r, *yp = Qa + (long)(lo + cn) * n + r;
                             const double x = *xp, y = *yp;
-                            *xp = c * x + s * y;
-                            *yp = c * y - s * x;
+                            *xp = (QS)(c * x + s * y);
+                            *yp = (QS)(c * y - s * x);
                         }
                     }
                 }
@@ -1568,36 +1665,52 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                     secular_root(j, K, sh.dl + lo, sh.z2 + lo, rho, o, tu);
                     sh.orig[lo + j] = o;
                     sh.tau[lo + j] = tu;
+                    sh.rc[lo + j] = sh.dl[lo + o];      // origin pole of root j (rc is free after P3)
                 }
             }
             __syncthreads();
-            // P5: Gu-Eisenstat recomputed weights
+            // (root j = rc[j] + tau[j]; the pole distances below are (dl_i - rc_j) - tau_j)
+            // P5: Gu-Eisenstat recomputed weights (two partial products; the j == i
+            // factor is 1)
             if (merging) {
-                const double *dl = sh.dl + lo;
+                const double *dl = sh.dl + lo, *ro = sh.rc + lo, *ta = sh.tau + lo;
                 for (int i = lane; i < K; i += G) {
-                    double p = (dl[i] - dl[sh.orig[lo + i]]) - sh.tau[lo + i];
-                    for (int j = 0; j < K; j++) {
-                        if (j == i) continue;
-                        const double del = (dl[i] - dl[sh.orig[lo + j]]) - sh.tau[lo + j];
-                        p *= del * rcp_nr(dl[i] - dl[j]);
+                    const double di = dl[i];
+                    double pa = (di - ro[i]) - ta[i], pb = 1.0;
+                    int j = 0;
+                    for (; j + 1 < K; j += 2) {
+                        const bool sa = j == i, sb = j + 1 == i;
+                        const double fa = ((di - ro[j]) - ta[j]) * rcp_nr(sa ? 1.0 : di - dl[j]);
+                        const double fb = ((di - ro[j + 1]) - ta[j + 1]) * rcp_nr(sb ? 1.0 : di - dl[j + 1]);
+                        pa *= sa ? 1.0 : fa;
+                        pb *= sb ? 1.0 : fb;
                     }
+                    if (j < K && j != i) pa *= ((di - ro[j]) - ta[j]) * rcp_nr(di - dl[j]);
+                    const double p = pa * pb;
                     sh.what[lo + i] = copysign(sqrt(fmax(-p, 0.0)), sh.wz[lo + i]);
                 }
             }
             __syncthreads();
-            // P6: eigenvector norms; output values and ranks
+            // P6: eigenvector norms (two partial sums); output values and ranks
             if (merging) {
-                const double *dl = sh.dl + lo;
+                const double *dl = sh.dl + lo, *wh = sh.what + lo;
                 for (int j = lane; j < K; j += G) {
-                    const int oj = sh.orig[lo + j];
+                    const double oj = sh.rc[lo + j];
                     const double tj = sh.tau[lo + j];
-                    double s2 = 0.0;
-                    for (int i = 0; i < K; i++) {
-                        const double u = sh.what[lo + i] * rcp_nr((dl[i] - dl[oj]) - tj);
-                        s2 += u * u;
+                    double sa = 0.0, sb = 0.0;
+                    int i = 0;
+                    for (; i + 1 < K; i += 2) {
+                        const double ua = wh[i] * rcp_nr((dl[i] - oj) - tj);
+                        const double ub = wh[i + 1] * rcp_nr((dl[i + 1] - oj) - tj);
+                        sa += ua * ua;
+                        sb += ub * ub;
                     }
-                    sh.inrm[lo + j] = 1.0 / sqrt(s2);
-                    sh.vals[lo + j] = dl[oj] + tj;
+                    if (i < K) {
+                        const double ua = wh[i] * rcp_nr((dl[i] - oj) - tj);
+                        sa += ua * ua;
+                    }
+                    sh.inrm[lo + j] = 1.0 / sqrt(sa + sb);
+                    sh.vals[lo + j] = oj + tj;
                 }
                 for (int q = lane; q < nd; q += G) sh.vals[lo + K + q] = sh.ds[lo + sh.dfx[lo + q]];
             }
@@ -1624,8 +1737,8 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                 const double *dl = sh.dl + lo;
                 for (int idx = lane; idx < K * K; idx += G) {
                     const int i = idx / K, q = idx % K;
-                    const double del = (dl[i] - dl[sh.orig[lo + q]]) - sh.tau[lo + q];
-                    S[(long)(lo + i) * n + q] = sh.what[lo + i] * rcp_nr(del) * sh.inrm[lo + q];
+                    const double del = (dl[i] - sh.rc[lo + q]) - sh.tau[lo + q];
+                    S[(long)(lo + i) * n + q] = (QS)(sh.what[lo + i] * rcp_nr(del) * sh.inrm[lo + q]);
                 }
             }
             __syncthreads();
@@ -1633,88 +1746,7 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
             // Rows of the left half only meet columns of lstL (left child or rotated), rows
             // of the right half only lstR (the block-diagonal structure dlaed3 exploits).
             if (mfma) {
-                // task = 32 rows (2 row blocks) x 64 output columns (4 column tiles):
-                // each gathered Qa value feeds 4 MFMAs, each generated S value 2
-                using v4 = Mf<double>::v4;
-                constexpr int QT = 4, RB = 2;
-                const int wv = lane / kWave, nwv = G / kWave;
-                const int l = lane & (kWave - 1), li = l & 15, lq = l >> 4;
-                const double *dl = sh.dl + lo;
-                const int nqt = (K + 15) / 16, nqr = (nqt + QT - 1) / QT;
-                const int rows1 = hi - mid;                       // right half (may be short)
-                const int ng0 = (n1 + 31) / 32, ng1 = (rows1 + 31) / 32;
-                const int ntask = (ng0 + ng1) * nqr;
-                for (int task = wv; task < ntask; task += nwv) {
-                    const int qr = task % nqr, g = task / nqr;
-                    const bool right = g >= ng0;
-                    const int rbase = right ? mid + (g - ng0) * 32 : lo + g * 32;
-                    const int rend = right ? hi : mid;
-                    const int cnt = right ? KR : KL;
-                    const int *lst = (right ? sh.lstR : sh.lstL) + lo;
-                    const int qt0 = qr * QT, nq = min(QT, nqt - qt0);     // wave-uniform
-                    double qo[QT], qtu[QT], qiv[QT];
-#pragma unroll
-                    for (int u = 0; u < QT; u++) {
-                        const int q = (qt0 + u) * 16 + li;
-                        const bool ok = u < nq && q < K;
-                        qo[u] = ok ? dl[sh.orig[lo + q]] : 0.0;
-                        qtu[u] = ok ? sh.tau[lo + q] : 0.0;
-                        qiv[u] = ok ? sh.inrm[lo + q] : 0.0;          // 0 marks an empty column
-                    }
-                    v4 acc[RB][QT];
-#pragma unroll
-                    for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-                        for (int u = 0; u < QT; u++) acc[rb][u] = v4{0.0, 0.0, 0.0, 0.0};
-                    // K-step operands one step ahead: list entry, weights and the gathered
-                    // Qa values of step idx0 + 4 load while step idx0 runs on the matrix cores
-                    bool kin_n;
-                    double wi_n, di_n, av_n[RB];
-                    auto fetch = [&](int idx) {
-                        kin_n = idx < cnt;
-                        const int i = kin_n ? lst[idx] : 0;
-                        wi_n = sh.what[lo + i];
-                        di_n = dl[i];
-                        const double *col = Qa + (long)(lo + sh.pcol[lo + i]) * n;
-#pragma unroll
-                        for (int rb = 0; rb < RB; rb++) {
-                            const int r = rbase + rb * 16 + li;
-                            av_n[rb] = (kin_n && r < rend) ? col[r] : 0.0;
-                        }
-                    };
-                    fetch(lq);
-                    for (int idx0 = 0; idx0 < cnt; idx0 += 4) {
-                        const bool kin = kin_n;
-                        const double wi = wi_n, di = di_n;
-                        double av[RB];
-#pragma unroll
-                        for (int rb = 0; rb < RB; rb++) av[rb] = av_n[rb];
-                        if (idx0 + 4 < cnt) fetch(idx0 + 4 + lq);
-#pragma unroll
-                        for (int u = 0; u < QT; u++) {
-                            if (u < nq) {
-                                const double bv0 = wi * rcp_nr((di - qo[u]) - qtu[u]) * qiv[u];
-                                const double bv = (kin && qiv[u] != 0.0) ? bv0 : 0.0;
-#pragma unroll
-                                for (int rb = 0; rb < RB; rb++) acc[rb][u] = Mf<double>::mma(av[rb], bv, acc[rb][u]);
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < QT; u++) {
-                        const int q = (qt0 + u) * 16 + li;
-                        if (u < nq && q < K) {
-                            double *ocol = Qb + (long)(lo + sh.rank[lo + q]) * n;
-#pragma unroll
-                            for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-                                for (int v = 0; v < 4; v++) {
-                                    const int r = rbase + rb * 16 + Mf<double>::row(l, v);
-                                    if (r < rend) ocol[r] = acc[rb][u][v];
-                                }
-                        }
-                    }
-                }
+                merge_gemm_mfma<QS>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
             } else if (merging) {
                 // VALU path: each task is one row and 4 consecutive output columns,
                 // consecutive lanes on consecutive rows
@@ -1731,16 +1763,16 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
                     for (int idx = 0; idx < cntl; idx++) {
                         const int i = lst[idx];
                         const double x = Qa[(long)(lo + sh.pcol[lo + i]) * n + r];
-                        const double *srow = S + (long)(lo + i) * n;
+                        const QS *srow = S + (long)(lo + i) * n;
                         a0 = fma(x, srow[qa], a0);
                         a1 = fma(x, srow[qb], a1);
                         a2 = fma(x, srow[qc], a2);
                         a3 = fma(x, srow[qd], a3);
                     }
-                    Qb[(long)(lo + sh.rank[lo + q0]) * n + r] = a0;
-                    if (q0 + 1 < K) Qb[(long)(lo + sh.rank[lo + q0 + 1]) * n + r] = a1;
-                    if (q0 + 2 < K) Qb[(long)(lo + sh.rank[lo + q0 + 2]) * n + r] = a2;
-                    if (q0 + 3 < K) Qb[(long)(lo + sh.rank[lo + q0 + 3]) * n + r] = a3;
+                    Qb[(long)(lo + sh.rank[lo + q0]) * n + r] = (QS)a0;
+                    if (q0 + 1 < K) Qb[(long)(lo + sh.rank[lo + q0 + 1]) * n + r] = (QS)a1;
+                    if (q0 + 2 < K) Qb[(long)(lo + sh.rank[lo + q0 + 2]) * n + r] = (QS)a2;
+                    if (q0 + 3 < K) Qb[(long)(lo + sh.rank[lo + q0 + 3]) * n + r] = (QS)a3;
                 }
             }
             if (merging) {
@@ -1758,7 +1790,7 @@ __device__ double *stedc_block(int n, const double *dIn, const double *eIn, doub
             }
             __syncthreads();
         }
-        double *tq = Qa; Qa = Qb; Qb = tq;
+        QS *tq = Qa; Qa = Qb; Qb = tq;
         double *td = dq; dq = dqn; dqn = td;
     }
     for (int k = t; k < n; k += NT) dq[k] *= scale;
@@ -1779,14 +1811,16 @@ __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restri
     using C = cx<T>;
     extern __shared__ double smem[];
     const int t = threadIdx.x;
-    // per-workgroup global workspace: Qa, Qb, S (n*n doubles each) [+ the node arrays]
+    // per-workgroup global workspace: Qa, Qb, S (n*n double slots each, holding the
+    // eigenvector matrices in the precision T of the output) [+ the node arrays]
     const long wsz = GSH ? stedc_global_work_doubles(n) : 3L * n * n;
-    double *Qa0 = work + (long)blockIdx.x * wsz;
-    double *Qb0 = Qa0 + (long)n * n;
-    double *S = Qb0 + (long)n * n;
+    double *wg = work + (long)blockIdx.x * wsz;
+    T *Qa0 = reinterpret_cast<T *>(wg);
+    T *Qb0 = reinterpret_cast<T *>(wg + (long)n * n);
+    T *S = reinterpret_cast<T *>(wg + 2L * n * n);
     DcShared sh;
     {
-        double *p = GSH ? S + (long)n * n : smem;
+        double *p = GSH ? wg + 3L * n * n : smem;
         sh.dq = p; p += n; sh.dqn = p; p += n; sh.ds = p; p += n; sh.zs = p; p += n;
         sh.dl = p; p += n; sh.wz = p; p += n; sh.z2 = p; p += n; sh.tau = p; p += n; sh.vals = p; p += n;
         sh.rc = p; p += n; sh.rs = p; p += n; sh.what = p; p += n; sh.inrm = p; p += n;
@@ -1803,7 +1837,7 @@ __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restri
         double *W = Wout + b * strideW;
         T *U = Uout + 2 * b * strideU;
         double *dqf;
-        double *Z = stedc_block<NT>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf);
+        T *Z = stedc_block<NT, T>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf);
         for (int k = t; k < n; k += NT) W[k] = dqf[k];
         __syncthreads();
         // U[r*n + j] = Z[j*n + r] (column-major Z -> component-major U) through 32x32

@@ -283,14 +283,30 @@ int launch_expand_band(const double *H, int n_phi, int off, long nkp, double *A,
 
 // Mixed-precision refinement: FP64 Rayleigh quotients of the FP32 eigenvectors
 // against the FP64 matrix, lambda_j = Re(u^H A u) / (u^H u), replacing the
-// eigenvalues of the FP32-reduced tridiagonal. Y = A U on the FP64 matrix cores:
-// one wave per 16 states, looping over 16-row blocks of Y; for row block r0 only
-// the columns of the circular band window [r0 - bw, r0 + 15 + bw] enter (A is zero
-// outside |c - r| <= bw mod n; the window is the whole row when it would wrap onto
-// itself). The FP32 vectors are widened exactly. Four waves per workgroup, the
-// workgroups of one matrix on one XCD (blockIdx % 8). resid[kp*nsb + sb] = max over
+// eigenvalues of the FP32-reduced tridiagonal. The product runs on the FP64 matrix
+// cores: one wave per 16 states, looping over 16-row blocks; for row block r0 only
+// the columns of the circular band window enter (A is zero outside |c - r| <= bw mod
+// n; the window is the whole row when it would wrap onto itself). The FP32 vectors
+// are widened exactly. Four waves per workgroup, the workgroups of one matrix on one
+// XCD (blockIdx % 8). With Hb (band form, off = bw) the matrix entries come from it
+// instead of A.
+// HALF (eigenvalues only): A = B + B^H with B the upper half of the band (the
+// entries at circular offsets 1..bw of each row and half the diagonal), so
+// u^H A u = 2 Re(u^H B u) and the window of row block r0 is [r0, r0 + 15 + bw]
+// (16 + bw columns instead of 16 + 2 bw; the whole upper triangle when dense).
+// !HALF: the full product A u, eigenvalues (if W) and resid[kp*nsb + sb] = max over
 // the 16 states of block sb of ||A u - lambda u|| / ||u|| (diagnostic).
-// With Hb (band form, off = bw) the matrix entries come from it instead of A.
+__device__ inline cd band_entry(const double *Ak, const double *Hk, int n, int bw, int nb, int r, int c) {
+    if (!Hk) return ld(Ak, (long)c * n + r);
+    if (r <= c) {
+        const int b = band_b(c - r, n, bw);
+        return b >= 0 ? cconj(ld(Hk, (long)r * nb + b)) : cd{0.0, 0.0};
+    }
+    const int b = band_b(r - c, n, bw);
+    return b >= 0 ? ld(Hk, (long)c * nb + b) : cd{0.0, 0.0};
+}
+
+template <bool HALF>
 __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict__ A, const double *__restrict__ Hb,
                                                        const float *__restrict__ U, int n, int bw, long nkp,
                                                        int wg_per_kp, double *__restrict__ W,
@@ -312,27 +328,31 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
     const int j = sb * 16 + li;
     const bool jin = j < n;
     const bool dense = 16 + 2 * bw + 3 >= n;
-    const int kw = dense ? n : 16 + 2 * bw;
     double num = 0.0, yy = 0.0, uu = 0.0;
     for (int r0 = 0; r0 < n; r0 += 16) {
         v4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
-        const int cs = dense ? 0 : r0 - bw;
+        int cs, kw;
+        if (HALF) {
+            cs = r0;
+            kw = dense ? n - r0 : 16 + bw;
+        } else {
+            cs = dense ? 0 : r0 - bw;
+            kw = dense ? n : 16 + 2 * bw;
+        }
         const int r = r0 + li;
         for (int k0 = 0; k0 < kw; k0 += 4) {
             const int kk = k0 + lq;
             int c = cs + kk;
             c = c < 0 ? c + n : (c >= n ? c - n : c);
             const bool kin = kk < kw;
-            cd a{0.0, 0.0};                                                               // A(r, c)
+            cd a{0.0, 0.0};                                                               // A(r, c) or B(r, c)
             if (kin && r < n) {
-                if (!Hk) {
-                    a = ld(Ak, (long)c * n + r);
-                } else if (r <= c) {
-                    const int b = band_b(c - r, n, bw);
-                    if (b >= 0) a = cconj(ld(Hk, (long)r * nb + b));
+                if (!HALF) {
+                    a = band_entry(Ak, Hk, n, bw, nb, r, c);
                 } else {
-                    const int b = band_b(r - c, n, bw);
-                    if (b >= 0) a = ld(Hk, (long)c * nb + b);
+                    const int dl = kk - li;                                               // c - r along the window
+                    if (dl == 0) a = cd{0.5 * band_entry(Ak, Hk, n, bw, nb, r, c).x, 0.0};
+                    else if (dl > 0) a = band_entry(Ak, Hk, n, bw, nb, r, c);
                 }
             }
             const cx<float> u = (kin && jin) ? ldx(Uk, (long)c * n + j) : cx<float>{0.f, 0.f};   // U[c][j]
@@ -349,7 +369,7 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
                 const cx<float> u = ldx(Uk, (long)rr * n + j);
                 const double ur = u.x, ui = u.y;
                 num += ur * yr[v] + ui * yi[v];
-                yy += yr[v] * yr[v] + yi[v] * yi[v];
+                if (!HALF) yy += yr[v] * yr[v] + yi[v] * yi[v];
                 uu += ur * ur + ui * ui;
             }
         }
@@ -357,19 +377,22 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
     // sum over the four lanes holding state j (lane bits 4-5)
     for (int off = 16; off < kWave; off <<= 1) {
         num += __shfl_xor(num, off, kWave);
-        yy += __shfl_xor(yy, off, kWave);
+        if (!HALF) yy += __shfl_xor(yy, off, kWave);
         uu += __shfl_xor(uu, off, kWave);
     }
+    if (HALF) num *= 2.0;
     double worst = 0.0;
     if (jin && uu > 0.0) {
         const double lam = num / uu;
-        if (lq == 0) W[kp * n + j] = lam;
-        worst = sqrt(fmax(yy / uu - lam * lam, 0.0));
-    } else if (jin && lq == 0) {
+        if (W && lq == 0) W[kp * n + j] = lam;
+        if (!HALF) worst = sqrt(fmax(yy / uu - lam * lam, 0.0));
+    } else if (jin && W && lq == 0) {
         W[kp * n + j] = 0.0;
     }
-    for (int off = 1; off < 16; off <<= 1) worst = fmax(worst, __shfl_xor(worst, off, kWave));
-    if (resid && lane == 0) resid[kp * nsb + sb] = worst;
+    if (!HALF) {
+        for (int off = 1; off < 16; off <<= 1) worst = fmax(worst, __shfl_xor(worst, off, kWave));
+        if (resid && lane == 0) resid[kp * nsb + sb] = worst;
+    }
 }
 
 int rayleigh_resid_slots(int n) { return (n + 15) / 16; }
@@ -379,8 +402,12 @@ int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, do
     if (nkp <= 0) return 0;
     const int nsb = (n + 15) / 16, wg = (nsb + 3) / 4;
     const long groups = (nkp + 7) / 8;
-    hipLaunchKernelGGL(rayleigh_kernel, dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A, Hb, U, n, bw, nkp,
-                       wg, W, resid);
+    if (resid)
+        hipLaunchKernelGGL(rayleigh_kernel<false>, dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A, Hb, U,
+                           n, bw, nkp, wg, W, resid);
+    else
+        hipLaunchKernelGGL(rayleigh_kernel<true>, dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A, Hb, U,
+                           n, bw, nkp, wg, W, resid);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

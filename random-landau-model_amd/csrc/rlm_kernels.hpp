@@ -20,8 +20,10 @@ int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, i
                  long nkp, double *H, hipStream_t stream);
 int launch_expand_band(const double *H, int n_phi, int off, long nkp, double *A, long strideA, hipStream_t stream);
 // FP64 Rayleigh quotients of FP32 eigenvectors against the FP64 matrices (circular
-// band |c - r| <= bw), dense A or, with Hb, the band form of offset bw.
-// resid (optional): nkp * rayleigh_resid_slots(n) values.
+// band |c - r| <= bw), dense A or, with Hb, the band form of offset bw, into W.
+// Without resid: from the upper half of the band (u^H A u = 2 Re u^H B u). With
+// resid (nkp * rayleigh_resid_slots(n) values): from the full product, writing the
+// residual norms ||A u - lambda u|| / ||u|| as well (W may then be null).
 int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, double *W, double *resid,
                     hipStream_t stream, const double *Hb = nullptr);
 int rayleigh_resid_slots(int n);
