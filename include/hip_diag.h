@@ -145,9 +145,17 @@ int rlm_plan_max_residual(rlm_plan_t plan, double *out);
  * u(t1, g)[i] = u(t1, 0)[(i+1) % n_phi] and u(g, t2)[i] = e^{i 2 pi i/n_phi}
  * u(0, t2)[i] up to a phase per state). With RLM_MESH=full in the environment at
  * rlm_plan_create, every point is solved and kp = (s*(g+1)+t1)*(g+1)+t2.
- * Eigenvectors are complex128 (FP64 plans) or complex64 (mixed plans). */
+ * Eigenvectors are complex128 (FP64 plans and refined mixed plans) or complex64
+ * (mixed plans without refinement). */
 const double *rlm_plan_eigenvalues(rlm_plan_t plan);
 const double *rlm_plan_eigenvectors(rlm_plan_t plan);
+/* Mixed plans refine their FP32 eigenvectors in FP64 (one windowed Ogita-Aishima
+ * step after exact Rayleigh-Ritz rotations of near-degenerate chunks) when n_phi > 512,
+ * or as RLM_REFINE=0 / 1 in the environment at rlm_plan_create says. out2[0] = 1 if
+ * the plan refines, out2[1] = chunks of the last execute that were cut at the chunk
+ * size limit or had no Cholesky factor (0 when every coupled run was solved exactly).
+ * No counterpart in the reference (whose FP32 builds have no refinement). */
+int rlm_plan_refine_info(rlm_plan_t plan, long *out2);
 int rlm_plan_destroy(rlm_plan_t plan);
 
 /* Replaces the hlm sample / theta loops of hlm/main_hlm_cpu.cpp:75-196 (with

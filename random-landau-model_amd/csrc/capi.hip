@@ -15,6 +15,7 @@
 #include "eig.hpp"
 #include "hlm_kernels.hpp"
 #include "host_rng.hpp"
+#include "refine.hpp"
 #include "rlm_kernels.hpp"
 
 using namespace rlm;
@@ -466,6 +467,10 @@ struct rlm_plan_s {
     std::vector<double> cmn, cmt, cjm, hx, hy, hs;
     // device
     DevBuf dcmn, dcmt, dE, dX, dY, dS, dV, dW, dH, dA, dU, dEig, dL, dOut, dA32, dResid;
+    // FP64 refinement of the mixed-precision eigenvectors (refine.hip): on by default for
+    // n_phi > 512, RLM_REFINE=0 / 1 at rlm_plan_create turns it off / on
+    bool refine = false;
+    DevBuf dXd, dXr, dSw, dGw, dQg, dCl, dFlag;
     EigWork ws;
     int nb = 0;              // band width of the band form (dW: S*(g+1) x d x nb, dH: nkp x N x nb)
     bool band_trd = false;   // mixed plans: the FP32 reduction reads the band form (no dense matrices)
@@ -502,6 +507,10 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
     const long nkp = S * p->pl * p->pl;   // solved twist points
     const long nlk = S * g1 * g1;         // twist points with links
     p->nb = band_width((int)N, (int)p->off);
+    if (precision == HIP_DIAG_MIXED) {
+        const char *rf = getenv("RLM_REFINE");
+        p->refine = rf ? strcmp(rf, "0") != 0 : N > kSmallMaxN;
+    }
     p->band_trd = precision == HIP_DIAG_MIXED && chetrd_band_supported((int)N);
     const long nb = p->nb;
     p->cmn.resize(2 * d * d);
@@ -527,6 +536,12 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
         (rc = p->dResid.ensure(8 * nkp * rayleigh_resid_slots((int)N))) ||
         (rc = p->dEig.ensure(8 * nkp * N)) || (rc = p->dL.ensure(32 * nlk * N)) ||
         (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)))
+        return fail(rc);
+    if (p->refine &&
+        ((rc = p->dXd.ensure(16 * nkp * N * N)) || (rc = p->dXr.ensure(16 * nkp * N * N)) ||
+         (rc = p->dSw.ensure(16 * nkp * N * refine_window())) || (rc = p->dGw.ensure(16 * nkp * N * refine_window())) ||
+         (rc = p->dQg.ensure(16 * nkp * N * refine_max_cluster())) || (rc = p->dCl.ensure(4 * nkp * N)) ||
+         (rc = p->dFlag.ensure(4 * nkp))))
         return fail(rc);
     if (hipMemcpy(p->dcmn.p, p->cmn.data(), 8 * p->cmn.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->dcmt.p, p->cmt.data(), 8 * p->cmt.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -603,16 +618,34 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
                                  p->band_trd ? p->dH.d() : nullptr, off)))
             return rc;
         RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
-        // FP64 Rayleigh quotients against the FP64 matrix (band |c-r| <= off_head)
-        if ((rc = launch_rayleigh(p->band_trd ? nullptr : p->dA.d(), U32, N, off, nkp, p->dEig.d(), nullptr, s,
-                                  p->band_trd ? p->dH.d() : nullptr)))
-            return rc;
-        RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
-        if (derive)
-            rc = launch_links_derived(U32, true, (long)N * N, N, g, S, L1, L2, s);
-        else
-            rc = launch_links_f32(U32, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s);
-        if (rc) return rc;
+        if (p->refine) {
+            // FP64 eigenvector refinement, then Rayleigh quotients and links of the
+            // refined FP64 vectors
+            double *Xr = p->dXr.d();
+            if ((rc = launch_refine(p->dH.d(), U32, N, off, nkp, p->dXd.d(), Xr, p->dSw.d(), p->dGw.d(), p->dQg.d(),
+                                    static_cast<int *>(p->dCl.p), static_cast<int *>(p->dFlag.p), s)))
+                return rc;
+            if ((rc = launch_rayleigh(nullptr, static_cast<const double *>(Xr), N, off, nkp, p->dEig.d(), nullptr, s,
+                                      p->dH.d())))
+                return rc;
+            RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
+            if (derive)
+                rc = launch_links_derived(Xr, false, (long)N * N, N, g, S, L1, L2, s);
+            else
+                rc = launch_links(Xr, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s);
+            if (rc) return rc;
+        } else {
+            // FP64 Rayleigh quotients against the FP64 matrix (band |c-r| <= off_head)
+            if ((rc = launch_rayleigh(p->band_trd ? nullptr : p->dA.d(), U32, N, off, nkp, p->dEig.d(), nullptr, s,
+                                      p->band_trd ? p->dH.d() : nullptr)))
+                return rc;
+            RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
+            if (derive)
+                rc = launch_links_derived(U32, true, (long)N * N, N, g, S, L1, L2, s);
+            else
+                rc = launch_links_f32(U32, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s);
+            if (rc) return rc;
+        }
     }
     RLM_HIP_CHECK(hipEventRecord(p->ev[7], s));
     double *out = p->dOut.d();
@@ -686,9 +719,13 @@ extern "C" int rlm_plan_max_residual(rlm_plan_t p, double *out) {
     const long nkp = (long)p->loaded * p->pl * p->pl;
     const int N = (int)p->n_phi;
     int rc;
-    if ((rc = launch_rayleigh(p->band_trd ? nullptr : p->dA.d(), static_cast<const float *>(p->dU.p), N, (int)p->off,
-                              nkp, nullptr, p->dResid.d(), p->stream, p->band_trd ? p->dH.d() : nullptr)))
-        return rc;
+    if (p->refine)
+        rc = launch_rayleigh(nullptr, static_cast<const double *>(p->dXr.p), N, (int)p->off, nkp, nullptr,
+                             p->dResid.d(), p->stream, p->dH.d());
+    else
+        rc = launch_rayleigh(p->band_trd ? nullptr : p->dA.d(), static_cast<const float *>(p->dU.p), N, (int)p->off,
+                             nkp, nullptr, p->dResid.d(), p->stream, p->band_trd ? p->dH.d() : nullptr);
+    if (rc) return rc;
     const long nres = nkp * rayleigh_resid_slots(N);
     std::vector<double> h(nres);
     RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
@@ -700,13 +737,31 @@ extern "C" int rlm_plan_max_residual(rlm_plan_t p, double *out) {
 }
 
 extern "C" const double *rlm_plan_eigenvalues(rlm_plan_t p) { return p ? p->dEig.d() : nullptr; }
-extern "C" const double *rlm_plan_eigenvectors(rlm_plan_t p) { return p ? p->dU.d() : nullptr; }
+extern "C" const double *rlm_plan_eigenvectors(rlm_plan_t p) {
+    return p ? (p->refine ? p->dXr.d() : p->dU.d()) : nullptr;
+}
+
+extern "C" int rlm_plan_refine_info(rlm_plan_t p, long *out) {
+    if (!p || !out) return -22;
+    out[0] = p->refine ? 1 : 0;
+    out[1] = 0;
+    if (p->refine && p->loaded > 0) {
+        const long nkp = (long)p->loaded * p->pl * p->pl;
+        std::vector<int> h(nkp);
+        RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+        RLM_HIP_CHECK(hipMemcpy(h.data(), p->dFlag.p, sizeof(int) * nkp, hipMemcpyDeviceToHost));
+        for (int v : h) out[1] += v;
+    }
+    return 0;
+}
 
 extern "C" int rlm_plan_destroy(rlm_plan_t p) {
     if (!p) return 0;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    DevBuf *bufs[] = {&p->dcmn, &p->dcmt, &p->dE,   &p->dX, &p->dY,   &p->dS,   &p->dV,     &p->dW,
-                      &p->dH,   &p->dA,   &p->dU,   &p->dEig, &p->dL, &p->dOut, &p->dA32, &p->dResid};
+    DevBuf *bufs[] = {&p->dcmn, &p->dcmt, &p->dE,   &p->dX,   &p->dY,   &p->dS,   &p->dV,
+                      &p->dW,   &p->dH,   &p->dA,   &p->dU,   &p->dEig, &p->dL,   &p->dOut,
+                      &p->dA32, &p->dResid, &p->dXd, &p->dXr, &p->dSw,  &p->dGw,  &p->dQg,
+                      &p->dCl,  &p->dFlag};
     for (DevBuf *b : bufs) b->release();
     p->ws.release();
     for (int k = 0; k <= rlm_plan_s::kStages; k++)

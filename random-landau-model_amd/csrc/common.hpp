@@ -226,4 +226,17 @@ __device__ inline int band_b(int D, int N, int off) {
     return D >= N - off ? D - N + 2 * off + 1 : -1;
 }
 
+// Entry (r, c) of a circular-band matrix: from the dense column-major matrix Ak when
+// Hk is null, else from the band form Hk (n x nb, offset bw) with the convention of
+// the dense expansion: conj(H[r][b(c - r)]) for c >= r, H[c][b(r - c)] below.
+__device__ inline cd band_entry(const double *Ak, const double *Hk, int n, int bw, int nb, int r, int c) {
+    if (!Hk) return ld(Ak, (long)c * n + r);
+    if (r <= c) {
+        const int b = band_b(c - r, n, bw);
+        return b >= 0 ? cconj(ld(Hk, (long)r * nb + b)) : cd{0.0, 0.0};
+    }
+    const int b = band_b(r - c, n, bw);
+    return b >= 0 ? ld(Hk, (long)c * nb + b) : cd{0.0, 0.0};
+}
+
 }  // namespace rlm

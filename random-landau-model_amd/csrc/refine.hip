@@ -1,0 +1,407 @@
+// FP64 eigenvector refinement of the mixed-precision solver for large matrices.
+//
+// The FP32 reduction leaves eigenvector errors of order eps_32 ||A|| / gap, which
+// for near-degenerate states at n_phi = 3000 are large enough to move plaquette
+// phases across +-pi (the reference's FP32 builds lose Chern accuracy above
+// n_phi ~ 1500, README.md:117). This file refines the FP32 eigenvectors X of each
+// matrix with one step of the Ogita-Aishima iteration (X' = X + X E from
+// S = X^H A X and G = X^H X), restricted to a window of 2 kRefW + 1 neighbouring
+// states (eigenvalues ascending), after an exact FP64 Rayleigh-Ritz rotation inside
+// every cluster of strongly coupled states:
+//   ref_apply_kernel    Y = A X on the FP64 matrix cores (band form), X widened to FP64
+//   ref_window_kernel   S_kj = x_k^H y_j and G_kj = x_k^H x_j for |k - j| <= kRefW
+//   ref_cluster_kernel  runs of states whose first-order coupling
+//                       |S_kj - l_j G_kj| / |l_j - l_k| exceeds kRefCoupling
+//                       (l_j = S_jj / G_jj), in chunks of at most kRefMaxCluster:
+//                       S_C q = l G_C q in FP64 (Cholesky, Jacobi), X_C <- X_C Q,
+//                       Y_C <- Y_C Q
+//   ref_window_kernel   again, for the rotated vectors
+//   ref_update_kernel   X' = X + X E: E_kj = (S_kj - l_j G_kj) / (l_j - l_k) for k in
+//                       another chunk, -G_kj / 2 within a chunk, E_jj = (1 - G_jj) / 2
+// The plans then take the FP64 Rayleigh quotients and the links of X'.
+// Layouts: X, Y, X' component-major per matrix ([i*n + j] = component i of state j,
+// complex double); S, G: [kp][j][kRefWin] with entry d holding k = j + d - kRefW.
+#include "common.hpp"
+#include "refine.hpp"
+
+namespace rlm {
+
+constexpr int kRefW = 16;
+constexpr int kRefWin = 2 * kRefW + 1;
+constexpr int kRefMaxCluster = 8;
+constexpr double kRefCoupling = 1e-3;
+
+int refine_window() { return kRefWin; }
+int refine_max_cluster() { return kRefMaxCluster; }
+
+// ----------------------------------------------------------------------------
+// Y = A X for 16 states per wave (the band product of the Rayleigh kernel, stored),
+// Xd = X widened to FP64. Four waves per workgroup, the workgroups of one matrix on
+// one XCD.
+template <typename TX>
+__global__ __launch_bounds__(256) void ref_apply_kernel(const double *__restrict__ Hb, const TX *__restrict__ X,
+                                                        int n, int bw, long nkp, int wg_per_kp,
+                                                        double *__restrict__ Xd, double *__restrict__ Y) {
+    using M = Mf<double>;
+    using v4 = M::v4;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const long q = blockIdx.x >> 3;
+    const long kp = (q / wg_per_kp) * 8 + (blockIdx.x & 7);
+    const int sb = (int)(q % wg_per_kp) * 4 + w;
+    const int nsb = (n + 15) / 16;
+    if (kp >= nkp || sb >= nsb) return;                      // no barriers in this kernel
+    const int nb = band_nb(n, bw);
+    const double *Hk = Hb + 2 * kp * (long)n * nb;
+    const TX *Xk = X + 2 * kp * (long)n * n;
+    double *Xdk = Xd + 2 * kp * (long)n * n;
+    double *Yk = Y + 2 * kp * (long)n * n;
+    const int j = sb * 16 + li;
+    const bool jin = j < n;
+    const bool dense = 16 + 2 * bw + 3 >= n;
+    const int kw = dense ? n : 16 + 2 * bw;
+    for (int r0 = 0; r0 < n; r0 += 16) {
+        v4 yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+        const int cs = dense ? 0 : r0 - bw;
+        const int r = r0 + li;
+        for (int k0 = 0; k0 < kw; k0 += 4) {
+            const int kk = k0 + lq;
+            int c = cs + kk;
+            c = c < 0 ? c + n : (c >= n ? c - n : c);
+            const bool kin = kk < kw;
+            const cd a = (kin && r < n) ? band_entry(nullptr, Hk, n, bw, nb, r, c) : cd{0.0, 0.0};
+            const cx<TX> u = (kin && jin) ? ldx(Xk, (long)c * n + j) : cx<TX>{(TX)0, (TX)0};
+            const double ur = u.x, ui = u.y;
+            yr = M::mma(a.x, ur, yr);
+            yr = M::mma(-a.y, ui, yr);
+            yi = M::mma(a.x, ui, yi);
+            yi = M::mma(a.y, ur, yi);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int rr = r0 + M::row(lane, v);
+            if (rr < n && jin) {
+                st(Yk, (long)rr * n + j, cd{yr[v], yi[v]});
+                const cx<TX> u = ldx(Xk, (long)rr * n + j);
+                st(Xdk, (long)rr * n + j, cd{(double)u.x, (double)u.y});
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// S_kj = sum_i conj(X[i][k]) Y[i][j], G_kj = sum_i conj(X[i][k]) X[i][j] for the 16
+// states j of a wave and the 48 states k = j0 - 16 .. j0 + 31 (three 16-wide tiles of
+// FP64 MFMAs, K = components); the entries with |k - j| <= kRefW are stored.
+__global__ __launch_bounds__(256) void ref_window_kernel(const double *__restrict__ Xd, const double *__restrict__ Y,
+                                                         int n, long nkp, int wg_per_kp, double *__restrict__ Sw,
+                                                         double *__restrict__ Gw) {
+    using M = Mf<double>;
+    using v4 = M::v4;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const long q = blockIdx.x >> 3;
+    const long kp = (q / wg_per_kp) * 8 + (blockIdx.x & 7);
+    const int sb = (int)(q % wg_per_kp) * 4 + w;
+    const int nsb = (n + 15) / 16;
+    if (kp >= nkp || sb >= nsb) return;
+    const double *Xk = Xd + 2 * kp * (long)n * n;
+    const double *Yk = Y + 2 * kp * (long)n * n;
+    const int j0 = sb * 16, jb = j0 + li;
+    v4 sr[3], si[3], gr[3], gi[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) sr[t] = si[t] = gr[t] = gi[t] = v4{0.0, 0.0, 0.0, 0.0};
+    for (int i0 = 0; i0 < n; i0 += 4) {
+        const int i = i0 + lq;
+        const bool iin = i < n;
+        const cd yb = (iin && jb < n) ? ld(Yk, (long)i * n + jb) : cd{0.0, 0.0};
+        const cd xb = (iin && jb < n) ? ld(Xk, (long)i * n + jb) : cd{0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const int k = j0 - 16 + 16 * t + li;
+            const cd xa = (iin && k >= 0 && k < n) ? ld(Xk, (long)i * n + k) : cd{0.0, 0.0};
+            // conj(xa) * yb and conj(xa) * xb
+            sr[t] = M::mma(xa.x, yb.x, sr[t]);
+            sr[t] = M::mma(xa.y, yb.y, sr[t]);
+            si[t] = M::mma(xa.x, yb.y, si[t]);
+            si[t] = M::mma(-xa.y, yb.x, si[t]);
+            gr[t] = M::mma(xa.x, xb.x, gr[t]);
+            gr[t] = M::mma(xa.y, xb.y, gr[t]);
+            gi[t] = M::mma(xa.x, xb.y, gi[t]);
+            gi[t] = M::mma(-xa.y, xb.x, gi[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int k = j0 - 16 + 16 * t + M::row(lane, v);
+            const int d = k - jb;
+            if (jb < n && k >= 0 && k < n && d >= -kRefW && d <= kRefW) {
+                const long o = (kp * n + jb) * kRefWin + kRefW + d;
+                st(Sw, o, cd{sr[t][v], si[t][v]});
+                st(Gw, o, cd{gr[t][v], gi[t][v]});
+            }
+        }
+}
+
+// ----------------------------------------------------------------------------
+// S q = l G q for a chunk of m <= kRefMaxCluster states, one thread, FP64:
+// G = L L^H (Cholesky), M = L^-1 S L^-H, cyclic Jacobi M = V diag(l) V^H, Q = L^-H V
+// with the eigenvalues ascending. Returns false if G is not positive definite.
+__device__ bool small_geneig(int m, cd (&S)[kRefMaxCluster][kRefMaxCluster], cd (&G)[kRefMaxCluster][kRefMaxCluster],
+                             cd (&Q)[kRefMaxCluster][kRefMaxCluster]) {
+    constexpr int MC = kRefMaxCluster;
+    cd L[MC][MC];
+    for (int a = 0; a < m; a++)
+        for (int b = 0; b < m; b++) L[a][b] = cd{0.0, 0.0};
+    for (int c = 0; c < m; c++) {
+        double dsum = G[c][c].x;
+        for (int k = 0; k < c; k++) dsum -= cabs2(L[c][k]);
+        if (!(dsum > 0.0)) return false;
+        const double lcc = sqrt(dsum);
+        L[c][c] = cd{lcc, 0.0};
+        for (int r = c + 1; r < m; r++) {
+            cd sacc = G[r][c];
+            for (int k = 0; k < c; k++) sacc = csub(sacc, cmul(L[r][k], cconj(L[c][k])));
+            L[r][c] = cscale(sacc, 1.0 / lcc);
+        }
+    }
+    // T = L^-1 S (columns by forward substitution), M = (L^-1 T^H)^H
+    cd T[MC][MC], Mt[MC][MC];
+    for (int col = 0; col < m; col++)
+        for (int r = 0; r < m; r++) {
+            cd sacc = S[r][col];
+            for (int k = 0; k < r; k++) sacc = csub(sacc, cmul(L[r][k], T[k][col]));
+            T[r][col] = cscale(sacc, 1.0 / L[r][r].x);
+        }
+    for (int col = 0; col < m; col++)           // column col of L^-1 T^H = row col of M^H
+        for (int r = 0; r < m; r++) {
+            cd sacc = cconj(T[col][r]);
+            for (int k = 0; k < r; k++) sacc = csub(sacc, cmul(L[r][k], Mt[k][col]));
+            Mt[r][col] = cscale(sacc, 1.0 / L[r][r].x);
+        }
+    cd Mm[MC][MC], V[MC][MC];
+    for (int a = 0; a < m; a++)
+        for (int b = 0; b < m; b++) {
+            Mm[a][b] = cconj(Mt[b][a]);
+            V[a][b] = cd{a == b ? 1.0 : 0.0, 0.0};
+        }
+    for (int a = 0; a < m; a++) Mm[a][a].y = 0.0;
+    for (int sweep = 0; sweep < 30; sweep++) {
+        double off = 0.0, dia = 0.0;
+        for (int a = 0; a < m; a++) {
+            dia += Mm[a][a].x * Mm[a][a].x;
+            for (int b = a + 1; b < m; b++) off += cabs2(Mm[a][b]);
+        }
+        if (off <= 1e-34 * dia || off == 0.0) break;
+        for (int p = 0; p < m; p++)
+            for (int qq = p + 1; qq < m; qq++) {
+                const cd c = Mm[p][qq];
+                const double r = sqrt(cabs2(c));
+                if (r == 0.0) continue;
+                const cd ph{c.x / r, -c.y / r};                   // e^{-i phi}
+                const double tau = (Mm[qq][qq].x - Mm[p][p].x) / (2.0 * r);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+                // U = [[cs, sn], [-sn e^{-i phi}, cs e^{-i phi}]] on columns p, qq
+                const cd upp{cs, 0.0}, upq{sn, 0.0}, uqp = cscale(ph, -sn), uqq = cscale(ph, cs);
+                for (int k = 0; k < m; k++) {
+                    const cd xp = Mm[k][p], xq = Mm[k][qq];
+                    Mm[k][p] = cadd(cmul(xp, upp), cmul(xq, uqp));
+                    Mm[k][qq] = cadd(cmul(xp, upq), cmul(xq, uqq));
+                    const cd vp = V[k][p], vq = V[k][qq];
+                    V[k][p] = cadd(cmul(vp, upp), cmul(vq, uqp));
+                    V[k][qq] = cadd(cmul(vp, upq), cmul(vq, uqq));
+                }
+                for (int k = 0; k < m; k++) {
+                    const cd xp = Mm[p][k], xq = Mm[qq][k];
+                    Mm[p][k] = cadd(cmul(cconj(upp), xp), cmul(cconj(uqp), xq));
+                    Mm[qq][k] = cadd(cmul(cconj(upq), xp), cmul(cconj(uqq), xq));
+                }
+                Mm[p][qq] = Mm[qq][p] = cd{0.0, 0.0};
+                Mm[p][p].y = Mm[qq][qq].y = 0.0;
+            }
+    }
+    // ascending order (selection sort of the columns)
+    int ord[MC];
+    for (int a = 0; a < m; a++) ord[a] = a;
+    for (int a = 0; a < m; a++)
+        for (int b = a + 1; b < m; b++)
+            if (Mm[ord[b]][ord[b]].x < Mm[ord[a]][ord[a]].x) {
+                const int tmp = ord[a];
+                ord[a] = ord[b];
+                ord[b] = tmp;
+            }
+    // Q = L^-H V (back substitution, columns in ascending order)
+    for (int col = 0; col < m; col++)
+        for (int r = m - 1; r >= 0; r--) {
+            cd sacc = V[r][ord[col]];
+            for (int k = r + 1; k < m; k++) sacc = csub(sacc, cmul(cconj(L[k][r]), Q[k][col]));
+            Q[r][col] = cscale(sacc, 1.0 / L[r][r].x);
+        }
+    return true;
+}
+
+// One workgroup per matrix: chunks of coupled states, their Rayleigh-Ritz rotations
+// (Qg[kp][j][b] = component b of the rotation column of state j) applied to X and Y,
+// Cl[kp][j] = first state of j's chunk (-1 outside chunks). nflag[kp] counts chunks
+// cut at kRefMaxCluster and failed Cholesky factorisations (diagnostic).
+__global__ __launch_bounds__(256) void ref_cluster_kernel(double *__restrict__ Xd, double *__restrict__ Y, int n,
+                                                          const double *__restrict__ Sw,
+                                                          const double *__restrict__ Gw, double *__restrict__ Qg,
+                                                          int *__restrict__ Cl, int *__restrict__ nflag) {
+    extern __shared__ double lds_ref[];
+    double *lam = lds_ref;                                   // n
+    int *bound = reinterpret_cast<int *>(lam + n);           // n: states b and b+1 coupled
+    int *clist = bound + n;                                  // (first state, size) pairs, <= n/2
+    __shared__ int ncl;
+    const long kp = blockIdx.x;
+    const int t = threadIdx.x;
+    const double *Sk = Sw + 2 * kp * (long)n * kRefWin;
+    const double *Gk = Gw + 2 * kp * (long)n * kRefWin;
+    int *Ck = Cl + kp * (long)n;
+    if (t == 0) {
+        ncl = 0;
+        nflag[kp] = 0;
+    }
+    for (int j = t; j < n; j += 256) {
+        lam[j] = Sk[2 * ((long)j * kRefWin + kRefW)] / Gk[2 * ((long)j * kRefWin + kRefW)];
+        bound[j] = 0;
+        Ck[j] = -1;
+    }
+    __syncthreads();
+    for (int j = t; j < n; j += 256)
+        for (int d = 1; d <= kRefW && j - d >= 0; d++) {
+            const int k = j - d;
+            const long o = (long)j * kRefWin + kRefW - d;   // (k, j)
+            const cd s = ld(Sk, o), g = ld(Gk, o);
+            const cd num = csub(s, cscale(g, lam[j]));
+            if (sqrt(cabs2(num)) > kRefCoupling * fabs(lam[j] - lam[k]))
+                for (int b = k; b < j; b++) bound[b] = 1;   // same value from every writer
+        }
+    __syncthreads();
+    for (int s0 = t; s0 < n - 1; s0 += 256) {
+        if (!bound[s0] || (s0 > 0 && bound[s0 - 1])) continue;
+        int e = s0;
+        while (e < n - 1 && bound[e]) e++;                  // run [s0, e]
+        for (int c0 = s0; c0 <= e; c0 += kRefMaxCluster) {
+            const int m = min(kRefMaxCluster, e - c0 + 1);
+            if (c0 > s0) atomicAdd(nflag + kp, 1);
+            if (m < 2) continue;
+            cd S[kRefMaxCluster][kRefMaxCluster], G[kRefMaxCluster][kRefMaxCluster], Q[kRefMaxCluster][kRefMaxCluster];
+            for (int a = 0; a < m; a++)
+                for (int b = 0; b < m; b++) {
+                    // entry (k = c0 + a, j = c0 + b) sits in row j at d = a - b
+                    const long o = (long)(c0 + b) * kRefWin + kRefW + (a - b);
+                    S[a][b] = ld(Sk, o);
+                    G[a][b] = ld(Gk, o);
+                }
+            if (!small_geneig(m, S, G, Q)) {
+                atomicAdd(nflag + kp, 1);
+                continue;
+            }
+            for (int a = 0; a < m; a++) {
+                Ck[c0 + a] = c0;
+                for (int b = 0; b < kRefMaxCluster; b++)
+                    st(Qg, (kp * n + c0 + a) * kRefMaxCluster + b, b < m ? Q[b][a] : cd{0.0, 0.0});
+            }
+            const int slot = atomicAdd(&ncl, 1);
+            clist[2 * slot] = c0;
+            clist[2 * slot + 1] = m;
+        }
+    }
+    __syncthreads();
+    // X_C <- X_C Q, Y_C <- Y_C Q, row by row
+    const int nc = ncl;
+    double *Xk = Xd + 2 * kp * (long)n * n;
+    double *Yk = Y + 2 * kp * (long)n * n;
+    for (int c = 0; c < nc; c++) {
+        const int c0 = clist[2 * c], m = clist[2 * c + 1];
+        for (int i = t; i < 2 * n; i += 256) {
+            double *Bk = i < n ? Xk : Yk;
+            const long row = (long)(i < n ? i : i - n) * n;
+            cd old[kRefMaxCluster];
+            for (int b = 0; b < m; b++) old[b] = ld(Bk, row + c0 + b);
+            for (int a = 0; a < m; a++) {
+                cd acc{0.0, 0.0};
+                for (int b = 0; b < m; b++) cfma(acc, old[b], ld(Qg, (kp * n + c0 + a) * kRefMaxCluster + b));
+                st(Bk, row + c0 + a, acc);
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// X' = X + X E for 64 states per workgroup (E in LDS), threads over (row, state).
+__global__ __launch_bounds__(256) void ref_update_kernel(const double *__restrict__ Xd, const double *__restrict__ Sw,
+                                                         const double *__restrict__ Gw, const int *__restrict__ Cl,
+                                                         int n, int wpk, double *__restrict__ Xout) {
+    __shared__ cd E[64][kRefWin];
+    const long kp = blockIdx.x / wpk;
+    const int j0 = (int)(blockIdx.x % wpk) * 64;
+    const int t = threadIdx.x;
+    const double *Sk = Sw + 2 * kp * (long)n * kRefWin;
+    const double *Gk = Gw + 2 * kp * (long)n * kRefWin;
+    const int *Ck = Cl + kp * (long)n;
+    for (int e = t; e < 64 * kRefWin; e += 256) {
+        const int jj = e / kRefWin, dd = e % kRefWin;
+        const int j = j0 + jj, k = j + dd - kRefW;
+        cd v{0.0, 0.0};
+        if (j < n && k >= 0 && k < n) {
+            const long oj = (long)j * kRefWin;
+            const cd s = ld(Sk, oj + dd), g = ld(Gk, oj + dd);
+            const double gjj = Gk[2 * (oj + kRefW)];
+            const double lj = Sk[2 * (oj + kRefW)] / gjj;
+            if (k == j) {
+                v = cd{0.5 * (1.0 - gjj), 0.0};
+            } else if (Ck[j] >= 0 && Ck[j] == Ck[k]) {
+                v = cscale(g, -0.5);
+            } else {
+                const long ok = (long)k * kRefWin + kRefW;
+                const double lk = Sk[2 * ok] / Gk[2 * ok];
+                v = cscale(csub(s, cscale(g, lj)), 1.0 / (lj - lk));
+            }
+        }
+        E[jj][dd] = v;
+    }
+    __syncthreads();
+    const int jj = t & 63, j = j0 + jj;
+    if (j >= n) return;                                      // no barriers below
+    const double *Xk = Xd + 2 * kp * (long)n * n;
+    double *Ok = Xout + 2 * kp * (long)n * n;
+    for (int i = t >> 6; i < n; i += 4) {
+        const long row = (long)i * n;
+        cd acc = ld(Xk, row + j);
+        for (int dd = 0; dd < kRefWin; dd++) {
+            const int k = j + dd - kRefW;
+            if (k >= 0 && k < n) cfma(acc, ld(Xk, row + k), E[jj][dd]);
+        }
+        st(Ok, row + j, acc);
+    }
+}
+
+// ----------------------------------------------------------------------------
+int launch_refine(const double *Hb, const float *U32, int n, int bw, long nkp, double *Xd, double *Y, double *Sw,
+                  double *Gw, double *Qg, int *Cl, int *nflag, hipStream_t s) {
+    if (nkp <= 0) return 0;
+    const int nsb = (n + 15) / 16, wg = (nsb + 3) / 4;
+    const long groups = (nkp + 7) / 8;
+    const dim3 gw((unsigned)(groups * 8 * wg));
+    hipLaunchKernelGGL(ref_apply_kernel<float>, gw, dim3(256), 0, s, Hb, U32, n, bw, nkp, wg, Xd, Y);
+    hipLaunchKernelGGL(ref_window_kernel, gw, dim3(256), 0, s, Xd, Y, n, nkp, wg, Sw, Gw);
+    const size_t lds = sizeof(double) * n + sizeof(int) * (n + 2 * (n / 2 + 1));
+    if (lds > 64 * 1024)
+        RLM_HIP_CHECK(
+            hipFuncSetAttribute((const void *)ref_cluster_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(ref_cluster_kernel, dim3((unsigned)nkp), dim3(256), lds, s, Xd, Y, n, Sw, Gw, Qg, Cl, nflag);
+    hipLaunchKernelGGL(ref_window_kernel, gw, dim3(256), 0, s, Xd, Y, n, nkp, wg, Sw, Gw);
+    const int wpk = (n + 63) / 64;
+    hipLaunchKernelGGL(ref_update_kernel, dim3((unsigned)(nkp * wpk)), dim3(256), 0, s, Xd, Sw, Gw, Cl, n, wpk, Y);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace rlm

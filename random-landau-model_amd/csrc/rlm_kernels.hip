@@ -296,19 +296,9 @@ int launch_expand_band(const double *H, int n_phi, int off, long nkp, double *A,
 // (16 + bw columns instead of 16 + 2 bw; the whole upper triangle when dense).
 // !HALF: the full product A u, eigenvalues (if W) and resid[kp*nsb + sb] = max over
 // the 16 states of block sb of ||A u - lambda u|| / ||u|| (diagnostic).
-__device__ inline cd band_entry(const double *Ak, const double *Hk, int n, int bw, int nb, int r, int c) {
-    if (!Hk) return ld(Ak, (long)c * n + r);
-    if (r <= c) {
-        const int b = band_b(c - r, n, bw);
-        return b >= 0 ? cconj(ld(Hk, (long)r * nb + b)) : cd{0.0, 0.0};
-    }
-    const int b = band_b(r - c, n, bw);
-    return b >= 0 ? ld(Hk, (long)c * nb + b) : cd{0.0, 0.0};
-}
-
-template <bool HALF>
+template <bool HALF, typename TU>
 __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict__ A, const double *__restrict__ Hb,
-                                                       const float *__restrict__ U, int n, int bw, long nkp,
+                                                       const TU *__restrict__ U, int n, int bw, long nkp,
                                                        int wg_per_kp, double *__restrict__ W,
                                                        double *__restrict__ resid) {
     using M = Mf<double>;
@@ -324,7 +314,7 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
     const int nb = band_nb(n, bw);
     const double *Ak = Hb ? nullptr : A + 2 * kp * (long)n * n;
     const double *Hk = Hb ? Hb + 2 * kp * (long)n * nb : nullptr;
-    const float *Uk = U + 2 * kp * (long)n * n;
+    const TU *Uk = U + 2 * kp * (long)n * n;
     const int j = sb * 16 + li;
     const bool jin = j < n;
     const bool dense = 16 + 2 * bw + 3 >= n;
@@ -355,7 +345,7 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
                     else if (dl > 0) a = band_entry(Ak, Hk, n, bw, nb, r, c);
                 }
             }
-            const cx<float> u = (kin && jin) ? ldx(Uk, (long)c * n + j) : cx<float>{0.f, 0.f};   // U[c][j]
+            const cx<TU> u = (kin && jin) ? ldx(Uk, (long)c * n + j) : cx<TU>{(TU)0, (TU)0};   // U[c][j]
             const double ur = u.x, ui = u.y;
             yr = M::mma(a.x, ur, yr);
             yr = M::mma(-a.y, ui, yr);
@@ -366,7 +356,7 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
         for (int v = 0; v < 4; v++) {
             const int rr = r0 + M::row(lane, v);
             if (rr < n && jin) {
-                const cx<float> u = ldx(Uk, (long)rr * n + j);
+                const cx<TU> u = ldx(Uk, (long)rr * n + j);
                 const double ur = u.x, ui = u.y;
                 num += ur * yr[v] + ui * yi[v];
                 if (!HALF) yy += yr[v] * yr[v] + yi[v] * yi[v];
@@ -397,19 +387,28 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
 
 int rayleigh_resid_slots(int n) { return (n + 15) / 16; }
 
-int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, double *W, double *resid,
-                    hipStream_t stream, const double *Hb) {
+template <typename TU>
+static int launch_rayleigh_t(const double *A, const TU *U, int n, int bw, long nkp, double *W, double *resid,
+                             hipStream_t stream, const double *Hb) {
     if (nkp <= 0) return 0;
     const int nsb = (n + 15) / 16, wg = (nsb + 3) / 4;
     const long groups = (nkp + 7) / 8;
     if (resid)
-        hipLaunchKernelGGL(rayleigh_kernel<false>, dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A, Hb, U,
-                           n, bw, nkp, wg, W, resid);
+        hipLaunchKernelGGL((rayleigh_kernel<false, TU>), dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A,
+                           Hb, U, n, bw, nkp, wg, W, resid);
     else
-        hipLaunchKernelGGL(rayleigh_kernel<true>, dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A, Hb, U,
-                           n, bw, nkp, wg, W, resid);
+        hipLaunchKernelGGL((rayleigh_kernel<true, TU>), dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A,
+                           Hb, U, n, bw, nkp, wg, W, resid);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
+}
+int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, double *W, double *resid,
+                    hipStream_t stream, const double *Hb) {
+    return launch_rayleigh_t<float>(A, U, n, bw, nkp, W, resid, stream, Hb);
+}
+int launch_rayleigh(const double *A, const double *U, int n, int bw, long nkp, double *W, double *resid,
+                    hipStream_t stream, const double *Hb) {
+    return launch_rayleigh_t<double>(A, U, n, bw, nkp, W, resid, stream, Hb);
 }
 
 }  // namespace rlm

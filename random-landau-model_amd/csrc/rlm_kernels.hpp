@@ -26,5 +26,8 @@ int launch_expand_band(const double *H, int n_phi, int off, long nkp, double *A,
 // residual norms ||A u - lambda u|| / ||u|| as well (W may then be null).
 int launch_rayleigh(const double *A, const float *U, int n, int bw, long nkp, double *W, double *resid,
                     hipStream_t stream, const double *Hb = nullptr);
+// the same for FP64 eigenvectors (the refined vectors of large mixed plans)
+int launch_rayleigh(const double *A, const double *U, int n, int bw, long nkp, double *W, double *resid,
+                    hipStream_t stream, const double *Hb = nullptr);
 int rayleigh_resid_slots(int n);
 }  // namespace rlm

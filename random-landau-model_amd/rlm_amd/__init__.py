@@ -70,6 +70,7 @@ def lib() -> ctypes.CDLL:
         "rlm_plan_eigenvalues": ([vp], vp),
         "rlm_plan_max_residual": ([vp, dp], i),
         "rlm_plan_eigenvectors": ([vp], vp),
+        "rlm_plan_refine_info": ([vp, ctypes.POINTER(ctypes.c_long)], i),
         "rlm_plan_destroy": ([vp], i),
         "hlm_plan_create": ([ctypes.POINTER(vp), i, i, i, i, i, ctypes.c_ulong, d, i, i], i),
         "hlm_plan_load": ([vp, l, i], i),
@@ -204,6 +205,12 @@ class RlmPlan:
         out = np.zeros(1)
         _check(lib().rlm_plan_max_residual(self._h, _ptr(out)), "rlm_plan_max_residual")
         return float(out[0])
+
+    def refine_info(self) -> tuple[bool, int]:
+        """(refines, chunks cut or unfactorised in the last execute)."""
+        out = (ctypes.c_long * 2)()
+        _check(lib().rlm_plan_refine_info(self._h, out), "rlm_plan_refine_info")
+        return bool(out[0]), int(out[1])
 
     def eigenvalues_ptr(self) -> int:
         return lib().rlm_plan_eigenvalues(self._h) or 0

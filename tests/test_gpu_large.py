@@ -78,10 +78,14 @@ def test_rlm_large_vs_live_oracle(oracle, precision):
 
 
 def _run_n3000(precision):
-    from rlm_amd import RlmPlan
+    from rlm_amd import RlmPlan, HIP_DIAG_MIXED
     plan = RlmPlan(3000, 1.0, 16.0, 4, 3, samples_per_batch=4, precision=precision)
     e, c, m = plan.run(0, 4)
     res = plan.max_residual()
+    if precision == HIP_DIAG_MIXED:
+        refines, cut = plan.refine_info()
+        print(f"[n3000] mixed plan refines {refines}, chunks cut or unfactorised {cut}", flush=True)
+        assert refines
     plan.close()
     return e, c, m, res
 
@@ -101,12 +105,12 @@ def test_rlm_n3000_g4_fp64():
     assert np.all(np.diff(e, axis=1) >= 0)
 
 
-@pytest.mark.xfail(reason="at n_phi = 3000 the FP32 eigenvectors of near-degenerate states flip plaquette "
-                          "phases with margins ~1e-3 rad: one state of sample 0 differs from FP64 and the energies "
-                          "differ by ~3.5e-10 relative until the vectors get an FP64 refinement", strict=False)
 def test_rlm_n3000_g4_mixed_vs_fp64():
     """Mixed plan at the n_phi = 3000 configuration against the FP64 plan: Chern integers
-    identical, energies within 1e-10 relative."""
+    identical, energies within 1e-10 relative. Plain FP32 vectors with FP64 Rayleigh
+    quotients flip one state of sample 0 here (plaquette margins ~1e-3 rad) and leave
+    3.5e-10 in the energies; the plan's FP64 eigenvector refinement (on for n_phi > 512)
+    removes both."""
     from rlm_amd import HIP_DIAG_FP64, HIP_DIAG_MIXED
     print("[test_rlm_n3000_g4_mixed_vs_fp64]", flush=True)
     e64, c64, m64, _ = _run_n3000(HIP_DIAG_FP64)
@@ -132,3 +136,20 @@ def test_n3000_random_hermitian(precision):
     prec = HIP_DIAG_MIXED if precision == "mixed" else HIP_DIAG_FP64
     w, U = heevd_batched(torch.from_numpy(H).cuda(), precision=prec)
     (_check_mixed if precision == "mixed" else _check)(H, w.cpu().numpy(), U.cpu().numpy())
+
+
+@pytest.mark.parametrize("n_phi,g,refine", [(600, 2, "0"), (256, 3, "1")])
+def test_rlm_mixed_refinement_switch_vs_live_oracle(oracle, monkeypatch, n_phi, g, refine):
+    """RLM_REFINE=0 (FP32 vectors with FP64 Rayleigh quotients at n_phi = 600) and
+    RLM_REFINE=1 (the FP64 refinement at n_phi = 256, band-form reduction) against the
+    CPU oracle: Chern integers identical, energies within 1e-10 relative."""
+    from rlm_amd import RlmPlan, HIP_DIAG_MIXED
+    print(f"[test_rlm_mixed_refinement_switch_vs_live_oracle] n={n_phi} g={g} RLM_REFINE={refine}", flush=True)
+    monkeypatch.setenv("RLM_REFINE", refine)
+    plan = RlmPlan(n_phi, 1.0, 16.0, g, 3, samples_per_batch=1, precision=HIP_DIAG_MIXED)
+    assert plan.refine_info()[0] == (refine == "1")
+    e, c, m = plan.run(0, 1)
+    plan.close()
+    r = oracle.rlm_sample(n_phi, 1.0, 16.0, g, 3, 0, threads=8)
+    assert m.min() > 1e-3, m.min()
+    _cmp_sample(e[0], c[0], r.energy, r.chern, (n_phi, g, "refine", refine))
