@@ -198,7 +198,11 @@ def main():
 
     ints_identical = bool(np.array_equal(np.rint(mixed["c"]), np.rint(fp64["c"])))
     e_rel = float(np.abs(mixed["e"] - fp64["e"]).max() / np.abs(fp64["e"]).max())
-    mixed_valid = ints_identical and chern_ok(mixed) and e_rel <= 1e-10
+    # the mixed path is reported when it reproduces the FP64 path: identical per-state
+    # integers (hence identical per-sample sums, which are 1 once the mesh resolves every
+    # plaquette phase: n_phi = 256 at g = 32, not n_phi = 3000 at g = 4), every value
+    # integral, energies within 1e-10 relative
+    mixed_valid = ints_identical and chern_dev(mixed) < 1e-4 and e_rel <= 1e-10
     main = mixed if mixed_valid else fp64
     path = "mixed" if mixed_valid else "fp64"
 
