@@ -88,6 +88,79 @@ def _load_traffic(key):
         return None
 
 
+def _timed(plan, steps, warmup, tag):
+    """Runs warmup + steps executes of a loaded plan; returns (seconds per step, stage ms)."""
+    import torch
+    for _ in range(warmup):
+        plan.execute()
+        torch.cuda.synchronize()
+    stage_sum = None
+    t0 = time.perf_counter()
+    for k in range(steps):
+        plan.execute()
+        st = plan.stage_times()
+        stage_sum = st if stage_sum is None else {a: stage_sum[a] + st[a] for a in st}
+        print(f"[bench] {tag} step {k}: " + " ".join(f"{a}={v:.1f}ms" for a, v in st.items()), file=sys.stderr,
+              flush=True)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, {a: round(v / steps, 3) for a, v in stage_sum.items()}
+
+
+def _bench_hlm(args):
+    """hlm 24x24 lattice, flux 1/8, lowest subband (72 states), on-site disorder of width
+    W = 1 from seed 1, g = 32 (33 x 32 solved twist points), 64 samples in one plan; FP64."""
+    import numpy as np
+    from rlm_amd import HlmPlan
+    S, g = 64, 32
+    plan = HlmPlan(24, 24, 8, band=0, n_mesh=g, seed=1, W=1.0, samples_per_batch=S)
+    plan.load(0, S)
+    sec, stage = _timed(plan, args.steps, args.warmup, "hlm")
+    e, c, m = plan.fetch()
+    plan.close()
+    ci = np.rint(c)
+    print(json.dumps({"metric": "samples/hour hlm 24x24 q=8 b=0 W=1 g=32 (FP64)", "value": 3600.0 * S / sec,
+                      "unit": "samples/hour", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": sec * 1e3, "higher_is_better": True, "dtype": "f64",
+                      "data": "synthetic (reference hlm seed chain, seed 1, samples 0..63)",
+                      "config": {"workload": "hlm 24x24 q=8 b=0 W=1 g=32 x 64 samples", "n_phi": 72,
+                                 "solves_per_step": S * (g + 1) * g,
+                                 "time_per_solve_us": sec / (S * (g + 1) * g) * 1e6, "stage_ms": stage,
+                                 "chern_max_dev_from_integer": float(np.abs(c - ci).max()),
+                                 "chern_sums": sorted(set(ci.sum(axis=1).tolist())),
+                                 "min_plaquette_margin_rad": float(m.min())}}), flush=True)
+
+
+def _bench_uneven(args):
+    """rlm n_phi = 64, 128, 192, 256, 384, 512 (-q 1 -i 16), g = 16, 16 samples of seed 11
+    each, one plan per size and precision; per size: samples/hour of the FP64 and mixed plans,
+    mixed integers against FP64."""
+    import numpy as np
+    from rlm_amd import RlmPlan, HIP_DIAG_FP64, HIP_DIAG_MIXED
+    S, g, seed = 16, 16, 11
+    rows = []
+    for n_phi in (64, 128, 192, 256, 384, 512):
+        res = {}
+        for prec, tag in ((HIP_DIAG_FP64, "fp64"), (HIP_DIAG_MIXED, "mixed")):
+            plan = RlmPlan(n_phi, 1.0, 16.0, g, seed, samples_per_batch=S, precision=prec)
+            plan.load(0, S)
+            sec, stage = _timed(plan, args.steps, args.warmup, f"n={n_phi} {tag}")
+            e, c, m = plan.fetch()
+            plan.close()
+            res[tag] = (sec, e, c)
+        e_rel = float(np.abs(res["mixed"][1] - res["fp64"][1]).max() / np.abs(res["fp64"][1]).max())
+        rows.append({"n_phi": n_phi, "fp64_samples_per_hour": 3600.0 * S / res["fp64"][0],
+                     "mixed_samples_per_hour": 3600.0 * S / res["mixed"][0],
+                     "fp64_ms_per_step": res["fp64"][0] * 1e3, "mixed_ms_per_step": res["mixed"][0] * 1e3,
+                     "mixed_chern_ints_identical_to_fp64": bool(np.array_equal(np.rint(res["mixed"][2]),
+                                                                               np.rint(res["fp64"][2]))),
+                     "mixed_energy_max_rel_diff_vs_fp64": e_rel})
+    print(json.dumps({"metric": "samples/hour rlm uneven n_phi=64..512 g=16 (per size)", "unit": "samples/hour",
+                      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                      "data": "synthetic (reference seed chain, seed 11, samples 0..15 per size)",
+                      "config": {"workload": "rlm n_phi in {64,128,192,256,384,512} q=1 i=16 g=16 x 16 samples",
+                                 "sizes": rows}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +173,10 @@ def main():
     ap.add_argument("--cpu-lines", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--path", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--config", choices=["c3", "c2", "c4", "c5"], default="c3",
+                    help="c3 (default): rlm n_phi=256 g=32 x 32 samples (the contract line); c2: hlm 24x24 q=8 "
+                         "b=0 W=1 g=32 x 64 samples, FP64; c4: rlm n_phi=64..512 g=16 x 16 samples of seed 11, one "
+                         "plan per size, FP64 and mixed; c5: rlm n_phi=3000 g=4 x 4 samples of seed 3")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -124,6 +201,16 @@ def main():
     import numpy as np
     import torch
     from rlm_amd import RlmPlan, HIP_DIAG_FP64, HIP_DIAG_MIXED
+
+    if args.config == "c2":
+        _bench_hlm(args)
+        return
+    if args.config == "c4":
+        _bench_uneven(args)
+        return
+    if args.config == "c5":
+        args.n_phi, args.g, args.samples, args.seed = 3000, 4, 4, 3
+        args.no_cpu_baseline = True
 
     n_phi, g, S, seed = args.n_phi, args.g, args.samples, args.seed
     g1 = g + 1

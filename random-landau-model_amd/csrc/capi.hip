@@ -265,7 +265,7 @@ int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork 
 
 std::mutex g_single_mu;
 EigWork g_single_ws;
-DevBuf g_single_pack, g_single_a32;
+DevBuf g_single_pack, g_single_a32, g_single_pair;
 
 }  // namespace
 
@@ -308,6 +308,14 @@ extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA
         RLM_HIP_CHECK(hipGetLastError());
         if ((rc = eig_packed_f32(A32, n, batch, Wp, U32, g_single_ws, s))) return rc;
         if ((rc = launch_rayleigh(P, U32, n, n, batch, Wp, nullptr, s))) return rc;
+        if (n <= kSmallMaxN) {
+            // exact Rayleigh-Ritz of near-degenerate runs (n > 512 has no such step here;
+            // the rlm plans refine those sizes in FP64)
+            const int cap = pair_list_capacity(n, batch);
+            if ((rc = g_single_pair.ensure(sizeof(int) * (3 * (size_t)cap + 2)))) return rc;
+            int *pl = static_cast<int *>(g_single_pair.p);
+            if ((rc = launch_pair_rr(P, nullptr, U32, n, n, batch, Wp, pl + 2, cap, pl, s))) return rc;
+        }
         hipLaunchKernelGGL(transpose_out_f32_kernel, grid, dim3(256), 0, s, U32, nn, n, d_A, lda, strideA, nch);
     } else {
         if ((rc = eig_packed(P, n, batch, Wp, g_single_ws.U.d(), g_single_ws, s))) return rc;
@@ -471,6 +479,7 @@ struct rlm_plan_s {
     // n_phi > 512, RLM_REFINE=0 / 1 at rlm_plan_create turns it off / on
     bool refine = false;
     DevBuf dXd, dXr, dSw, dGw, dQg, dCl, dFlag;
+    DevBuf dPair;   // near-degenerate runs of the unrefined mixed path: list + 2 counters
     EigWork ws;
     int nb = 0;              // band width of the band form (dW: S*(g+1) x d x nb, dH: nkp x N x nb)
     bool band_trd = false;   // mixed plans: the FP32 reduction reads the band form (no dense matrices)
@@ -536,6 +545,9 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
         (rc = p->dResid.ensure(8 * nkp * rayleigh_resid_slots((int)N))) ||
         (rc = p->dEig.ensure(8 * nkp * N)) || (rc = p->dL.ensure(32 * nlk * N)) ||
         (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)))
+        return fail(rc);
+    if (precision == HIP_DIAG_MIXED && !p->refine && N <= kSmallMaxN &&
+        (rc = p->dPair.ensure(sizeof(int) * (3 * (size_t)pair_list_capacity((int)N, nkp) + 2))))
         return fail(rc);
     if (p->refine &&
         ((rc = p->dXd.ensure(16 * nkp * N * N)) || (rc = p->dXr.ensure(16 * nkp * N * N)) ||
@@ -635,10 +647,17 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
                 rc = launch_links(Xr, (long)N * N, N, N, (int)g1, (int)g1, g, 0, nkp, L1, L2, s);
             if (rc) return rc;
         } else {
-            // FP64 Rayleigh quotients against the FP64 matrix (band |c-r| <= off_head)
+            // FP64 Rayleigh quotients against the FP64 matrix (band |c-r| <= off_head), then
+            // exact Rayleigh-Ritz of the near-degenerate runs
             if ((rc = launch_rayleigh(p->band_trd ? nullptr : p->dA.d(), U32, N, off, nkp, p->dEig.d(), nullptr, s,
                                       p->band_trd ? p->dH.d() : nullptr)))
                 return rc;
+            if (N <= kSmallMaxN) {
+                const int cap = pair_list_capacity(N, nkp);
+                int *pl = static_cast<int *>(p->dPair.p);
+                if ((rc = launch_pair_rr(nullptr, p->dH.d(), U32, N, off, nkp, p->dEig.d(), pl + 2, cap, pl, s)))
+                    return rc;
+            }
             RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
             if (derive)
                 rc = launch_links_derived(U32, true, (long)N * N, N, g, S, L1, L2, s);
@@ -761,7 +780,7 @@ extern "C" int rlm_plan_destroy(rlm_plan_t p) {
     DevBuf *bufs[] = {&p->dcmn, &p->dcmt, &p->dE,   &p->dX,   &p->dY,   &p->dS,   &p->dV,
                       &p->dW,   &p->dH,   &p->dA,   &p->dU,   &p->dEig, &p->dL,   &p->dOut,
                       &p->dA32, &p->dResid, &p->dXd, &p->dXr, &p->dSw,  &p->dGw,  &p->dQg,
-                      &p->dCl,  &p->dFlag};
+                      &p->dCl,  &p->dFlag,  &p->dPair};
     for (DevBuf *b : bufs) b->release();
     p->ws.release();
     for (int k = 0; k <= rlm_plan_s::kStages; k++)

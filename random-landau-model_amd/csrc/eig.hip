@@ -2463,7 +2463,8 @@ int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *
 template <typename T>
 static int launch_stedc_global_t(int n, const double *D, const double *E, long strideV, double *W, long strideW, T *U,
                                  long strideU, double *work, int batch, int grid, hipStream_t stream) {
-    constexpr int NT = 256;
+    // 512 threads: few, large matrices (one workgroup each), so twice the lanes per merge
+    constexpr int NT = 512;
     if (n < 1 || batch < 1) return batch < 1 ? 0 : -22;
     if (grid > batch) grid = batch;
     const size_t lds = 32 * 33 * sizeof(double);

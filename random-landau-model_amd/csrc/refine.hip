@@ -24,6 +24,8 @@
 #include "common.hpp"
 #include "refine.hpp"
 
+#include <algorithm>
+
 namespace rlm {
 
 constexpr int kRefW = 16;
@@ -381,6 +383,165 @@ __global__ __launch_bounds__(256) void ref_update_kernel(const double *__restric
         }
         st(Ok, row + j, acc);
     }
+}
+
+// ----------------------------------------------------------------------------
+// Cluster Rayleigh-Ritz for the mixed solver without the full refinement: after the
+// FP64 Rayleigh quotients, runs of neighbouring states whose eigenvalue gaps are below
+// kPairTau times the spectrum's extent (near-degenerate states, whose FP32 vectors mix
+// and whose Rayleigh quotients then err by |coupling|^2 / gap) are collected, and each
+// run of up to kPairMax states gets an exact FP64 Rayleigh-Ritz: y_a = A u_a from the
+// band form, S_ab = u_a^H y_b, G_ab = u_a^H u_b, S q = l G q, eigenvalues into W and the
+// FP32 vectors rotated by Q. Runs longer than kPairMax are cut (counted). The
+// threshold: the FP32 vectors' couplings |u_k^H A u_j| measure ~5e-7 at n_phi = 256,
+// so a gap g leaves |coupling|^2 / g in the Rayleigh quotient, below 1e-10 of the
+// spectrum's extent for g above ~1e-4 of it.
+constexpr double kPairTau = 1e-4;
+constexpr int kPairMax = 4;
+constexpr int kPairMaxN = 512;                 // n of the unrefined mixed path (LDS stage)
+
+// count[0] = runs appended, count[1] = runs cut at kPairMax
+__global__ void pair_zero_kernel(int *count) {
+    if (threadIdx.x < 2) count[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void pair_flag_kernel(const double *__restrict__ W, int n, long nkp, int cap,
+                                                        int *__restrict__ list, int *__restrict__ count) {
+    const long kp = blockIdx.y;
+    if (kp >= nkp) return;
+    const double *w = W + kp * n;
+    const double tau = kPairTau * (fabs(w[0]) + fabs(w[n - 1]));
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n - 1; j += gridDim.x * blockDim.x) {
+        const bool here = w[j + 1] - w[j] < tau;
+        const bool before = j > 0 && w[j] - w[j - 1] < tau;
+        if (!here || before) continue;                        // not the first state of a run
+        int e = j + 1;                                        // run [j, e]
+        while (e < n - 1 && w[e + 1] - w[e] < tau) e++;
+        for (int c0 = j; c0 <= e; c0 += kPairMax) {
+            const int m = min(kPairMax, e - c0 + 1);
+            if (c0 > j) atomicAdd(count + 1, 1);
+            if (m < 2) continue;
+            const int slot = atomicAdd(count, 1);
+            if (slot < cap) {
+                list[3 * slot] = (int)kp;
+                list[3 * slot + 1] = c0;
+                list[3 * slot + 2] = m;
+            }
+        }
+    }
+}
+
+// One wave per run (persistent over the list); lanes over rows.
+__global__ __launch_bounds__(64) void pair_rr_kernel(const double *__restrict__ A, const double *__restrict__ Hb,
+                                                     float *__restrict__ U, int n, int bw, int cap,
+                                                     const int *__restrict__ list, const int *__restrict__ count,
+                                                     double *__restrict__ W) {
+    constexpr int MC = kPairMax;
+    __shared__ cd Qs[MC][MC];
+    __shared__ double ls[MC];
+    __shared__ int okflag;
+    __shared__ cx<float> ust[MC][kPairMaxN];       // the run's vectors
+    const int lane = threadIdx.x;
+    const int total = min(count[0], cap);
+    const int nb = band_nb(n, bw);
+    const bool dense = 2 * bw + 1 >= n;
+    for (int r_ = blockIdx.x; r_ < total; r_ += gridDim.x) {
+        const long kp = list[3 * r_];
+        const int c0 = list[3 * r_ + 1], m = list[3 * r_ + 2];
+        const double *Hk = Hb ? Hb + 2 * kp * (long)n * nb : nullptr;
+        const double *Ak = Hb ? nullptr : A + 2 * kp * (long)n * n;
+        float *Uk = U + 2 * kp * (long)n * n;
+        cd S[MC][MC], G[MC][MC];
+        for (int a = 0; a < MC; a++)
+            for (int b = 0; b < MC; b++) S[a][b] = G[a][b] = cd{0.0, 0.0};
+        for (int e = lane; e < MC * n; e += 64) {
+            const int a = e / n, r = e - a * n;
+            ust[a][r] = a < m ? ldx(Uk, (long)r * n + c0 + a) : cx<float>{0.f, 0.f};
+        }
+        __syncthreads();
+        for (int r = lane; r < n; r += 64) {
+            cd u[MC], y[MC];
+            for (int a = 0; a < MC; a++) {
+                u[a] = cd{ust[a][r].x, ust[a][r].y};
+                y[a] = cd{0.0, 0.0};
+            }
+            // y_a[r] = sum_c A(r, c) u_a[c] over the circular band of row r
+            const int nc = dense ? n : 2 * bw + 1;
+            for (int k = 0; k < nc; k++) {
+                int c = dense ? k : r - bw + k;
+                c = c < 0 ? c + n : (c >= n ? c - n : c);
+                const cd arc = band_entry(Ak, Hk, n, bw, nb, r, c);
+#pragma unroll
+                for (int a = 0; a < MC; a++) cfma(y[a], arc, cd{ust[a][c].x, ust[a][c].y});
+            }
+            for (int a = 0; a < MC; a++)
+                for (int b = 0; b < MC; b++) {
+                    cfmac(S[a][b], u[a], y[b]);
+                    cfmac(G[a][b], u[a], u[b]);
+                }
+        }
+        for (int a = 0; a < MC; a++)
+            for (int b = 0; b < MC; b++) {
+                S[a][b] = cd{wave_sum(S[a][b].x), wave_sum(S[a][b].y)};
+                G[a][b] = cd{wave_sum(G[a][b].x), wave_sum(G[a][b].y)};
+            }
+        if (lane == 0) {
+            cd Sq[kRefMaxCluster][kRefMaxCluster], Gq[kRefMaxCluster][kRefMaxCluster], Q[kRefMaxCluster][kRefMaxCluster];
+            for (int a = 0; a < m; a++)
+                for (int b = 0; b < m; b++) {
+                    // Hermitian parts (S, G are Hermitian up to rounding)
+                    Sq[a][b] = cscale(cadd(S[a][b], cconj(S[b][a])), 0.5);
+                    Gq[a][b] = cscale(cadd(G[a][b], cconj(G[b][a])), 0.5);
+                }
+            const bool ok = small_geneig(m, Sq, Gq, Q);
+            for (int a = 0; a < MC; a++) {
+                double la = 0.0;
+                if (ok && a < m) {
+                    // l_a = q_a^H S q_a (Q^H G Q = I)
+                    cd acc{0.0, 0.0};
+                    for (int b = 0; b < m; b++) {
+                        cd t{0.0, 0.0};
+                        for (int c = 0; c < m; c++) cfma(t, Sq[b][c], Q[c][a]);
+                        cfmac(acc, Q[b][a], t);
+                    }
+                    la = acc.x;
+                }
+                ls[a] = la;
+                for (int b = 0; b < MC; b++) Qs[b][a] = (ok && a < m && b < m) ? Q[b][a] : cd{0.0, 0.0};
+            }
+            if (ok)
+                for (int a = 0; a < m; a++) W[kp * n + c0 + a] = ls[a];
+            okflag = ok ? 1 : 0;
+        }
+        __syncthreads();
+        // u_a <- sum_b u_b Q[b][a] (skipped when G had no Cholesky factor)
+        if (okflag) {
+            for (int r = lane; r < n; r += 64) {
+                cd u[MC];
+                for (int b = 0; b < MC; b++) u[b] = cd{ust[b][r].x, ust[b][r].y};
+                for (int a = 0; a < m; a++) {
+                    cd acc{0.0, 0.0};
+                    for (int b = 0; b < m; b++) cfma(acc, u[b], Qs[b][a]);
+                    stx(Uk, (long)r * n + c0 + a, cx<float>{(float)acc.x, (float)acc.y});
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+int pair_list_capacity(int n, long nkp) { return (int)std::min<long>(nkp * (long)(n / 2), 1L << 26); }
+
+int launch_pair_rr(const double *A, const double *Hb, float *U32, int n, int bw, long nkp, double *W, int *list,
+                   int cap, int *count, hipStream_t s) {
+    if (nkp <= 0 || n < 2) return 0;
+    if (n > kPairMaxN) return -22;
+    hipLaunchKernelGGL(pair_zero_kernel, dim3(1), dim3(64), 0, s, count);
+    hipLaunchKernelGGL(pair_flag_kernel, dim3((n + 255) / 256, (unsigned)nkp), dim3(256), 0, s, W, n, nkp, cap, list,
+                       count);
+    hipLaunchKernelGGL(pair_rr_kernel, dim3(4096), dim3(64), 0, s, A, Hb, U32, n, bw, cap, list, count, W);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 // ----------------------------------------------------------------------------

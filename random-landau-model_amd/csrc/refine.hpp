@@ -13,4 +13,12 @@ int refine_max_cluster();
 // at the size limit or without a Cholesky factor).
 int launch_refine(const double *Hb, const float *U32, int n, int bw, long nkp, double *Xd, double *Y, double *Sw,
                   double *Gw, double *Qg, int *Cl, int *nflag, hipStream_t s);
+// Exact FP64 Rayleigh-Ritz of the runs of near-degenerate neighbouring states (gaps
+// below 1e-3 of the spectrum's extent, up to 4 states) after the Rayleigh quotients:
+// eigenvalues W updated, FP32 vectors U32 rotated. Matrices from the band form Hb
+// (offset bw) or, with Hb null, dense column-major A. list: pair_list_capacity(n, nkp)
+// x 3 ints, count: 2 ints (runs found, runs cut at 4 states).
+int pair_list_capacity(int n, long nkp);
+int launch_pair_rr(const double *A, const double *Hb, float *U32, int n, int bw, long nkp, double *W, int *list,
+                   int cap, int *count, hipStream_t s);
 }  // namespace rlm
