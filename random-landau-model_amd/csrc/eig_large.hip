@@ -5,15 +5,18 @@
 // zlarfg reflectors kept in A, compact-WY back-transform), so it replaces the
 // per-k-point zheevd of diag_wrappers/mkl_diag.cpp:2-9 at these sizes.
 //
-//   Tridiagonalisation (zhetrd/zlatrd structure, panels of kLgNB columns, full
-//   Hermitian storage so every Hermitian product reads whole rows):
+//   Tridiagonalisation (zhetrd/zlatrd structure, panels of kLgNB columns; only the
+//   lower triangle is read and updated):
 //     lg_col_kernel    one workgroup per matrix and column: finishes w of the
 //                      previous column (w = tau y - 1/2 tau (w^H v) v), applies
 //                      the panel's pending update to column c, builds reflector c
 //                      (zlarfg) and the dot products W^H v, V^H v;
-//     lg_symv_kernel   y = A22 v - V (W^H v) - W (V^H v), 64 rows per workgroup:
-//                      the trailing block is streamed once per column;
-//     lg_her2k_kernel  after each panel A22 -= V W^H + W V^H on the matrix cores.
+//     lg_symv_lower_kernel + lg_symv_reduce_kernel
+//                      y = A22 v - V (W^H v) - W (V^H v) from the lower triangle
+//                      in 32 x 32 tiles (row and column sums from one read): the
+//                      trailing lower triangle is streamed once per column;
+//     lg_her2k_kernel  after each panel A22 -= V W^H + W V^H on the lower tiles, on
+//                      the matrix cores.
 //   Divide & conquer: stedc_kernel with its node arrays in global memory.
 //   Back-transform: larft_kernel with several workgroups per matrix, then
 //     lg_backtransform_kernel: a slab of 32 eigenvector columns per workgroup,
@@ -33,8 +36,15 @@ int heevd_large_max_n() { return kLgColNT * kLgRPT; }
 
 // Per-matrix panel workspace, complex elements: Vp (kLgNB x n, column l = reflector
 // k0 + l with explicit zeros above and the unit entry), Wp (kLgNB x n), P (2 kLgNB:
-// W^H v then V^H v of the current column).
-long heevd_large_aux_complex(int n) { return 2L * kLgNB * n + 2L * kLgNB; }
+// W^H v then V^H v of the current column), then the Hermitian product's partial sums:
+// Rs (ns x 32: row sums of each 32-row strip of the trailing block) and Cs (ns x ns x
+// 32: column sums of tile (I, J), I >= J), ns = number of strips of the first column.
+constexpr int kLgTS = 32;
+static long lg_strips(int n) { return (n - 1 + kLgTS - 1) / kLgTS; }
+long heevd_large_aux_complex(int n) {
+    const long ns = lg_strips(n);
+    return 2L * kLgNB * n + 2L * kLgNB + ns * kLgTS + ns * ns * kLgTS;
+}
 
 // ----------------------------------------------------------------------------
 // Column c = k0 + i of the panel starting at k0 (nbk reflectors). i == nbk: only
@@ -179,52 +189,130 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
 }
 
 // ----------------------------------------------------------------------------
-// y = A(c+1:n, c+1:n) v - V (W^H v) - W (V^H v) into Wp(:, i), rows c+1.. of row
-// block rb (64 rows): lane = row, wave w takes the columns c+1+w, +4, ...; the four
-// wave partials are summed in a fixed order.
+// Hermitian product from the lower triangle: y = A22 v with A22 = A(s:n, s:n), s = c+1,
+// in 32 x 32 tiles (I, J), I >= J, of the trailing block. A workgroup takes the strips
+// p and ns - 1 - p (balanced work), its waves the tiles J = w, w + 4, ... of each strip;
+// lane (lrg, lcg) = (lane & 7, lane >> 3) holds the 4 x 4 sub-block at tile rows
+// 4 lrg.., columns 4 lcg.., read once for both y_I += A_IJ v_J (row sums, reduced over
+// lcg) and y_J += A_IJ^H v_I (column sums, reduced over lrg). Row sums of a strip stay in
+// registers and are summed over the waves in LDS (fixed order) into Rs[I]; the column
+// sums of tile (I, J) go to Cs[I][J] (the diagonal tile's strictly-lower part too), and
+// lg_symv_reduce_kernel adds them up in a fixed order.
 template <typename T>
-__global__ __launch_bounds__(256) void lg_symv_kernel(const T *__restrict__ Abase, long strideA, int n,
-                                                      T *__restrict__ aux, long strideAux, int k0, int i, int nrb) {
+__global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict__ Abase, long strideA, int n,
+                                                            T *__restrict__ aux, long strideAux, int k0, int i,
+                                                            int npair, int ns0) {
     using C = cx<T>;
-    extern __shared__ double lds_sv[];
-    C *vs = reinterpret_cast<C *>(lds_sv);                   // v(c+1 .. n-1)
-    __shared__ C part[4][kWave];
-    const long b = blockIdx.x / nrb;
-    const int rb = (int)(blockIdx.x % nrb);
+    extern __shared__ double lds_svl[];
+    C *vs = reinterpret_cast<C *>(lds_svl);                  // v(s .. n-1)
+    __shared__ C rpart[4][kLgTS];
+    const long b = blockIdx.x / npair;
+    const int p = (int)(blockIdx.x % npair);
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-    const int c = k0 + i, m = n - c - 1;
+    const int lrg = lane & 7, lcg = lane >> 3;
+    const int c = k0 + i, s0 = c + 1, m = n - s0;
+    const int ns = (m + kLgTS - 1) / kLgTS;
     const T *A = Abase + 2 * b * strideA;
     C *Vp = reinterpret_cast<C *>(aux + 2 * b * strideAux);
-    C *Wp = Vp + (long)kLgNB * n;
-    const C *P = Wp + (long)kLgNB * n;
-    for (int e = t; e < m; e += 256) vs[e] = Vp[(long)i * n + c + 1 + e];
+    C *Rs = Vp + 2L * kLgNB * n + 2L * kLgNB;
+    C *Cs = Rs + (long)ns0 * kLgTS;
+    for (int e = t; e < m; e += 256) vs[e] = Vp[(long)i * n + s0 + e];
     __syncthreads();
-    const int r = c + 1 + rb * 64 + lane;
-    const long rr = r < n ? r : n - 1;                       // clamped; masked at the end
-    const T *Acol = A + 2 * ((long)(c + 1) * n + rr);        // A(rr, c+1)
-    C acc0{(T)0, (T)0}, acc1{(T)0, (T)0};
-    int e = w;
-    for (; e + 12 < m; e += 16) {
-        const C a0 = ldx(Acol, (long)e * n), a1 = ldx(Acol, (long)(e + 4) * n);
-        const C a2 = ldx(Acol, (long)(e + 8) * n), a3 = ldx(Acol, (long)(e + 12) * n);
-        fmax_(acc0, a0, vs[e]);
-        fmax_(acc1, a1, vs[e + 4]);
-        fmax_(acc0, a2, vs[e + 8]);
-        fmax_(acc1, a3, vs[e + 12]);
-    }
-    for (; e < m; e += 4) fmax_(acc0, ldx(Acol, (long)e * n), vs[e]);
-    part[w][lane] = acc0 + acc1;
-    __syncthreads();
-    if (w == 0 && r < n) {
-        C y = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-        for (int l = 0; l < i; l++) y = y - (Vp[(long)l * n + r] * P[l] + Wp[(long)l * n + r] * P[kLgNB + l]);
-        Wp[(long)i * n + r] = y;
+    for (int half = 0; half < 2; half++) {
+        const int I = half == 0 ? p : ns - 1 - p;
+        if (I < 0 || I >= ns || (half == 1 && I == p)) continue;       // workgroup-uniform
+        C racc[4];
+#pragma unroll
+        for (int ru = 0; ru < 4; ru++) racc[ru] = C{(T)0, (T)0};
+        const int rb = I * kLgTS + 4 * lrg;                  // local row of the sub-block
+        C vr[4];
+#pragma unroll
+        for (int ru = 0; ru < 4; ru++) vr[ru] = rb + ru < m ? vs[rb + ru] : C{(T)0, (T)0};
+        for (int J = w; J <= I; J += 4) {
+            const int cb = J * kLgTS + 4 * lcg;
+            C a[4][4], vc[4];
+#pragma unroll
+            for (int cu = 0; cu < 4; cu++) {
+                vc[cu] = cb + cu < m ? vs[cb + cu] : C{(T)0, (T)0};
+                const long colo = (long)(s0 + min(cb + cu, m - 1)) * n;
+#pragma unroll
+                for (int ru = 0; ru < 4; ru++) a[ru][cu] = ldx(A, colo + s0 + min(rb + ru, m - 1));
+            }
+            const bool diag = J == I;
+            C csum[4];
+#pragma unroll
+            for (int cu = 0; cu < 4; cu++) csum[cu] = C{(T)0, (T)0};
+#pragma unroll
+            for (int ru = 0; ru < 4; ru++)
+#pragma unroll
+                for (int cu = 0; cu < 4; cu++) {
+                    const int lr = rb + ru, lc = cb + cu;
+                    const bool in = lr < m && lc < m && (!diag || lc <= lr);
+                    const C x = in ? a[ru][cu] : C{(T)0, (T)0};
+                    fmax_(racc[ru], x, vc[cu]);                                  // A(r, c) v_c
+                    if (!diag || lc < lr) fmacx(csum[cu], x, vr[ru]);           // conj(A(r, c)) v_r
+                }
+            // column sums over lrg (lane bits 0-2), then lanes lrg == 0 store them
+#pragma unroll
+            for (int cu = 0; cu < 4; cu++)
+#pragma unroll
+                for (int off = 1; off < 8; off <<= 1) {
+                    csum[cu].x += __shfl_xor(csum[cu].x, off, kWave);
+                    csum[cu].y += __shfl_xor(csum[cu].y, off, kWave);
+                }
+            if (lrg == 0) {
+                C *dst = Cs + ((long)I * ns0 + J) * kLgTS + 4 * lcg;
+#pragma unroll
+                for (int cu = 0; cu < 4; cu++) dst[cu] = csum[cu];
+            }
+        }
+        // row sums over lcg (lane bits 3-5), then over the waves through LDS
+#pragma unroll
+        for (int ru = 0; ru < 4; ru++)
+#pragma unroll
+            for (int off = 8; off < kWave; off <<= 1) {
+                racc[ru].x += __shfl_xor(racc[ru].x, off, kWave);
+                racc[ru].y += __shfl_xor(racc[ru].y, off, kWave);
+            }
+        if (lcg == 0) {
+#pragma unroll
+            for (int ru = 0; ru < 4; ru++) rpart[w][4 * lrg + ru] = racc[ru];
+        }
+        __syncthreads();
+        if (t < kLgTS) Rs[(long)I * kLgTS + t] = ((rpart[0][t] + rpart[1][t]) + rpart[2][t]) + rpart[3][t];
+        __syncthreads();
     }
 }
 
+// y(r) = Rs[K][r'] + sum_{I >= K} Cs[I][K][r'] - V (W^H v) - W (V^H v) for the 32 rows of
+// strip K, into Wp(:, i).
+template <typename T>
+__global__ __launch_bounds__(64) void lg_symv_reduce_kernel(T *__restrict__ aux, long strideAux, int n, int k0, int i,
+                                                            int ns0) {
+    using C = cx<T>;
+    const int c = k0 + i, s0 = c + 1, m = n - s0;
+    const int ns = (m + kLgTS - 1) / kLgTS;
+    const long b = blockIdx.x / ns;
+    const int K = (int)(blockIdx.x % ns);
+    const int t = threadIdx.x;
+    C *Vp = reinterpret_cast<C *>(aux + 2 * b * strideAux);
+    C *Wp = Vp + (long)kLgNB * n;
+    const C *P = Wp + (long)kLgNB * n;
+    const C *Rs = Vp + 2L * kLgNB * n + 2L * kLgNB;
+    const C *Cs = Rs + (long)ns0 * kLgTS;
+    const int lr = K * kLgTS + t;
+    if (t >= kLgTS || lr >= m) return;
+    C y = Rs[(long)K * kLgTS + t];
+    for (int I = K; I < ns; I++) y = y + Cs[((long)I * ns0 + K) * kLgTS + t];
+    const int r = s0 + lr;
+    for (int l = 0; l < i; l++) y = y - (Vp[(long)l * n + r] * P[l] + Wp[(long)l * n + r] * P[kLgNB + l]);
+    Wp[(long)i * n + r] = y;
+}
+
 // ----------------------------------------------------------------------------
-// A(r, c') -= sum_l V(r,l) conj(W(c',l)) + W(r,l) conj(V(c',l)) for r, c' >= s0,
-// both triangles. With X = [V W], Y = [W V] (n x 2 nbk) the update is X Y^H; the
+// A(r, c') -= sum_l V(r,l) conj(W(c',l)) + W(r,l) conj(V(c',l)) for r, c' >= s0 on the
+// 64 x 64 tiles of the lower triangle (diagonal tiles whole; the upper triangle is never
+// read again). With X = [V W], Y = [W V] (n x 2 nbk) the update is X Y^H; the
 // matrix cores compute its transpose D[c'][r] = sum_k (-conj(Y(c',k))) X(r,k) so
 // that the 16 lanes of an accumulator row hold 16 consecutive rows of A
 // (coalesced in column-major storage). 64 x 64 tile per workgroup, 32 x 32 per wave.
@@ -235,10 +323,13 @@ __global__ __launch_bounds__(256) void lg_her2k_kernel(T *__restrict__ Abase, lo
     using M = Mf<T>;
     using v4 = typename M::v4;
     using C = cx<T>;
-    const int tiles = nt * nt;
+    const int tiles = nt * (nt + 1) / 2;                    // tiles (tr, tc), tr >= tc
     const long b = blockIdx.x / tiles;
     const int tile = (int)(blockIdx.x % tiles);
-    const int tr = tile % nt, tc = tile / nt;
+    int tr = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+    if ((tr + 1) * (tr + 2) / 2 <= tile) tr++;
+    if (tr * (tr + 1) / 2 > tile) tr--;
+    const int tc = tile - tr * (tr + 1) / 2;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int li = lane & 15, lq = lane >> 4;
     const int R0 = s0 + 64 * tr + 32 * (w & 1), C0 = s0 + 64 * tc + 32 * (w >> 1);
@@ -472,9 +563,10 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
     if (lds_col > 32 * 1024) {                               // dynamic + static beyond the 64 KB default
         RLM_HIP_CHECK(hipFuncSetAttribute((const void *)lg_col_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds_col));
-        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)lg_symv_kernel<T>,
+        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)lg_symv_lower_kernel<T>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_col));
     }
+    const int ns0 = (int)lg_strips(n);
     for (int k0 = 0; k0 < n - 1; k0 += kLgNB) {
         const int nbk = std::min(kLgNB, n - 1 - k0);
         for (int i = 0; i <= nbk; i++) {
@@ -482,14 +574,17 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
                                strideV, aux, strideAux, k0, i, nbk);
             if (i == nbk) break;
             const int m = n - (k0 + i) - 1;
-            const int nrb = (m + 63) / 64;
-            hipLaunchKernelGGL(lg_symv_kernel<T>, dim3((unsigned)((long)nb * nrb)), dim3(256), sizeof(C) * m, s, A,
-                               strideA, n, aux, strideAux, k0, i, nrb);
+            const int nsm = (m + kLgTS - 1) / kLgTS, npair = (nsm + 1) / 2;
+            hipLaunchKernelGGL(lg_symv_lower_kernel<T>, dim3((unsigned)((long)nb * npair)), dim3(256), sizeof(C) * m,
+                               s, A, strideA, n, aux, strideAux, k0, i, npair, ns0);
+            hipLaunchKernelGGL(lg_symv_reduce_kernel<T>, dim3((unsigned)((long)nb * nsm)), dim3(64), 0, s, aux,
+                               strideAux, n, k0, i, ns0);
         }
         const int s0 = k0 + nbk;
         if (s0 < n - 1) {
             const int nt = (n - s0 + 63) / 64;
-            hipLaunchKernelGGL(lg_her2k_kernel<T>, dim3((unsigned)((long)nb * nt * nt)), dim3(256), 0, s, A, strideA,
+            hipLaunchKernelGGL(lg_her2k_kernel<T>, dim3((unsigned)((long)nb * nt * (nt + 1) / 2)), dim3(256), 0, s, A,
+                               strideA,
                                n, aux, strideAux, s0, nbk, nt);
         }
     }
