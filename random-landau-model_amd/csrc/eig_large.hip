@@ -517,15 +517,52 @@ __global__ __launch_bounds__(256) void lg_backtransform_kernel(const T *__restri
             wi[cb] = v4{0, 0, 0, 0};
         }
         const int lo = bk > 0 ? j0 - 15 : j0 + 1;            // first row either block needs
-        for (int rbk = w; 16 * rbk < n; rbk += 4) {
+        // this wave's row blocks R = 16 (w + 4 k) >= lo - 15, software-pipelined: the U
+        // rows and both V operands of the next row block load while the current one runs
+        // on the matrix cores
+        int rbk = w;
+        if (16 * rbk + 15 < lo) rbk += 4 * ((lo - 15 - 16 * rbk + 63) / 64);
+        v4 nur[NCB], nui[NCB];
+        CT nxu[4], nxw[4];
+        auto fetch = [&](int R) {
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) {
+                const int col = c0 + 16 * cb + li;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int row = R + M::row(lane, v);
+                    const CT x = (row < n && col < n) ? ldx(U, (long)row * n + col) : CT{(T)0, (T)0};
+                    nur[cb][v] = x.x;
+                    nui[cb][v] = x.y;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                nxu[s] = lg_refl(A, n, j0, R + li, M::row(lane, s));                   // V[R + li][k]
+                nxw[s] = bk > 0 ? lg_refl(A, n, j0 - 16, R + M::row(lane, s), li)      // V_{bk-1}[row][li]
+                                : CT{(T)0, (T)0};
+            }
+        };
+        if (16 * rbk < n) fetch(16 * rbk);
+        for (; 16 * rbk < n; rbk += 4) {
             const int R = 16 * rbk;
-            if (R + 15 < lo) continue;
-            load_u(R);
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) {
+                ur[cb] = nur[cb];
+                ui[cb] = nui[cb];
+            }
+            CT xu[4], xw[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                xu[s] = nxu[s];
+                xw[s] = nxw[s];
+            }
+            if (16 * (rbk + 4) < n) fetch(16 * (rbk + 4));
             if (R + 15 >= j0 + 1) {
                 // U -= V W2 on rows R.. (K index = reflector row(l, s))
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
-                    const CT x = lg_refl(A, n, j0, R + li, M::row(lane, s));
+                    const CT x = xu[s];
 #pragma unroll
                     for (int cb = 0; cb < NCB; cb++) {
                         ur[cb] = M::mma(-x.x, w2r[cb][s], ur[cb]);
@@ -544,7 +581,20 @@ __global__ __launch_bounds__(256) void lg_backtransform_kernel(const T *__restri
                     }
                 }
             }
-            if (bk > 0) acc_w(j0 - 16, R, wr, wi);
+            if (bk > 0) {
+                // W partial of block bk - 1 from the updated rows (K-step v = rows R + row(lane, v))
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const CT x = xw[v];
+#pragma unroll
+                    for (int cb = 0; cb < NCB; cb++) {
+                        wr[cb] = M::mma(x.x, ur[cb][v], wr[cb]);
+                        wr[cb] = M::mma(x.y, ui[cb][v], wr[cb]);
+                        wi[cb] = M::mma(x.x, ui[cb][v], wi[cb]);
+                        wi[cb] = M::mma(-x.y, ur[cb][v], wi[cb]);
+                    }
+                }
+            }
         }
     }
 }
