@@ -229,7 +229,10 @@ def main():
 
     def trd_name(path):
         # "auto": register-resident kernel where n fits it (complex float n <= 256,
-        # complex double n <= 128), else the lower-storage kernel
+        # complex double n <= 128), else the lower-storage kernel; n > 512: the per-column
+        # Hermitian product of the multi-workgroup solver
+        if n_phi > 512:
+            return "lg_symv_lower_kernel"
         if variant in names:
             if variant == "resident" and n_phi > (256 if path == "mixed" else 128):
                 return names["lower"]
@@ -296,7 +299,8 @@ def main():
     # dominant kernel of the reported path: algorithmic flops per launch / measured launch time
     dom = max(flops, key=lambda k: main["stage"][k])
     flops_unit = flops[dom]
-    kname = trd_name(path) if dom == "tridiagonalisation" else "backtransform_kernel"
+    bt_name = "backtransform_mfma_kernel" if n_phi <= 256 else "lg_backtransform_kernel"
+    kname = trd_name(path) if dom == "tridiagonalisation" else bt_name
     t_launch = main["stage"][dom] / 1e3
     achieved = flops_unit * nsolved / t_launch / 1e12
     peak = FP32_PEAK_TFLOPS if (path == "mixed") else FP64_PEAK_TFLOPS

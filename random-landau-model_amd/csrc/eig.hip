@@ -2332,6 +2332,15 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
         RLM_HIP_CHECK(hipGetLastError());
         return 0;
     }
+    if (tws && lda == n) {
+        // 256 < n: T factors, then the streaming matrix-core back-transform (the slab no
+        // longer fits the accumulators)
+        const long strideT = backtransform_tws_complex(n);
+        hipLaunchKernelGGL((larft_kernel<T>), dim3(batch), dim3(256), 0, stream, A, strideA, lda, n, TAU, strideV, tws,
+                           strideT, batch, 1);
+        RLM_HIP_CHECK(hipGetLastError());
+        return launch_backtransform_stream(A, strideA, n, tws, strideT, U, strideU, batch, stream);
+    }
     // reflectors per compact-WY block: the V block (n x NBW complex) lives in LDS
     constexpr int NBW = sizeof(T) == 4 ? 32 : 16;
     const size_t lds_bt = backtransform_lds_bytes(n, NBW, sizeof(T));

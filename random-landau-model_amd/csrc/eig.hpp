@@ -61,6 +61,12 @@ int launch_larft(const float *A, long strideA, int lda, int n, const float *TAU,
 // heevd_large_aux_complex(n) complex elements of the precision of A; work: grid_dc *
 // stedc_global_work_doubles(n) doubles; tws: batch * backtransform_tws_complex(n)
 // complex. W: eigenvalues (stride n), U: eigenvectors component-major (stride n*n).
+// Streaming matrix-core back-transform U = Q U (lda = n, reflectors and T factors as
+// larft_kernel leaves them; any n >= 2), a slab of eigenvector columns per workgroup.
+int launch_backtransform_stream(const double *A, long strideA, int n, const double *tws, long strideT, double *U,
+                                long strideU, int batch, hipStream_t s);
+int launch_backtransform_stream(const float *A, long strideA, int n, const float *tws, long strideT, float *U,
+                                long strideU, int batch, hipStream_t s);
 int heevd_large_max_n();
 long heevd_large_aux_complex(int n);
 int heevd_large(double *A, int n, long batch, double *D, double *E, double *TAU, long strideV, double *aux, double *W,

@@ -657,6 +657,27 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
     return 0;
 }
 
+template <typename T>
+static int launch_bt_stream_t(const T *A, long strideA, int n, const T *tws, long strideT, T *U, long strideU,
+                              int batch, hipStream_t s) {
+    if (batch < 1 || n < 2) return 0;
+    constexpr int NCB = sizeof(T) == sizeof(float) ? 2 : 1;
+    const int nslab = (n + 16 * NCB - 1) / (16 * NCB);
+    const long groups = (batch + 7) / 8;
+    hipLaunchKernelGGL((lg_backtransform_kernel<T, NCB>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0, s, A,
+                       strideA, n, tws, strideT, U, strideU, nslab, batch);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+int launch_backtransform_stream(const double *A, long strideA, int n, const double *tws, long strideT, double *U,
+                                long strideU, int batch, hipStream_t s) {
+    return launch_bt_stream_t<double>(A, strideA, n, tws, strideT, U, strideU, batch, s);
+}
+int launch_backtransform_stream(const float *A, long strideA, int n, const float *tws, long strideT, float *U,
+                                long strideU, int batch, hipStream_t s) {
+    return launch_bt_stream_t<float>(A, strideA, n, tws, strideT, U, strideU, batch, s);
+}
+
 int heevd_large(double *A, int n, long batch, double *D, double *E, double *TAU, long strideV, double *aux, double *W,
                 double *U, double *work, int grid_dc, double *tws, hipStream_t s, hipEvent_t mid, hipEvent_t mid2) {
     return heevd_large_t<double>(A, n, batch, D, E, TAU, strideV, aux, W, U, work, grid_dc, tws, s, mid, mid2);
