@@ -1238,10 +1238,13 @@ __device__ inline double rcp_nr(double x) {
 }
 
 // Root j of 1 + sum_i z2_i / (dl_i - lambda) = 0 with z2_i = rho w_i^2, poles dl
-// ascending (sum_i w_i^2 = 1).
+// ascending (sum_i w_i^2 = 1), to relative accuracy teps (2^-53 for FP64
+// eigenvectors; 2^-40 for the FP32 eigenvector matrices of the mixed-precision
+// solver, whose vectors carry FP32 accuracy and whose eigenvalues are replaced by FP64
+// Rayleigh quotients: one iteration fewer for most roots).
 // Returns origin index o and tau with lambda = dl[o] + tau.
 __device__ void secular_root(int j, int K, const double *dl, const double *z2, double rho, int &o_out,
-                             double &tau_out) {
+                             double &tau_out, double teps) {
     int o;
     double lo, hi, tau;
     if (j < K - 1) {
@@ -1288,8 +1291,8 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
         const double f = 1.0 + psi + phi;
         if (f > 0.0) hi = fmin(hi, tau);
         else if (f < 0.0) lo = fmax(lo, tau);
-        if (f == 0.0 || fabs(f) <= 8.0 * dc_eps() * (1.0 + err)) break;
-        if (hi - lo <= 4.0 * dc_eps() * fmax(fabs(lo), fabs(hi))) break;
+        if (f == 0.0 || fabs(f) <= 8.0 * teps * (1.0 + err)) break;
+        if (hi - lo <= 4.0 * teps * fmax(fabs(lo), fabs(hi))) break;
         const double bq = dpsi * delj * delj;
         const double aq = psi - dpsi * delj;
         double nt;
@@ -1326,29 +1329,31 @@ __device__ inline int count_below(const double *a, int len, double v, bool inclu
     return lo;
 }
 
-// Matrix-core merge GEMM of one node, Qb = Qa S (see stedc_block): task = 32 rows
-// (2 row blocks) x 64 output columns (4 column tiles); each gathered Qa value feeds 4
-// MFMAs, each generated S value 2. S[i][q] = what_i / (dl_i - lambda_q) * inrm_q is
-// generated in FP64 per K-step and never stored. QS: storage and matrix-core type of
-// the eigenvector matrices (double; float in the mixed-precision solver, whose
-// eigenvectors carry the FP32 accuracy of its reflectors anyway).
-template <typename QS>
+// Matrix-core merge GEMM of one node, Qb = Qa S (see stedc_block): task = 16 RB rows
+// (RB row blocks) x 16 QT output columns (QT column tiles); each gathered Qa value
+// feeds QT MFMAs, each generated S value RB. S[i][q] = what_i / (dl_i - lambda_q) *
+// inrm_q is generated in FP64 per K-step and never stored; every row group of a node
+// generates it anew, so RB = 4, QT = 2 halves that FP64 work against RB = 2, QT = 4
+// (same accumulator registers, twice the Qa loads) where the halves have >= 64 rows. QS: storage and matrix-core type of the eigenvector matrices (double; float in
+// the mixed-precision solver, whose eigenvectors carry the FP32 accuracy of its
+// reflectors anyway).
+template <typename QS, int RB, int QT>
 __device__ __forceinline__ void merge_gemm_mfma(int lane, int G, int lo, int mid, int hi, int n1, int K, int KL, int KR,
                                                 int n, const QS *Qa, QS *Qb, const DcShared &sh) {
     using MQ = Mf<QS>;
     using v4 = typename MQ::v4;
-    constexpr int QT = 4, RB = 2;
+    constexpr int RH = 16 * RB;
     const int wv = lane / kWave, nwv = G / kWave;
     const int l = lane & (kWave - 1), li = l & 15, lq = l >> 4;
     const double *dl = sh.dl + lo;
     const int nqt = (K + 15) / 16, nqr = (nqt + QT - 1) / QT;
     const int rows1 = hi - mid;                       // right half (may be short)
-    const int ng0 = (n1 + 31) / 32, ng1 = (rows1 + 31) / 32;
+    const int ng0 = (n1 + RH - 1) / RH, ng1 = (rows1 + RH - 1) / RH;
     const int ntask = (ng0 + ng1) * nqr;
     for (int task = wv; task < ntask; task += nwv) {
         const int qr = task % nqr, g = task / nqr;
         const bool right = g >= ng0;
-        const int rbase = right ? mid + (g - ng0) * 32 : lo + g * 32;
+        const int rbase = right ? mid + (g - ng0) * RH : lo + g * RH;
         const int rend = right ? hi : mid;
         const int cnt = right ? KR : KL;
         const int *lst = (right ? sh.lstR : sh.lstL) + lo;
@@ -1662,7 +1667,7 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                 for (int j = lane; j < K; j += G) {
                     int o;
                     double tu;
-                    secular_root(j, K, sh.dl + lo, sh.z2 + lo, rho, o, tu);
+                    secular_root(j, K, sh.dl + lo, sh.z2 + lo, rho, o, tu, sizeof(QS) == sizeof(float) ? 0x1p-40 : dc_eps());
                     sh.orig[lo + j] = o;
                     sh.tau[lo + j] = tu;
                     sh.rc[lo + j] = sh.dl[lo + o];      // origin pole of root j (rc is free after P3)
@@ -1746,7 +1751,10 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
             // Rows of the left half only meet columns of lstL (left child or rotated), rows
             // of the right half only lstR (the block-diagonal structure dlaed3 exploits).
             if (mfma) {
-                merge_gemm_mfma<QS>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
+                if (sizeof(QS) == sizeof(float) && half >= 64)
+                    merge_gemm_mfma<QS, 4, 2>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
+                else
+                    merge_gemm_mfma<QS, 2, 4>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
             } else if (merging) {
                 // VALU path: each task is one row and 4 consecutive output columns,
                 // consecutive lanes on consecutive rows

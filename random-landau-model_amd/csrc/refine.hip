@@ -431,7 +431,176 @@ __global__ __launch_bounds__(256) void pair_flag_kernel(const double *__restrict
     }
 }
 
-// One wave per run (persistent over the list); lanes over rows.
+// small_geneig for m = 2 in registers: the same Cholesky G = L L^H, M = L^-1 S L^-H,
+// one Jacobi rotation (exact for 2 x 2), ascending order and Q = L^-H V.
+__host__ __device__ inline bool small_geneig2(const cd (&S)[2][2], const cd (&G)[2][2], cd (&Q)[2][2]) {
+    const double g11 = G[0][0].x;
+    if (!(g11 > 0.0)) return false;
+    const double l11 = sqrt(g11);
+    const cd l21 = cscale(G[1][0], 1.0 / l11);
+    const double d2 = G[1][1].x - cabs2(l21);
+    if (!(d2 > 0.0)) return false;
+    const double l22 = sqrt(d2);
+    // X = L^-1 S, M = X L^-H
+    cd X[2][2];
+    for (int c = 0; c < 2; c++) {
+        X[0][c] = cscale(S[0][c], 1.0 / l11);
+        X[1][c] = cscale(csub(S[1][c], cmul(l21, X[0][c])), 1.0 / l22);
+    }
+    const double a = X[0][0].x / l11;
+    const double d = (csub(X[1][1], cmul(X[1][0], cscale(cconj(l21), 1.0 / l11)))).x / l22;
+    const cd b = cscale(csub(X[0][1], cmul(X[0][0], cscale(cconj(l21), 1.0 / l11))), 1.0 / l22);
+    // rotation [[cs, sn], [-sn e^{-i phi}, cs e^{-i phi}]], b = |b| e^{i phi}: diagonal a - t|b|, d + t|b|
+    const double r = sqrt(cabs2(b));
+    cd V[2][2] = {{cd{1.0, 0.0}, cd{0.0, 0.0}}, {cd{0.0, 0.0}, cd{1.0, 0.0}}};
+    double l0 = a, l1 = d;
+    if (r > 0.0) {
+        const cd ph{b.x / r, -b.y / r};
+        const double tau = (d - a) / (2.0 * r);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+        V[0][0] = cd{cs, 0.0};
+        V[0][1] = cd{sn, 0.0};
+        V[1][0] = cscale(ph, -sn);
+        V[1][1] = cscale(ph, cs);
+        l0 = a - t * r;
+        l1 = d + t * r;
+    }
+    const int o0 = l1 < l0 ? 1 : 0, o1 = 1 - o0;
+    for (int col = 0; col < 2; col++) {
+        const int src = col == 0 ? o0 : o1;
+        const cd v0 = src ? V[0][1] : V[0][0], v1 = src ? V[1][1] : V[1][0];
+        Q[1][col] = cscale(v1, 1.0 / l22);
+        Q[0][col] = cscale(csub(v0, cmul(cconj(l21), Q[1][col])), 1.0 / l11);
+    }
+    return true;
+}
+
+// One run of M >= m states at kp (one wave, lanes over rows). S = U^H A U from the
+// upper half of the band, A = B + B^H with B(r, c) = A(r, c) at circular offsets
+// c - r = 1..bw and B(r, r) = A(r, r) / 2 (the upper triangle when the band covers
+// every column): P = U^H B U, S = P + P^H.
+template <int M>
+__device__ void pair_rr_run(const double *Ak, const double *Hk, float *Uk, double *Wk, int n, int bw, int nb,
+                            bool dense, int c0, int m, int lane, cx<float> (*ust)[kPairMaxN], cd (*Qs)[kPairMax],
+                            double *ls, int *okflag) {
+    for (int e = lane; e < M * n; e += 64) {
+        const int a = e / n, r = e - a * n;
+        ust[a][r] = a < m ? ldx(Uk, (long)r * n + c0 + a) : cx<float>{0.f, 0.f};
+    }
+    __syncthreads();
+    cd P[M][M], G[M][M];
+#pragma unroll
+    for (int a = 0; a < M; a++)
+#pragma unroll
+        for (int b = 0; b < M; b++) P[a][b] = G[a][b] = cd{0.0, 0.0};
+    for (int r = lane; r < n; r += 64) {
+        cd u[M], y[M];
+#pragma unroll
+        for (int a = 0; a < M; a++) {
+            u[a] = cd{ust[a][r].x, ust[a][r].y};
+            y[a] = cd{0.0, 0.0};
+        }
+        const int nc = dense ? n - r : bw + 1;
+        for (int k = 0; k < nc; k++) {
+            const int c = r + k >= n ? r + k - n : r + k;
+            cd arc = band_entry(Ak, Hk, n, bw, nb, r, c);
+            if (k == 0) arc = cd{0.5 * arc.x, 0.0};
+#pragma unroll
+            for (int a = 0; a < M; a++) cfma(y[a], arc, cd{ust[a][c].x, ust[a][c].y});
+        }
+#pragma unroll
+        for (int a = 0; a < M; a++)
+#pragma unroll
+            for (int b = 0; b < M; b++) {
+                cfmac(P[a][b], u[a], y[b]);
+                cfmac(G[a][b], u[a], u[b]);
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < M; a++)
+#pragma unroll
+        for (int b = 0; b < M; b++) {
+            P[a][b] = cd{wave_sum(P[a][b].x), wave_sum(P[a][b].y)};
+            G[a][b] = cd{wave_sum(G[a][b].x), wave_sum(G[a][b].y)};
+        }
+    if (lane == 0) {
+        // S = P + P^H; G Hermitian part (Hermitian up to rounding)
+        bool ok;
+        cd Sq2[2][2], Gq2[2][2], Q2[2][2];
+        cd Sq[kRefMaxCluster][kRefMaxCluster], Gq[kRefMaxCluster][kRefMaxCluster], Q[kRefMaxCluster][kRefMaxCluster];
+        if constexpr (M == 2) {
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    Sq2[a][b] = cadd(P[a][b], cconj(P[b][a]));
+                    Gq2[a][b] = cscale(cadd(G[a][b], cconj(G[b][a])), 0.5);
+                }
+            ok = small_geneig2(Sq2, Gq2, Q2);
+        } else {
+            for (int a = 0; a < m; a++)
+                for (int b = 0; b < m; b++) {
+                    Sq[a][b] = cadd(P[a][b], cconj(P[b][a]));
+                    Gq[a][b] = cscale(cadd(G[a][b], cconj(G[b][a])), 0.5);
+                }
+            ok = small_geneig(m, Sq, Gq, Q);
+        }
+        const int mm = M == 2 ? 2 : m;    // compile-time bounds on the pair path (registers only)
+#pragma unroll
+        for (int a = 0; a < M; a++) {
+            double la = 0.0;
+            cd qa[M];
+#pragma unroll
+            for (int b = 0; b < M; b++) {
+                if constexpr (M == 2) qa[b] = Q2[b][a];
+                else qa[b] = (a < m && b < m) ? Q[b][a] : cd{0.0, 0.0};
+            }
+            if (ok && a < mm) {
+                // l_a = q_a^H S q_a (Q^H G Q = I); qa is zero beyond m
+                cd acc{0.0, 0.0};
+#pragma unroll
+                for (int b = 0; b < M; b++) {
+                    cd t{0.0, 0.0};
+#pragma unroll
+                    for (int c = 0; c < M; c++) {
+                        cd sbc;
+                        if constexpr (M == 2) sbc = Sq2[b][c];
+                        else sbc = (b < m && c < m) ? Sq[b][c] : cd{0.0, 0.0};
+                        cfma(t, sbc, qa[c]);
+                    }
+                    cfmac(acc, qa[b], t);
+                }
+                la = acc.x;
+            }
+            ls[a] = la;
+#pragma unroll
+            for (int b = 0; b < M; b++) Qs[b][a] = (ok && a < mm && b < mm) ? qa[b] : cd{0.0, 0.0};
+        }
+        if (ok)
+            for (int a = 0; a < mm; a++) Wk[c0 + a] = ls[a];
+        *okflag = ok ? 1 : 0;
+    }
+    __syncthreads();
+    // u_a <- sum_b u_b Q[b][a] (skipped when G had no Cholesky factor)
+    if (*okflag) {
+        for (int r = lane; r < n; r += 64) {
+            cd u[M];
+#pragma unroll
+            for (int b = 0; b < M; b++) u[b] = cd{ust[b][r].x, ust[b][r].y};
+            for (int a = 0; a < (M == 2 ? 2 : m); a++) {
+                cd acc{0.0, 0.0};
+#pragma unroll
+                for (int b = 0; b < M; b++) cfma(acc, u[b], Qs[b][a]);
+                stx(Uk, (long)r * n + c0 + a, cx<float>{(float)acc.x, (float)acc.y});
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// One wave per run (persistent over the list); pairs, the common case, on the
+// register-resident 2 x 2 path.
 __global__ __launch_bounds__(64) void pair_rr_kernel(const double *__restrict__ A, const double *__restrict__ Hb,
                                                      float *__restrict__ U, int n, int bw, int cap,
                                                      const int *__restrict__ list, const int *__restrict__ count,
@@ -451,82 +620,10 @@ __global__ __launch_bounds__(64) void pair_rr_kernel(const double *__restrict__ 
         const double *Hk = Hb ? Hb + 2 * kp * (long)n * nb : nullptr;
         const double *Ak = Hb ? nullptr : A + 2 * kp * (long)n * n;
         float *Uk = U + 2 * kp * (long)n * n;
-        cd S[MC][MC], G[MC][MC];
-        for (int a = 0; a < MC; a++)
-            for (int b = 0; b < MC; b++) S[a][b] = G[a][b] = cd{0.0, 0.0};
-        for (int e = lane; e < MC * n; e += 64) {
-            const int a = e / n, r = e - a * n;
-            ust[a][r] = a < m ? ldx(Uk, (long)r * n + c0 + a) : cx<float>{0.f, 0.f};
-        }
-        __syncthreads();
-        for (int r = lane; r < n; r += 64) {
-            cd u[MC], y[MC];
-            for (int a = 0; a < MC; a++) {
-                u[a] = cd{ust[a][r].x, ust[a][r].y};
-                y[a] = cd{0.0, 0.0};
-            }
-            // y_a[r] = sum_c A(r, c) u_a[c] over the circular band of row r
-            const int nc = dense ? n : 2 * bw + 1;
-            for (int k = 0; k < nc; k++) {
-                int c = dense ? k : r - bw + k;
-                c = c < 0 ? c + n : (c >= n ? c - n : c);
-                const cd arc = band_entry(Ak, Hk, n, bw, nb, r, c);
-#pragma unroll
-                for (int a = 0; a < MC; a++) cfma(y[a], arc, cd{ust[a][c].x, ust[a][c].y});
-            }
-            for (int a = 0; a < MC; a++)
-                for (int b = 0; b < MC; b++) {
-                    cfmac(S[a][b], u[a], y[b]);
-                    cfmac(G[a][b], u[a], u[b]);
-                }
-        }
-        for (int a = 0; a < MC; a++)
-            for (int b = 0; b < MC; b++) {
-                S[a][b] = cd{wave_sum(S[a][b].x), wave_sum(S[a][b].y)};
-                G[a][b] = cd{wave_sum(G[a][b].x), wave_sum(G[a][b].y)};
-            }
-        if (lane == 0) {
-            cd Sq[kRefMaxCluster][kRefMaxCluster], Gq[kRefMaxCluster][kRefMaxCluster], Q[kRefMaxCluster][kRefMaxCluster];
-            for (int a = 0; a < m; a++)
-                for (int b = 0; b < m; b++) {
-                    // Hermitian parts (S, G are Hermitian up to rounding)
-                    Sq[a][b] = cscale(cadd(S[a][b], cconj(S[b][a])), 0.5);
-                    Gq[a][b] = cscale(cadd(G[a][b], cconj(G[b][a])), 0.5);
-                }
-            const bool ok = small_geneig(m, Sq, Gq, Q);
-            for (int a = 0; a < MC; a++) {
-                double la = 0.0;
-                if (ok && a < m) {
-                    // l_a = q_a^H S q_a (Q^H G Q = I)
-                    cd acc{0.0, 0.0};
-                    for (int b = 0; b < m; b++) {
-                        cd t{0.0, 0.0};
-                        for (int c = 0; c < m; c++) cfma(t, Sq[b][c], Q[c][a]);
-                        cfmac(acc, Q[b][a], t);
-                    }
-                    la = acc.x;
-                }
-                ls[a] = la;
-                for (int b = 0; b < MC; b++) Qs[b][a] = (ok && a < m && b < m) ? Q[b][a] : cd{0.0, 0.0};
-            }
-            if (ok)
-                for (int a = 0; a < m; a++) W[kp * n + c0 + a] = ls[a];
-            okflag = ok ? 1 : 0;
-        }
-        __syncthreads();
-        // u_a <- sum_b u_b Q[b][a] (skipped when G had no Cholesky factor)
-        if (okflag) {
-            for (int r = lane; r < n; r += 64) {
-                cd u[MC];
-                for (int b = 0; b < MC; b++) u[b] = cd{ust[b][r].x, ust[b][r].y};
-                for (int a = 0; a < m; a++) {
-                    cd acc{0.0, 0.0};
-                    for (int b = 0; b < m; b++) cfma(acc, u[b], Qs[b][a]);
-                    stx(Uk, (long)r * n + c0 + a, cx<float>{(float)acc.x, (float)acc.y});
-                }
-            }
-        }
-        __syncthreads();
+        if (m == 2)
+            pair_rr_run<2>(Ak, Hk, Uk, W + kp * n, n, bw, nb, dense, c0, m, lane, ust, Qs, ls, &okflag);
+        else
+            pair_rr_run<MC>(Ak, Hk, Uk, W + kp * n, n, bw, nb, dense, c0, m, lane, ust, Qs, ls, &okflag);
     }
 }
 

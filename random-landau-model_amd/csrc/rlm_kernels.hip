@@ -212,21 +212,22 @@ __global__ __launch_bounds__(256) void hband_kernel(const double *__restrict__ E
     const int nrb = (N + 15) / 16, ncb = (nb + 15) / 16;
     for (int tile = w; tile < nrb * ncb; tile += 4) {
         const int j0 = (tile / ncb) * 16, b0 = (tile % ncb) * 16;
-        v4 hr = {0.0, 0.0, 0.0, 0.0}, hi = {0.0, 0.0, 0.0, 0.0};
+        // three real products per complex one: P1 = (er + ei) xr, P2 = er (xi - xr),
+        // P3 = ei (xr + xi); Re = P1 - P3, Im = P1 + P2
+        v4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
         for (int m0 = 0; m0 < d; m0 += 4) {
             const int m = m0 + lq;
             const cd e = (m < d && j0 + li < N) ? ld(Et, (long)m * N + j0 + li) : cd{0.0, 0.0};    // E[m][j]
             const cd x = (m < d && b0 + li < nb) ? ld(Wz, (long)m * nb + b0 + li) : cd{0.0, 0.0}; // Wb[m][b]
-            hr = M::mma(e.x, x.x, hr);
-            hr = M::mma(-e.y, x.y, hr);
-            hi = M::mma(e.x, x.y, hi);
-            hi = M::mma(e.y, x.x, hi);
+            p1 = M::mma(e.x + e.y, x.x, p1);
+            p2 = M::mma(e.x, x.y - x.x, p2);
+            p3 = M::mma(e.y, x.x + x.y, p3);
         }
         const int b = b0 + li;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int j = j0 + M::row(lane, v);
-            if (j < N && b < nb) st(Hk, (long)j * nb + b, cd{hr[v], b == 0 ? 0.0 : hi[v]});
+            if (j < N && b < nb) st(Hk, (long)j * nb + b, cd{p1[v] - p3[v], b == 0 ? 0.0 : p1[v] + p2[v]});
         }
     }
 }
@@ -385,6 +386,117 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
     }
 }
 
+// Eigenvalues-only Rayleigh quotients for a banded window (16 + 2 bw + 3 < n): the
+// HALF product 2 Re(u^H B u) of rayleigh_kernel with two state blocks (32 states) per
+// wave, so that each gathered band entry feeds both, the row blocks and their K-steps
+// run as one sequence whose operands are loaded one step ahead, and each complex
+// product takes three real MFMAs instead of four: with a = ar + i ai, u = ur + i ui,
+// P1 = (ar + ai) ur, P2 = ar (ui - ur), P3 = ai (ur + ui), Re(a u) = P1 - P3 and
+// Im(a u) = P1 + P2 (absolute error a few ulps of |a||u|, as for the direct product).
+// Row r of block r0 meets columns r .. r + bw only (offsets kk - li in [0, bw] of the
+// window [r0, r0 + 15 + bw]); the window may wrap past column n - 1.
+template <typename TU>
+__global__ __launch_bounds__(256) void rayleigh_half_band_kernel(const double *__restrict__ A,
+                                                                 const double *__restrict__ Hb,
+                                                                 const TU *__restrict__ U, int n, int bw, long nkp,
+                                                                 int wg_per_kp, double *__restrict__ W) {
+    using M = Mf<double>;
+    using v4 = M::v4;
+    constexpr int SB = 2;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const long q = blockIdx.x >> 3;
+    const long kp = (q / wg_per_kp) * 8 + (blockIdx.x & 7);
+    const int sb0 = ((int)(q % wg_per_kp) * 4 + w) * SB;   // first state block of this wave
+    const int nsb = (n + 15) / 16;
+    if (kp >= nkp || sb0 >= nsb) return;                    // no barriers in this kernel
+    const int nb = band_nb(n, bw);
+    const double *Ak = Hb ? nullptr : A + 2 * kp * (long)n * n;
+    const double *Hk = Hb ? Hb + 2 * kp * (long)n * nb : nullptr;
+    const TU *Uk = U + 2 * kp * (long)n * n;
+    int jj[SB];
+    bool jin[SB];
+#pragma unroll
+    for (int b = 0; b < SB; b++) {
+        jj[b] = (sb0 + b) * 16 + li;
+        jin[b] = jj[b] < n;
+    }
+    const int kw = 16 + bw, nks = (kw + 3) / 4, nrb = (n + 15) / 16;
+    // operands of K-step k0 of row block r0: this lane's A element and U[c][j] of each block
+    auto fetch = [&](int r0, int k0, double &ar, double &ai, cx<TU> (&u)[SB]) {
+        const int kk = k0 + lq, r = r0 + li, dl = kk - li;
+        int c = r0 + kk;
+        c = c >= n ? c - n : c;
+        cd a{0.0, 0.0};
+        if (r < n && dl >= 0 && dl <= bw) {
+            a = band_entry(Ak, Hk, n, bw, nb, r, c);
+            if (dl == 0) a = cd{0.5 * a.x, 0.0};
+        }
+        ar = a.x;
+        ai = a.y;
+        const bool kin = kk < kw;
+#pragma unroll
+        for (int b = 0; b < SB; b++) u[b] = (kin && jin[b]) ? ldx(Uk, (long)c * n + jj[b]) : cx<TU>{(TU)0, (TU)0};
+    };
+    double num[SB], uu[SB];
+    v4 p1[SB], p2[SB], p3[SB];
+#pragma unroll
+    for (int b = 0; b < SB; b++) {
+        num[b] = 0.0;
+        uu[b] = 0.0;
+        p1[b] = p2[b] = p3[b] = v4{0.0, 0.0, 0.0, 0.0};
+    }
+    double ar_n, ai_n;
+    cx<TU> u_n[SB];
+    fetch(0, 0, ar_n, ai_n, u_n);
+    int r0 = 0, ks = 0;
+    while (r0 < n) {
+        const double ar = ar_n, ai = ai_n;
+        cx<TU> u[SB];
+#pragma unroll
+        for (int b = 0; b < SB; b++) u[b] = u_n[b];
+        const bool last_k = ks + 1 == nks;
+        const int r0n = last_k ? r0 + 16 : r0, ksn = last_k ? 0 : ks + 1;
+        if (r0n < n) fetch(r0n, 4 * ksn, ar_n, ai_n, u_n);
+        const double apb = ar + ai;
+#pragma unroll
+        for (int b = 0; b < SB; b++) {
+            const double ur = u[b].x, ui = u[b].y;
+            p1[b] = M::mma(apb, ur, p1[b]);
+            p2[b] = M::mma(ar, ui - ur, p2[b]);
+            p3[b] = M::mma(ai, ur + ui, p3[b]);
+        }
+        if (last_k) {
+#pragma unroll
+            for (int b = 0; b < SB; b++) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int rr = r0 + M::row(lane, v);
+                    if (rr < n && jin[b]) {
+                        const cx<TU> x = ldx(Uk, (long)rr * n + jj[b]);
+                        const double ur = x.x, ui = x.y;
+                        num[b] += ur * (p1[b][v] - p3[b][v]) + ui * (p1[b][v] + p2[b][v]);
+                        uu[b] += ur * ur + ui * ui;
+                    }
+                }
+                p1[b] = p2[b] = p3[b] = v4{0.0, 0.0, 0.0, 0.0};
+            }
+        }
+        r0 = r0n;
+        ks = ksn;
+    }
+#pragma unroll
+    for (int b = 0; b < SB; b++) {
+        // sum over the four lanes holding state j (lane bits 4-5)
+        for (int off = 16; off < kWave; off <<= 1) {
+            num[b] += __shfl_xor(num[b], off, kWave);
+            uu[b] += __shfl_xor(uu[b], off, kWave);
+        }
+        if (jin[b] && W && lq == 0) W[kp * n + jj[b]] = uu[b] > 0.0 ? 2.0 * num[b] / uu[b] : 0.0;
+    }
+}
+
 int rayleigh_resid_slots(int n) { return (n + 15) / 16; }
 
 template <typename TU>
@@ -396,7 +508,11 @@ static int launch_rayleigh_t(const double *A, const TU *U, int n, int bw, long n
     if (resid)
         hipLaunchKernelGGL((rayleigh_kernel<false, TU>), dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A,
                            Hb, U, n, bw, nkp, wg, W, resid);
-    else
+    else if (16 + 2 * bw + 3 < n) {
+        const int wg2 = (nsb + 7) / 8;   // 4 waves x 2 state blocks
+        hipLaunchKernelGGL((rayleigh_half_band_kernel<TU>), dim3((unsigned)(groups * 8 * wg2)), dim3(256), 0, stream,
+                           A, Hb, U, n, bw, nkp, wg2, W);
+    } else
         hipLaunchKernelGGL((rayleigh_kernel<true, TU>), dim3((unsigned)(groups * 8 * wg)), dim3(256), 0, stream, A,
                            Hb, U, n, bw, nkp, wg, W, resid);
     RLM_HIP_CHECK(hipGetLastError());
