@@ -61,13 +61,21 @@ __global__ __launch_bounds__(256) void links_kernel(const TU *__restrict__ U, lo
 // diag(e^{i 2 pi j/N}) A diag(e^{-i 2 pi j/N})):
 //   u(t1, g)[i] = u(t1, 0)[(i+1) % N],  u(g, t2)[i] = e^{i 2 pi i/N} u(0, t2)[i].
 // Links are written for the full (g+1)^2 mesh, kp = (s*(g+1) + t1)*(g+1) + t2, as
-// links_kernel does; grid (S*(g+1)^2, state chunks of 256), block 256, ns complex
-// doubles of dynamic LDS.
+// links_kernel does; grid (8 * ceil(nkp / 8), state chunks of 256), block 256, ns
+// complex doubles of dynamic LDS. Workgroups go to the 8 XCDs round-robin; with FP64
+// vectors XCD x (blockIdx.x % 8) takes the contiguous run of points x * per .. and
+// walks it in order, so that the neighbours (t1, t2 + 1) and (t1 + 1, t2) a point reads
+// are read by the same XCD's workgroups at about the same time, from its L2 (n_phi =
+// 256, 34848 points: 13.1 -> 11.6 ms); FP32 vectors keep the dispatch order, which
+// measures faster for them (7.2 against 7.8 ms).
 template <typename TU>
 __global__ __launch_bounds__(256) void links_derived_kernel(const TU *__restrict__ U, long strideU, int ns, int g,
-                                                            double *__restrict__ Lk1, double *__restrict__ Lk2) {
+                                                            long nkp, double *__restrict__ Lk1,
+                                                            double *__restrict__ Lk2) {
     extern __shared__ cd phs[];                              // e^{i 2 pi i/N}
-    const long kp = blockIdx.x;
+    const long per = (nkp + 7) / 8;
+    const long kp = sizeof(TU) == sizeof(double) ? (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3) : (long)blockIdx.x;
+    if (kp >= nkp) return;                                   // whole workgroup, before any barrier
     const int g1 = g + 1;
     const long s = kp / ((long)g1 * g1);
     const int rem = (int)(kp % ((long)g1 * g1));
@@ -196,14 +204,14 @@ int launch_links_derived(const void *U, bool fp32, long strideU, int ns, int g, 
     const long nkp = (long)n_samples * (g + 1) * (g + 1);
     if (nkp <= 0) return 0;
     if (ns > 4096 || g < 1) return -22;   // phase table: ns complex doubles of LDS
-    const dim3 grid((unsigned)nkp, (ns + 255) / 256);
+    const dim3 grid((unsigned)(8 * ((nkp + 7) / 8)), (ns + 255) / 256);
     const size_t lds = sizeof(cd) * ns;
     if (fp32)
         hipLaunchKernelGGL(links_derived_kernel<float>, grid, dim3(256), lds, stream, static_cast<const float *>(U),
-                           strideU, ns, g, Lk1, Lk2);
+                           strideU, ns, g, nkp, Lk1, Lk2);
     else
         hipLaunchKernelGGL(links_derived_kernel<double>, grid, dim3(256), lds, stream, static_cast<const double *>(U),
-                           strideU, ns, g, Lk1, Lk2);
+                           strideU, ns, g, nkp, Lk1, Lk2);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -396,7 +396,7 @@ __global__ __launch_bounds__(256) void rayleigh_kernel(const double *__restrict_
 // Row r of block r0 meets columns r .. r + bw only (offsets kk - li in [0, bw] of the
 // window [r0, r0 + 15 + bw]); the window may wrap past column n - 1.
 template <typename TU>
-__global__ __launch_bounds__(256) void rayleigh_half_band_kernel(const double *__restrict__ A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void rayleigh_half_band_kernel(const double *__restrict__ A,
                                                                  const double *__restrict__ Hb,
                                                                  const TU *__restrict__ U, int n, int bw, long nkp,
                                                                  int wg_per_kp, double *__restrict__ W) {
@@ -423,18 +423,32 @@ __global__ __launch_bounds__(256) void rayleigh_half_band_kernel(const double *_
         jin[b] = jj[b] < n;
     }
     const int kw = 16 + bw, nks = (kw + 3) / 4, nrb = (n + 15) / 16;
-    // operands of K-step k0 of row block r0: this lane's A element and U[c][j] of each block
-    auto fetch = [&](int r0, int k0, double &ar, double &ai, cx<TU> (&u)[SB]) {
+    // operands of K-step k0 of row block r0: this lane's A element, loaded raw with a
+    // mode applied when it is used (0: outside the half band, 1: as stored, 2: conjugate,
+    // 3: half the real part, the diagonal), so that no load is waited for before the
+    // current step's MFMAs issue; U[c][j] of each block. Band form: A(r, c) =
+    // conj(H[r][c - r]) for c >= r, H[c][2 bw + 1 - (kk - li)] where the window wrapped.
+    auto fetch = [&](int r0, int k0, cd &araw, int &mode, cx<TU> (&u)[SB]) {
         const int kk = k0 + lq, r = r0 + li, dl = kk - li;
         int c = r0 + kk;
         c = c >= n ? c - n : c;
-        cd a{0.0, 0.0};
-        if (r < n && dl >= 0 && dl <= bw) {
-            a = band_entry(Ak, Hk, n, bw, nb, r, c);
-            if (dl == 0) a = cd{0.5 * a.x, 0.0};
+        long off;
+        int md;
+        if (Hk) {
+            const bool wrap = c < r;
+            off = wrap ? (long)c * nb + (2 * bw + 1 - dl) : (long)r * nb + dl;
+            md = wrap ? 1 : 2;
+        } else {
+            off = (long)c * n + r;
+            md = 1;
         }
-        ar = a.x;
-        ai = a.y;
+        if (dl == 0) md = 3;
+        if (!(r < n && dl >= 0 && dl <= bw)) {
+            off = 0;
+            md = 0;
+        }
+        araw = ld(Hk ? Hk : Ak, off);
+        mode = md;
         const bool kin = kk < kw;
 #pragma unroll
         for (int b = 0; b < SB; b++) u[b] = (kin && jin[b]) ? ldx(Uk, (long)c * n + jj[b]) : cx<TU>{(TU)0, (TU)0};
@@ -447,18 +461,29 @@ __global__ __launch_bounds__(256) void rayleigh_half_band_kernel(const double *_
         uu[b] = 0.0;
         p1[b] = p2[b] = p3[b] = v4{0.0, 0.0, 0.0, 0.0};
     }
-    double ar_n, ai_n;
-    cx<TU> u_n[SB];
-    fetch(0, 0, ar_n, ai_n, u_n);
+    cd a_n;
+    int mode_n;
+    cx<TU> u_n[SB], eu[SB][4];
+    fetch(0, 0, a_n, mode_n, u_n);
     int r0 = 0, ks = 0;
     while (r0 < n) {
-        const double ar = ar_n, ai = ai_n;
+        const double ar = mode_n == 0 ? 0.0 : (mode_n == 3 ? 0.5 * a_n.x : a_n.x);
+        const double ai = (mode_n == 0 || mode_n == 3) ? 0.0 : (mode_n == 2 ? -a_n.y : a_n.y);
         cx<TU> u[SB];
 #pragma unroll
         for (int b = 0; b < SB; b++) u[b] = u_n[b];
+        if (ks == 0) {      // this row block's U rows for the final dot products, loaded early
+#pragma unroll
+            for (int b = 0; b < SB; b++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int rr = r0 + M::row(lane, v);
+                    eu[b][v] = (rr < n && jin[b]) ? ldx(Uk, (long)rr * n + jj[b]) : cx<TU>{(TU)0, (TU)0};
+                }
+        }
         const bool last_k = ks + 1 == nks;
         const int r0n = last_k ? r0 + 16 : r0, ksn = last_k ? 0 : ks + 1;
-        if (r0n < n) fetch(r0n, 4 * ksn, ar_n, ai_n, u_n);
+        if (r0n < n) fetch(r0n, 4 * ksn, a_n, mode_n, u_n);
         const double apb = ar + ai;
 #pragma unroll
         for (int b = 0; b < SB; b++) {
@@ -472,13 +497,9 @@ __global__ __launch_bounds__(256) void rayleigh_half_band_kernel(const double *_
             for (int b = 0; b < SB; b++) {
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int rr = r0 + M::row(lane, v);
-                    if (rr < n && jin[b]) {
-                        const cx<TU> x = ldx(Uk, (long)rr * n + jj[b]);
-                        const double ur = x.x, ui = x.y;
-                        num[b] += ur * (p1[b][v] - p3[b][v]) + ui * (p1[b][v] + p2[b][v]);
-                        uu[b] += ur * ur + ui * ui;
-                    }
+                    const double ur = eu[b][v].x, ui = eu[b][v].y;     // zero outside the matrix
+                    num[b] += ur * (p1[b][v] - p3[b][v]) + ui * (p1[b][v] + p2[b][v]);
+                    uu[b] += ur * ur + ui * ui;
                 }
                 p1[b] = p2[b] = p3[b] = v4{0.0, 0.0, 0.0, 0.0};
             }
