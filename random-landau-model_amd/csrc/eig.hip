@@ -1236,6 +1236,18 @@ __device__ inline double rcp_nr(double x) {
     r = fma(r, fma(-x, r, 1.0), r);
     return r;
 }
+// One Newton-Raphson step: the estimate measures within 4.6e-8 (2^-24.4) relative on
+// MI355X, so one step leaves ~2^-48, enough for the FP32 eigenvector matrices of the
+// mixed-precision solver and for its secular roots at 2^-40.
+__device__ inline double rcp_nr1(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+// the reciprocal for eigenvector matrices of storage type QS
+template <typename QS>
+__device__ inline double rcp_q(double x) {
+    return sizeof(QS) == sizeof(float) ? rcp_nr1(x) : rcp_nr(x);
+}
 
 // Root j of 1 + sum_i z2_i / (dl_i - lambda) = 0 with z2_i = rho w_i^2, poles dl
 // ascending (sum_i w_i^2 = 1), to relative accuracy teps (2^-53 for FP64
@@ -1245,6 +1257,7 @@ __device__ inline double rcp_nr(double x) {
 // Returns origin index o and tau with lambda = dl[o] + tau.
 __device__ void secular_root(int j, int K, const double *dl, const double *z2, double rho, int &o_out,
                              double &tau_out, double teps) {
+    const bool one_nr = teps > dc_eps();      // see rcp_nr1
     int o;
     double lo, hi, tau;
     if (j < K - 1) {
@@ -1255,7 +1268,7 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
 #pragma unroll 4
         for (int i = 0; i < K; i++) {
             const double del = (dl[i] - dl[j]) - tm;
-            f += z2[i] * rcp_nr(del);
+            f += z2[i] * (one_nr ? rcp_nr1(del) : rcp_nr(del));
         }
         if (f >= 0.0) { o = j; lo = 0.0; hi = tm; tau = tm; }
         else { o = j + 1; lo = -tm; hi = 0.0; tau = -tm; }
@@ -1272,7 +1285,7 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
         // at most 63 and the two loops together run about K + 64 iterations
 #pragma unroll 4
         for (int i = 0; i <= j; i++) {
-            const double inv = rcp_nr((dl[i] - dlo) - tau);
+            const double inv = one_nr ? rcp_nr1((dl[i] - dlo) - tau) : rcp_nr((dl[i] - dlo) - tau);
             const double tt = z2[i] * inv;
             psi += tt;
             dpsi += tt * inv;
@@ -1280,7 +1293,7 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
         }
 #pragma unroll 4
         for (int i = j + 1; i < K; i++) {
-            const double inv = rcp_nr((dl[i] - dlo) - tau);
+            const double inv = one_nr ? rcp_nr1((dl[i] - dlo) - tau) : rcp_nr((dl[i] - dlo) - tau);
             const double tt = z2[i] * inv;
             phi += tt;
             dphi += tt * inv;
@@ -1400,7 +1413,7 @@ __device__ __forceinline__ void merge_gemm_mfma(int lane, int G, int lo, int mid
 #pragma unroll
             for (int u = 0; u < QT; u++) {
                 if (u < nq) {
-                    const double bv0 = wi * rcp_nr((di - qo[u]) - qtu[u]) * qiv[u];
+                    const double bv0 = wi * rcp_q<QS>((di - qo[u]) - qtu[u]) * qiv[u];
                     const QS bv = (kin && qiv[u] != 0.0) ? (QS)bv0 : (QS)0;
 #pragma unroll
                     for (int rb = 0; rb < RB; rb++) acc[rb][u] = MQ::mma(av[rb], bv, acc[rb][u]);
@@ -1685,12 +1698,12 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                     int j = 0;
                     for (; j + 1 < K; j += 2) {
                         const bool sa = j == i, sb = j + 1 == i;
-                        const double fa = ((di - ro[j]) - ta[j]) * rcp_nr(sa ? 1.0 : di - dl[j]);
-                        const double fb = ((di - ro[j + 1]) - ta[j + 1]) * rcp_nr(sb ? 1.0 : di - dl[j + 1]);
+                        const double fa = ((di - ro[j]) - ta[j]) * rcp_q<QS>(sa ? 1.0 : di - dl[j]);
+                        const double fb = ((di - ro[j + 1]) - ta[j + 1]) * rcp_q<QS>(sb ? 1.0 : di - dl[j + 1]);
                         pa *= sa ? 1.0 : fa;
                         pb *= sb ? 1.0 : fb;
                     }
-                    if (j < K && j != i) pa *= ((di - ro[j]) - ta[j]) * rcp_nr(di - dl[j]);
+                    if (j < K && j != i) pa *= ((di - ro[j]) - ta[j]) * rcp_q<QS>(di - dl[j]);
                     const double p = pa * pb;
                     sh.what[lo + i] = copysign(sqrt(fmax(-p, 0.0)), sh.wz[lo + i]);
                 }
@@ -1705,13 +1718,13 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                     double sa = 0.0, sb = 0.0;
                     int i = 0;
                     for (; i + 1 < K; i += 2) {
-                        const double ua = wh[i] * rcp_nr((dl[i] - oj) - tj);
-                        const double ub = wh[i + 1] * rcp_nr((dl[i + 1] - oj) - tj);
+                        const double ua = wh[i] * rcp_q<QS>((dl[i] - oj) - tj);
+                        const double ub = wh[i + 1] * rcp_q<QS>((dl[i + 1] - oj) - tj);
                         sa += ua * ua;
                         sb += ub * ub;
                     }
                     if (i < K) {
-                        const double ua = wh[i] * rcp_nr((dl[i] - oj) - tj);
+                        const double ua = wh[i] * rcp_q<QS>((dl[i] - oj) - tj);
                         sa += ua * ua;
                     }
                     sh.inrm[lo + j] = 1.0 / sqrt(sa + sb);
@@ -1743,7 +1756,7 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                 for (int idx = lane; idx < K * K; idx += G) {
                     const int i = idx / K, q = idx % K;
                     const double del = (dl[i] - sh.rc[lo + q]) - sh.tau[lo + q];
-                    S[(long)(lo + i) * n + q] = (QS)(sh.what[lo + i] * rcp_nr(del) * sh.inrm[lo + q]);
+                    S[(long)(lo + i) * n + q] = (QS)(sh.what[lo + i] * rcp_q<QS>(del) * sh.inrm[lo + q]);
                 }
             }
             __syncthreads();
