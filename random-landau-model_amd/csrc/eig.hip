@@ -2060,7 +2060,7 @@ __global__ __launch_bounds__(256) void larft_kernel(const T *__restrict__ Abase,
 // W2 = T W, U -= V W2. Workgroups are mapped so that the slabs of one matrix share
 // blockIdx % 8, i.e. one XCD and its L2, which then serves V and T to all of them.
 template <typename T, int CW>
-__global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__restrict__ Abase, long strideA, int lda,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void backtransform_mfma_kernel(const T *__restrict__ Abase, long strideA, int lda,
                                                                  int n, const T *__restrict__ Tin, long strideT,
                                                                  T *__restrict__ Uout, long strideU, int nslab,
                                                                  int batch) {
@@ -2127,13 +2127,17 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
                 for (int kk = 0; kk < 4; kk++) vst[w][i][(lq + 4 * kk) * VP + li] = vpre[i][kk];
             __builtin_amdgcn_wave_barrier();
         }
-        v4 wr[NCB], wi[NCB];
+        // complex products from three real MFMAs: a b with P1 = (a_r + a_i) b_r,
+        // P2 = a_r (b_i - b_r), P3 = a_i (b_r + b_i), Re = P1 - P3, Im = P1 + P2 (P1 in q1,
+        // -P3 and P2 straight into the real / imaginary accumulators)
+        v4 wr[NCB], wi[NCB], q1[NCB];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
             wr[cb] = v4{0, 0, 0, 0};
             wi[cb] = v4{0, 0, 0, 0};
+            q1[cb] = v4{0, 0, 0, 0};
         }
-        // W = V^H U over this wave's rows
+        // W = V^H U over this wave's rows (a = conj V)
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
             const int R = (w + 4 * i) * 16;
@@ -2147,12 +2151,13 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const CT x = vst[w][si][li * VP + M::row(lane, v)];  // V[R + row(l, v)][k = li]
+                const T xd = x.x - x.y;
 #pragma unroll
                 for (int cb = 0; cb < NCB; cb++) {
-                    wr[cb] = M::mma(x.x, ur[i][cb][v], wr[cb]);
-                    wr[cb] = M::mma(x.y, ui[i][cb][v], wr[cb]);
-                    wi[cb] = M::mma(x.x, ui[i][cb][v], wi[cb]);
-                    wi[cb] = M::mma(-x.y, ur[i][cb][v], wi[cb]);
+                    const T br = ur[i][cb][v], bi = ui[i][cb][v];
+                    q1[cb] = M::mma(xd, br, q1[cb]);
+                    wr[cb] = M::mma(x.y, br + bi, wr[cb]);
+                    wi[cb] = M::mma(x.x, bi - br, wi[cb]);
                 }
             }
             if (!STAGE_ALL) __builtin_amdgcn_wave_barrier();   // the next row block rewrites vst
@@ -2163,8 +2168,8 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
         for (int cb = 0; cb < NCB; cb++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                wb[((0 * NCB + cb) * 4 + v) * kWave + lane] = wr[cb][v];
-                wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = wi[cb][v];
+                wb[((0 * NCB + cb) * 4 + v) * kWave + lane] = q1[cb][v] + wr[cb][v];
+                wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = q1[cb][v] + wi[cb][v];
             }
         __syncthreads();
         if (PREFETCH && bk > 0) load_v(bk - 1);           // in flight during T W and the update
@@ -2187,17 +2192,24 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
         for (int cb = 0; cb < NCB; cb++) {
             w2r[cb] = v4{0, 0, 0, 0};
             w2i[cb] = v4{0, 0, 0, 0};
+            q1[cb] = v4{0, 0, 0, 0};
         }
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             const CT tv = Tb[bk * 256 + M::row(lane, s) * 16 + li];   // T[k1 = li][k2 = row(l, s)]
+            const T ts = tv.x + tv.y;
 #pragma unroll
             for (int cb = 0; cb < NCB; cb++) {
-                w2r[cb] = M::mma(tv.x, wr[cb][s], w2r[cb]);
-                w2r[cb] = M::mma(-tv.y, wi[cb][s], w2r[cb]);
-                w2i[cb] = M::mma(tv.x, wi[cb][s], w2i[cb]);
-                w2i[cb] = M::mma(tv.y, wr[cb][s], w2i[cb]);
+                const T br = wr[cb][s], bi = wi[cb][s];
+                q1[cb] = M::mma(ts, br, q1[cb]);
+                w2r[cb] = M::mma(-tv.y, br + bi, w2r[cb]);
+                w2i[cb] = M::mma(tv.x, bi - br, w2i[cb]);
             }
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCB; cb++) {
+            w2r[cb] += q1[cb];
+            w2i[cb] += q1[cb];
         }
         // U -= V W2 on this wave's rows (K index = reflector row(l, s))
 #pragma unroll
@@ -2205,16 +2217,24 @@ __global__ __launch_bounds__(256) void backtransform_mfma_kernel(const T *__rest
             const int R = (w + 4 * i) * 16;
             if (R + 15 < r0) continue;
 #pragma unroll
+            for (int cb = 0; cb < NCB; cb++) q1[cb] = v4{0, 0, 0, 0};
+#pragma unroll
             for (int s = 0; s < 4; s++) {
                 const CT x = STAGE_ALL ? vst[w][STAGE_ALL ? i : 0][M::row(lane, s) * VP + li]
                                        : refl_get(A, lda, n, j0, R + li, M::row(lane, s));   // V[R + li][k = row(l, s)]
+                const T xs = x.x + x.y;
 #pragma unroll
                 for (int cb = 0; cb < NCB; cb++) {
-                    ur[i][cb] = M::mma(-x.x, w2r[cb][s], ur[i][cb]);
-                    ur[i][cb] = M::mma(x.y, w2i[cb][s], ur[i][cb]);
-                    ui[i][cb] = M::mma(-x.x, w2i[cb][s], ui[i][cb]);
-                    ui[i][cb] = M::mma(-x.y, w2r[cb][s], ui[i][cb]);
+                    const T br = w2r[cb][s], bi = w2i[cb][s];
+                    q1[cb] = M::mma(xs, br, q1[cb]);                     // P1
+                    ur[i][cb] = M::mma(x.y, br + bi, ur[i][cb]);         // U_r -= P1 - P3
+                    ui[i][cb] = M::mma(-x.x, bi - br, ui[i][cb]);        // U_i -= P1 + P2
                 }
+            }
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) {
+                ur[i][cb] -= q1[cb];
+                ui[i][cb] -= q1[cb];
             }
         }
         __builtin_amdgcn_wave_barrier();                   // the next block rewrites vst
