@@ -350,6 +350,16 @@ __global__ __launch_bounds__(256) void lg_her2k_kernel(T *__restrict__ Abase, lo
                 di[cb][rb][v] = x.y;
             }
         }
+    // D -= conj(ya) xb. Complex float: three real MFMAs per complex product (a = conj ya,
+    // b = xb): P1 = (a_r + a_i) b_r in q, D_r -= P1 - P3 and D_i -= P1 + P2 with
+    // P3 = a_i (b_r + b_i), P2 = a_r (b_i - b_r) (n_phi = 3000: 63.2 -> 56.8 ms). Complex
+    // double keeps four: the q accumulators would cost it a wave per SIMD (128 -> 153 ms).
+    constexpr bool G3 = sizeof(T) == sizeof(float);
+    v4 q[2][2];
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) q[cb][rb] = v4{0, 0, 0, 0};
     const int K = 2 * nbk;
     for (int k4 = 0; k4 < K; k4 += 4) {
         const int k = k4 + lq;
@@ -364,13 +374,31 @@ __global__ __launch_bounds__(256) void lg_her2k_kernel(T *__restrict__ Abase, lo
             xb[u] = (kin && r < n) ? Xc[r] : C{(T)0, (T)0};
         }
 #pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const T yd = ya[cb].x - ya[cb].y;
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++) {
+                const T br = xb[rb].x, bi = xb[rb].y;
+                if constexpr (G3) {
+                    q[cb][rb] = M::mma(yd, br, q[cb][rb]);
+                    dr[cb][rb] = M::mma(-ya[cb].y, br + bi, dr[cb][rb]);
+                    di[cb][rb] = M::mma(-ya[cb].x, bi - br, di[cb][rb]);
+                } else {
+                    dr[cb][rb] = M::mma(-ya[cb].x, br, dr[cb][rb]);
+                    dr[cb][rb] = M::mma(-ya[cb].y, bi, dr[cb][rb]);
+                    di[cb][rb] = M::mma(-ya[cb].x, bi, di[cb][rb]);
+                    di[cb][rb] = M::mma(ya[cb].y, br, di[cb][rb]);
+                }
+            }
+        }
+    }
+    if constexpr (G3) {
+#pragma unroll
         for (int cb = 0; cb < 2; cb++)
 #pragma unroll
             for (int rb = 0; rb < 2; rb++) {
-                dr[cb][rb] = M::mma(-ya[cb].x, xb[rb].x, dr[cb][rb]);
-                dr[cb][rb] = M::mma(-ya[cb].y, xb[rb].y, dr[cb][rb]);
-                di[cb][rb] = M::mma(-ya[cb].x, xb[rb].y, di[cb][rb]);
-                di[cb][rb] = M::mma(ya[cb].y, xb[rb].x, di[cb][rb]);
+                dr[cb][rb] -= q[cb][rb];
+                di[cb][rb] -= q[cb][rb];
             }
     }
 #pragma unroll
@@ -439,26 +467,31 @@ __global__ __launch_bounds__(256) void lg_backtransform_kernel(const T *__restri
             }
         }
     };
-    // W partial of block j0 from the row block R held in ur/ui: K-step v stands for
-    // rows R + row(lane, v) (the rows of accumulator element v)
-    auto acc_w = [&](int j0, int R, v4 (&wr)[NCB], v4 (&wi)[NCB]) {
+    // Complex products from three real MFMAs: a b with P1 = (a_r + a_i) b_r,
+    // P2 = a_r (b_i - b_r), P3 = a_i (b_r + b_i), Re = P1 - P3, Im = P1 + P2 (P1 in its
+    // own accumulator, -P3 and P2 straight into the real / imaginary ones).
+    // W partial of block j0 from the row block R held in ur/ui (a = conj V): K-step v
+    // stands for rows R + row(lane, v) (the rows of accumulator element v); W = q1 + (wr, wi)
+    auto acc_w = [&](const CT (&xv)[4], v4 (&wr)[NCB], v4 (&wi)[NCB], v4 (&q1)[NCB]) {
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const CT x = lg_refl(A, n, j0, R + M::row(lane, v), li);   // conj(V)[k = li][row]
+            const CT x = xv[v];                                        // V[row][k = li]
+            const T xd = x.x - x.y;
 #pragma unroll
             for (int cb = 0; cb < NCB; cb++) {
-                wr[cb] = M::mma(x.x, ur[cb][v], wr[cb]);
-                wr[cb] = M::mma(x.y, ui[cb][v], wr[cb]);
-                wi[cb] = M::mma(x.x, ui[cb][v], wi[cb]);
-                wi[cb] = M::mma(-x.y, ur[cb][v], wi[cb]);
+                const T br = ur[cb][v], bi = ui[cb][v];
+                q1[cb] = M::mma(xd, br, q1[cb]);
+                wr[cb] = M::mma(x.y, br + bi, wr[cb]);
+                wi[cb] = M::mma(x.x, bi - br, wi[cb]);
             }
         }
     };
-    v4 wr[NCB], wi[NCB];
+    v4 wr[NCB], wi[NCB], q1[NCB];
 #pragma unroll
     for (int cb = 0; cb < NCB; cb++) {
         wr[cb] = v4{0, 0, 0, 0};
         wi[cb] = v4{0, 0, 0, 0};
+        q1[cb] = v4{0, 0, 0, 0};
     }
     {
         const int j0 = 16 * (nblk - 1);
@@ -466,7 +499,10 @@ __global__ __launch_bounds__(256) void lg_backtransform_kernel(const T *__restri
             const int R = 16 * rbk;
             if (R + 15 < j0 + 1) continue;
             load_u(R);
-            acc_w(j0, R, wr, wi);
+            CT xv[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) xv[v] = lg_refl(A, n, j0, R + M::row(lane, v), li);
+            acc_w(xv, wr, wi, q1);
         }
     }
     for (int bk = nblk - 1; bk >= 0; bk--) {
@@ -476,8 +512,8 @@ __global__ __launch_bounds__(256) void lg_backtransform_kernel(const T *__restri
         for (int cb = 0; cb < NCB; cb++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                wb[((0 * NCB + cb) * 4 + v) * kWave + lane] = wr[cb][v];
-                wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = wi[cb][v];
+                wb[((0 * NCB + cb) * 4 + v) * kWave + lane] = q1[cb][v] + wr[cb][v];
+                wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = q1[cb][v] + wi[cb][v];
             }
         __syncthreads();
 #pragma unroll
@@ -499,22 +535,27 @@ __global__ __launch_bounds__(256) void lg_backtransform_kernel(const T *__restri
         for (int cb = 0; cb < NCB; cb++) {
             w2r[cb] = v4{0, 0, 0, 0};
             w2i[cb] = v4{0, 0, 0, 0};
+            q1[cb] = v4{0, 0, 0, 0};
         }
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             const CT tv = Tb[bk * 256 + M::row(lane, s) * 16 + li];   // T[k1 = li][k2 = row(l, s)]
+            const T ts = tv.x + tv.y;
 #pragma unroll
             for (int cb = 0; cb < NCB; cb++) {
-                w2r[cb] = M::mma(tv.x, wr[cb][s], w2r[cb]);
-                w2r[cb] = M::mma(-tv.y, wi[cb][s], w2r[cb]);
-                w2i[cb] = M::mma(tv.x, wi[cb][s], w2i[cb]);
-                w2i[cb] = M::mma(tv.y, wr[cb][s], w2i[cb]);
+                const T br = wr[cb][s], bi = wi[cb][s];
+                q1[cb] = M::mma(ts, br, q1[cb]);
+                w2r[cb] = M::mma(-tv.y, br + bi, w2r[cb]);
+                w2i[cb] = M::mma(tv.x, bi - br, w2i[cb]);
             }
         }
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
+            w2r[cb] += q1[cb];
+            w2i[cb] += q1[cb];
             wr[cb] = v4{0, 0, 0, 0};
             wi[cb] = v4{0, 0, 0, 0};
+            q1[cb] = v4{0, 0, 0, 0};
         }
         const int lo = bk > 0 ? j0 - 15 : j0 + 1;            // first row either block needs
         // this wave's row blocks R = 16 (w + 4 k) >= lo - 15, software-pipelined: the U
@@ -560,16 +601,25 @@ __global__ __launch_bounds__(256) void lg_backtransform_kernel(const T *__restri
             if (16 * (rbk + 4) < n) fetch(16 * (rbk + 4));
             if (R + 15 >= j0 + 1) {
                 // U -= V W2 on rows R.. (K index = reflector row(l, s))
+                v4 qu[NCB];
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) qu[cb] = v4{0, 0, 0, 0};
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
                     const CT x = xu[s];
+                    const T xs = x.x + x.y;
 #pragma unroll
                     for (int cb = 0; cb < NCB; cb++) {
-                        ur[cb] = M::mma(-x.x, w2r[cb][s], ur[cb]);
-                        ur[cb] = M::mma(x.y, w2i[cb][s], ur[cb]);
-                        ui[cb] = M::mma(-x.x, w2i[cb][s], ui[cb]);
-                        ui[cb] = M::mma(-x.y, w2r[cb][s], ui[cb]);
+                        const T br = w2r[cb][s], bi = w2i[cb][s];
+                        qu[cb] = M::mma(xs, br, qu[cb]);                 // P1
+                        ur[cb] = M::mma(x.y, br + bi, ur[cb]);           // U_r -= P1 - P3
+                        ui[cb] = M::mma(-x.x, bi - br, ui[cb]);          // U_i -= P1 + P2
                     }
+                }
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) {
+                    ur[cb] -= qu[cb];
+                    ui[cb] -= qu[cb];
                 }
 #pragma unroll
                 for (int cb = 0; cb < NCB; cb++) {
@@ -581,20 +631,7 @@ __global__ __launch_bounds__(256) void lg_backtransform_kernel(const T *__restri
                     }
                 }
             }
-            if (bk > 0) {
-                // W partial of block bk - 1 from the updated rows (K-step v = rows R + row(lane, v))
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const CT x = xw[v];
-#pragma unroll
-                    for (int cb = 0; cb < NCB; cb++) {
-                        wr[cb] = M::mma(x.x, ur[cb][v], wr[cb]);
-                        wr[cb] = M::mma(x.y, ui[cb][v], wr[cb]);
-                        wi[cb] = M::mma(x.x, ui[cb][v], wi[cb]);
-                        wi[cb] = M::mma(-x.y, ur[cb][v], wi[cb]);
-                    }
-                }
-            }
+            if (bk > 0) acc_w(xw, wr, wi, q1);    // W partial of block bk - 1 from the updated rows
         }
     }
 }
