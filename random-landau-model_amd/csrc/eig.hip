@@ -1226,6 +1226,16 @@ struct DcShared {
 };
 
 __device__ inline double dc_eps() { return 1.1102230246251565e-16; }  // 2^-53
+// Deflation threshold unit: 2^-53 for FP64 eigenvector matrices; 2^-30 for the FP32
+// ones of the mixed-precision solver, whose tridiagonal carries FP32 rounding anyway and
+// whose eigenvalues are replaced by FP64 Rayleigh quotients (a deflated z_i perturbs T by
+// at most 8 * 2^-30 of its scale; divide & conquer at n_phi = 256: 69.2 -> 66.5 ms, at
+// n_phi = 3000: 426 -> 327 ms; energies and Chern integers unchanged against FP64; 2^-26
+// measured no faster).
+template <typename QS>
+__device__ inline double defl_eps() {
+    return sizeof(QS) == sizeof(float) ? 0x1p-30 : dc_eps();
+}
 
 // 1/x from the hardware reciprocal estimate refined by two Newton-Raphson steps
 // r <- r + r (1 - x r) (each step squares the relative error); x must be finite
@@ -1532,7 +1542,7 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                     dmx = fmax(dmx, __shfl_xor(dmx, off, kWave));
                     zmx = fmax(zmx, __shfl_xor(zmx, off, kWave));
                 }
-                const double tol = 8.0 * dc_eps() * fmax(dmx, zmx);
+                const double tol = 8.0 * defl_eps<QS>() * fmax(dmx, zmx);
                 const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
                 int K = 0, nd = 0, KL = 0, KR = 0;
                 bool have = false, conflict = false;
@@ -1595,7 +1605,7 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                     dmax = fmax(dmax, fabs(sh.ds[lo + i]));
                     zmax = fmax(zmax, fabs(sh.zs[lo + i]));
                 }
-                const double tol = 8.0 * dc_eps() * fmax(dmax, zmax);
+                const double tol = 8.0 * defl_eps<QS>() * fmax(dmax, zmax);
                 int K = 0, nd = 0, nr = 0, pj = -1, KL = 0, KR = 0;
                 // a non-deflated column q: record its node-local source column and which
                 // halves of the node's rows it can be non-zero in (a rotation mixes halves)
