@@ -903,14 +903,12 @@ extern "C" int hlm_plan_load(hlm_plan_t p, long first_sample, int count) {
 extern "C" int hlm_plan_execute(hlm_plan_t p) {
     if (!p || p->loaded < 1) return -22;
     const int S = p->loaded, N = p->n_phi, ns = p->n_sites, g = p->g;
-    const long ntw = p->ntw, nkp = S * ntw, nn = (long)N * N, sz = (long)ns * N;
+    const long ntw = p->ntw, nkp = S * ntw, nn = (long)N * N;
     hipStream_t s = p->stream;
     int rc;
     RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
-    // Htr[(s*ntw + tw)] = Phi(tw)^H diag(V_s) Phi(tw): b1 = sample, b2 = twist
-    if ((rc = launch_zgemm(N, N, ns, p->dPhi.d(), 1, ns, 0, sz, 1, p->dPhi.d(), 1, ns, 0, sz, p->dV.d(), ns, 0,
-                           p->dH.d(), 1, N, ntw * nn, nn, S, (int)ntw, s)))
-        return rc;
+    // Htr[(s*ntw + tw)] = Phi(tw)^H diag(V_s) Phi(tw), upper triangle
+    if ((rc = launch_hlm_project(p->dPhi.d(), p->dV.d(), N, ns, S, ntw, p->dH.d(), s))) return rc;
     // the reference's zheevd reads the upper triangle of Htr (mkl_diag.cpp:6)
     {
         hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)nkp), dim3(256), 0, s, p->dH.d(), (long)N, nn, N, 0, 1);

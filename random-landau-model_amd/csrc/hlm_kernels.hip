@@ -146,13 +146,79 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, const d
         }
 }
 
+// Projected Hamiltonians Htr[s*ntw + tw] = Phi(tw)^H diag(V_s) Phi(tw) (column-major
+// N x N, upper triangle: the rest is set from it by hermitize_kernel, as the
+// reference's zheevd reads the upper triangle) on the FP64 matrix cores: one workgroup
+// per (tw, s), blockIdx.x = tw * S + s; the 16 x 16 output blocks (I <= J) dealt to the
+// 4 waves; K = the n_sites sites in steps of 4. Phi(tw) is stored state-major,
+// Phi(x, a) at a * ns + x. Each complex product takes three real MFMAs (a = conj(Phi(x,
+// a)) V_s(x), b = Phi(x, b): P1 = (a_r + a_i) b_r, P2 = a_r (b_i - b_r), P3 = a_i (b_r +
+// b_i), Re = P1 - P3, Im = P1 + P2), the operands of the next K-step loading while the
+// current one runs.
+__global__ __launch_bounds__(256) void hlm_project_kernel(const double *__restrict__ Phi, const double *__restrict__ V,
+                                                          int N, int ns, int S, long ntw, double *__restrict__ H) {
+    using M = Mf<double>;
+    using v4 = M::v4;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const long tw = blockIdx.x / S;
+    const int smp = (int)(blockIdx.x % S);
+    const long nn = (long)N * N;
+    const double *Pt = Phi + 2 * tw * (long)ns * N;
+    const double *Vs = V + (long)smp * ns;
+    double *Hk = H + 2 * ((long)smp * ntw + tw) * nn;
+    const int nbk = (N + 15) / 16, nblk = nbk * (nbk + 1) / 2;
+    for (int t = w; t < nblk; t += 4) {
+        // block t -> (I, J), I <= J, row-major over the upper block triangle
+        int I = 0, rem = t;
+        while (rem >= nbk - I) {
+            rem -= nbk - I;
+            I++;
+        }
+        const int J = I + rem;
+        const int a = I * 16 + li, b = J * 16 + li;
+        const bool ain = a < N, bin = b < N;
+        auto fetch = [&](int x0, cd &pa, cd &pb, double &vx) {
+            const int x = x0 + lq;
+            const bool xin = x < ns;
+            pa = (ain && xin) ? ld(Pt, (long)a * ns + x) : cd{0.0, 0.0};
+            pb = (bin && xin) ? ld(Pt, (long)b * ns + x) : cd{0.0, 0.0};
+            vx = xin ? Vs[x] : 0.0;
+        };
+        v4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+        cd pa_n, pb_n;
+        double v_n;
+        fetch(0, pa_n, pb_n, v_n);
+        for (int x0 = 0; x0 < ns; x0 += 4) {
+            const cd pa = pa_n, pb = pb_n;
+            const double vx = v_n;
+            if (x0 + 4 < ns) fetch(x0 + 4, pa_n, pb_n, v_n);
+            const double ar = pa.x * vx, ai = -pa.y * vx;
+            p1 = M::mma(ar + ai, pb.x, p1);
+            p2 = M::mma(ar, pb.y - pb.x, p2);
+            p3 = M::mma(ai, pb.x + pb.y, p3);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = I * 16 + M::row(lane, v);
+            if (r < N && bin) st(Hk, (long)b * N + r, cd{p1[v] - p3[v], p1[v] + p2[v]});
+        }
+    }
+}
+
 // hlm links: L[kp][j] = u_j(k)^H M(k,k') u_j(k'). Component-major eigenvectors
-// U[kp][i*ns + j]. M column-major N x N per link. One workgroup per kp (threads = states).
+// U[kp][i*ns + j]; M column-major N x N per link, M(i, l) at l * ns + i. One workgroup
+// of 64 * ceil(ns / 16) threads per kp: wave J owns states 16 J .., forms Y = M U(k')
+// block by block on the FP64 matrix cores (three real MFMAs per complex product, as
+// above) and accumulates conj(U(k)) . Y down each column.
 // Twist points kp = (s*(g+1) + t1)*g + t2 (theta_2 = g is theta_2 = 0).
 // Link matrices: M1[t1*g + t2] (t1 < g), M2[t1*g + t2].
-__global__ __launch_bounds__(128) void hlm_links_kernel(const double *__restrict__ U, int ns, int n_mesh,
-                                                        const double *__restrict__ M1, const double *__restrict__ M2,
-                                                        double *__restrict__ Lk1, double *__restrict__ Lk2) {
+__global__ __launch_bounds__(1024) void hlm_links_kernel(const double *__restrict__ U, int ns, int n_mesh,
+                                                         const double *__restrict__ M1, const double *__restrict__ M2,
+                                                         double *__restrict__ Lk1, double *__restrict__ Lk2) {
+    using M = Mf<double>;
+    using v4 = M::v4;
     const long kp = blockIdx.x;
     const int g = n_mesh;
     const int t2 = (int)(kp % g);
@@ -161,24 +227,47 @@ __global__ __launch_bounds__(128) void hlm_links_kernel(const double *__restrict
     const long nn = (long)ns * ns;
     const double *u0 = U + 2 * kp * nn;
     const bool has1 = t1 < g;
-    const double *u1 = U + 2 * (kp + g) * nn;
-    const double *u2 = U + 2 * (base_line + (t2 + 1) % g) * nn;
-    const double *m1 = M1 + 2 * ((long)t1 * g + t2) * nn;
-    const double *m2 = M2 + 2 * ((long)t1 * g + t2) * nn;
-    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
-        cd a1{0.0, 0.0}, a2{0.0, 0.0};
-        for (int i = 0; i < ns; i++) {
-            cd y1{0.0, 0.0}, y2{0.0, 0.0};
-            for (int l = 0; l < ns; l++) {
-                if (has1) cfma(y1, ld(m1, (long)l * ns + i), ld(u1, (long)l * ns + j));
-                cfma(y2, ld(m2, (long)l * ns + i), ld(u2, (long)l * ns + j));
+    const int lane = threadIdx.x & (kWave - 1);
+    const int J = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const int nbk = (ns + 15) / 16;
+    const int j = J * 16 + li;
+    const bool jin = j < ns;
+#pragma unroll 1
+    for (int d = 0; d < 2; d++) {
+        cd acc{0.0, 0.0};
+        if (d == 1 || has1) {
+            const double *uq = d == 0 ? U + 2 * (kp + g) * nn : U + 2 * (base_line + (t2 + 1) % g) * nn;
+            const double *m = (d == 0 ? M1 : M2) + 2 * ((long)t1 * g + t2) * nn;
+            for (int I = 0; I < nbk; I++) {
+                const int i = I * 16 + li;
+                const bool iin = i < ns;
+                v4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+                for (int l0 = 0; l0 < ns; l0 += 4) {
+                    const int l = l0 + lq;
+                    const cd a = (iin && l < ns) ? ld(m, (long)l * ns + i) : cd{0.0, 0.0};    // M(i, l)
+                    const cd b = (jin && l < ns) ? ld(uq, (long)l * ns + j) : cd{0.0, 0.0};   // U'(l, j)
+                    p1 = M::mma(a.x + a.y, b.x, p1);
+                    p2 = M::mma(a.x, b.y - b.x, p2);
+                    p3 = M::mma(a.y, b.x + b.y, p3);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int r = I * 16 + M::row(lane, v);
+                    if (r < ns && jin) {
+                        const cd x = ld(u0, (long)r * ns + j);
+                        const cd y{p1[v] - p3[v], p1[v] + p2[v]};
+                        cfmac(acc, x, y);
+                    }
+                }
             }
-            const cd x = ld(u0, (long)i * ns + j);
-            if (has1) cfmac(a1, x, y1);
-            cfmac(a2, x, y2);
         }
-        st(Lk1, kp * ns + j, a1);
-        st(Lk2, kp * ns + j, a2);
+        // sum over the four lanes holding state j (lane bits 4-5)
+        for (int off = 16; off < kWave; off <<= 1) {
+            acc.x += __shfl_xor(acc.x, off, kWave);
+            acc.y += __shfl_xor(acc.y, off, kWave);
+        }
+        if (jin && lq == 0) st(d == 0 ? Lk1 : Lk2, kp * ns + j, acc);
     }
 }
 
@@ -210,7 +299,16 @@ int launch_zgemm(int M, int N, int K, const double *A, long rsA, long csA, long 
 int launch_hlm_links(const double *U, int ns, int n_mesh, long nkp, const double *M1, const double *M2, double *Lk1,
                      double *Lk2, hipStream_t s) {
     if (nkp <= 0) return 0;
-    hipLaunchKernelGGL(hlm_links_kernel, dim3((unsigned)nkp), dim3(128), 0, s, U, ns, n_mesh, M1, M2, Lk1, Lk2);
+    if (ns < 1 || ns > 256) return -22;                  // one wave per 16 states, at most 16 waves
+    const int nbk = (ns + 15) / 16;
+    hipLaunchKernelGGL(hlm_links_kernel, dim3((unsigned)nkp), dim3(64 * nbk), 0, s, U, ns, n_mesh, M1, M2, Lk1, Lk2);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_hlm_project(const double *Phi, const double *V, int N, int ns, int S, long ntw, double *H, hipStream_t s) {
+    if (S <= 0 || ntw <= 0) return 0;
+    hipLaunchKernelGGL(hlm_project_kernel, dim3((unsigned)(ntw * S)), dim3(256), 0, s, Phi, V, N, ns, S, ntw, H);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

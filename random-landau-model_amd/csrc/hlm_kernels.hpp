@@ -11,4 +11,5 @@ int launch_zgemm(int M, int N, int K, const double *A, long rsA, long csA, long 
                  double *C, long rsC, long csC, long sC1, long sC2, int nb1, int nb2, hipStream_t s);
 int launch_hlm_links(const double *U, int ns, int n_mesh, long nkp, const double *M1, const double *M2, double *Lk1,
                      double *Lk2, hipStream_t s);
+int launch_hlm_project(const double *Phi, const double *V, int N, int ns, int S, long ntw, double *H, hipStream_t s);
 }  // namespace rlm

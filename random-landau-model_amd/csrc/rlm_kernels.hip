@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
         jj[b] = (sb0 + b) * 16 + li;
         jin[b] = jj[b] < n;
     }
-    const int kw = 16 + bw, nks = (kw + 3) / 4, nrb = (n + 15) / 16;
+    const int kw = 16 + bw, nks = (kw + 3) / 4;
     // operands of K-step k0 of row block r0: this lane's A element, loaded raw with a
     // mode applied when it is used (0: outside the half band, 1: as stored, 2: conjugate,
     // 3: half the real part, the diagonal), so that no load is waited for before the
