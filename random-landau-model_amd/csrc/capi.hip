@@ -161,8 +161,9 @@ constexpr int kSmallMaxN = 512;
 // divide & conquer workgroups (each with its own n*n workspaces) for n > kSmallMaxN
 constexpr int kGridTailLarge = 64;
 
+// n <= 128: the divide & conquer runs two-wave workgroups, twice as many fit the CUs
 static int tail_grid(int n, long batch) {
-    const long g = n > kSmallMaxN ? kGridTailLarge : kGridTail;
+    const long g = n > kSmallMaxN ? kGridTailLarge : n <= 128 ? 2 * kGridTail : kGridTail;
     return (int)(batch < g ? batch : g);
 }
 
@@ -215,7 +216,7 @@ int eig_packed(double *A, int n, long batch, double *W, double *Uout, EigWork &w
     const int gt = (int)(batch < kGridTrd ? batch : kGridTrd);
     if ((rc = launch_zhetrd(A, (long)n * n, n, n, D, E, TAU, strideV, (int)batch, gt, s))) return rc;
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
-    const int gl = (int)(batch < kGridTail ? batch : kGridTail);
+    const int gl = tail_grid(n, batch);
     if ((size_t)gl * 3 * n * n * sizeof(double) > ws.work.bytes) return -22;
     if ((size_t)batch * backtransform_tws_complex(n) * 2 * sizeof(double) > ws.tw.bytes) return -22;
     if ((rc = launch_tail(A, (long)n * n, n, n, D, E, TAU, strideV, W, n, Uout, (long)n * n, ws.work.d(), ws.tw.d(),
@@ -254,7 +255,7 @@ int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork 
         return rc;
     }
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, s));
-    const int gl = (int)(batch < kGridTail ? batch : kGridTail);
+    const int gl = tail_grid(n, batch);
     if ((size_t)gl * 3 * n * n * sizeof(double) > ws.work.bytes) return -22;
     if ((size_t)batch * backtransform_tws_complex(n) * 2 * sizeof(float) > ws.tw.bytes) return -22;
     if ((rc = launch_tail_f32(A, (long)n * n, n, n, D, E, TAU, strideV, W, n, Uout, (long)n * n, ws.work.d(),

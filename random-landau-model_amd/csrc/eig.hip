@@ -918,7 +918,7 @@ struct ResidentDeal {
 // tiles only (its code path is fixed at compile time), slots 1.. off-diagonal ones.
 // Off-diagonal tiles of column 0 that find no register slot go to LDS (at most NL,
 // one per wave). Returns false if the tiles do not fit.
-static bool resident_deal(int TN, int NS, bool diag0, int NL, ResidentDeal &d) {
+static bool resident_deal(int TN, int NS, bool diag0, int NL, int nw, ResidentDeal &d) {
     memset(&d, -1, sizeof(d));
     int load[8] = {0, 0, 0, 0, 0, 0, 0, 0}, used[8] = {0, 0, 0, 0, 0, 0, 0, 0}, nlds = 0;
     bool slot0[8] = {false, false, false, false, false, false, false, false};
@@ -933,7 +933,7 @@ static bool resident_deal(int TN, int NS, bool diag0, int NL, ResidentDeal &d) {
         for (int r = c; r < TN; r++) {
             const bool diag = r == c;
             int best = -1;
-            for (int w = 0; w < 8; w++) {
+            for (int w = 0; w < nw; w++) {
                 const bool fits = diag0 ? (diag ? !slot0[w] : used[w] < NS - 1) : used[w] < NS;
                 if (fits && better(w, best)) best = w;
             }
@@ -947,7 +947,7 @@ static bool resident_deal(int TN, int NS, bool diag0, int NL, ResidentDeal &d) {
                 continue;
             }
             if (diag || c != 0 || nlds >= NL) return false;
-            for (int w = 0; w < 8; w++)
+            for (int w = 0; w < nw; w++)
                 if (d.lr[w] < 0 && better(w, best)) best = w;
             if (best < 0) return false;
             d.lr[best] = (signed char)r;
@@ -960,17 +960,17 @@ static bool resident_deal(int TN, int NS, bool diag0, int NL, ResidentDeal &d) {
 
 // Hb (optional): the matrices in the band form of offset hb_off (complex double,
 // n x band_nb(n, hb_off) per matrix) instead of A; A then only receives the reflectors.
-template <typename T, int NS, int NL, bool DIAG0>
+template <typename T, int NS, int NL, bool DIAG0, int NW, int NMAX>
 __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
                                                              double *__restrict__ Dout, double *__restrict__ Eout,
                                                              T *__restrict__ TAUout, long strideV, int batch,
                                                              const double *__restrict__ Hb, int hb_off,
                                                              const ResidentDeal deal) {
     using CT = cx<T>;
-    constexpr int NT = 512, NW = NT / kWave, TS = 32, NMAX = 256, NQ = NMAX / kWave;
+    constexpr int NT = NW * kWave, TS = 32, NQ = NMAX / kWave;
     constexpr int NLS = NL > 0 ? NL : 1;
     __shared__ CT vbuf[2][NMAX], wbuf[2][NMAX], colb[NMAX], yp[NW][NMAX];
-    __shared__ CT ltile[NLS][16 * kWave];
+    __shared__ CT ltile[NLS][NL > 0 ? 16 * kWave : 1];
     __shared__ T sd[NMAX], se[NMAX];      // d, e and tau, written to memory at the end
     __shared__ CT stau[NMAX];
     const int t = threadIdx.x, lane = t & (kWave - 1);
@@ -984,7 +984,7 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
         tC[s] = deal.c[wid][s];
     }
     const int lR = NL > 0 ? deal.lr[wid] : -1, lq = lR >= 0 ? deal.lq[wid] : 0;
-    CT *lsub = &ltile[lq][lane];
+    CT *lsub = NL > 0 ? &ltile[lq][lane] : nullptr;
 
     {
         const long b = blockIdx.x;            // one workgroup per matrix (grid = batch)
@@ -2294,9 +2294,18 @@ int launch_zhetrd_t(T *A, long strideA, int lda, int n, double *D, double *E, T 
     constexpr int NT = 256;
     const size_t cs = 2 * sizeof(T);
     size_t lds = (size_t)(4 * n + (NT / kWave) * n) * cs + (2 * (NT / kWave) + 8) * sizeof(T);
-    if (grid > batch) grid = batch;
     if (lds > 64 * 1024) RLM_HIP_CHECK(hipFuncSetAttribute((const void *)zhetrd_kernel<T, NT, 2>,
                                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // persistent workgroups: as many as the CUs hold at once (small n: several per CU)
+    {
+        int dev = 0, cus = 0, per_cu = 0;
+        RLM_HIP_CHECK(hipGetDevice(&dev));
+        RLM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        RLM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)zhetrd_kernel<T, NT, 2>, NT,
+                                                                   lds));
+        if (per_cu > 0 && cus > 0 && per_cu * cus > grid) grid = per_cu * cus;
+    }
+    if (grid > batch) grid = batch;
     hipLaunchKernelGGL((zhetrd_kernel<T, NT, 2>), dim3(grid), dim3(NT), lds, stream, A, strideA, lda, n, D, E, TAU,
                        strideV, batch);
     RLM_HIP_CHECK(hipGetLastError());
@@ -2355,15 +2364,21 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
                   long strideV, double *W, long strideW, T *U, long strideU, double *work, T *tws, int batch, int grid,
                   hipStream_t stream, hipEvent_t mid) {
     if (n < 1 || n > 512) return -22;
-    constexpr int NT = 256;
     const size_t lds_dc = heevd_dc_lds_bytes(n);
     if (lds_dc > 160 * 1024) return -22;
     if (grid > batch) grid = batch;
-    if (lds_dc > 64 * 1024)
-        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)stedc_kernel<T, NT, false>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_dc));
-    hipLaunchKernelGGL((stedc_kernel<T, NT, false>), dim3(grid), dim3(NT), lds_dc, stream, n, D, E, strideV, W,
-                       strideW, U, strideU, work, batch);
+    if (n <= 128) {
+        // two waves per matrix: more matrices per CU where the nodes are small
+        hipLaunchKernelGGL((stedc_kernel<T, 128, false>), dim3(grid), dim3(128), lds_dc, stream, n, D, E, strideV, W,
+                           strideW, U, strideU, work, batch);
+    } else {
+        constexpr int NT = 256;
+        if (lds_dc > 64 * 1024)
+            RLM_HIP_CHECK(hipFuncSetAttribute((const void *)stedc_kernel<T, NT, false>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_dc));
+        hipLaunchKernelGGL((stedc_kernel<T, NT, false>), dim3(grid), dim3(NT), lds_dc, stream, n, D, E, strideV, W,
+                           strideW, U, strideU, work, batch);
+    }
     RLM_HIP_CHECK(hipGetLastError());
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, stream));
     if (n == 1) return 0;
@@ -2393,7 +2408,7 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
         return launch_backtransform_stream(A, strideA, n, tws, strideT, U, strideU, batch, stream);
     }
     // reflectors per compact-WY block: the V block (n x NBW complex) lives in LDS
-    constexpr int NBW = sizeof(T) == 4 ? 32 : 16;
+    constexpr int NT = 256, NBW = sizeof(T) == 4 ? 32 : 16;
     const size_t lds_bt = backtransform_lds_bytes(n, NBW, sizeof(T));
     if (lds_bt > 160 * 1024) return -22;
     if (lds_bt > 64 * 1024)
@@ -2405,14 +2420,14 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
     return 0;
 }
 
-template <typename T, int NS, int NL, bool DIAG0>
+template <typename T, int NS, int NL, bool DIAG0, int NW = 8, int NMAX = 256>
 static int launch_zhetrd_resident_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
                                     int batch, hipStream_t stream, const double *Hb, int hb_off) {
     const int tn = (n + 31) / 32;
     ResidentDeal deal;
-    if (n < 1 || n > 256 || !resident_deal(tn, NS, DIAG0, NL, deal)) return -22;
-    hipLaunchKernelGGL((zhetrd_resident_kernel<T, NS, NL, DIAG0>), dim3(batch), dim3(512), 0, stream, A, strideA, lda,
-                       n, D, E, TAU, strideV, batch, Hb, hb_off, deal);
+    if (n < 1 || n > NMAX || !resident_deal(tn, NS, DIAG0, NL, NW, deal)) return -22;
+    hipLaunchKernelGGL((zhetrd_resident_kernel<T, NS, NL, DIAG0, NW, NMAX>), dim3(batch), dim3(64 * NW), 0, stream, A,
+                       strideA, lda, n, D, E, TAU, strideV, batch, Hb, hb_off, deal);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -2451,6 +2466,12 @@ static int launch_resident_any(T *A, long strideA, int lda, int n, double *D, do
                                int batch, hipStream_t stream, const double *Hb, int hb_off) {
     const int tn = (n + 31) / 32;
     if constexpr (sizeof(T) == sizeof(float)) {
+        // n <= 96 (at most 6 tiles): 4 waves with two register tiles each, so that two
+        // matrices share a CU and one's single-wave reflector phase overlaps the other's
+        // work (complex double would spill at two tiles per wave)
+        if (tn <= 3)
+            return launch_zhetrd_resident_t<T, 2, 0, false, 4, 128>(A, strideA, lda, n, D, E, TAU, strideV, batch,
+                                                                    stream, Hb, hb_off);
         if (tn <= 4)
             return launch_zhetrd_resident_t<T, 1, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
                                                             hb_off);
@@ -2470,7 +2491,11 @@ static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, doubl
                             int grid, hipStream_t stream) {
     int v = hetrd_variant();
     const int rmax = resident_max_n(sizeof(T));
-    if (v == kTrdAuto) v = n <= rmax ? kTrdResident : kTrdLower;
+    // auto: complex double n <= 96 takes the fused kernel (its small LDS footprint lets
+    // many matrices share a CU, where the resident kernel holds one per CU: hlm n = 72,
+    // 67584 matrices: 52.7 against 111 ms; n = 64: 6.5 against 9.8 ms per C4 step; at
+    // n = 128 the resident kernel leads, 23.8 against 26.5 ms)
+    if (v == kTrdAuto) v = (sizeof(T) == sizeof(double) && n <= 96) ? kTrdFused : n <= rmax ? kTrdResident : kTrdLower;
     if ((v == kTrdResident || v == kTrdHybrid) && n <= rmax)
         return launch_resident_any<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, nullptr, 0);
     if (v == kTrdHybrid && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024) {
