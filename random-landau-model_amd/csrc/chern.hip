@@ -137,32 +137,44 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ 
     const double TWO_PI = 6.283185307179586232;   // (M_PI*2.0)
     const double PI = 3.141592653589793116;
     for (int j = threadIdx.x; j < ns; j += blockDim.x) {
-        // energies (reference order: line by line, theta_2 ascending)
+        // energies (reference order: line by line, theta_2 ascending; the loads of a line
+        // are issued before its additions, which keep that order)
         double e = 0.0;
         for (int t1 = 0; t1 <= g; t1++) {
             const long lb = kbase + (long)t1 * L2n;
             if (mode == 0) {
+#pragma unroll 8
                 for (int t2 = 0; t2 <= g; t2++) e += Wv[(lb + t2) * ns + j];
             } else if (mode == 2) {
                 const long sb = ((long)s * g + (t1 % g)) * g;
+#pragma unroll 8
                 for (int t2 = 0; t2 <= g; t2++) e += Wv[(sb + t2 % g) * ns + j];
             } else {
+#pragma unroll 8
                 for (int t2 = 0; t2 < g; t2++) e += Wv[(lb + t2) * ns + j];
                 e += Wv[lb * ns + j];
             }
         }
         e /= (double)(g1 * g1);
-        // plaquettes (a = theta_1 - 1, b = m)
+        // plaquettes (a = theta_1 - 1, b = m) in the reference's order; the four links of
+        // the next plaquette load while the current one's phase is taken
+        auto links = [&](int a, int b, cd &l1, cd &l2, cd &l3, cd &l4) {
+            const long k00 = kbase + (long)a * L2n + (b % L2n);
+            const long k10 = k00 + L2n;
+            const long k01 = kbase + (long)a * L2n + ((b + 1) % L2n);
+            l1 = ld(Lk1, k00 * ns + j);     // <00|10>
+            l2 = ld(Lk2, k10 * ns + j);     // <10|11>
+            l3 = ld(Lk1, k01 * ns + j);     // conj: <11|01>
+            l4 = ld(Lk2, k00 * ns + j);     // conj: <01|00>
+        };
         double c = 0.0, mg = 4.0;
+        cd n1, n2, n3, n4;
+        if (g > 0) links(0, 0, n1, n2, n3, n4);
         for (int a = 0; a < g; a++) {
             for (int b = 0; b < g; b++) {
-                const long k00 = kbase + (long)a * L2n + (b % L2n);
-                const long k10 = k00 + L2n;
-                const long k01 = kbase + (long)a * L2n + ((b + 1) % L2n);
-                const cd l1 = ld(Lk1, k00 * ns + j);     // <00|10>
-                const cd l2 = ld(Lk2, k10 * ns + j);     // <10|11>
-                const cd l3 = cconj(ld(Lk1, k01 * ns + j));  // <11|01>
-                const cd l4 = cconj(ld(Lk2, k00 * ns + j));  // <01|00>
+                const cd l1 = n1, l2 = n2, l3 = cconj(n3), l4 = cconj(n4);
+                if (b + 1 < g) links(a, b + 1, n1, n2, n3, n4);
+                else if (a + 1 < g) links(a + 1, 0, n1, n2, n3, n4);
                 cd prod{1.0, 0.0};
                 prod = cmul(prod, l1);
                 prod = cmul(prod, l2);
