@@ -209,7 +209,8 @@ __global__ __launch_bounds__(256) void hlm_project_kernel(const double *__restri
 
 // hlm links: L[kp][j] = u_j(k)^H M(k,k') u_j(k'). Component-major eigenvectors
 // U[kp][i*ns + j]; M column-major N x N per link, M(i, l) at l * ns + i. One workgroup
-// of 64 * ceil(ns / 16) threads per kp: wave J owns states 16 J .., forms Y = M U(k')
+// of 64 * min(ceil(ns / 16), 16) threads per kp: wave J owns states 16 J .. (and every
+// further 16 waves' worth of blocks when ns > 256), forms Y = M U(k')
 // block by block on the FP64 matrix cores (three real MFMAs per complex product, as
 // above) and accumulates conj(U(k)) . Y down each column.
 // Twist points kp = (s*(g+1) + t1)*g + t2 (theta_2 = g is theta_2 = 0).
@@ -228,46 +229,48 @@ __global__ __launch_bounds__(1024) void hlm_links_kernel(const double *__restric
     const double *u0 = U + 2 * kp * nn;
     const bool has1 = t1 < g;
     const int lane = threadIdx.x & (kWave - 1);
-    const int J = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), nwv = blockDim.x / kWave;
     const int li = lane & 15, lq = lane >> 4;
     const int nbk = (ns + 15) / 16;
-    const int j = J * 16 + li;
-    const bool jin = j < ns;
+    for (int J = wv; J < nbk; J += nwv) {
+        const int j = J * 16 + li;
+        const bool jin = j < ns;
 #pragma unroll 1
-    for (int d = 0; d < 2; d++) {
-        cd acc{0.0, 0.0};
-        if (d == 1 || has1) {
-            const double *uq = d == 0 ? U + 2 * (kp + g) * nn : U + 2 * (base_line + (t2 + 1) % g) * nn;
-            const double *m = (d == 0 ? M1 : M2) + 2 * ((long)t1 * g + t2) * nn;
-            for (int I = 0; I < nbk; I++) {
-                const int i = I * 16 + li;
-                const bool iin = i < ns;
-                v4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
-                for (int l0 = 0; l0 < ns; l0 += 4) {
-                    const int l = l0 + lq;
-                    const cd a = (iin && l < ns) ? ld(m, (long)l * ns + i) : cd{0.0, 0.0};    // M(i, l)
-                    const cd b = (jin && l < ns) ? ld(uq, (long)l * ns + j) : cd{0.0, 0.0};   // U'(l, j)
-                    p1 = M::mma(a.x + a.y, b.x, p1);
-                    p2 = M::mma(a.x, b.y - b.x, p2);
-                    p3 = M::mma(a.y, b.x + b.y, p3);
-                }
+        for (int d = 0; d < 2; d++) {
+            cd acc{0.0, 0.0};
+            if (d == 1 || has1) {
+                const double *uq = d == 0 ? U + 2 * (kp + g) * nn : U + 2 * (base_line + (t2 + 1) % g) * nn;
+                const double *m = (d == 0 ? M1 : M2) + 2 * ((long)t1 * g + t2) * nn;
+                for (int I = 0; I < nbk; I++) {
+                    const int i = I * 16 + li;
+                    const bool iin = i < ns;
+                    v4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+                    for (int l0 = 0; l0 < ns; l0 += 4) {
+                        const int l = l0 + lq;
+                        const cd a = (iin && l < ns) ? ld(m, (long)l * ns + i) : cd{0.0, 0.0};    // M(i, l)
+                        const cd b = (jin && l < ns) ? ld(uq, (long)l * ns + j) : cd{0.0, 0.0};   // U'(l, j)
+                        p1 = M::mma(a.x + a.y, b.x, p1);
+                        p2 = M::mma(a.x, b.y - b.x, p2);
+                        p3 = M::mma(a.y, b.x + b.y, p3);
+                    }
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int r = I * 16 + M::row(lane, v);
-                    if (r < ns && jin) {
-                        const cd x = ld(u0, (long)r * ns + j);
-                        const cd y{p1[v] - p3[v], p1[v] + p2[v]};
-                        cfmac(acc, x, y);
+                    for (int v = 0; v < 4; v++) {
+                        const int r = I * 16 + M::row(lane, v);
+                        if (r < ns && jin) {
+                            const cd x = ld(u0, (long)r * ns + j);
+                            const cd y{p1[v] - p3[v], p1[v] + p2[v]};
+                            cfmac(acc, x, y);
+                        }
                     }
                 }
             }
+            // sum over the four lanes holding state j (lane bits 4-5)
+            for (int off = 16; off < kWave; off <<= 1) {
+                acc.x += __shfl_xor(acc.x, off, kWave);
+                acc.y += __shfl_xor(acc.y, off, kWave);
+            }
+            if (jin && lq == 0) st(d == 0 ? Lk1 : Lk2, kp * ns + j, acc);
         }
-        // sum over the four lanes holding state j (lane bits 4-5)
-        for (int off = 16; off < kWave; off <<= 1) {
-            acc.x += __shfl_xor(acc.x, off, kWave);
-            acc.y += __shfl_xor(acc.y, off, kWave);
-        }
-        if (jin && lq == 0) st(d == 0 ? Lk1 : Lk2, kp * ns + j, acc);
     }
 }
 
@@ -299,9 +302,9 @@ int launch_zgemm(int M, int N, int K, const double *A, long rsA, long csA, long 
 int launch_hlm_links(const double *U, int ns, int n_mesh, long nkp, const double *M1, const double *M2, double *Lk1,
                      double *Lk2, hipStream_t s) {
     if (nkp <= 0) return 0;
-    if (ns < 1 || ns > 256) return -22;                  // one wave per 16 states, at most 16 waves
-    const int nbk = (ns + 15) / 16;
-    hipLaunchKernelGGL(hlm_links_kernel, dim3((unsigned)nkp), dim3(64 * nbk), 0, s, U, ns, n_mesh, M1, M2, Lk1, Lk2);
+    if (ns < 1) return -22;
+    const int nbk = (ns + 15) / 16, nw = nbk < 16 ? nbk : 16;   // one wave per 16 states, at most 16 waves
+    hipLaunchKernelGGL(hlm_links_kernel, dim3((unsigned)nkp), dim3(64 * nw), 0, s, U, ns, n_mesh, M1, M2, Lk1, Lk2);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

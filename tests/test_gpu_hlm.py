@@ -39,3 +39,24 @@ def test_config_fixture(fixtures):
     for k, row in enumerate(fx["samples"]):
         _cmp(e[k], c[k], row["energy"], row["chern"], ("hlm fixture", k))
     plan.close()
+
+
+def test_more_than_256_states_vs_live_oracle(oracle):
+    """24x24 lattice at Q = 2 (288 projected states: the link kernel's waves each take more
+    than one block of 16 states), g = 2, one sample: energies within 1e-10 relative, every
+    per-state Chern value within 1e-4 of the oracle's and the same integers. On this 2 x 2
+    twist mesh the values sit near 0 (the per-sample sum need not be 1) and some link
+    overlaps are tiny, so their phases carry the rounding of the overlap sums: the
+    matrix-core sums and the oracle's sequential ones differ by 3.8e-6 in one state's value
+    here, where a wrong link block would be off by O(1)."""
+    from rlm_amd import HlmPlan
+    print("[hlm 288 states] 24x24 Q=2 g=2", flush=True)
+    plan = HlmPlan(24, 24, 2, 0, 2, 5, 2.0, samples_per_batch=1)
+    e, c, m = plan.run(0, 1)
+    plan.close()
+    pot = oracle.hlm_disorder(5, 24 * 24, 1)[0]
+    r = oracle.hlm_sample(24, 24, 2, 0, 2, pot)
+    ref_e = np.asarray(r.energy)
+    assert np.abs(e[0] - ref_e).max() <= 1e-10 * np.abs(ref_e).max()
+    assert np.abs(c[0] - np.asarray(r.chern)).max() < 1e-4
+    assert np.array_equal(np.rint(c[0]), np.rint(np.asarray(r.chern)))
