@@ -149,12 +149,12 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, const d
 // Projected Hamiltonians Htr[s*ntw + tw] = Phi(tw)^H diag(V_s) Phi(tw) (column-major
 // N x N, upper triangle: the rest is set from it by hermitize_kernel, as the
 // reference's zheevd reads the upper triangle) on the FP64 matrix cores: one workgroup
-// per (tw, s), blockIdx.x = tw * S + s; the 16 x 16 output blocks (I <= J) dealt to the
-// 4 waves; K = the n_sites sites in steps of 4. Phi(tw) is stored state-major,
-// Phi(x, a) at a * ns + x. Each complex product takes three real MFMAs (a = conj(Phi(x,
-// a)) V_s(x), b = Phi(x, b): P1 = (a_r + a_i) b_r, P2 = a_r (b_i - b_r), P3 = a_i (b_r +
-// b_i), Re = P1 - P3, Im = P1 + P2), the operands of the next K-step loading while the
-// current one runs.
+// per twist and pair of samples (blockIdx.x = tw * ceil(S / 2) + pair), which share every
+// loaded Phi value; the 16 x 16 output blocks (I <= J) dealt to the 4 waves; K = the
+// n_sites sites in steps of 4. Phi(tw) is stored state-major, Phi(x, a) at a * ns + x.
+// Each complex product takes three real MFMAs (a = conj(Phi(x, a)) V_s(x), b = Phi(x, b):
+// P1 = (a_r + a_i) b_r, P2 = a_r (b_i - b_r), P3 = a_i (b_r + b_i), Re = P1 - P3,
+// Im = P1 + P2), the operands of the next K-step loading while the current one runs.
 __global__ __launch_bounds__(256) void hlm_project_kernel(const double *__restrict__ Phi, const double *__restrict__ V,
                                                           int N, int ns, int S, long ntw, double *__restrict__ H) {
     using M = Mf<double>;
@@ -162,12 +162,15 @@ __global__ __launch_bounds__(256) void hlm_project_kernel(const double *__restri
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int li = lane & 15, lq = lane >> 4;
-    const long tw = blockIdx.x / S;
-    const int smp = (int)(blockIdx.x % S);
+    const int npair = (S + 1) / 2;
+    const long tw = blockIdx.x / npair;
+    const int s0 = 2 * (int)(blockIdx.x % npair);
+    const bool two = s0 + 1 < S;
     const long nn = (long)N * N;
     const double *Pt = Phi + 2 * tw * (long)ns * N;
-    const double *Vs = V + (long)smp * ns;
-    double *Hk = H + 2 * ((long)smp * ntw + tw) * nn;
+    const double *V0 = V + (long)s0 * ns, *V1 = V + (long)(two ? s0 + 1 : s0) * ns;
+    double *H0 = H + 2 * ((long)s0 * ntw + tw) * nn;
+    double *H1 = H + 2 * ((long)(s0 + 1) * ntw + tw) * nn;
     const int nbk = (N + 15) / 16, nblk = nbk * (nbk + 1) / 2;
     for (int t = w; t < nblk; t += 4) {
         // block t -> (I, J), I <= J, row-major over the upper block triangle
@@ -179,30 +182,42 @@ __global__ __launch_bounds__(256) void hlm_project_kernel(const double *__restri
         const int J = I + rem;
         const int a = I * 16 + li, b = J * 16 + li;
         const bool ain = a < N, bin = b < N;
-        auto fetch = [&](int x0, cd &pa, cd &pb, double &vx) {
+        auto fetch = [&](int x0, cd &pa, cd &pb, double &v0, double &v1) {
             const int x = x0 + lq;
             const bool xin = x < ns;
             pa = (ain && xin) ? ld(Pt, (long)a * ns + x) : cd{0.0, 0.0};
             pb = (bin && xin) ? ld(Pt, (long)b * ns + x) : cd{0.0, 0.0};
-            vx = xin ? Vs[x] : 0.0;
+            v0 = xin ? V0[x] : 0.0;
+            v1 = xin ? V1[x] : 0.0;
         };
-        v4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+        v4 p1[2], p2[2], p3[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) p1[q] = p2[q] = p3[q] = v4{0.0, 0.0, 0.0, 0.0};
         cd pa_n, pb_n;
-        double v_n;
-        fetch(0, pa_n, pb_n, v_n);
+        double v0_n, v1_n;
+        fetch(0, pa_n, pb_n, v0_n, v1_n);
         for (int x0 = 0; x0 < ns; x0 += 4) {
             const cd pa = pa_n, pb = pb_n;
-            const double vx = v_n;
-            if (x0 + 4 < ns) fetch(x0 + 4, pa_n, pb_n, v_n);
-            const double ar = pa.x * vx, ai = -pa.y * vx;
-            p1 = M::mma(ar + ai, pb.x, p1);
-            p2 = M::mma(ar, pb.y - pb.x, p2);
-            p3 = M::mma(ai, pb.x + pb.y, p3);
+            const double vx[2] = {v0_n, v1_n};
+            if (x0 + 4 < ns) fetch(x0 + 4, pa_n, pb_n, v0_n, v1_n);
+            const double bd = pb.y - pb.x, bs = pb.x + pb.y;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const double ar = pa.x * vx[q], ai = -pa.y * vx[q];
+                p1[q] = M::mma(ar + ai, pb.x, p1[q]);
+                p2[q] = M::mma(ar, bd, p2[q]);
+                p3[q] = M::mma(ai, bs, p3[q]);
+            }
         }
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int r = I * 16 + M::row(lane, v);
-            if (r < N && bin) st(Hk, (long)b * N + r, cd{p1[v] - p3[v], p1[v] + p2[v]});
+        for (int q = 0; q < 2; q++) {
+            if (q == 1 && !two) break;
+            double *Hk = q == 0 ? H0 : H1;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = I * 16 + M::row(lane, v);
+                if (r < N && bin) st(Hk, (long)b * N + r, cd{p1[q][v] - p3[q][v], p1[q][v] + p2[q][v]});
+            }
         }
     }
 }
@@ -311,7 +326,8 @@ int launch_hlm_links(const double *U, int ns, int n_mesh, long nkp, const double
 
 int launch_hlm_project(const double *Phi, const double *V, int N, int ns, int S, long ntw, double *H, hipStream_t s) {
     if (S <= 0 || ntw <= 0) return 0;
-    hipLaunchKernelGGL(hlm_project_kernel, dim3((unsigned)(ntw * S)), dim3(256), 0, s, Phi, V, N, ns, S, ntw, H);
+    hipLaunchKernelGGL(hlm_project_kernel, dim3((unsigned)(ntw * ((S + 1) / 2))), dim3(256), 0, s, Phi, V, N, ns, S,
+                       ntw, H);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
