@@ -195,7 +195,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; bench.py --gpus N with N > 1 is not "
-                                   "supported (samples shard across GPUs by seed skip-ahead, see DESIGN.md)"}))
+                                   "supported (samples would shard across GPUs by seed skip-ahead)"}))
         sys.exit(2)
 
     import numpy as np

@@ -1,6 +1,6 @@
 """GPU parity of the batched rlm pipeline (librlm_hip.so) against the CPU oracle.
 
-Bar (BASELINE.json north_star): every per-state Chern number rounded to an integer
+Bar: every per-state Chern number rounded to an integer
 identical to the oracle's, every per-sample sum = 1, mesh-averaged energies within
 1e-10 relative. Intermediate stages (V_mn, assembled H) are compared elementwise
 with rtol 1e-12 (FP64 sums in a different order).
