@@ -193,7 +193,8 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
 // in 32 x 32 tiles (I, J), I >= J, of the trailing block. A workgroup takes the strips
 // p and ns - 1 - p (balanced work), its waves the tiles J = w, w + 4, ... of each strip;
 // lane (lrg, lcg) = (lane & 7, lane >> 3) holds the 4 x 4 sub-block at tile rows
-// 4 lrg.., columns 4 lcg.., read once for both y_I += A_IJ v_J (row sums, reduced over
+// lrg + 8 ru (so that the 8 lanes of a column group read 8 consecutive rows of the
+// column-major matrix per load), columns 4 lcg.., read once for both y_I += A_IJ v_J (row sums, reduced over
 // lcg) and y_J += A_IJ^H v_I (column sums, reduced over lrg). Row sums of a strip stay in
 // registers and are summed over the waves in LDS (fixed order) into Rs[I]; the column
 // sums of tile (I, J) go to Cs[I][J] (the diagonal tile's strictly-lower part too), and
@@ -224,10 +225,10 @@ __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict_
         C racc[4];
 #pragma unroll
         for (int ru = 0; ru < 4; ru++) racc[ru] = C{(T)0, (T)0};
-        const int rb = I * kLgTS + 4 * lrg;                  // local row of the sub-block
+        const int rb = I * kLgTS + lrg;                      // local rows rb + 8 ru of the sub-block
         C vr[4];
 #pragma unroll
-        for (int ru = 0; ru < 4; ru++) vr[ru] = rb + ru < m ? vs[rb + ru] : C{(T)0, (T)0};
+        for (int ru = 0; ru < 4; ru++) vr[ru] = rb + 8 * ru < m ? vs[rb + 8 * ru] : C{(T)0, (T)0};
         for (int J = w; J <= I; J += 4) {
             const int cb = J * kLgTS + 4 * lcg;
             C a[4][4], vc[4];
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict_
                 vc[cu] = cb + cu < m ? vs[cb + cu] : C{(T)0, (T)0};
                 const long colo = (long)(s0 + min(cb + cu, m - 1)) * n;
 #pragma unroll
-                for (int ru = 0; ru < 4; ru++) a[ru][cu] = ldx(A, colo + s0 + min(rb + ru, m - 1));
+                for (int ru = 0; ru < 4; ru++) a[ru][cu] = ldx(A, colo + s0 + min(rb + 8 * ru, m - 1));
             }
             const bool diag = J == I;
             C csum[4];
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict_
             for (int ru = 0; ru < 4; ru++)
 #pragma unroll
                 for (int cu = 0; cu < 4; cu++) {
-                    const int lr = rb + ru, lc = cb + cu;
+                    const int lr = rb + 8 * ru, lc = cb + cu;
                     const bool in = lr < m && lc < m && (!diag || lc <= lr);
                     const C x = in ? a[ru][cu] : C{(T)0, (T)0};
                     fmax_(racc[ru], x, vc[cu]);                                  // A(r, c) v_c
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict_
             }
         if (lcg == 0) {
 #pragma unroll
-            for (int ru = 0; ru < 4; ru++) rpart[w][4 * lrg + ru] = racc[ru];
+            for (int ru = 0; ru < 4; ru++) rpart[w][lrg + 8 * ru] = racc[ru];
         }
         __syncthreads();
         if (t < kLgTS) Rs[(long)I * kLgTS + t] = ((rpart[0][t] + rpart[1][t]) + rpart[2][t]) + rpart[3][t];
