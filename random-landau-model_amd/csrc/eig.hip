@@ -2064,12 +2064,12 @@ __global__ __launch_bounds__(256) void larft_kernel(const T *__restrict__ Abase,
     }
 }
 
-// U = Q U for a slab of 32 eigenvector columns per workgroup (4 waves; wave w owns
-// the 16-row blocks w, w+4, w+8, w+12). Per reflector block, last first:
+// U = Q U for a slab of CW eigenvector columns per workgroup (NWV waves; wave w owns
+// the 16-row blocks w, w + NWV, w + 2 NWV, w + 3 NWV: n <= 64 NWV). Per reflector block, last first:
 // W = V^H U (each wave over its rows, summed through LDS in a fixed order),
 // W2 = T W, U -= V W2. Workgroups are mapped so that the slabs of one matrix share
 // blockIdx % 8, i.e. one XCD and its L2, which then serves V and T to all of them.
-template <typename T, int CW>
+template <typename T, int CW, int NWV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void backtransform_mfma_kernel(const T *__restrict__ Abase, long strideA, int lda,
                                                                  int n, const T *__restrict__ Tin, long strideT,
                                                                  T *__restrict__ Uout, long strideU, int nslab,
@@ -2083,8 +2083,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     // in the update (LDS and registers then allow two workgroups per CU)
     constexpr bool STAGE_ALL = sizeof(T) == sizeof(float);
     constexpr bool PREFETCH = STAGE_ALL;
-    __shared__ CT vst[4][STAGE_ALL ? RBW : 1][16 * VP];          // per wave: V(rows of row block i, k)
-    __shared__ T wred[2][4][2 * NCB * 4 * kWave];
+    __shared__ CT vst[NWV][STAGE_ALL ? RBW : 1][16 * VP];        // per wave: V(rows of row block i, k)
+    __shared__ T wred[2][NWV][2 * NCB * 4 * kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int li = lane & 15, lq = lane >> 4;
@@ -2099,7 +2099,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     v4 ur[RBW][NCB], ui[RBW][NCB];
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
-        const int R = (w + 4 * i) * 16;
+        const int R = (w + NWV * i) * 16;
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
             const int col = c0 + cb * 16 + li;
@@ -2119,7 +2119,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
         if (!STAGE_ALL) return;
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
-            const int R = (w + 4 * i) * 16;
+            const int R = (w + NWV * i) * 16;
 #pragma unroll
             for (int kk = 0; kk < 4; kk++)
                 vpre[i][kk] = (R + 15 >= bk * 16 + 1) ? refl_get(A, lda, n, bk * 16, R + li, lq + 4 * kk)
@@ -2150,7 +2150,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
         // W = V^H U over this wave's rows (a = conj V)
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
-            const int R = (w + 4 * i) * 16;
+            const int R = (w + NWV * i) * 16;
             if (R + 15 < r0) continue;                    // wave-uniform: V is zero above row r0
             const int si = STAGE_ALL ? i : 0;
             if (!STAGE_ALL) {
@@ -2172,7 +2172,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
             }
             if (!STAGE_ALL) __builtin_amdgcn_wave_barrier();   // the next row block rewrites vst
         }
-        // sum W over the 4 waves (double-buffered by block parity: one barrier per block)
+        // sum W over the waves (double-buffered by block parity: one barrier per block)
         T *wb = wred[bk & 1][w];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++)
@@ -2189,7 +2189,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
             for (int v = 0; v < 4; v++) {
                 T sr = 0, si = 0;
 #pragma unroll
-                for (int ww = 0; ww < 4; ww++) {
+                for (int ww = 0; ww < NWV; ww++) {
                     sr += wred[bk & 1][ww][((0 * NCB + cb) * 4 + v) * kWave + lane];
                     si += wred[bk & 1][ww][((1 * NCB + cb) * 4 + v) * kWave + lane];
                 }
@@ -2224,7 +2224,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
         // U -= V W2 on this wave's rows (K index = reflector row(l, s))
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
-            const int R = (w + 4 * i) * 16;
+            const int R = (w + NWV * i) * 16;
             if (R + 15 < r0) continue;
 #pragma unroll
             for (int cb = 0; cb < NCB; cb++) q1[cb] = v4{0, 0, 0, 0};
@@ -2251,7 +2251,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     }
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
-        const int R = (w + 4 * i) * 16;
+        const int R = (w + NWV * i) * 16;
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
             const int col = c0 + cb * 16 + li;
@@ -2393,8 +2393,13 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
         constexpr int CW = sizeof(T) == sizeof(float) ? 32 : 16;
         const int nslab = (n + CW - 1) / CW;
         const long groups = (batch + 7) / 8;
-        hipLaunchKernelGGL((backtransform_mfma_kernel<T, CW>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0,
-                           stream, A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
+        // n <= 128: two waves per slab (twice the workgroups per CU, a 2-way W reduction)
+        if (n <= 128)
+            hipLaunchKernelGGL((backtransform_mfma_kernel<T, CW, 2>), dim3((unsigned)(groups * 8 * nslab)), dim3(128),
+                               0, stream, A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
+        else
+            hipLaunchKernelGGL((backtransform_mfma_kernel<T, CW, 4>), dim3((unsigned)(groups * 8 * nslab)), dim3(256),
+                               0, stream, A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
         RLM_HIP_CHECK(hipGetLastError());
         return 0;
     }
