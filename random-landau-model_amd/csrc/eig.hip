@@ -1214,6 +1214,192 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
     }
 }
 
+// Small matrices (n <= 32 RPT, n <= 8 CPT), register-resident in full storage: one
+// 256-thread workgroup per matrix (persistent over the batch), thread t holding A(r, c)
+// for rows r = t % 32 + 32 k and columns c = t / 32 + 8 q, so that several matrices share a
+// CU (the fused kernel keeps them in L2 and streams the trailing block twice per column).
+// Per column i: the column's owners write it to LDS; wave 0 builds reflector i (zlarfg,
+// as the other kernels); every thread forms its part of y = A_i v_i (row sums over its
+// columns, then over the 8 column groups); wave 0 forms w_i = tau y - 1/2 tau (tau y)^H v v;
+// every thread applies A -= v w^H + w v^H to its entries. Same outputs as zhetrd_kernel.
+template <typename T, int RPT, int CPT>
+__global__ __launch_bounds__(256) void zhetrd_small_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
+                                                          double *__restrict__ Dout, double *__restrict__ Eout,
+                                                          T *__restrict__ TAUout, long strideV, int batch) {
+    using CT = cx<T>;
+    constexpr int MAXN = 32 * RPT, NQV = (MAXN + kWave - 1) / kWave;
+    __shared__ CT col[MAXN], vsh[MAXN], wsh[MAXN], ypart[4][MAXN];
+    __shared__ T sc[2];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    const int rl = t & 31, cg = t >> 5;                      // row lane, column group
+    for (long b = blockIdx.x; b < batch; b += gridDim.x) {
+        T *A = Abase + 2 * b * strideA;
+        double *Dv = Dout + b * strideV;
+        double *Ev = Eout + b * strideV;
+        T *Tv = TAUout + b * strideV * (long)(sizeof(double) / sizeof(T));
+        CT a[RPT][CPT];
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+#pragma unroll
+            for (int q = 0; q < CPT; q++) {
+                const int r = rl + 32 * k, c = cg + 8 * q;
+                a[k][q] = (r < n && c < n) ? ldx(A, (long)c * lda + r) : CT{(T)0, (T)0};
+                if (r == c) a[k][q].y = (T)0;
+            }
+        for (int i = 0; i < n - 1; i++) {
+            const int i1 = i + 1;
+            // (1) column i (updated through step i-1), rows >= i, to LDS; 0/1 weights rather
+            // than selects, which the compiler would fold into a dynamically indexed
+            // (scratch-resident) array
+            if (cg == (i & 7)) {
+                const int qi = i >> 3;
+                T m[CPT];
+#pragma unroll
+                for (int q = 0; q < CPT; q++) m[q] = (q == qi) ? (T)1 : (T)0;
+#pragma unroll
+                for (int k = 0; k < RPT; k++) {
+                    CT x{(T)0, (T)0};
+#pragma unroll
+                    for (int q = 0; q < CPT; q++) {
+                        x.x = fma(m[q], a[k][q].x, x.x);
+                        x.y = fma(m[q], a[k][q].y, x.y);
+                    }
+                    const int r = rl + 32 * k;
+                    if (r >= i && r < n) col[r] = x;
+                }
+            }
+            __syncthreads();
+            // (2) reflector i from col[i..n-1] (wave 0); v (absolute rows, zero outside
+            // i+1..n-1) to vsh, beta and the scaled vector into A(i+1:n, i)
+            if (wid == 0) {
+                T s2 = 0;
+                for (int r = i + 2 + lane; r < n; r += kWave) s2 += abs2x(col[r]);
+                const T nrm2 = sum_wave_valu(s2);
+                const T dii = col[i].x;
+                const T ar = col[i1].x, ai = col[i1].y;
+                T beta, taur, taui, sr, si;
+                if (nrm2 == (T)0 && ai == (T)0) {
+                    beta = ar; taur = 0; taui = 0; sr = 0; si = 0;
+                } else {
+                    const T nrm = sqrt(ar * ar + ai * ai + nrm2);
+                    beta = (ar >= (T)0) ? -nrm : nrm;
+                    taur = (beta - ar) / beta;
+                    taui = -ai / beta;
+                    const T dr = ar - beta, di = ai;
+                    const T inv = (T)1 / (dr * dr + di * di);
+                    sr = dr * inv; si = -di * inv;
+                }
+                const CT scal{sr, si};
+                for (int r = lane; r < MAXN; r += kWave) {
+                    CT v{(T)0, (T)0};
+                    if (r == i1) {
+                        v = CT{(T)1, (T)0};
+                        stx(A, (long)i * lda + r, CT{beta, (T)0});
+                    } else if (r > i1 && r < n) {
+                        v = col[r] * scal;
+                        stx(A, (long)i * lda + r, v);
+                    }
+                    vsh[r] = v;
+                }
+                if (lane == 0) {
+                    Dv[i] = (double)dii;
+                    Ev[i] = (double)beta;
+                    Tv[2 * i] = taur;
+                    Tv[2 * i + 1] = taui;
+                    sc[0] = taur;
+                    sc[1] = taui;
+                }
+            }
+            __syncthreads();
+            // (3) y = A_i v: row sums over this thread's columns, then over the column groups
+            {
+                CT y[RPT];
+#pragma unroll
+                for (int k = 0; k < RPT; k++) y[k] = CT{(T)0, (T)0};
+#pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    const int c = cg + 8 * q;
+                    const CT vc = c < MAXN ? vsh[c] : CT{(T)0, (T)0};
+#pragma unroll
+                    for (int k = 0; k < RPT; k++) fmax_(y[k], a[k][q], vc);
+                }
+#pragma unroll
+                for (int k = 0; k < RPT; k++) {
+                    y[k].x += __shfl_xor(y[k].x, 32, kWave);        // the wave's two column groups
+                    y[k].y += __shfl_xor(y[k].y, 32, kWave);
+                    if (lane < 32) ypart[wid][rl + 32 * k] = y[k];
+                }
+            }
+            __syncthreads();
+            // (4) w_i (wave 0)
+            if (wid == 0) {
+                const CT tau{sc[0], sc[1]};
+                CT pv[NQV], vv[NQV];
+                CT dl{(T)0, (T)0};
+#pragma unroll
+                for (int k = 0; k < NQV; k++) {
+                    const int r = lane + kWave * k;
+                    CT p{(T)0, (T)0}, v{(T)0, (T)0};
+                    if (r < MAXN) {
+                        p = ((ypart[0][r] + ypart[1][r]) + ypart[2][r]) + ypart[3][r];
+                        v = vsh[r];
+                    }
+                    p = (r > i && r < n) ? tau * p : CT{(T)0, (T)0};
+                    fmacx(dl, p, v);
+                    pv[k] = p;
+                    vv[k] = v;
+                }
+                dl.x = sum_wave_valu(dl.x);
+                dl.y = sum_wave_valu(dl.y);
+                const CT alpha2 = (tau * dl) * (T)(-0.5);
+#pragma unroll
+                for (int k = 0; k < NQV; k++) {
+                    const int r = lane + kWave * k;
+                    if (r < MAXN) wsh[r] = (r > i && r < n) ? pv[k] + alpha2 * vv[k] : CT{(T)0, (T)0};
+                }
+            }
+            __syncthreads();
+            // (5) A -= v w^H + w v^H on this thread's entries (v, w vanish outside i+1..n-1)
+            {
+                CT vr[RPT], wr[RPT];
+#pragma unroll
+                for (int k = 0; k < RPT; k++) {
+                    vr[k] = vsh[rl + 32 * k];
+                    wr[k] = wsh[rl + 32 * k];
+                }
+#pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    const int c = cg + 8 * q;
+                    const CT wc = c < MAXN ? conjx(wsh[c]) : CT{(T)0, (T)0};
+                    const CT vc = c < MAXN ? conjx(vsh[c]) : CT{(T)0, (T)0};
+#pragma unroll
+                    for (int k = 0; k < RPT; k++) {
+                        CT x = a[k][q];
+                        fms_(x, vr[k], wc);
+                        fms_(x, wr[k], vc);
+                        if (rl + 32 * k == c) x.y = (T)0;           // Hermitian diagonal stays real
+                        a[k][q] = x;
+                    }
+                }
+            }
+        }
+        // last diagonal element
+        {
+            const int r = n - 1;
+            if (rl == (r & 31) && cg == (r & 7)) {
+                const int kk = r >> 5, qq = r >> 3;
+                T x = 0;
+#pragma unroll
+                for (int k = 0; k < RPT; k++)
+#pragma unroll
+                    for (int q = 0; q < CPT; q++) x = fma((k == kk && q == qq) ? (T)1 : (T)0, a[k][q].x, x);
+                Dv[n - 1] = (double)x;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ============================================================================
 // 2. Divide & conquer on the symmetric tridiagonal + 3. back-transform
 // ============================================================================
@@ -2287,6 +2473,32 @@ int heevd_dc_workspace_doubles(int n, int grid_tail, long *per_matrix, long *per
     return 0;
 }
 
+// register-resident small-n reduction (zhetrd_small_kernel), n <= 72 (wider column
+// ranges would cost the second workgroup per CU); persistent workgroups as many as the
+// CUs hold at once
+template <typename T, int RPT, int CPT>
+static int launch_zhetrd_small_rc(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
+                                  int batch, hipStream_t stream) {
+    int dev = 0, cus = 0, per_cu = 0;
+    RLM_HIP_CHECK(hipGetDevice(&dev));
+    RLM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RLM_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)zhetrd_small_kernel<T, RPT, CPT>,
+                                                               256, 0));
+    int grid = per_cu > 0 && cus > 0 ? per_cu * cus : 1024;
+    if (grid > batch) grid = batch;
+    hipLaunchKernelGGL((zhetrd_small_kernel<T, RPT, CPT>), dim3(grid), dim3(256), 0, stream, A, strideA, lda, n, D, E,
+                       TAU, strideV, batch);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+template <typename T>
+static int launch_zhetrd_small_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
+                                 int batch, hipStream_t stream) {
+    if (n < 1 || n > 72) return -22;
+    if (n <= 64) return launch_zhetrd_small_rc<T, 2, 8>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
+    return launch_zhetrd_small_rc<T, 3, 9>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
+}
+
 template <typename T>
 int launch_zhetrd_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
                     int grid, hipStream_t stream) {
@@ -2496,11 +2708,16 @@ static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, doubl
                             int grid, hipStream_t stream) {
     int v = hetrd_variant();
     const int rmax = resident_max_n(sizeof(T));
-    // auto: complex double n <= 96 takes the fused kernel (its small LDS footprint lets
-    // many matrices share a CU, where the resident kernel holds one per CU: hlm n = 72,
-    // 67584 matrices: 52.7 against 111 ms; n = 64: 6.5 against 9.8 ms per C4 step; at
-    // n = 128 the resident kernel leads, 23.8 against 26.5 ms)
-    if (v == kTrdAuto) v = (sizeof(T) == sizeof(double) && n <= 96) ? kTrdFused : n <= rmax ? kTrdResident : kTrdLower;
+    // auto: complex double n <= 72 takes zhetrd_small_kernel (registers, two matrices per
+    // CU: hlm n = 72, 67584 matrices: 30.3 ms), 72 < n <= 96 the fused kernel (its small
+    // LDS footprint lets five matrices share a CU, where the resident kernel holds one:
+    // hlm n = 72 52.7 against 111 ms), n <= 128 the resident kernel (n = 128: 23.8 against
+    // 26.5 ms fused)
+    if (v == kTrdAuto) {
+        if (sizeof(T) == sizeof(double) && n <= 72)
+            return launch_zhetrd_small_t<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
+        v = (sizeof(T) == sizeof(double) && n <= 96) ? kTrdFused : n <= rmax ? kTrdResident : kTrdLower;
+    }
     if ((v == kTrdResident || v == kTrdHybrid) && n <= rmax)
         return launch_resident_any<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, nullptr, 0);
     if (v == kTrdHybrid && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024) {
