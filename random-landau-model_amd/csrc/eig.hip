@@ -958,9 +958,286 @@ static bool resident_deal(int TN, int NS, bool diag0, int NL, int nw, ResidentDe
     return true;
 }
 
+// Tile layouts of the register-resident kernel (its TL parameter). Each gives the
+// register storage of one 32x32 tile (R, C) and, for it: the load; the pending rank-2
+// update of step i-1 (update: a pass of its own over every live tile before the
+// products when kSplit, else fused into product); the product pass into the wave's
+// partial y, which also captures column i1 = i + 1 into colb; the same for a tile of
+// tile column 0 held in LDS (load_lds, col0_lds, step_lds: entry k of lane l at
+// p[k * 64], p = tile base + l).
+
+// 4x4 sub-blocks: lane (lrg, lcg) = (lane & 7, lane >> 3) holds tile rows 4 lrg..,
+// columns 4 lcg..; update and products in one VALU pass.
+template <typename T>
+struct ResidentSub4 {
+    using CT = cx<T>;
+    struct Tile {
+        CT a[16];
+    };
+    static constexpr bool kSplit = false;
+    template <typename F>
+    static __device__ __forceinline__ void load(Tile &t, int R, int C, int lane, int n, F entry) {
+        const int rb = R * 32 + 4 * (lane & 7), cb = C * 32 + 4 * (lane >> 3);
+#pragma unroll
+        for (int cu = 0; cu < 4; cu++)
+#pragma unroll
+            for (int ru = 0; ru < 4; ru++) {
+                const int r = rb + ru, c = cb + cu;
+                CT x{(T)0, (T)0};
+                if (R >= 0 && r < n && c < n) x = r >= c ? entry(r, c) : conjx(entry(c, r));
+                if (r == c) x.y = (T)0;
+                t.a[ru * 4 + cu] = x;
+            }
+    }
+    template <typename F>
+    static __device__ __forceinline__ void load_lds(CT *l, int R, int lane, int n, F entry) {
+        const int rb = R * 32 + 4 * (lane & 7), cb = 4 * (lane >> 3);
+#pragma unroll
+        for (int cu = 0; cu < 4; cu++)
+#pragma unroll
+            for (int ru = 0; ru < 4; ru++) {
+                const int r = rb + ru, c = cb + cu;
+                l[(ru * 4 + cu) * kWave] = r < n ? entry(r, c) : CT{(T)0, (T)0};
+            }
+    }
+    static __device__ __forceinline__ void col0(const Tile &t, int R, int lane, int n, CT *colb) {
+        if ((lane >> 3) != 0) return;
+#pragma unroll
+        for (int ru = 0; ru < 4; ru++) {
+            const int r = R * 32 + 4 * (lane & 7) + ru;
+            if (r < n) colb[r] = t.a[ru * 4];
+        }
+    }
+    static __device__ __forceinline__ void col0_lds(const CT *l, int R, int lane, int n, CT *colb) {
+        if ((lane >> 3) != 0) return;
+#pragma unroll
+        for (int ru = 0; ru < 4; ru++) {
+            const int r = R * 32 + 4 * (lane & 7) + ru;
+            if (r < n) colb[r] = l[(ru * 4) * kWave];
+        }
+    }
+    static __device__ __forceinline__ void update(Tile &, bool, int, int, int, int, const CT *, const CT *) {}
+    static __device__ __forceinline__ void product(Tile &t, bool diag, int R, int C, int lane, int i1, const CT *vn,
+                                                   const CT *vo, const CT *wo, CT *yw, CT *colb) {
+        const int lrg = lane & 7, lcg = lane >> 3;
+        int rb = R * 32 + 4 * lrg, cb = C * 32 + 4 * lcg;
+        asm volatile("" : "+v"(rb), "+v"(cb));       // keep per-slot indices out of loop-invariant hoisting
+        if (diag)
+            resident_tile_diag<T>(t.a, rb, cb, lrg, lcg, vn, vo, wo, yw);
+        else
+            resident_tile_off<T>(RegSub<T>{t.a}, rb, cb, lrg, lcg, vn, vo, wo, yw);
+        if (C == i1 / 32 && lcg == (i1 % 32) / 4) resident_capture<T>(RegSub<T>{t.a}, rb, i1 & 3, colb);
+    }
+    static __device__ __forceinline__ void step_lds(CT *l, int R, int lane, int i1, const CT *vn, const CT *vo,
+                                                    const CT *wo, CT *yw, CT *colb) {
+        const int lrg = lane & 7, lcg = lane >> 3;
+        const int rb = R * 32 + 4 * lrg, cb = 4 * lcg;
+        resident_tile_off<T>(LdsSub<T>{l}, rb, cb, lrg, lcg, vn, vo, wo, yw);
+        if (i1 / 32 == 0 && lcg == (i1 % 32) / 4) resident_capture<T>(LdsSub<T>{l}, rb, i1 & 3, colb);
+    }
+};
+
+// Sum over the 16 lanes of each DPP row (lane bits 0-3), scattered: lane l returns the
+// sum of x[l & 15]. Each step halves the set of values a lane keeps and adds its
+// partner's copy of that set: bit 3 by a row rotation of 8 (xor 8), bit 2 by the
+// half-row mirror (xor 7), bits 1 and 0 by quad permutations (xor 2, xor 1). The four
+// partner sets together reach all 16 lanes of the row.
+__device__ __forceinline__ float reduce_scatter16(const float (&x)[16], int li) {
+    const bool b3 = (li & 8) != 0, b2 = (li & 4) != 0, b1 = (li & 2) != 0, b0 = (li & 1) != 0;
+    float y[8], z[4], w[2];
+#pragma unroll
+    for (int k = 0; k < 8; k++) y[k] = (b3 ? x[8 + k] : x[k]) + dpp_mov<0x128>(b3 ? x[k] : x[8 + k]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) z[k] = (b2 ? y[4 + k] : y[k]) + dpp_mov<0x141>(b2 ? y[k] : y[4 + k]);
+#pragma unroll
+    for (int k = 0; k < 2; k++) w[k] = (b1 ? z[2 + k] : z[k]) + dpp_mov<0x4E>(b1 ? z[k] : z[2 + k]);
+    return (b0 ? w[1] : w[0]) + dpp_mov<0xB1>(b0 ? w[0] : w[1]);
+}
+
+// Matrix-core layout (complex float): the tile is four 16x16 blocks (p, q) in the
+// accumulator layout of v_mfma_f32_16x16x4_f32, real and imaginary parts apart: lane
+// (li, lq) = (lane & 15, lane >> 4) holds rows 16 p + 4 lq + u (u = 0..3) of column
+// 16 q + li. The pending rank-2 update A -= v w^H + w v^H is a real rank-4 update of
+// each part,
+//   Re A -= [vr vi wr wi]_r [wr wi vr vi]_c^T,   Im A -= [vi -vr wi -wr]_r [wr wi vr vi]_c^T,
+// one 16x16x4 product per block and part; the update pass issues them for every live
+// tile before the VALU products read the results. Column sums reduce over lq (two lane
+// swaps), row sums over li (reduce_scatter16). Diagonal tiles (full Hermitian blocks)
+// add their part of y as column sums, y_c = sum_r conj(A(r, c)) v_r. Column blocks
+// q = 0 wholly left of the trailing block are skipped: their vectors are zero.
+struct ResidentMfma {
+    using CT = cx<float>;
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    struct Tile {
+        v4 re[2][2], im[2][2];
+    };
+    static constexpr bool kSplit = true;
+    template <typename F>
+    static __device__ __forceinline__ void load(Tile &t, int R, int C, int lane, int n, F entry) {
+        const int li = lane & 15, lq = lane >> 4;
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int r = R * 32 + 16 * p + 4 * lq + u, c = C * 32 + 16 * q + li;
+                    CT x{0.f, 0.f};
+                    if (R >= 0 && r < n && c < n) x = r >= c ? entry(r, c) : conjx(entry(c, r));
+                    if (r == c) x.y = 0.f;
+                    t.re[p][q][u] = x.x;
+                    t.im[p][q][u] = x.y;
+                }
+    }
+    static __device__ __forceinline__ void to_lds(const Tile &t, CT *l) {
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) l[((p * 2 + q) * 4 + u) * kWave] = CT{t.re[p][q][u], t.im[p][q][u]};
+    }
+    static __device__ __forceinline__ void from_lds(Tile &t, const CT *l) {
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const CT x = l[((p * 2 + q) * 4 + u) * kWave];
+                    t.re[p][q][u] = x.x;
+                    t.im[p][q][u] = x.y;
+                }
+    }
+    template <typename F>
+    static __device__ __forceinline__ void load_lds(CT *l, int R, int lane, int n, F entry) {
+        Tile t;
+        load(t, R, 0, lane, n, entry);
+        to_lds(t, l);
+    }
+    // column c1 (lanes li == c1 % 16 of tile column c1 / 32) to colb; 0/1 weights pick the
+    // column block (selects would index the tile dynamically and move it to scratch)
+    static __device__ __forceinline__ void capture(const Tile &t, int R, int C, int c1, int lane, CT *colb) {
+        if (C != c1 / 32 || (lane & 15) != c1 % 16) return;
+        const int lq = lane >> 4;
+        const float m1 = (c1 & 16) ? 1.f : 0.f, m0 = 1.f - m1;
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                colb[R * 32 + 16 * p + 4 * lq + u] = CT{fmaf(m1, t.re[p][1][u], m0 * t.re[p][0][u]),
+                                                        fmaf(m1, t.im[p][1][u], m0 * t.im[p][0][u])};
+    }
+    static __device__ __forceinline__ void col0(const Tile &t, int R, int lane, int, CT *colb) {
+        capture(t, R, 0, 0, lane, colb);
+    }
+    static __device__ __forceinline__ void col0_lds(const CT *l, int R, int lane, int n, CT *colb) {
+        Tile t;
+        from_lds(t, l);
+        col0(t, R, lane, n, colb);
+    }
+    template <int Q0>
+    static __device__ __forceinline__ void update_q(Tile &t, bool diag, const float (&ar)[2], const float (&ai)[2],
+                                                    const float (&bc)[2]) {
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            if (diag && p < Q0) continue;
+#pragma unroll
+            for (int q = Q0; q < 2; q++) {
+                t.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], bc[q], t.re[p][q], 0, 0, 0);
+                t.im[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[p], bc[q], t.im[p][q], 0, 0, 0);
+            }
+        }
+    }
+    static __device__ __forceinline__ void update(Tile &t, bool diag, int R, int C, int lane, int i1, const CT *vo,
+                                                  const CT *wo) {
+        const int li = lane & 15, lq = lane >> 4;
+        const bool sw = lq >= 2, sy = (lq & 1) != 0;
+        float ar[2], ai[2], bc[2];          // operands: k = lq of [vr vi wr wi]_r (negated), of the
+#pragma unroll                              // Im row operand, of [wr wi vr vi]_c
+        for (int p = 0; p < 2; p++) {
+            const int r = R * 32 + 16 * p + li;
+            const CT s = sw ? wo[r] : vo[r];
+            ar[p] = sy ? -s.y : -s.x;
+            ai[p] = sy ? s.x : -s.y;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int c = C * 32 + 16 * q + li;
+            const CT s = sw ? vo[c] : wo[c];
+            bc[q] = sy ? s.y : s.x;
+        }
+        if (i1 > C * 32 + 15)
+            update_q<1>(t, diag, ar, ai, bc);
+        else
+            update_q<0>(t, diag, ar, ai, bc);
+    }
+    template <int Q0>
+    static __device__ __forceinline__ void product_q(Tile &t, bool diag, int R, int C, int lane, const CT *vn,
+                                                     CT *yw) {
+        const int li = lane & 15, lq = lane >> 4;
+        CT vr[2][4];
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) vr[p][u] = vn[R * 32 + 16 * p + 4 * lq + u];
+        if (diag) {
+            const int dv = li - 4 * lq;       // entry u of block (p, p) is on the diagonal
+#pragma unroll
+            for (int p = Q0; p < 2; p++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) t.im[p][p][u] = dv == u ? 0.f : t.im[p][p][u];
+        } else {
+            float x[16];
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    CT rs{0.f, 0.f};
+#pragma unroll
+                    for (int q = Q0; q < 2; q++) fmax_(rs, CT{t.re[p][q][u], t.im[p][q][u]}, vn[C * 32 + 16 * q + li]);
+                    x[(p * 4 + u) * 2] = rs.x;
+                    x[(p * 4 + u) * 2 + 1] = rs.y;
+                }
+            const float s = reduce_scatter16(x, li);
+            float *yf = reinterpret_cast<float *>(yw);
+            const int e = 2 * (R * 32 + 16 * (li >> 3) + 4 * lq + ((li >> 1) & 3)) + (li & 1);
+            yf[e] += s;
+        }
+#pragma unroll
+        for (int q = Q0; q < 2; q++) {
+            CT cs{0.f, 0.f};
+#pragma unroll
+            for (int p = diag ? Q0 : 0; p < 2; p++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) fmacx(cs, CT{t.re[p][q][u], t.im[p][q][u]}, vr[p][u]);
+            cs.x = sum_xor32(sum_xor16(cs.x));
+            cs.y = sum_xor32(sum_xor16(cs.y));
+            const int c = C * 32 + 16 * q + li;
+            yw[c] = yw[c] + cs;              // the 4 lanes of a column hold the same sum
+        }
+    }
+    static __device__ __forceinline__ void product(Tile &t, bool diag, int R, int C, int lane, int i1, const CT *vn,
+                                                   const CT *, const CT *, CT *yw, CT *colb) {
+        if (i1 > C * 32 + 15)
+            product_q<1>(t, diag, R, C, lane, vn, yw);
+        else
+            product_q<0>(t, diag, R, C, lane, vn, yw);
+        capture(t, R, C, i1, lane, colb);
+    }
+    static __device__ __forceinline__ void step_lds(CT *l, int R, int lane, int i1, const CT *vn, const CT *vo,
+                                                    const CT *wo, CT *yw, CT *colb) {
+        Tile t;
+        from_lds(t, l);
+        update(t, false, R, 0, lane, i1, vo, wo);
+        to_lds(t, l);
+        product(t, false, R, 0, lane, i1, vn, vo, wo, yw, colb);
+    }
+};
+
 // Hb (optional): the matrices in the band form of offset hb_off (complex double,
 // n x band_nb(n, hb_off) per matrix) instead of A; A then only receives the reflectors.
-template <typename T, int NS, int NL, bool DIAG0, int NW, int NMAX>
+template <typename T, int NS, int NL, bool DIAG0, int NW, int NMAX, typename TL>
 __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Abase, long strideA, int lda, int n,
                                                              double *__restrict__ Dout, double *__restrict__ Eout,
                                                              T *__restrict__ TAUout, long strideV, int batch,
@@ -975,7 +1252,6 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
     __shared__ CT stau[NMAX];
     const int t = threadIdx.x, lane = t & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
-    const int lrg = lane & 7, lcg = lane >> 3;
     // tiles of this wave (host-dealt, see resident_deal)
     int tR[NS], tC[NS];
 #pragma unroll
@@ -1008,31 +1284,10 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
             return CT{(T)h.x, r == c ? (T)0 : (T)h.y};
         };
         // diagonal tiles hold both triangles: A(r, c) = conj(A(c, r)) above the diagonal
-        CT a[NS][16];
+        typename TL::Tile a[NS];
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const int rb = tR[s] * TS + 4 * lrg, cb = tC[s] * TS + 4 * lcg;
-#pragma unroll
-            for (int cu = 0; cu < 4; cu++)
-#pragma unroll
-                for (int ru = 0; ru < 4; ru++) {
-                    const int r = rb + ru, c = cb + cu;
-                    CT x{(T)0, (T)0};
-                    if (tR[s] >= 0 && r < n && c < n) x = r >= c ? lower_entry(r, c) : conjx(lower_entry(c, r));
-                    if (r == c) x.y = (T)0;
-                    a[s][ru * 4 + cu] = x;
-                }
-        }
-        if (NL > 0 && lR >= 0) {
-            const int rb = lR * TS + 4 * lrg, cb = 4 * lcg;
-#pragma unroll
-            for (int cu = 0; cu < 4; cu++)
-#pragma unroll
-                for (int ru = 0; ru < 4; ru++) {
-                    const int r = rb + ru, c = cb + cu;
-                    lsub[(ru * 4 + cu) * kWave] = r < n ? lower_entry(r, c) : CT{(T)0, (T)0};
-                }
-        }
+        for (int s = 0; s < NS; s++) TL::load(a[s], tR[s], tC[s], lane, n, lower_entry);
+        if (NL > 0 && lR >= 0) TL::load_lds(lsub, lR, lane, n, lower_entry);
         for (int e = t; e < NMAX; e += NT) {
             vbuf[0][e] = CT{(T)0, (T)0};
             vbuf[1][e] = CT{(T)0, (T)0};
@@ -1041,22 +1296,9 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
         }
         // column 0 to LDS
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            if (tC[s] == 0 && lcg == 0) {
-#pragma unroll
-                for (int ru = 0; ru < 4; ru++) {
-                    const int r = tR[s] * TS + 4 * lrg + ru;
-                    if (r < n) colb[r] = a[s][ru * 4];
-                }
-            }
-        }
-        if (NL > 0 && lR >= 0 && lcg == 0) {
-#pragma unroll
-            for (int ru = 0; ru < 4; ru++) {
-                const int r = lR * TS + 4 * lrg + ru;
-                if (r < n) colb[r] = lsub[(ru * 4) * kWave];
-            }
-        }
+        for (int s = 0; s < NS; s++)
+            if (tC[s] == 0) TL::col0(a[s], tR[s], lane, n, colb);
+        if (NL > 0 && lR >= 0) TL::col0_lds(lsub, lR, lane, n, colb);
         __syncthreads();
 
         // wave 0 only: reflector of step i from cq[q] = column i at row lane + 64 q
@@ -1133,7 +1375,6 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
             const CT *vo = vbuf[cur ^ 1], *wo = wbuf[cur ^ 1];   // v_{i-1}, w_{i-1}
             CT *yw = yp[wid];
             for (int e = lane; e < NMAX; e += kWave) yw[e] = CT{(T)0, (T)0};
-            const int c1t = i1 / TS, c1g = (i1 % TS) / 4, c1u = i1 & 3;   // owner of column i+1
             if (wid == (i + 1) % NW) {        // reflector i into A(i+1:n, i) (beta at i+1), a wave in turn
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
@@ -1141,22 +1382,19 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                     if (r > i && r < n) stx(A, (long)i * lda + r, r == i1 ? CT{se[i], (T)0} : vn[r]);
                 }
             }
+            if constexpr (TL::kSplit) {
+#pragma unroll
+                for (int s = 0; s < NS; s++) {
+                    if (tR[s] < 0 || tC[s] * TS + TS - 1 < i1) continue;   // wave-uniform
+                    TL::update(a[s], DIAG0 ? s == 0 : tR[s] == tC[s], tR[s], tC[s], lane, i1, vo, wo);
+                }
+            }
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 if (tR[s] < 0 || tC[s] * TS + TS - 1 < i1) continue;   // wave-uniform
-                int rb = tR[s] * TS + 4 * lrg, cb = tC[s] * TS + 4 * lcg;
-                asm volatile("" : "+v"(rb), "+v"(cb));       // keep per-slot indices out of loop-invariant hoisting
-                if (DIAG0 ? s == 0 : tR[s] == tC[s])
-                    resident_tile_diag<T>(a[s], rb, cb, lrg, lcg, vn, vo, wo, yw);
-                else
-                    resident_tile_off<T>(RegSub<T>{a[s]}, rb, cb, lrg, lcg, vn, vo, wo, yw);
-                if (tC[s] == c1t && lcg == c1g) resident_capture<T>(RegSub<T>{a[s]}, rb, c1u, colb);
+                TL::product(a[s], DIAG0 ? s == 0 : tR[s] == tC[s], tR[s], tC[s], lane, i1, vn, vo, wo, yw, colb);
             }
-            if (NL > 0 && lR >= 0 && i1 < TS) {
-                const int rb = lR * TS + 4 * lrg, cb = 4 * lcg;
-                resident_tile_off<T>(LdsSub<T>{lsub}, rb, cb, lrg, lcg, vn, vo, wo, yw);
-                if (c1t == 0 && lcg == c1g) resident_capture<T>(LdsSub<T>{lsub}, rb, c1u, colb);
-            }
+            if (NL > 0 && lR >= 0 && i1 < TS) TL::step_lds(lsub, lR, lane, i1, vn, vo, wo, yw, colb);
             __syncthreads();
             if (wid == 0) {
                 // every LDS read up front and unconditional (all rows < NMAX), then
@@ -2637,14 +2875,14 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
     return 0;
 }
 
-template <typename T, int NS, int NL, bool DIAG0, int NW = 8, int NMAX = 256>
+template <typename T, int NS, int NL, bool DIAG0, int NW = 8, int NMAX = 256, typename TL = ResidentSub4<T>>
 static int launch_zhetrd_resident_t(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV,
                                     int batch, hipStream_t stream, const double *Hb, int hb_off) {
     const int tn = (n + 31) / 32;
     ResidentDeal deal;
     if (n < 1 || n > NMAX || !resident_deal(tn, NS, DIAG0, NL, NW, deal)) return -22;
-    hipLaunchKernelGGL((zhetrd_resident_kernel<T, NS, NL, DIAG0, NW, NMAX>), dim3(batch), dim3(64 * NW), 0, stream, A,
-                       strideA, lda, n, D, E, TAU, strideV, batch, Hb, hb_off, deal);
+    hipLaunchKernelGGL((zhetrd_resident_kernel<T, NS, NL, DIAG0, NW, NMAX, TL>), dim3(batch), dim3(64 * NW), 0, stream,
+                       A, strideA, lda, n, D, E, TAU, strideV, batch, Hb, hb_off, deal);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -2695,8 +2933,11 @@ static int launch_resident_any(T *A, long strideA, int lda, int n, double *D, do
         if (tn <= 5)
             return launch_zhetrd_resident_t<T, 2, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
                                                             hb_off);
-        return launch_zhetrd_resident_t<T, 4, 4, true>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
-                                                       hb_off);
+        if (getenv("RLM_RESIDENT_VALU"))
+            return launch_zhetrd_resident_t<T, 4, 4, true>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
+                                                           hb_off);
+        return launch_zhetrd_resident_t<T, 4, 4, true, 8, 256, ResidentMfma>(A, strideA, lda, n, D, E, TAU, strideV,
+                                                                            batch, stream, Hb, hb_off);
     } else {
         return launch_zhetrd_resident_t<T, 1, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
                                                         hb_off);
