@@ -1136,12 +1136,11 @@ struct ResidentMfma {
         from_lds(t, l);
         col0(t, R, lane, n, colb);
     }
-    template <int Q0>
-    static __device__ __forceinline__ void update_q(Tile &t, bool diag, const float (&ar)[2], const float (&ai)[2],
+    template <int Q0, bool DG>
+    static __device__ __forceinline__ void update_q(Tile &t, const float (&ar)[2], const float (&ai)[2],
                                                     const float (&bc)[2]) {
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
-            if (diag && p < Q0) continue;
+        for (int p = DG ? Q0 : 0; p < 2; p++) {
 #pragma unroll
             for (int q = Q0; q < 2; q++) {
                 t.re[p][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[p], bc[q], t.re[p][q], 0, 0, 0);
@@ -1167,21 +1166,26 @@ struct ResidentMfma {
             const CT s = sw ? vo[c] : wo[c];
             bc[q] = sy ? s.y : s.x;
         }
-        if (i1 > C * 32 + 15)
-            update_q<1>(t, diag, ar, ai, bc);
-        else
-            update_q<0>(t, diag, ar, ai, bc);
+        // (diag and the column-block skip as template arguments: loop bounds that
+        // index the tile must be compile-time constants)
+        const bool half = i1 > C * 32 + 15;
+        if (diag) {
+            if (half) update_q<1, true>(t, ar, ai, bc);
+            else update_q<0, true>(t, ar, ai, bc);
+        } else {
+            if (half) update_q<1, false>(t, ar, ai, bc);
+            else update_q<0, false>(t, ar, ai, bc);
+        }
     }
-    template <int Q0>
-    static __device__ __forceinline__ void product_q(Tile &t, bool diag, int R, int C, int lane, const CT *vn,
-                                                     CT *yw) {
+    template <int Q0, bool DG>
+    static __device__ __forceinline__ void product_q(Tile &t, int R, int C, int lane, const CT *vn, CT *yw) {
         const int li = lane & 15, lq = lane >> 4;
         CT vr[2][4];
 #pragma unroll
         for (int p = 0; p < 2; p++)
 #pragma unroll
             for (int u = 0; u < 4; u++) vr[p][u] = vn[R * 32 + 16 * p + 4 * lq + u];
-        if (diag) {
+        if constexpr (DG) {
             const int dv = li - 4 * lq;       // entry u of block (p, p) is on the diagonal
 #pragma unroll
             for (int p = Q0; p < 2; p++)
@@ -1208,7 +1212,7 @@ struct ResidentMfma {
         for (int q = Q0; q < 2; q++) {
             CT cs{0.f, 0.f};
 #pragma unroll
-            for (int p = diag ? Q0 : 0; p < 2; p++)
+            for (int p = DG ? Q0 : 0; p < 2; p++)
 #pragma unroll
                 for (int u = 0; u < 4; u++) fmacx(cs, CT{t.re[p][q][u], t.im[p][q][u]}, vr[p][u]);
             cs.x = sum_xor32(sum_xor16(cs.x));
@@ -1219,10 +1223,14 @@ struct ResidentMfma {
     }
     static __device__ __forceinline__ void product(Tile &t, bool diag, int R, int C, int lane, int i1, const CT *vn,
                                                    const CT *, const CT *, CT *yw, CT *colb) {
-        if (i1 > C * 32 + 15)
-            product_q<1>(t, diag, R, C, lane, vn, yw);
-        else
-            product_q<0>(t, diag, R, C, lane, vn, yw);
+        const bool half = i1 > C * 32 + 15;
+        if (diag) {
+            if (half) product_q<1, true>(t, R, C, lane, vn, yw);
+            else product_q<0, true>(t, R, C, lane, vn, yw);
+        } else {
+            if (half) product_q<1, false>(t, R, C, lane, vn, yw);
+            else product_q<0, false>(t, R, C, lane, vn, yw);
+        }
         capture(t, R, C, i1, lane, colb);
     }
     static __device__ __forceinline__ void step_lds(CT *l, int R, int lane, int i1, const CT *vn, const CT *vo,
@@ -2924,18 +2932,30 @@ static int launch_resident_any(T *A, long strideA, int lda, int n, double *D, do
         // n <= 96 (at most 6 tiles): 4 waves with two register tiles each, so that two
         // matrices share a CU and one's single-wave reflector phase overlaps the other's
         // work (complex double would spill at two tiles per wave)
-        if (tn <= 3)
-            return launch_zhetrd_resident_t<T, 2, 0, false, 4, 128>(A, strideA, lda, n, D, E, TAU, strideV, batch,
-                                                                    stream, Hb, hb_off);
-        if (tn <= 4)
-            return launch_zhetrd_resident_t<T, 1, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
-                                                            hb_off);
-        if (tn <= 5)
-            return launch_zhetrd_resident_t<T, 2, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
-                                                            hb_off);
-        if (getenv("RLM_RESIDENT_VALU"))
+        // RLM_RESIDENT_VALU set: the VALU tile layout (update and products in one VALU
+        // pass) instead of the matrix-core update, for comparison
+        if (getenv("RLM_RESIDENT_VALU")) {
+            if (tn <= 3)
+                return launch_zhetrd_resident_t<T, 2, 0, false, 4, 128>(A, strideA, lda, n, D, E, TAU, strideV, batch,
+                                                                        stream, Hb, hb_off);
+            if (tn <= 4)
+                return launch_zhetrd_resident_t<T, 1, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream,
+                                                                Hb, hb_off);
+            if (tn <= 5)
+                return launch_zhetrd_resident_t<T, 2, 4, false>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream,
+                                                                Hb, hb_off);
             return launch_zhetrd_resident_t<T, 4, 4, true>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, Hb,
                                                            hb_off);
+        }
+        if (tn <= 3)
+            return launch_zhetrd_resident_t<T, 2, 0, false, 4, 128, ResidentMfma>(A, strideA, lda, n, D, E, TAU,
+                                                                                  strideV, batch, stream, Hb, hb_off);
+        if (tn <= 4)
+            return launch_zhetrd_resident_t<T, 1, 4, false, 8, 256, ResidentMfma>(A, strideA, lda, n, D, E, TAU,
+                                                                                  strideV, batch, stream, Hb, hb_off);
+        if (tn <= 5)
+            return launch_zhetrd_resident_t<T, 2, 4, false, 8, 256, ResidentMfma>(A, strideA, lda, n, D, E, TAU,
+                                                                                  strideV, batch, stream, Hb, hb_off);
         return launch_zhetrd_resident_t<T, 4, 4, true, 8, 256, ResidentMfma>(A, strideA, lda, n, D, E, TAU, strideV,
                                                                             batch, stream, Hb, hb_off);
     } else {
