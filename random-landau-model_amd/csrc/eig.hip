@@ -2904,8 +2904,9 @@ static int resident_max_n(size_t real) { return real == sizeof(float) ? 256 : 12
 // registers), "fused" (unblocked, full storage), "blocked" (panels, full storage),
 // "lower" (panels, lower storage, tiled symv), "hybrid" (lower for the leading
 // columns, resident for the trailing block it fits; slower than lower for complex
-// double at n = 256, kept selectable). Unset or "auto": resident where n fits it,
-// else lower. Read at every launch so one process can compare them.
+// double at n = 256). Unset or "auto": resident where n fits it, else hybrid for
+// complex float and lower for complex double. Read at every launch so one process can
+// compare them.
 enum { kTrdAuto = 0, kTrdFused, kTrdBlocked, kTrdLower, kTrdResident, kTrdHybrid };
 static int hetrd_variant() {
     const char *e = getenv("RLM_HETRD");
@@ -2977,7 +2978,13 @@ static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, doubl
     if (v == kTrdAuto) {
         if (sizeof(T) == sizeof(double) && n <= 72)
             return launch_zhetrd_small_t<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
-        v = (sizeof(T) == sizeof(double) && n <= 96) ? kTrdFused : n <= rmax ? kTrdResident : kTrdLower;
+        // complex float n > 256: panels for the leading n - 256 columns, then the trailing
+        // 256 x 256 block in registers (n = 384: 143 -> 99 ms, n = 512: 290 -> 248 ms for
+        // 4624 matrices against panels throughout)
+        v = (sizeof(T) == sizeof(double) && n <= 96) ? kTrdFused
+            : n <= rmax                              ? kTrdResident
+            : sizeof(T) == sizeof(float)             ? kTrdHybrid
+                                                     : kTrdLower;
     }
     if ((v == kTrdResident || v == kTrdHybrid) && n <= rmax)
         return launch_resident_any<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, nullptr, 0);
@@ -2991,9 +2998,13 @@ static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, doubl
         return launch_resident_any<T>(A + 2 * ((long)ncols * lda + ncols), strideA, lda, n - ncols, D + ncols,
                                       E + ncols, TAU + 2 * ncols, strideV, batch, stream, nullptr, 0);
     }
-    // the panel kernels keep W (n x 16) in LDS: beyond 160 KB fall back to the fused kernel
+    // the panel kernels keep W (n x NB) in LDS: panels of 8 where 16 do not fit (complex
+    // double n = 512: 148 KB; 448 against 1848 ms fused for 4624 matrices), beyond 160 KB
+    // the fused kernel
     if ((v == kTrdLower || v == kTrdResident) && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024)
         return launch_zhetrd_lower_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+    if ((v == kTrdLower || v == kTrdResident) && lower_lds_bytes(n, 8, sizeof(T)) <= 160 * 1024)
+        return launch_zhetrd_lower_t<T, 8>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
     if (v == kTrdBlocked && blocked_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024)
         return launch_zhetrd_blocked_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
     return launch_zhetrd_t<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
