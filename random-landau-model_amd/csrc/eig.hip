@@ -960,11 +960,10 @@ static bool resident_deal(int TN, int NS, bool diag0, int NL, int nw, ResidentDe
 
 // Tile layouts of the register-resident kernel (its TL parameter). Each gives the
 // register storage of one 32x32 tile (R, C) and, for it: the load; the pending rank-2
-// update of step i-1 (update: a pass of its own over every live tile before the
-// products when kSplit, else fused into product); the product pass into the wave's
-// partial y, which also captures column i1 = i + 1 into colb; the same for a tile of
-// tile column 0 held in LDS (load_lds, col0_lds, step_lds: entry k of lane l at
-// p[k * 64], p = tile base + l).
+// update of step i-1 (update; a no-op where product applies it itself); the product
+// pass into the wave's partial y, which also captures column i1 = i + 1 into colb; the
+// same for a tile of tile column 0 held in LDS (load_lds, col0_lds, step_lds: entry k of
+// lane l at p[k * 64], p = tile base + l).
 
 // 4x4 sub-blocks: lane (lrg, lcg) = (lane & 7, lane >> 3) holds tile rows 4 lrg..,
 // columns 4 lcg..; update and products in one VALU pass.
@@ -974,7 +973,6 @@ struct ResidentSub4 {
     struct Tile {
         CT a[16];
     };
-    static constexpr bool kSplit = false;
     template <typename F>
     static __device__ __forceinline__ void load(Tile &t, int R, int C, int lane, int n, F entry) {
         const int rb = R * 32 + 4 * (lane & 7), cb = C * 32 + 4 * (lane >> 3);
@@ -1060,18 +1058,17 @@ __device__ __forceinline__ float reduce_scatter16(const float (&x)[16], int li) 
 // 16 q + li. The pending rank-2 update A -= v w^H + w v^H is a real rank-4 update of
 // each part,
 //   Re A -= [vr vi wr wi]_r [wr wi vr vi]_c^T,   Im A -= [vi -vr wi -wr]_r [wr wi vr vi]_c^T,
-// one 16x16x4 product per block and part; the update pass issues them for every live
-// tile before the VALU products read the results. Column sums reduce over lq (two lane
-// swaps), row sums over li (reduce_scatter16). Diagonal tiles (full Hermitian blocks)
-// add their part of y as column sums, y_c = sum_r conj(A(r, c)) v_r. Column blocks
-// q = 0 wholly left of the trailing block are skipped: their vectors are zero.
+// one 16x16x4 product per block and part, ahead of the VALU products of the same tile.
+// Column sums reduce over lq (two lane swaps), row sums over li (reduce_scatter16).
+// Diagonal tiles (full Hermitian blocks) add their part of y as column sums,
+// y_c = sum_r conj(A(r, c)) v_r. Column blocks q = 0 wholly left of the trailing block
+// are skipped: their vectors are zero.
 struct ResidentMfma {
     using CT = cx<float>;
     typedef float v4 __attribute__((ext_vector_type(4)));
     struct Tile {
         v4 re[2][2], im[2][2];
     };
-    static constexpr bool kSplit = true;
     template <typename F>
     static __device__ __forceinline__ void load(Tile &t, int R, int C, int lane, int n, F entry) {
         const int li = lane & 15, lq = lane >> 4;
@@ -1390,17 +1387,15 @@ __global__ __launch_bounds__(512) void zhetrd_resident_kernel(T *__restrict__ Ab
                     if (r > i && r < n) stx(A, (long)i * lda + r, r == i1 ? CT{se[i], (T)0} : vn[r]);
                 }
             }
-            if constexpr (TL::kSplit) {
-#pragma unroll
-                for (int s = 0; s < NS; s++) {
-                    if (tR[s] < 0 || tC[s] * TS + TS - 1 < i1) continue;   // wave-uniform
-                    TL::update(a[s], DIAG0 ? s == 0 : tR[s] == tC[s], tR[s], tC[s], lane, i1, vo, wo);
-                }
-            }
+            // per live tile, its update then its products (one basic block per tile: the other
+            // wave of the SIMD fills the VALU while this one's update runs; 111.3 -> 107.8 ms at
+            // n = 256 against issuing every tile's update first)
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 if (tR[s] < 0 || tC[s] * TS + TS - 1 < i1) continue;   // wave-uniform
-                TL::product(a[s], DIAG0 ? s == 0 : tR[s] == tC[s], tR[s], tC[s], lane, i1, vn, vo, wo, yw, colb);
+                const bool dg = DIAG0 ? s == 0 : tR[s] == tC[s];
+                TL::update(a[s], dg, tR[s], tC[s], lane, i1, vo, wo);
+                TL::product(a[s], dg, tR[s], tC[s], lane, i1, vn, vo, wo, yw, colb);
             }
             if (NL > 0 && lR >= 0 && i1 < TS) TL::step_lds(lsub, lR, lane, i1, vn, vo, wo, yw, colb);
             __syncthreads();

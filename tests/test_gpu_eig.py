@@ -146,18 +146,24 @@ def _check_mixed(H, w, U, tol_vec=3e-5):
         assert np.abs(O).max() <= tol_vec * sc, (k, np.abs(O).max())
 
 
-@pytest.mark.parametrize("variant", ["auto", "lower", "fused"])
+@pytest.mark.parametrize("variant", ["auto", "valu", "lower", "fused"])
 @pytest.mark.parametrize("n,b", [(1, 2), (2, 3), (3, 3), (31, 3), (33, 3), (64, 2), (96, 2), (97, 2), (128, 3),
-                                 (129, 2), (160, 2), (161, 2), (200, 2), (224, 2), (225, 2), (256, 4)])
+                                 (129, 2), (160, 2), (161, 2), (200, 2), (224, 2), (225, 2), (256, 4), (300, 2),
+                                 (384, 2)])
 def test_mixed_precision(monkeypatch, variant, n, b):
     """HIP_DIAG_MIXED: FP32 tridiagonalisation (register-resident kernel for n <= 256 under
     "auto", with 1, 2 or 4 register tiles per wave and LDS tiles where 8 waves x slots fall
-    short) and back-transform, FP64 D&C and Rayleigh quotients."""
+    short; its rank-2 update on the matrix cores, or on the VALU under "valu"
+    (RLM_RESIDENT_VALU); above 256 panels, then the register-resident kernel on the trailing
+    256 block) and back-transform, FP64 D&C and Rayleigh quotients."""
     import torch
     from rlm_amd import heevd_batched, HIP_DIAG_MIXED
     print(f"[test_mixed_precision] {variant} n={n} b={b}", flush=True)
-    if variant == "auto":
+    monkeypatch.delenv("RLM_RESIDENT_VALU", raising=False)
+    if variant in ("auto", "valu"):
         monkeypatch.delenv("RLM_HETRD", raising=False)
+        if variant == "valu":
+            monkeypatch.setenv("RLM_RESIDENT_VALU", "1")
     else:
         monkeypatch.setenv("RLM_HETRD", variant)
     rng = np.random.default_rng(11 * n + b)
