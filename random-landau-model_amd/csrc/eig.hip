@@ -2414,7 +2414,7 @@ __global__ __launch_bounds__(NT) void backtransform_kernel(const T *__restrict__
 }
 
 // ----------------------------------------------------------------------------
-// Back-transform on the matrix cores (n <= 256), with the MFMA helpers Mf<T> of
+// Back-transform on the matrix cores (n <= 256; complex float n <= 512), with the MFMA helpers Mf<T> of
 // common.hpp.
 // Element (row, k) of the reflector block starting at reflector j0 (zhetrd lower
 // storage: unit at row j+1, A(j+2:n, j) below, zero above; zero for j >= n-1).
@@ -2497,7 +2497,7 @@ __global__ __launch_bounds__(256) void larft_kernel(const T *__restrict__ Abase,
 // W2 = T W, U -= V W2. Workgroups are mapped so that the slabs of one matrix share
 // blockIdx % 8, i.e. one XCD and its L2, which then serves V and T to all of them.
 template <typename T, int CW, int NWV>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void backtransform_mfma_kernel(const T *__restrict__ Abase, long strideA, int lda,
+__global__ __launch_bounds__(NWV > 4 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2))) void backtransform_mfma_kernel(const T *__restrict__ Abase, long strideA, int lda,
                                                                  int n, const T *__restrict__ Tin, long strideT,
                                                                  T *__restrict__ Uout, long strideU, int nslab,
                                                                  int batch) {
@@ -2856,9 +2856,26 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
         RLM_HIP_CHECK(hipGetLastError());
         return 0;
     }
+    // complex float 256 < n <= 512: the same slabs with eight waves, each still holding four
+    // 16-row blocks (4624 matrices, n = 384 / 512: 40.8 / 69.2 ms against 51.6 / 108.6 ms
+    // streaming; complex double, at 256 VGPRs, loses: 102 / 207 against 73 / 161 ms)
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (n <= 512 && tws) {
+            const long strideT = backtransform_tws_complex(n);
+            hipLaunchKernelGGL((larft_kernel<T>), dim3(batch), dim3(256), 0, stream, A, strideA, lda, n, TAU, strideV, tws,
+                               strideT, batch, 1);
+            RLM_HIP_CHECK(hipGetLastError());
+            const int nslab = (n + 31) / 32;
+            const long groups = (batch + 7) / 8;
+            hipLaunchKernelGGL((backtransform_mfma_kernel<T, 32, 8>), dim3((unsigned)(groups * 8 * nslab)), dim3(512), 0,
+                               stream, A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
+            RLM_HIP_CHECK(hipGetLastError());
+            return 0;
+        }
+    }
     if (tws && lda == n) {
-        // 256 < n: T factors, then the streaming matrix-core back-transform (the slab no
-        // longer fits the accumulators)
+        // T factors, then the streaming matrix-core back-transform (the slab no longer fits
+        // the accumulators)
         const long strideT = backtransform_tws_complex(n);
         hipLaunchKernelGGL((larft_kernel<T>), dim3(batch), dim3(256), 0, stream, A, strideA, lda, n, TAU, strideV, tws,
                            strideT, batch, 1);
