@@ -1177,6 +1177,15 @@ struct ResidentMfma {
     template <int Q0, bool DG>
     static __device__ __forceinline__ void product_q(Tile &t, int R, int C, int lane, const CT *vn, CT *yw) {
         const int li = lane & 15, lq = lane >> 4;
+        // the partial-y entries this tile adds to, read before any of its stores: the row
+        // band R and the column band C of an off-diagonal tile are disjoint, so both reads
+        // issue at once instead of the column read waiting for the row store
+        float *yf = reinterpret_cast<float *>(yw);
+        const int e = 2 * (R * 32 + 16 * (li >> 3) + 4 * lq + ((li >> 1) & 3)) + (li & 1);
+        const float yr0 = DG ? 0.f : yf[e];
+        CT yc0[2];
+#pragma unroll
+        for (int q = Q0; q < 2; q++) yc0[q] = yw[C * 32 + 16 * q + li];
         CT vr[2][4];
 #pragma unroll
         for (int p = 0; p < 2; p++)
@@ -1200,10 +1209,7 @@ struct ResidentMfma {
                     x[(p * 4 + u) * 2] = rs.x;
                     x[(p * 4 + u) * 2 + 1] = rs.y;
                 }
-            const float s = reduce_scatter16(x, li);
-            float *yf = reinterpret_cast<float *>(yw);
-            const int e = 2 * (R * 32 + 16 * (li >> 3) + 4 * lq + ((li >> 1) & 3)) + (li & 1);
-            yf[e] += s;
+            yf[e] = yr0 + reduce_scatter16(x, li);
         }
 #pragma unroll
         for (int q = Q0; q < 2; q++) {
@@ -1214,8 +1220,7 @@ struct ResidentMfma {
                 for (int u = 0; u < 4; u++) fmacx(cs, CT{t.re[p][q][u], t.im[p][q][u]}, vr[p][u]);
             cs.x = sum_xor32(sum_xor16(cs.x));
             cs.y = sum_xor32(sum_xor16(cs.y));
-            const int c = C * 32 + 16 * q + li;
-            yw[c] = yw[c] + cs;              // the 4 lanes of a column hold the same sum
+            yw[C * 32 + 16 * q + li] = yc0[q] + cs;   // the 4 lanes of a column hold the same sum
         }
     }
     static __device__ __forceinline__ void product(Tile &t, bool diag, int R, int C, int lane, int i1, const CT *vn,
