@@ -85,6 +85,16 @@ int rlm_prepare_coeff(double *coeff_mn, double *coeff_m_theta, double *coeff_jm,
 int rlm_generate_disorder_potential(double *x, double *y, double *intensity, long impurity_num, double L1, double L2,
                                     unsigned long seed, long sample_offset);
 
+/* The same stream chain generated on the device for samples sample_offset ..
+ * sample_offset + n_samples - 1 (sample k's values at d_x + k*impurity_num,
+ * d_y + k*impurity_num, d_intensity + k*impurity_num): one workgroup per sample
+ * seeds the MT19937 window, jumps it by 3*impurity_num*sample words (F2
+ * jump-ahead polynomial t^J mod phi, cost O(log J) on the host and O(19937)
+ * on the device, independent of the sample index) and generates its words.
+ * Bit-identical to rlm_generate_disorder_potential. Synchronous. */
+int hip_generate_disorder_potential(double *d_x, double *d_y, double *d_intensity, long impurity_num, double L1,
+                                    double L2, unsigned long seed, long sample_offset, int n_samples);
+
 /* Replaces prepare_potential_coeff (rlm/potential_coeff.h,
  * potential_coeff.cu:21-26) in FP64 with the CPU build's sign convention
  * (get_potential_coeff, matrix_coefficients.cpp:27-44):
@@ -113,10 +123,13 @@ int hip_set_hamil_matrix(const double *d_coeff_m_theta_1, const double *d_coeff_
 typedef struct rlm_plan_s *rlm_plan_t;
 int rlm_plan_create(rlm_plan_t *plan, long n_phi, double quanta_concentration, double impurity_concentration,
                     long n_mesh, long off_override, unsigned long seed, int samples_per_batch, int precision);
-/* Generates samples first..first+count-1 (count <= samples_per_batch) on the
- * host and uploads them. */
+/* Selects samples first..first+count-1 (count <= samples_per_batch) for the
+ * next execute. Their disorder is generated on the device by rlm_plan_execute
+ * from the seed (the jump to the first sample is computed here on the host,
+ * O(log first) polynomial work that does not depend on the seed). */
 int rlm_plan_load(rlm_plan_t plan, long first_sample, int count);
-/* Runs every GPU stage for the loaded samples on the plan's stream. */
+/* Runs every GPU stage for the loaded samples on the plan's stream: disorder
+ * streams, V_mn, assembly, eigensolve, links, plaquettes and energies. */
 int rlm_plan_execute(rlm_plan_t plan);
 /* Copies energies / Chern values / plaquette margins ([count][n_phi], host)
  * after synchronising the stream. margin may be NULL. */
@@ -131,6 +144,9 @@ int rlm_plan_run(rlm_plan_t plan, long first_sample, long n_samples, double *ene
 int rlm_plan_stage_times(rlm_plan_t plan, double *ms);
 /* The plan's hipStream_t. */
 void *rlm_plan_stream(rlm_plan_t plan);
+/* Samples of the current batch (the count of the last rlm_plan_load, also inside
+ * rlm_plan_run): rlm_plan_fetch writes loaded * n_phi values per output. */
+int rlm_plan_loaded(rlm_plan_t plan);
 /* Sizes derived as rlm/main_rlm_cpu.cpp:29-34: out[0]=n_phi, [1]=impurity_num,
  * [2]=off_head, [3]=dim_m, [4]=n_mesh, [5]=samples_per_batch. */
 int rlm_plan_info(rlm_plan_t plan, long *out6);
@@ -177,10 +193,18 @@ int hlm_plan_run(hlm_plan_t plan, long first_sample, long n_samples, double *ene
  * Returns 7. */
 int hlm_plan_stage_times(hlm_plan_t plan, double *ms);
 void *hlm_plan_stream(hlm_plan_t plan);
+/* Samples of the current batch, as rlm_plan_loaded. */
+int hlm_plan_loaded(hlm_plan_t plan);
 int hlm_plan_destroy(hlm_plan_t plan);
 /* hlm/main_hlm_cpu.cpp:66-67,79: samples first..first+n_samples-1 of the run's
  * single stream, n_sites values each, scaled to [-W/2, W/2). Host memory. */
 int hlm_generate_disorder(double *V, long n_sites, long n_samples, unsigned long seed, long first_sample, double W);
+/* The same values generated on the device into d_V (n_samples x n_sites), one
+ * workgroup per sample jumping to word n_sites*(first_sample + k) of the stream
+ * (as hip_generate_disorder_potential). hlm plans generate their samples this
+ * way at every hlm_plan_execute. Synchronous. */
+int hip_hlm_generate_disorder(double *d_V, long n_sites, long n_samples, unsigned long seed, long first_sample,
+                              double W);
 
 #ifdef __cplusplus
 }

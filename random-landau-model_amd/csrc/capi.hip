@@ -17,6 +17,7 @@
 #include "host_rng.hpp"
 #include "refine.hpp"
 #include "rlm_kernels.hpp"
+#include "rng.hpp"
 
 using namespace rlm;
 
@@ -154,6 +155,64 @@ struct DevBuf {
         bytes = 0;
     }
     double *d() { return static_cast<double *>(p); }
+};
+
+// Device copy of the jumps of a run's disorder streams (rng.hip): the jump of each
+// sample of a batch relative to the batch's first sample (B, fixed at creation) and the
+// jump from the seed's initial window to the batch's first sample (A, per batch).
+struct StreamJumps {
+    long stride = 0;   // words per sample
+    MtJumpTable b;
+    DevBuf dB, dBp, dA;
+    long a_first = -1;
+    int a_kind = 0, a_mono = 0;
+    int create(long count, long stride_words) {
+        if (count < 1 || stride_words < 1) return -22;
+        if (!mt_jump_ready()) return -1;
+        stride = stride_words;
+        b.set(count);
+        for (long k = 0; k < count; k++) b.fill(k, (uint64_t)k * (uint64_t)stride);
+        int rc;
+        if ((rc = dB.ensure(sizeof(int) * 2 * (size_t)count)) || (rc = dBp.ensure(8 * b.poly.size())) ||
+            (rc = dA.ensure(8 * (size_t)mtj::kPolyWords)))
+            return rc;
+        std::vector<int> km(2 * count);
+        std::copy(b.kind.begin(), b.kind.end(), km.begin());
+        std::copy(b.mono.begin(), b.mono.end(), km.begin() + count);
+        RLM_HIP_CHECK(hipMemcpy(dB.p, km.data(), sizeof(int) * km.size(), hipMemcpyHostToDevice));
+        RLM_HIP_CHECK(hipMemcpy(dBp.p, b.poly.data(), 8 * b.poly.size(), hipMemcpyHostToDevice));
+        return 0;
+    }
+    // jump A to sample `first`; a new polynomial is copied once the stream's earlier
+    // work (which may still read the old one) has finished
+    int select(long first, hipStream_t s) {
+        if (first < 0) return -22;
+        if (first == a_first) return 0;
+        MtJumpTable a;
+        a.set(1);
+        a.fill(0, (uint64_t)first * (uint64_t)stride);
+        if (a.kind[0] == 2) {
+            RLM_HIP_CHECK(hipStreamSynchronize(s));
+            RLM_HIP_CHECK(hipMemcpy(dA.p, a.poly.data(), 8 * (size_t)mtj::kPolyWords, hipMemcpyHostToDevice));
+        }
+        a_kind = a.kind[0];
+        a_mono = a.mono[0];
+        a_first = first;
+        return 0;
+    }
+    int launch(uint32_t seed, int count, int mode, long words, long n_sec, double s0, double s1, double *o0,
+               double *o1, double *o2, hipStream_t s) const {
+        if (count > b.count) return -22;
+        const int *kb = static_cast<const int *>(dB.p);
+        return launch_mt_streams(seed, a_kind, a_mono, static_cast<const uint64_t *>(dA.p), kb, kb + b.count,
+                                 static_cast<const uint64_t *>(dBp.p), count, words, mode, n_sec, s0, s1, o0, o1, o2, s);
+    }
+    void release() {
+        dB.release();
+        dBp.release();
+        dA.release();
+        a_first = -1;
+    }
 };
 
 // n above this goes to the multi-workgroup solver (eig_large.hip)
@@ -425,6 +484,34 @@ extern "C" int rlm_generate_disorder_potential(double *x, double *y, double *int
     return 0;
 }
 
+extern "C" int hip_generate_disorder_potential(double *d_x, double *d_y, double *d_intensity, long impurity_num,
+                                               double L1, double L2, unsigned long seed, long sample_offset,
+                                               int n_samples) {
+    if (!d_x || !d_y || !d_intensity || impurity_num < 0 || sample_offset < 0 || n_samples < 0) return -22;
+    if (impurity_num == 0 || n_samples == 0) return 0;
+    StreamJumps j;
+    int rc = j.create(n_samples, 3 * impurity_num);
+    if (!rc) rc = j.select(sample_offset, nullptr);
+    if (!rc) rc = j.launch((uint32_t)seed, n_samples, 0, 3 * impurity_num, impurity_num, L1, L2, d_x, d_y, d_intensity,
+                           nullptr);
+    if (!rc && hipStreamSynchronize(nullptr) != hipSuccess) rc = -1;
+    j.release();
+    return rc;
+}
+
+extern "C" int hip_hlm_generate_disorder(double *d_V, long n_sites, long n_samples, unsigned long seed,
+                                         long first_sample, double W) {
+    if (!d_V || n_sites < 1 || n_samples < 0 || first_sample < 0 || n_samples > (1L << 30)) return -22;
+    if (n_samples == 0) return 0;
+    StreamJumps j;
+    int rc = j.create(n_samples, n_sites);
+    if (!rc) rc = j.select(first_sample, nullptr);
+    if (!rc) rc = j.launch((uint32_t)seed, (int)n_samples, 1, n_sites, 0, 0.5 * W, 0.0, d_V, nullptr, nullptr, nullptr);
+    if (!rc && hipStreamSynchronize(nullptr) != hipSuccess) rc = -1;
+    j.release();
+    return rc;
+}
+
 extern "C" int hip_prepare_potential_coeff(const double *d_x, const double *d_y, const double *d_intensity,
                                            double *d_Vmn, const double *d_coeff_mn, int dim_m, int dim_n,
                                            int off_head, long impurity_num) {
@@ -473,7 +560,9 @@ struct rlm_plan_s {
     int precision;
     hipStream_t stream = nullptr;
     // host
-    std::vector<double> cmn, cmt, cjm, hx, hy, hs;
+    std::vector<double> cmn, cmt, cjm;
+    // disorder streams, generated on the device at every execute
+    StreamJumps jumps;
     // device
     DevBuf dcmn, dcmt, dE, dX, dY, dS, dV, dW, dH, dA, dU, dEig, dL, dOut, dA32, dResid;
     // FP64 refinement of the mixed-precision eigenvectors (refine.hip): on by default for
@@ -556,6 +645,7 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
          (rc = p->dQg.ensure(16 * nkp * N * refine_max_cluster())) || (rc = p->dCl.ensure(4 * nkp * N)) ||
          (rc = p->dFlag.ensure(4 * nkp))))
         return fail(rc);
+    if ((rc = p->jumps.create(S, 3 * p->n_imp))) return fail(rc);
     if (hipMemcpy(p->dcmn.p, p->cmn.data(), 8 * p->cmn.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->dcmt.p, p->cmt.data(), 8 * p->cmt.size(), hipMemcpyHostToDevice) != hipSuccess)
         return fail(-1);
@@ -574,17 +664,8 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
 
 extern "C" int rlm_plan_load(rlm_plan_t p, long first_sample, int count) {
     if (!p || count < 1 || count > p->spb || first_sample < 0) return -22;
-    const long ni = p->n_imp;
-    p->hx.resize(count * ni);
-    p->hy.resize(count * ni);
-    p->hs.resize(count * ni);
-    for (int s = 0; s < count; s++)
-        host_rlm_disorder(p->hx.data() + s * ni, p->hy.data() + s * ni, p->hs.data() + s * ni, ni, p->L1, p->L2,
-                          p->seed, first_sample + s);
-    RLM_HIP_CHECK(hipMemcpyAsync(p->dX.p, p->hx.data(), 8 * count * ni, hipMemcpyHostToDevice, p->stream));
-    RLM_HIP_CHECK(hipMemcpyAsync(p->dY.p, p->hy.data(), 8 * count * ni, hipMemcpyHostToDevice, p->stream));
-    RLM_HIP_CHECK(hipMemcpyAsync(p->dS.p, p->hs.data(), 8 * count * ni, hipMemcpyHostToDevice, p->stream));
-    RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+    int rc;
+    if ((rc = p->jumps.select(first_sample, p->stream))) return rc;
     p->loaded = count;
     p->first = first_sample;
     return 0;
@@ -599,6 +680,10 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     hipStream_t s = p->stream;
     int rc;
     RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
+    // disorder of the loaded samples from the seed (rlm/disorder_potential.cpp:3-21)
+    if ((rc = p->jumps.launch(p->seed, S, 0, 3 * p->n_imp, p->n_imp, p->L1, p->L2, p->dX.d(), p->dY.d(), p->dS.d(),
+                              s)))
+        return rc;
     if ((rc = launch_potential(p->dX.d(), p->dY.d(), p->dS.d(), p->n_imp, p->dcmn.d(), d, off, p->dV.d(), S, s)))
         return rc;
     if ((rc = launch_wband(p->dV.d(), p->dcmt.d(), N, d, off, g, S, p->dW.d(), s))) return rc;
@@ -706,6 +791,8 @@ extern "C" int rlm_plan_run(rlm_plan_t p, long first_sample, long n_samples, dou
 
 extern "C" void *rlm_plan_stream(rlm_plan_t p) { return p ? (void *)p->stream : nullptr; }
 
+extern "C" int rlm_plan_loaded(rlm_plan_t p) { return p ? p->loaded : -22; }
+
 extern "C" int rlm_plan_info(rlm_plan_t p, long *o) {
     if (!p || !o) return -22;
     o[0] = p->n_phi;
@@ -784,6 +871,7 @@ extern "C" int rlm_plan_destroy(rlm_plan_t p) {
                       &p->dCl,  &p->dFlag,  &p->dPair};
     for (DevBuf *b : bufs) b->release();
     p->ws.release();
+    p->jumps.release();
     for (int k = 0; k <= rlm_plan_s::kStages; k++)
         if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -797,11 +885,12 @@ extern "C" int rlm_plan_destroy(rlm_plan_t p) {
 extern "C" int hlm_generate_disorder(double *V, long n_sites, long n_samples, unsigned long seed, long first_sample,
                                      double W) {
     if (!V || n_sites < 1 || n_samples < 0 || first_sample < 0) return -22;
-    Mt19937 gen((uint32_t)seed);
-    gen.discard((uint64_t)n_sites * (uint64_t)first_sample);
+    uint32_t win[mtj::kN];
+    mtj::seed_window((uint32_t)seed, (uint64_t)n_sites * (uint64_t)first_sample, win);
+    mtj::WindowGen gen(win);
     const double half = 0.5 * W;
     for (long i = 0; i < n_sites * n_samples; i++) {
-        const double u = gen.uniform(-1.0, 1.0);   // hlm/main_hlm_cpu.cpp:79
+        const double u = mt_uniform(gen, -1.0, 1.0);   // hlm/main_hlm_cpu.cpp:79
         V[i] = (W == 2.0) ? u : half * u;
     }
     return 0;
@@ -814,7 +903,7 @@ struct hlm_plan_s {
     int precision;
     long ntw;                       // (g+1)*g twist points per sample
     hipStream_t stream = nullptr;
-    std::vector<double> hV;
+    StreamJumps jumps;   // the run's single disorder stream, n_sites words per sample
     DevBuf dPhi, dM1, dM2, dV, dH, dU, dEig, dL, dOut;
     EigWork ws;
     int loaded = 0;
@@ -849,7 +938,8 @@ extern "C" int hlm_plan_create(hlm_plan_t *plan, int lx, int ly, int Q, int band
         (rc = p->dM2.ensure(16 * ntw * N * N)) || (rc = p->dV.ensure(8 * S * n_sites)) ||
         (rc = p->dH.ensure(16 * nkp * N * N)) || (rc = p->dU.ensure(16 * nkp * N * N)) ||
         (rc = p->dEig.ensure(8 * nkp * N)) || (rc = p->dL.ensure(32 * nkp * N)) ||
-        (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)))
+        (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)) ||
+        (rc = p->jumps.create(S, n_sites)))
         return fail(rc);
     // Bloch basis for every twist (disorder independent)
     {
@@ -892,11 +982,8 @@ extern "C" int hlm_plan_create(hlm_plan_t *plan, int lx, int ly, int Q, int band
 
 extern "C" int hlm_plan_load(hlm_plan_t p, long first_sample, int count) {
     if (!p || count < 1 || count > p->spb || first_sample < 0) return -22;
-    p->hV.resize((size_t)count * p->n_sites);
-    int rc = hlm_generate_disorder(p->hV.data(), p->n_sites, count, p->seed, first_sample, p->W);
-    if (rc) return rc;
-    RLM_HIP_CHECK(hipMemcpyAsync(p->dV.p, p->hV.data(), 8 * p->hV.size(), hipMemcpyHostToDevice, p->stream));
-    RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+    int rc;
+    if ((rc = p->jumps.select(first_sample, p->stream))) return rc;
     p->loaded = count;
     return 0;
 }
@@ -908,6 +995,8 @@ extern "C" int hlm_plan_execute(hlm_plan_t p) {
     hipStream_t s = p->stream;
     int rc;
     RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
+    // on-site disorder of the loaded samples (hlm/main_hlm_cpu.cpp:79, width W)
+    if ((rc = p->jumps.launch(p->seed, S, 1, ns, 0, 0.5 * p->W, 0.0, p->dV.d(), nullptr, nullptr, s))) return rc;
     // Htr[(s*ntw + tw)] = Phi(tw)^H diag(V_s) Phi(tw), upper triangle
     if ((rc = launch_hlm_project(p->dPhi.d(), p->dV.d(), N, ns, S, ntw, p->dH.d(), s))) return rc;
     // the reference's zheevd reads the upper triangle of Htr (mkl_diag.cpp:6)
@@ -971,12 +1060,15 @@ extern "C" int hlm_plan_stage_times(hlm_plan_t p, double *ms) {
 
 extern "C" void *hlm_plan_stream(hlm_plan_t p) { return p ? (void *)p->stream : nullptr; }
 
+extern "C" int hlm_plan_loaded(hlm_plan_t p) { return p ? p->loaded : -22; }
+
 extern "C" int hlm_plan_destroy(hlm_plan_t p) {
     if (!p) return 0;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     DevBuf *bufs[] = {&p->dPhi, &p->dM1, &p->dM2, &p->dV, &p->dH, &p->dU, &p->dEig, &p->dL, &p->dOut};
     for (DevBuf *b : bufs) b->release();
     p->ws.release();
+    p->jumps.release();
     for (int k = 0; k <= hlm_plan_s::kStages; k++)
         if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
     if (p->stream) (void)hipStreamDestroy(p->stream);

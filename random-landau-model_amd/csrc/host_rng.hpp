@@ -5,92 +5,33 @@
 // (rlm/disorder_potential.cpp:5, hlm/main_hlm_cpu.cpp:67), seeds it with
 // init_by_array({seed mod 2^32}); vdRngUniform(UNIFORM_STD, a, b) maps one
 // 32-bit word w to a + (b-a) * w * 2^-32; viRngUniformBits returns the raw
-// words; vslSkipAheadStream(n) skips n words.
+// words; vslSkipAheadStream(n) skips n words (here by the jump-ahead of
+// mt_jump.hpp, whose cost grows with log n).
 #pragma once
 #include <cmath>
 #include <cstdint>
 
+#include "mt_jump.hpp"
+
 namespace rlm {
 
-class Mt19937 {
-  public:
-    explicit Mt19937(uint32_t seed) { init_by_array(&seed, 1); }
+// vdRngUniform(VSL_RNG_METHOD_UNIFORM_STD, a, b) of one word
+inline double mt_uniform(mtj::WindowGen &g, double a, double b) { return a + (b - a) * ((double)g.next() * 0x1p-32); }
 
-    uint32_t next() {
-        if (mti_ >= kN) twist();
-        uint32_t y = mt_[mti_++];
-        y ^= (y >> 11);
-        y ^= (y << 7) & 0x9d2c5680u;
-        y ^= (y << 15) & 0xefc60000u;
-        y ^= (y >> 18);
-        return y;
-    }
-    void discard(uint64_t n) {
-        // whole blocks of 624 words are skipped by regenerating the state
-        while (n > 0) {
-            if (mti_ >= kN) twist();
-            const uint64_t avail = (uint64_t)(kN - mti_);
-            const uint64_t take = n < avail ? n : avail;
-            mti_ += (int)take;
-            n -= take;
-        }
-    }
-    double uniform(double a, double b) { return a + (b - a) * ((double)next() * 0x1p-32); }
-
-  private:
-    static constexpr int kN = 624, kM = 397;
-    uint32_t mt_[kN];
-    int mti_ = kN + 1;
-
-    void init_genrand(uint32_t s) {
-        mt_[0] = s;
-        for (int i = 1; i < kN; i++) mt_[i] = 1812433253u * (mt_[i - 1] ^ (mt_[i - 1] >> 30)) + (uint32_t)i;
-        mti_ = kN;
-    }
-    void init_by_array(const uint32_t *key, int len) {
-        init_genrand(19650218u);
-        int i = 1, j = 0;
-        for (int k = (kN > len ? kN : len); k; k--) {
-            mt_[i] = (mt_[i] ^ ((mt_[i - 1] ^ (mt_[i - 1] >> 30)) * 1664525u)) + key[j] + (uint32_t)j;
-            i++;
-            j++;
-            if (i >= kN) { mt_[0] = mt_[kN - 1]; i = 1; }
-            if (j >= len) j = 0;
-        }
-        for (int k = kN - 1; k; k--) {
-            mt_[i] = (mt_[i] ^ ((mt_[i - 1] ^ (mt_[i - 1] >> 30)) * 1566083941u)) - (uint32_t)i;
-            i++;
-            if (i >= kN) { mt_[0] = mt_[kN - 1]; i = 1; }
-        }
-        mt_[0] = 0x80000000u;
-        mti_ = kN;
-    }
-    void twist() {
-        static const uint32_t mag01[2] = {0u, 0x9908b0dfu};
-        int kk;
-        uint32_t y;
-        for (kk = 0; kk < kN - kM; kk++) {
-            y = (mt_[kk] & 0x80000000u) | (mt_[kk + 1] & 0x7fffffffu);
-            mt_[kk] = mt_[kk + kM] ^ (y >> 1) ^ mag01[y & 1u];
-        }
-        for (; kk < kN - 1; kk++) {
-            y = (mt_[kk] & 0x80000000u) | (mt_[kk + 1] & 0x7fffffffu);
-            mt_[kk] = mt_[kk + (kM - kN)] ^ (y >> 1) ^ mag01[y & 1u];
-        }
-        y = (mt_[kN - 1] & 0x80000000u) | (mt_[0] & 0x7fffffffu);
-        mt_[kN - 1] = mt_[kM - 1] ^ (y >> 1) ^ mag01[y & 1u];
-        mti_ = 0;
-    }
-};
-
-// rlm/disorder_potential.cpp:3-21 (one sample of the seed chain).
+// rlm/disorder_potential.cpp:3-21 (samples sample .. sample + count - 1 of the seed
+// chain; sample k + 1 starts where sample k ends, 3 * n_imp words later, so one
+// generator serves them all)
 inline void host_rlm_disorder(double *x, double *y, double *s, long n_imp, double L1, double L2, uint32_t seed,
-                              long sample) {
-    Mt19937 g(seed);
-    g.discard((uint64_t)n_imp * 3u * (uint64_t)sample);
-    for (long i = 0; i < n_imp; i++) x[i] = g.uniform(0.0, L1);
-    for (long i = 0; i < n_imp; i++) y[i] = g.uniform(0.0, L2);
-    for (long i = 0; i < n_imp; i++) s[i] = (g.next() % 2u == 0u) ? 1.0 : -1.0;
+                              long sample, long count = 1) {
+    uint32_t w[mtj::kN];
+    mtj::seed_window(seed, (uint64_t)n_imp * 3u * (uint64_t)sample, w);
+    mtj::WindowGen g(w);
+    for (long k = 0; k < count; k++) {
+        double *xk = x + k * n_imp, *yk = y + k * n_imp, *sk = s + k * n_imp;
+        for (long i = 0; i < n_imp; i++) xk[i] = mt_uniform(g, 0.0, L1);
+        for (long i = 0; i < n_imp; i++) yk[i] = mt_uniform(g, 0.0, L2);
+        for (long i = 0; i < n_imp; i++) sk[i] = (g.next() % 2u == 0u) ? 1.0 : -1.0;
+    }
 }
 
 // rlm/matrix_coefficients.cpp:3-25 prepare_coeff (double-precision build), with

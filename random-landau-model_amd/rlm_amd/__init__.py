@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
         "hip_cal_chern": ([dp, dp, i, i, i, i], i),
         "rlm_prepare_coeff": ([dp, dp, dp, i, i, i, i, i, d, d], i),
         "rlm_generate_disorder_potential": ([dp, dp, dp, l, d, d, ctypes.c_ulong, l], i),
+        "hip_generate_disorder_potential": ([dp, dp, dp, l, d, d, ctypes.c_ulong, l, i], i),
+        "hip_hlm_generate_disorder": ([dp, l, l, ctypes.c_ulong, l, d], i),
         "hip_prepare_potential_coeff": ([dp, dp, dp, dp, dp, i, i, i, l], i),
         "hip_set_hamil_matrix": ([dp, dp, dp, dp, dp, i, i, i], i),
         "rlm_plan_create": ([ctypes.POINTER(vp), l, d, d, l, l, ctypes.c_ulong, i, i], i),
@@ -71,6 +73,7 @@ def lib() -> ctypes.CDLL:
         "rlm_plan_max_residual": ([vp, dp], i),
         "rlm_plan_eigenvectors": ([vp], vp),
         "rlm_plan_refine_info": ([vp, ctypes.POINTER(ctypes.c_long)], i),
+        "rlm_plan_loaded": ([vp], i),
         "rlm_plan_destroy": ([vp], i),
         "hlm_plan_create": ([ctypes.POINTER(vp), i, i, i, i, i, ctypes.c_ulong, d, i, i], i),
         "hlm_plan_load": ([vp, l, i], i),
@@ -79,6 +82,7 @@ def lib() -> ctypes.CDLL:
         "hlm_plan_run": ([vp, l, l, dp, dp, dp], i),
         "hlm_plan_stage_times": ([vp, dp], i),
         "hlm_plan_stream": ([vp], vp),
+        "hlm_plan_loaded": ([vp], i),
         "hlm_plan_destroy": ([vp], i),
         "hlm_generate_disorder": ([dp, l, l, ctypes.c_ulong, l, d], i),
     }
@@ -164,11 +168,14 @@ class RlmPlan:
         info = (ctypes.c_long * 6)()
         _check(L.rlm_plan_info(self._h, info), "rlm_plan_info")
         self.n_phi, self.n_imp, self.off, self.d, self.n_mesh, self.spb = list(info)
-        self.loaded = 0
+
+    @property
+    def loaded(self) -> int:
+        """Samples of the plan's current batch (the library's count, which sizes fetch())."""
+        return int(lib().rlm_plan_loaded(self._h))
 
     def load(self, first_sample: int, count: int):
         _check(lib().rlm_plan_load(self._h, first_sample, count), "rlm_plan_load")
-        self.loaded = count
 
     def execute(self):
         _check(lib().rlm_plan_execute(self._h), "rlm_plan_execute")
@@ -236,6 +243,26 @@ def hlm_disorder(n_sites: int, n_samples: int, seed: int, first_sample: int = 0,
     return V.reshape(n_samples, n_sites)
 
 
+def device_disorder(n_imp: int, L1: float, L2: float, seed: int, first_sample: int, n_samples: int):
+    """hip_generate_disorder_potential -> numpy (x, y, s), each [n_samples, n_imp]."""
+    torch = _torch()
+    out = [torch.zeros(max(n_samples * n_imp, 1), dtype=torch.float64, device="cuda") for _ in range(3)]
+    torch.cuda.synchronize()
+    _check(lib().hip_generate_disorder_potential(out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(), n_imp, L1, L2,
+                                                 seed, first_sample, n_samples), "hip_generate_disorder_potential")
+    return tuple(t[:n_samples * n_imp].cpu().numpy().reshape(n_samples, n_imp) for t in out)
+
+
+def device_hlm_disorder(n_sites: int, n_samples: int, seed: int, first_sample: int = 0, W: float = 2.0):
+    """hip_hlm_generate_disorder -> numpy [n_samples, n_sites]."""
+    torch = _torch()
+    V = torch.zeros(max(n_samples * n_sites, 1), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    _check(lib().hip_hlm_generate_disorder(V.data_ptr(), n_sites, n_samples, seed, first_sample, W),
+           "hip_hlm_generate_disorder")
+    return V[:n_samples * n_sites].cpu().numpy().reshape(n_samples, n_sites)
+
+
 class HlmPlan:
     """hlm_plan_* (batched hlm pipeline)."""
 
@@ -246,11 +273,15 @@ class HlmPlan:
         _check(lib().hlm_plan_create(ctypes.byref(self._h), lx, ly, Q, band, n_mesh, seed, W, samples_per_batch,
                                      precision), "hlm_plan_create")
         self.n_phi = lx * ly // Q
-        self.loaded = 0
+        self.spb = samples_per_batch
+
+    @property
+    def loaded(self) -> int:
+        """Samples of the plan's current batch (the library's count, which sizes fetch())."""
+        return int(lib().hlm_plan_loaded(self._h))
 
     def load(self, first_sample: int, count: int):
         _check(lib().hlm_plan_load(self._h, first_sample, count), "hlm_plan_load")
-        self.loaded = count
 
     def execute(self):
         _check(lib().hlm_plan_execute(self._h), "hlm_plan_execute")

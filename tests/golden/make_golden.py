@@ -138,9 +138,63 @@ def write_oracle():
     print("wrote oracle_fixtures.json")
 
 
+def write_oracle_large():
+    """Oracle outputs at the full configuration shapes that the GPU tests compare against
+    without rerunning the CPU path: rlm n_phi=3000 g=4 seed 3 (samples 0-1), the uneven
+    batch's n_phi = 192/256/384/512 at g=16 seed 11 (sample 0), and the hlm 24x24 q=8 b=0
+    g=32 seed 1 configuration at disorder width W=1 (half of the reference's W=2 stream,
+    samples 0-3). Each sample row also carries max |C - round C| of the oracle."""
+    from oracle import oracle as O
+    path = os.path.join(HERE, "oracle_fixtures_large.json")
+    out = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            out = json.load(f)
+
+    def save():
+        with open(path, "w") as f:
+            json.dump(out, f)
+
+    def row(s, r):
+        c = np.asarray(r.chern)
+        return {"sample": s, "energy": r.energy.tolist(), "chern": c.tolist(),
+                "max_int_dev": float(np.abs(c - np.rint(c)).max())}
+
+    rlm_cases = [("rlm_n192_g16_s11", dict(n_phi=192, q=1.0, imp=16.0, n_mesh=16, seed=11), [0]),
+                 ("rlm_n256_g16_s11", dict(n_phi=256, q=1.0, imp=16.0, n_mesh=16, seed=11), [0]),
+                 ("rlm_n384_g16_s11", dict(n_phi=384, q=1.0, imp=16.0, n_mesh=16, seed=11), [0]),
+                 ("rlm_n512_g16_s11", dict(n_phi=512, q=1.0, imp=16.0, n_mesh=16, seed=11), [0]),
+                 ("rlm_n3000_g4_s3", dict(n_phi=3000, q=1.0, imp=16.0, n_mesh=4, seed=3), [0, 1])]
+    for name, kw, samples in rlm_cases:
+        have = {r["sample"] for r in out.get(name, {}).get("samples", [])}
+        for s in samples:
+            if s in have:
+                continue
+            r = O.rlm_sample(kw["n_phi"], kw["q"], kw["imp"], kw["n_mesh"], kw["seed"], s, threads=8)
+            out.setdefault(name, {"params": kw, "samples": []})["samples"].append(row(s, r))
+            print(name, s, "sum C =", r.chern.sum(), flush=True)
+            save()
+    name = "hlm_24x24_q8_b0_g32_s1_w1"
+    lx = ly = 24
+    Q, band, g, seed, S = 8, 0, 32, 1, 4
+    pots = 0.5 * O.hlm_disorder(seed, lx * ly, S)       # W=1: half of the W=2 stream, exactly
+    have = {r["sample"] for r in out.get(name, {}).get("samples", [])}
+    for s in range(S):
+        if s in have:
+            continue
+        r = O.hlm_sample(lx, ly, Q, band, g, pots[s], threads=8)
+        out.setdefault(name, {"params": dict(lx=lx, ly=ly, Q=Q, band=band, n_mesh=g, seed=seed, width=1.0),
+                              "samples": []})["samples"].append(row(s, r))
+        print(name, s, "sum C =", r.chern.sum(), flush=True)
+        save()
+    print("wrote oracle_fixtures_large.json")
+
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["mkl", "oracle"]
     if "mkl" in what:
         write_mkl()
     if "oracle" in what:
         write_oracle()
+    if "large" in what:
+        write_oracle_large()
