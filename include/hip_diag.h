@@ -64,8 +64,28 @@ int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA, double *d
 /* Replaces cal_Chern (diag_wrappers/chern.h:29, chern.cpp:3-35): for one
  * theta_1 line, chern_theta[m + n*n_mesh] = -arg(prod of the 4 link overlaps
  * of plaquette m for state n)/(2 pi), reading the 2-slot host ring buffer
- * wfs_full[(slot*(n_mesh+1) + theta_2)*n_phi*dim_vec + state*dim_vec + i]. */
+ * wfs_full[(slot*(n_mesh+1) + theta_2)*n_phi*dim_vec + state*dim_vec + i]
+ * (complex128; slot theta_1 % 2 holds line theta_1, the other slot line
+ * theta_1 - 1). The ring is copied to the device once; overlaps, products and
+ * phases run there (one wave per plaquette and state). Host chern_theta. */
 int hip_cal_chern(const double *wfs_full, double *chern_theta, int n_phi, int dim_vec, int n_mesh, int theta_1);
+
+/* Replaces write_wfs (diag_wrappers/wfs_file.h:11, wfs_file.cpp:2-12) of the
+ * -DwfsIO builds: writes `size` complex values of host wfs (complex128) raw, no
+ * header, to <dir>/wfs_<theta_1>_<theta_2>.bin (dir NULL or "": the working
+ * directory, as the reference), as complex128 (elem_bytes 16) or complex64
+ * (elem_bytes 8, the reference's FP32 builds). Callers pass theta_1 % 2, as
+ * rlm/main_rlm_gpu.cu:178,215 do. Returns -5 on an I/O error. */
+int hip_write_wfs(const char *dir, int theta_1, int theta_2, const double *wfs, unsigned long size, int elem_bytes);
+
+/* Replaces cal_Chern_wfs_IO (diag_wrappers/chern.h:30, chern.cpp:77-123): the
+ * theta_1 line's chern_theta[m + n*n_mesh] from the files
+ * <dir>/wfs_<slot>_<theta_2>.bin of both ring slots (theta_2 = 0..n_mesh, each
+ * n_phi state vectors of dim_vec components, complex128 or complex64 by
+ * elem_bytes), with the reference's factor order <p_{l+1}|p_l> and sign
+ * (+arg/2 pi), on the device. Returns -2 if a file is missing, -5 if short. */
+int hip_cal_chern_wfs_io(double *chern_theta, int n_phi, int dim_vec, int n_mesh, int theta_1, const char *dir,
+                         int elem_bytes);
 
 /* ------------------------------------------------------------------------ */
 /* rlm model entry points                                                    */
@@ -165,6 +185,17 @@ int rlm_plan_max_residual(rlm_plan_t plan, double *out);
  * (mixed plans without refinement). */
 const double *rlm_plan_eigenvalues(rlm_plan_t plan);
 const double *rlm_plan_eigenvectors(rlm_plan_t plan);
+/* -DwfsIO dump of the plan (rlm/main_rlm_gpu.cu:177-179,214-216 with
+ * diag_wrappers/wfs_file.cpp:2-12): after rlm_plan_execute, writes the
+ * eigenvectors of line theta_1 (0..n_mesh) of batch sample `sample` (0..loaded-1)
+ * as <dir>/wfs_<theta_1 % 2>_<theta_2>.bin for theta_2 = 0..n_mesh, each
+ * n_phi x n_phi complex values state-major (state n's components contiguous, the
+ * column-major eigenvector matrix the reference's solvers leave), raw, as
+ * complex128 (elem_bytes 16) or complex64 (elem_bytes 8). The theta = 2 pi points
+ * the plan derives are written from the exact relations rlm_plan_eigenvectors
+ * states. Writing lines theta_1 - 1 and theta_1 and calling hip_cal_chern_wfs_io
+ * reproduces the plan's plaquettes of that line. */
+int rlm_plan_write_wfs(rlm_plan_t plan, int sample, int theta_1, const char *dir, int elem_bytes);
 /* Mixed plans refine their FP32 eigenvectors in FP64 (one windowed Ogita-Aishima
  * step after exact Rayleigh-Ritz rotations of near-degenerate chunks) when n_phi > 512,
  * or as RLM_REFINE=0 / 1 in the environment at rlm_plan_create says. out2[0] = 1 if

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/hip_diag.h"
@@ -417,54 +418,84 @@ extern "C" int hip_heevd(double *A, double *w, int n) {
     return rc;
 }
 
+// Device copy of a 2-slot ring (cal_Chern and the wfs file variant): grown as needed,
+// shared by the calls of the process (serialised by g_ring_mu).
+std::mutex g_ring_mu;
+DevBuf g_ring_buf, g_ring_out;
+
+static int ring_chern_device(const double *host_ring, double *chern_theta, int n_phi, int dim_vec, int n_mesh,
+                             int theta_1, int io_order) {
+    const size_t ring_doubles = 2 * 2 * (size_t)(n_mesh + 1) * n_phi * dim_vec;
+    const size_t out_doubles = (size_t)n_mesh * n_phi;
+    std::lock_guard<std::mutex> lk(g_ring_mu);
+    int rc;
+    if ((rc = g_ring_buf.ensure(sizeof(double) * ring_doubles)) || (rc = g_ring_out.ensure(sizeof(double) * out_doubles)))
+        return rc;
+    RLM_HIP_CHECK(hipMemcpy(g_ring_buf.p, host_ring, sizeof(double) * ring_doubles, hipMemcpyHostToDevice));
+    if ((rc = launch_ring_plaquettes(g_ring_buf.d(), n_phi, dim_vec, n_mesh, theta_1, io_order, g_ring_out.d(),
+                                     nullptr)))
+        return rc;
+    RLM_HIP_CHECK(hipMemcpy(chern_theta, g_ring_out.p, sizeof(double) * out_doubles, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 extern "C" int hip_cal_chern(const double *wfs_full, double *chern_theta, int n_phi, int dim_vec, int n_mesh,
                              int theta_1) {
     if (!wfs_full || !chern_theta || n_phi < 1 || dim_vec < 1 || n_mesh < 1 || theta_1 < 1) return -22;
-    // Ring slots: slot (theta_1-1)%2 holds line theta_1-1, slot theta_1%2 holds line theta_1.
-    // Re-pack the two lines as component-major twist points kp = t*(g+1)+t2 (t = 0: line theta_1-1).
-    const int g1 = n_mesh + 1;
-    const long per = (long)n_phi * dim_vec;
-    std::vector<double> host(2 * 2 * g1 * per);
-    const int slot_prev = (theta_1 - 1) % 2, slot_cur = theta_1 % 2;
-    for (int t = 0; t < 2; t++) {
-        const int slot = t == 0 ? slot_prev : slot_cur;
-        for (int t2 = 0; t2 < g1; t2++) {
-            const double *src = wfs_full + 2 * ((long)slot * g1 + t2) * per;
-            double *dst = host.data() + 2 * ((long)t * g1 + t2) * per;
-            for (int st_ = 0; st_ < n_phi; st_++)
-                for (int i = 0; i < dim_vec; i++) {
-                    dst[2 * ((long)i * n_phi + st_)] = src[2 * ((long)st_ * dim_vec + i)];
-                    dst[2 * ((long)i * n_phi + st_) + 1] = src[2 * ((long)st_ * dim_vec + i) + 1];
-                }
-        }
+    return ring_chern_device(wfs_full, chern_theta, n_phi, dim_vec, n_mesh, theta_1, 0);
+}
+
+static std::string wfs_path(const char *dir, int theta_1, int theta_2) {
+    char name[64];
+    snprintf(name, sizeof(name), "wfs_%d_%d.bin", theta_1, theta_2);   // diag_wrappers/wfs_file.cpp:3-4
+    if (!dir || !*dir) return std::string(name);
+    std::string p(dir);
+    if (p.back() != '/') p += '/';
+    return p + name;
+}
+
+// raw complex values, no header; complex double input written as elem_bytes = 16
+// (complex double) or 8 (complex float)
+static int write_raw(const std::string &path, const double *v, size_t count, int elem_bytes) {
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f) return -5;
+    size_t ok;
+    if (elem_bytes == 16) {
+        ok = fwrite(v, 16, count, f);
+    } else {
+        std::vector<float> tmp(2 * count);
+        for (size_t k = 0; k < 2 * count; k++) tmp[k] = (float)v[k];
+        ok = fwrite(tmp.data(), 8, count, f);
     }
-    double *dU = nullptr, *dL = nullptr;
-    const long nkp = 2L * g1;
-    int rc = 0;
-    if (hipMalloc(&dU, sizeof(double) * host.size()) != hipSuccess) return -12;
-    if (hipMalloc(&dL, sizeof(double) * 4 * nkp * n_phi) != hipSuccess) rc = -12;
-    if (!rc && hipMemcpy(dU, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice) != hipSuccess) rc = -1;
-    if (!rc) rc = launch_links(dU, per, dim_vec, n_phi, 2, g1, n_mesh, 0, nkp, dL, dL + 2 * nkp * n_phi, nullptr);
-    // The per-plaquette phases of one line are 4 products per (m, state): finish on the host.
-    std::vector<double> L(4 * nkp * n_phi);
-    if (!rc && hipMemcpy(L.data(), dL, sizeof(double) * L.size(), hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
-    if (!rc) {
-        const double *L1 = L.data(), *L2 = L.data() + 2 * nkp * n_phi;
-        for (int m = 0; m < n_mesh; m++) {
-            const long k00 = m, k10 = g1 + m, k01 = m + 1;
-            for (int j = 0; j < n_phi; j++) {
-                cd p{1.0, 0.0};
-                p = cmul(p, cd{L1[2 * (k00 * n_phi + j)], L1[2 * (k00 * n_phi + j) + 1]});
-                p = cmul(p, cd{L2[2 * (k10 * n_phi + j)], L2[2 * (k10 * n_phi + j) + 1]});
-                p = cmul(p, cconj(cd{L1[2 * (k01 * n_phi + j)], L1[2 * (k01 * n_phi + j) + 1]}));
-                p = cmul(p, cconj(cd{L2[2 * (k00 * n_phi + j)], L2[2 * (k00 * n_phi + j) + 1]}));
-                chern_theta[m + (long)j * n_mesh] = -atan2(p.y, p.x) / (M_PI * 2.0);
-            }
+    const int rc = fclose(f);
+    return (ok == count && rc == 0) ? 0 : -5;
+}
+
+extern "C" int hip_write_wfs(const char *dir, int theta_1, int theta_2, const double *wfs, unsigned long size,
+                             int elem_bytes) {
+    if (!wfs || (elem_bytes != 8 && elem_bytes != 16)) return -22;
+    return write_raw(wfs_path(dir, theta_1, theta_2), wfs, size, elem_bytes);
+}
+
+extern "C" int hip_cal_chern_wfs_io(double *chern_theta, int n_phi, int dim_vec, int n_mesh, int theta_1,
+                                    const char *dir, int elem_bytes) {
+    if (!chern_theta || n_phi < 1 || dim_vec < 1 || n_mesh < 1 || theta_1 < 1 || (elem_bytes != 8 && elem_bytes != 16))
+        return -22;
+    const size_t dim_wfs = (size_t)n_phi * dim_vec;
+    std::vector<double> ring(2 * 2 * (size_t)(n_mesh + 1) * dim_wfs);
+    std::vector<float> tmp(elem_bytes == 8 ? 2 * dim_wfs : 0);
+    for (int slot = 0; slot < 2; slot++)
+        for (int t2 = 0; t2 <= n_mesh; t2++) {
+            FILE *f = fopen(wfs_path(dir, slot, t2).c_str(), "rb");
+            if (!f) return -2;
+            double *dst = ring.data() + 2 * ((size_t)slot * (n_mesh + 1) + t2) * dim_wfs;
+            size_t got = elem_bytes == 16 ? fread(dst, 16, dim_wfs, f) : fread(tmp.data(), 8, dim_wfs, f);
+            fclose(f);
+            if (got != dim_wfs) return -5;
+            if (elem_bytes == 8)
+                for (size_t k = 0; k < 2 * dim_wfs; k++) dst[k] = (double)tmp[k];
         }
-    }
-    (void)hipFree(dU);
-    (void)hipFree(dL);
-    return rc;
+    return ring_chern_device(ring.data(), chern_theta, n_phi, dim_vec, n_mesh, theta_1, 1);
 }
 
 // ============================================================================
@@ -570,6 +601,7 @@ struct rlm_plan_s {
     bool refine = false;
     DevBuf dXd, dXr, dSw, dGw, dQg, dCl, dFlag;
     DevBuf dPair;   // near-degenerate runs of the unrefined mixed path: list + 2 counters
+    DevBuf dWfs;    // staging of rlm_plan_write_wfs
     EigWork ws;
     int nb = 0;              // band width of the band form (dW: S*(g+1) x d x nb, dH: nkp x N x nb)
     bool band_trd = false;   // mixed plans: the FP32 reduction reads the band form (no dense matrices)
@@ -848,6 +880,32 @@ extern "C" const double *rlm_plan_eigenvectors(rlm_plan_t p) {
     return p ? (p->refine ? p->dXr.d() : p->dU.d()) : nullptr;
 }
 
+extern "C" int rlm_plan_write_wfs(rlm_plan_t p, int sample, int theta_1, const char *dir, int elem_bytes) {
+    if (!p || p->loaded < 1 || sample < 0 || sample >= p->loaded || theta_1 < 0 || theta_1 > p->g ||
+        (elem_bytes != 8 && elem_bytes != 16))
+        return -22;
+    const int N = (int)p->n_phi, g = (int)p->g;
+    const size_t per = (size_t)N * N;
+    const size_t count = per * (g + 1);
+    int rc;
+    if ((rc = p->dWfs.ensure((size_t)elem_bytes * count))) return rc;
+    const bool u32 = p->precision == HIP_DIAG_MIXED && !p->refine;
+    const void *U = rlm_plan_eigenvectors(p);
+    if ((rc = launch_gather_line(U, u32, (long)N * N, N, g, p->pl, sample, theta_1, p->dWfs.p, elem_bytes == 8,
+                                 p->stream)))
+        return rc;
+    std::vector<unsigned char> h((size_t)elem_bytes * count);
+    RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+    RLM_HIP_CHECK(hipMemcpy(h.data(), p->dWfs.p, h.size(), hipMemcpyDeviceToHost));
+    for (int t2 = 0; t2 <= g; t2++) {
+        FILE *f = fopen(wfs_path(dir, theta_1 % 2, t2).c_str(), "wb");
+        if (!f) return -5;
+        const size_t ok = fwrite(h.data() + (size_t)elem_bytes * per * t2, elem_bytes, per, f);
+        if (fclose(f) != 0 || ok != per) return -5;
+    }
+    return 0;
+}
+
 extern "C" int rlm_plan_refine_info(rlm_plan_t p, long *out) {
     if (!p || !out) return -22;
     out[0] = p->refine ? 1 : 0;
@@ -868,7 +926,7 @@ extern "C" int rlm_plan_destroy(rlm_plan_t p) {
     DevBuf *bufs[] = {&p->dcmn, &p->dcmt, &p->dE,   &p->dX,   &p->dY,   &p->dS,   &p->dV,
                       &p->dW,   &p->dH,   &p->dA,   &p->dU,   &p->dEig, &p->dL,   &p->dOut,
                       &p->dA32, &p->dResid, &p->dXd, &p->dXr, &p->dSw,  &p->dGw,  &p->dQg,
-                      &p->dCl,  &p->dFlag,  &p->dPair};
+                      &p->dCl,  &p->dFlag,  &p->dPair, &p->dWfs};
     for (DevBuf *b : bufs) b->release();
     p->ws.release();
     p->jumps.release();

@@ -236,4 +236,119 @@ int launch_reduce(const double *Lk1, const double *Lk2, const double *Wv, int ns
     return 0;
 }
 
+// ---------------------------------------------------------------- ring-buffer entry points
+// One theta_1 line of the reference's 2-slot ring (diag_wrappers/chern.cpp:3-35): vertex
+// (slot, theta_2) holds n_phi state vectors of dim_vec components, state-major
+// (ring[((slot*(g+1) + theta_2)*n_phi + n)*dim_vec + i], the column-major eigenvector
+// matrices the reference's solvers leave). One wave per (plaquette m, state n); the
+// four overlaps follow the reference's path (slot 1 - theta_1 % 2, m) -> ... and its
+// product order. io_order: the order of cal_Chern_wfs_IO (chern.cpp:111-118), factors
+// <p_{l+1}|p_l> and +arg / 2 pi, instead of <p_l|p_{l+1}> and -arg / 2 pi.
+// grid ceil(n_mesh * n_phi / 4), block 256; no barriers (waves past the end return).
+__global__ __launch_bounds__(256) void ring_plaquette_kernel(const double *__restrict__ ring, int n_phi,
+                                                             int dim_vec, int n_mesh, int theta_1, int io_order,
+                                                             double *__restrict__ chern_theta) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const long pair = (long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (pair >= (long)n_mesh * n_phi) return;
+    const int m = (int)(pair / n_phi), n = (int)(pair % n_phi);
+    const long dim_wfs = (long)n_phi * dim_vec;
+    int dki[4], dkj[4];
+    dki[2] = (theta_1 % 2 == 0) ? 1 : -1;
+    dki[0] = -dki[2];
+    dki[1] = dki[3] = 0;
+    dkj[0] = dkj[2] = 0;
+    dkj[3] = -1;
+    dkj[1] = 1;
+    int lki = 1 - theta_1 % 2, lkj = m;
+    cd prod{1.0, 0.0};
+    for (int l = 0; l < 4; l++) {
+        const int rki = lki + dki[l], rkj = lkj + dkj[l];
+        const double *ul = ring + 2 * (((long)lki * (n_mesh + 1) + lkj) * dim_wfs + (long)dim_vec * n);
+        const double *ur = ring + 2 * (((long)rki * (n_mesh + 1) + rkj) * dim_wfs + (long)dim_vec * n);
+        cd a{0.0, 0.0};
+        for (int i = lane; i < dim_vec; i += kWave) {
+            if (io_order) cfmac(a, ld(ur, i), ld(ul, i));   // <p_{l+1}|p_l>
+            else cfmac(a, ld(ul, i), ld(ur, i));            // <p_l|p_{l+1}>
+        }
+        a.x = wave_sum(a.x);
+        a.y = wave_sum(a.y);
+        prod = cmul(prod, a);
+        lki = rki;
+        lkj = rkj;
+    }
+    if (lane == 0) {
+        const double TWO_PI = 6.283185307179586232;
+        const double ph = atan2(prod.y, prod.x);
+        chern_theta[m + (long)n * n_mesh] = io_order ? ph / TWO_PI : -ph / TWO_PI;
+    }
+}
+
+// Eigenvectors of line t1 of sample s (theta_2 = 0..g) in the reference's ring layout
+// (state-major, out[(t2*ns + j)*ns + i] = component i of state j), as complex double or
+// float. U holds the solved points, component-major: pl = g + 1 (every point solved,
+// kp = (s*(g+1) + t1)*(g+1) + t2) or pl = g (theta < g solved, kp = (s*g + t1)*g + t2,
+// the theta = 2 pi points derived as links_derived_kernel derives them).
+// grid (g + 1, ceil(ns*ns / 256)), block 256.
+template <typename TU, typename TO>
+__global__ __launch_bounds__(256) void gather_line_kernel(const TU *__restrict__ U, long strideU, int ns, int g,
+                                                          int pl, long s, int t1, TO *__restrict__ out) {
+    const int t2 = blockIdx.x;
+    const long e = (long)blockIdx.y * blockDim.x + threadIdx.x;
+    if (e >= (long)ns * ns) return;
+    const int j = (int)(e / ns), i = (int)(e % ns);
+    int a = t1, b = t2, sh = 0;
+    bool ph = false;
+    if (pl == g) {
+        sh = (b == g) ? 1 : 0;
+        ph = (a == g);
+        a = (a == g) ? 0 : a;
+        b = (b == g) ? 0 : b;
+    }
+    const TU *base = U + 2 * (((s * pl + a) * pl + b) * strideU);
+    int ii = i + sh;
+    ii = ii >= ns ? ii - ns : ii;
+    const cx<TU> v = ldx(base, (long)ii * ns + j);
+    cd x{(double)v.x, (double)v.y};
+    if (ph) {
+        const double TWO_PI = 6.283185307179586232;
+        double sn, cs;
+        sincos(TWO_PI * (double)i / (double)ns, &sn, &cs);
+        x = cmul(cd{cs, sn}, x);
+    }
+    TO *o = out + 2 * (((long)t2 * ns + j) * ns + i);
+    o[0] = (TO)x.x;
+    o[1] = (TO)x.y;
+}
+
+int launch_ring_plaquettes(const double *ring, int n_phi, int dim_vec, int n_mesh, int theta_1, int io_order,
+                           double *chern_theta, hipStream_t stream) {
+    if (n_phi < 1 || dim_vec < 1 || n_mesh < 1 || theta_1 < 1) return -22;
+    const long pairs = (long)n_mesh * n_phi;
+    hipLaunchKernelGGL(ring_plaquette_kernel, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, stream, ring, n_phi,
+                       dim_vec, n_mesh, theta_1, io_order, chern_theta);
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_gather_line(const void *U, bool fp32_in, long strideU, int ns, int g, int pl, long s, int t1, void *out,
+                       bool fp32_out, hipStream_t stream) {
+    if (ns < 1 || g < 1 || (pl != g && pl != g + 1) || t1 < 0 || t1 > g) return -22;
+    const dim3 grid((unsigned)(g + 1), (unsigned)(((long)ns * ns + 255) / 256));
+    if (fp32_in && fp32_out)
+        hipLaunchKernelGGL((gather_line_kernel<float, float>), grid, dim3(256), 0, stream, static_cast<const float *>(U),
+                           strideU, ns, g, pl, s, t1, static_cast<float *>(out));
+    else if (fp32_in)
+        hipLaunchKernelGGL((gather_line_kernel<float, double>), grid, dim3(256), 0, stream,
+                           static_cast<const float *>(U), strideU, ns, g, pl, s, t1, static_cast<double *>(out));
+    else if (fp32_out)
+        hipLaunchKernelGGL((gather_line_kernel<double, float>), grid, dim3(256), 0, stream,
+                           static_cast<const double *>(U), strideU, ns, g, pl, s, t1, static_cast<float *>(out));
+    else
+        hipLaunchKernelGGL((gather_line_kernel<double, double>), grid, dim3(256), 0, stream,
+                           static_cast<const double *>(U), strideU, ns, g, pl, s, t1, static_cast<double *>(out));
+    RLM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 }  // namespace rlm

@@ -14,4 +14,12 @@ int launch_links_derived(const void *U, bool fp32, long strideU, int ns, int g, 
 // mode 0 = rlm, 1 = hlm, 2 = rlm with eigenvalues of the solved points only
 int launch_reduce(const double *Lk1, const double *Lk2, const double *Wv, int ns, int n_mesh, int L2n, int mode,
                   double n_phi_d, int n_samples, double *energy, double *chern, double *margin, hipStream_t stream);
+// One theta_1 line of the reference's 2-slot ring buffer (device copy, complex double,
+// state-major): chern_theta[m + n*n_mesh] (io_order: cal_Chern_wfs_IO's factor order).
+int launch_ring_plaquettes(const double *ring, int n_phi, int dim_vec, int n_mesh, int theta_1, int io_order,
+                           double *chern_theta, hipStream_t stream);
+// Line t1 of sample s of a plan's eigenvectors in the ring layout (g + 1 points,
+// state-major), complex double or float in and out; pl = g (derived edges) or g + 1.
+int launch_gather_line(const void *U, bool fp32_in, long strideU, int ns, int g, int pl, long s, int t1, void *out,
+                       bool fp32_out, hipStream_t stream);
 }  // namespace rlm

@@ -55,6 +55,9 @@ def lib() -> ctypes.CDLL:
         "hip_heevd": ([dp, dp, i], i),
         "hip_diag_heevd_batched": ([i, dp, l, l, dp, l, i, ctypes.c_char, i, vp], i),
         "hip_cal_chern": ([dp, dp, i, i, i, i], i),
+        "hip_write_wfs": ([ctypes.c_char_p, i, i, dp, ctypes.c_ulong, i], i),
+        "hip_cal_chern_wfs_io": ([dp, i, i, i, i, ctypes.c_char_p, i], i),
+        "rlm_plan_write_wfs": ([vp, i, i, ctypes.c_char_p, i], i),
         "rlm_prepare_coeff": ([dp, dp, dp, i, i, i, i, i, d, d], i),
         "rlm_generate_disorder_potential": ([dp, dp, dp, l, d, d, ctypes.c_ulong, l], i),
         "hip_generate_disorder_potential": ([dp, dp, dp, l, d, d, ctypes.c_ulong, l, i], i),
@@ -124,6 +127,23 @@ def generate_disorder(n_imp: int, L1: float, L2: float, seed: int, sample: int):
     _check(lib().rlm_generate_disorder_potential(_ptr(x), _ptr(y), _ptr(s), n_imp, L1, L2, seed, sample),
            "rlm_generate_disorder_potential")
     return x, y, s
+
+
+def cal_chern(ring: np.ndarray, n_phi: int, dim_vec: int, n_mesh: int, theta_1: int) -> np.ndarray:
+    """hip_cal_chern on a host ring [2, n_mesh+1, n_phi, dim_vec] (complex128)."""
+    ring = np.ascontiguousarray(ring, dtype=np.complex128)
+    out = np.zeros(n_phi * n_mesh)
+    _check(lib().hip_cal_chern(_ptr(ring), _ptr(out), n_phi, dim_vec, n_mesh, theta_1), "hip_cal_chern")
+    return out
+
+
+def cal_chern_wfs_io(n_phi: int, dim_vec: int, n_mesh: int, theta_1: int, directory: str,
+                     elem_bytes: int = 16) -> np.ndarray:
+    """hip_cal_chern_wfs_io: the line's chern_theta from <dir>/wfs_<slot>_<theta_2>.bin."""
+    out = np.zeros(n_phi * n_mesh)
+    _check(lib().hip_cal_chern_wfs_io(_ptr(out), n_phi, dim_vec, n_mesh, theta_1, directory.encode(), elem_bytes),
+           "hip_cal_chern_wfs_io")
+    return out
 
 
 # ---------------------------------------------------------------- device helpers (torch-owned memory)
@@ -218,6 +238,11 @@ class RlmPlan:
         out = (ctypes.c_long * 2)()
         _check(lib().rlm_plan_refine_info(self._h, out), "rlm_plan_refine_info")
         return bool(out[0]), int(out[1])
+
+    def write_wfs(self, sample: int, theta_1: int, directory: str, elem_bytes: int = 16):
+        """rlm_plan_write_wfs: line theta_1 of batch sample `sample` -> <dir>/wfs_<theta_1%2>_<theta_2>.bin."""
+        _check(lib().rlm_plan_write_wfs(self._h, sample, theta_1, directory.encode(), elem_bytes),
+               "rlm_plan_write_wfs")
 
     def eigenvalues_ptr(self) -> int:
         return lib().rlm_plan_eigenvalues(self._h) or 0
