@@ -7,8 +7,10 @@
  * hipDoubleComplex layout. "d_" pointers are device (HBM) memory; the others
  * are host memory.
  *
- * Status: 0 = OK; > 0 = solver info (as LAPACK's info); < 0 = error
- * (-22 invalid argument, -12 out of memory, other negatives = -hipError_t).
+ * Status: 0 = OK; > 0 = solver info (as LAPACK's info: 1 + the index of the
+ * first matrix, or sample, whose input or result is not finite); < 0 = error
+ * (-22 invalid argument, -12 out of memory, -2 / -5 file missing / I/O error,
+ * other negatives = -hipError_t).
  * Every entry point is synchronous unless it names a stream.
  */
 #ifndef HIP_DIAG_H
@@ -57,7 +59,10 @@ int hip_heevd(double *A, double *w, int n);
  * precision HIP_DIAG_FP64: FP64 throughout. HIP_DIAG_MIXED: FP32 Householder
  * reduction and back-transform, FP64 divide & conquer, eigenvalues = FP64
  * Rayleigh quotients of the FP32 eigenvectors against the FP64 matrix
- * (eigenvectors carry FP32 accuracy, ~1e-6 relative residual). */
+ * (eigenvectors carry FP32 accuracy, ~1e-6 relative residual).
+ * A NaN or infinity in the triangle read returns 1 + the index of the first such
+ * matrix before any solver kernel runs (the matrices are left as packed copies
+ * never written back; d_A and d_W are untouched). */
 int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA, double *d_W, long strideW, int batch,
                            char uplo, int precision, void *stream);
 
@@ -152,7 +157,9 @@ int rlm_plan_load(rlm_plan_t plan, long first_sample, int count);
  * streams, V_mn, assembly, eigensolve, links, plaquettes and energies. */
 int rlm_plan_execute(rlm_plan_t plan);
 /* Copies energies / Chern values / plaquette margins ([count][n_phi], host)
- * after synchronising the stream. margin may be NULL. */
+ * after synchronising the stream. margin may be NULL. Returns 1 + the batch
+ * sample holding a non-finite energy or Chern value (every value is copied
+ * anyway), 0 when all are finite; rlm_plan_run returns 1 + the run's sample. */
 int rlm_plan_fetch(rlm_plan_t plan, double *energy, double *chern, double *margin);
 /* load + execute + fetch over any number of samples, batch by batch. */
 int rlm_plan_run(rlm_plan_t plan, long first_sample, long n_samples, double *energy, double *chern, double *margin);
@@ -203,6 +210,15 @@ int rlm_plan_write_wfs(rlm_plan_t plan, int sample, int theta_1, const char *dir
  * size limit or had no Cholesky factor (0 when every coupled run was solved exactly).
  * No counterpart in the reference (whose FP32 builds have no refinement). */
 int rlm_plan_refine_info(rlm_plan_t plan, long *out2);
+/* Mixed plans up to n_phi = 512 without the refinement give every run of
+ * near-degenerate neighbouring states (gaps below 1e-4 of the spectrum's extent)
+ * an exact FP64 Rayleigh-Ritz; runs longer than 8 states are split into balanced
+ * chunks of at most 8. For the last execute: out4[0] = chunks solved, [1] = runs
+ * split, [2] = chunks not solved (beyond the list capacity), [3] = chunks left
+ * unrotated (Gram matrix without a Cholesky factor). Nonzero [2] or [3] means some
+ * energies may carry the FP32 vectors' coupling error; all zero for FP64 plans. No
+ * counterpart in the reference. */
+int rlm_plan_pair_info(rlm_plan_t plan, long *out4);
 int rlm_plan_destroy(rlm_plan_t plan);
 
 /* Replaces the hlm sample / theta loops of hlm/main_hlm_cpu.cpp:75-196 (with

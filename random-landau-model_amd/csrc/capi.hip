@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/hip_diag.h"
 #include "chern.hpp"
 #include "common.hpp"
@@ -102,6 +104,28 @@ __global__ __launch_bounds__(256) void maxabs_kernel(const double *P, long nn, i
         part[blockIdx.x] = mx;
     }
 }
+
+// 1 in flag[b * nch + chunk] if that chunk of packed matrix b holds a NaN or an
+// infinity (real or imaginary part), else 0. 256 threads.
+__global__ __launch_bounds__(256) void nonfinite_kernel(const double *P, long nn, int nch, int *flag) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    const double *M = P + 2 * (blockIdx.x / nch) * nn;
+    const long step = (long)nch * blockDim.x;
+    int bad = 0;
+    for (long i = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; i < nn; i += step)
+        bad |= !(isfinite(M[2 * i]) && isfinite(M[2 * i + 1]));
+    if (bad) any = 1;   // same-value writes
+    __syncthreads();
+    if (threadIdx.x == 0) flag[blockIdx.x] = any;
+}
+
+// RAII roctx range (rocprofv3 --marker-trace shows the stage launches of a plan)
+struct Range {
+    explicit Range(const char *name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+};
 
 // FP64 -> FP32 copy of packed complex matrices (mixed-precision solver input), each
 // matrix scaled by a power of two to max |entry| in [1/2, 1) (from the maxabs_kernel
@@ -326,7 +350,7 @@ int eig_packed_f32(float *A, int n, long batch, double *W, float *Uout, EigWork 
 
 std::mutex g_single_mu;
 EigWork g_single_ws;
-DevBuf g_single_pack, g_single_a32, g_single_pair;
+DevBuf g_single_pack, g_single_a32, g_single_pair, g_single_flag;
 
 }  // namespace
 
@@ -356,6 +380,20 @@ extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA
     hipLaunchKernelGGL(hermitize_kernel, grid, dim3(256), 0, s, P, (long)n, nn, n,
                        (uplo == 'L' || uplo == 'l') ? 1 : 0, nch);
     RLM_HIP_CHECK(hipGetLastError());
+    {
+        // a NaN or infinity in the triangle read stops the call before any solver kernel:
+        // status 1 + the index of the first such matrix (LAPACK's info > 0 convention)
+        const size_t nfl = (size_t)batch * nch;
+        if ((rc = g_single_flag.ensure(sizeof(int) * nfl))) return rc;
+        int *fl = static_cast<int *>(g_single_flag.p);
+        hipLaunchKernelGGL(nonfinite_kernel, grid, dim3(256), 0, s, P, nn, nch, fl);
+        RLM_HIP_CHECK(hipGetLastError());
+        std::vector<int> h(nfl);
+        RLM_HIP_CHECK(hipMemcpyAsync(h.data(), fl, sizeof(int) * nfl, hipMemcpyDeviceToHost, s));
+        RLM_HIP_CHECK(hipStreamSynchronize(s));
+        for (size_t k = 0; k < nfl; k++)
+            if (h[k]) return 1 + (int)(k / nch);
+    }
     if (precision == HIP_DIAG_MIXED) {
         // FP32 reduction and back-transform, FP64 divide & conquer, then FP64
         // Rayleigh quotients of the FP32 vectors against the FP64 matrix
@@ -373,9 +411,17 @@ extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA
             // exact Rayleigh-Ritz of near-degenerate runs (n > 512 has no such step here;
             // the rlm plans refine those sizes in FP64)
             const int cap = pair_list_capacity(n, batch);
-            if ((rc = g_single_pair.ensure(sizeof(int) * (3 * (size_t)cap + 2)))) return rc;
+            if ((rc = g_single_pair.ensure(sizeof(int) * (3 * (size_t)cap + 4)))) return rc;
             int *pl = static_cast<int *>(g_single_pair.p);
-            if ((rc = launch_pair_rr(P, nullptr, U32, n, n, batch, Wp, pl + 2, cap, pl, s))) return rc;
+            if ((rc = launch_pair_rr(P, nullptr, U32, n, n, batch, Wp, pl + 4, cap, pl, s))) return rc;
+            int cnt[kPairCounters];
+            RLM_HIP_CHECK(hipMemcpyAsync(cnt, pl, sizeof(cnt), hipMemcpyDeviceToHost, s));
+            RLM_HIP_CHECK(hipStreamSynchronize(s));
+            if (cnt[0] > cap || cnt[2] > 0)
+                fprintf(stderr,
+                        "[rlm_hip] mixed solver: %d near-degenerate chunks beyond the list capacity, %d without a "
+                        "Cholesky factor (their FP64 Rayleigh quotients stand)\n",
+                        cnt[0] > cap ? cnt[0] - cap : 0, cnt[2]);
         }
         hipLaunchKernelGGL(transpose_out_f32_kernel, grid, dim3(256), 0, s, U32, nn, n, d_A, lda, strideA, nch);
     } else {
@@ -584,6 +630,14 @@ extern "C" int hip_set_hamil_matrix(const double *d_coeff_m_theta_1, const doubl
 // ============================================================================
 // rlm batched pipeline
 // ============================================================================
+// energies [SN] then Chern values [SN] of a fetched batch: 1 + the first sample holding a
+// non-finite value (every output is still copied), 0 if all are finite
+static int first_nonfinite_sample(const double *h, long SN, long n_phi) {
+    for (long k = 0; k < SN; k++)
+        if (!std::isfinite(h[k]) || !std::isfinite(h[SN + k])) return 1 + (int)(k / n_phi);
+    return 0;
+}
+
 struct rlm_plan_s {
     long n_phi, n_imp, off, d, g, spb;
     double L1, L2;
@@ -669,7 +723,7 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
         (rc = p->dOut.ensure(8 * 3 * S * N)) || (rc = p->ws.ensure((int)N, nkp, false)))
         return fail(rc);
     if (precision == HIP_DIAG_MIXED && !p->refine && N <= kSmallMaxN &&
-        (rc = p->dPair.ensure(sizeof(int) * (3 * (size_t)pair_list_capacity((int)N, nkp) + 2))))
+        (rc = p->dPair.ensure(sizeof(int) * (3 * (size_t)pair_list_capacity((int)N, nkp) + 4))))
         return fail(rc);
     if (p->refine &&
         ((rc = p->dXd.ensure(16 * nkp * N * N)) || (rc = p->dXr.ensure(16 * nkp * N * N)) ||
@@ -711,6 +765,8 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     const bool derive = p->pl == g;
     hipStream_t s = p->stream;
     int rc;
+    Range range("rlm_plan_execute");
+    roctxMarkA("rlm: disorder streams + V_mn + band tables");
     RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
     // disorder of the loaded samples from the seed (rlm/disorder_potential.cpp:3-21)
     if ((rc = p->jumps.launch(p->seed, S, 0, 3 * p->n_imp, p->n_imp, p->L1, p->L2, p->dX.d(), p->dY.d(), p->dS.d(),
@@ -720,6 +776,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
         return rc;
     if ((rc = launch_wband(p->dV.d(), p->dcmt.d(), N, d, off, g, S, p->dW.d(), s))) return rc;
     RLM_HIP_CHECK(hipEventRecord(p->ev[1], s));
+    roctxMarkA("rlm: assembly");
     // band form of every k-point's Hamiltonian; dense FP64 matrices where a kernel needs them
     const bool mixed = p->precision == HIP_DIAG_MIXED;
     float *A32 = mixed ? static_cast<float *>(p->dA32.p) : nullptr;
@@ -732,6 +789,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
         }
     }
     RLM_HIP_CHECK(hipEventRecord(p->ev[2], s));
+    roctxMarkA("rlm: eigensolve + refinement + links");
     double *L1 = p->dL.d(), *L2 = L1 + 2 * nlk * N;
     if (!mixed) {
         if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[3], p->ev[4]))) return rc;
@@ -773,7 +831,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
             if (N <= kSmallMaxN) {
                 const int cap = pair_list_capacity(N, nkp);
                 int *pl = static_cast<int *>(p->dPair.p);
-                if ((rc = launch_pair_rr(nullptr, p->dH.d(), U32, N, off, nkp, p->dEig.d(), pl + 2, cap, pl, s)))
+                if ((rc = launch_pair_rr(nullptr, p->dH.d(), U32, N, off, nkp, p->dEig.d(), pl + 4, cap, pl, s)))
                     return rc;
             }
             RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
@@ -785,6 +843,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
         }
     }
     RLM_HIP_CHECK(hipEventRecord(p->ev[7], s));
+    roctxMarkA("rlm: plaquettes + energies");
     double *out = p->dOut.d();
     if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, (int)g1, derive ? 2 : 0, (double)N, S, out, out + (long)S * N,
                             out + 2L * S * N, s)))
@@ -802,7 +861,7 @@ extern "C" int rlm_plan_fetch(rlm_plan_t p, double *energy, double *chern, doubl
     if (energy) memcpy(energy, h.data(), 8 * SN);
     if (chern) memcpy(chern, h.data() + SN, 8 * SN);
     if (margin) memcpy(margin, h.data() + 2 * SN, 8 * SN);
-    return 0;
+    return first_nonfinite_sample(h.data(), SN, p->n_phi);
 }
 
 extern "C" int rlm_plan_run(rlm_plan_t p, long first_sample, long n_samples, double *energy, double *chern,
@@ -816,7 +875,7 @@ extern "C" int rlm_plan_run(rlm_plan_t p, long first_sample, long n_samples, dou
         const long off = s0 * p->n_phi;
         if ((rc = rlm_plan_fetch(p, energy ? energy + off : nullptr, chern ? chern + off : nullptr,
                                  margin ? margin + off : nullptr)))
-            return rc;
+            return rc > 0 ? rc + (int)s0 : rc;   // > 0: 1 + the sample (of the run) with a non-finite value
     }
     return 0;
 }
@@ -903,6 +962,21 @@ extern "C" int rlm_plan_write_wfs(rlm_plan_t p, int sample, int theta_1, const c
         const size_t ok = fwrite(h.data() + (size_t)elem_bytes * per * t2, elem_bytes, per, f);
         if (fclose(f) != 0 || ok != per) return -5;
     }
+    return 0;
+}
+
+extern "C" int rlm_plan_pair_info(rlm_plan_t p, long *out) {
+    if (!p || !out) return -22;
+    for (int k = 0; k < 4; k++) out[k] = 0;
+    if (!p->dPair.p || p->loaded < 1) return 0;
+    int cnt[kPairCounters];
+    RLM_HIP_CHECK(hipStreamSynchronize(p->stream));
+    RLM_HIP_CHECK(hipMemcpy(cnt, p->dPair.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    const long cap = pair_list_capacity((int)p->n_phi, (long)p->loaded * p->pl * p->pl);
+    out[0] = cnt[0] < cap ? cnt[0] : cap;
+    out[1] = cnt[1];
+    out[2] = cnt[0] > cap ? cnt[0] - cap : 0;
+    out[3] = cnt[2];
     return 0;
 }
 
@@ -1052,6 +1126,7 @@ extern "C" int hlm_plan_execute(hlm_plan_t p) {
     const long ntw = p->ntw, nkp = S * ntw, nn = (long)N * N;
     hipStream_t s = p->stream;
     int rc;
+    Range range("hlm_plan_execute");
     RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
     // on-site disorder of the loaded samples (hlm/main_hlm_cpu.cpp:79, width W)
     if ((rc = p->jumps.launch(p->seed, S, 1, ns, 0, 0.5 * p->W, 0.0, p->dV.d(), nullptr, nullptr, s))) return rc;
@@ -1086,7 +1161,7 @@ extern "C" int hlm_plan_fetch(hlm_plan_t p, double *energy, double *chern, doubl
     if (energy) memcpy(energy, h.data(), 8 * SN);
     if (chern) memcpy(chern, h.data() + SN, 8 * SN);
     if (margin) memcpy(margin, h.data() + 2 * SN, 8 * SN);
-    return 0;
+    return first_nonfinite_sample(h.data(), SN, p->n_phi);
 }
 
 extern "C" int hlm_plan_run(hlm_plan_t p, long first_sample, long n_samples, double *energy, double *chern,
@@ -1100,7 +1175,7 @@ extern "C" int hlm_plan_run(hlm_plan_t p, long first_sample, long n_samples, dou
         const long off = s0 * p->n_phi;
         if ((rc = hlm_plan_fetch(p, energy ? energy + off : nullptr, chern ? chern + off : nullptr,
                                  margin ? margin + off : nullptr)))
-            return rc;
+            return rc > 0 ? rc + (int)s0 : rc;   // > 0: 1 + the sample (of the run) with a non-finite value
     }
     return 0;
 }

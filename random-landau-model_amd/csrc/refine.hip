@@ -21,6 +21,8 @@
 // The plans then take the FP64 Rayleigh quotients and the links of X'.
 // Layouts: X, Y, X' component-major per matrix ([i*n + j] = component i of state j,
 // complex double); S, G: [kp][j][kRefWin] with entry d holding k = j + d - kRefW.
+#include <mutex>
+
 #include "common.hpp"
 #include "refine.hpp"
 
@@ -390,19 +392,22 @@ __global__ __launch_bounds__(256) void ref_update_kernel(const double *__restric
 // FP64 Rayleigh quotients, runs of neighbouring states whose eigenvalue gaps are below
 // kPairTau times the spectrum's extent (near-degenerate states, whose FP32 vectors mix
 // and whose Rayleigh quotients then err by |coupling|^2 / gap) are collected, and each
-// run of up to kPairMax states gets an exact FP64 Rayleigh-Ritz: y_a = A u_a from the
-// band form, S_ab = u_a^H y_b, G_ab = u_a^H u_b, S q = l G q, eigenvalues into W and the
-// FP32 vectors rotated by Q. Runs longer than kPairMax are cut (counted). The
-// threshold: the FP32 vectors' couplings |u_k^H A u_j| measure ~5e-7 at n_phi = 256,
-// so a gap g leaves |coupling|^2 / g in the Rayleigh quotient, below 1e-10 of the
-// spectrum's extent for g above ~1e-4 of it.
+// run gets an exact FP64 Rayleigh-Ritz: y_a = A u_a from the band form, S_ab = u_a^H
+// y_b, G_ab = u_a^H u_b, S q = l G q, eigenvalues into W and the FP32 vectors rotated by
+// Q. Pairs (the common case) take a register-resident 2 x 2 solver, runs of 3 to
+// kPairMax states an LDS-staged general one; longer runs are split into balanced chunks
+// of at most kPairMax (never a single state) and counted. The threshold: the FP32
+// vectors' couplings |u_k^H A u_j| measure ~5e-7 at n_phi = 256, so a gap g leaves
+// |coupling|^2 / g in the Rayleigh quotient, below 1e-10 of the spectrum's extent for g
+// above ~1e-4 of it.
 constexpr double kPairTau = 1e-4;
-constexpr int kPairMax = 4;
+constexpr int kPairMax = kRefMaxCluster;       // 8
 constexpr int kPairMaxN = 512;                 // n of the unrefined mixed path (LDS stage)
 
-// count[0] = runs appended, count[1] = runs cut at kPairMax
+// count[0] = runs (chunks) appended, count[1] = runs longer than kPairMax (split),
+// count[2] = chunks whose Gram matrix had no Cholesky factor (left unrotated)
 __global__ void pair_zero_kernel(int *count) {
-    if (threadIdx.x < 2) count[threadIdx.x] = 0;
+    if (threadIdx.x < kPairCounters) count[threadIdx.x] = 0;
 }
 
 __global__ __launch_bounds__(256) void pair_flag_kernel(const double *__restrict__ W, int n, long nkp, int cap,
@@ -417,16 +422,19 @@ __global__ __launch_bounds__(256) void pair_flag_kernel(const double *__restrict
         if (!here || before) continue;                        // not the first state of a run
         int e = j + 1;                                        // run [j, e]
         while (e < n - 1 && w[e + 1] - w[e] < tau) e++;
-        for (int c0 = j; c0 <= e; c0 += kPairMax) {
-            const int m = min(kPairMax, e - c0 + 1);
-            if (c0 > j) atomicAdd(count + 1, 1);
-            if (m < 2) continue;
+        const int len = e - j + 1;
+        const int nch = (len + kPairMax - 1) / kPairMax;      // balanced chunks, each >= 2 states
+        if (nch > 1) atomicAdd(count + 1, 1);
+        const int base = len / nch, rem = len % nch;
+        for (int q = 0, c0 = j; q < nch; q++) {
+            const int m = base + (q < rem ? 1 : 0);
             const int slot = atomicAdd(count, 1);
             if (slot < cap) {
                 list[3 * slot] = (int)kp;
                 list[3 * slot + 1] = c0;
                 list[3 * slot + 2] = m;
             }
+            c0 += m;
         }
     }
 }
@@ -476,28 +484,27 @@ __host__ __device__ inline bool small_geneig2(const cd (&S)[2][2], const cd (&G)
     return true;
 }
 
-// One run of M >= m states at kp (one wave, lanes over rows). S = U^H A U from the
-// upper half of the band, A = B + B^H with B(r, c) = A(r, c) at circular offsets
-// c - r = 1..bw and B(r, r) = A(r, r) / 2 (the upper triangle when the band covers
-// every column): P = U^H B U, S = P + P^H.
-template <int M>
-__device__ void pair_rr_run(const double *Ak, const double *Hk, float *Uk, double *Wk, int n, int bw, int nb,
-                            bool dense, int c0, int m, int lane, cx<float> (*ust)[kPairMaxN], cd (*Qs)[kPairMax],
-                            double *ls, int *okflag) {
-    for (int e = lane; e < M * n; e += 64) {
+// One pair (states c0, c0 + 1) at kp: one wave, lanes over rows, P and G in registers.
+// S = U^H A U from the upper half of the band, A = B + B^H with B(r, c) = A(r, c) at
+// circular offsets c - r = 1..bw and B(r, r) = A(r, r) / 2 (the upper triangle when the
+// band covers every column): P = U^H B U, S = P + P^H.
+__device__ void pair_rr_two(const double *Ak, const double *Hk, float *Uk, double *Wk, int n, int bw, int nb,
+                            bool dense, int c0, int lane, cx<float> (*ust)[kPairMaxN], cd (*Qs)[2], int *okflag,
+                            int *nfail) {
+    for (int e = lane; e < 2 * n; e += 64) {
         const int a = e / n, r = e - a * n;
-        ust[a][r] = a < m ? ldx(Uk, (long)r * n + c0 + a) : cx<float>{0.f, 0.f};
+        ust[a][r] = ldx(Uk, (long)r * n + c0 + a);
     }
     __syncthreads();
-    cd P[M][M], G[M][M];
+    cd P[2][2], G[2][2];
 #pragma unroll
-    for (int a = 0; a < M; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < M; b++) P[a][b] = G[a][b] = cd{0.0, 0.0};
+        for (int b = 0; b < 2; b++) P[a][b] = G[a][b] = cd{0.0, 0.0};
     for (int r = lane; r < n; r += 64) {
-        cd u[M], y[M];
+        cd u[2], y[2];
 #pragma unroll
-        for (int a = 0; a < M; a++) {
+        for (int a = 0; a < 2; a++) {
             u[a] = cd{ust[a][r].x, ust[a][r].y};
             y[a] = cd{0.0, 0.0};
         }
@@ -507,91 +514,64 @@ __device__ void pair_rr_run(const double *Ak, const double *Hk, float *Uk, doubl
             cd arc = band_entry(Ak, Hk, n, bw, nb, r, c);
             if (k == 0) arc = cd{0.5 * arc.x, 0.0};
 #pragma unroll
-            for (int a = 0; a < M; a++) cfma(y[a], arc, cd{ust[a][c].x, ust[a][c].y});
+            for (int a = 0; a < 2; a++) cfma(y[a], arc, cd{ust[a][c].x, ust[a][c].y});
         }
 #pragma unroll
-        for (int a = 0; a < M; a++)
+        for (int a = 0; a < 2; a++)
 #pragma unroll
-            for (int b = 0; b < M; b++) {
+            for (int b = 0; b < 2; b++) {
                 cfmac(P[a][b], u[a], y[b]);
                 cfmac(G[a][b], u[a], u[b]);
             }
     }
 #pragma unroll
-    for (int a = 0; a < M; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < M; b++) {
+        for (int b = 0; b < 2; b++) {
             P[a][b] = cd{wave_sum(P[a][b].x), wave_sum(P[a][b].y)};
             G[a][b] = cd{wave_sum(G[a][b].x), wave_sum(G[a][b].y)};
         }
     if (lane == 0) {
-        // S = P + P^H; G Hermitian part (Hermitian up to rounding)
-        bool ok;
-        cd Sq2[2][2], Gq2[2][2], Q2[2][2];
-        cd Sq[kRefMaxCluster][kRefMaxCluster], Gq[kRefMaxCluster][kRefMaxCluster], Q[kRefMaxCluster][kRefMaxCluster];
-        if constexpr (M == 2) {
+        cd Sq[2][2], Gq[2][2], Q[2][2];
 #pragma unroll
-            for (int a = 0; a < 2; a++)
+        for (int a = 0; a < 2; a++)
 #pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    Sq2[a][b] = cadd(P[a][b], cconj(P[b][a]));
-                    Gq2[a][b] = cscale(cadd(G[a][b], cconj(G[b][a])), 0.5);
-                }
-            ok = small_geneig2(Sq2, Gq2, Q2);
-        } else {
-            for (int a = 0; a < m; a++)
-                for (int b = 0; b < m; b++) {
-                    Sq[a][b] = cadd(P[a][b], cconj(P[b][a]));
-                    Gq[a][b] = cscale(cadd(G[a][b], cconj(G[b][a])), 0.5);
-                }
-            ok = small_geneig(m, Sq, Gq, Q);
-        }
-        const int mm = M == 2 ? 2 : m;    // compile-time bounds on the pair path (registers only)
-#pragma unroll
-        for (int a = 0; a < M; a++) {
-            double la = 0.0;
-            cd qa[M];
-#pragma unroll
-            for (int b = 0; b < M; b++) {
-                if constexpr (M == 2) qa[b] = Q2[b][a];
-                else qa[b] = (a < m && b < m) ? Q[b][a] : cd{0.0, 0.0};
+            for (int b = 0; b < 2; b++) {
+                Sq[a][b] = cadd(P[a][b], cconj(P[b][a]));
+                Gq[a][b] = cscale(cadd(G[a][b], cconj(G[b][a])), 0.5);
             }
-            if (ok && a < mm) {
-                // l_a = q_a^H S q_a (Q^H G Q = I); qa is zero beyond m
+        const bool ok = small_geneig2(Sq, Gq, Q);
+        if (ok) {
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                // l_a = q_a^H S q_a (Q^H G Q = I)
                 cd acc{0.0, 0.0};
 #pragma unroll
-                for (int b = 0; b < M; b++) {
+                for (int b = 0; b < 2; b++) {
                     cd t{0.0, 0.0};
 #pragma unroll
-                    for (int c = 0; c < M; c++) {
-                        cd sbc;
-                        if constexpr (M == 2) sbc = Sq2[b][c];
-                        else sbc = (b < m && c < m) ? Sq[b][c] : cd{0.0, 0.0};
-                        cfma(t, sbc, qa[c]);
-                    }
-                    cfmac(acc, qa[b], t);
+                    for (int c = 0; c < 2; c++) cfma(t, Sq[b][c], Q[c][a]);
+                    cfmac(acc, Q[b][a], t);
                 }
-                la = acc.x;
-            }
-            ls[a] = la;
+                Wk[c0 + a] = acc.x;
 #pragma unroll
-            for (int b = 0; b < M; b++) Qs[b][a] = (ok && a < mm && b < mm) ? qa[b] : cd{0.0, 0.0};
+                for (int b = 0; b < 2; b++) Qs[b][a] = Q[b][a];
+            }
+        } else {
+            atomicAdd(nfail, 1);
         }
-        if (ok)
-            for (int a = 0; a < mm; a++) Wk[c0 + a] = ls[a];
         *okflag = ok ? 1 : 0;
     }
     __syncthreads();
     // u_a <- sum_b u_b Q[b][a] (skipped when G had no Cholesky factor)
     if (*okflag) {
         for (int r = lane; r < n; r += 64) {
-            cd u[M];
+            const cd u0{ust[0][r].x, ust[0][r].y}, u1{ust[1][r].x, ust[1][r].y};
 #pragma unroll
-            for (int b = 0; b < M; b++) u[b] = cd{ust[b][r].x, ust[b][r].y};
-            for (int a = 0; a < (M == 2 ? 2 : m); a++) {
+            for (int a = 0; a < 2; a++) {
                 cd acc{0.0, 0.0};
-#pragma unroll
-                for (int b = 0; b < M; b++) cfma(acc, u[b], Qs[b][a]);
+                cfma(acc, u0, Qs[0][a]);
+                cfma(acc, u1, Qs[1][a]);
                 stx(Uk, (long)r * n + c0 + a, cx<float>{(float)acc.x, (float)acc.y});
             }
         }
@@ -599,31 +579,131 @@ __device__ void pair_rr_run(const double *Ak, const double *Hk, float *Uk, doubl
     __syncthreads();
 }
 
-// One wave per run (persistent over the list); pairs, the common case, on the
-// register-resident 2 x 2 path.
+// One wave per pair (persistent over the list; chunks of 3 or more states are left to
+// pair_rr_long_kernel).
 __global__ __launch_bounds__(64) void pair_rr_kernel(const double *__restrict__ A, const double *__restrict__ Hb,
                                                      float *__restrict__ U, int n, int bw, int cap,
-                                                     const int *__restrict__ list, const int *__restrict__ count,
+                                                     const int *__restrict__ list, int *__restrict__ count,
                                                      double *__restrict__ W) {
-    constexpr int MC = kPairMax;
-    __shared__ cd Qs[MC][MC];
-    __shared__ double ls[MC];
+    __shared__ cd Qs[2][2];
     __shared__ int okflag;
-    __shared__ cx<float> ust[MC][kPairMaxN];       // the run's vectors
+    __shared__ cx<float> ust[2][kPairMaxN];
     const int lane = threadIdx.x;
     const int total = min(count[0], cap);
     const int nb = band_nb(n, bw);
     const bool dense = 2 * bw + 1 >= n;
     for (int r_ = blockIdx.x; r_ < total; r_ += gridDim.x) {
+        if (list[3 * r_ + 2] != 2) continue;
         const long kp = list[3 * r_];
-        const int c0 = list[3 * r_ + 1], m = list[3 * r_ + 2];
+        const int c0 = list[3 * r_ + 1];
+        const double *Hk = Hb ? Hb + 2 * kp * (long)n * nb : nullptr;
+        const double *Ak = Hb ? nullptr : A + 2 * kp * (long)n * n;
+        pair_rr_two(Ak, Hk, U + 2 * kp * (long)n * n, W + kp * n, n, bw, nb, dense, c0, lane, ust, Qs, &okflag,
+                    count + 2);
+    }
+}
+
+// Chunks of 3..kPairMax states: one wave per chunk; the chunk's vectors (FP32) and
+// y_a = B u_a (FP64) staged in LDS (dynamic: kPairMax * kPairMaxN * 24 bytes), lane
+// (a, b) forms P_ab and G_ab, lane 0 solves S q = l G q (small_geneig), all lanes rotate.
+__global__ __launch_bounds__(64) void pair_rr_long_kernel(const double *__restrict__ A, const double *__restrict__ Hb,
+                                                          float *__restrict__ U, int n, int bw, int cap,
+                                                          const int *__restrict__ list, int *__restrict__ count,
+                                                          double *__restrict__ W) {
+    constexpr int MC = kPairMax;
+    extern __shared__ unsigned char lds_raw[];
+    cd(*ys)[kPairMaxN] = reinterpret_cast<cd(*)[kPairMaxN]>(lds_raw);
+    cx<float>(*ust)[kPairMaxN] = reinterpret_cast<cx<float>(*)[kPairMaxN]>(lds_raw + sizeof(cd) * MC * kPairMaxN);
+    __shared__ cd Ps[MC][MC], Gs[MC][MC], Qs[MC][MC];
+    __shared__ int okflag;
+    const int lane = threadIdx.x;
+    const int total = min(count[0], cap);
+    const int nb = band_nb(n, bw);
+    const bool dense = 2 * bw + 1 >= n;
+    for (int r_ = blockIdx.x; r_ < total; r_ += gridDim.x) {
+        const int m = list[3 * r_ + 2];
+        if (m <= 2) continue;
+        const long kp = list[3 * r_];
+        const int c0 = list[3 * r_ + 1];
         const double *Hk = Hb ? Hb + 2 * kp * (long)n * nb : nullptr;
         const double *Ak = Hb ? nullptr : A + 2 * kp * (long)n * n;
         float *Uk = U + 2 * kp * (long)n * n;
-        if (m == 2)
-            pair_rr_run<2>(Ak, Hk, Uk, W + kp * n, n, bw, nb, dense, c0, m, lane, ust, Qs, ls, &okflag);
-        else
-            pair_rr_run<MC>(Ak, Hk, Uk, W + kp * n, n, bw, nb, dense, c0, m, lane, ust, Qs, ls, &okflag);
+        double *Wk = W + kp * n;
+        for (int e = lane; e < MC * n; e += 64) {
+            const int a = e / n, r = e - a * n;
+            ust[a][r] = a < m ? ldx(Uk, (long)r * n + c0 + a) : cx<float>{0.f, 0.f};
+        }
+        __syncthreads();
+        for (int r = lane; r < n; r += 64) {
+            cd y[MC];
+#pragma unroll
+            for (int a = 0; a < MC; a++) y[a] = cd{0.0, 0.0};
+            const int nc = dense ? n - r : bw + 1;
+            for (int k = 0; k < nc; k++) {
+                const int c = r + k >= n ? r + k - n : r + k;
+                cd arc = band_entry(Ak, Hk, n, bw, nb, r, c);
+                if (k == 0) arc = cd{0.5 * arc.x, 0.0};
+#pragma unroll
+                for (int a = 0; a < MC; a++) cfma(y[a], arc, cd{ust[a][c].x, ust[a][c].y});
+            }
+#pragma unroll
+            for (int a = 0; a < MC; a++) ys[a][r] = y[a];
+        }
+        __syncthreads();
+        {
+            const int a = lane >> 3, b = lane & 7;
+            cd p{0.0, 0.0}, gg{0.0, 0.0};
+            if (a < m && b < m)
+                for (int r = 0; r < n; r++) {
+                    const cd ua{ust[a][r].x, ust[a][r].y}, ub{ust[b][r].x, ust[b][r].y};
+                    cfmac(p, ua, ys[b][r]);
+                    cfmac(gg, ua, ub);
+                }
+            Ps[a][b] = p;
+            Gs[a][b] = gg;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            cd Sq[MC][MC], Gq[MC][MC], Q[MC][MC];
+            for (int a = 0; a < MC; a++)
+                for (int b = 0; b < MC; b++) {
+                    const bool in = a < m && b < m;
+                    Sq[a][b] = in ? cadd(Ps[a][b], cconj(Ps[b][a])) : cd{0.0, 0.0};
+                    Gq[a][b] = in ? cscale(cadd(Gs[a][b], cconj(Gs[b][a])), 0.5) : cd{0.0, 0.0};
+                }
+            const bool ok = small_geneig(m, Sq, Gq, Q);
+            if (ok) {
+                for (int a = 0; a < m; a++) {
+                    cd acc{0.0, 0.0};
+                    for (int b = 0; b < m; b++) {
+                        cd t{0.0, 0.0};
+                        for (int c = 0; c < m; c++) cfma(t, Sq[b][c], Q[c][a]);
+                        cfmac(acc, Q[b][a], t);
+                    }
+                    Wk[c0 + a] = acc.x;
+                }
+                for (int a = 0; a < MC; a++)
+                    for (int b = 0; b < MC; b++) Qs[b][a] = (a < m && b < m) ? Q[b][a] : cd{0.0, 0.0};
+            } else {
+                atomicAdd(count + 2, 1);
+            }
+            okflag = ok ? 1 : 0;
+        }
+        __syncthreads();
+        if (okflag) {
+            for (int r = lane; r < n; r += 64) {
+                cd u[MC];
+#pragma unroll
+                for (int b = 0; b < MC; b++) u[b] = cd{ust[b][r].x, ust[b][r].y};
+                for (int a = 0; a < m; a++) {
+                    cd acc{0.0, 0.0};
+#pragma unroll
+                    for (int b = 0; b < MC; b++) cfma(acc, u[b], Qs[b][a]);
+                    stx(Uk, (long)r * n + c0 + a, cx<float>{(float)acc.x, (float)acc.y});
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -637,6 +717,15 @@ int launch_pair_rr(const double *A, const double *Hb, float *U32, int n, int bw,
     hipLaunchKernelGGL(pair_flag_kernel, dim3((n + 255) / 256, (unsigned)nkp), dim3(256), 0, s, W, n, nkp, cap, list,
                        count);
     hipLaunchKernelGGL(pair_rr_kernel, dim3(4096), dim3(64), 0, s, A, Hb, U32, n, bw, cap, list, count, W);
+    const size_t lds = (sizeof(cd) + sizeof(cx<float>)) * kPairMax * kPairMaxN;
+    static std::once_flag once;
+    static hipError_t attr = hipSuccess;
+    std::call_once(once, [&] {
+        attr = hipFuncSetAttribute((const void *)pair_rr_long_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds);
+    });
+    RLM_HIP_CHECK(attr);
+    hipLaunchKernelGGL(pair_rr_long_kernel, dim3(256), dim3(64), lds, s, A, Hb, U32, n, bw, cap, list, count, W);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -14,10 +14,13 @@ int refine_max_cluster();
 int launch_refine(const double *Hb, const float *U32, int n, int bw, long nkp, double *Xd, double *Y, double *Sw,
                   double *Gw, double *Qg, int *Cl, int *nflag, hipStream_t s);
 // Exact FP64 Rayleigh-Ritz of the runs of near-degenerate neighbouring states (gaps
-// below 1e-3 of the spectrum's extent, up to 4 states) after the Rayleigh quotients:
-// eigenvalues W updated, FP32 vectors U32 rotated. Matrices from the band form Hb
-// (offset bw) or, with Hb null, dense column-major A. list: pair_list_capacity(n, nkp)
-// x 3 ints, count: 2 ints (runs found, runs cut at 4 states).
+// below 1e-4 of the spectrum's extent; runs longer than 8 states split into balanced
+// chunks of at most 8) after the Rayleigh quotients: eigenvalues W updated, FP32 vectors
+// U32 rotated. Matrices from the band form Hb (offset bw) or, with Hb null, dense
+// column-major A. list: pair_list_capacity(n, nkp) x 3 ints; count: kPairCounters ints
+// (chunks found (those beyond the list capacity are not solved), runs split, chunks
+// without a Cholesky factor of their Gram matrix).
+constexpr int kPairCounters = 3;
 int pair_list_capacity(int n, long nkp);
 int launch_pair_rr(const double *A, const double *Hb, float *U32, int n, int bw, long nkp, double *W, int *list,
                    int cap, int *count, hipStream_t s);

@@ -30,17 +30,18 @@ static void usage(const char *target) {
     cout << "  -s  seed to generate seeds\n";
     cout << "  -t  # of threads\n";
     cout << "  -p  precision: 0 = FP64 (default), 1 = mixed (FP32 reduction + FP64 refinement)\n";
+    cout << "  -f  first sample (resume a run: samples f..f+c-1, rows appended)\n";
 }
 
 int main(int argc, char *argv[]) {
     // defaults of rlm/init_rlm.cpp:20-28
-    long n_phi = 32, n_mesh = 32, n_sample = 1, num_threads = 1, off_override = -1;
+    long n_phi = 32, n_mesh = 32, n_sample = 1, num_threads = 1, off_override = -1, first = 0;
     int precision = HIP_DIAG_FP64;
     double q = 1, imp = 16;
     unsigned long seed = std::chrono::high_resolution_clock::now().time_since_epoch().count();
     int ch, err_flag = 0;
     // the reference's optstring is "n:q:i:g:c:s:h:t:"; -d (documented there) is accepted here
-    while ((ch = getopt(argc, argv, "n:q:i:g:d:c:s:h:t:p:")) != -1) {
+    while ((ch = getopt(argc, argv, "n:q:i:g:d:c:s:h:t:p:f:")) != -1) {
         switch (ch) {
             case 'n': n_phi = atoi(optarg); break;
             case 'q': q = atof(optarg); break;
@@ -51,10 +52,11 @@ int main(int argc, char *argv[]) {
             case 's': seed = atoi(optarg); break;
             case 't': num_threads = atoi(optarg); break;
             case 'p': precision = atoi(optarg) == 1 ? HIP_DIAG_MIXED : HIP_DIAG_FP64; break;
+            case 'f': first = atol(optarg); break;
             default: err_flag++; break;
         }
     }
-    if (err_flag) {
+    if (err_flag || first < 0) {
         usage(argv[0]);
         exit(2);
     }
@@ -87,17 +89,30 @@ int main(int argc, char *argv[]) {
     const double coeff_time = chrono::duration_cast<chrono::microseconds>(t2 - t1).count() / 1.0E6;
     cerr << std::left << setw(40) << "coefficients initialization time: " << coeff_time << " s\n";
 
-    ofstream fchern("energy_Chern_No.txt");
+    ofstream fchern("energy_Chern_No.txt", first > 0 ? ios::app : ios::trunc);
     vector<double> E(spb * n_phi), C(spb * n_phi);
     const double nkp = (double)(n_mesh + 1) * (n_mesh + 1);
     for (long s0 = 0; s0 < n_sample; s0 += spb) {
         const int cnt = (int)min(spb, n_sample - s0);
         t1 = chrono::high_resolution_clock::now();
-        if ((rc = rlm_plan_load(plan, s0, cnt))) break;
+        if ((rc = rlm_plan_load(plan, first + s0, cnt))) break;
         t2 = chrono::high_resolution_clock::now();
         const double load_time = chrono::duration_cast<chrono::microseconds>(t2 - t1).count() / 1.0E6;
         if ((rc = rlm_plan_execute(plan))) break;
-        if ((rc = rlm_plan_fetch(plan, E.data(), C.data(), nullptr))) break;
+        if (precision == HIP_DIAG_MIXED) {
+            // the mixed path's own diagnostics: near-degenerate runs it could not solve exactly
+            long pi[4] = {0, 0, 0, 0}, ri[2] = {0, 0};
+            rlm_plan_pair_info(plan, pi);
+            rlm_plan_refine_info(plan, ri);
+            if (pi[2] || pi[3] || ri[1])
+                cerr << "warning: samples " << first + s0 << ".." << first + s0 + cnt - 1 << ": " << pi[2] + pi[3]
+                     << " near-degenerate runs unsolved, " << ri[1]
+                     << " refinement chunks cut or unfactorised (energies may carry FP32 coupling errors)" << endl;
+        }
+        if ((rc = rlm_plan_fetch(plan, E.data(), C.data(), nullptr))) {
+            if (rc > 0) rc += (int)s0;   // 1 + the run's sample with a non-finite value
+            break;
+        }
         if (s0 == 0) {
             double ms[8];
             rlm_plan_stage_times(plan, ms);
@@ -123,12 +138,15 @@ int main(int argc, char *argv[]) {
         // rlm/main_rlm_cpu.cpp:241-242
         for (int s = 0; s < cnt; s++)
             for (long i = 0; i < n_phi; i++)
-                fchern << std::left << setw(8) << s0 + s << " " << setw(15) << setprecision(8)
+                fchern << std::left << setw(8) << first + s0 + s << " " << setw(15) << setprecision(8)
                        << E[s * n_phi + i] << " " << C[s * n_phi + i] << endl;
     }
     rlm_plan_destroy(plan);
     if (rc) {
-        cerr << "rlm pipeline failed with status " << rc << endl;
+        if (rc > 0)
+            cerr << "rlm pipeline: sample " << first + rc - 1 << " has a non-finite energy or Chern value" << endl;
+        else
+            cerr << "rlm pipeline failed with status " << rc << endl;
         return 1;
     }
     return 0;

@@ -76,6 +76,7 @@ def lib() -> ctypes.CDLL:
         "rlm_plan_max_residual": ([vp, dp], i),
         "rlm_plan_eigenvectors": ([vp], vp),
         "rlm_plan_refine_info": ([vp, ctypes.POINTER(ctypes.c_long)], i),
+        "rlm_plan_pair_info": ([vp, ctypes.POINTER(ctypes.c_long)], i),
         "rlm_plan_loaded": ([vp], i),
         "rlm_plan_destroy": ([vp], i),
         "hlm_plan_create": ([ctypes.POINTER(vp), i, i, i, i, i, ctypes.c_ulong, d, i, i], i),
@@ -243,6 +244,12 @@ class RlmPlan:
         """rlm_plan_write_wfs: line theta_1 of batch sample `sample` -> <dir>/wfs_<theta_1%2>_<theta_2>.bin."""
         _check(lib().rlm_plan_write_wfs(self._h, sample, theta_1, directory.encode(), elem_bytes),
                "rlm_plan_write_wfs")
+
+    def pair_info(self) -> dict:
+        """rlm_plan_pair_info of the last execute."""
+        out = (ctypes.c_long * 4)()
+        _check(lib().rlm_plan_pair_info(self._h, out), "rlm_plan_pair_info")
+        return dict(zip(("solved", "split", "dropped", "unrotated"), list(out)))
 
     def eigenvalues_ptr(self) -> int:
         return lib().rlm_plan_eigenvalues(self._h) or 0

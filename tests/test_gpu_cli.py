@@ -40,9 +40,29 @@ def test_rlm_cli(tmp_path, oracle, prec):
         c = np.array([x[2] for x in got])
         assert np.abs(e - ref.energy).max() <= 1e-7 * np.abs(ref.energy).max()   # 8 printed digits
         assert np.array_equal(np.rint(c), np.rint(ref.chern))
-        # the energy column is the reference's left-aligned setw(15) %.8g field
+        # the energy column is the reference's left-aligned setw(15) %.8g field: the FP64
+        # path prints the oracle's text exactly; the mixed one (energies within 1e-12 of FP64,
+        # which can move the 8th digit) prints the same layout
         for (_, _, _, line), ev in zip(got, ref.energy):
-            assert line.startswith(f"{s:<8} {'%.8g' % ev:<15} ") or prec == "1"
+            field = line[9:24]
+            assert line[:9] == f"{s:<8} " and line[24] == " ", line
+            assert field == f"{'%.8g' % float(field):<15}", line
+            if prec == "0":
+                assert line.startswith(f"{s:<8} {'%.8g' % ev:<15} "), line
+
+
+def test_rlm_cli_resume_from_sample(tmp_path):
+    """-f resumes at a sample offset (rlm/disorder_potential.cpp:6 makes every sample
+    addressable): samples 0-1 of one run then sample 2 of a resumed run equal samples 0-2
+    of a single run, rows appended."""
+    args = [os.path.join(BIN, "rlm"), "-n", "16", "-g", "6", "-s", "9"]
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    assert subprocess.run(args + ["-c", "3"], cwd=a, capture_output=True, timeout=120).returncode == 0
+    assert subprocess.run(args + ["-c", "2"], cwd=b, capture_output=True, timeout=120).returncode == 0
+    assert subprocess.run(args + ["-c", "1", "-f", "2"], cwd=b, capture_output=True, timeout=120).returncode == 0
+    assert (a / "energy_Chern_No.txt").read_text() == (b / "energy_Chern_No.txt").read_text()
 
 
 def test_rlm_cli_rejects_bad_flags(tmp_path):
@@ -66,3 +86,6 @@ def test_hlm_cli(tmp_path, oracle):
         assert np.abs(e - ref.energy).max() <= 1e-7 * np.abs(ref.energy).max()
         assert np.array_equal(np.rint(c), np.rint(ref.chern))
         assert got[0][3].startswith(f"{s:>8} ")
+    for line in ("diagonalization time/k-point:", "Chern No. calculation time/k-point:", "estimated total time:",
+                 "estimated # of samples/hour:", "# module", "diagonalization", "Chern No. calculation"):
+        assert line in r.stderr, line

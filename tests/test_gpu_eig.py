@@ -120,6 +120,64 @@ def test_uplo_and_hamil_entry():
     assert np.abs(wh - wr).max() <= 1e-12 * np.linalg.norm(H, 2)
 
 
+def test_diag_hamil_void_entry():
+    """hip_diag_hamil_v, the void signature of cusolver_diag_hamil / magma_diag_hamil
+    (rlm/main_rlm_gpu.cu:176,216): same result as hip_diag_hamil."""
+    import torch
+    from rlm_amd import lib
+    print("[test_diag_hamil_void_entry]", flush=True)
+    n = 40
+    H = _rand_herm(np.random.default_rng(11), 1, n)[0]
+    A = torch.from_numpy(H.T.copy()).cuda()
+    w = torch.zeros(n, dtype=torch.float64, device="cuda")
+    lib().hip_diag_hamil_v(A.data_ptr(), w.data_ptr(), n)
+    torch.cuda.synchronize()
+    _check(H[None], w.cpu().numpy()[None], A.cpu().numpy().T[None])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_nonfinite_input_status(precision):
+    """A NaN or an infinity in the triangle read stops the call before any solver kernel
+    with status 1 + the index of the first such matrix; d_A and d_W stay untouched, and
+    the next call on finite input solves normally."""
+    import ctypes
+    import torch
+    from rlm_amd import lib
+    print(f"[test_nonfinite_input_status] precision {precision}", flush=True)
+    b, n = 4, 24
+    H = _rand_herm(np.random.default_rng(2), b, n)
+
+    def call(Hs, uplo=b"L"):
+        A = torch.from_numpy(Hs.transpose(0, 2, 1).copy()).cuda()
+        A0 = A.clone()
+        W = torch.full((b, n), 7.0, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        rc = lib().hip_diag_heevd_batched(n, A.data_ptr(), n, n * n, W.data_ptr(), n, b, uplo, precision, None)
+        return rc, A, A0, W
+
+    bad = H.copy()
+    bad[2, 5, 3] = np.nan                    # lower triangle (row 5, column 3)
+    rc, A, A0, W = call(bad)
+    assert rc == 3, rc
+    assert torch.equal(torch.isnan(A), torch.isnan(A0)) and torch.equal(A.nan_to_num(), A0.nan_to_num())
+    assert bool((W == 7.0).all())
+    bad = H.copy()
+    bad[1, 0, 9] = np.inf + 0j               # upper triangle: ignored by 'L', caught by 'U'
+    rc, *_ = call(bad, b"L")
+    assert rc == 0
+    rc, *_ = call(bad, b"U")
+    assert rc == 2, rc
+    rc, A, _, W = call(H)
+    assert rc == 0
+    if precision == 0:
+        _check(H, W.cpu().numpy(), A.cpu().numpy().transpose(0, 2, 1))
+    # host entry (mkl_heevd's replacement)
+    h = np.ascontiguousarray(H[0].T)
+    h[0, 0] = np.nan
+    wh = np.zeros(n)
+    assert lib().hip_heevd(h.ctypes.data_as(ctypes.c_void_p), wh.ctypes.data_as(ctypes.c_void_p), n) == 1
+
+
 def test_invalid_arguments():
     from rlm_amd import lib
     print("[test_invalid_arguments]", flush=True)
@@ -190,3 +248,30 @@ def test_mixed_precision_structured():
     H = np.stack([(M + M.conj().T) / 2 for M in mats])
     w, U = heevd_batched(torch.from_numpy(H).cuda(), precision=HIP_DIAG_MIXED)
     _check_mixed(H, w.cpu().numpy(), U.cpu().numpy())
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_mixed_near_degenerate_clusters(n):
+    """Mixed solver on H = Q diag(l) Q^H with a 5-fold and an 8-fold cluster of eigenvalues
+    1e-7 apart (relative to an extent of 2) and a pair: the FP32 vectors mix inside each
+    cluster, and a Rayleigh quotient would then err by |coupling|^2 / gap ~ 1e-7. The exact
+    FP64 Rayleigh-Ritz of each run (up to 8 states in one chunk) must bring every
+    eigenvalue within 1e-10 of the constructed spectrum. (Longer runs are split into
+    chunks, whose couplings across the split remain: rlm_plan_pair_info counts them.)"""
+    import torch
+    from rlm_amd import heevd_batched, HIP_DIAG_MIXED
+    print(f"[test_mixed_near_degenerate_clusters] n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    lam = np.linspace(-1.0, 1.0, n)          # gaps 2/(n-1), far above 1e-4 of the extent
+    for c0, m in ((n // 4, 5), (n // 2, 8), (3 * n // 4, 2)):
+        lam[c0:c0 + m] = lam[c0] + 1e-7 * np.arange(m)
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    Q, _ = np.linalg.qr(X)
+    H = (Q * lam[None, :]) @ Q.conj().T
+    H = (H + H.conj().T) / 2
+    w, U = heevd_batched(torch.from_numpy(H[None]).cuda(), precision=HIP_DIAG_MIXED)
+    w = w.cpu().numpy()[0]
+    wr = np.linalg.eigvalsh(H)
+    assert np.abs(w - wr).max() <= 1e-10, np.abs(w - wr).max()
+    U = U.cpu().numpy()[0]
+    assert np.abs(H @ U - U * w[None, :]).max() <= 3e-5 * max(1, n / 64)

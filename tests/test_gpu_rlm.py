@@ -173,6 +173,23 @@ def test_derived_edges_match_full_mesh(monkeypatch, n_phi, g, seed, precision):
           f"{np.abs(c - np.rint(c)).max():.3e}, full {np.abs(cf - np.rint(cf)).max():.3e}", flush=True)
 
 
+def test_mixed_pair_info_reports_runs():
+    """rlm_plan_pair_info after a mixed execute at n_phi = 256 (seed 11 holds a
+    near-degenerate pair at every twist point): runs found and solved, none dropped or
+    left unrotated; FP64 plans report zeros."""
+    from rlm_amd import RlmPlan, HIP_DIAG_FP64, HIP_DIAG_MIXED
+    print("[test_mixed_pair_info_reports_runs]", flush=True)
+    plan = RlmPlan(256, 1.0, 16.0, 4, 11, samples_per_batch=1, precision=HIP_DIAG_MIXED)
+    plan.run(0, 1)
+    info = plan.pair_info()
+    plan.close()
+    assert info["solved"] >= 16 and info["dropped"] == 0 and info["unrotated"] == 0, info
+    plan = RlmPlan(64, 1.0, 16.0, 4, 11, samples_per_batch=1, precision=HIP_DIAG_FP64)
+    plan.run(0, 1)
+    assert plan.pair_info() == {"solved": 0, "split": 0, "dropped": 0, "unrotated": 0}
+    plan.close()
+
+
 def test_mixed_near_degenerate_pair_vs_fp64():
     """The uneven-batch size n_phi = 256 at g = 16, 16 samples of seed 11: sample 0 holds a
     pair of states 2e-5 apart (states 15 and 16 at every twist point) whose FP32 vectors mix;
