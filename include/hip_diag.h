@@ -46,6 +46,17 @@ int hip_diag_hamil(double *d_A, double *d_w, int n);
  * (rlm/main_rlm_gpu.cu:176,216). Errors are printed to stderr. */
 void hip_diag_hamil_v(double *d_A, double *d_w, int n);
 
+/* No counterpart in the reference: hip_diag_hamil for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same contract as hip_diag_hamil (FP64, lower
+ * triangle read, in place, synchronous) for n <= 256; n > 256 returns -22. Every
+ * intermediate lives in one 4 MiB device array of the library (per GPU, held
+ * until the process ends, located with hipGetSymbolAddress at the first call on
+ * each device); no hipMalloc, hipFree or allocating library call. Calls are
+ * serialised within the process and share the array, which keeps the last call's
+ * contents; calls from several processes use their own copies. */
+int hip_diag_hamil_noalloc(double *d_A, double *d_w, int n);
+
 /* Replaces mkl_heevd (diag_wrappers/mkl_diag.h:13, mkl_diag.cpp:2-9):
  * LAPACKE_zheevd(COL_MAJOR, 'V', 'U', n, A, n, w) on host memory, computed on
  * the GPU. */

@@ -440,6 +440,76 @@ extern "C" int hip_diag_hamil(double *d_A, double *d_w, int n) {
     return hip_diag_heevd_batched(n, d_A, n, (long)n * n, d_w, n, 1, 'L', HIP_DIAG_FP64, nullptr);
 }
 
+// ---------------------------------------------------------------- allocation-free entry
+// hip_diag_hamil_noalloc keeps every intermediate (packed matrix, eigenvalues, d / e /
+// tau, eigenvectors, divide & conquer workspace, triangular factors, status flags) in
+// one file-scope device array: kNoAllocMaxN = 256 needs 468 224 doubles (3.6 MiB); the
+// array holds 4 MiB. Each GPU holds its own copy; its address on the current device
+// comes from hipGetSymbolAddress once per device (the array's name is a host-side
+// stand-in, never a device address).
+constexpr int kNoAllocMaxN = 256;
+constexpr size_t kScratchBytes = 4u << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;                        // serialises the calls of the process
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// regions of the array, each starting on a 256-byte boundary (32 doubles)
+static size_t pad32(size_t d) { return (d + 31) & ~(size_t)31; }
+static size_t noalloc_doubles(int n) {
+    const size_t nn = (size_t)n * n;
+    return pad32(2 * nn) + pad32(n) + pad32(4 * (size_t)n) + pad32(2 * nn) + pad32(3 * nn) +
+           pad32(2 * (size_t)backtransform_tws_complex(n)) + pad32((size_t)chunks_per_matrix(n));   // + flags
+}
+
+extern "C" int hip_diag_hamil_noalloc(double *d_A, double *d_w, int n) {
+    if (!d_A || !d_w || n < 1 || n > kNoAllocMaxN) return -22;
+    if (noalloc_doubles(n) * sizeof(double) > kScratchBytes) return -22;
+    int dev = 0;
+    RLM_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return -22;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        RLM_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = ptr;
+    }
+    double *base = static_cast<double *>(g_scratch_ptr[dev]);
+    const long nn = (long)n * n;
+    const size_t tws = 2 * (size_t)backtransform_tws_complex(n);
+    double *P = base;                                  // packed full Hermitian matrix (2 nn)
+    double *Wp = P + pad32(2 * nn);                    // eigenvalues (n)
+    EigWork view;                                      // non-owning views (never released)
+    view.dev.p = Wp + pad32(n);                        // d | e | tau (4 n)
+    view.dev.bytes = sizeof(double) * 4 * (size_t)n;
+    view.U.p = static_cast<double *>(view.dev.p) + pad32(4 * (size_t)n);   // eigenvectors (2 nn)
+    view.U.bytes = sizeof(double) * 2 * (size_t)nn;
+    view.work.p = static_cast<double *>(view.U.p) + pad32(2 * nn);          // divide & conquer (3 nn)
+    view.work.bytes = sizeof(double) * 3 * (size_t)nn;
+    view.tw.p = static_cast<double *>(view.work.p) + pad32(3 * nn);         // triangular factors
+    view.tw.bytes = sizeof(double) * tws;
+    int *flags = reinterpret_cast<int *>(static_cast<double *>(view.tw.p) + pad32(tws));
+    const int nch = chunks_per_matrix(n);
+    const dim3 grid((unsigned)nch);
+    hipStream_t s = nullptr;
+    hipLaunchKernelGGL(pack_kernel, grid, dim3(256), 0, s, d_A, (long)n, nn, n, P, nch);
+    hipLaunchKernelGGL(hermitize_kernel, grid, dim3(256), 0, s, P, (long)n, nn, n, 1, nch);
+    hipLaunchKernelGGL(nonfinite_kernel, grid, dim3(256), 0, s, P, nn, nch, flags);
+    RLM_HIP_CHECK(hipGetLastError());
+    int hflag[256];
+    RLM_HIP_CHECK(hipMemcpyAsync(hflag, flags, sizeof(int) * nch, hipMemcpyDeviceToHost, s));
+    RLM_HIP_CHECK(hipStreamSynchronize(s));
+    for (int k = 0; k < nch; k++)
+        if (hflag[k]) return 1;
+    int rc;
+    if ((rc = eig_packed(P, n, 1, Wp, view.U.d(), view, s))) return rc;
+    hipLaunchKernelGGL(transpose_out_kernel, grid, dim3(256), 0, s, view.U.d(), nn, n, d_A, (long)n, nn, nch);
+    RLM_HIP_CHECK(hipGetLastError());
+    RLM_HIP_CHECK(hipMemcpyAsync(d_w, Wp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    RLM_HIP_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
 extern "C" void hip_diag_hamil_v(double *d_A, double *d_w, int n) {
     int rc = hip_diag_hamil(d_A, d_w, n);
     if (rc) fprintf(stderr, "[rlm_hip] hip_diag_hamil failed with status %d\n", rc);

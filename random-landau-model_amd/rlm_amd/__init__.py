@@ -52,6 +52,7 @@ def lib() -> ctypes.CDLL:
     sig = {
         "hip_diag_hamil": ([dp, dp, i], i),
         "hip_diag_hamil_v": ([dp, dp, i], None),
+        "hip_diag_hamil_noalloc": ([dp, dp, i], i),
         "hip_heevd": ([dp, dp, i], i),
         "hip_diag_heevd_batched": ([i, dp, l, l, dp, l, i, ctypes.c_char, i, vp], i),
         "hip_cal_chern": ([dp, dp, i, i, i, i], i),
