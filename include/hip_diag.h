@@ -178,8 +178,11 @@ int rlm_plan_run(rlm_plan_t plan, long first_sample, long n_samples, double *ene
  * recorded on the plan's stream: [0] V_mn + band tables, [1] assembly,
  * [2] tridiagonalisation, [3] divide & conquer, [4] back-transform, [5] FP64
  * refinement (mixed plans; 0 otherwise), [6] link overlaps, [7] plaquettes +
- * energies. Returns 8 (the number of stages). */
+ * energies. Returns 8 (the number of stages), or 0 after an execute in graph mode
+ * (whose events carry no stage timing). */
 int rlm_plan_stage_times(rlm_plan_t plan, double *ms);
+/* GPU time of the whole last rlm_plan_execute in ms (eager and graph mode). */
+int rlm_plan_elapsed_ms(rlm_plan_t plan, double *ms);
 /* The plan's hipStream_t. */
 void *rlm_plan_stream(rlm_plan_t plan);
 /* Samples of the current batch (the count of the last rlm_plan_load, also inside
@@ -232,6 +235,30 @@ int rlm_plan_refine_info(rlm_plan_t plan, long *out2);
 int rlm_plan_pair_info(rlm_plan_t plan, long *out4);
 int rlm_plan_destroy(rlm_plan_t plan);
 
+/* Graph mode of an rlm plan (no reference counterpart; the reference launches
+ * every kernel from the host, rlm/main_rlm_gpu.cu:198-229): with enable != 0 the
+ * launches of rlm_plan_execute are captured once into a hipGraph (again when the
+ * loaded count or the first sample's jump changes) and replayed with one
+ * hipGraphLaunch; rlm_plan_elapsed_ms times it. RLM_GRAPH=1 in the environment at
+ * rlm_plan_create enables it. */
+int rlm_plan_set_graph(rlm_plan_t plan, int enable);
+
+/* Several rlm sizes in one launch (finite-size scaling batches of different n_phi;
+ * no reference counterpart): one plan per n_phi[k] (same q, impurity
+ * concentration, mesh, seed, samples per batch and precision), executed as ONE
+ * hipGraph launch whose branches, one per size, run concurrently on the GPU.
+ * rlm_multi_fetch(k) returns size k's results as rlm_plan_fetch does;
+ * rlm_multi_elapsed_ms the GPU time of the last launch. */
+typedef struct rlm_multi_s *rlm_multi_t;
+int rlm_multi_create(rlm_multi_t *multi, int nsizes, const long *n_phi, double quanta_concentration,
+                     double impurity_concentration, long n_mesh, unsigned long seed, int samples_per_batch,
+                     int precision);
+int rlm_multi_load(rlm_multi_t multi, long first_sample, int count);
+int rlm_multi_execute(rlm_multi_t multi);
+int rlm_multi_fetch(rlm_multi_t multi, int size_index, double *energy, double *chern, double *margin);
+int rlm_multi_elapsed_ms(rlm_multi_t multi, double *ms);
+int rlm_multi_destroy(rlm_multi_t multi);
+
 /* Replaces the hlm sample / theta loops of hlm/main_hlm_cpu.cpp:75-196 (with
  * hlm/hamiltonian_hlm_cpu.cpp:2-56,149-184): lx x ly lattice, flux 1/Q, the
  * Hamiltonian projected on Bloch band `band` (n_phi = lx*ly/Q states), twist
@@ -254,6 +281,14 @@ void *hlm_plan_stream(hlm_plan_t plan);
 /* Samples of the current batch, as rlm_plan_loaded. */
 int hlm_plan_loaded(hlm_plan_t plan);
 int hlm_plan_destroy(hlm_plan_t plan);
+
+/* The tridiagonalisation kernel the library's autotuner chose for matrices of size
+ * n (1..512) in precision HIP_DIAG_FP64 / HIP_DIAG_MIXED (timed at the first plan or
+ * batched call of that size and kept per process and in the tune file
+ * RLM_TUNE_CACHE, default $HOME/.cache/rlm_hip_tune.txt): "resident", "lower",
+ * "lower8", "hybrid", "fused", "blocked", "small", or "auto" (not tuned: fixed
+ * defaults). RLM_TUNE=0 disables tuning, RLM_HETRD=<name> forces a kernel. */
+const char *hip_diag_tuned_kernel(int n, int precision);
 /* hlm/main_hlm_cpu.cpp:66-67,79: samples first..first+n_samples-1 of the run's
  * single stream, n_sites values each, scaled to [-W/2, W/2). Host memory. */
 int hlm_generate_disorder(double *V, long n_sites, long n_samples, unsigned long seed, long first_sample, double W);

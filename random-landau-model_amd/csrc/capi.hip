@@ -368,6 +368,7 @@ extern "C" int hip_diag_heevd_batched(int n, double *d_A, long lda, long strideA
     hipStream_t s = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(g_single_mu);
     int rc;
+    if (n <= kSmallMaxN && (rc = tune_hetrd(n, precision == HIP_DIAG_MIXED, batch))) return rc;
     if ((rc = g_single_ws.ensure(n, batch))) return rc;
     if ((rc = g_single_pack.ensure(sizeof(double) * 2 * (size_t)n * n * batch + sizeof(double) * (size_t)n * batch)))
         return rc;
@@ -737,6 +738,11 @@ struct rlm_plan_s {
     long first = 0;
     static constexpr int kStages = 8;
     hipEvent_t ev[kStages + 1] = {};
+    // graph mode (rlm_plan_set_graph, or RLM_GRAPH=1 at creation)
+    bool graph = false;
+    bool staged = false;   // the last execute recorded every stage event (eager mode)
+    hipGraphExec_t gexec = nullptr;
+    long gkey[3] = {0, 0, 0};
 };
 
 extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double imp, long n_mesh, long off_override,
@@ -754,6 +760,10 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
     p->spb = samples_per_batch;
     p->seed = (uint32_t)seed;
     p->precision = precision;
+    {
+        const char *gm = getenv("RLM_GRAPH");
+        p->graph = gm && strcmp(gm, "1") == 0;
+    }
     const long N = n_phi, d = p->d, g1 = n_mesh + 1, S = samples_per_batch;
     {
         const char *mesh = getenv("RLM_MESH");
@@ -762,9 +772,16 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
     const long nkp = S * p->pl * p->pl;   // solved twist points
     const long nlk = S * g1 * g1;         // twist points with links
     p->nb = band_width((int)N, (int)p->off);
+    int rc0 = 0;
     if (precision == HIP_DIAG_MIXED) {
         const char *rf = getenv("RLM_REFINE");
         p->refine = rf ? strcmp(rf, "0") != 0 : N > kSmallMaxN;
+    }
+    // autotuned tridiagonalisation kernel for this size and precision (decides whether the
+    // mixed reduction can read the band form)
+    if (N <= kSmallMaxN && (rc0 = tune_hetrd((int)N, precision == HIP_DIAG_MIXED, nkp))) {
+        rlm_plan_destroy(p);
+        return rc0;
     }
     p->band_trd = precision == HIP_DIAG_MIXED && chetrd_band_supported((int)N);
     const long nb = p->nb;
@@ -827,8 +844,8 @@ extern "C" int rlm_plan_load(rlm_plan_t p, long first_sample, int count) {
     return 0;
 }
 
-extern "C" int rlm_plan_execute(rlm_plan_t p) {
-    if (!p || p->loaded < 1) return -22;
+// Every launch of one execute on the plan's stream (stage events when ev).
+static int rlm_launches(rlm_plan_t p, bool ev) {
     const int S = p->loaded;
     const int N = (int)p->n_phi, d = (int)p->d, off = (int)p->off, g = (int)p->g;
     const long g1 = g + 1, nkp = (long)S * p->pl * p->pl, nlk = (long)S * g1 * g1;
@@ -837,7 +854,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     int rc;
     Range range("rlm_plan_execute");
     roctxMarkA("rlm: disorder streams + V_mn + band tables");
-    RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
+    if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
     // disorder of the loaded samples from the seed (rlm/disorder_potential.cpp:3-21)
     if ((rc = p->jumps.launch(p->seed, S, 0, 3 * p->n_imp, p->n_imp, p->L1, p->L2, p->dX.d(), p->dY.d(), p->dS.d(),
                               s)))
@@ -845,7 +862,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
     if ((rc = launch_potential(p->dX.d(), p->dY.d(), p->dS.d(), p->n_imp, p->dcmn.d(), d, off, p->dV.d(), S, s)))
         return rc;
     if ((rc = launch_wband(p->dV.d(), p->dcmt.d(), N, d, off, g, S, p->dW.d(), s))) return rc;
-    RLM_HIP_CHECK(hipEventRecord(p->ev[1], s));
+    if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[1], s));
     roctxMarkA("rlm: assembly");
     // band form of every k-point's Hamiltonian; dense FP64 matrices where a kernel needs them
     const bool mixed = p->precision == HIP_DIAG_MIXED;
@@ -858,13 +875,13 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
             RLM_HIP_CHECK(hipGetLastError());
         }
     }
-    RLM_HIP_CHECK(hipEventRecord(p->ev[2], s));
+    if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[2], s));
     roctxMarkA("rlm: eigensolve + refinement + links");
     double *L1 = p->dL.d(), *L2 = L1 + 2 * nlk * N;
     if (!mixed) {
-        if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, p->ev[3], p->ev[4]))) return rc;
-        RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
-        RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
+        if ((rc = eig_packed(p->dA.d(), N, nkp, p->dEig.d(), p->dU.d(), p->ws, s, ev ? p->ev[3] : nullptr, ev ? p->ev[4] : nullptr))) return rc;
+        if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
+        if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
         if (derive)
             rc = launch_links_derived(p->dU.p, false, (long)N * N, N, g, S, L1, L2, s);
         else
@@ -872,10 +889,10 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
         if (rc) return rc;
     } else {
         float *U32 = static_cast<float *>(p->dU.p);
-        if ((rc = eig_packed_f32(A32, N, nkp, p->dEig.d(), U32, p->ws, s, p->ev[3], p->ev[4],
+        if ((rc = eig_packed_f32(A32, N, nkp, p->dEig.d(), U32, p->ws, s, ev ? p->ev[3] : nullptr, ev ? p->ev[4] : nullptr,
                                  p->band_trd ? p->dH.d() : nullptr, off)))
             return rc;
-        RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
+        if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[5], s));
         if (p->refine) {
             // FP64 eigenvector refinement, then Rayleigh quotients and links of the
             // refined FP64 vectors
@@ -886,7 +903,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
             if ((rc = launch_rayleigh(nullptr, static_cast<const double *>(Xr), N, off, nkp, p->dEig.d(), nullptr, s,
                                       p->dH.d())))
                 return rc;
-            RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
+            if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
             if (derive)
                 rc = launch_links_derived(Xr, false, (long)N * N, N, g, S, L1, L2, s);
             else
@@ -904,7 +921,7 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
                 if ((rc = launch_pair_rr(nullptr, p->dH.d(), U32, N, off, nkp, p->dEig.d(), pl + 4, cap, pl, s)))
                     return rc;
             }
-            RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
+            if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[6], s));
             if (derive)
                 rc = launch_links_derived(U32, true, (long)N * N, N, g, S, L1, L2, s);
             else
@@ -912,13 +929,67 @@ extern "C" int rlm_plan_execute(rlm_plan_t p) {
             if (rc) return rc;
         }
     }
-    RLM_HIP_CHECK(hipEventRecord(p->ev[7], s));
+    if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[7], s));
     roctxMarkA("rlm: plaquettes + energies");
     double *out = p->dOut.d();
     if ((rc = launch_reduce(L1, L2, p->dEig.d(), N, g, (int)g1, derive ? 2 : 0, (double)N, S, out, out + (long)S * N,
                             out + 2L * S * N, s)))
         return rc;
-    RLM_HIP_CHECK(hipEventRecord(p->ev[8], s));
+    if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[8], s));
+    return 0;
+}
+
+// Graph mode: the launches of one execute captured once into a hipGraph (per loaded
+// count and first-sample jump, the kernel arguments that change between batches) and
+// replayed with one hipGraphLaunch. The stage events are captured with it.
+static int rlm_graph_execute(rlm_plan_t p) {
+    const long key[3] = {p->loaded, p->jumps.a_kind, p->jumps.a_mono};
+    if (!p->gexec || memcmp(key, p->gkey, sizeof(key)) != 0) {
+        if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+        p->gexec = nullptr;
+        RLM_HIP_CHECK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = rlm_launches(p, false);
+        hipGraph_t gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(p->stream, &gr);
+        if (rc || ec != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc ? rc : -(int)ec;
+        }
+        const hipError_t ei = hipGraphInstantiate(&p->gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ei != hipSuccess) {
+            p->gexec = nullptr;
+            return -(int)ei;
+        }
+        memcpy(p->gkey, key, sizeof(key));
+    }
+    // events recorded inside a graph carry no timing: time the whole launch
+    RLM_HIP_CHECK(hipEventRecord(p->ev[0], p->stream));
+    RLM_HIP_CHECK(hipGraphLaunch(p->gexec, p->stream));
+    RLM_HIP_CHECK(hipEventRecord(p->ev[rlm_plan_s::kStages], p->stream));
+    p->staged = false;
+    return 0;
+}
+
+extern "C" int rlm_plan_execute(rlm_plan_t p) {
+    if (!p || p->loaded < 1) return -22;
+    if (p->graph) return rlm_graph_execute(p);
+    p->staged = true;
+    return rlm_launches(p, true);
+}
+
+extern "C" int rlm_plan_elapsed_ms(rlm_plan_t p, double *ms) {
+    if (!p || !ms) return -22;
+    RLM_HIP_CHECK(hipEventSynchronize(p->ev[rlm_plan_s::kStages]));
+    float t = 0.f;
+    RLM_HIP_CHECK(hipEventElapsedTime(&t, p->ev[0], p->ev[rlm_plan_s::kStages]));
+    *ms = t;
+    return 0;
+}
+
+extern "C" int rlm_plan_set_graph(rlm_plan_t p, int enable) {
+    if (!p) return -22;
+    p->graph = enable != 0;
     return 0;
 }
 
@@ -967,6 +1038,7 @@ extern "C" int rlm_plan_info(rlm_plan_t p, long *o) {
 
 extern "C" int rlm_plan_stage_times(rlm_plan_t p, double *ms) {
     if (!p || !ms) return -22;
+    if (!p->staged) return 0;   // graph mode: rlm_plan_elapsed_ms has the total
     RLM_HIP_CHECK(hipEventSynchronize(p->ev[rlm_plan_s::kStages]));
     for (int k = 0; k < rlm_plan_s::kStages; k++) {
         float t = 0.f;
@@ -1074,10 +1146,130 @@ extern "C" int rlm_plan_destroy(rlm_plan_t p) {
     for (DevBuf *b : bufs) b->release();
     p->ws.release();
     p->jumps.release();
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     for (int k = 0; k <= rlm_plan_s::kStages; k++)
         if (p->ev[k]) (void)hipEventDestroy(p->ev[k]);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
+    return 0;
+}
+
+// ============================================================================
+// rlm plans of several sizes in one graph launch
+// ============================================================================
+struct rlm_multi_s {
+    std::vector<rlm_plan_t> plans;
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
+    std::vector<hipEvent_t> joins;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<long> gkey;
+};
+
+extern "C" int rlm_multi_destroy(rlm_multi_t mp) {
+    if (!mp) return 0;
+    if (mp->origin) (void)hipStreamSynchronize(mp->origin);
+    if (mp->gexec) (void)hipGraphExecDestroy(mp->gexec);
+    for (rlm_plan_t p : mp->plans) rlm_plan_destroy(p);
+    for (hipEvent_t e : mp->joins)
+        if (e) (void)hipEventDestroy(e);
+    hipEvent_t evs[] = {mp->fork, mp->e0, mp->e1};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (mp->origin) (void)hipStreamDestroy(mp->origin);
+    delete mp;
+    return 0;
+}
+
+extern "C" int rlm_multi_create(rlm_multi_t *out, int nsizes, const long *n_phi, double q, double imp, long n_mesh,
+                                unsigned long seed, int samples_per_batch, int precision) {
+    if (!out || nsizes < 1 || !n_phi) return -22;
+    auto *mp = new rlm_multi_s();
+    auto fail = [&](int r) {
+        rlm_multi_destroy(mp);
+        return r;
+    };
+    for (int k = 0; k < nsizes; k++) {
+        rlm_plan_t p = nullptr;
+        const int rc = rlm_plan_create(&p, n_phi[k], q, imp, n_mesh, -1, seed, samples_per_batch, precision);
+        if (rc) return fail(rc);
+        mp->plans.push_back(p);
+    }
+    if (hipStreamCreate(&mp->origin) != hipSuccess || hipEventCreate(&mp->e0) != hipSuccess ||
+        hipEventCreate(&mp->e1) != hipSuccess ||
+        hipEventCreateWithFlags(&mp->fork, hipEventDisableTiming) != hipSuccess)
+        return fail(-1);
+    mp->joins.assign(nsizes, nullptr);
+    for (int k = 0; k < nsizes; k++)
+        if (hipEventCreateWithFlags(&mp->joins[k], hipEventDisableTiming) != hipSuccess) return fail(-1);
+    *out = mp;
+    return 0;
+}
+
+extern "C" int rlm_multi_load(rlm_multi_t mp, long first_sample, int count) {
+    if (!mp) return -22;
+    for (rlm_plan_t p : mp->plans) {
+        const int rc = rlm_plan_load(p, first_sample, count);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+extern "C" int rlm_multi_execute(rlm_multi_t mp) {
+    if (!mp) return -22;
+    std::vector<long> key;
+    for (rlm_plan_t p : mp->plans) {
+        if (p->loaded < 1) return -22;
+        key.push_back(p->loaded);
+        key.push_back(p->jumps.a_kind);
+        key.push_back(p->jumps.a_mono);
+    }
+    if (!mp->gexec || key != mp->gkey) {
+        if (mp->gexec) (void)hipGraphExecDestroy(mp->gexec);
+        mp->gexec = nullptr;
+        // one branch per size: each plan's launches on its own stream, forked from and
+        // joined back into the capture of the origin stream
+        RLM_HIP_CHECK(hipStreamBeginCapture(mp->origin, hipStreamCaptureModeThreadLocal));
+        int rc = hipEventRecord(mp->fork, mp->origin) == hipSuccess ? 0 : -1;
+        for (size_t k = 0; k < mp->plans.size() && !rc; k++) {
+            rlm_plan_t p = mp->plans[k];
+            if (hipStreamWaitEvent(p->stream, mp->fork, 0) != hipSuccess) rc = -1;
+            if (!rc) rc = rlm_launches(p, false);
+            if (!rc && hipEventRecord(mp->joins[k], p->stream) != hipSuccess) rc = -1;
+            if (!rc && hipStreamWaitEvent(mp->origin, mp->joins[k], 0) != hipSuccess) rc = -1;
+        }
+        hipGraph_t gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(mp->origin, &gr);
+        if (rc || ec != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc ? rc : -(int)ec;
+        }
+        const hipError_t ei = hipGraphInstantiate(&mp->gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ei != hipSuccess) {
+            mp->gexec = nullptr;
+            return -(int)ei;
+        }
+        mp->gkey = key;
+    }
+    RLM_HIP_CHECK(hipEventRecord(mp->e0, mp->origin));
+    RLM_HIP_CHECK(hipGraphLaunch(mp->gexec, mp->origin));
+    RLM_HIP_CHECK(hipEventRecord(mp->e1, mp->origin));
+    return 0;
+}
+
+extern "C" int rlm_multi_fetch(rlm_multi_t mp, int size_index, double *energy, double *chern, double *margin) {
+    if (!mp || size_index < 0 || size_index >= (int)mp->plans.size()) return -22;
+    RLM_HIP_CHECK(hipStreamSynchronize(mp->origin));
+    return rlm_plan_fetch(mp->plans[size_index], energy, chern, margin);
+}
+
+extern "C" int rlm_multi_elapsed_ms(rlm_multi_t mp, double *ms) {
+    if (!mp || !ms) return -22;
+    RLM_HIP_CHECK(hipEventSynchronize(mp->e1));
+    float t = 0.f;
+    RLM_HIP_CHECK(hipEventElapsedTime(&t, mp->e0, mp->e1));
+    *ms = t;
     return 0;
 }
 
@@ -1136,6 +1328,7 @@ extern "C" int hlm_plan_create(hlm_plan_t *plan, int lx, int ly, int Q, int band
         if (hipEventCreate(&p->ev[k]) != hipSuccess) return fail(-1);
     const long N = n_phi, S = samples_per_batch, ntw = p->ntw, nkp = S * ntw;
     int rc;
+    if ((rc = tune_hetrd((int)N, false, nkp))) return fail(rc);
     if ((rc = p->dPhi.ensure(16 * ntw * n_sites * N)) || (rc = p->dM1.ensure(16 * ntw * N * N)) ||
         (rc = p->dM2.ensure(16 * ntw * N * N)) || (rc = p->dV.ensure(8 * S * n_sites)) ||
         (rc = p->dH.ensure(16 * nkp * N * N)) || (rc = p->dU.ensure(16 * nkp * N * N)) ||
@@ -1264,6 +1457,10 @@ extern "C" int hlm_plan_stage_times(hlm_plan_t p, double *ms) {
 extern "C" void *hlm_plan_stream(hlm_plan_t p) { return p ? (void *)p->stream : nullptr; }
 
 extern "C" int hlm_plan_loaded(hlm_plan_t p) { return p ? p->loaded : -22; }
+
+extern "C" const char *hip_diag_tuned_kernel(int n, int precision) {
+    return hetrd_choice_name(n, precision == HIP_DIAG_MIXED);
+}
 
 extern "C" int hlm_plan_destroy(hlm_plan_t p) {
     if (!p) return 0;

@@ -25,6 +25,14 @@
 #include "common.hpp"
 #include "eig.hpp"
 
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
 namespace rlm {
 
 // ============================================================================
@@ -2919,26 +2927,43 @@ static int resident_max_n(size_t real) { return real == sizeof(float) ? 256 : 12
 
 // RLM_HETRD selects the tridiagonalisation kernel: "resident" (lower triangle in
 // registers), "fused" (unblocked, full storage), "blocked" (panels, full storage),
-// "lower" (panels, lower storage, tiled symv), "hybrid" (lower for the leading
-// columns, resident for the trailing block it fits; slower than lower for complex
-// double at n = 256). Unset or "auto": resident where n fits it, else hybrid for
-// complex float and lower for complex double. Read at every launch so one process can
-// compare them.
-enum { kTrdAuto = 0, kTrdFused, kTrdBlocked, kTrdLower, kTrdResident, kTrdHybrid };
+// "lower" (panels of 16, lower storage, tiled symv), "lower8" (panels of 8), "hybrid"
+// (lower for the leading columns, resident for the trailing block it fits), "small"
+// (complex double n <= 72, full storage in registers). Unset or "auto": the autotuner's
+// choice for (n, precision) when tune_hetrd has run, else the fixed defaults below.
+// Read at every launch so one process can compare them.
+enum { kTrdAuto = 0, kTrdFused, kTrdBlocked, kTrdLower, kTrdResident, kTrdHybrid, kTrdSmall, kTrdLower8 };
+static const char *const kTrdNames[] = {"auto", "fused", "blocked", "lower", "resident", "hybrid", "small", "lower8"};
 static int hetrd_variant() {
     const char *e = getenv("RLM_HETRD");
     if (!e) return kTrdAuto;
-    if (strcmp(e, "fused") == 0) return kTrdFused;
-    if (strcmp(e, "blocked") == 0) return kTrdBlocked;
-    if (strcmp(e, "lower") == 0) return kTrdLower;
-    if (strcmp(e, "resident") == 0) return kTrdResident;
-    if (strcmp(e, "hybrid") == 0) return kTrdHybrid;
+    for (int v = 1; v < (int)(sizeof(kTrdNames) / sizeof(kTrdNames[0])); v++)
+        if (strcmp(e, kTrdNames[v]) == 0) return v;
     return kTrdAuto;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Choices of tune_hetrd per (n, precision), per process; also kept in a text file
+// (RLM_TUNE_CACHE, default $HOME/.cache/rlm_hip_tune.txt: lines "<device> <n> <fp32>
+// <variant>") so that later processes on the same kind of GPU skip the timing.
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, int> g_tuned;
+
+static int tuned_variant(int n, bool fp32) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find({n, fp32 ? 1 : 0});
+    return it == g_tuned.end() ? kTrdAuto : it->second;
+}
+
+// the variant a launch uses: RLM_HETRD, else the tuned choice, else kTrdAuto (defaults)
+static int effective_variant(int n, bool fp32) {
+    const int v = hetrd_variant();
+    return v != kTrdAuto ? v : tuned_variant(n, fp32);
 }
 
 // True when launch_chetrd_band can take band-form input (register-resident kernel).
 bool chetrd_band_supported(int n) {
-    const int v = hetrd_variant();
+    const int v = effective_variant(n, true);
     return (v == kTrdAuto || v == kTrdResident) && n <= resident_max_n(sizeof(float));
 }
 
@@ -2982,30 +3007,49 @@ static int launch_resident_any(T *A, long strideA, int lda, int n, double *D, do
     }
 }
 
+// Whether variant v can reduce complex T matrices of size n.
+template <typename T>
+static bool variant_valid(int v, int n) {
+    const bool dbl = sizeof(T) == sizeof(double);
+    switch (v) {
+        case kTrdSmall: return dbl && n <= 72;
+        case kTrdFused: return n <= 512;
+        case kTrdResident: return n <= resident_max_n(sizeof(T));
+        case kTrdHybrid: return !dbl && n > resident_max_n(sizeof(T)) && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024;
+        case kTrdLower: return n <= 512 && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024;
+        case kTrdLower8: return n <= 512 && lower_lds_bytes(n, 8, sizeof(T)) <= 160 * 1024;
+        case kTrdBlocked: return n <= 512 && blocked_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024;
+        default: return false;
+    }
+}
+
 template <typename T>
 static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, double *E, T *TAU, long strideV, int batch,
                             int grid, hipStream_t stream) {
-    int v = hetrd_variant();
+    int v = effective_variant(n, sizeof(T) == sizeof(float));
     const int rmax = resident_max_n(sizeof(T));
-    // auto: complex double n <= 72 takes zhetrd_small_kernel (registers, two matrices per
-    // CU: hlm n = 72, 67584 matrices: 30.3 ms), 72 < n <= 96 the fused kernel (its small
-    // LDS footprint lets five matrices share a CU, where the resident kernel holds one:
-    // hlm n = 72 52.7 against 111 ms), n <= 128 the resident kernel (n = 128: 23.8 against
-    // 26.5 ms fused)
+    if (v != kTrdAuto && !variant_valid<T>(v, n)) v = kTrdAuto;
+    // defaults (no tuned choice): complex double n <= 72 takes zhetrd_small_kernel
+    // (registers, two matrices per CU: hlm n = 72, 67584 matrices: 30.3 ms), 72 < n <= 96
+    // the fused kernel (its small LDS footprint lets five matrices share a CU, where the
+    // resident kernel holds one: hlm n = 72 52.7 against 111 ms), n <= 128 the resident
+    // kernel (n = 128: 23.8 against 26.5 ms fused); complex float n <= 256 resident, n > 256
+    // hybrid (panels for the leading n - 256 columns, then the trailing 256 x 256 block in
+    // registers: n = 384: 143 -> 99 ms, n = 512: 290 -> 248 ms for 4624 matrices)
     if (v == kTrdAuto) {
-        if (sizeof(T) == sizeof(double) && n <= 72)
-            return launch_zhetrd_small_t<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
-        // complex float n > 256: panels for the leading n - 256 columns, then the trailing
-        // 256 x 256 block in registers (n = 384: 143 -> 99 ms, n = 512: 290 -> 248 ms for
-        // 4624 matrices against panels throughout)
-        v = (sizeof(T) == sizeof(double) && n <= 96) ? kTrdFused
-            : n <= rmax                              ? kTrdResident
-            : sizeof(T) == sizeof(float)             ? kTrdHybrid
-                                                     : kTrdLower;
+        if (sizeof(T) == sizeof(double) && n <= 72) v = kTrdSmall;
+        else if (sizeof(T) == sizeof(double) && n <= 96) v = kTrdFused;
+        else if (n <= rmax) v = kTrdResident;
+        else if (sizeof(T) == sizeof(float) && variant_valid<T>(kTrdHybrid, n)) v = kTrdHybrid;
+        else if (variant_valid<T>(kTrdLower, n)) v = kTrdLower;
+        else if (variant_valid<T>(kTrdLower8, n)) v = kTrdLower8;
+        else v = kTrdFused;
     }
-    if ((v == kTrdResident || v == kTrdHybrid) && n <= rmax)
+    if constexpr (sizeof(T) == sizeof(double))
+        if (v == kTrdSmall) return launch_zhetrd_small_t<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream);
+    if (v == kTrdResident && n <= rmax)
         return launch_resident_any<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, stream, nullptr, 0);
-    if (v == kTrdHybrid && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024) {
+    if (v == kTrdHybrid && variant_valid<T>(kTrdHybrid, n)) {
         // columns 0..ncols-1 by panels of 16 through memory; the updated trailing block
         // A(ncols:n, ncols:n) (at most rmax) is then a Hermitian matrix whose
         // reduction continues on chip, writing d, e, tau and the reflectors in place
@@ -3016,15 +3060,165 @@ static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, doubl
                                       E + ncols, TAU + 2 * ncols, strideV, batch, stream, nullptr, 0);
     }
     // the panel kernels keep W (n x NB) in LDS: panels of 8 where 16 do not fit (complex
-    // double n = 512: 148 KB; 448 against 1848 ms fused for 4624 matrices), beyond 160 KB
-    // the fused kernel
-    if ((v == kTrdLower || v == kTrdResident) && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024)
+    // double n = 512: 148 KB; 448 against 1848 ms fused for 4624 matrices)
+    if ((v == kTrdLower || v == kTrdResident || v == kTrdHybrid) && variant_valid<T>(kTrdLower, n))
         return launch_zhetrd_lower_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
-    if ((v == kTrdLower || v == kTrdResident) && lower_lds_bytes(n, 8, sizeof(T)) <= 160 * 1024)
+    if ((v == kTrdLower || v == kTrdLower8 || v == kTrdResident || v == kTrdHybrid) && variant_valid<T>(kTrdLower8, n))
         return launch_zhetrd_lower_t<T, 8>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
-    if (v == kTrdBlocked && blocked_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024)
+    if (v == kTrdBlocked && variant_valid<T>(kTrdBlocked, n))
         return launch_zhetrd_blocked_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
     return launch_zhetrd_t<T>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream);
+}
+
+// Random Hermitian test matrices of the autotuner (full storage, values in [-1, 1)).
+template <typename T>
+__global__ void random_herm_kernel(T *A, long strideA, int n, unsigned seed) {
+    T *M = A + 2 * blockIdx.x * strideA;
+    for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
+        const int c = (int)(idx / n), r = (int)(idx % n);
+        const int lo = r < c ? r : c, hi = r < c ? c : r;
+        unsigned h = seed ^ (blockIdx.x * 0x9E3779B9u) ^ ((unsigned)lo * 0x85EBCA6Bu) ^ ((unsigned)hi * 0xC2B2AE35u);
+        h ^= h >> 16; h *= 0x7FEB352Du; h ^= h >> 15; h *= 0x846CA68Bu; h ^= h >> 16;
+        const T re = (T)((h & 0xFFFF) * (1.0 / 32768.0) - 1.0), im = (T)((h >> 16) * (1.0 / 32768.0) - 1.0);
+        M[2 * idx] = re;
+        M[2 * idx + 1] = r == c ? (T)0 : (r > c ? im : -im);
+    }
+}
+
+static std::string tune_cache_path() {
+    const char *e = getenv("RLM_TUNE_CACHE");
+    if (e && *e) return std::string(e);
+    const char *home = getenv("HOME");
+    return std::string(home ? home : "/tmp") + "/.cache/rlm_hip_tune.txt";
+}
+
+static std::string device_tag() {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+    std::string arch(prop.gcnArchName);
+    for (char &c : arch)
+        if (c == ' ' || c == ':') c = '_';
+    return arch + "_cu" + std::to_string(prop.multiProcessorCount);
+}
+
+template <typename T>
+static int tune_hetrd_t(int n, long batch_hint, int *chosen) {
+    const bool fp32 = sizeof(T) == sizeof(float);
+    std::vector<int> cand;
+    for (int v = kTrdFused; v <= kTrdLower8; v++)
+        if (variant_valid<T>(v, n)) cand.push_back(v);
+    if (cand.size() <= 1) {
+        *chosen = cand.empty() ? kTrdAuto : cand[0];
+        return 0;
+    }
+    // a sample batch that fills the chip as the real launch does
+    const long cap = n <= 128 ? 1024 : n <= 256 ? 512 : 256;
+    const int S = (int)std::max<long>(1, std::min<long>(batch_hint, cap));
+    const long strideV = 4L * n;
+    T *A = nullptr;
+    double *V = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    auto cleanup = [&]() {
+        if (A) (void)hipFree(A);
+        if (V) (void)hipFree(V);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+    };
+    if (hipMalloc(&A, sizeof(T) * 2 * (size_t)n * n * S) != hipSuccess ||
+        hipMalloc(&V, sizeof(double) * (size_t)strideV * S) != hipSuccess || hipStreamCreate(&s) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        return -12;
+    }
+    double *D = V, *E = V + n;
+    T *TAU = reinterpret_cast<T *>(V + 2 * n);
+    float best = 1e30f;
+    int best_v = kTrdAuto;
+    for (int v : cand) {
+        float t_min = 1e30f;
+        for (int rep = 0; rep < 3 && !rc; rep++) {   // rep 0 warms up (code object load)
+            hipLaunchKernelGGL(random_herm_kernel<T>, dim3(S), dim3(256), 0, s, A, (long)n * n, n, 1234u + rep);
+            if (hipEventRecord(e0, s) != hipSuccess) rc = -1;
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                g_tuned[{n, fp32 ? 1 : 0}] = v;      // launch through the normal dispatch
+            }
+            if (!rc) rc = launch_hetrd_any<T>(A, (long)n * n, n, n, D, E, TAU, strideV, S, 1024, s);
+            if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = -1;
+            if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = -1;
+            float ms = 0.f;
+            if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = -1;
+            if (!rc && rep > 0) t_min = std::min(t_min, ms);
+        }
+        if (rc) break;
+        if (t_min < best) {
+            best = t_min;
+            best_v = v;
+        }
+    }
+    cleanup();
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned.erase({n, fp32 ? 1 : 0});
+    }
+    if (rc) return rc;
+    *chosen = best_v;
+    return 0;
+}
+
+int tune_hetrd(int n, bool fp32, long batch_hint) {
+    if (n < 2 || n > 512) return 0;
+    const char *en = getenv("RLM_TUNE");
+    if ((en && strcmp(en, "0") == 0) || hetrd_variant() != kTrdAuto) return 0;
+    if (tuned_variant(n, fp32) != kTrdAuto) return 0;
+    const std::string tag = device_tag(), path = tune_cache_path();
+    {
+        FILE *f = fopen(path.c_str(), "r");
+        if (f) {
+            char dtag[128], name[32];
+            int fn = 0, ff = 0, found = kTrdAuto;
+            while (fscanf(f, "%127s %d %d %31s", dtag, &fn, &ff, name) == 4)
+                if (tag == dtag && fn == n && ff == (fp32 ? 1 : 0))
+                    for (int v = 1; v < (int)(sizeof(kTrdNames) / sizeof(kTrdNames[0])); v++)
+                        if (strcmp(name, kTrdNames[v]) == 0) found = v;
+            fclose(f);
+            const bool ok = fp32 ? variant_valid<float>(found, n) : variant_valid<double>(found, n);
+            if (found != kTrdAuto && ok) {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                g_tuned[{n, fp32 ? 1 : 0}] = found;
+                return 0;
+            }
+        }
+    }
+    int v = kTrdAuto, rc = fp32 ? tune_hetrd_t<float>(n, batch_hint, &v) : tune_hetrd_t<double>(n, batch_hint, &v);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[{n, fp32 ? 1 : 0}] = v;
+    }
+    if (v != kTrdAuto) {
+        const size_t slash = path.rfind('/');
+        if (slash != std::string::npos) {
+            const std::string dir = path.substr(0, slash);
+            (void)mkdir(dir.c_str(), 0755);
+        }
+        FILE *f = fopen(path.c_str(), "a");
+        if (f) {
+            fprintf(f, "%s %d %d %s\n", tag.c_str(), n, fp32 ? 1 : 0, kTrdNames[v]);
+            fclose(f);
+        }
+    }
+    return 0;
+}
+
+int hetrd_choice(int n, bool fp32) { return effective_variant(n, fp32); }
+const char *hetrd_choice_name(int n, bool fp32) {
+    const int v = effective_variant(n, fp32);
+    return kTrdNames[v >= 0 && v < (int)(sizeof(kTrdNames) / sizeof(kTrdNames[0])) ? v : 0];
 }
 
 int launch_zhetrd(double *A, long strideA, int lda, int n, double *D, double *E, double *TAU, long strideV, int batch,

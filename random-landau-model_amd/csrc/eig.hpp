@@ -39,6 +39,13 @@ int launch_tail_f32(const float *A, long strideA, int lda, int n, const double *
 
 size_t heevd_dc_lds_bytes(int n);
 
+// Autotuner of the tridiagonalisation kernel (n <= 512): times every valid variant on
+// min(batch_hint, 256..1024) random Hermitian matrices and keeps the fastest for
+// (n, precision) in this process (and in the tune file). RLM_TUNE=0 or RLM_HETRD set:
+// no tuning. hetrd_choice_name: the variant launches of (n, precision) use now.
+int tune_hetrd(int n, bool fp32, long batch_hint);
+const char *hetrd_choice_name(int n, bool fp32);
+
 // ---------------------------------------------------------------- 512 < n <= 4096
 // Doubles of global workspace per workgroup of the divide & conquer with its node
 // arrays in global memory: Qa, Qb, S (n*n each), 13 double and 16 int arrays of n,

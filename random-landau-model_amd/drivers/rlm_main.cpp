@@ -114,8 +114,8 @@ int main(int argc, char *argv[]) {
             break;
         }
         if (s0 == 0) {
-            double ms[8];
-            rlm_plan_stage_times(plan, ms);
+            double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (rlm_plan_stage_times(plan, ms) == 0) rlm_plan_elapsed_ms(plan, &ms[2]);   // graph mode: total only
             const double pot = (load_time + ms[0] / 1e3) / cnt;
             const double diag = (ms[1] + ms[2] + ms[3] + ms[4] + ms[5]) / 1e3 / (cnt * nkp);
             const double chern = (ms[6] + ms[7]) / 1e3 / (cnt * nkp);

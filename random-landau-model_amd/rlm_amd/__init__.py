@@ -79,6 +79,15 @@ def lib() -> ctypes.CDLL:
         "rlm_plan_refine_info": ([vp, ctypes.POINTER(ctypes.c_long)], i),
         "rlm_plan_pair_info": ([vp, ctypes.POINTER(ctypes.c_long)], i),
         "rlm_plan_loaded": ([vp], i),
+        "rlm_plan_set_graph": ([vp, i], i),
+        "rlm_plan_elapsed_ms": ([vp, dp], i),
+        "rlm_multi_create": ([ctypes.POINTER(vp), i, ctypes.POINTER(ctypes.c_long), d, d, l, ctypes.c_ulong, i, i], i),
+        "rlm_multi_load": ([vp, l, i], i),
+        "rlm_multi_execute": ([vp], i),
+        "rlm_multi_fetch": ([vp, i, dp, dp, dp], i),
+        "rlm_multi_elapsed_ms": ([vp, dp], i),
+        "rlm_multi_destroy": ([vp], i),
+        "hip_diag_tuned_kernel": ([i, i], ctypes.c_char_p),
         "rlm_plan_destroy": ([vp], i),
         "hlm_plan_create": ([ctypes.POINTER(vp), i, i, i, i, i, ctypes.c_ulong, d, i, i], i),
         "hlm_plan_load": ([vp, l, i], i),
@@ -219,9 +228,18 @@ class RlmPlan:
     STAGES = ("potential+tables", "assembly", "tridiagonalisation", "divide_conquer", "backtransform", "refinement",
               "links", "plaquettes")
 
+    def elapsed_ms(self) -> float:
+        """rlm_plan_elapsed_ms: GPU time of the last execute."""
+        out = np.zeros(1)
+        _check(lib().rlm_plan_elapsed_ms(self._h, _ptr(out)), "rlm_plan_elapsed_ms")
+        return float(out[0])
+
     def stage_times(self) -> dict:
+        """Per-stage GPU times of the last execute ({} after an execute in graph mode)."""
         ms = np.zeros(len(self.STAGES))
         rc = lib().rlm_plan_stage_times(self._h, _ptr(ms))
+        if rc == 0:
+            return {}
         if rc != len(self.STAGES):
             raise HipDiagError(f"rlm_plan_stage_times failed with status {rc}")
         return dict(zip(self.STAGES, ms.tolist()))
@@ -240,6 +258,10 @@ class RlmPlan:
         out = (ctypes.c_long * 2)()
         _check(lib().rlm_plan_refine_info(self._h, out), "rlm_plan_refine_info")
         return bool(out[0]), int(out[1])
+
+    def set_graph(self, enable: bool = True):
+        """rlm_plan_set_graph: replay the execute's launches as one captured hipGraph."""
+        _check(lib().rlm_plan_set_graph(self._h, 1 if enable else 0), "rlm_plan_set_graph")
 
     def write_wfs(self, sample: int, theta_1: int, directory: str, elem_bytes: int = 16):
         """rlm_plan_write_wfs: line theta_1 of batch sample `sample` -> <dir>/wfs_<theta_1%2>_<theta_2>.bin."""
@@ -261,6 +283,53 @@ class RlmPlan:
     def close(self):
         if self._h:
             lib().rlm_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tuned_kernel(n: int, precision: int = HIP_DIAG_FP64) -> str:
+    """hip_diag_tuned_kernel: the tridiagonalisation kernel the autotuner chose."""
+    return lib().hip_diag_tuned_kernel(n, precision).decode()
+
+
+class RlmMulti:
+    """rlm_multi_*: several n_phi in one hipGraph launch (one branch per size)."""
+
+    def __init__(self, sizes, q=1.0, imp=16.0, n_mesh=16, seed=1, samples_per_batch=1, precision=HIP_DIAG_FP64):
+        self._h = ctypes.c_void_p()
+        self.sizes = [int(n) for n in sizes]
+        arr = (ctypes.c_long * len(self.sizes))(*self.sizes)
+        _check(lib().rlm_multi_create(ctypes.byref(self._h), len(self.sizes), arr, q, imp, n_mesh, seed,
+                                      samples_per_batch, precision), "rlm_multi_create")
+        self.loaded = 0
+
+    def load(self, first_sample: int, count: int):
+        _check(lib().rlm_multi_load(self._h, first_sample, count), "rlm_multi_load")
+        self.loaded = count
+
+    def execute(self):
+        _check(lib().rlm_multi_execute(self._h), "rlm_multi_execute")
+
+    def fetch(self, k: int):
+        n = self.loaded * self.sizes[k]
+        e, c, m = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().rlm_multi_fetch(self._h, k, _ptr(e), _ptr(c), _ptr(m)), "rlm_multi_fetch")
+        shp = (self.loaded, self.sizes[k])
+        return e.reshape(shp), c.reshape(shp), m.reshape(shp)
+
+    def elapsed_ms(self) -> float:
+        out = np.zeros(1)
+        _check(lib().rlm_multi_elapsed_ms(self._h, _ptr(out)), "rlm_multi_elapsed_ms")
+        return float(out[0])
+
+    def close(self):
+        if self._h:
+            lib().rlm_multi_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):
