@@ -29,23 +29,45 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32
 HBM_PEAK_GBS = 8000.0
 
 
-def _cpu_baseline(n_phi, g, seed, lines, threads):
+def _blas_info():
+    """LAPACK/BLAS library numpy's zheevd runs on (vendor and version)."""
+    try:
+        import threadpoolctl
+        for lib in threadpoolctl.threadpool_info():
+            if lib.get("user_api") == "blas":
+                return f"{lib.get('internal_api')} {lib.get('version')} ({os.path.basename(lib.get('filepath', ''))})"
+    except Exception:
+        pass
+    return "unknown"
+
+
+def _cpu_baseline(n_phi, g, seed, lines, threads, pot_fraction=1):
     """Times the CPU oracle (C + LAPACK zheevd via numpy, one BLAS thread per worker,
-    workers over theta_2 like the reference's pthreads) on `lines` theta_1 lines of
-    sample 0 in a child process, and extrapolates to samples/hour."""
+    workers over theta_2 as the reference's pthreads: at most g + 1 busy, rlm/main_rlm_cpu.cpp:87-90)
+    on `lines` theta_1 lines of sample 0 in a child process, and extrapolates to
+    samples/hour. pot_fraction > 1 times V_mn on 1/pot_fraction of the impurities and
+    scales it (V_mn is linear in the impurity count)."""
     code = f"""
 import json, sys, time
 sys.path.insert(0, {ROOT!r})
 from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 from oracle import oracle as O
-n_phi, g, seed, lines, threads = {n_phi}, {g}, {seed}, {lines}, {threads}
+n_phi, g, seed, lines, threads, pf = {n_phi}, {g}, {seed}, {lines}, {threads}, {pot_fraction}
 t0 = time.perf_counter()
 sz = O.rlm_sizes(n_phi)
 cmn, cmt, cjm = O.rlm_tables(sz, g)
 x, y, s = O.rlm_disorder(sz, seed, 0)
-V = O.rlm_potential(sz, x, y, s, cmn)
-t_setup = time.perf_counter() - t0
+if pf > 1:
+    import dataclasses
+    k = sz.n_imp // pf
+    t_pre = time.perf_counter() - t0
+    tp = time.perf_counter()
+    V = O.rlm_potential(dataclasses.replace(sz, n_imp=k), x[:k], y[:k], s[:k], cmn)   # d x d, as the full one
+    t_setup = t_pre + (time.perf_counter() - tp) * sz.n_imp / k
+else:
+    V = O.rlm_potential(sz, x, y, s, cmn)
+    t_setup = time.perf_counter() - t0
 pool = ThreadPoolExecutor(threads)
 ring = np.zeros((2, g + 1, n_phi, n_phi), dtype=np.complex128)
 t1 = time.perf_counter()
@@ -67,11 +89,20 @@ print(json.dumps({{"per_sample_s": per_sample, "t_lines": t_lines, "t_setup": t_
         "value": 3600.0 / r["per_sample_s"],
         "unit": "samples/hour",
         "cores": threads,
+        "host_cores": os.cpu_count(),
+        "lapack": _blas_info(),
         "kind": "port",
-        "sample": (f"CPU oracle (C assembly + OpenBLAS zheevd, workers over theta_2 as rlm_cpu_dp) on {lines} of "
-                   f"{g + 1} theta_1 lines of sample 0 (n_phi={n_phi}, g={g}, seed {seed}); "
-                   f"{r['t_lines']:.2f} s wall for {lines * (g + 1)} solves, extrapolated to the full sample"),
+        "sample": (f"CPU oracle (C assembly + LAPACK zheevd, one BLAS thread per worker, {threads} workers over "
+                   f"theta_2 as rlm_cpu_dp) on {lines} of {g + 1} theta_1 lines of sample 0 (n_phi={n_phi}, g={g}, "
+                   f"seed {seed}); {r['t_lines']:.2f} s wall for {lines * (g + 1)} solves"
+                   + (f", V_mn timed on 1/{pot_fraction} of the impurities and scaled" if pot_fraction > 1 else "")
+                   + ", extrapolated to the full sample"),
     }
+
+
+def _tuned(n, precision):
+    from rlm_amd import tuned_kernel
+    return tuned_kernel(n, precision) if n <= 512 else "n/a (multi-workgroup solver)"
 
 
 def _load_traffic(key):
@@ -118,7 +149,8 @@ def _bench_hlm(args):
     e, c, m = plan.fetch()
     plan.close()
     ci = np.rint(c)
-    print(json.dumps({"metric": "samples/hour hlm 24x24 q=8 b=0 W=1 g=32 (FP64)", "value": 3600.0 * S / sec,
+    print(json.dumps({"metric": "samples/hour hlm 24x24 q=8 b=0 W=1 g=32 x 64 samples (FP64)",
+                      "value": 3600.0 * S / sec,
                       "unit": "samples/hour", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                       "ms_per_step": sec * 1e3, "higher_is_better": True, "dtype": "f64",
                       "data": "synthetic (reference hlm seed chain, seed 1, samples 0..63)",
@@ -153,12 +185,41 @@ def _bench_uneven(args):
                      "fp64_ms_per_step": res["fp64"][0] * 1e3, "mixed_ms_per_step": res["mixed"][0] * 1e3,
                      "mixed_chern_ints_identical_to_fp64": bool(np.array_equal(np.rint(res["mixed"][2]),
                                                                                np.rint(res["fp64"][2]))),
-                     "mixed_energy_max_rel_diff_vs_fp64": e_rel})
+                     "mixed_energy_max_rel_diff_vs_fp64": e_rel,
+                     "tuned_kernel": {"fp64": _tuned(n_phi, 0), "mixed": _tuned(n_phi, 1)},
+                     "_fp64_c": res["fp64"][2], "_mixed_c": res["mixed"][2]})
+    # the whole uneven batch as ONE hipGraph launch (rlm_multi_*: one branch per size,
+    # the branches run concurrently), against the per-size plans run one after another
+    from rlm_amd import RlmMulti
+    sizes = [64, 128, 192, 256, 384, 512]
+    one = {}
+    for prec, tag in ((HIP_DIAG_FP64, "fp64"), (HIP_DIAG_MIXED, "mixed")):
+        mp = RlmMulti(sizes, 1.0, 16.0, g, seed, samples_per_batch=S, precision=prec)
+        mp.load(0, S)
+        for _ in range(args.warmup):
+            mp.execute()
+            mp.elapsed_ms()
+        t0 = time.perf_counter()
+        gpu_ms = 0.0
+        for _ in range(args.steps):
+            mp.execute()
+            gpu_ms += mp.elapsed_ms()
+        sec = (time.perf_counter() - t0) / args.steps
+        same = all(np.array_equal(np.rint(mp.fetch(k)[1]), np.rint(res_c)) for k, res_c in
+                   enumerate(r[f"_{tag}_c"] for r in rows))
+        mp.close()
+        seq = sum(r[f"{tag}_ms_per_step"] for r in rows) / 1e3
+        one[tag] = {"ms_per_step": sec * 1e3, "gpu_ms_per_step": gpu_ms / args.steps,
+                    "samples_per_hour_all_sizes": 3600.0 * S / sec, "sequential_plans_ms": seq * 1e3,
+                    "speedup_vs_sequential_plans": seq / sec, "chern_ints_identical_to_plans": bool(same)}
+    for r in rows:
+        r.pop("_fp64_c")
+        r.pop("_mixed_c")
     print(json.dumps({"metric": "samples/hour rlm uneven n_phi=64..512 g=16 (per size)", "unit": "samples/hour",
                       "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
                       "data": "synthetic (reference seed chain, seed 11, samples 0..15 per size)",
                       "config": {"workload": "rlm n_phi in {64,128,192,256,384,512} q=1 i=16 g=16 x 16 samples",
-                                 "sizes": rows}}), flush=True)
+                                 "sizes": rows, "one_graph_launch": one}}), flush=True)
 
 
 def main():
@@ -210,7 +271,6 @@ def main():
         return
     if args.config == "c5":
         args.n_phi, args.g, args.samples, args.seed = 3000, 4, 4, 3
-        args.no_cpu_baseline = True
 
     n_phi, g, S, seed = args.n_phi, args.g, args.samples, args.seed
     g1 = g + 1
@@ -228,16 +288,17 @@ def main():
              "resident": "zhetrd_resident_kernel"}
 
     def trd_name(path):
-        # "auto": register-resident kernel where n fits it (complex float n <= 256,
-        # complex double n <= 128), else the lower-storage kernel; n > 512: the per-column
-        # Hermitian product of the multi-workgroup solver
+        # the kernel the library's autotuner chose for this size and precision (tuned when
+        # the plan was created); n > 512: the per-column Hermitian product of the
+        # multi-workgroup solver
         if n_phi > 512:
             return "lg_symv_lower_kernel"
-        if variant in names:
-            if variant == "resident" and n_phi > (256 if path == "mixed" else 128):
-                return names["lower"]
-            return names[variant]
-        return names["resident"] if n_phi <= (256 if path == "mixed" else 128) else names["lower"]
+        from rlm_amd import tuned_kernel
+        v = tuned_kernel(n_phi, HIP_DIAG_MIXED if path == "mixed" else HIP_DIAG_FP64)
+        return {"resident": "zhetrd_resident_kernel", "lower": "zhetrd_lower_kernel",
+                "lower8": "zhetrd_lower_kernel", "hybrid": "zhetrd_lower_kernel+zhetrd_resident_kernel",
+                "fused": "zhetrd_kernel", "blocked": "zhetrd_blocked_kernel", "small": "zhetrd_small_kernel"}.get(
+                    v, names.get(variant, "zhetrd_resident_kernel"))
 
     flops = {"tridiagonalisation": 16.0 / 3.0 * N ** 3, "backtransform": 8.0 * N ** 3}
 
@@ -307,7 +368,9 @@ def main():
     traffic = _load_traffic(f"{kname}<{'float' if path == 'mixed' else 'double'}>")
 
     line = {
-        "metric": "samples/hour rlm n_phi=256 g=32: mixed-prec vs FP64 GPU, all Chern ints exact",
+        "metric": (f"samples/hour rlm n_phi={n_phi} g={g}: mixed-prec vs FP64 GPU, all Chern ints exact"
+                   if (n_phi, g) != (256, 32) else
+                   "samples/hour rlm n_phi=256 g=32: mixed-prec vs FP64 GPU, all Chern ints exact"),
         "value": main["value"],
         "unit": "samples/hour",
         "n_gpus": 1,
@@ -336,6 +399,7 @@ def main():
             "chern_integral_and_sum1": {"fp64": chern_ok(fp64), "mixed": chern_ok(mixed)},
             "chern_max_dev_from_integer": {"fp64": chern_dev(fp64), "mixed": chern_dev(mixed)},
             "tridiagonalisation_kernel": {"fp64": trd_name("fp64"), "mixed": trd_name("mixed")},
+            "autotuned": {"fp64": _tuned(n_phi, 0), "mixed": _tuned(n_phi, 1)},
             "min_plaquette_margin_rad": {"fp64": float(fp64["m"].min()), "mixed": float(mixed["m"].min())},
         },
         "roofline": {
@@ -352,9 +416,12 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        # the reference's threads work over theta_2 within a line: at most g + 1 busy
+        threads = min(os.cpu_count() or 1, g + 1)
+        c5 = n_phi > 512
         try:
-            line["cpu_baseline"] = _cpu_baseline(n_phi, g, seed, args.cpu_lines, threads)
+            line["cpu_baseline"] = _cpu_baseline(n_phi, g, seed, 1 if c5 else args.cpu_lines, threads,
+                                                 pot_fraction=16 if c5 else 1)
         except Exception as ex:                         # reported, never silently replaced
             line["cpu_baseline"] = {"error": str(ex)[:500]}
     print(json.dumps(line), flush=True)
