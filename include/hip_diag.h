@@ -30,6 +30,19 @@ extern "C" {
 #define HIP_DIAG_FP64 0
 #define HIP_DIAG_MIXED 1
 
+/* Flags of rlm_plan_create_ex.
+ * HIP_DIAG_CONV_GPU_FP32: the conventions of the reference's single-precision GPU
+ * builds (rlm_gpu_magma / rlm_gpu_cusolver, which write energy_Chern_No.dat):
+ * float L1, L2 and impurity count (rlm/main_rlm_gpu.cu:29-33), float tables
+ * (matrix_coefficients.cpp with dtype float), float impurity positions
+ * (vsRngUniform: L * float(word) * 2^-32 in float), V_mn = coeff_mn sum_i s_i
+ * e^{+i phase} (rlm/potential_coeff.cu:14-15) and h_jk = sum_m coeff_jm[j][m]
+ * c2[m] sum_n V[m][n] conj(c1[n]) (rlm/hamiltonian_rlm_gpu.cu:23,36), every twist
+ * point solved. The arithmetic after those float inputs is this library's (FP64,
+ * or the mixed solver): results agree with the FP32 builds to their FP32
+ * accuracy and with an FP64 evaluation of their formulas to 1e-10. */
+#define HIP_DIAG_CONV_GPU_FP32 1
+
 /* ------------------------------------------------------------------------ */
 /* diag_wrappers-level entry points                                          */
 /* ------------------------------------------------------------------------ */
@@ -159,6 +172,10 @@ int hip_set_hamil_matrix(const double *d_coeff_m_theta_1, const double *d_coeff_
 typedef struct rlm_plan_s *rlm_plan_t;
 int rlm_plan_create(rlm_plan_t *plan, long n_phi, double quanta_concentration, double impurity_concentration,
                     long n_mesh, long off_override, unsigned long seed, int samples_per_batch, int precision);
+/* rlm_plan_create with flags (0 or HIP_DIAG_CONV_GPU_FP32). */
+int rlm_plan_create_ex(rlm_plan_t *plan, long n_phi, double quanta_concentration, double impurity_concentration,
+                       long n_mesh, long off_override, unsigned long seed, int samples_per_batch, int precision,
+                       int flags);
 /* Selects samples first..first+count-1 (count <= samples_per_batch) for the
  * next execute. Their disorder is generated on the device by rlm_plan_execute
  * from the seed (the jump to the first sample is computed here on the host,

@@ -50,6 +50,9 @@ def lib():
         L.orc_mt_state_size.restype = ctypes.c_size_t
         L.orc_expf.argtypes = [ctypes.c_float]
         L.orc_expf.restype = ctypes.c_float
+        for f in (L.orc_cosf, L.orc_sinf):
+            f.argtypes = [ctypes.c_float]
+            f.restype = ctypes.c_float
         _lib = L
     return _lib
 
@@ -99,6 +102,79 @@ def rlm_tables(sz: RlmSizes, n_mesh: int):
     lib().orc_prepare_coeff(sz.n_phi, sz.off, d, n_mesh, sz.L1, sz.L2,
                             _dp(cmn.view(np.float64)), _dp(cmt.view(np.float64)), _dp(cjm.view(np.float64)))
     return cmn, cmt, cjm
+
+
+# --------------------------------------------------------------------------
+# The reference's single-precision GPU builds (rlm_gpu_*: dtype float, writing
+# energy_Chern_No.dat). Their conventions, restated; arithmetic after the float
+# inputs in FP64.
+# --------------------------------------------------------------------------
+def rlm_sizes_f32(n_phi: int, q: float = 1.0, imp: float = 16.0) -> RlmSizes:
+    """rlm/main_rlm_gpu.cu:29-34: float L1 = L2 = sqrt(n_phi*q), float impurity count."""
+    f = np.float32
+    L = float(np.sqrt(f(n_phi) * f(q)))
+    n_imp = int(f(imp) * f(n_phi) / f(q))
+    off = int(math.sqrt(float(n_imp)) / 2 + 1)
+    return RlmSizes(n_phi, L, L, n_imp, off, 2 * off + 1)
+
+
+def rlm_tables_f32(sz: RlmSizes, n_mesh: int):
+    """rlm/matrix_coefficients.cpp:3-25 compiled with dtype float -> complex tables
+    (float values as complex128)."""
+    f = np.float32
+    L = lib()
+    N, o, d = sz.n_phi, sz.off, sz.d
+    L1, L2 = f(sz.L1), f(sz.L2)
+    cmn = np.zeros((d, d), dtype=np.complex128)
+    for m in range(d):
+        for n in range(d):
+            ph = math.pi * (m - o) * (n - o) / N
+            q1 = f((m - o) * (m - o)) * L2 / L1
+            q2 = f((n - o) * (n - o)) * L1 / L2
+            arg = -math.pi * float(f(q1 + q2)) / 2.0 / N
+            e = f(L.orc_expf(float(f(arg))))
+            r = f(1.0) / f(N)
+            cmn[m, n] = complex(float((f(math.cos(ph)) * e) * r), float((f(-math.sin(ph)) * e) * r))
+    cmt = np.zeros((n_mesh + 1, d), dtype=np.complex128)
+    for t in range(n_mesh + 1):
+        dth = f(t * (math.pi * 2.0) / n_mesh)
+        for m in range(d):
+            a = f(f(m - o) * dth) / f(N)
+            cmt[t, m] = complex(L.orc_cosf(float(a)), L.orc_sinf(float(a)))
+    cjm = np.zeros((N, d), dtype=np.complex128)
+    for j in range(N):
+        for m in range(d):
+            a = (math.pi * 2.0) * (m - o) * (j + 1) / N
+            cjm[j, m] = complex(float(f(math.cos(a))), float(f(-math.sin(a))))
+    return cmn, cmt, cjm
+
+
+def rlm_disorder_f32(sz: RlmSizes, seed: int, sample: int):
+    """rlm/disorder_potential.cpp:3-21 with dtype float: vsRngUniform gives
+    L * float(word) * 2^-32 in float arithmetic (pinned against MKL)."""
+    f = np.float32
+    w = mt_words(seed, 3 * sz.n_imp * sample, 3 * sz.n_imp)
+    n = sz.n_imp
+    x = (f(sz.L1) * (w[:n].astype(np.float32) * f(2.0 ** -32))).astype(np.float64)
+    y = (f(sz.L2) * (w[n:2 * n].astype(np.float32) * f(2.0 ** -32))).astype(np.float64)
+    s = np.where(w[2 * n:] % 2 == 0, 1.0, -1.0)
+    return x, y, s
+
+
+def rlm_sample_gpu_conv(n_phi: int, q: float, imp: float, n_mesh: int, seed: int, sample: int,
+                        threads: int = 1) -> SampleResult:
+    """One sample of the FP32 GPU builds' pipeline (rlm/main_rlm_gpu.cu:138-300) with FP64
+    arithmetic after the float inputs: V_mn = coeff_mn sum_i s_i e^{+i phase}
+    (rlm/potential_coeff.cu:14-15) and h_jk = sum_m coeff_jm[j][m] c2[m] sum_n V[m][n] conj(c1[n])
+    (rlm/hamiltonian_rlm_gpu.cu:23,36): the CPU formulas of rlm_potential / rlm_assemble
+    evaluated at negated positions and with conj(coeff_m_theta). Energies, Chern sums and
+    the +1/N offset as the CPU build (main_rlm_gpu.cu:230-297 has the same loops)."""
+    sz = rlm_sizes_f32(n_phi, q, imp)
+    cmn, cmt, cjm = rlm_tables_f32(sz, n_mesh)
+    x, y, s = rlm_disorder_f32(sz, seed, sample)
+    V = rlm_potential(sz, -x, -y, s, cmn)
+    return rlm_sample(n_phi, q, imp, n_mesh, seed, sample, threads=threads,
+                      tables=(cmn, np.conj(cmt), cjm), sizes=sz, potential=V)
 
 
 def rlm_disorder(sz: RlmSizes, seed: int, sample: int):
@@ -179,12 +255,16 @@ def _solve_line_rlm(sz, V, cmt, cjm, theta_1, n_mesh, pool):
 
 
 def rlm_sample(n_phi: int, q: float, imp: float, n_mesh: int, seed: int, sample: int,
-               threads: int = 1, off_override: int | None = None, tables=None) -> SampleResult:
+               threads: int = 1, off_override: int | None = None, tables=None, sizes=None,
+               potential=None) -> SampleResult:
     """rlm/main_rlm_cpu.cpp:109-243 for one sample (same accumulation order)."""
-    sz = rlm_sizes(n_phi, q, imp, off_override)
+    sz = sizes if sizes is not None else rlm_sizes(n_phi, q, imp, off_override)
     cmn, cmt, cjm = tables if tables is not None else rlm_tables(sz, n_mesh)
-    x, y, s = rlm_disorder(sz, seed, sample)
-    V = rlm_potential(sz, x, y, s, cmn)
+    if potential is None:
+        x, y, s = rlm_disorder(sz, seed, sample)
+        V = rlm_potential(sz, x, y, s, cmn)
+    else:
+        V = potential
     pool = ThreadPoolExecutor(threads) if threads > 1 else None
     try:
         N = n_phi

@@ -258,3 +258,7 @@ void orc_chern_line(const double *wfs, int n_phi, int dim_vec, int n_mesh, int t
 
 /* float expf as the DP build calls it, exposed so tests can compare tables. */
 float orc_expf(float x) { return expf(x); }
+/* libm's single-precision cosf/sinf: coeff_m_theta of the reference's FP32 builds takes
+ * cos/sin of a float argument (rlm/matrix_coefficients.cpp:17 with dtype float). */
+float orc_cosf(float x) { return cosf(x); }
+float orc_sinf(float x) { return sinf(x); }

@@ -712,6 +712,7 @@ static int first_nonfinite_sample(const double *h, long SN, long n_phi) {
 struct rlm_plan_s {
     long n_phi, n_imp, off, d, g, spb;
     double L1, L2;
+    bool gpu_conv = false;   // HIP_DIAG_CONV_GPU_FP32: the single-precision GPU builds' conventions
     uint32_t seed;
     int precision;
     hipStream_t stream = nullptr;
@@ -747,13 +748,27 @@ struct rlm_plan_s {
 
 extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double imp, long n_mesh, long off_override,
                                unsigned long seed, int samples_per_batch, int precision) {
+    return rlm_plan_create_ex(plan, n_phi, q, imp, n_mesh, off_override, seed, samples_per_batch, precision, 0);
+}
+
+extern "C" int rlm_plan_create_ex(rlm_plan_t *plan, long n_phi, double q, double imp, long n_mesh, long off_override,
+                                  unsigned long seed, int samples_per_batch, int precision, int flags) {
     if (!plan || n_phi < 1 || n_phi > heevd_large_max_n() || n_mesh < 1 || samples_per_batch < 1 || q <= 0.0)
         return -22;
     if (precision != HIP_DIAG_FP64 && precision != HIP_DIAG_MIXED) return -22;
+    if (flags & ~HIP_DIAG_CONV_GPU_FP32) return -22;
     auto *p = new rlm_plan_s();
     p->n_phi = n_phi;
-    p->L1 = p->L2 = std::sqrt((double)n_phi * q);                     // rlm/main_rlm_cpu.cpp:29-30
-    p->n_imp = (long)(imp * (double)n_phi / q);                       // :31
+    p->gpu_conv = (flags & HIP_DIAG_CONV_GPU_FP32) != 0;
+    if (p->gpu_conv) {
+        // the single-precision GPU builds' sizes (rlm/main_rlm_gpu.cu:29-33: float L1, L2 and
+        // a float impurity count)
+        p->L1 = p->L2 = (double)std::sqrt((float)n_phi * (float)q);
+        p->n_imp = (long)((float)imp * (float)n_phi / (float)q);
+    } else {
+        p->L1 = p->L2 = std::sqrt((double)n_phi * q);                 // rlm/main_rlm_cpu.cpp:29-30
+        p->n_imp = (long)(imp * (double)n_phi / q);                   // :31
+    }
     p->off = off_override >= 0 ? off_override : (long)(std::sqrt((double)p->n_imp) / 2 + 1);  // :32
     p->d = 2 * p->off + 1;                                            // :33
     p->g = n_mesh;
@@ -766,8 +781,10 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
     }
     const long N = n_phi, d = p->d, g1 = n_mesh + 1, S = samples_per_batch;
     {
+        // the GPU builds' conventions solve every point (the derived-edge relations hold
+        // for the CPU build's phase conventions)
         const char *mesh = getenv("RLM_MESH");
-        p->pl = (mesh && !strcmp(mesh, "full")) ? (int)g1 : (int)n_mesh;
+        p->pl = ((mesh && !strcmp(mesh, "full")) || p->gpu_conv) ? (int)g1 : (int)n_mesh;
     }
     const long nkp = S * p->pl * p->pl;   // solved twist points
     const long nlk = S * g1 * g1;         // twist points with links
@@ -788,8 +805,17 @@ extern "C" int rlm_plan_create(rlm_plan_t *plan, long n_phi, double q, double im
     p->cmn.resize(2 * d * d);
     p->cmt.resize(2 * g1 * d);
     p->cjm.resize(2 * N * d);
-    host_prepare_coeff(p->cmn.data(), p->cmt.data(), p->cjm.data(), (int)N, (int)p->off, (int)d, (int)n_mesh, p->L1,
-                       p->L2);
+    if (p->gpu_conv) {
+        host_prepare_coeff_f32(p->cmn.data(), p->cmt.data(), p->cjm.data(), (int)N, (int)p->off, (int)d, (int)n_mesh,
+                               (float)p->L1, (float)p->L2);
+        // h_jk = sum_m coeff_jm[j][m] c2[m] sum_n V[m][n] conj(c1[n]) in the GPU builds
+        // (rlm/hamiltonian_rlm_gpu.cu:23,36) against conj(c2[m]) coeff_jm[j][m] and c1[n] in
+        // the CPU build: the tables take conj(coeff_m_theta)
+        for (size_t k = 1; k < p->cmt.size(); k += 2) p->cmt[k] = -p->cmt[k];
+    } else {
+        host_prepare_coeff(p->cmn.data(), p->cmt.data(), p->cjm.data(), (int)N, (int)p->off, (int)d, (int)n_mesh,
+                           p->L1, p->L2);
+    }
     int rc = 0;
     auto fail = [&](int r) {
         rlm_plan_destroy(p);
@@ -856,8 +882,8 @@ static int rlm_launches(rlm_plan_t p, bool ev) {
     roctxMarkA("rlm: disorder streams + V_mn + band tables");
     if (ev) RLM_HIP_CHECK(hipEventRecord(p->ev[0], s));
     // disorder of the loaded samples from the seed (rlm/disorder_potential.cpp:3-21)
-    if ((rc = p->jumps.launch(p->seed, S, 0, 3 * p->n_imp, p->n_imp, p->L1, p->L2, p->dX.d(), p->dY.d(), p->dS.d(),
-                              s)))
+    if ((rc = p->jumps.launch(p->seed, S, p->gpu_conv ? 2 : 0, 3 * p->n_imp, p->n_imp, p->L1, p->L2, p->dX.d(),
+                              p->dY.d(), p->dS.d(), s)))
         return rc;
     if ((rc = launch_potential(p->dX.d(), p->dY.d(), p->dS.d(), p->n_imp, p->dcmn.d(), d, off, p->dV.d(), S, s)))
         return rc;

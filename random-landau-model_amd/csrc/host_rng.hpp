@@ -71,4 +71,44 @@ inline void host_prepare_coeff(double *coeff_mn, double *coeff_m_theta, double *
         }
 }
 
+// rlm/matrix_coefficients.cpp:3-25 as the reference's single-precision GPU builds
+// compile it (dtype float: complex<float> tables; the double cos/sin of coeff_mn and
+// coeff_jm rounded to float; coeff_m_theta from float cos/sin of a float argument;
+// float L1, L2). Values written as doubles (exactly the floats).
+inline void host_prepare_coeff_f32(double *coeff_mn, double *coeff_m_theta, double *coeff_jm, int n_phi, int off,
+                                   int d, int n_mesh, float L1, float L2) {
+    const double PI2 = M_PI * 2.0;
+    for (int m = 0; m < d; m++)
+        for (int n = 0; n < d; n++) {
+            const double ph = M_PI * (m - off) * (n - off) / n_phi;
+            double sn, cs;
+            ::sincos(ph, &sn, &cs);
+            // (m-o)^2 L2 / L1 + (n-o)^2 L1 / L2 in float, times -pi, / 2.0 / n_phi in double
+            const float q1 = (float)((m - off) * (m - off)) * L2 / L1;
+            const float q2 = (float)((n - off) * (n - off)) * L1 / L2;
+            const double arg = -M_PI * (double)(q1 + q2) / 2.0 / n_phi;
+            const float e = ::expf((float)arg);
+            const float r = 1.0f / (float)n_phi;
+            const float cr = (float)cs, ci = (float)(-sn);
+            coeff_mn[2 * (m * d + n)] = (double)((cr * e) * r);
+            coeff_mn[2 * (m * d + n) + 1] = (double)((ci * e) * r);
+        }
+    for (int t = 0; t <= n_mesh; t++) {
+        const float dtheta = (float)(t * PI2 / n_mesh);
+        for (int m = 0; m < d; m++) {
+            const float a = (float)(m - off) * dtheta / (float)n_phi;
+            coeff_m_theta[2 * (t * d + m)] = (double)::cosf(a);
+            coeff_m_theta[2 * (t * d + m) + 1] = (double)::sinf(a);
+        }
+    }
+    for (int j = 0; j < n_phi; j++)
+        for (int m = 0; m < d; m++) {
+            const double a = PI2 * (m - off) * (j + 1) / n_phi;
+            double sn, cs;
+            ::sincos(a, &sn, &cs);
+            coeff_jm[2 * (j * d + m)] = (double)(float)cs;
+            coeff_jm[2 * (j * d + m) + 1] = (double)(float)(-sn);
+        }
+}
+
 }  // namespace rlm

@@ -88,6 +88,10 @@ __device__ void apply_jump(uint32_t *seq, int kind, int mono, const uint64_t *g)
 //     even word, -1 for an odd one (rlm/disorder_potential.cpp:9-17);
 //   mode 1 (hlm): out0[w] = -1 + 2 u, times scale0 unless scale0 == 1
 //     (hlm/main_hlm_cpu.cpp:79, width W: scale0 = W / 2);
+//   mode 2 (rlm, the single-precision GPU builds' stream): as mode 0 with x and y the
+//     float values L * float(word) * 2^-32 (vsRngUniform), written negated: the GPU
+//     builds' V_mn sums e^{+i phase} (rlm/potential_coeff.cu:14-15), which the FP64
+//     potential kernel's e^{-i phase} gives for negated positions (exactly);
 // with u = word * 2^-32 (vdRngUniform's a + (b - a) * word * 2^-32). Outputs of stream
 // b start at out + b * n_sec (mode 0) or out0 + b * words (mode 1).
 __global__ __launch_bounds__(kNT) void mt_stream_kernel(uint32_t seed, int kindA, int monoA,
@@ -120,9 +124,10 @@ __global__ __launch_bounds__(kNT) void mt_stream_kernel(uint32_t seed, int kindA
         }
         __syncthreads();
     }
-    double *o0 = mode == 0 ? out0 + b * n_sec : out0 + b * words;
-    double *o1 = mode == 0 ? out1 + b * n_sec : nullptr;
-    double *o2 = mode == 0 ? out2 + b * n_sec : nullptr;
+    const bool secs = mode != 1;
+    double *o0 = secs ? out0 + b * n_sec : out0 + b * words;
+    double *o1 = secs ? out1 + b * n_sec : nullptr;
+    double *o2 = secs ? out2 + b * n_sec : nullptr;
     // output w is the tempered x_{624 + w}, computed from x_w, x_{w+1}, x_{w+397}
     for (long c = 0; c < words; c += kChunk) {
         const long w = c + threadIdx.x;
@@ -137,6 +142,18 @@ __global__ __launch_bounds__(kNT) void mt_stream_kernel(uint32_t seed, int kindA
                     o0[w] = __dmul_rn(scale0, u);
                 } else if (w < 2 * n_sec) {
                     o1[w - n_sec] = __dmul_rn(scale1, u);
+                } else {
+                    o2[w - 2 * n_sec] = (t & 1u) ? -1.0 : 1.0;
+                }
+            } else if (mode == 2) {
+                // single-precision stream of the reference's FP32 builds (vsRngUniform:
+                // L * float(word) * 2^-32 in float arithmetic), positions negated (see
+                // launch_mt_streams)
+                const float uf = (float)t * 0x1p-32f;
+                if (w < n_sec) {
+                    o0[w] = -(double)__fmul_rn((float)scale0, uf);
+                } else if (w < 2 * n_sec) {
+                    o1[w - n_sec] = -(double)__fmul_rn((float)scale1, uf);
                 } else {
                     o2[w - 2 * n_sec] = (t & 1u) ? -1.0 : 1.0;
                 }
@@ -181,7 +198,8 @@ size_t mt_stream_lds_bytes() { return sizeof(uint32_t) * (size_t)(kSeq > kMtN + 
 int launch_mt_streams(uint32_t seed, int kindA, int monoA, const uint64_t *d_polyA, const int *d_kindB,
                       const int *d_monoB, const uint64_t *d_polyB, int n_streams, long words, int mode, long n_sec,
                       double scale0, double scale1, double *out0, double *out1, double *out2, hipStream_t stream) {
-    if (n_streams < 1 || words < 0 || (mode == 0 && 3 * n_sec != words) || (mode != 0 && mode != 1)) return -22;
+    if (n_streams < 1 || words < 0 || ((mode == 0 || mode == 2) && 3 * n_sec != words) || mode < 0 || mode > 2)
+        return -22;
     if (kindA < 0 || kindA > 2 || (kindA == 1 && (monoA < 1 || monoA >= mtj::kDeg)) || (kindA == 2 && !d_polyA))
         return -22;
     const size_t lds = mt_stream_lds_bytes();

@@ -31,17 +31,19 @@ static void usage(const char *target) {
     cout << "  -t  # of threads\n";
     cout << "  -p  precision: 0 = FP64 (default), 1 = mixed (FP32 reduction + FP64 refinement)\n";
     cout << "  -f  first sample (resume a run: samples f..f+c-1, rows appended)\n";
+    cout << "  -G  conventions of the single-precision GPU builds (float inputs, energy_Chern_No.dat)\n";
 }
 
 int main(int argc, char *argv[]) {
     // defaults of rlm/init_rlm.cpp:20-28
     long n_phi = 32, n_mesh = 32, n_sample = 1, num_threads = 1, off_override = -1, first = 0;
-    int precision = HIP_DIAG_FP64;
+    int precision = HIP_DIAG_FP64, flags = 0;
+    bool prec_set = false;
     double q = 1, imp = 16;
     unsigned long seed = std::chrono::high_resolution_clock::now().time_since_epoch().count();
     int ch, err_flag = 0;
     // the reference's optstring is "n:q:i:g:c:s:h:t:"; -d (documented there) is accepted here
-    while ((ch = getopt(argc, argv, "n:q:i:g:d:c:s:h:t:p:f:")) != -1) {
+    while ((ch = getopt(argc, argv, "n:q:i:g:d:c:s:h:t:p:f:G")) != -1) {
         switch (ch) {
             case 'n': n_phi = atoi(optarg); break;
             case 'q': q = atof(optarg); break;
@@ -51,7 +53,11 @@ int main(int argc, char *argv[]) {
             case 'c': n_sample = atoi(optarg); break;
             case 's': seed = atoi(optarg); break;
             case 't': num_threads = atoi(optarg); break;
-            case 'p': precision = atoi(optarg) == 1 ? HIP_DIAG_MIXED : HIP_DIAG_FP64; break;
+            case 'p':
+                precision = atoi(optarg) == 1 ? HIP_DIAG_MIXED : HIP_DIAG_FP64;
+                prec_set = true;
+                break;
+            case 'G': flags |= HIP_DIAG_CONV_GPU_FP32; break;
             case 'f': first = atol(optarg); break;
             default: err_flag++; break;
         }
@@ -61,6 +67,8 @@ int main(int argc, char *argv[]) {
         exit(2);
     }
     (void)num_threads;  // the GPU path has no host worker threads
+    const bool gpu_build = (flags & HIP_DIAG_CONV_GPU_FP32) != 0;
+    if (gpu_build && !prec_set) precision = HIP_DIAG_MIXED;   // the GPU builds' FP32 solvers
     {
         ofstream flog("log_rlm.txt");
         flog << "########################################################" << endl;
@@ -80,7 +88,7 @@ int main(int argc, char *argv[]) {
 
     auto t1 = chrono::high_resolution_clock::now();
     rlm_plan_t plan = nullptr;
-    int rc = rlm_plan_create(&plan, n_phi, q, imp, n_mesh, off_override, seed, (int)spb, precision);
+    int rc = rlm_plan_create_ex(&plan, n_phi, q, imp, n_mesh, off_override, seed, (int)spb, precision, flags);
     if (rc) {
         cerr << "rlm_plan_create failed with status " << rc << endl;
         return 1;
@@ -89,7 +97,8 @@ int main(int argc, char *argv[]) {
     const double coeff_time = chrono::duration_cast<chrono::microseconds>(t2 - t1).count() / 1.0E6;
     cerr << std::left << setw(40) << "coefficients initialization time: " << coeff_time << " s\n";
 
-    ofstream fchern("energy_Chern_No.txt", first > 0 ? ios::app : ios::trunc);
+    // the GPU builds write energy_Chern_No.dat (rlm/main_rlm_gpu.cu:129), the CPU builds .txt
+    ofstream fchern(gpu_build ? "energy_Chern_No.dat" : "energy_Chern_No.txt", first > 0 ? ios::app : ios::trunc);
     vector<double> E(spb * n_phi), C(spb * n_phi);
     const double nkp = (double)(n_mesh + 1) * (n_mesh + 1);
     for (long s0 = 0; s0 < n_sample; s0 += spb) {
@@ -135,11 +144,17 @@ int main(int argc, char *argv[]) {
             cerr << setw(40) << "Chern No. calculation" << chern * n_mesh * (n_mesh + 1) * n_sample / total * 100.0
                  << endl;
         }
-        // rlm/main_rlm_cpu.cpp:241-242
+        // rlm/main_rlm_cpu.cpp:241-242; GPU builds: right-aligned, float values
+        // (rlm/main_rlm_gpu.cu:296-297)
         for (int s = 0; s < cnt; s++)
-            for (long i = 0; i < n_phi; i++)
-                fchern << std::left << setw(8) << first + s0 + s << " " << setw(15) << setprecision(8)
-                       << E[s * n_phi + i] << " " << C[s * n_phi + i] << endl;
+            for (long i = 0; i < n_phi; i++) {
+                if (gpu_build)
+                    fchern << std::right << setw(8) << first + s0 + s << " " << setw(15) << setprecision(8)
+                           << (float)E[s * n_phi + i] << " " << (float)C[s * n_phi + i] << endl;
+                else
+                    fchern << std::left << setw(8) << first + s0 + s << " " << setw(15) << setprecision(8)
+                           << E[s * n_phi + i] << " " << C[s * n_phi + i] << endl;
+            }
     }
     rlm_plan_destroy(plan);
     if (rc) {

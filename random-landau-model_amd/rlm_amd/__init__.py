@@ -19,6 +19,7 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "hip_diag.h")
 
 HIP_DIAG_FP64 = 0
 HIP_DIAG_MIXED = 1
+HIP_DIAG_CONV_GPU_FP32 = 1
 
 _lib = None
 
@@ -66,6 +67,7 @@ def lib() -> ctypes.CDLL:
         "hip_prepare_potential_coeff": ([dp, dp, dp, dp, dp, i, i, i, l], i),
         "hip_set_hamil_matrix": ([dp, dp, dp, dp, dp, i, i, i], i),
         "rlm_plan_create": ([ctypes.POINTER(vp), l, d, d, l, l, ctypes.c_ulong, i, i], i),
+        "rlm_plan_create_ex": ([ctypes.POINTER(vp), l, d, d, l, l, ctypes.c_ulong, i, i, i], i),
         "rlm_plan_load": ([vp, l, i], i),
         "rlm_plan_execute": ([vp], i),
         "rlm_plan_fetch": ([vp, dp, dp, dp], i),
@@ -191,11 +193,11 @@ class RlmPlan:
     """rlm_plan_* (batched rlm pipeline)."""
 
     def __init__(self, n_phi, q=1.0, imp=16.0, n_mesh=32, seed=1, samples_per_batch=1, off_override=-1,
-                 precision=HIP_DIAG_FP64):
+                 precision=HIP_DIAG_FP64, flags=0):
         L = lib()
         self._h = ctypes.c_void_p()
-        _check(L.rlm_plan_create(ctypes.byref(self._h), n_phi, q, imp, n_mesh, off_override, seed,
-                                 samples_per_batch, precision), "rlm_plan_create")
+        _check(L.rlm_plan_create_ex(ctypes.byref(self._h), n_phi, q, imp, n_mesh, off_override, seed,
+                                    samples_per_batch, precision, flags), "rlm_plan_create_ex")
         info = (ctypes.c_long * 6)()
         _check(L.rlm_plan_info(self._h, info), "rlm_plan_info")
         self.n_phi, self.n_imp, self.off, self.d, self.n_mesh, self.spb = list(info)

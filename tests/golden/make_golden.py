@@ -2,8 +2,9 @@
 
 1. ``mkl_rng.npz`` -- known-answer vectors of Intel MKL's VSL MT19937 stream,
    produced by calling MKL's own exported functions (vslNewStream,
-   vslSkipAheadStream, vdRngUniform, viRngUniformBits) through ctypes, in the
-   exact call sequences of rlm/disorder_potential.cpp:5-16 and
+   vslSkipAheadStream, vdRngUniform, vsRngUniform, viRngUniformBits) through
+   ctypes, in the exact call sequences of rlm/disorder_potential.cpp:5-16 (double
+   builds, and float builds for the single-precision GPU executables) and
    hlm/main_hlm_cpu.cpp:67,79. MKL is the third-party library the reference
    links (MKLROOT, Makefile:13,21); the runtime found here is
    /opt/conda/lib/libmkl_rt.so.1 (version printed into the file).
@@ -38,6 +39,8 @@ def _mkl():
     mkl.vslSkipAheadStream.argtypes = [vp, ctypes.c_longlong]
     mkl.vdRngUniform.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                  ctypes.c_double, ctypes.c_double]
+    mkl.vsRngUniform.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                 ctypes.c_float, ctypes.c_float]
     mkl.viRngUniformBits.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint)]
     mkl.vslDeleteStream.argtypes = [ctypes.POINTER(vp)]
     return mkl
@@ -67,6 +70,20 @@ def rlm_stream(mkl, seed, skip, n, L):
     assert mkl.viRngUniformBits(METHOD_STD, st, n, b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))) == 0
     mkl.vslDeleteStream(ctypes.byref(st))
     return x, y, b
+
+
+def rlm_stream_f32(mkl, seed, skip, n, L):
+    """rlm/disorder_potential.cpp:5-16 in the FP32 builds (dtype float: vsRngUniform)."""
+    st = ctypes.c_void_p()
+    assert mkl.vslNewStream(ctypes.byref(st), VSL_BRNG_MT19937, seed) == 0
+    assert mkl.vslSkipAheadStream(st, skip) == 0
+    x = np.empty(n, dtype=np.float32)
+    y = np.empty(n, dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    assert mkl.vsRngUniform(METHOD_STD, st, n, x.ctypes.data_as(fp), 0.0, float(np.float32(L))) == 0
+    assert mkl.vsRngUniform(METHOD_STD, st, n, y.ctypes.data_as(fp), 0.0, float(np.float32(L))) == 0
+    mkl.vslDeleteStream(ctypes.byref(st))
+    return x, y
 
 
 def hlm_stream(mkl, seed, n_sites, n_samples):
@@ -101,7 +118,16 @@ def write_mkl():
         cases.append((key, seed, n_imp, L, sample, n))
     for seed, n_sites, n_samples in [(1, 576, 3), (5, 9, 4)]:
         arrays[f"hlm_s{seed}_n{n_sites}"] = hlm_stream(mkl, seed, n_sites, n_samples)
-    meta = {"mkl_version": _mkl_version(mkl), "rlm_cases": cases}
+    # FP32 builds: float L = sqrtf(n_phi * q), vsRngUniform (first 2048 of each draw kept)
+    f32_cases = []
+    for seed, n_phi, n_imp, sample in [(1, 16, 256, 1), (7, 256, 4096, 3), (3, 3000, 48000, 1), (11, 300, 4800, 2)]:
+        L = float(np.sqrt(np.float32(n_phi)))
+        x, y = rlm_stream_f32(mkl, seed, 3 * n_imp * sample, n_imp, L)
+        key = f"rlm32_s{seed}_n{n_imp}_k{sample}"
+        arrays[key + "_x"] = x[:2048]
+        arrays[key + "_y"] = y[:2048]
+        f32_cases.append((key, seed, n_phi, n_imp, sample, min(n_imp, 2048)))
+    meta = {"mkl_version": _mkl_version(mkl), "rlm_cases": cases, "rlm_f32_cases": f32_cases}
     arrays["meta_json"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez_compressed(os.path.join(HERE, "mkl_rng.npz"), **arrays)
     print("wrote mkl_rng.npz:", meta["mkl_version"])

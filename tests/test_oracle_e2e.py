@@ -9,7 +9,10 @@ import pytest
 @pytest.mark.parametrize("n_phi,g,seed,sample", [(16, 8, 1, 0), (16, 10, 2, 3), (24, 6, 9, 1), (32, 4, 5, 0)])
 def test_rlm_sample_invariants(oracle, n_phi, g, seed, sample):
     r = oracle.rlm_sample(n_phi, 1.0, 16.0, g, seed, sample)
-    assert np.abs(r.chern - np.rint(r.chern)).max() < 1e-9
+    # the float theta tables make the theta = 2 pi twist rows differ from the theta = 0
+    # ones at the float rounding level, so the plaquette sums telescope to integers up
+    # to ~1e-7 only (as in the FP32 builds)
+    assert np.abs(r.chern - np.rint(r.chern)).max() < 1e-6
     assert np.rint(r.chern).sum() == 1
     sz = oracle.rlm_sizes(n_phi)
     _, _, s = oracle.rlm_disorder(sz, seed, sample)
@@ -42,3 +45,37 @@ def test_hlm_sample_invariants(oracle, lx, ly, Q, g, seed):
         r = oracle.hlm_sample(lx, ly, Q, 0, g, pots[s])
         assert np.abs(r.chern - np.rint(r.chern)).max() < 1e-9
         assert np.rint(r.chern).sum() == 1
+
+
+def test_fp32_build_conventions_invariants(oracle):
+    """The FP32 GPU builds' conventions (e^{+i phase} in V_mn, conj(c1) and c2 in the
+    assembly): the Hamiltonian stays Hermitian with the CPU build's trace
+    N * fl32(1/N) * sum(s) (V[o][o] is real) but is a different matrix than the CPU
+    build's at the same twist point; the mesh-averaged energies and per-state Chern
+    integers equal those of the CPU formulas on the same float inputs (the sign flips
+    amount to reflecting the disorder and the twists, which the averages over the
+    symmetric mesh do not see; the float tables' rounding leaves ~1e-7), and each
+    sample's Chern integers sum to 1. (Since the averaged outputs cannot tell the sign
+    conventions apart, the restatement of rlm/potential_coeff.cu:14-15 and
+    rlm/hamiltonian_rlm_gpu.cu:23,36 is parity unpinned beyond these invariants: the
+    FP32 GPU builds cannot run here.)"""
+    n, g = 16, 8
+    sz = oracle.rlm_sizes_f32(n)
+    cmn, cmt, cjm = oracle.rlm_tables_f32(sz, g)
+    x, y, s = oracle.rlm_disorder_f32(sz, 1, 0)
+    V = oracle.rlm_potential(sz, -x, -y, s, cmn)
+    H = oracle.rlm_hamiltonian(sz, V, np.conj(cmt), cjm, 2, 3)
+    assert np.abs(H - H.conj().T).max() < 1e-13
+    assert abs(np.trace(H).real - n * float(np.float32(1.0) / np.float32(n)) * s.sum()) < 1e-10
+    Hc = oracle.rlm_hamiltonian(sz, oracle.rlm_potential(sz, x, y, s, cmn), cmt, cjm, 2, 3)
+    assert np.abs(H - Hc).max() > 1e-2
+    r = oracle.rlm_sample_gpu_conv(n, 1.0, 16.0, g, 1, 0)
+    rc = oracle.rlm_sample(n, 1.0, 16.0, g, 1, 0, tables=(cmn, cmt, cjm), sizes=sz,
+                           potential=oracle.rlm_potential(sz, x, y, s, cmn))
+    assert np.abs(r.energy - rc.energy).max() < 1e-6 * np.abs(rc.energy).max()
+    assert np.array_equal(np.rint(r.chern), np.rint(rc.chern))
+    # the float theta tables make the theta = 2 pi twist rows differ from the theta = 0
+    # ones at the float rounding level, so the plaquette sums telescope to integers up
+    # to ~1e-7 only (as in the FP32 builds)
+    assert np.abs(r.chern - np.rint(r.chern)).max() < 1e-6
+    assert np.rint(r.chern).sum() == 1

@@ -60,3 +60,18 @@ def test_uniform_mapping_edges(oracle):
     x, _, _ = oracle.rlm_disorder(sz, 1, 0)
     assert np.array_equal(x, 4.0 * (w * 2.0**-32))
     assert x.min() >= 0.0 and x.max() < 4.0
+
+
+def test_fp32_build_disorder_matches_mkl(oracle):
+    """The single-precision GPU builds' stream (dtype float: vsRngUniform, float L and
+    impurity count) against MKL's own float output, including a non-power-of-two L
+    (n_phi = 3000, 300): L * float(word) * 2^-32 in float arithmetic."""
+    z = np.load(GOLDEN, allow_pickle=False)
+    meta = json.loads(bytes(z["meta_json"]).decode())
+    assert meta["rlm_f32_cases"]
+    for key, seed, n_phi, n_imp, sample, n in meta["rlm_f32_cases"]:
+        sz = oracle.rlm_sizes_f32(n_phi)
+        assert sz.n_imp == n_imp
+        x, y, _ = oracle.rlm_disorder_f32(sz, seed, sample)
+        assert np.array_equal(x[:n].astype(np.float32), z[key + "_x"]), key
+        assert np.array_equal(y[:n].astype(np.float32), z[key + "_y"]), key
