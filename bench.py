@@ -381,7 +381,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64 (mixed)" if path == "mixed" else "f64",
-        "data": "synthetic (reference seed chain: MKL-compatible MT19937, seed 7, samples 0..31)",
+        "data": f"synthetic (reference seed chain: MKL-compatible MT19937, seed {seed}, samples 0..{S - 1})",
         "config": {
             "workload": f"rlm n_phi={n_phi} q=1 i=16 g={g} ({g1 * g1} twist points) x {S} samples, seed {seed}",
             "path": path,

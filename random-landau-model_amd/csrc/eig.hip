@@ -1891,9 +1891,13 @@ __device__ __forceinline__ void merge_gemm_mfma(int lane, int G, int lo, int mid
 // pointer swap) and eigenvectors of T in the returned buffer (row-major n x n).
 // QS: storage type of the eigenvector matrices Qa, Qb and of S (see merge_gemm_mfma);
 // every other quantity is FP64.
+// split_top: stop at the top merge once its secular roots, weights, norms and ranks are
+// known (nullptr returned, the scale left in sh.red[0]); its eigenvector GEMM then runs on
+// several workgroups per matrix (stedc_top_gemm_kernel) and stedc_finish_kernel writes
+// the outputs.
 template <int NT, typename QS>
 __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, QS *Qb, QS *S, DcShared &sh,
-                           double **dq_final) {
+                           double **dq_final, bool split_top = false) {
     const int t = threadIdx.x;
     // scale T to unit max-norm (as LAPACK dstedc does) so that the deflation
     // tolerance is relative to the matrix, whatever its magnitude
@@ -2196,6 +2200,11 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                 }
             }
             __syncthreads();
+            if (split_top && blk >= n) {            // the top merge (one node, uniform exit)
+                if (t == 0) sh.red[0] = scale;
+                *dq_final = dqn;
+                return nullptr;
+            }
             // Matrix-core path for halves of >= 32 rows (whole waves per node): Qb = Qa S
             // with S[i][q] = what_i / (dl_i - lambda_q) * inrm_q generated per K-step,
             // S never stored. Below that, S goes through memory and the VALU path.
@@ -2274,12 +2283,53 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
 // the tridiagonal to U (component-major complex, zero imaginary part).
 // GSH: the per-node arrays live in the workgroup's global workspace after Qa, Qb, S
 // (n > 512, where they exceed the LDS); the LDS then holds only the transpose tile.
+// Node arrays of the divide & conquer at p (13 double arrays of n, 8 doubles of
+// reduction scratch, 11 int arrays of n and 5 ints per node slot).
+__device__ inline void dc_shared_at(double *p, int n, DcShared &sh) {
+    sh.dq = p; p += n; sh.dqn = p; p += n; sh.ds = p; p += n; sh.zs = p; p += n;
+    sh.dl = p; p += n; sh.wz = p; p += n; sh.z2 = p; p += n; sh.tau = p; p += n; sh.vals = p; p += n;
+    sh.rc = p; p += n; sh.rs = p; p += n; sh.what = p; p += n; sh.inrm = p; p += n;
+    sh.red = p; p += 8;
+    int *ip = reinterpret_cast<int *>(p);
+    sh.perm = ip; ip += n; sh.orig = ip; ip += n; sh.rank = ip; ip += n; sh.rpj = ip; ip += n;
+    sh.rnj = ip; ip += n; sh.ndx = ip; ip += n; sh.dfx = ip; ip += n;
+    sh.pcol = ip; ip += n; sh.lstL = ip; ip += n; sh.lstR = ip; ip += n; sh.rflag = ip; ip += n;
+    sh.cnt = ip; ip += 5 * n;
+}
+
+// Component-major complex output U[r*n + j] = Z[j*n + r] (column-major Z) through 32x32
+// LDS tiles.
+template <typename T, int NT>
+__device__ inline void dc_transpose_out(int n, const T *Z, T *U, double *tile) {
+    using C = cx<T>;
+    const int t = threadIdx.x;
+    for (int tr = 0; tr < n; tr += 32)
+        for (int tc = 0; tc < n; tc += 32) {
+            for (int e = t; e < 1024; e += NT) {
+                const int jj = e >> 5, rr = e & 31;            // read column tc+jj, row tr+rr
+                if (tc + jj < n && tr + rr < n) tile[rr * 33 + jj] = Z[(long)(tc + jj) * n + tr + rr];
+            }
+            __syncthreads();
+            for (int e = t; e < 1024; e += NT) {
+                const int rr = e >> 5, jj = e & 31;            // write row tr+rr, column tc+jj
+                if (tc + jj < n && tr + rr < n) stx(U, (long)(tr + rr) * n + tc + jj, C{(T)tile[rr * 33 + jj], (T)0});
+            }
+            __syncthreads();
+        }
+}
+
+// Stage 1 kernel: divide & conquer per matrix; eigenvalues to W, eigenvectors of
+// the tridiagonal to U (component-major complex, zero imaginary part).
+// GSH: the per-node arrays live in the workgroup's global workspace after Qa, Qb, S
+// (n > 512, where they exceed the LDS); the LDS then holds only the transpose tile.
+// split_top (GSH only): stop before the top merge's GEMM (see stedc_block); workgroup b
+// handles matrix b (grid = batch) and keeps its state in workspace slot b.
 template <typename T, int NT, bool GSH>
 __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restrict__ Din,
                                                    const double *__restrict__ Ein, long strideV,
                                                    double *__restrict__ Wout, long strideW, T *__restrict__ Uout,
-                                                   long strideU, double *__restrict__ work, int batch) {
-    using C = cx<T>;
+                                                   long strideU, double *__restrict__ work, int batch,
+                                                   int split_top) {
     extern __shared__ double smem[];
     const int t = threadIdx.x;
     // per-workgroup global workspace: Qa, Qb, S (n*n double slots each, holding the
@@ -2290,44 +2340,76 @@ __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restri
     T *Qb0 = reinterpret_cast<T *>(wg + (long)n * n);
     T *S = reinterpret_cast<T *>(wg + 2L * n * n);
     DcShared sh;
-    {
-        double *p = GSH ? wg + 3L * n * n : smem;
-        sh.dq = p; p += n; sh.dqn = p; p += n; sh.ds = p; p += n; sh.zs = p; p += n;
-        sh.dl = p; p += n; sh.wz = p; p += n; sh.z2 = p; p += n; sh.tau = p; p += n; sh.vals = p; p += n;
-        sh.rc = p; p += n; sh.rs = p; p += n; sh.what = p; p += n; sh.inrm = p; p += n;
-        sh.red = p; p += 8;
-        int *ip = reinterpret_cast<int *>(p);
-        sh.perm = ip; ip += n; sh.orig = ip; ip += n; sh.rank = ip; ip += n; sh.rpj = ip; ip += n;
-        sh.rnj = ip; ip += n; sh.ndx = ip; ip += n; sh.dfx = ip; ip += n;
-        sh.pcol = ip; ip += n; sh.lstL = ip; ip += n; sh.lstR = ip; ip += n; sh.rflag = ip; ip += n;
-        sh.cnt = ip; ip += 5 * n;
-    }
+    dc_shared_at(GSH ? wg + 3L * n * n : smem, n, sh);
     for (long b = blockIdx.x; b < batch; b += gridDim.x) {
         const double *dIn = Din + b * strideV;
         const double *eIn = Ein + b * strideV;
         double *W = Wout + b * strideW;
         T *U = Uout + 2 * b * strideU;
         double *dqf;
-        T *Z = stedc_block<NT, T>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf);
+        T *Z = stedc_block<NT, T>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf, GSH && split_top);
+        if (!Z) return;                    // split: the top merge continues in other kernels
         for (int k = t; k < n; k += NT) W[k] = dqf[k];
         __syncthreads();
-        // U[r*n + j] = Z[j*n + r] (column-major Z -> component-major U) through 32x32
-        // LDS tiles (the D&C arrays are free now; the launch sizes LDS for the tile)
-        double *tile = smem;                       // 32 x 33
-        for (int tr = 0; tr < n; tr += 32)
-            for (int tc = 0; tc < n; tc += 32) {
-                for (int e = t; e < 1024; e += NT) {
-                    const int jj = e >> 5, rr = e & 31;            // read column tc+jj, row tr+rr
-                    if (tc + jj < n && tr + rr < n) tile[rr * 33 + jj] = Z[(long)(tc + jj) * n + tr + rr];
-                }
-                __syncthreads();
-                for (int e = t; e < 1024; e += NT) {
-                    const int rr = e >> 5, jj = e & 31;            // write row tr+rr, column tc+jj
-                    if (tc + jj < n && tr + rr < n) stx(U, (long)(tr + rr) * n + tc + jj, C{(T)tile[rr * 33 + jj], (T)0});
-                }
-                __syncthreads();
-            }
+        // the D&C arrays are free now; the launch sizes LDS for the tile
+        dc_transpose_out<T, NT>(n, Z, U, smem);
     }
+}
+
+// Level of the top merge of stedc_block (blk = 2^L >= n), and which buffer holds its
+// input eigenvector matrix (Qa0 at odd levels, Qb0 at even ones: the buffers swap after
+// every level) and its eigenvalues (dqn: sh.dqn at odd levels, sh.dq at even ones).
+__device__ inline int dc_top_level(int n) {
+    int L = 1;
+    while ((1 << L) < n) L++;
+    return L;
+}
+
+// The top merge's GEMM Qb = Qa S of stedc_block, nwg workgroups per matrix (the wave
+// tasks of merge_gemm_mfma dealt over all of them), and its deflated columns.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void stedc_top_gemm_kernel(int n, double *__restrict__ work, int nwg) {
+    const long b = blockIdx.x / nwg;
+    const int part = blockIdx.x % nwg;
+    double *wg = work + b * stedc_global_work_doubles(n);
+    DcShared sh;
+    dc_shared_at(wg + 3L * n * n, n, sh);
+    const int L = dc_top_level(n);
+    T *Qa0 = reinterpret_cast<T *>(wg), *Qb0 = reinterpret_cast<T *>(wg + (long)n * n);
+    const T *Qa = (L & 1) ? Qa0 : Qb0;
+    T *Qb = (L & 1) ? Qb0 : Qa0;
+    const int half = 1 << (L - 1), lo = 0, mid = half, hi = n, n1 = half, m = n;
+    const int K = sh.cnt[0], nd = sh.cnt[1], KL = sh.cnt[3], KR = sh.cnt[4];
+    const int lane = part * NT + threadIdx.x, G = nwg * NT;
+    if (sizeof(T) == sizeof(float) && half >= 64)
+        merge_gemm_mfma<T, 4, 2>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
+    else
+        merge_gemm_mfma<T, 2, 4>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
+    for (long idx = lane; idx < (long)m * nd; idx += G) {
+        const int qq = (int)(idx / m), rloc = (int)(idx - (long)qq * m);
+        const int r = lo + rloc;
+        Qb[(long)(lo + sh.rank[lo + K + qq]) * n + r] = Qa[(long)(lo + sh.perm[lo + sh.dfx[lo + qq]]) * n + r];
+    }
+}
+
+// Outputs of a split divide & conquer: eigenvalues (scaled back) to W, eigenvectors to U.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void stedc_finish_kernel(int n, double *__restrict__ work,
+                                                          double *__restrict__ Wout, long strideW,
+                                                          T *__restrict__ Uout, long strideU) {
+    extern __shared__ double smem[];
+    const long b = blockIdx.x;
+    double *wg = work + b * stedc_global_work_doubles(n);
+    DcShared sh;
+    dc_shared_at(wg + 3L * n * n, n, sh);
+    const int L = dc_top_level(n);
+    const T *Z = reinterpret_cast<const T *>((L & 1) ? wg + (long)n * n : wg);
+    const double *dqn = (L & 1) ? sh.dqn : sh.dq;
+    const double scale = sh.red[0];
+    double *W = Wout + b * strideW;
+    for (int k = threadIdx.x; k < n; k += NT) W[k] = dqn[k] * scale;
+    __syncthreads();
+    dc_transpose_out<T, NT>(n, Z, Uout + 2 * b * strideU, smem);
 }
 
 // Stage 2 kernel: back-transform U = Q U with compact-WY blocks of nb_wy reflectors
@@ -2836,14 +2918,14 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
     if (n <= 128) {
         // two waves per matrix: more matrices per CU where the nodes are small
         hipLaunchKernelGGL((stedc_kernel<T, 128, false>), dim3(grid), dim3(128), lds_dc, stream, n, D, E, strideV, W,
-                           strideW, U, strideU, work, batch);
+                           strideW, U, strideU, work, batch, 0);
     } else {
         constexpr int NT = 256;
         if (lds_dc > 64 * 1024)
             RLM_HIP_CHECK(hipFuncSetAttribute((const void *)stedc_kernel<T, NT, false>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_dc));
         hipLaunchKernelGGL((stedc_kernel<T, NT, false>), dim3(grid), dim3(NT), lds_dc, stream, n, D, E, strideV, W,
-                           strideW, U, strideU, work, batch);
+                           strideW, U, strideU, work, batch, 0);
     }
     RLM_HIP_CHECK(hipGetLastError());
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, stream));
@@ -3258,9 +3340,32 @@ static int launch_stedc_global_t(int n, const double *D, const double *E, long s
     if (n < 1 || batch < 1) return batch < 1 ? 0 : -22;
     if (grid > batch) grid = batch;
     const size_t lds = 32 * 33 * sizeof(double);
-    hipLaunchKernelGGL((stedc_kernel<T, NT, true>), dim3(grid), dim3(NT), lds, stream, n, D, E, strideV, W, strideW,
-                       U, strideU, work, batch);
-    RLM_HIP_CHECK(hipGetLastError());
+    const char *sp = getenv("RLM_DC_SPLIT");
+    const bool split = n >= 1024 && !(sp && strcmp(sp, "0") == 0);
+    if (!split) {
+        hipLaunchKernelGGL((stedc_kernel<T, NT, true>), dim3(grid), dim3(NT), lds, stream, n, D, E, strideV, W,
+                           strideW, U, strideU, work, batch, 0);
+        RLM_HIP_CHECK(hipGetLastError());
+        return 0;
+    }
+    // large n: the top merge's GEMM (about 70 % of the divide & conquer's matrix-core
+    // work) on several workgroups per matrix instead of one, in rounds of `grid` matrices
+    // (one workspace slot each)
+    int dev = 0, cus = 0;
+    RLM_HIP_CHECK(hipGetDevice(&dev));
+    RLM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    for (int b0 = 0; b0 < batch; b0 += grid) {
+        const int cnt = std::min(grid, batch - b0);
+        const int nwg = std::max(1, std::min(16, (2 * std::max(cus, 1) + cnt - 1) / cnt));
+        hipLaunchKernelGGL((stedc_kernel<T, NT, true>), dim3(cnt), dim3(NT), lds, stream, n, D + (long)b0 * strideV,
+                           E + (long)b0 * strideV, strideV, W + (long)b0 * strideW, strideW, U + 2L * b0 * strideU,
+                           strideU, work, cnt, 1);
+        hipLaunchKernelGGL((stedc_top_gemm_kernel<T, NT>), dim3((unsigned)((long)cnt * nwg)), dim3(NT), 0, stream, n,
+                           work, nwg);
+        hipLaunchKernelGGL((stedc_finish_kernel<T, NT>), dim3(cnt), dim3(NT), lds, stream, n, work,
+                           W + (long)b0 * strideW, strideW, U + 2L * b0 * strideU, strideU);
+        RLM_HIP_CHECK(hipGetLastError());
+    }
     return 0;
 }
 int launch_stedc_global(int n, const double *D, const double *E, long strideV, double *W, long strideW, double *U,

@@ -39,14 +39,15 @@ def test_large_random_hermitian_mixed(n, b):
     _check_mixed(H, w.cpu().numpy(), U.cpu().numpy())
 
 
+@pytest.mark.parametrize("n", [600, 1030])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_large_structured(precision):
+def test_large_structured(precision, n):
     """Zero, identity (fully degenerate: every merge deflates), diagonal, circular-band
-    (the rlm Hamiltonian's sparsity at n = 600, off_head = 49) and 1e+-30-scaled matrices."""
+    (the rlm Hamiltonian's sparsity at n = 600, off_head = 49) and 1e+-30-scaled matrices;
+    at n = 1030 the top merge's eigenvector product runs on several workgroups per matrix."""
     import torch
     from rlm_amd import heevd_batched, HIP_DIAG_FP64, HIP_DIAG_MIXED
-    print(f"[test_large_structured] {precision}", flush=True)
-    n = 600
+    print(f"[test_large_structured] {precision} n={n}", flush=True)
     rng = np.random.default_rng(17)
     mats = [np.zeros((n, n), dtype=np.complex128), np.eye(n, dtype=np.complex128),
             np.diag(np.linspace(-1, 1, n)).astype(np.complex128)]
