@@ -1729,7 +1729,7 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
     const double Dj = dl[j] - dlo;
     const double Dj1 = (j < K - 1) ? (dl[j + 1] - dlo) : 0.0;
     for (int it = 0; it < 120; it++) {
-        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, err = 0.0;
+        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, err;
         // poles at or below j feed psi, the others phi (all terms of one sign each);
         // the roots of a wave are consecutive, so its lanes' split points differ by
         // at most 63 and the two loops together run about K + 64 iterations
@@ -1739,7 +1739,6 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
             const double tt = z2[i] * inv;
             psi += tt;
             dpsi += tt * inv;
-            err += fabs(tt);
         }
 #pragma unroll 4
         for (int i = j + 1; i < K; i++) {
@@ -1747,8 +1746,10 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
             const double tt = z2[i] * inv;
             phi += tt;
             dphi += tt * inv;
-            err += fabs(tt);
         }
+        // sum of |terms|: the psi terms are all <= 0 (poles below the root), the phi terms
+        // all >= 0, so it is phi - psi without a third accumulation
+        err = phi - psi;
         const double delj = (dl[j] - dlo) - tau;
         const double delj1 = (j + 1 < K) ? (dl[j + 1] - dlo) - tau : 0.0;
         const double f = 1.0 + psi + phi;
@@ -1800,9 +1801,12 @@ __device__ inline int count_below(const double *a, int len, double v, bool inclu
 // (same accumulator registers, twice the Qa loads) where the halves have >= 64 rows. QS: storage and matrix-core type of the eigenvector matrices (double; float in
 // the mixed-precision solver, whose eigenvectors carry the FP32 accuracy of its
 // reflectors anyway).
+// Uo (top merge only, lo = 0): the products go straight to the component-major complex
+// output, Uo[r * n + rank] = (value, 0), instead of column rank of Qb.
 template <typename QS, int RB, int QT>
 __device__ __forceinline__ void merge_gemm_mfma(int lane, int G, int lo, int mid, int hi, int n1, int K, int KL, int KR,
-                                                int n, const QS *Qa, QS *Qb, const DcShared &sh) {
+                                                int n, const QS *Qa, QS *Qb, const DcShared &sh,
+                                                QS *Uo = nullptr) {
     using MQ = Mf<QS>;
     using v4 = typename MQ::v4;
     constexpr int RH = 16 * RB;
@@ -1874,13 +1878,17 @@ __device__ __forceinline__ void merge_gemm_mfma(int lane, int G, int lo, int mid
         for (int u = 0; u < QT; u++) {
             const int q = (qt0 + u) * 16 + li;
             if (u < nq && q < K) {
-                QS *ocol = Qb + (long)(lo + sh.rank[lo + q]) * n;
+                const int oc = lo + sh.rank[lo + q];
+                QS *ocol = Qb + (long)oc * n;
 #pragma unroll
                 for (int rb = 0; rb < RB; rb++)
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         const int r = rbase + rb * 16 + MQ::row(l, v);
-                        if (r < rend) ocol[r] = acc[rb][u][v];
+                        if (r < rend) {
+                            if (Uo) stx(Uo, (long)r * n + oc, cx<QS>{acc[rb][u][v], (QS)0});
+                            else ocol[r] = acc[rb][u][v];
+                        }
                     }
             }
         }
@@ -1893,11 +1901,13 @@ __device__ __forceinline__ void merge_gemm_mfma(int lane, int G, int lo, int mid
 // every other quantity is FP64.
 // split_top: stop at the top merge once its secular roots, weights, norms and ranks are
 // known (nullptr returned, the scale left in sh.red[0]); its eigenvector GEMM then runs on
-// several workgroups per matrix (stedc_top_gemm_kernel) and stedc_finish_kernel writes
-// the outputs.
+// several workgroups per matrix (stedc_top_gemm_kernel), which writes the outputs.
+// Uout (optional): where the top merge takes the matrix-core path, its products and
+// deflated columns go straight to the component-major complex output Uout[r * n + j]
+// (*direct set), which saves a transposing pass over the eigenvectors.
 template <int NT, typename QS>
 __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, QS *Qb, QS *S, DcShared &sh,
-                           double **dq_final, bool split_top = false) {
+                           double **dq_final, bool split_top = false, QS *Uout = nullptr, bool *direct = nullptr) {
     const int t = threadIdx.x;
     // scale T to unit max-norm (as LAPACK dstedc does) so that the deflation
     // tolerance is relative to the matrix, whatever its magnitude
@@ -1935,6 +1945,9 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
         const int nnodes = (n + blk - 1) / blk;
         int G = 1;
         while (G * 2 <= NT / nnodes && G * 2 <= blk) G *= 2;
+        // halves of >= 32 rows merge on the matrix cores, which take a whole wave per node:
+        // fewer nodes per round rather than the VALU product
+        if (half >= 32 && G < kWave) G = kWave;
         if (G > NT) G = NT;
         const int per = NT / G;  // nodes handled per round
         for (int nb0 = 0; nb0 < nnodes; nb0 += per) {
@@ -2222,11 +2235,15 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
             // P8: Qb block = Qa block (rotated, gathered) * U; deflated columns copied.
             // Rows of the left half only meet columns of lstL (left child or rotated), rows
             // of the right half only lstR (the block-diagonal structure dlaed3 exploits).
+            // the top merge on the matrix cores writes the output itself (top_direct is
+            // uniform over the workgroup; Uo is set on the lanes of node 0)
+            const bool top_direct = Uout && blk >= n && G >= kWave && half >= 32;
+            QS *Uo = (mfma && top_direct) ? Uout : nullptr;
             if (mfma) {
                 if (sizeof(QS) == sizeof(float) && half >= 64)
-                    merge_gemm_mfma<QS, 4, 2>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
+                    merge_gemm_mfma<QS, 4, 2>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh, Uo);
                 else
-                    merge_gemm_mfma<QS, 2, 4>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
+                    merge_gemm_mfma<QS, 2, 4>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh, Uo);
             } else if (merging) {
                 // VALU path: each task is one row and 4 consecutive output columns,
                 // consecutive lanes on consecutive rows
@@ -2255,7 +2272,13 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                     if (q0 + 3 < K) Qb[(long)(lo + sh.rank[lo + q0 + 3]) * n + r] = (QS)a3;
                 }
             }
-            if (merging) {
+            if (Uo) {
+                // deflated columns, consecutive lanes on consecutive output columns of a row
+                for (int idx = lane; idx < m * nd; idx += G) {
+                    const int r = idx / nd, qq = idx - r * nd;
+                    stx(Uo, (long)r * n + sh.rank[K + qq], cx<QS>{Qa[(long)sh.perm[sh.dfx[qq]] * n + r], (QS)0});
+                }
+            } else if (merging) {
                 for (int idx = lane; idx < m * nd; idx += G) {
                     const int qq = idx / m, rloc = idx - qq * m;
                     const int r = lo + rloc;
@@ -2268,6 +2291,7 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                 }
                 for (int q = lane; q < m; q += G) dqn[lo + q] = dq[lo + q];
             }
+            if (top_direct) *direct = true;
             __syncthreads();
         }
         QS *tq = Qa; Qa = Qb; Qb = tq;
@@ -2347,12 +2371,13 @@ __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restri
         double *W = Wout + b * strideW;
         T *U = Uout + 2 * b * strideU;
         double *dqf;
-        T *Z = stedc_block<NT, T>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf, GSH && split_top);
+        bool direct = false;
+        T *Z = stedc_block<NT, T>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf, GSH && split_top, U, &direct);
         if (!Z) return;                    // split: the top merge continues in other kernels
         for (int k = t; k < n; k += NT) W[k] = dqf[k];
         __syncthreads();
         // the D&C arrays are free now; the launch sizes LDS for the tile
-        dc_transpose_out<T, NT>(n, Z, U, smem);
+        if (!direct) dc_transpose_out<T, NT>(n, Z, U, smem);
     }
 }
 
@@ -2365,10 +2390,13 @@ __device__ inline int dc_top_level(int n) {
     return L;
 }
 
-// The top merge's GEMM Qb = Qa S of stedc_block, nwg workgroups per matrix (the wave
-// tasks of merge_gemm_mfma dealt over all of them), and its deflated columns.
+// The top merge's product of stedc_block, nwg workgroups per matrix (the wave tasks of
+// merge_gemm_mfma dealt over all of them), written with its deflated columns straight to
+// the component-major output U; the eigenvalues (scaled back) to W.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void stedc_top_gemm_kernel(int n, double *__restrict__ work, int nwg) {
+__global__ __launch_bounds__(NT) void stedc_top_gemm_kernel(int n, double *__restrict__ work, int nwg,
+                                                            double *__restrict__ Wout, long strideW,
+                                                            T *__restrict__ Uout, long strideU) {
     const long b = blockIdx.x / nwg;
     const int part = blockIdx.x % nwg;
     double *wg = work + b * stedc_global_work_doubles(n);
@@ -2378,38 +2406,24 @@ __global__ __launch_bounds__(NT) void stedc_top_gemm_kernel(int n, double *__res
     T *Qa0 = reinterpret_cast<T *>(wg), *Qb0 = reinterpret_cast<T *>(wg + (long)n * n);
     const T *Qa = (L & 1) ? Qa0 : Qb0;
     T *Qb = (L & 1) ? Qb0 : Qa0;
+    T *U = Uout + 2 * b * strideU;
     const int half = 1 << (L - 1), lo = 0, mid = half, hi = n, n1 = half, m = n;
     const int K = sh.cnt[0], nd = sh.cnt[1], KL = sh.cnt[3], KR = sh.cnt[4];
     const int lane = part * NT + threadIdx.x, G = nwg * NT;
     if (sizeof(T) == sizeof(float) && half >= 64)
-        merge_gemm_mfma<T, 4, 2>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
+        merge_gemm_mfma<T, 4, 2>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh, U);
     else
-        merge_gemm_mfma<T, 2, 4>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh);
+        merge_gemm_mfma<T, 2, 4>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh, U);
     for (long idx = lane; idx < (long)m * nd; idx += G) {
-        const int qq = (int)(idx / m), rloc = (int)(idx - (long)qq * m);
-        const int r = lo + rloc;
-        Qb[(long)(lo + sh.rank[lo + K + qq]) * n + r] = Qa[(long)(lo + sh.perm[lo + sh.dfx[lo + qq]]) * n + r];
+        const int r = (int)(idx / nd), qq = (int)(idx - (long)r * nd);
+        stx(U, (long)r * n + sh.rank[K + qq], cx<T>{Qa[(long)sh.perm[sh.dfx[qq]] * n + r], (T)0});
     }
-}
-
-// Outputs of a split divide & conquer: eigenvalues (scaled back) to W, eigenvectors to U.
-template <typename T, int NT>
-__global__ __launch_bounds__(NT) void stedc_finish_kernel(int n, double *__restrict__ work,
-                                                          double *__restrict__ Wout, long strideW,
-                                                          T *__restrict__ Uout, long strideU) {
-    extern __shared__ double smem[];
-    const long b = blockIdx.x;
-    double *wg = work + b * stedc_global_work_doubles(n);
-    DcShared sh;
-    dc_shared_at(wg + 3L * n * n, n, sh);
-    const int L = dc_top_level(n);
-    const T *Z = reinterpret_cast<const T *>((L & 1) ? wg + (long)n * n : wg);
-    const double *dqn = (L & 1) ? sh.dqn : sh.dq;
-    const double scale = sh.red[0];
-    double *W = Wout + b * strideW;
-    for (int k = threadIdx.x; k < n; k += NT) W[k] = dqn[k] * scale;
-    __syncthreads();
-    dc_transpose_out<T, NT>(n, Z, Uout + 2 * b * strideU, smem);
+    if (part == 0) {
+        const double *dqn = (L & 1) ? sh.dqn : sh.dq;
+        const double scale = sh.red[0];
+        double *W = Wout + b * strideW;
+        for (int k = threadIdx.x; k < n; k += NT) W[k] = dqn[k] * scale;
+    }
 }
 
 // Stage 2 kernel: back-transform U = Q U with compact-WY blocks of nb_wy reflectors
@@ -3361,9 +3375,7 @@ static int launch_stedc_global_t(int n, const double *D, const double *E, long s
                            E + (long)b0 * strideV, strideV, W + (long)b0 * strideW, strideW, U + 2L * b0 * strideU,
                            strideU, work, cnt, 1);
         hipLaunchKernelGGL((stedc_top_gemm_kernel<T, NT>), dim3((unsigned)((long)cnt * nwg)), dim3(NT), 0, stream, n,
-                           work, nwg);
-        hipLaunchKernelGGL((stedc_finish_kernel<T, NT>), dim3(cnt), dim3(NT), lds, stream, n, work,
-                           W + (long)b0 * strideW, strideW, U + 2L * b0 * strideU, strideU);
+                           work, nwg, W + (long)b0 * strideW, strideW, U + 2L * b0 * strideU, strideU);
         RLM_HIP_CHECK(hipGetLastError());
     }
     return 0;
