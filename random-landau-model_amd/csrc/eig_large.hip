@@ -109,11 +109,24 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
 #pragma unroll
     for (int q = 0; q < kLgRPT; q++) {
         const int r = c + t + q * NT;
-        C x{(T)0, (T)0};
+        a[q] = r < n ? ldx(A, (long)c * n + r) : C{(T)0, (T)0};
+    }
+    // the pending update, one panel column at a time over all rows of the thread (the
+    // 2 kLgRPT loads of a step are independent: latency overlaps)
+    for (int l = 0; l < i; l++) {
+        const C wc = conjx(rowc[l]), vc = conjx(rowc[kLgNB + l]);
+        const C *vl = Vp + (long)l * n, *wl = Wp + (long)l * n;
+#pragma unroll
+        for (int q = 0; q < kLgRPT; q++) {
+            const int r = c + t + q * NT;
+            if (r < n) a[q] = a[q] - (vl[r] * wc + wl[r] * vc);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kLgRPT; q++) {
+        const int r = c + t + q * NT;
+        C x = a[q];
         if (r < n) {
-            x = ldx(A, (long)c * n + r);
-            for (int l = 0; l < i; l++)
-                x = x - (Vp[(long)l * n + r] * conjx(rowc[l]) + Wp[(long)l * n + r] * conjx(rowc[kLgNB + l]));
             if (r == c) {
                 x.y = (T)0;
                 sc[0] = x.x;
@@ -179,9 +192,18 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
     // (d) P[l] = W(:, l)^H v, P[kLgNB + l] = V(:, l)^H v over rows > c, one wave per product
     const int m = n - c - 1;
     for (int qd = wid; qd < 2 * i; qd += NW) {
-        const C *col = (qd < i) ? Wp + (long)qd * n : Vp + (long)(qd - i) * n;
+        const C *col = ((qd < i) ? Wp + (long)qd * n : Vp + (long)(qd - i) * n) + c + 1;
         C s{(T)0, (T)0};
-        for (int e = lane; e < m; e += kWave) fmacx(s, col[c + 1 + e], vsh[e]);
+        int e = lane;
+        // eight loads in flight per step, accumulated in the same order as one at a time
+        for (; e + 7 * kWave < m; e += 8 * kWave) {
+            C x[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) x[k] = col[e + k * kWave];
+#pragma unroll
+            for (int k = 0; k < 8; k++) fmacx(s, x[k], vsh[e + k * kWave]);
+        }
+        for (; e < m; e += kWave) fmacx(s, col[e], vsh[e]);
         s.x = wave_sum_t(s.x);
         s.y = wave_sum_t(s.y);
         if (lane == 0) P[qd < i ? qd : kLgNB + (qd - i)] = s;
@@ -303,10 +325,30 @@ __global__ __launch_bounds__(64) void lg_symv_reduce_kernel(T *__restrict__ aux,
     const C *Cs = Rs + (long)ns0 * kLgTS;
     const int lr = K * kLgTS + t;
     if (t >= kLgTS || lr >= m) return;
+    // loads issued eight ahead of the fixed-order sums
     C y = Rs[(long)K * kLgTS + t];
-    for (int I = K; I < ns; I++) y = y + Cs[((long)I * ns0 + K) * kLgTS + t];
+    int I = K;
+    for (; I + 8 <= ns; I += 8) {
+        C x[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) x[k] = Cs[((long)(I + k) * ns0 + K) * kLgTS + t];
+#pragma unroll
+        for (int k = 0; k < 8; k++) y = y + x[k];
+    }
+    for (; I < ns; I++) y = y + Cs[((long)I * ns0 + K) * kLgTS + t];
     const int r = s0 + lr;
-    for (int l = 0; l < i; l++) y = y - (Vp[(long)l * n + r] * P[l] + Wp[(long)l * n + r] * P[kLgNB + l]);
+    int l = 0;
+    for (; l + 4 <= i; l += 4) {
+        C xv[4], xw[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            xv[k] = Vp[(long)(l + k) * n + r];
+            xw[k] = Wp[(long)(l + k) * n + r];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) y = y - (xv[k] * P[l + k] + xw[k] * P[kLgNB + l + k]);
+    }
+    for (; l < i; l++) y = y - (Vp[(long)l * n + r] * P[l] + Wp[(long)l * n + r] * P[kLgNB + l]);
     Wp[(long)i * n + r] = y;
 }
 
