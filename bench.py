@@ -105,8 +105,8 @@ def _tuned(n, precision):
     return tuned_kernel(n, precision) if n <= 512 else "n/a (multi-workgroup solver)"
 
 
-def _load_traffic(key):
-    """HBM bytes per launch of the dominant kernel from rocprofv3 FETCH_SIZE/WRITE_SIZE
+def _load_traffic(workload, key):
+    """HBM bytes per launch of a kernel on a workload, from rocprofv3 FETCH_SIZE/WRITE_SIZE
     counters (profiles/traffic.json, written from the committed counter summaries)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -114,7 +114,7 @@ def _load_traffic(key):
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get(key)
+        return t.get(workload, {}).get(key)
     except Exception:
         return None
 
@@ -306,6 +306,7 @@ def main():
         print(f"[bench] {tag}: n_phi={n_phi} g={g} samples={S} seed={seed}: {nkp} twist points, {nsolved} solved", file=sys.stderr,
               flush=True)
         plan = RlmPlan(n_phi, 1.0, 16.0, g, seed, samples_per_batch=S, precision=precision)
+        off = plan.off
         plan.load(0, S)
         torch.cuda.synchronize()
         for w in range(args.warmup):
@@ -323,13 +324,22 @@ def main():
                   file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        symv = None
+        if n_phi > 512:
+            # one more (untimed) step with HIP events around every launch of the Hermitian
+            # product kernel, the n > 512 solver's dominant kernel (one launch per column)
+            from rlm_amd import symv_timing
+            symv_timing(True)
+            plan.execute()
+            tot_ms, launches = symv_timing(False)
+            symv = {"total_ms": tot_ms, "launches": launches}
         e, c, m = plan.fetch()
         res = plan.max_residual()
         plan.close()
         del plan
         stage_avg = {a: v / args.steps for a, v in stage_sum.items()}
         return {"elapsed": elapsed, "stage": stage_avg, "e": e, "c": c, "m": m, "resid": res,
-                "value": 3600.0 * S * args.steps / elapsed}
+                "value": 3600.0 * S * args.steps / elapsed, "symv": symv, "off": off}
 
     fp64 = run_path(HIP_DIAG_FP64, "fp64") if args.path in ("both", "fp64") else None
     mixed = run_path(HIP_DIAG_MIXED, "mixed") if args.path in ("both", "mixed") else None
@@ -357,15 +367,74 @@ def main():
     main = mixed if mixed_valid else fp64
     path = "mixed" if mixed_valid else "fp64"
 
-    # dominant kernel of the reported path: algorithmic flops per launch / measured launch time
-    dom = max(flops, key=lambda k: main["stage"][k])
-    flops_unit = flops[dom]
-    bt_name = "backtransform_mfma_kernel" if n_phi <= 256 else "lg_backtransform_kernel"
-    kname = trd_name(path) if dom == "tridiagonalisation" else bt_name
-    t_launch = main["stage"][dom] / 1e3
-    achieved = flops_unit * nsolved / t_launch / 1e12
-    peak = FP32_PEAK_TFLOPS if (path == "mixed") else FP64_PEAK_TFLOPS
-    traffic = _load_traffic(f"{kname}<{'float' if path == 'mixed' else 'double'}>")
+    # dominant kernel of the reported path: algorithmic work per launch / measured launch time
+    fp32 = path == "mixed"
+    peak_flops = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+    ctype = "float" if fp32 else "double"
+    bt_name = "backtransform_mfma_kernel" if n_phi <= 256 else (
+        "backtransform_mfma_kernel" if n_phi <= 512 else "lg_backtransform_kernel")
+    workload_key = f"rlm n_phi={n_phi} g={g}"
+    if n_phi > 512:
+        # n > 512: the Hermitian product of the reduction (one launch per column over the
+        # batch) streams the trailing lower triangle: (m (m + 1) / 2) complex elements per
+        # matrix and column, m = n - 1 - column, read once per launch
+        esz = 8 if fp32 else 16
+        cols = n_phi - 1
+        tri = sum((n_phi - 1 - c) * (n_phi - c) // 2 for c in range(cols))
+        bytes_per_launch = nsolved * tri * esz / cols
+        sv = main["symv"]
+        t_launch = sv["total_ms"] / sv["launches"] / 1e3
+        kname = "lg_symv_lower_kernel"
+        roof = {
+            "kernel": kname,
+            "bound": "hbm",
+            "achieved": bytes_per_launch / t_launch / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": bytes_per_launch / t_launch / 1e9 / HBM_PEAK_GBS,
+            "traffic": _load_traffic(workload_key, f"{kname}<{ctype}>"),
+            "bytes_per_launch": bytes_per_launch,
+            "launch_ms": t_launch * 1e3,
+            "launches_per_step": sv["launches"],
+        }
+    else:
+        dom = max(flops, key=lambda k: main["stage"][k])
+        kname = trd_name(path) if dom == "tridiagonalisation" else bt_name
+        t_launch = main["stage"][dom] / 1e3
+        achieved = flops[dom] * nsolved / t_launch / 1e12
+        roof = {
+            "kernel": kname,
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": peak_flops,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak_flops,
+            "traffic": _load_traffic(workload_key, f"{kname}<{ctype}>"),
+            "flops_per_matrix": flops[dom],
+            "matrices_per_launch": nsolved,
+        }
+    # every stage of the reported path against its bound: the solver stages by their
+    # algorithmic flops, assembly and links by their algorithmic HBM bytes
+    hnb = min(2 * main["off"] + 1, n_phi)
+    vec_bytes = 8 if (fp32 and n_phi <= 512) else 16     # links read FP32 vectors, or refined FP64 ones
+    # band form written; dense matrices too where the solver takes them (FP64 plans, n > 512)
+    asm_bytes = nsolved * n_phi * hnb * 16 + (0 if (fp32 and n_phi <= 512) else nsolved * n_phi * n_phi * 16)
+    link_bytes = nsolved * n_phi * n_phi * vec_bytes + 2 * nkp * n_phi * 16
+    st = main["stage"]
+    roof["stages"] = {
+        "tridiagonalisation": {"tflops": flops["tridiagonalisation"] * nsolved / st["tridiagonalisation"] / 1e9,
+                               "frac_of_peak": flops["tridiagonalisation"] * nsolved / st["tridiagonalisation"] / 1e9
+                               / peak_flops},
+        "backtransform": {"tflops": flops["backtransform"] * nsolved / st["backtransform"] / 1e9,
+                          "frac_of_peak": flops["backtransform"] * nsolved / st["backtransform"] / 1e9 / peak_flops},
+        "assembly": {"gbs": asm_bytes / st["assembly"] / 1e6,
+                     "frac_of_peak": asm_bytes / st["assembly"] / 1e6 / HBM_PEAK_GBS,
+                     "bytes": asm_bytes},
+        "links": {"gbs": link_bytes / st["links"] / 1e6, "frac_of_peak": link_bytes / st["links"] / 1e6 / HBM_PEAK_GBS,
+                  "bytes": link_bytes},
+        "peak_tflops": peak_flops,
+        "peak_gbs": HBM_PEAK_GBS,
+    }
 
     line = {
         "metric": (f"samples/hour rlm n_phi={n_phi} g={g}: mixed-prec vs FP64 GPU, all Chern ints exact"
@@ -402,17 +471,7 @@ def main():
             "autotuned": {"fp64": _tuned(n_phi, 0), "mixed": _tuned(n_phi, 1)},
             "min_plaquette_margin_rad": {"fp64": float(fp64["m"].min()), "mixed": float(mixed["m"].min())},
         },
-        "roofline": {
-            "kernel": kname,
-            "bound": "mfma",
-            "achieved": achieved,
-            "peak": peak,
-            "unit": "TFLOP/s",
-            "frac": achieved / peak,
-            "traffic": traffic,
-            "flops_per_matrix": flops_unit,
-            "matrices_per_launch": nsolved,
-        },
+        "roofline": roof,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

@@ -306,6 +306,12 @@ int hlm_plan_destroy(hlm_plan_t plan);
  * "lower8", "hybrid", "fused", "blocked", "small", or "auto" (not tuned: fixed
  * defaults). RLM_TUNE=0 disables tuning, RLM_HETRD=<name> forces a kernel. */
 const char *hip_diag_tuned_kernel(int n, int precision);
+/* Measurement aid, no counterpart in the reference: enable = 1 starts recording HIP
+ * events around every eager launch of the n > 512 solver's Hermitian product kernel
+ * (lg_symv_lower_kernel, one launch per column; launches under graph capture are not
+ * timed); enable = 0 stops, waits for the recorded launches and returns their summed
+ * time in ms and their count. */
+int hip_diag_symv_timing(int enable, double *total_ms, long *launches);
 /* hlm/main_hlm_cpu.cpp:66-67,79: samples first..first+n_samples-1 of the run's
  * single stream, n_sites values each, scaled to [-W/2, W/2). Host memory. */
 int hlm_generate_disorder(double *V, long n_sites, long n_samples, unsigned long seed, long first_sample, double W);

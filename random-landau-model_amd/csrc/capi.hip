@@ -1488,6 +1488,10 @@ extern "C" const char *hip_diag_tuned_kernel(int n, int precision) {
     return hetrd_choice_name(n, precision == HIP_DIAG_MIXED);
 }
 
+extern "C" int hip_diag_symv_timing(int enable, double *total_ms, long *launches) {
+    return symv_timing(enable, total_ms, launches);
+}
+
 extern "C" int hlm_plan_destroy(hlm_plan_t p) {
     if (!p) return 0;
     if (p->stream) (void)hipStreamSynchronize(p->stream);

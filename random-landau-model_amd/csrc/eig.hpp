@@ -75,6 +75,9 @@ int launch_backtransform_stream(const double *A, long strideA, int n, const doub
 int launch_backtransform_stream(const float *A, long strideA, int n, const float *tws, long strideT, float *U,
                                 long strideU, int batch, hipStream_t s);
 int heevd_large_max_n();
+// Launch timing of lg_symv_lower_kernel (measurement): enable = 1 starts recording events
+// around its eager launches; enable = 0 stops, waits, and returns the summed time and count.
+int symv_timing(int enable, double *total_ms, long *launches);
 long heevd_large_aux_complex(int n);
 int heevd_large(double *A, int n, long batch, double *D, double *E, double *TAU, long strideV, double *aux, double *W,
                 double *U, double *work, int grid_dc, double *tws, hipStream_t s, hipEvent_t mid = nullptr,

@@ -25,6 +25,8 @@
 #include "eig.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 namespace rlm {
 
@@ -33,6 +35,53 @@ constexpr int kLgColNT = 512;  // threads of the column kernel
 constexpr int kLgRPT = 8;      // rows per thread of the column kernel: n <= 4096
 
 int heevd_large_max_n() { return kLgColNT * kLgRPT; }
+
+// Launch timing of the Hermitian product kernel (lg_symv_lower_kernel, one launch per
+// column): while enabled, eager launches are bracketed by HIP events from a pool.
+namespace {
+std::mutex g_kt_mu;
+bool g_kt_on = false;
+std::vector<hipEvent_t> g_kt_ev;
+size_t g_kt_used = 0;
+
+// the two events of the next timed launch (nullptr: timing off or the stream is capturing)
+void kt_pair(hipStream_t s, hipEvent_t &e0, hipEvent_t &e1) {
+    e0 = e1 = nullptr;
+    std::lock_guard<std::mutex> lk(g_kt_mu);
+    if (!g_kt_on) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    while (g_kt_ev.size() < g_kt_used + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        g_kt_ev.push_back(e);
+    }
+    e0 = g_kt_ev[g_kt_used];
+    e1 = g_kt_ev[g_kt_used + 1];
+    g_kt_used += 2;
+}
+}  // namespace
+
+int symv_timing(int enable, double *total_ms, long *launches) {
+    std::lock_guard<std::mutex> lk(g_kt_mu);
+    if (enable) {
+        g_kt_on = true;
+        g_kt_used = 0;
+        return 0;
+    }
+    g_kt_on = false;
+    double tot = 0.0;
+    if (g_kt_used) RLM_HIP_CHECK(hipEventSynchronize(g_kt_ev[g_kt_used - 1]));
+    for (size_t k = 0; k + 1 < g_kt_used; k += 2) {
+        float ms = 0.f;
+        RLM_HIP_CHECK(hipEventElapsedTime(&ms, g_kt_ev[k], g_kt_ev[k + 1]));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (long)(g_kt_used / 2);
+    g_kt_used = 0;
+    return 0;
+}
 
 // Per-matrix panel workspace, complex elements: Vp (kLgNB x n, column l = reflector
 // k0 + l with explicit zeros above and the unit entry), Wp (kLgNB x n), P (2 kLgNB:
@@ -705,8 +754,12 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
             if (i == nbk) break;
             const int m = n - (k0 + i) - 1;
             const int nsm = (m + kLgTS - 1) / kLgTS, npair = (nsm + 1) / 2;
+            hipEvent_t te0, te1;
+            kt_pair(s, te0, te1);
+            if (te0) RLM_HIP_CHECK(hipEventRecord(te0, s));
             hipLaunchKernelGGL(lg_symv_lower_kernel<T>, dim3((unsigned)((long)nb * npair)), dim3(256), sizeof(C) * m,
                                s, A, strideA, n, aux, strideAux, k0, i, npair, ns0);
+            if (te1) RLM_HIP_CHECK(hipEventRecord(te1, s));
             hipLaunchKernelGGL(lg_symv_reduce_kernel<T>, dim3((unsigned)((long)nb * nsm)), dim3(64), 0, s, aux,
                                strideAux, n, k0, i, ns0);
         }

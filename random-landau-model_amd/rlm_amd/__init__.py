@@ -90,6 +90,7 @@ def lib() -> ctypes.CDLL:
         "rlm_multi_elapsed_ms": ([vp, dp], i),
         "rlm_multi_destroy": ([vp], i),
         "hip_diag_tuned_kernel": ([i, i], ctypes.c_char_p),
+        "hip_diag_symv_timing": ([i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)], i),
         "rlm_plan_destroy": ([vp], i),
         "hlm_plan_create": ([ctypes.POINTER(vp), i, i, i, i, i, ctypes.c_ulong, d, i, i], i),
         "hlm_plan_load": ([vp, l, i], i),
@@ -297,6 +298,17 @@ class RlmPlan:
 def tuned_kernel(n: int, precision: int = HIP_DIAG_FP64) -> str:
     """hip_diag_tuned_kernel: the tridiagonalisation kernel the autotuner chose."""
     return lib().hip_diag_tuned_kernel(n, precision).decode()
+
+
+def symv_timing(enable: bool):
+    """hip_diag_symv_timing: start (enable=True, returns None) or stop and read
+    (enable=False, returns (total_ms, launches)) the launch timing of the n > 512
+    solver's Hermitian product kernel."""
+    tot = ctypes.c_double(0.0)
+    cnt = ctypes.c_long(0)
+    _check(lib().hip_diag_symv_timing(1 if enable else 0, ctypes.byref(tot), ctypes.byref(cnt)),
+           "hip_diag_symv_timing")
+    return None if enable else (tot.value, cnt.value)
 
 
 class RlmMulti:
