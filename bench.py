@@ -238,6 +238,9 @@ def main():
                     help="c3 (default): rlm n_phi=256 g=32 x 32 samples (the contract line); c2: hlm 24x24 q=8 "
                          "b=0 W=1 g=32 x 64 samples, FP64; c4: rlm n_phi=64..512 g=16 x 16 samples of seed 11, one "
                          "plan per size, FP64 and mixed; c5: rlm n_phi=3000 g=4 x 4 samples of seed 3")
+    ap.add_argument("--no-symv-timing", action="store_true",
+                    help="n > 512: skip the extra step that times the Hermitian product kernel's launches with HIP "
+                         "events (the roofline then reports the kernel without a measured launch time)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -325,7 +328,7 @@ def main():
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         symv = None
-        if n_phi > 512:
+        if n_phi > 512 and not args.no_symv_timing:
             # one more (untimed) step with HIP events around every launch of the Hermitian
             # product kernel, the n > 512 solver's dominant kernel (one launch per column)
             from rlm_amd import symv_timing
@@ -383,19 +386,20 @@ def main():
         tri = sum((n_phi - 1 - c) * (n_phi - c) // 2 for c in range(cols))
         bytes_per_launch = nsolved * tri * esz / cols
         sv = main["symv"]
-        t_launch = sv["total_ms"] / sv["launches"] / 1e3
+        t_launch = sv["total_ms"] / sv["launches"] / 1e3 if sv else None
         kname = "lg_symv_lower_kernel"
+        achieved = bytes_per_launch / t_launch / 1e9 if t_launch else None
         roof = {
             "kernel": kname,
             "bound": "hbm",
-            "achieved": bytes_per_launch / t_launch / 1e9,
+            "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": bytes_per_launch / t_launch / 1e9 / HBM_PEAK_GBS,
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": _load_traffic(workload_key, f"{kname}<{ctype}>"),
             "bytes_per_launch": bytes_per_launch,
-            "launch_ms": t_launch * 1e3,
-            "launches_per_step": sv["launches"],
+            "launch_ms": t_launch * 1e3 if t_launch else None,
+            "launches_per_step": sv["launches"] if sv else n_phi - 1,
         }
     else:
         dom = max(flops, key=lambda k: main["stage"][k])
