@@ -241,9 +241,31 @@ def main():
     ap.add_argument("--no-symv-timing", action="store_true",
                     help="n > 512: skip the extra step that times the Hermitian product kernel's launches with HIP "
                          "events (the roofline then reports the kernel without a measured launch time)")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="only measure plain streaming rates of this GPU (torch.sum over 4 GiB, a 4 GiB device "
+                         "copy): the practical ceiling beside the published HBM peak")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+
+    if args.stream_ceiling:
+        import torch
+        x = torch.ones(1 << 30, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        out = {}
+        for name, fn, nbytes in (("sum_read_gbs", lambda: x.sum(), 4 << 30),
+                                 ("copy_read_write_gbs", lambda: y.copy_(x), 8 << 30)):
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(10):
+                fn()
+            torch.cuda.synchronize()
+            out[name] = nbytes / ((time.perf_counter() - t0) / 10) / 1e9
+        out["published_hbm_gbs"] = HBM_PEAK_GBS
+        print(json.dumps(out))
+        return
 
     if args.copy_calibration_mib:
         import torch
