@@ -156,7 +156,11 @@ int hip_prepare_potential_coeff(const double *d_x, const double *d_y, const doub
  * (hamiltonian_rlm_cpu.cpp:14-48): d_wfs receives the full Hermitian n_phi x
  * n_phi matrix, column-major, exactly the matrix LAPACK sees in the CPU build
  * (A(r,c) = conj(h_rc) for c >= r). Rows of coeff_m_theta are the theta_1 and
- * theta_2 rows; d_coeff_jm is the n_phi x dim_n table. */
+ * theta_2 rows; d_coeff_jm is the n_phi x dim_n table. Swapped into
+ * rlm/main_rlm_gpu.cu as is, it builds the CPU build's matrix, not the GPU kernel's
+ * (conj(c1[n]) c2[m] with e^{+i phase} potentials, hamiltonian_rlm_gpu.cu:29-34,43 and
+ * potential_coeff.cu:14-15); conjugated theta rows and negated impurity positions give
+ * the GPU build's matrix exactly (the equivalence HIP_DIAG_CONV_GPU_FP32 uses). */
 int hip_set_hamil_matrix(const double *d_coeff_m_theta_1, const double *d_coeff_m_theta_2, const double *d_coeff_jm,
                          const double *d_Vmn, double *d_wfs, int n_phi, int dim_n, int off_head);
 
