@@ -160,9 +160,32 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
         const int r = c + t + q * NT;
         a[q] = r < n ? ldx(A, (long)c * n + r) : C{(T)0, (T)0};
     }
-    // the pending update, one panel column at a time over all rows of the thread (the
-    // 2 kLgRPT loads of a step are independent: latency overlaps)
-    for (int l = 0; l < i; l++) {
+    // the pending update over all rows of the thread, UL panel columns per step: the
+    // 2 UL kLgRPT loads of a step are independent, so their latencies overlap (the products
+    // are applied in panel-column order, as one column at a time would); complex double
+    // keeps one column per step (two would not fit 256 registers)
+    constexpr int UL = sizeof(T) == sizeof(float) ? 2 : 1;
+    int l = 0;
+    for (; l + UL <= i; l += UL) {
+        C xv[UL][kLgRPT], xw[UL][kLgRPT];
+#pragma unroll
+        for (int u = 0; u < UL; u++) {
+            const C *vl = Vp + (long)(l + u) * n, *wl = Wp + (long)(l + u) * n;
+#pragma unroll
+            for (int q = 0; q < kLgRPT; q++) {
+                const int r = c + t + q * NT;
+                xv[u][q] = r < n ? vl[r] : C{(T)0, (T)0};
+                xw[u][q] = r < n ? wl[r] : C{(T)0, (T)0};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UL; u++) {
+            const C wc = conjx(rowc[l + u]), vc = conjx(rowc[kLgNB + l + u]);
+#pragma unroll
+            for (int q = 0; q < kLgRPT; q++) a[q] = a[q] - (xv[u][q] * wc + xw[u][q] * vc);
+        }
+    }
+    for (; l < i; l++) {
         const C wc = conjx(rowc[l]), vc = conjx(rowc[kLgNB + l]);
         const C *vl = Vp + (long)l * n, *wl = Wp + (long)l * n;
 #pragma unroll
