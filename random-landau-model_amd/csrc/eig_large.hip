@@ -9,12 +9,15 @@
 //   lower triangle is read and updated):
 //     lg_col_kernel    one workgroup per matrix and column: finishes w of the
 //                      previous column (w = tau y - 1/2 tau (w^H v) v), applies
-//                      the panel's pending update to column c, builds reflector c
-//                      (zlarfg) and the dot products W^H v, V^H v;
+//                      the newest term of the panel's pending update to column c
+//                      and builds reflector c (zlarfg);
 //     lg_symv_lower_kernel + lg_symv_reduce_kernel
 //                      y = A22 v - V (W^H v) - W (V^H v) from the lower triangle
 //                      in 32 x 32 tiles (row and column sums from one read): the
-//                      trailing lower triangle is streamed once per column;
+//                      trailing lower triangle is streamed once per column; one
+//                      extra workgroup per matrix forms W^H v, V^H v beside the
+//                      tiles, and the reduction's idle half-wave applies the older
+//                      pending terms to the next column, strip by strip;
 //     lg_her2k_kernel  after each panel A22 -= V W^H + W V^H on the lower tiles, on
 //                      the matrix cores.
 //   Divide & conquer: stedc_kernel with its node arrays in global memory.
@@ -108,14 +111,11 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
     __shared__ T red[2 * NW];
     __shared__ C rowc[2 * kLgNB];
     __shared__ T sc[8];
-    extern __shared__ double lds_col[];
-    C *vsh = reinterpret_cast<C *>(lds_col);   // v(c+1 .. n-1)
     const long b = blockIdx.x;
-    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    const int t = threadIdx.x;
     T *A = Abase + 2 * b * strideA;
     C *Vp = reinterpret_cast<C *>(aux + 2 * b * strideAux);
     C *Wp = Vp + (long)kLgNB * n;
-    C *P = Wp + (long)kLgNB * n;
     double *Dv = Dout + b * strideV, *Ev = Eout + b * strideV;
     T *Tv = TAUout + b * strideV * (long)(sizeof(double) / sizeof(T));
     const int c = k0 + i;
@@ -147,8 +147,12 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
     const bool refl = i < nbk && c < n - 1;
     const bool last = c == n - 1;
     if (!refl && !last) return;
-    // (b) column c (rows >= c) with the panel's pending rank-2i update
-    if (t < i) {
+    // (b) column c (rows >= c) with the panel's pending rank-2i update: inside a panel
+    // (0 < i < nbk) lg_symv_reduce_kernel of column c - 1 has applied the terms l < i - 1
+    // to A(c:n, c) already, so only l = i - 1 remains; the column after the panel's last
+    // (i == nbk, the last column of the matrix) takes them all
+    const int lfirst = (i > 0 && i < nbk) ? i - 1 : 0;
+    if (t >= lfirst && t < i) {
         rowc[t] = Wp[(long)t * n + c];
         rowc[kLgNB + t] = Vp[(long)t * n + c];
     }
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
     // are applied in panel-column order, as one column at a time would); complex double
     // keeps one column per step (two would not fit 256 registers)
     constexpr int UL = sizeof(T) == sizeof(float) ? 2 : 1;
-    int l = 0;
+    int l = lfirst;
     for (; l + UL <= i; l += UL) {
         C xv[UL][kLgRPT], xw[UL][kLgRPT];
 #pragma unroll
@@ -238,7 +242,7 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
     }
     __syncthreads();
     const C scal{sc[5], sc[6]};
-    // (c) v: reflector storage in A, panel column i of Vp, LDS copy; rows k0..c of
+    // (c) v: reflector storage in A, panel column i of Vp; rows k0..c of
     //     Vp(:, i) and Wp(:, i) cleared (the product fills Wp below row c)
 #pragma unroll
     for (int q = 0; q < kLgRPT; q++) {
@@ -253,7 +257,6 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
                 stx(A, (long)c * n + r, v);
             }
             Vp[(long)i * n + r] = v;
-            if (r > c) vsh[r - c - 1] = v;
         }
     }
     for (int r = k0 + t; r <= c; r += NT) {
@@ -261,25 +264,8 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
         Wp[(long)i * n + r] = C{(T)0, (T)0};
     }
     __syncthreads();
-    // (d) P[l] = W(:, l)^H v, P[kLgNB + l] = V(:, l)^H v over rows > c, one wave per product
-    const int m = n - c - 1;
-    for (int qd = wid; qd < 2 * i; qd += NW) {
-        const C *col = ((qd < i) ? Wp + (long)qd * n : Vp + (long)(qd - i) * n) + c + 1;
-        C s{(T)0, (T)0};
-        int e = lane;
-        // eight loads in flight per step, accumulated in the same order as one at a time
-        for (; e + 7 * kWave < m; e += 8 * kWave) {
-            C x[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) x[k] = col[e + k * kWave];
-#pragma unroll
-            for (int k = 0; k < 8; k++) fmacx(s, x[k], vsh[e + k * kWave]);
-        }
-        for (; e < m; e += kWave) fmacx(s, col[e], vsh[e]);
-        s.x = wave_sum_t(s.x);
-        s.y = wave_sum_t(s.y);
-        if (lane == 0) P[qd < i ? qd : kLgNB + (qd - i)] = s;
-    }
+    // (the dot products W^H v, V^H v of this reflector are formed by the first workgroups
+    // of lg_symv_lower_kernel, beside the Hermitian product)
 }
 
 // ----------------------------------------------------------------------------
@@ -296,14 +282,43 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
 template <typename T>
 __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict__ Abase, long strideA, int n,
                                                             T *__restrict__ aux, long strideAux, int k0, int i,
-                                                            int npair, int ns0) {
+                                                            int npair, int ns0, int nb) {
     using C = cx<T>;
     extern __shared__ double lds_svl[];
     C *vs = reinterpret_cast<C *>(lds_svl);                  // v(s .. n-1)
     __shared__ C rpart[4][kLgTS];
-    const long b = blockIdx.x / npair;
-    const int p = (int)(blockIdx.x % npair);
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    if (blockIdx.x < (unsigned)nb) {
+        // workgroup b < nb: P[l] = W(:, l)^H v, P[kLgNB + l] = V(:, l)^H v over the rows > c
+        // of matrix b, one wave per product (dispatched first, they run beside the tiles)
+        const long bm = blockIdx.x;
+        const int c = k0 + i, m = n - c - 1;
+        C *Vq = reinterpret_cast<C *>(aux + 2 * bm * strideAux);
+        C *Wq = Vq + (long)kLgNB * n;
+        C *P = Wq + (long)kLgNB * n;
+        for (int e = t; e < m; e += 256) vs[e] = Vq[(long)i * n + c + 1 + e];
+        __syncthreads();
+        for (int qd = w; qd < 2 * i; qd += 4) {
+            const C *col = ((qd < i) ? Wq + (long)qd * n : Vq + (long)(qd - i) * n) + c + 1;
+            C s{(T)0, (T)0};
+            int e = lane;
+            // eight loads in flight per step, accumulated in the same order as one at a time
+            for (; e + 7 * kWave < m; e += 8 * kWave) {
+                C x[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) x[k] = col[e + k * kWave];
+#pragma unroll
+                for (int k = 0; k < 8; k++) fmacx(s, x[k], vs[e + k * kWave]);
+            }
+            for (; e < m; e += kWave) fmacx(s, col[e], vs[e]);
+            s.x = wave_sum_t(s.x);
+            s.y = wave_sum_t(s.y);
+            if (lane == 0) P[qd < i ? qd : kLgNB + (qd - i)] = s;
+        }
+        return;
+    }
+    const long b = (blockIdx.x - nb) / npair;
+    const int p = (int)((blockIdx.x - nb) % npair);
     const int lrg = lane & 7, lcg = lane >> 3;
     const int c = k0 + i, s0 = c + 1, m = n - s0;
     const int ns = (m + kLgTS - 1) / kLgTS;
@@ -381,9 +396,13 @@ __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict_
 
 // y(r) = Rs[K][r'] + sum_{I >= K} Cs[I][K][r'] - V (W^H v) - W (V^H v) for the 32 rows of
 // strip K, into Wp(:, i).
+// Threads 32..63 meanwhile apply to column c + 1 (rows of the strip, all >= c + 1) the
+// terms l < i of the panel's pending update, when column c + 1 is in the same panel
+// (i + 1 < nbk): A(r, c+1) -= V(r, l) conj(W(c+1, l)) + W(r, l) conj(V(c+1, l)), in the
+// order lg_col_kernel would apply them; lg_col_kernel then adds only the term l = i.
 template <typename T>
-__global__ __launch_bounds__(64) void lg_symv_reduce_kernel(T *__restrict__ aux, long strideAux, int n, int k0, int i,
-                                                            int ns0) {
+__global__ __launch_bounds__(64) void lg_symv_reduce_kernel(T *__restrict__ Abase, long strideA, T *__restrict__ aux,
+                                                            long strideAux, int n, int k0, int i, int nbk, int ns0) {
     using C = cx<T>;
     const int c = k0 + i, s0 = c + 1, m = n - s0;
     const int ns = (m + kLgTS - 1) / kLgTS;
@@ -395,8 +414,19 @@ __global__ __launch_bounds__(64) void lg_symv_reduce_kernel(T *__restrict__ aux,
     const C *P = Wp + (long)kLgNB * n;
     const C *Rs = Vp + 2L * kLgNB * n + 2L * kLgNB;
     const C *Cs = Rs + (long)ns0 * kLgTS;
+    if (t >= kLgTS) {
+        const int lr = K * kLgTS + (t - kLgTS);
+        if (i + 1 >= nbk || i == 0 || lr >= m) return;
+        const int r = s0 + lr, cn = c + 1;
+        T *A = Abase + 2 * b * strideA;
+        C x = ldx(A, (long)cn * n + r);
+        for (int l = 0; l < i; l++)
+            x = x - (Vp[(long)l * n + r] * conjx(Wp[(long)l * n + cn]) + Wp[(long)l * n + r] * conjx(Vp[(long)l * n + cn]));
+        stx(A, (long)cn * n + r, x);
+        return;
+    }
     const int lr = K * kLgTS + t;
-    if (t >= kLgTS || lr >= m) return;
+    if (lr >= m) return;
     // loads issued eight ahead of the fixed-order sums
     C y = Rs[(long)K * kLgTS + t];
     int I = K;
@@ -761,18 +791,15 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
     const long strideA = (long)n * n;
     const long strideAux = heevd_large_aux_complex(n);
     const int nb = (int)batch;
-    const size_t lds_col = sizeof(C) * n;
-    if (lds_col > 32 * 1024) {                               // dynamic + static beyond the 64 KB default
-        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)lg_col_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)lds_col));
+    const size_t lds_v = sizeof(C) * n;                     // the Hermitian product's copy of v
+    if (lds_v > 32 * 1024)                                   // dynamic + static beyond the 64 KB default
         RLM_HIP_CHECK(hipFuncSetAttribute((const void *)lg_symv_lower_kernel<T>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_col));
-    }
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_v));
     const int ns0 = (int)lg_strips(n);
     for (int k0 = 0; k0 < n - 1; k0 += kLgNB) {
         const int nbk = std::min(kLgNB, n - 1 - k0);
         for (int i = 0; i <= nbk; i++) {
-            hipLaunchKernelGGL(lg_col_kernel<T>, dim3(nb), dim3(kLgColNT), lds_col, s, A, strideA, n, D, E, TAU,
+            hipLaunchKernelGGL(lg_col_kernel<T>, dim3(nb), dim3(kLgColNT), 0, s, A, strideA, n, D, E, TAU,
                                strideV, aux, strideAux, k0, i, nbk);
             if (i == nbk) break;
             const int m = n - (k0 + i) - 1;
@@ -780,11 +807,11 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
             hipEvent_t te0, te1;
             kt_pair(s, te0, te1);
             if (te0) RLM_HIP_CHECK(hipEventRecord(te0, s));
-            hipLaunchKernelGGL(lg_symv_lower_kernel<T>, dim3((unsigned)((long)nb * npair)), dim3(256), sizeof(C) * m,
-                               s, A, strideA, n, aux, strideAux, k0, i, npair, ns0);
+            hipLaunchKernelGGL(lg_symv_lower_kernel<T>, dim3((unsigned)((long)nb * npair + nb)), dim3(256),
+                               sizeof(C) * m, s, A, strideA, n, aux, strideAux, k0, i, npair, ns0, nb);
             if (te1) RLM_HIP_CHECK(hipEventRecord(te1, s));
-            hipLaunchKernelGGL(lg_symv_reduce_kernel<T>, dim3((unsigned)((long)nb * nsm)), dim3(64), 0, s, aux,
-                               strideAux, n, k0, i, ns0);
+            hipLaunchKernelGGL(lg_symv_reduce_kernel<T>, dim3((unsigned)((long)nb * nsm)), dim3(64), 0, s, A, strideA,
+                               aux, strideAux, n, k0, i, nbk, ns0);
         }
         const int s0 = k0 + nbk;
         if (s0 < n - 1) {
