@@ -1899,16 +1899,30 @@ __device__ __forceinline__ void merge_gemm_mfma(int lane, int G, int lo, int mid
 // pointer swap) and eigenvectors of T in the returned buffer (row-major n x n).
 // QS: storage type of the eigenvector matrices Qa, Qb and of S (see merge_gemm_mfma);
 // every other quantity is FP64.
-// split_top: stop at the top merge once its secular roots, weights, norms and ranks are
-// known (nullptr returned, the scale left in sh.red[0]); its eigenvector GEMM then runs on
-// several workgroups per matrix (stedc_top_gemm_kernel), which writes the outputs.
+// lev_stop > 0: stop at merge level lev_stop once its secular roots, weights, norms and
+// ranks are known (nullptr returned, the scale left in sh.red[0]); its eigenvector GEMM
+// then runs on several workgroups per matrix (stedc_level_gemm_kernel, or
+// stedc_top_gemm_kernel for the top level, which writes the outputs).
+// lev_begin > 1: resume at that level from the state a stopped call and the GEMM kernel
+// left in sh, Qa / Qb (the caller passes the workspace's first and second matrix; the
+// buffers and eigenvalue arrays swap after every level) and the scale in sh.red[0].
 // Uout (optional): where the top merge takes the matrix-core path, its products and
 // deflated columns go straight to the component-major complex output Uout[r * n + j]
 // (*direct set), which saves a transposing pass over the eigenvectors.
 template <int NT, typename QS>
 __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, QS *Qb, QS *S, DcShared &sh,
-                           double **dq_final, bool split_top = false, QS *Uout = nullptr, bool *direct = nullptr) {
+                           double **dq_final, int lev_stop = 0, QS *Uout = nullptr, bool *direct = nullptr,
+                           int lev_begin = 1) {
     const int t = threadIdx.x;
+    double *dq = sh.dq, *dqn = sh.dqn;
+    double scale;
+    if (lev_begin > 1) {
+        scale = sh.red[0];
+        if (!(lev_begin & 1)) {
+            QS *tq = Qa; Qa = Qb; Qb = tq;
+            double *td = dq; dq = dqn; dqn = td;
+        }
+    } else {
     // scale T to unit max-norm (as LAPACK dstedc does) so that the deflation
     // tolerance is relative to the matrix, whatever its magnitude
     double amax = 0.0;
@@ -1922,13 +1936,13 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
     amax = 0.0;
     for (int w = 0; w < NT / kWave; w++) amax = fmax(amax, sh.red[w]);
     __syncthreads();
-    const double scale = (amax > 0.0) ? amax : 1.0;
-    const double iscale = 1.0 / scale;
+    scale = (amax > 0.0) ? amax : 1.0;
+    const double is0 = 1.0 / scale;
     // init: torn diagonal (scaled), identity / zero
     for (int k = t; k < n; k += NT) {
-        double v = dIn[k] * iscale;
-        if (k > 0) v -= fabs(eIn[k - 1] * iscale);
-        if (k < n - 1) v -= fabs(eIn[k] * iscale);
+        double v = dIn[k] * is0;
+        if (k > 0) v -= fabs(eIn[k - 1] * is0);
+        if (k < n - 1) v -= fabs(eIn[k] * is0);
         sh.dq[k] = v;
     }
     // Qa, Qb: eigenvector matrices of the current level, column-major (column c =
@@ -1939,8 +1953,9 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
         Qb[idx] = (QS)0;
     }
     __syncthreads();
-    double *dq = sh.dq, *dqn = sh.dqn;
-    for (int L = 1; (1 << (L - 1)) < n; L++) {
+    }
+    const double iscale = 1.0 / scale;
+    for (int L = lev_begin; (1 << (L - 1)) < n; L++) {
         const int half = 1 << (L - 1), blk = half << 1;
         const int nnodes = (n + blk - 1) / blk;
         int G = 1;
@@ -2213,7 +2228,7 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                 }
             }
             __syncthreads();
-            if (split_top && blk >= n) {            // the top merge (one node, uniform exit)
+            if (L == lev_stop) {                    // the level's GEMM runs elsewhere (uniform exit)
                 if (t == 0) sh.red[0] = scale;
                 *dq_final = dqn;
                 return nullptr;
@@ -2346,14 +2361,14 @@ __device__ inline void dc_transpose_out(int n, const T *Z, T *U, double *tile) {
 // the tridiagonal to U (component-major complex, zero imaginary part).
 // GSH: the per-node arrays live in the workgroup's global workspace after Qa, Qb, S
 // (n > 512, where they exceed the LDS); the LDS then holds only the transpose tile.
-// split_top (GSH only): stop before the top merge's GEMM (see stedc_block); workgroup b
-// handles matrix b (grid = batch) and keeps its state in workspace slot b.
+// lev_stop > 0 (GSH only): stop before merge level lev_stop's GEMM (see stedc_block);
+// workgroup b then handles matrix b (grid = batch) and keeps its state in workspace slot b.
 template <typename T, int NT, bool GSH>
 __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restrict__ Din,
                                                    const double *__restrict__ Ein, long strideV,
                                                    double *__restrict__ Wout, long strideW, T *__restrict__ Uout,
                                                    long strideU, double *__restrict__ work, int batch,
-                                                   int split_top) {
+                                                   int lev_stop) {
     extern __shared__ double smem[];
     const int t = threadIdx.x;
     // per-workgroup global workspace: Qa, Qb, S (n*n double slots each, holding the
@@ -2372,7 +2387,7 @@ __global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restri
         T *U = Uout + 2 * b * strideU;
         double *dqf;
         bool direct = false;
-        T *Z = stedc_block<NT, T>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf, GSH && split_top, U, &direct);
+        T *Z = stedc_block<NT, T>(n, dIn, eIn, Qa0, Qb0, S, sh, &dqf, GSH ? lev_stop : 0, U, &direct);
         if (!Z) return;                    // split: the top merge continues in other kernels
         for (int k = t; k < n; k += NT) W[k] = dqf[k];
         __syncthreads();
@@ -2388,6 +2403,64 @@ __device__ inline int dc_top_level(int n) {
     int L = 1;
     while ((1 << L) < n) L++;
     return L;
+}
+
+// A stopped divide & conquer resumed at merge level lev (its secular roots, weights,
+// norms and ranks) after stedc_level_gemm_kernel of level lev - 1; stops again before the
+// level's GEMM. One workgroup per matrix of the round (workspace slot = blockIdx.x).
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void stedc_resume_kernel(int n, const double *__restrict__ Din,
+                                                          const double *__restrict__ Ein, long strideV,
+                                                          double *__restrict__ work, int lev) {
+    const long b = blockIdx.x;
+    double *wg = work + b * stedc_global_work_doubles(n);
+    DcShared sh;
+    dc_shared_at(wg + 3L * n * n, n, sh);
+    T *Qa0 = reinterpret_cast<T *>(wg), *Qb0 = reinterpret_cast<T *>(wg + (long)n * n);
+    T *S = reinterpret_cast<T *>(wg + 2L * n * n);
+    double *dqf;
+    (void)stedc_block<NT, T>(n, Din + b * strideV, Ein + b * strideV, Qa0, Qb0, S, sh, &dqf, lev, nullptr, nullptr,
+                             lev);
+}
+
+// The eigenvector products Qb = Qa S of every node of merge level L (below the top), nwg
+// workgroups per matrix (each node's wave tasks dealt over all of them), with the
+// nodes' deflated columns, and the copies of the nodes that do not merge at this level.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void stedc_level_gemm_kernel(int n, double *__restrict__ work, int nwg, int L) {
+    const long b = blockIdx.x / nwg;
+    const int part = blockIdx.x % nwg;
+    double *wg = work + b * stedc_global_work_doubles(n);
+    DcShared sh;
+    dc_shared_at(wg + 3L * n * n, n, sh);
+    T *Qa0 = reinterpret_cast<T *>(wg), *Qb0 = reinterpret_cast<T *>(wg + (long)n * n);
+    const T *Qa = (L & 1) ? Qa0 : Qb0;
+    T *Qb = (L & 1) ? Qb0 : Qa0;
+    const double *dq = (L & 1) ? sh.dq : sh.dqn;
+    double *dqn = (L & 1) ? sh.dqn : sh.dq;
+    const int half = 1 << (L - 1), blk = half << 1, nnodes = (n + blk - 1) / blk;
+    const int lane = part * NT + threadIdx.x, G = nwg * NT;
+    for (int k = 0; k < nnodes; k++) {
+        const int lo = k * blk, mid = lo + half, hi = min(lo + blk, n), m = hi - lo;
+        if (mid < n) {
+            const int K = sh.cnt[5 * k], nd = sh.cnt[5 * k + 1], KL = sh.cnt[5 * k + 3], KR = sh.cnt[5 * k + 4];
+            if (sizeof(T) == sizeof(float) && half >= 64)
+                merge_gemm_mfma<T, 4, 2>(lane, G, lo, mid, hi, half, K, KL, KR, n, Qa, Qb, sh);
+            else
+                merge_gemm_mfma<T, 2, 4>(lane, G, lo, mid, hi, half, K, KL, KR, n, Qa, Qb, sh);
+            for (long idx = lane; idx < (long)m * nd; idx += G) {
+                const int qq = (int)(idx / m), rloc = (int)(idx - (long)qq * m);
+                const int r = lo + rloc;
+                Qb[(long)(lo + sh.rank[lo + K + qq]) * n + r] = Qa[(long)(lo + sh.perm[lo + sh.dfx[lo + qq]]) * n + r];
+            }
+        } else {
+            for (long idx = lane; idx < (long)m * m; idx += G) {
+                const long c = lo + idx / m, r = lo + idx % m;
+                Qb[c * n + r] = Qa[c * n + r];
+            }
+            for (int q = lane; q < m; q += G) dqn[lo + q] = dq[lo + q];
+        }
+    }
 }
 
 // The top merge's product of stedc_block, nwg workgroups per matrix (the wave tasks of
@@ -3362,18 +3435,30 @@ static int launch_stedc_global_t(int n, const double *D, const double *E, long s
         RLM_HIP_CHECK(hipGetLastError());
         return 0;
     }
-    // large n: the top merge's GEMM (about 70 % of the divide & conquer's matrix-core
-    // work) on several workgroups per matrix instead of one, in rounds of `grid` matrices
-    // (one workspace slot each)
+    // large n: the GEMMs of the merge levels with halves of >= 512 rows (the top one
+    // alone is about 70 % of the divide & conquer's matrix-core work) on several
+    // workgroups per matrix instead of one, in rounds of `grid` matrices (one workspace
+    // slot each): the single-workgroup kernel runs the levels below and the secular phase
+    // of the first split level, then GEMM and resume kernels alternate up to the top
+    int top = 1;
+    while ((1 << top) < n) top++;                  // level whose block covers n
+    constexpr int first = 10;                      // first level with halves of 512 rows
     int dev = 0, cus = 0;
     RLM_HIP_CHECK(hipGetDevice(&dev));
     RLM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     for (int b0 = 0; b0 < batch; b0 += grid) {
         const int cnt = std::min(grid, batch - b0);
         const int nwg = std::max(1, std::min(16, (2 * std::max(cus, 1) + cnt - 1) / cnt));
-        hipLaunchKernelGGL((stedc_kernel<T, NT, true>), dim3(cnt), dim3(NT), lds, stream, n, D + (long)b0 * strideV,
-                           E + (long)b0 * strideV, strideV, W + (long)b0 * strideW, strideW, U + 2L * b0 * strideU,
-                           strideU, work, cnt, 1);
+        const double *Db = D + (long)b0 * strideV, *Eb = E + (long)b0 * strideV;
+        const int lev0 = std::min(first, top);
+        hipLaunchKernelGGL((stedc_kernel<T, NT, true>), dim3(cnt), dim3(NT), lds, stream, n, Db, Eb, strideV,
+                           W + (long)b0 * strideW, strideW, U + 2L * b0 * strideU, strideU, work, cnt, lev0);
+        for (int L = lev0; L < top; L++) {
+            hipLaunchKernelGGL((stedc_level_gemm_kernel<T, NT>), dim3((unsigned)((long)cnt * nwg)), dim3(NT), 0,
+                               stream, n, work, nwg, L);
+            hipLaunchKernelGGL((stedc_resume_kernel<T, NT>), dim3(cnt), dim3(NT), 0, stream, n, Db, Eb, strideV, work,
+                               L + 1);
+        }
         hipLaunchKernelGGL((stedc_top_gemm_kernel<T, NT>), dim3((unsigned)((long)cnt * nwg)), dim3(NT), 0, stream, n,
                            work, nwg, W + (long)b0 * strideW, strideW, U + 2L * b0 * strideU, strideU);
         RLM_HIP_CHECK(hipGetLastError());
