@@ -2228,11 +2228,7 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                 }
             }
             __syncthreads();
-            if (L == lev_stop) {                    // the level's GEMM runs elsewhere (uniform exit)
-                if (t == 0) sh.red[0] = scale;
-                *dq_final = dqn;
-                return nullptr;
-            }
+            if (L == lev_stop) continue;            // the level's GEMM runs elsewhere (uniform)
             // Matrix-core path for halves of >= 32 rows (whole waves per node): Qb = Qa S
             // with S[i][q] = what_i / (dl_i - lambda_q) * inrm_q generated per K-step,
             // S never stored. Below that, S goes through memory and the VALU path.
@@ -2308,6 +2304,11 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
             }
             if (top_direct) *direct = true;
             __syncthreads();
+        }
+        if (L == lev_stop) {                        // every round's secular phase done: stop
+            if (t == 0) sh.red[0] = scale;
+            *dq_final = dqn;
+            return nullptr;
         }
         QS *tq = Qa; Qa = Qb; Qb = tq;
         double *td = dq; dq = dqn; dqn = td;
@@ -3435,14 +3436,17 @@ static int launch_stedc_global_t(int n, const double *D, const double *E, long s
         RLM_HIP_CHECK(hipGetLastError());
         return 0;
     }
-    // large n: the GEMMs of the merge levels with halves of >= 512 rows (the top one
+    // large n: the GEMMs of the merge levels with halves of >= 256 rows (the top one
     // alone is about 70 % of the divide & conquer's matrix-core work) on several
     // workgroups per matrix instead of one, in rounds of `grid` matrices (one workspace
     // slot each): the single-workgroup kernel runs the levels below and the secular phase
     // of the first split level, then GEMM and resume kernels alternate up to the top
     int top = 1;
     while ((1 << top) < n) top++;                  // level whose block covers n
-    constexpr int first = 10;                      // first level with halves of 512 rows
+    // first split level: halves of 256 rows (n_phi = 3000: 78 -> 72 ms against splitting
+    // from 512 rows on, 71 ms from 128; RLM_DC_SPLIT_LEVEL overrides, >= 6)
+    int first = 9;
+    if (const char *e = getenv("RLM_DC_SPLIT_LEVEL")) first = std::max(6, atoi(e));
     int dev = 0, cus = 0;
     RLM_HIP_CHECK(hipGetDevice(&dev));
     RLM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

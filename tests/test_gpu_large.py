@@ -127,6 +127,22 @@ def test_rlm_n3000_g4_mixed_vs_fp64():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_split_levels_many_nodes(monkeypatch, precision):
+    """n = 1030 with every merge level from halves of 32 rows up split across workgroups
+    (RLM_DC_SPLIT_LEVEL=6: 17 nodes at the first split level, several rounds of nodes
+    per workgroup before it stops) against LAPACK."""
+    import torch
+    from rlm_amd import heevd_batched, HIP_DIAG_FP64, HIP_DIAG_MIXED
+    print(f"[test_split_levels_many_nodes] {precision}", flush=True)
+    monkeypatch.setenv("RLM_DC_SPLIT_LEVEL", "6")
+    rng = np.random.default_rng(1030)
+    H = _rand_herm(rng, 2, 1030)
+    prec = HIP_DIAG_MIXED if precision == "mixed" else HIP_DIAG_FP64
+    w, U = heevd_batched(torch.from_numpy(H).cuda(), precision=prec)
+    (_check_mixed if precision == "mixed" else _check)(H, w.cpu().numpy(), U.cpu().numpy())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_n3000_random_hermitian(precision):
     """One random 3000 x 3000 Hermitian matrix (the C5 size) against LAPACK."""
     import torch
