@@ -22,8 +22,8 @@
 //                      the matrix cores.
 //   Divide & conquer: stedc_kernel with its node arrays in global memory.
 //   Back-transform: larft_kernel with several workgroups per matrix, then
-//     lg_backtransform_kernel: a slab of 32 eigenvector columns per workgroup,
-//     one read and one write of the slab per block of 16 reflectors.
+//     lg_backtransform2_kernel: a slab of 32 eigenvector columns per workgroup,
+//     one read and one write of the slab per pair of 16-reflector blocks.
 #include "common.hpp"
 #include "eig.hpp"
 
@@ -563,10 +563,7 @@ __global__ __launch_bounds__(256) void lg_her2k_kernel(T *__restrict__ Abase, lo
 // Back-transform U = Q U, Q = H_0 H_1 ... H_{n-2}: blocks of 16 reflectors
 // I - V T V^H (T from larft_kernel), last block first. A workgroup owns a slab of
 // 16 NCB eigenvector columns of one matrix; wave w owns the 16-row blocks w, w+4,
-// ... of it in every phase, so U is only ever re-read by the wave that wrote it.
-// Per block: the wave partials of W = V^H U are summed in LDS (fixed order), W2 =
-// T W, then one pass over the wave's row blocks updates U -= V W2 and, from the
-// updated rows still in registers, accumulates the next block's W partial. The
+// ... of it in every phase, so U is only ever re-read by the wave that wrote it. The
 // slabs of one matrix share blockIdx % 8 (one XCD, whose L2 serves V and T).
 // Element (row, k) of reflector block j0 (unit at row j+1, A(j+2:n, j) below).
 template <typename T>
@@ -577,19 +574,30 @@ __device__ inline cx<T> lg_refl(const T *A, int n, int j0, int row, int k) {
     return ldx(A, (long)j * n + row);
 }
 
-// Two waves per SIMD: the register allocation then stays within 256 (165 for complex
-// float instead of 220 + 52 accumulation registers), and the second workgroup per CU hides
-// the per-block barrier and LDS sum (n_phi = 3000, 64 matrices: 368 -> 300 ms complex float,
-// 562 -> 557 ms complex double).
+// Two reflector blocks per pass (blocks B0 = bk and B1 = bk - 1, applied in that order):
+// the slab's U rows are read and written once per 32 reflectors instead of once per 16.
+// Entering a pass, the waves hold partials of W0 = V_B0^H U, W1' = V_B1^H U and
+// C2 = V_B0^H V_B1 (all from the U before B0); summed over the waves in LDS (fixed
+// order) they give W2_0 = T_B0 W0, and W1 = W1' - C W2_0 with C = C2^H (the W of B1 after
+// B0 is applied), W2_1 = T_B1 W1. The pass then applies U -= V_B0 W2_0 + V_B1 W2_1 row
+// block by row block and accumulates the next pair's W0, W1' and C2 from the updated
+// rows. Complex float prefetches the next row block's U rows while the current one runs
+// on the matrix cores (prefetching its V operands too, or the U rows in complex double,
+// would not fit 256 registers). Two waves per SIMD (at most 256 registers): the second
+// workgroup per CU hides the per-pair barriers and LDS sums. n_phi = 3000, 64 matrices:
+// one block per pass took 368 ms (complex float, one wave per SIMD) / 300 ms (two) and
+// 527 ms (complex double); two blocks per pass take 247 / 425 ms (the U stream is the
+// bound: skipping the stores alone saved 138 ms).
 template <typename T, int NCB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void lg_backtransform_kernel(const T *__restrict__ Abase, long strideA, int n,
-                                                               const T *__restrict__ Tin, long strideT,
-                                                               T *__restrict__ Uout, long strideU, int nslab,
-                                                               int batch) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void lg_backtransform2_kernel(
+        const T *__restrict__ Abase, long strideA, int n, const T *__restrict__ Tin, long strideT,
+        T *__restrict__ Uout, long strideU, int nslab, int batch) {
     using M = Mf<T>;
     using v4 = typename M::v4;
     using CT = cx<T>;
-    __shared__ T wred[2][4][2 * NCB * 4 * kWave];
+    constexpr int WS = 2 * NCB * 4 * kWave;               // one W partial (re, im) of a wave
+    constexpr bool PREFU = sizeof(T) == sizeof(float);
+    __shared__ T wred[4][2 * WS + 2 * 4 * kWave];          // W0, W1', C2 partials per wave
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int li = lane & 15;
@@ -602,8 +610,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
     T *U = Uout + 2 * b * strideU;
     const int c0 = slab * 16 * NCB;
     const int nblk = (n - 1 + 15) / 16;
-    v4 ur[NCB], ui[NCB];                                     // one 16-row block of the slab
-    auto load_u = [&](int R) {
+    // V operands of block bk for the row block R: apply form V[R + li][k = row(l, s)],
+    // accumulate form V[R + row(l, v)][k = li]; zero for a block index < 0
+    auto vapply = [&](int bk, int R, CT (&x)[4]) {
+#pragma unroll
+        for (int s2 = 0; s2 < 4; s2++)
+            x[s2] = bk >= 0 ? lg_refl(A, n, 16 * bk, R + li, M::row(lane, s2)) : CT{(T)0, (T)0};
+    };
+    auto vacc = [&](int bk, int R, CT (&x)[4]) {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            x[v] = bk >= 0 ? lg_refl(A, n, 16 * bk, R + M::row(lane, v), li) : CT{(T)0, (T)0};
+    };
+    auto loadu = [&](int R, v4 (&ur)[NCB], v4 (&ui)[NCB]) {
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
             const int col = c0 + 16 * cb + li;
@@ -616,153 +635,209 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
             }
         }
     };
-    // Complex products from three real MFMAs: a b with P1 = (a_r + a_i) b_r,
-    // P2 = a_r (b_i - b_r), P3 = a_i (b_r + b_i), Re = P1 - P3, Im = P1 + P2 (P1 in its
-    // own accumulator, -P3 and P2 straight into the real / imaginary ones).
-    // W partial of block j0 from the row block R held in ur/ui (a = conj V): K-step v
-    // stands for rows R + row(lane, v) (the rows of accumulator element v); W = q1 + (wr, wi)
-    auto acc_w = [&](const CT (&xv)[4], v4 (&wr)[NCB], v4 (&wi)[NCB], v4 (&q1)[NCB]) {
+    // acc += conj(V)^T B (three real products per complex one: W = q + (r, i))
+    auto accw = [&](const CT (&xv)[4], const v4 (&br_)[NCB], const v4 (&bi_)[NCB], v4 (&r)[NCB], v4 (&i)[NCB],
+                    v4 (&qq)[NCB]) {
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const CT x = xv[v];                                        // V[row][k = li]
+            const CT x = xv[v];
             const T xd = x.x - x.y;
 #pragma unroll
             for (int cb = 0; cb < NCB; cb++) {
-                const T br = ur[cb][v], bi = ui[cb][v];
-                q1[cb] = M::mma(xd, br, q1[cb]);
-                wr[cb] = M::mma(x.y, br + bi, wr[cb]);
-                wi[cb] = M::mma(x.x, bi - br, wi[cb]);
+                const T br = br_[cb][v], bi = bi_[cb][v];
+                qq[cb] = M::mma(xd, br, qq[cb]);
+                r[cb] = M::mma(x.y, br + bi, r[cb]);
+                i[cb] = M::mma(x.x, bi - br, i[cb]);
             }
         }
     };
-    v4 wr[NCB], wi[NCB], q1[NCB];
+    auto accc = [&](const CT (&xa)[4], const CT (&xb)[4], v4 &r, v4 &i, v4 &qq) {   // C2 += conj(Va)^T Vb
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const CT x = xa[v];
+            const T xd = x.x - x.y, br = xb[v].x, bi = xb[v].y;
+            qq = M::mma(xd, br, qq);
+            r = M::mma(x.y, br + bi, r);
+            i = M::mma(x.x, bi - br, i);
+        }
+    };
+    const v4 z4 = v4{0, 0, 0, 0};
+    v4 p0r[NCB], p0i[NCB], p0q[NCB], p1r[NCB], p1i[NCB], p1q[NCB];     // W0, W1' partials
+    v4 pcr = z4, pci = z4, pcq = z4;                                    // C2 partial
 #pragma unroll
     for (int cb = 0; cb < NCB; cb++) {
-        wr[cb] = v4{0, 0, 0, 0};
-        wi[cb] = v4{0, 0, 0, 0};
-        q1[cb] = v4{0, 0, 0, 0};
+        p0r[cb] = p0i[cb] = p0q[cb] = z4;
+        p1r[cb] = p1i[cb] = p1q[cb] = z4;
     }
+    // partials of the first pair from the original U
     {
-        const int j0 = 16 * (nblk - 1);
+        const int B0 = nblk - 1, B1 = nblk - 2;
+        const int lo = 16 * (B1 >= 0 ? B1 : B0) + 1;
         for (int rbk = w; 16 * rbk < n; rbk += 4) {
             const int R = 16 * rbk;
-            if (R + 15 < j0 + 1) continue;
-            load_u(R);
-            CT xv[4];
-#pragma unroll
-            for (int v = 0; v < 4; v++) xv[v] = lg_refl(A, n, j0, R + M::row(lane, v), li);
-            acc_w(xv, wr, wi, q1);
+            if (R + 15 < lo) continue;
+            v4 ur[NCB], ui[NCB];
+            loadu(R, ur, ui);
+            CT xa[4], xb[4];
+            vacc(B0, R, xa);
+            vacc(B1, R, xb);
+            accw(xa, ur, ui, p0r, p0i, p0q);
+            if (B1 >= 0) {
+                accw(xb, ur, ui, p1r, p1i, p1q);
+                accc(xa, xb, pcr, pci, pcq);
+            }
         }
     }
-    for (int bk = nblk - 1; bk >= 0; bk--) {
-        const int j0 = 16 * bk;
-        T *wb = wred[bk & 1][w];
+    for (int B0 = nblk - 1; B0 >= 0; B0 -= 2) {
+        const int B1 = B0 - 1, N0 = B0 - 2, N1 = B0 - 3;             // this pair, the next pair
+        T *wb = wred[w];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                wb[((0 * NCB + cb) * 4 + v) * kWave + lane] = q1[cb][v] + wr[cb][v];
-                wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = q1[cb][v] + wi[cb][v];
+                wb[((0 * NCB + cb) * 4 + v) * kWave + lane] = p0q[cb][v] + p0r[cb][v];
+                wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = p0q[cb][v] + p0i[cb][v];
+                wb[WS + ((0 * NCB + cb) * 4 + v) * kWave + lane] = p1q[cb][v] + p1r[cb][v];
+                wb[WS + ((1 * NCB + cb) * 4 + v) * kWave + lane] = p1q[cb][v] + p1i[cb][v];
             }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            wb[2 * WS + v * kWave + lane] = pcq[v] + pcr[v];
+            wb[2 * WS + (4 + v) * kWave + lane] = pcq[v] + pci[v];
+        }
         __syncthreads();
+        v4 w0r[NCB], w0i[NCB], w1r[NCB], w1i[NCB];
+        T c2r[4], c2i[4];
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                T sr = 0, si = 0;
+                T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 #pragma unroll
                 for (int ww = 0; ww < 4; ww++) {
-                    sr += wred[bk & 1][ww][((0 * NCB + cb) * 4 + v) * kWave + lane];
-                    si += wred[bk & 1][ww][((1 * NCB + cb) * 4 + v) * kWave + lane];
+                    a0 += wred[ww][((0 * NCB + cb) * 4 + v) * kWave + lane];
+                    a1 += wred[ww][((1 * NCB + cb) * 4 + v) * kWave + lane];
+                    a2 += wred[ww][WS + ((0 * NCB + cb) * 4 + v) * kWave + lane];
+                    a3 += wred[ww][WS + ((1 * NCB + cb) * 4 + v) * kWave + lane];
                 }
-                wr[cb][v] = sr;
-                wi[cb][v] = si;
+                w0r[cb][v] = a0;
+                w0i[cb][v] = a1;
+                w1r[cb][v] = a2;
+                w1i[cb][v] = a3;
             }
-        // W2 = T W (K index = reflector k2 = row(l, s))
-        v4 w2r[NCB], w2i[NCB];
 #pragma unroll
-        for (int cb = 0; cb < NCB; cb++) {
-            w2r[cb] = v4{0, 0, 0, 0};
-            w2i[cb] = v4{0, 0, 0, 0};
-            q1[cb] = v4{0, 0, 0, 0};
+        for (int v = 0; v < 4; v++) {
+            T a0 = 0, a1 = 0;
+#pragma unroll
+            for (int ww = 0; ww < 4; ww++) {
+                a0 += wred[ww][2 * WS + v * kWave + lane];
+                a1 += wred[ww][2 * WS + (4 + v) * kWave + lane];
+            }
+            c2r[v] = a0;
+            c2i[v] = a1;
         }
+        __syncthreads();                                     // wred is rewritten by the next pair
+        // D = X Y for a 16 x 16 X given as the A operand of each K-step s (xs[s] =
+        // X[li][row(l, s)]) and Y in accumulator layout (lane holds Y[row(l, s)][li])
+        auto mul = [&](const CT (&xs)[4], const v4 (&yr)[NCB], const v4 (&yi)[NCB], v4 (&dr)[NCB], v4 (&di)[NCB]) {
+            v4 qq[NCB];
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const CT tv = Tb[bk * 256 + M::row(lane, s) * 16 + li];   // T[k1 = li][k2 = row(l, s)]
-            const T ts = tv.x + tv.y;
+            for (int cb = 0; cb < NCB; cb++) dr[cb] = di[cb] = qq[cb] = z4;
+#pragma unroll
+            for (int s2 = 0; s2 < 4; s2++) {
+                const CT tv = xs[s2];
+                const T ts = tv.x + tv.y;
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) {
+                    const T br = yr[cb][s2], bi = yi[cb][s2];
+                    qq[cb] = M::mma(ts, br, qq[cb]);
+                    dr[cb] = M::mma(-tv.y, br + bi, dr[cb]);
+                    di[cb] = M::mma(tv.x, bi - br, di[cb]);
+                }
+            }
 #pragma unroll
             for (int cb = 0; cb < NCB; cb++) {
-                const T br = wr[cb][s], bi = wi[cb][s];
-                q1[cb] = M::mma(ts, br, q1[cb]);
-                w2r[cb] = M::mma(-tv.y, br + bi, w2r[cb]);
-                w2i[cb] = M::mma(tv.x, bi - br, w2i[cb]);
+                dr[cb] += qq[cb];
+                di[cb] += qq[cb];
             }
+        };
+        v4 w20r[NCB], w20i[NCB], w21r[NCB], w21i[NCB];
+        {
+            CT t0[4];
+#pragma unroll
+            for (int s2 = 0; s2 < 4; s2++) t0[s2] = Tb[B0 * 256 + M::row(lane, s2) * 16 + li];   // T_B0[li][row]
+            mul(t0, w0r, w0i, w20r, w20i);                                                       // W2_0 = T_B0 W0
+        }
+        if (B1 >= 0) {
+            CT cs[4];
+#pragma unroll
+            for (int s2 = 0; s2 < 4; s2++) cs[s2] = CT{c2r[s2], -c2i[s2]};    // C[li][row(l, s)] = conj C2[row][li]
+            v4 dr[NCB], di[NCB];
+            mul(cs, w20r, w20i, dr, di);
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) {
+                w1r[cb] -= dr[cb];
+                w1i[cb] -= di[cb];
+            }
+            CT t1[4];
+#pragma unroll
+            for (int s2 = 0; s2 < 4; s2++) t1[s2] = Tb[B1 * 256 + M::row(lane, s2) * 16 + li];
+            mul(t1, w1r, w1i, w21r, w21i);                                                       // W2_1 = T_B1 W1
+        } else {
+#pragma unroll
+            for (int cb = 0; cb < NCB; cb++) w21r[cb] = w21i[cb] = z4;
         }
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++) {
-            w2r[cb] += q1[cb];
-            w2i[cb] += q1[cb];
-            wr[cb] = v4{0, 0, 0, 0};
-            wi[cb] = v4{0, 0, 0, 0};
-            q1[cb] = v4{0, 0, 0, 0};
+            p0r[cb] = p0i[cb] = p0q[cb] = z4;
+            p1r[cb] = p1i[cb] = p1q[cb] = z4;
         }
-        const int lo = bk > 0 ? j0 - 15 : j0 + 1;            // first row either block needs
-        // this wave's row blocks R = 16 (w + 4 k) >= lo - 15, software-pipelined: the U
-        // rows and both V operands of the next row block load while the current one runs
-        // on the matrix cores
+        pcr = pci = pcq = z4;
+        // rows this pass touches: the applied blocks' and the next pair's supports
+        const int lowest = N1 >= 0 ? N1 : (N0 >= 0 ? N0 : (B1 >= 0 ? B1 : B0));
+        const int lo = 16 * lowest + 1;
+        const int alo = 16 * (B1 >= 0 ? B1 : B0) + 1;          // first row the applied blocks change
         int rbk = w;
         if (16 * rbk + 15 < lo) rbk += 4 * ((lo - 15 - 16 * rbk + 63) / 64);
         v4 nur[NCB], nui[NCB];
-        CT nxu[4], nxw[4];
-        auto fetch = [&](int R) {
-#pragma unroll
-            for (int cb = 0; cb < NCB; cb++) {
-                const int col = c0 + 16 * cb + li;
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int row = R + M::row(lane, v);
-                    const CT x = (row < n && col < n) ? ldx(U, (long)row * n + col) : CT{(T)0, (T)0};
-                    nur[cb][v] = x.x;
-                    nui[cb][v] = x.y;
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                nxu[s] = lg_refl(A, n, j0, R + li, M::row(lane, s));                   // V[R + li][k]
-                nxw[s] = bk > 0 ? lg_refl(A, n, j0 - 16, R + M::row(lane, s), li)      // V_{bk-1}[row][li]
-                                : CT{(T)0, (T)0};
-            }
-        };
-        if (16 * rbk < n) fetch(16 * rbk);
+        if (PREFU && 16 * rbk < n) loadu(16 * rbk, nur, nui);
         for (; 16 * rbk < n; rbk += 4) {
             const int R = 16 * rbk;
+            v4 ur[NCB], ui[NCB];
+            if (PREFU) {
 #pragma unroll
-            for (int cb = 0; cb < NCB; cb++) {
-                ur[cb] = nur[cb];
-                ui[cb] = nui[cb];
+                for (int cb = 0; cb < NCB; cb++) {
+                    ur[cb] = nur[cb];
+                    ui[cb] = nui[cb];
+                }
+            } else {
+                loadu(R, ur, ui);
             }
-            CT xu[4], xw[4];
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                xu[s] = nxu[s];
-                xw[s] = nxw[s];
-            }
-            if (16 * (rbk + 4) < n) fetch(16 * (rbk + 4));
-            if (R + 15 >= j0 + 1) {
-                // U -= V W2 on rows R.. (K index = reflector row(l, s))
+            CT x0[4], x1[4], y0[4], y1[4];
+            vapply(B0, R, x0);
+            vapply(B1, R, x1);
+            vacc(N0, R, y0);
+            vacc(N1, R, y1);
+            if (PREFU && 16 * (rbk + 4) < n) loadu(16 * (rbk + 4), nur, nui);
+            if (R + 15 >= alo) {
+                // U -= V_B0 W2_0 + V_B1 W2_1 on rows R..
                 v4 qu[NCB];
 #pragma unroll
-                for (int cb = 0; cb < NCB; cb++) qu[cb] = v4{0, 0, 0, 0};
+                for (int cb = 0; cb < NCB; cb++) qu[cb] = z4;
 #pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    const CT x = xu[s];
-                    const T xs = x.x + x.y;
+                for (int s2 = 0; s2 < 4; s2++) {
+                    const CT xa = x0[s2], xb = x1[s2];
+                    const T xsa = xa.x + xa.y, xsb = xb.x + xb.y;
 #pragma unroll
                     for (int cb = 0; cb < NCB; cb++) {
-                        const T br = w2r[cb][s], bi = w2i[cb][s];
-                        qu[cb] = M::mma(xs, br, qu[cb]);                 // P1
-                        ur[cb] = M::mma(x.y, br + bi, ur[cb]);           // U_r -= P1 - P3
-                        ui[cb] = M::mma(-x.x, bi - br, ui[cb]);          // U_i -= P1 + P2
+                        const T ar = w20r[cb][s2], ai = w20i[cb][s2];
+                        qu[cb] = M::mma(xsa, ar, qu[cb]);
+                        ur[cb] = M::mma(xa.y, ar + ai, ur[cb]);
+                        ui[cb] = M::mma(-xa.x, ai - ar, ui[cb]);
+                        const T br = w21r[cb][s2], bi = w21i[cb][s2];
+                        qu[cb] = M::mma(xsb, br, qu[cb]);
+                        ur[cb] = M::mma(xb.y, br + bi, ur[cb]);
+                        ui[cb] = M::mma(-xb.x, bi - br, ui[cb]);
                     }
                 }
 #pragma unroll
@@ -780,7 +855,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
                     }
                 }
             }
-            if (bk > 0) acc_w(xw, wr, wi, q1);    // W partial of block bk - 1 from the updated rows
+            if (N0 >= 0) {
+                accw(y0, ur, ui, p0r, p0i, p0q);
+                if (N1 >= 0) {
+                    accw(y1, ur, ui, p1r, p1i, p1q);
+                    accc(y0, y1, pcr, pci, pcq);
+                }
+            }
         }
     }
 }
@@ -838,7 +919,7 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
     constexpr int NCB = sizeof(T) == sizeof(float) ? 2 : 1;
     const int nslab = (n + 16 * NCB - 1) / (16 * NCB);
     const long groups = (batch + 7) / 8;
-    hipLaunchKernelGGL((lg_backtransform_kernel<T, NCB>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0, s, A,
+    hipLaunchKernelGGL((lg_backtransform2_kernel<T, NCB>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0, s, A,
                        strideA, n, tws, strideT, U, strideA, nslab, nb);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
@@ -851,7 +932,7 @@ static int launch_bt_stream_t(const T *A, long strideA, int n, const T *tws, lon
     constexpr int NCB = sizeof(T) == sizeof(float) ? 2 : 1;
     const int nslab = (n + 16 * NCB - 1) / (16 * NCB);
     const long groups = (batch + 7) / 8;
-    hipLaunchKernelGGL((lg_backtransform_kernel<T, NCB>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0, s, A,
+    hipLaunchKernelGGL((lg_backtransform2_kernel<T, NCB>), dim3((unsigned)(groups * 8 * nslab)), dim3(256), 0, s, A,
                        strideA, n, tws, strideT, U, strideU, nslab, batch);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
