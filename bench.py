@@ -397,7 +397,7 @@ def main():
     peak_flops = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
     ctype = "float" if fp32 else "double"
     bt_name = "backtransform_mfma_kernel" if n_phi <= 256 else (
-        "backtransform_mfma_kernel" if n_phi <= 512 else "lg_backtransform_kernel")
+        "backtransform_mfma_kernel" if n_phi <= 512 else "lg_backtransform2_kernel")
     workload_key = f"rlm n_phi={n_phi} g={g}"
     if n_phi > 512:
         # n > 512: the Hermitian product of the reduction (one launch per column over the
