@@ -187,30 +187,33 @@ __global__ void wband_kernel(const double *__restrict__ V, const double *__restr
     st(Wb, ((long)z * d + m) * nb + b, sum);
 }
 
-// H[kp][j][b] = sum_m E[t2][m][j] Wb[s,t1][m][b] on the FP64 matrix cores: one
-// workgroup per k-point, 16x16 output tiles dealt to the 4 waves, K = m in steps of
-// 4. Output kp is local (blockIdx.x); the k-point index is kp0 + blockIdx.x over a
-// mesh of pl points per line (theta_1 = 0..pl-1 and theta_2 = 0..pl-1 per sample;
-// pl = g for the solved points, g+1 for the whole mesh). Wb is laid out per
-// (s, theta_1) with g+1 lines per sample.
+// H[kp][j][b] = sum_m E[t2][m][j] Wb[s,t1][m][b] on the FP64 matrix cores: nsplit
+// workgroups per k-point (several where the k-points are too few to fill the chip),
+// 16x16 output tiles dealt to their waves, K = m in steps of 4. Output kp is local
+// (blockIdx.x / nsplit); the k-point index is kp0 + that over a mesh of pl points per
+// line (theta_1 = 0..pl-1 and theta_2 = 0..pl-1 per sample; pl = g for the solved
+// points, g+1 for the whole mesh). Wb is laid out per (s, theta_1) with g+1 lines per
+// sample.
 __global__ __launch_bounds__(256) void hband_kernel(const double *__restrict__ E, const double *__restrict__ Wb, int N,
                                                     int d, int nb, int n_mesh, int pl, long kp0,
-                                                    double *__restrict__ H) {
+                                                    double *__restrict__ H, int nsplit) {
     using M = Mf<double>;
     using v4 = M::v4;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int li = lane & 15, lq = lane >> 4;
-    const long kp = kp0 + blockIdx.x;
+    const long kpl = blockIdx.x / nsplit;
+    const int part = (int)(blockIdx.x % nsplit);
+    const long kp = kp0 + kpl;
     const int g1 = n_mesh + 1;
     const long s = kp / ((long)pl * pl);
     const int rem = (int)(kp % ((long)pl * pl));
     const int t1 = rem / pl, t2 = rem % pl;
     const double *Et = E + 2L * t2 * d * N;
     const double *Wz = Wb + 2L * (s * g1 + t1) * d * nb;
-    double *Hk = H + 2L * blockIdx.x * N * nb;
+    double *Hk = H + 2L * kpl * N * nb;
     const int nrb = (N + 15) / 16, ncb = (nb + 15) / 16;
-    for (int tile = w; tile < nrb * ncb; tile += 4) {
+    for (int tile = part * 4 + w; tile < nrb * ncb; tile += 4 * nsplit) {
         const int j0 = (tile / ncb) * 16, b0 = (tile % ncb) * 16;
         // three real products per complex one: P1 = (er + ei) xr, P2 = er (xi - xr),
         // P3 = ei (xr + xi); Re = P1 - P3, Im = P1 + P2
@@ -266,8 +269,16 @@ int launch_wband(const double *V, const double *cmt, int n_phi, int d, int off, 
 int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, int n_mesh, int pl, long kp0,
                  long nkp, double *H, hipStream_t stream) {
     if (nkp <= 0 || pl < 1 || pl > n_mesh + 1) return nkp <= 0 ? 0 : -22;
-    hipLaunchKernelGGL(hband_kernel, dim3((unsigned)nkp), dim3(256), 0, stream, E, Wb, n_phi, d,
-                       band_width(n_phi, off), n_mesh, pl, kp0, H);
+    // enough workgroups for two per CU (n_phi = 3000, 64 k-points: 19.5 ms with one per k-point)
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    const int ntile = ((n_phi + 15) / 16) * ((band_width(n_phi, off) + 15) / 16);
+    const long want = (2L * cus + nkp - 1) / nkp;
+    const int nsplit = (int)std::max(1L, std::min<long>(want, (ntile + 3) / 4));
+    hipLaunchKernelGGL(hband_kernel, dim3((unsigned)(nkp * nsplit)), dim3(256), 0, stream, E, Wb, n_phi, d,
+                       band_width(n_phi, off), n_mesh, pl, kp0, H, nsplit);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
