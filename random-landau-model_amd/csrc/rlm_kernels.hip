@@ -20,11 +20,16 @@
 namespace rlm {
 
 // V[s][m][n] = coeff_mn[m][n] * sum_i s_i e^{-i 2pi((m-o) x_i + (n-o) y_i)}
-// grid (ceil(d/64), d, S), block 256 = 64 n-values x 4 impurity slices.
+// grid (ceil(d/64), rows, S), block 256 = 64 n-values x 4 impurity slices.
+// mirror (d = 2o + 1): the phases of (m, n) and (2o - m, 2o - n) are exact negatives
+// (both products and their sum negate exactly), so the second sum is the conjugate of the
+// first (to the last bit of sincos's odd / even symmetry): rows m <= o are computed (row
+// o up to n = o) and each sum is also written, conjugated, to its mirror; half the sincos
+// (n_phi = 3000, 4 samples: 26 -> 14 ms).
 __global__ __launch_bounds__(256) void potential_kernel(const double *__restrict__ X, const double *__restrict__ Y,
                                                         const double *__restrict__ Sg, long n_imp,
                                                         const double *__restrict__ cmn, int d, int off,
-                                                        double *__restrict__ V) {
+                                                        double *__restrict__ V, int mirror) {
     __shared__ double red[2 * 256];
     const int s = blockIdx.z, m = blockIdx.y;
     const int nl = threadIdx.x & 63, slice = threadIdx.x >> 6;
@@ -33,8 +38,9 @@ __global__ __launch_bounds__(256) void potential_kernel(const double *__restrict
     const double kx = PI2 * (double)(m - off);
     const double ky = PI2 * (double)(n - off);
     const double *x = X + s * n_imp, *y = Y + s * n_imp, *sg = Sg + s * n_imp;
+    const bool act = n < d && (!mirror || m < off || n <= off);
     double re = 0.0, im = 0.0;
-    if (n < d) {
+    if (act) {
         for (long i = slice; i < n_imp; i += 4) {
             const double phase = __dadd_rn(__dmul_rn(kx, x[i]), __dmul_rn(ky, y[i]));
             double sn, cs;
@@ -46,13 +52,18 @@ __global__ __launch_bounds__(256) void potential_kernel(const double *__restrict
     red[2 * threadIdx.x] = re;
     red[2 * threadIdx.x + 1] = im;
     __syncthreads();
-    if (slice == 0 && n < d) {
+    if (slice == 0 && act) {
         for (int k = 1; k < 4; k++) {
             re += red[2 * (threadIdx.x + 64 * k)];
             im += red[2 * (threadIdx.x + 64 * k) + 1];
         }
         const cd c = ld(cmn, (long)m * d + n);
         st(V, (long)s * d * d + (long)m * d + n, cmul(cd{re, im}, c));
+        if (mirror && !(m == off && n == off)) {
+            const int m2 = 2 * off - m, n2 = 2 * off - n;
+            const cd c2 = ld(cmn, (long)m2 * d + n2);
+            st(V, (long)s * d * d + (long)m2 * d + n2, cmul(cd{re, -im}, c2));
+        }
     }
 }
 
@@ -127,8 +138,9 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double *__restrict_
 
 int launch_potential(const double *X, const double *Y, const double *S, long n_imp, const double *cmn, int d, int off,
                      double *V, int n_samples, hipStream_t stream) {
-    dim3 grid((d + 63) / 64, d, n_samples);
-    hipLaunchKernelGGL(potential_kernel, grid, dim3(256), 0, stream, X, Y, S, n_imp, cmn, d, off, V);
+    const int mirror = d == 2 * off + 1 ? 1 : 0;
+    dim3 grid((d + 63) / 64, mirror ? off + 1 : d, n_samples);
+    hipLaunchKernelGGL(potential_kernel, grid, dim3(256), 0, stream, X, Y, S, n_imp, cmn, d, off, V, mirror);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
