@@ -344,6 +344,7 @@ __global__ __launch_bounds__(256) void ref_update_kernel(const double *__restric
                                                          const double *__restrict__ Gw, const int *__restrict__ Cl,
                                                          int n, int wpk, double *__restrict__ Xout) {
     __shared__ cd E[64][kRefWin];
+    __shared__ cd xrow[4][64 + 2 * kRefW];
     const long kp = blockIdx.x / wpk;
     const int j0 = (int)(blockIdx.x % wpk) * 64;
     const int t = threadIdx.x;
@@ -372,18 +373,26 @@ __global__ __launch_bounds__(256) void ref_update_kernel(const double *__restric
         E[jj][dd] = v;
     }
     __syncthreads();
-    const int jj = t & 63, j = j0 + jj;
-    if (j >= n) return;                                      // no barriers below
+    // each wave takes every fourth row: the row's window X[i][j0 - kRefW .. j0 + 63 + kRefW]
+    // is staged in the wave's LDS buffer (two coalesced loads per lane instead of kRefWin
+    // scattered ones), then every lane forms its state's entry from it
+    const int jj = t & 63, j = j0 + jj, wv = t >> 6;
+    cd *xb = xrow[wv];
     const double *Xk = Xd + 2 * kp * (long)n * n;
     double *Ok = Xout + 2 * kp * (long)n * n;
-    for (int i = t >> 6; i < n; i += 4) {
+    for (int i = wv; i < n; i += 4) {
         const long row = (long)i * n;
-        cd acc = ld(Xk, row + j);
-        for (int dd = 0; dd < kRefWin; dd++) {
-            const int k = j + dd - kRefW;
-            if (k >= 0 && k < n) cfma(acc, ld(Xk, row + k), E[jj][dd]);
+        for (int e = jj; e < 64 + 2 * kRefW; e += 64) {
+            const int k = j0 - kRefW + e;
+            xb[e] = (k >= 0 && k < n) ? ld(Xk, row + k) : cd{0.0, 0.0};
         }
-        st(Ok, row + j, acc);
+        __builtin_amdgcn_wave_barrier();
+        if (j < n) {
+            cd acc = xb[jj + kRefW];
+            for (int dd = 0; dd < kRefWin; dd++) cfma(acc, xb[jj + dd], E[jj][dd]);
+            st(Ok, row + j, acc);
+        }
+        __builtin_amdgcn_wave_barrier();                     // the buffer is rewritten by the next row
     }
 }
 
