@@ -13,11 +13,12 @@
 //                      and builds reflector c (zlarfg);
 //     lg_symv_lower_kernel + lg_symv_reduce_kernel
 //                      y = A22 v - V (W^H v) - W (V^H v) from the lower triangle
-//                      in 32 x 32 tiles (row and column sums from one read): the
-//                      trailing lower triangle is streamed once per column; one
-//                      extra workgroup per matrix forms W^H v, V^H v beside the
-//                      tiles, and the reduction's idle half-wave applies the older
-//                      pending terms to the next column, strip by strip;
+//                      in 32 x 32 tiles (row and column sums from one read),
+//                      blocks of 4 x 16 tiles per workgroup: the trailing lower
+//                      triangle is streamed once per column; one extra workgroup
+//                      per matrix forms W^H v, V^H v beside the tiles, and the
+//                      reduction's idle half-wave applies the older pending terms
+//                      to the next column, strip by strip;
 //     lg_her2k_kernel  after each panel A22 -= V W^H + W V^H on the lower tiles, on
 //                      the matrix cores.
 //   Divide & conquer: stedc_kernel with its node arrays in global memory.
@@ -88,14 +89,21 @@ int symv_timing(int enable, double *total_ms, long *launches) {
 
 // Per-matrix panel workspace, complex elements: Vp (kLgNB x n, column l = reflector
 // k0 + l with explicit zeros above and the unit entry), Wp (kLgNB x n), P (2 kLgNB:
-// W^H v then V^H v of the current column), then the Hermitian product's partial sums:
-// Rs (ns x 32: row sums of each 32-row strip of the trailing block) and Cs (ns x ns x
-// 32: column sums of tile (I, J), I >= J), ns = number of strips of the first column.
-constexpr int kLgTS = 32;
-static long lg_strips(int n) { return (n - 1 + kLgTS - 1) / kLgTS; }
+// W^H v then V^H v of the current column), then the Hermitian product's partial sums
+// over strips of 32 rows (strip K = rows base + 32 K .., base = (c + 1) rounded down to
+// a multiple of 8, so that every 8-row group of a column is one aligned segment):
+// Rp (ns x njb x 32: row sums of strip K over the tiles of column block jb) and Cp
+// (ns x nib x 32: column sums of tile column K over the strips of row block ib),
+// ns = strips of the first column.
+constexpr int kLgTS = 32;   // tile edge
+constexpr int kLgTR = 4;    // strips per row block (one per wave)
+constexpr int kLgTC = 16;   // tiles per column block
+static long lg_strips(int n) { return (n + kLgTS - 1) / kLgTS; }
+__host__ __device__ inline long lg_row_blocks(long ns) { return (ns + kLgTR - 1) / kLgTR; }
+__host__ __device__ inline long lg_col_blocks(long ns) { return (ns + kLgTC - 1) / kLgTC; }
 long heevd_large_aux_complex(int n) {
     const long ns = lg_strips(n);
-    return 2L * kLgNB * n + 2L * kLgNB + ns * kLgTS + ns * ns * kLgTS;
+    return 2L * kLgNB * n + 2L * kLgNB + ns * (lg_row_blocks(ns) + lg_col_blocks(ns)) * kLgTS;
 }
 
 // ----------------------------------------------------------------------------
@@ -270,45 +278,51 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
 
 // ----------------------------------------------------------------------------
 // Hermitian product from the lower triangle: y = A22 v with A22 = A(s:n, s:n), s = c+1,
-// in 32 x 32 tiles (I, J), I >= J, of the trailing block. A workgroup takes the strips
-// p and ns - 1 - p (balanced work), its waves the tiles J = w, w + 4, ... of each strip;
-// lane (lrg, lcg) = (lane & 7, lane >> 3) holds the 4 x 4 sub-block at tile rows
-// lrg + 8 ru (so that the 8 lanes of a column group read 8 consecutive rows of the
-// column-major matrix per load), columns 4 lcg.., read once for both y_I += A_IJ v_J (row sums, reduced over
-// lcg) and y_J += A_IJ^H v_I (column sums, reduced over lrg). Row sums of a strip stay in
-// registers and are summed over the waves in LDS (fixed order) into Rs[I]; the column
-// sums of tile (I, J) go to Cs[I][J] (the diagonal tile's strictly-lower part too), and
-// lg_symv_reduce_kernel adds them up in a fixed order.
+// in 32 x 32 tiles (I, J), I >= J, of the strips of rows base + 32 I .. (rows and columns
+// below s masked; base = s rounded down to a multiple of 8). Workgroup (b, ib, jb) takes
+// the tiles of strips 4 ib .. 4 ib + 3 (wave w: strip 4 ib + w) and tile columns
+// 16 jb .. 16 jb + 15, so the four waves read neighbouring 256-byte segments of the same
+// columns together. Lane (lrg, lcg) = (lane & 7, lane >> 3) holds the 4 x 4 sub-block at
+// tile rows lrg + 8 ru (the 8 lanes of a column group read 8 consecutive rows of the
+// column-major matrix per load), columns 4 lcg.., read once for both y_I += A_IJ v_J
+// (row sums: kept in registers over the wave's tiles, reduced over lcg once, into
+// Rp[I][jb]) and y_J += A_IJ^H v_I (column sums: reduced over lrg per tile into LDS,
+// summed over the waves in a fixed order into Cp[J][ib]).
 template <typename T>
 __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict__ Abase, long strideA, int n,
                                                             T *__restrict__ aux, long strideAux, int k0, int i,
-                                                            int npair, int ns0, int nb) {
+                                                            int per, int ns0, int nb) {
     using C = cx<T>;
-    extern __shared__ double lds_svl[];
-    C *vs = reinterpret_cast<C *>(lds_svl);                  // v(s .. n-1)
-    __shared__ C rpart[4][kLgTS];
+    // complex float issues the next tile's loads before this tile's sums (complex double
+    // has no registers to spare for it and gains nothing)
+    constexpr bool kPrefetch = sizeof(T) == sizeof(float);
+    __shared__ C vrow[kLgTR * kLgTS];
+    __shared__ C vcol[kLgTC * kLgTS];
+    __shared__ C colp[kLgTR][kLgTC][kLgTS];
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    const int c = k0 + i, s0 = c + 1, m = n - s0;
     if (blockIdx.x < (unsigned)nb) {
         // workgroup b < nb: P[l] = W(:, l)^H v, P[kLgNB + l] = V(:, l)^H v over the rows > c
         // of matrix b, one wave per product (dispatched first, they run beside the tiles)
         const long bm = blockIdx.x;
-        const int c = k0 + i, m = n - c - 1;
         C *Vq = reinterpret_cast<C *>(aux + 2 * bm * strideAux);
         C *Wq = Vq + (long)kLgNB * n;
         C *P = Wq + (long)kLgNB * n;
-        for (int e = t; e < m; e += 256) vs[e] = Vq[(long)i * n + c + 1 + e];
-        __syncthreads();
+        const C *vs = Vq + (long)i * n + s0;
         for (int qd = w; qd < 2 * i; qd += 4) {
-            const C *col = ((qd < i) ? Wq + (long)qd * n : Vq + (long)(qd - i) * n) + c + 1;
+            const C *col = ((qd < i) ? Wq + (long)qd * n : Vq + (long)(qd - i) * n) + s0;
             C s{(T)0, (T)0};
             int e = lane;
-            // eight loads in flight per step, accumulated in the same order as one at a time
+            // eight products in flight per step, accumulated in the same order as one at a time
             for (; e + 7 * kWave < m; e += 8 * kWave) {
-                C x[8];
+                C x[8], y[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) x[k] = col[e + k * kWave];
+                for (int k = 0; k < 8; k++) {
+                    x[k] = col[e + k * kWave];
+                    y[k] = vs[e + k * kWave];
+                }
 #pragma unroll
-                for (int k = 0; k < 8; k++) fmacx(s, x[k], vs[e + k * kWave]);
+                for (int k = 0; k < 8; k++) fmacx(s, x[k], y[k]);
             }
             for (; e < m; e += kWave) fmacx(s, col[e], vs[e]);
             s.x = wave_sum_t(s.x);
@@ -317,85 +331,123 @@ __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict_
         }
         return;
     }
-    const long b = (blockIdx.x - nb) / npair;
-    const int p = (int)((blockIdx.x - nb) % npair);
     const int lrg = lane & 7, lcg = lane >> 3;
-    const int c = k0 + i, s0 = c + 1, m = n - s0;
-    const int ns = (m + kLgTS - 1) / kLgTS;
+    const int base = s0 & ~7, mb = n - base, lo = s0 - base;   // local rows / columns lo .. mb-1
+    const int ns = (mb + kLgTS - 1) / kLgTS, nib = (ns + kLgTR - 1) / kLgTR;
+    // workgroup -> (matrix b, row block ib, column block jb); row block ib has
+    // floor(min(4 ib + 3, ns - 1) / 16) + 1 column blocks
+    const long b = (blockIdx.x - nb) / per;
+    int q = (int)((blockIdx.x - nb) % per), ib = 0;
+    for (; ib < nib - 1; ib++) {
+        const int nj = min(kLgTR * ib + kLgTR - 1, ns - 1) / kLgTC + 1;
+        if (q < nj) break;
+        q -= nj;
+    }
+    const int jb = q;
+    const int I = kLgTR * ib + w;
+    const int J0 = jb * kLgTC, J1 = min(J0 + kLgTC, min(kLgTR * ib + kLgTR - 1, ns - 1) + 1);
     const T *A = Abase + 2 * b * strideA;
     C *Vp = reinterpret_cast<C *>(aux + 2 * b * strideAux);
-    C *Rs = Vp + 2L * kLgNB * n + 2L * kLgNB;
-    C *Cs = Rs + (long)ns0 * kLgTS;
-    for (int e = t; e < m; e += 256) vs[e] = Vp[(long)i * n + s0 + e];
+    const C *v = Vp + (long)i * n;                             // v(r) for rows r > c
+    C *Rp = Vp + 2L * kLgNB * n + 2L * kLgNB;
+    C *Cp = Rp + (long)ns0 * lg_col_blocks(ns0) * kLgTS;
+    const int njbm = (int)lg_col_blocks(ns0), nibm = (int)lg_row_blocks(ns0);
+    for (int e = t; e < kLgTR * kLgTS; e += 256) {
+        const int r = base + kLgTR * ib * kLgTS + e;
+        vrow[e] = (r >= s0 && r < n) ? v[r] : C{(T)0, (T)0};
+    }
+    for (int e = t; e < (J1 - J0) * kLgTS; e += 256) {
+        const int r = base + J0 * kLgTS + e;
+        vcol[e] = (r >= s0 && r < n) ? v[r] : C{(T)0, (T)0};
+    }
     __syncthreads();
-    for (int half = 0; half < 2; half++) {
-        const int I = half == 0 ? p : ns - 1 - p;
-        if (I < 0 || I >= ns || (half == 1 && I == p)) continue;       // workgroup-uniform
-        C racc[4];
+    const int rb = I * kLgTS + lrg;                          // local rows rb + 8 ru of the sub-block
+    C racc[4], vr[4];
 #pragma unroll
-        for (int ru = 0; ru < 4; ru++) racc[ru] = C{(T)0, (T)0};
-        const int rb = I * kLgTS + lrg;                      // local rows rb + 8 ru of the sub-block
-        C vr[4];
+    for (int ru = 0; ru < 4; ru++) {
+        racc[ru] = C{(T)0, (T)0};
+        vr[ru] = vrow[w * kLgTS + lrg + 8 * ru];
+    }
+    auto load_tile = [&](int J, C (&dst)[4][4]) {
+        const int cb = J * kLgTS + 4 * lcg;
 #pragma unroll
-        for (int ru = 0; ru < 4; ru++) vr[ru] = rb + 8 * ru < m ? vs[rb + 8 * ru] : C{(T)0, (T)0};
-        for (int J = w; J <= I; J += 4) {
-            const int cb = J * kLgTS + 4 * lcg;
-            C a[4][4], vc[4];
+        for (int cu = 0; cu < 4; cu++) {
+            const long colo = (long)(base + min(cb + cu, mb - 1)) * n + base;
 #pragma unroll
-            for (int cu = 0; cu < 4; cu++) {
-                vc[cu] = cb + cu < m ? vs[cb + cu] : C{(T)0, (T)0};
-                const long colo = (long)(s0 + min(cb + cu, m - 1)) * n;
-#pragma unroll
-                for (int ru = 0; ru < 4; ru++) a[ru][cu] = ldx(A, colo + s0 + min(rb + 8 * ru, m - 1));
-            }
-            const bool diag = J == I;
-            C csum[4];
-#pragma unroll
-            for (int cu = 0; cu < 4; cu++) csum[cu] = C{(T)0, (T)0};
-#pragma unroll
-            for (int ru = 0; ru < 4; ru++)
-#pragma unroll
-                for (int cu = 0; cu < 4; cu++) {
-                    const int lr = rb + 8 * ru, lc = cb + cu;
-                    const bool in = lr < m && lc < m && (!diag || lc <= lr);
-                    const C x = in ? a[ru][cu] : C{(T)0, (T)0};
-                    fmax_(racc[ru], x, vc[cu]);                                  // A(r, c) v_c
-                    if (!diag || lc < lr) fmacx(csum[cu], x, vr[ru]);           // conj(A(r, c)) v_r
-                }
-            // column sums over lrg (lane bits 0-2), then lanes lrg == 0 store them
-#pragma unroll
-            for (int cu = 0; cu < 4; cu++)
-#pragma unroll
-                for (int off = 1; off < 8; off <<= 1) {
-                    csum[cu].x += __shfl_xor(csum[cu].x, off, kWave);
-                    csum[cu].y += __shfl_xor(csum[cu].y, off, kWave);
-                }
-            if (lrg == 0) {
-                C *dst = Cs + ((long)I * ns0 + J) * kLgTS + 4 * lcg;
-#pragma unroll
-                for (int cu = 0; cu < 4; cu++) dst[cu] = csum[cu];
-            }
+            for (int ru = 0; ru < 4; ru++) dst[ru][cu] = ldx(A, colo + min(rb + 8 * ru, mb - 1));
         }
-        // row sums over lcg (lane bits 3-5), then over the waves through LDS
+    };
+    const int Je = I < ns ? min(J1, I + 1) : J0;            // this wave's tiles J0 .. Je-1
+    C a[4][4];
+    if (kPrefetch && J0 < Je) load_tile(J0, a);
+    for (int J = J0; J < Je; J++) {
+        const int cb = J * kLgTS + 4 * lcg;
+        C an[4][4], vc[4];
+        if (kPrefetch)
+            load_tile(min(J + 1, Je - 1), an);
+        else
+            load_tile(J, a);
+#pragma unroll
+        for (int cu = 0; cu < 4; cu++) vc[cu] = vcol[(J - J0) * kLgTS + 4 * lcg + cu];
+        const bool diag = J == I;
+        C csum[4];
+#pragma unroll
+        for (int cu = 0; cu < 4; cu++) csum[cu] = C{(T)0, (T)0};
 #pragma unroll
         for (int ru = 0; ru < 4; ru++)
 #pragma unroll
-            for (int off = 8; off < kWave; off <<= 1) {
-                racc[ru].x += __shfl_xor(racc[ru].x, off, kWave);
-                racc[ru].y += __shfl_xor(racc[ru].y, off, kWave);
+            for (int cu = 0; cu < 4; cu++) {
+                const int lr = rb + 8 * ru, lc = cb + cu;
+                const bool in = lr < mb && lc < mb && lr >= lo && lc >= lo && (!diag || lc <= lr);
+                const C x = in ? a[ru][cu] : C{(T)0, (T)0};
+                fmax_(racc[ru], x, vc[cu]);                                  // A(r, c) v_c
+                if (!diag || lc < lr) fmacx(csum[cu], x, vr[ru]);           // conj(A(r, c)) v_r
             }
-        if (lcg == 0) {
+        // column sums over lrg (lane bits 0-2), then lanes lrg == 0 keep them in LDS
 #pragma unroll
-            for (int ru = 0; ru < 4; ru++) rpart[w][lrg + 8 * ru] = racc[ru];
+        for (int cu = 0; cu < 4; cu++)
+#pragma unroll
+            for (int off = 1; off < 8; off <<= 1) {
+                csum[cu].x += __shfl_xor(csum[cu].x, off, kWave);
+                csum[cu].y += __shfl_xor(csum[cu].y, off, kWave);
+            }
+        if (lrg == 0) {
+#pragma unroll
+            for (int cu = 0; cu < 4; cu++) colp[w][J - J0][4 * lcg + cu] = csum[cu];
         }
-        __syncthreads();
-        if (t < kLgTS) Rs[(long)I * kLgTS + t] = ((rpart[0][t] + rpart[1][t]) + rpart[2][t]) + rpart[3][t];
-        __syncthreads();
+        if (kPrefetch) {
+#pragma unroll
+            for (int ru = 0; ru < 4; ru++)
+#pragma unroll
+                for (int cu = 0; cu < 4; cu++) a[ru][cu] = an[ru][cu];
+        }
+    }
+    // row sums over lcg (lane bits 3-5)
+#pragma unroll
+    for (int ru = 0; ru < 4; ru++)
+#pragma unroll
+        for (int off = 8; off < kWave; off <<= 1) {
+            racc[ru].x += __shfl_xor(racc[ru].x, off, kWave);
+            racc[ru].y += __shfl_xor(racc[ru].y, off, kWave);
+        }
+    if (I < ns && lcg == 0) {
+#pragma unroll
+        for (int ru = 0; ru < 4; ru++) Rp[((long)I * njbm + jb) * kLgTS + lrg + 8 * ru] = racc[ru];
+    }
+    __syncthreads();
+    // column sums of tile column J over the strips J .. of this row block, waves in order
+    for (int e = t; e < (J1 - J0) * kLgTS; e += 256) {
+        const int J = J0 + e / kLgTS;
+        C sacc{(T)0, (T)0};
+#pragma unroll
+        for (int ww = 0; ww < kLgTR; ww++)
+            if (kLgTR * ib + ww >= J && kLgTR * ib + ww < ns) sacc = sacc + colp[ww][e / kLgTS][e % kLgTS];
+        Cp[((long)J * nibm + ib) * kLgTS + e % kLgTS] = sacc;
     }
 }
 
-// y(r) = Rs[K][r'] + sum_{I >= K} Cs[I][K][r'] - V (W^H v) - W (V^H v) for the 32 rows of
-// strip K, into Wp(:, i).
+// y(r) = sum_jb Rp[K][jb][r'] + sum_{ib >= K/4} Cp[K][ib][r'] - V (W^H v) - W (V^H v) for
+// the rows r = base + 32 K + r' >= c + 1 of strip K, into Wp(:, i).
 // Threads 32..63 meanwhile apply to column c + 1 (rows of the strip, all >= c + 1) the
 // terms l < i of the panel's pending update, when column c + 1 is in the same panel
 // (i + 1 < nbk): A(r, c+1) -= V(r, l) conj(W(c+1, l)) + W(r, l) conj(V(c+1, l)), in the
@@ -404,20 +456,23 @@ template <typename T>
 __global__ __launch_bounds__(64) void lg_symv_reduce_kernel(T *__restrict__ Abase, long strideA, T *__restrict__ aux,
                                                             long strideAux, int n, int k0, int i, int nbk, int ns0) {
     using C = cx<T>;
-    const int c = k0 + i, s0 = c + 1, m = n - s0;
-    const int ns = (m + kLgTS - 1) / kLgTS;
+    const int c = k0 + i, s0 = c + 1;
+    const int base = s0 & ~7, mb = n - base;
+    const int ns = (mb + kLgTS - 1) / kLgTS, nib = (ns + kLgTR - 1) / kLgTR;
     const long b = blockIdx.x / ns;
     const int K = (int)(blockIdx.x % ns);
     const int t = threadIdx.x;
     C *Vp = reinterpret_cast<C *>(aux + 2 * b * strideAux);
     C *Wp = Vp + (long)kLgNB * n;
     const C *P = Wp + (long)kLgNB * n;
-    const C *Rs = Vp + 2L * kLgNB * n + 2L * kLgNB;
-    const C *Cs = Rs + (long)ns0 * kLgTS;
+    const int njbm = (int)lg_col_blocks(ns0), nibm = (int)lg_row_blocks(ns0);
+    const C *Rp = Vp + 2L * kLgNB * n + 2L * kLgNB;
+    const C *Cp = Rp + (long)ns0 * njbm * kLgTS;
+    const int r = base + K * kLgTS + (t & (kLgTS - 1));
+    if (r < s0 || r >= n) return;
     if (t >= kLgTS) {
-        const int lr = K * kLgTS + (t - kLgTS);
-        if (i + 1 >= nbk || i == 0 || lr >= m) return;
-        const int r = s0 + lr, cn = c + 1;
+        if (i + 1 >= nbk || i == 0) return;
+        const int cn = c + 1;
         T *A = Abase + 2 * b * strideA;
         C x = ldx(A, (long)cn * n + r);
         for (int l = 0; l < i; l++)
@@ -425,20 +480,19 @@ __global__ __launch_bounds__(64) void lg_symv_reduce_kernel(T *__restrict__ Abas
         stx(A, (long)cn * n + r, x);
         return;
     }
-    const int lr = K * kLgTS + t;
-    if (lr >= m) return;
-    // loads issued eight ahead of the fixed-order sums
-    C y = Rs[(long)K * kLgTS + t];
-    int I = K;
-    for (; I + 8 <= ns; I += 8) {
+    // loads issued ahead of the fixed-order sums
+    const int njb = K / kLgTC + 1, ib0 = K / kLgTR;
+    C y{(T)0, (T)0};
+    for (int jb = 0; jb < njb; jb++) y = y + Rp[((long)K * njbm + jb) * kLgTS + t];
+    int ib = ib0;
+    for (; ib + 8 <= nib; ib += 8) {
         C x[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) x[k] = Cs[((long)(I + k) * ns0 + K) * kLgTS + t];
+        for (int k = 0; k < 8; k++) x[k] = Cp[((long)K * nibm + ib + k) * kLgTS + t];
 #pragma unroll
         for (int k = 0; k < 8; k++) y = y + x[k];
     }
-    for (; I < ns; I++) y = y + Cs[((long)I * ns0 + K) * kLgTS + t];
-    const int r = s0 + lr;
+    for (; ib < nib; ib++) y = y + Cp[((long)K * nibm + ib) * kLgTS + t];
     int l = 0;
     for (; l + 4 <= i; l += 4) {
         C xv[4], xw[4];
@@ -876,10 +930,6 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
     const long strideA = (long)n * n;
     const long strideAux = heevd_large_aux_complex(n);
     const int nb = (int)batch;
-    const size_t lds_v = sizeof(C) * n;                     // the Hermitian product's copy of v
-    if (lds_v > 32 * 1024)                                   // dynamic + static beyond the 64 KB default
-        RLM_HIP_CHECK(hipFuncSetAttribute((const void *)lg_symv_lower_kernel<T>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_v));
     const int ns0 = (int)lg_strips(n);
     for (int k0 = 0; k0 < n - 1; k0 += kLgNB) {
         const int nbk = std::min(kLgNB, n - 1 - k0);
@@ -887,13 +937,17 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
             hipLaunchKernelGGL(lg_col_kernel<T>, dim3(nb), dim3(kLgColNT), 0, s, A, strideA, n, D, E, TAU,
                                strideV, aux, strideAux, k0, i, nbk);
             if (i == nbk) break;
-            const int m = n - (k0 + i) - 1;
-            const int nsm = (m + kLgTS - 1) / kLgTS, npair = (nsm + 1) / 2;
+            // strips of rows base .. n-1 (base = c + 1 rounded down to a multiple of 8) and the
+            // workgroups per matrix: row block ib has floor(min(4 ib + 3, ns - 1) / 16) + 1
+            const int mb = n - ((k0 + i + 1) & ~7);
+            const int nsm = (mb + kLgTS - 1) / kLgTS, nib = (nsm + kLgTR - 1) / kLgTR;
+            int per = 0;
+            for (int ib = 0; ib < nib; ib++) per += std::min(kLgTR * ib + kLgTR - 1, nsm - 1) / kLgTC + 1;
             hipEvent_t te0, te1;
             kt_pair(s, te0, te1);
             if (te0) RLM_HIP_CHECK(hipEventRecord(te0, s));
-            hipLaunchKernelGGL(lg_symv_lower_kernel<T>, dim3((unsigned)((long)nb * npair + nb)), dim3(256),
-                               sizeof(C) * m, s, A, strideA, n, aux, strideAux, k0, i, npair, ns0, nb);
+            hipLaunchKernelGGL(lg_symv_lower_kernel<T>, dim3((unsigned)((long)nb * per + nb)), dim3(256), 0, s, A,
+                               strideA, n, aux, strideAux, k0, i, per, ns0, nb);
             if (te1) RLM_HIP_CHECK(hipEventRecord(te1, s));
             hipLaunchKernelGGL(lg_symv_reduce_kernel<T>, dim3((unsigned)((long)nb * nsm)), dim3(64), 0, s, A, strideA,
                                aux, strideAux, n, k0, i, nbk, ns0);
