@@ -29,7 +29,9 @@
 #include "eig.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
+#include <string>
 #include <vector>
 
 namespace rlm {
@@ -921,6 +923,382 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
 }
 
 // ----------------------------------------------------------------------------
+// Back-transform in passes of NB reflectors (128 for complex float, 64 for complex double):
+// Q_P = H_j0 ... H_{j0+NB-1} = I - V T V^H with T (NB x NB, upper triangular) combined from
+// the 16-reflector factors of larft_kernel by the product rule of two compact-WY blocks,
+// T_{0:l, l} = -T_{0:l,0:l} (V_{0:l}^H V_l) T_ll over the 16-column blocks l. Per pass,
+// last first: W2 = T (V^H U) (lgbt_w_kernel, one workgroup per 64 eigenvector columns,
+// the K sum over every row below j0 staged through LDS), then U -= V W2
+// (lgbt_update_kernel, tiles of 4 WR rows x 64 columns). U is read twice and written
+// once per NB reflectors (the slab kernel above: read and written once per 32).
+
+// T of pass p of matrix b, column-major (Tbig[k' NB + k] = T(k, k')); one workgroup per
+// (matrix, pass). G_ml = V_m^H V_l (m < l) on the matrix cores, the block recurrence on the
+// VALU in LDS.
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void lgbt_tbig_kernel(const T *__restrict__ Abase, long strideA, int n,
+                                                        const T *__restrict__ Tin, long strideT,
+                                                        T *__restrict__ Tbig, long strideTb, int nbb, int batch) {
+    using M = Mf<T>;
+    using v4 = typename M::v4;
+    using CT = cx<T>;
+    constexpr int NS = NB / 16, NG = NS * (NS - 1) / 2, NTB = NS * (NS + 1) / 2, LP = 17;
+    __shared__ CT gsh[NG][16 * LP];          // G_ml, m < l, at l (l - 1) / 2 + m
+    __shared__ CT tsh[NTB][16 * LP];         // T_il, i <= l, at l (l + 1) / 2 + i
+    __shared__ CT xsh[NS - 1][16 * LP];      // X_m = G_ml T_ll of the current l
+    const long b = blockIdx.x / nbb;
+    const int p = (int)(blockIdx.x % nbb);
+    if (b >= batch) return;                  // whole workgroup
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(t / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const T *A = Abase + 2 * b * strideA;
+    const CT *T16 = reinterpret_cast<const CT *>(Tin + 2 * b * strideT);
+    CT *Tb = reinterpret_cast<CT *>(Tbig + 2 * b * strideTb) + (long)p * NB * NB;
+    const int j0 = p * NB, nblk = (n - 1 + 15) / 16;
+    for (int pi = w; pi < NG; pi += 4) {
+        int l = 1;
+        while (l * (l + 1) / 2 <= pi) l++;
+        const int m = pi - l * (l - 1) / 2;
+        v4 qq = {0, 0, 0, 0}, rr = {0, 0, 0, 0}, ii = {0, 0, 0, 0};
+        // rows below j0 + 16 l + 1 are zero in V_l
+        for (int r4 = (j0 + 16 * l + 1) & ~3; r4 < n; r4 += 4) {
+            const CT x = lg_refl(A, n, j0, r4 + lq, 16 * m + li);    // A operand: conj V_m[row][k = li]
+            const CT y = lg_refl(A, n, j0, r4 + lq, 16 * l + li);    // B operand: V_l[row][k = li]
+            qq = M::mma(x.x - x.y, y.x, qq);
+            rr = M::mma(x.y, y.x + y.y, rr);
+            ii = M::mma(x.x, y.y - y.x, ii);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) gsh[pi][M::row(lane, v) * LP + li] = CT{qq[v] + rr[v], qq[v] + ii[v]};
+    }
+    for (int e = t; e < NS * 256; e += 256) {
+        const int l = e >> 8, k2 = (e >> 4) & 15, k1 = e & 15;
+        const int bk = j0 / 16 + l;
+        tsh[l * (l + 1) / 2 + l][k1 * LP + k2] = bk < nblk ? T16[(long)bk * 256 + k2 * 16 + k1] : CT{(T)0, (T)0};
+    }
+    __syncthreads();
+    const int er = t >> 4, ec = t & 15;      // element (er, ec) of a 16 x 16 block
+    for (int l = 1; l < NS; l++) {
+        const CT *tll = tsh[l * (l + 1) / 2 + l];
+        for (int m = 0; m < l; m++) {
+            const CT *g = gsh[l * (l - 1) / 2 + m];
+            CT acc{(T)0, (T)0};
+#pragma unroll
+            for (int q = 0; q < 16; q++) fmax_(acc, g[er * LP + q], tll[q * LP + ec]);
+            xsh[m][er * LP + ec] = acc;
+        }
+        __syncthreads();
+        for (int i = 0; i < l; i++) {
+            CT acc{(T)0, (T)0};
+            for (int m = i; m < l; m++) {
+                const CT *tim = tsh[m * (m + 1) / 2 + i];
+#pragma unroll
+                for (int q = 0; q < 16; q++) fmax_(acc, tim[er * LP + q], xsh[m][q * LP + ec]);
+            }
+            tsh[l * (l + 1) / 2 + i][er * LP + ec] = CT{-acc.x, -acc.y};
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < NB * NB; e += 256) {
+        const int kc = e / NB, kr = e % NB, i = kr >> 4, l = kc >> 4;
+        Tb[e] = (i <= l) ? tsh[l * (l + 1) / 2 + i][(kr & 15) * LP + (kc & 15)] : CT{(T)0, (T)0};
+    }
+}
+
+// W2 = T (V^H U) for pass p: workgroup (matrix b, column tile ct) forms W' = V^H U(:, 64 ct ..)
+// over the rows j0 + 1 .. n - 1 in chunks of 16 rows (V and U staged through LDS, double
+// buffered: the next chunk's global loads issue before the current chunk's products), wave
+// w the rows k = w NB/4 .. of W'; then W2 = T W' from W' in LDS. The column tiles of one
+// matrix share blockIdx % 8 (one XCD, whose L2 serves V and T).
+template <typename T, int NB>
+__global__ __launch_bounds__(256, 2) void lgbt_w_kernel(const T *__restrict__ Abase, long strideA, int n,
+                                                        const T *__restrict__ Tbig, long strideTb,
+                                                        const T *__restrict__ Uin, long strideU, T *__restrict__ W2,
+                                                        long strideW2, int p, int nct, int batch) {
+    using M = Mf<T>;
+    using v4 = typename M::v4;
+    using CT = cx<T>;
+    constexpr int KW = NB / 4, NKT = KW / 16, CW = 64, NCT = CW / 16, KC = 16;
+    constexpr int VP = NB + 1, UP = CW + 1, WP = CW + 1;        // LDS row strides (complex)
+    constexpr int STG = KC * (VP + UP);
+    constexpr int LDSN = (2 * STG > NB * WP) ? 2 * STG : NB * WP;
+    constexpr int NVL = KC * NB / 256, NUL = KC * CW / 256;     // staged elements per thread
+    __shared__ CT lds[LDSN];
+    const long b = (blockIdx.x & 7) + 8L * (blockIdx.x / (8 * nct));
+    const int ct = (int)((blockIdx.x >> 3) % nct);
+    if (b >= batch) return;                  // whole workgroup
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(t / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const T *A = Abase + 2 * b * strideA;
+    const T *U = Uin + 2 * b * strideU;
+    const int j0 = p * NB, rs = j0 + 1, c0 = ct * CW;
+    const int nch = (n - rs + KC - 1) / KC;
+    CT vreg[NVL], ureg[NUL];
+    auto gload = [&](int ch) {
+        const int r0 = rs + ch * KC;
+#pragma unroll
+        for (int it = 0; it < NVL; it++) {
+            const int e = t + 256 * it, k = e / KC, rr = e % KC;
+            vreg[it] = lg_refl(A, n, j0, r0 + rr, k);
+        }
+#pragma unroll
+        for (int it = 0; it < NUL; it++) {
+            const int e = t + 256 * it, rr = e / CW, cc = e % CW;
+            const int row = r0 + rr, col = c0 + cc;
+            ureg[it] = (row < n && col < n) ? ldx(U, (long)row * n + col) : CT{(T)0, (T)0};
+        }
+    };
+    auto sstore = [&](int buf) {
+        CT *vs = lds + buf * STG, *us = vs + KC * VP;
+#pragma unroll
+        for (int it = 0; it < NVL; it++) {
+            const int e = t + 256 * it, k = e / KC, rr = e % KC;
+            vs[rr * VP + k] = vreg[it];
+        }
+#pragma unroll
+        for (int it = 0; it < NUL; it++) {
+            const int e = t + 256 * it, rr = e / CW, cc = e % CW;
+            us[rr * UP + cc] = ureg[it];
+        }
+    };
+    const v4 z4 = {0, 0, 0, 0};
+    v4 aq[NKT][NCT], ar[NKT][NCT], ai[NKT][NCT];
+#pragma unroll
+    for (int kt = 0; kt < NKT; kt++)
+#pragma unroll
+        for (int c = 0; c < NCT; c++) aq[kt][c] = ar[kt][c] = ai[kt][c] = z4;
+    if (nch > 0) {
+        gload(0);
+        sstore(0);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < nch; ch++) {
+        const int cur = ch & 1;
+        if (ch + 1 < nch) gload(ch + 1);
+        const CT *vs = lds + cur * STG, *us = vs + KC * VP;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int rr = 4 * s + lq;
+            CT x[NKT], u[NCT];
+#pragma unroll
+            for (int kt = 0; kt < NKT; kt++) x[kt] = vs[rr * VP + w * KW + 16 * kt + li];   // conj V: A[k][row]
+#pragma unroll
+            for (int c = 0; c < NCT; c++) u[c] = us[rr * UP + 16 * c + li];                  // U: B[row][col]
+#pragma unroll
+            for (int kt = 0; kt < NKT; kt++) {
+                const T xd = x[kt].x - x[kt].y;
+#pragma unroll
+                for (int c = 0; c < NCT; c++) {
+                    aq[kt][c] = M::mma(xd, u[c].x, aq[kt][c]);
+                    ar[kt][c] = M::mma(x[kt].y, u[c].x + u[c].y, ar[kt][c]);
+                    ai[kt][c] = M::mma(x[kt].x, u[c].y - u[c].x, ai[kt][c]);
+                }
+            }
+        }
+        if (ch + 1 < nch) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    // W' (NB x 64) to LDS, then W2 = T W' (T upper triangular: k' >= the wave's first row)
+    CT *ws = lds;
+#pragma unroll
+    for (int kt = 0; kt < NKT; kt++)
+#pragma unroll
+        for (int c = 0; c < NCT; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                ws[(w * KW + 16 * kt + M::row(lane, v)) * WP + 16 * c + li] =
+                    CT{aq[kt][c][v] + ar[kt][c][v], aq[kt][c][v] + ai[kt][c][v]};
+    __syncthreads();
+    const CT *Tp = reinterpret_cast<const CT *>(Tbig + 2 * b * strideTb) + (long)p * NB * NB;
+#pragma unroll
+    for (int kt = 0; kt < NKT; kt++)
+#pragma unroll
+        for (int c = 0; c < NCT; c++) aq[kt][c] = ar[kt][c] = ai[kt][c] = z4;
+    for (int k4 = w * KW; k4 < NB; k4 += 4) {
+        const int kk = k4 + lq;
+        CT tv[NKT], wv[NCT];
+#pragma unroll
+        for (int kt = 0; kt < NKT; kt++) tv[kt] = Tp[(long)kk * NB + w * KW + 16 * kt + li];    // T: A[k][k']
+#pragma unroll
+        for (int c = 0; c < NCT; c++) wv[c] = ws[kk * WP + 16 * c + li];                       // W': B[k'][col]
+#pragma unroll
+        for (int kt = 0; kt < NKT; kt++) {
+            const T ts = tv[kt].x + tv[kt].y;
+#pragma unroll
+            for (int c = 0; c < NCT; c++) {
+                aq[kt][c] = M::mma(ts, wv[c].x, aq[kt][c]);
+                ar[kt][c] = M::mma(-tv[kt].y, wv[c].x + wv[c].y, ar[kt][c]);
+                ai[kt][c] = M::mma(tv[kt].x, wv[c].y - wv[c].x, ai[kt][c]);
+            }
+        }
+    }
+    T *W2b = W2 + 2 * b * strideW2;
+#pragma unroll
+    for (int kt = 0; kt < NKT; kt++)
+#pragma unroll
+        for (int c = 0; c < NCT; c++) {
+            const int col = c0 + 16 * c + li;
+            if (col < n) {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    stx(W2b, (long)(w * KW + 16 * kt + M::row(lane, v)) * n + col,
+                        CT{aq[kt][c][v] + ar[kt][c][v], aq[kt][c][v] + ai[kt][c][v]});
+            }
+        }
+}
+
+// U -= V W2 for pass p on the rows j0 + 1 .. n - 1: workgroup (matrix b, row tile rt, column
+// tile ct) updates rows j0 + 1 + 4 WR rt .. (wave w: WR rows) x 64 columns, W2's 64-column
+// slice in LDS, V read directly (16 consecutive rows of a reflector per load), the next
+// four K-steps' V loaded ahead. U enters and leaves in accumulator layout.
+template <typename T, int NB, int WR>
+__global__ __launch_bounds__(256, 2) void lgbt_update_kernel(const T *__restrict__ Abase, long strideA, int n,
+                                                             const T *__restrict__ W2, long strideW2,
+                                                             T *__restrict__ Uout, long strideU, int p, int nrt,
+                                                             int nct, int batch) {
+    using M = Mf<T>;
+    using v4 = typename M::v4;
+    using CT = cx<T>;
+    constexpr int CW = 64, NCT = CW / 16, NRT = WR / 16, RT = 4 * WR, WP = CW + 1, NG = NB / 16;
+    __shared__ CT ws[NB * WP];
+    const int per = nrt * nct;
+    const long b = (blockIdx.x & 7) + 8L * (blockIdx.x / (8 * per));
+    const int q = (int)((blockIdx.x >> 3) % per), rt = q / nct, ct = q % nct;
+    if (b >= batch) return;                  // whole workgroup
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(t / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const T *A = Abase + 2 * b * strideA;
+    T *U = Uout + 2 * b * strideU;
+    const int j0 = p * NB, c0 = ct * CW;
+    const int rb = j0 + 1 + rt * RT + w * WR;
+    const CT *W2b = reinterpret_cast<const CT *>(W2 + 2 * b * strideW2);
+    for (int e = t; e < NB * CW; e += 256) {
+        const int k = e / CW, cc = e % CW;
+        ws[k * WP + cc] = (c0 + cc < n) ? W2b[(long)k * n + c0 + cc] : CT{(T)0, (T)0};
+    }
+    const v4 z4 = {0, 0, 0, 0};
+    v4 uq[NRT][NCT], ur[NRT][NCT], ui[NRT][NCT];
+#pragma unroll
+    for (int r2 = 0; r2 < NRT; r2++)
+#pragma unroll
+        for (int c = 0; c < NCT; c++) {
+            uq[r2][c] = z4;
+            const int col = c0 + 16 * c + li;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rb + 16 * r2 + M::row(lane, v);
+                const CT x = (row < n && col < n) ? ldx(U, (long)row * n + col) : CT{(T)0, (T)0};
+                ur[r2][c][v] = x.x;
+                ui[r2][c][v] = x.y;
+            }
+        }
+    CT xn[NRT][4];
+#pragma unroll
+    for (int r2 = 0; r2 < NRT; r2++)
+#pragma unroll
+        for (int s = 0; s < 4; s++) xn[r2][s] = lg_refl(A, n, j0, rb + 16 * r2 + li, 4 * s + lq);
+    __syncthreads();
+    for (int g = 0; g < NG; g++) {
+        CT xc[NRT][4];
+#pragma unroll
+        for (int r2 = 0; r2 < NRT; r2++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) xc[r2][s] = xn[r2][s];
+        if (g + 1 < NG) {
+#pragma unroll
+            for (int r2 = 0; r2 < NRT; r2++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) xn[r2][s] = lg_refl(A, n, j0, rb + 16 * r2 + li, 16 * (g + 1) + 4 * s + lq);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int kk = 16 * g + 4 * s + lq;
+            CT wv[NCT];
+#pragma unroll
+            for (int c = 0; c < NCT; c++) wv[c] = ws[kk * WP + 16 * c + li];             // W2: B[k][col]
+#pragma unroll
+            for (int r2 = 0; r2 < NRT; r2++) {
+                const CT x = xc[r2][s];                                                  // V: A[row][k]
+                const T xs = x.x + x.y;
+#pragma unroll
+                for (int c = 0; c < NCT; c++) {
+                    uq[r2][c] = M::mma(xs, wv[c].x, uq[r2][c]);
+                    ur[r2][c] = M::mma(x.y, wv[c].x + wv[c].y, ur[r2][c]);
+                    ui[r2][c] = M::mma(-x.x, wv[c].y - wv[c].x, ui[r2][c]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r2 = 0; r2 < NRT; r2++)
+#pragma unroll
+        for (int c = 0; c < NCT; c++) {
+            const int col = c0 + 16 * c + li;
+            if (col >= n) continue;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rb + 16 * r2 + M::row(lane, v);
+                if (row < n) stx(U, (long)row * n + col, CT{ur[r2][c][v] - uq[r2][c][v], ui[r2][c][v] - uq[r2][c][v]});
+            }
+        }
+}
+
+// Per-matrix workspace of the pass back-transform, in elements of T: the pass factors
+// (nbb NB x NB complex) and W2 (NB x n complex).
+template <typename T>
+constexpr int lgbt_nb() { return sizeof(T) == sizeof(float) ? 128 : 64; }
+template <typename T>
+static long lgbt_work_elems(int n) {
+    constexpr int NB = lgbt_nb<T>();
+    const long nbb = (n - 1 + NB - 1) / NB;
+    return 2L * nbb * NB * NB + 2L * NB * n;
+}
+
+// The pass back-transform over `batch` matrices, in groups that fit `cap_elems` elements of
+// T of workspace at `work`.
+template <typename T>
+static int launch_bt_gemm(const T *A, long strideA, int n, const T *tws, long strideT, T *U, long strideU,
+                          int batch, T *work, long cap_elems, hipStream_t s) {
+    constexpr int NB = lgbt_nb<T>(), WR = sizeof(T) == sizeof(float) ? 32 : 16, CW = 64;
+    const long per = lgbt_work_elems<T>(n);
+    const int nbb = (n - 1 + NB - 1) / NB;
+    const int nct = (n + CW - 1) / CW;
+    const long mg = std::min<long>(batch, cap_elems / per);
+    if (mg < 1) return -12;
+    const long strideTb = (long)nbb * NB * NB, strideW2 = (long)NB * n;   // complex elements
+    for (long b0 = 0; b0 < batch; b0 += mg) {
+        const int nbm = (int)std::min<long>(mg, batch - b0);
+        const T *Ab = A + 2 * b0 * strideA;
+        const T *tb = tws + 2 * b0 * strideT;
+        T *Ub = U + 2 * b0 * strideU;
+        T *Tbig = work, *W2 = work + 2L * nbm * strideTb;
+        hipLaunchKernelGGL((lgbt_tbig_kernel<T, NB>), dim3((unsigned)((long)nbm * nbb)), dim3(256), 0, s, Ab, strideA,
+                           n, tb, strideT, Tbig, strideTb, nbb, nbm);
+        const long groups = (nbm + 7) / 8;
+        for (int p = nbb - 1; p >= 0; p--) {
+            const int rows = n - (p * NB + 1);
+            if (rows <= 0) continue;
+            hipLaunchKernelGGL((lgbt_w_kernel<T, NB>), dim3((unsigned)(groups * 8 * nct)), dim3(256), 0, s, Ab, strideA,
+                               n, Tbig, strideTb, Ub, strideU, W2, strideW2, p, nct, nbm);
+            const int nrt = (rows + 4 * WR - 1) / (4 * WR);
+            hipLaunchKernelGGL((lgbt_update_kernel<T, NB, WR>), dim3((unsigned)(groups * 8 * nrt * nct)), dim3(256), 0,
+                               s, Ab, strideA, n, W2, strideW2, Ub, strideU, p, nrt, nct, nbm);
+        }
+        RLM_HIP_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+// RLM_LG_BT=slab selects the slab kernel for n > 512 (comparison); default: passes.
+static bool lg_bt_slab() {
+    const char *e = getenv("RLM_LG_BT");
+    return e && std::string(e) == "slab";
+}
+
+// ----------------------------------------------------------------------------
 // host
 template <typename T>
 static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, long strideV, T *aux, double *W,
@@ -968,6 +1346,11 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
     const long strideT = backtransform_tws_complex(n);
     const int nblk = (n - 1 + 15) / 16;
     if ((rc = launch_larft(A, strideA, n, n, TAU, strideV, tws, strideT, nb, (nblk + 3) / 4, s))) return rc;
+    if (!lg_bt_slab()) {
+        // the divide & conquer workspace is free now: the pass factors and W2 live there
+        const long cap = (long)grid_dc * stedc_global_work_doubles(n) * (long)(sizeof(double) / sizeof(T));
+        return launch_bt_gemm<T>(A, strideA, n, tws, strideT, U, strideA, nb, reinterpret_cast<T *>(work), cap, s);
+    }
     // 32 columns per slab for complex float, 16 for complex double (the W reduction
     // buffers then stay at 32 KB of LDS)
     constexpr int NCB = sizeof(T) == sizeof(float) ? 2 : 1;
