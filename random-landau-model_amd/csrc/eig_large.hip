@@ -933,8 +933,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
 // once per NB reflectors (the slab kernel above: read and written once per 32).
 
 // T of pass p of matrix b, column-major (Tbig[k' NB + k] = T(k, k')); one workgroup per
-// (matrix, pass). G_ml = V_m^H V_l (m < l) on the matrix cores, the block recurrence on the
-// VALU in LDS.
+// (matrix, pass). G_ml = V_m^H V_l (m < l) on the matrix cores over the rows j0 + 1 .. n - 1
+// in chunks of 16 (V staged through LDS, the next chunk loaded during the current one's
+// products; wave w the pairs w, w + 4, ..), then the block recurrence on the VALU in LDS.
 template <typename T, int NB>
 __global__ __launch_bounds__(256) void lgbt_tbig_kernel(const T *__restrict__ Abase, long strideA, int n,
                                                         const T *__restrict__ Tin, long strideT,
@@ -943,12 +944,18 @@ __global__ __launch_bounds__(256) void lgbt_tbig_kernel(const T *__restrict__ Ab
     using v4 = typename M::v4;
     using CT = cx<T>;
     constexpr int NS = NB / 16, NG = NS * (NS - 1) / 2, NTB = NS * (NS + 1) / 2, LP = 17;
-    __shared__ CT gsh[NG][16 * LP];          // G_ml, m < l, at l (l - 1) / 2 + m
-    __shared__ CT tsh[NTB][16 * LP];         // T_il, i <= l, at l (l + 1) / 2 + i
-    __shared__ CT xsh[NS - 1][16 * LP];      // X_m = G_ml T_ll of the current l
+    constexpr int NGW = (NG + 3) / 4;                  // pairs per wave
+    constexpr int KC = 16, VP = NB + 1, NVL = KC * NB / 256;
+    constexpr int STG = 2 * KC * VP;                   // double-buffered chunk of V
+    constexpr int R0 = STG > NG * 16 * LP ? STG : NG * 16 * LP;
+    __shared__ CT sm[R0 + (NTB + NS - 1) * 16 * LP];
+    CT *vst = sm;                                      // phase A: V chunks
+    CT *gsh = sm;                                      // then G_ml, m < l, at l (l - 1) / 2 + m
+    CT *tsh = sm + R0;                                 // T_il, i <= l, at l (l + 1) / 2 + i
+    CT *xsh = tsh + NTB * 16 * LP;                     // X_m = G_ml T_ll of the current l
     const long b = blockIdx.x / nbb;
     const int p = (int)(blockIdx.x % nbb);
-    if (b >= batch) return;                  // whole workgroup
+    if (b >= batch) return;                            // whole workgroup
     const int t = threadIdx.x, lane = t & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(t / kWave);
     const int li = lane & 15, lq = lane >> 4;
@@ -956,53 +963,98 @@ __global__ __launch_bounds__(256) void lgbt_tbig_kernel(const T *__restrict__ Ab
     const CT *T16 = reinterpret_cast<const CT *>(Tin + 2 * b * strideT);
     CT *Tb = reinterpret_cast<CT *>(Tbig + 2 * b * strideTb) + (long)p * NB * NB;
     const int j0 = p * NB, nblk = (n - 1 + 15) / 16;
-    for (int pi = w; pi < NG; pi += 4) {
+    int pm[NGW], pl[NGW];
+#pragma unroll
+    for (int q = 0; q < NGW; q++) {
+        const int pi = w + 4 * q;
         int l = 1;
         while (l * (l + 1) / 2 <= pi) l++;
-        const int m = pi - l * (l - 1) / 2;
-        v4 qq = {0, 0, 0, 0}, rr = {0, 0, 0, 0}, ii = {0, 0, 0, 0};
-        // rows below j0 + 16 l + 1 are zero in V_l
-        for (int r4 = (j0 + 16 * l + 1) & ~3; r4 < n; r4 += 4) {
-            const CT x = lg_refl(A, n, j0, r4 + lq, 16 * m + li);    // A operand: conj V_m[row][k = li]
-            const CT y = lg_refl(A, n, j0, r4 + lq, 16 * l + li);    // B operand: V_l[row][k = li]
-            qq = M::mma(x.x - x.y, y.x, qq);
-            rr = M::mma(x.y, y.x + y.y, rr);
-            ii = M::mma(x.x, y.y - y.x, ii);
-        }
+        pl[q] = pi < NG ? l : 1;                       // a spare slot computes pair (0, 1) unused
+        pm[q] = pi < NG ? pi - l * (l - 1) / 2 : 0;
+    }
+    const v4 z4 = {0, 0, 0, 0};
+    v4 gq[NGW], gr[NGW], gi[NGW];
 #pragma unroll
-        for (int v = 0; v < 4; v++) gsh[pi][M::row(lane, v) * LP + li] = CT{qq[v] + rr[v], qq[v] + ii[v]};
+    for (int q = 0; q < NGW; q++) gq[q] = gr[q] = gi[q] = z4;
+    const int rs = j0 + 1, nch = (n - rs + KC - 1) / KC;
+    CT vreg[NVL];
+    auto gload = [&](int ch) {
+#pragma unroll
+        for (int it = 0; it < NVL; it++) {
+            const int e = t + 256 * it;
+            vreg[it] = lg_refl(A, n, j0, rs + ch * KC + e % KC, e / KC);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < NVL; it++) {
+            const int e = t + 256 * it;
+            vst[buf * KC * VP + (e % KC) * VP + e / KC] = vreg[it];
+        }
+    };
+    if (nch > 0) {
+        gload(0);
+        sstore(0);
     }
     for (int e = t; e < NS * 256; e += 256) {
         const int l = e >> 8, k2 = (e >> 4) & 15, k1 = e & 15;
         const int bk = j0 / 16 + l;
-        tsh[l * (l + 1) / 2 + l][k1 * LP + k2] = bk < nblk ? T16[(long)bk * 256 + k2 * 16 + k1] : CT{(T)0, (T)0};
+        tsh[(l * (l + 1) / 2 + l) * 16 * LP + k1 * LP + k2] =
+            bk < nblk ? T16[(long)bk * 256 + k2 * 16 + k1] : CT{(T)0, (T)0};
+    }
+    __syncthreads();
+    for (int ch = 0; ch < nch; ch++) {
+        const int cur = ch & 1;
+        if (ch + 1 < nch) gload(ch + 1);
+        const CT *vs = vst + cur * KC * VP;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int rr = 4 * s + lq;
+#pragma unroll
+            for (int q = 0; q < NGW; q++) {
+                const CT x = vs[rr * VP + 16 * pm[q] + li];    // A operand: conj V_m[row][k = li]
+                const CT y = vs[rr * VP + 16 * pl[q] + li];    // B operand: V_l[row][k = li]
+                gq[q] = M::mma(x.x - x.y, y.x, gq[q]);
+                gr[q] = M::mma(x.y, y.x + y.y, gr[q]);
+                gi[q] = M::mma(x.x, y.y - y.x, gi[q]);
+            }
+        }
+        if (ch + 1 < nch) sstore(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NGW; q++) {
+        if (w + 4 * q >= NG) continue;
+        CT *g = gsh + (pl[q] * (pl[q] - 1) / 2 + pm[q]) * 16 * LP;
+#pragma unroll
+        for (int v = 0; v < 4; v++) g[M::row(lane, v) * LP + li] = CT{gq[q][v] + gr[q][v], gq[q][v] + gi[q][v]};
     }
     __syncthreads();
     const int er = t >> 4, ec = t & 15;      // element (er, ec) of a 16 x 16 block
     for (int l = 1; l < NS; l++) {
-        const CT *tll = tsh[l * (l + 1) / 2 + l];
+        const CT *tll = tsh + (l * (l + 1) / 2 + l) * 16 * LP;
         for (int m = 0; m < l; m++) {
-            const CT *g = gsh[l * (l - 1) / 2 + m];
+            const CT *g = gsh + (l * (l - 1) / 2 + m) * 16 * LP;
             CT acc{(T)0, (T)0};
 #pragma unroll
             for (int q = 0; q < 16; q++) fmax_(acc, g[er * LP + q], tll[q * LP + ec]);
-            xsh[m][er * LP + ec] = acc;
+            xsh[m * 16 * LP + er * LP + ec] = acc;
         }
         __syncthreads();
         for (int i = 0; i < l; i++) {
             CT acc{(T)0, (T)0};
             for (int m = i; m < l; m++) {
-                const CT *tim = tsh[m * (m + 1) / 2 + i];
+                const CT *tim = tsh + (m * (m + 1) / 2 + i) * 16 * LP;
 #pragma unroll
-                for (int q = 0; q < 16; q++) fmax_(acc, tim[er * LP + q], xsh[m][q * LP + ec]);
+                for (int q = 0; q < 16; q++) fmax_(acc, tim[er * LP + q], xsh[m * 16 * LP + q * LP + ec]);
             }
-            tsh[l * (l + 1) / 2 + i][er * LP + ec] = CT{-acc.x, -acc.y};
+            tsh[(l * (l + 1) / 2 + i) * 16 * LP + er * LP + ec] = CT{-acc.x, -acc.y};
         }
         __syncthreads();
     }
     for (int e = t; e < NB * NB; e += 256) {
         const int kc = e / NB, kr = e % NB, i = kr >> 4, l = kc >> 4;
-        Tb[e] = (i <= l) ? tsh[l * (l + 1) / 2 + i][(kr & 15) * LP + (kc & 15)] : CT{(T)0, (T)0};
+        Tb[e] = (i <= l) ? tsh[(l * (l + 1) / 2 + i) * 16 * LP + (kr & 15) * LP + (kc & 15)] : CT{(T)0, (T)0};
     }
 }
 
@@ -1246,6 +1298,182 @@ __global__ __launch_bounds__(256, 2) void lgbt_update_kernel(const T *__restrict
         }
 }
 
+// All passes for one slab of 64 eigenvector columns per workgroup (512 threads): the slab
+// streams through in chunks of 32 rows; in each chunk the waves first apply the current
+// pass p (U -= V_p W2_p, wave w the 16 x 16 tile (w >> 2, w & 3), K = NB), then
+// accumulate the next pass's W' = V_{p-1}^H U from the updated rows (wave w: k tile
+// w % (NB/16), its column tiles), so U is read and written once per NB reflectors. At
+// the end of a pass W2_{p-1} = T_{p-1} W' replaces W2_p in LDS. The next chunk's U rows
+// and V operands are loaded while the current chunk's W' products run (staging V_p's rows
+// in LDS instead, shared by the four waves of a row tile, measured slower: 163 -> 178 ms
+// at n = 3000, 64 matrices). Pass nbb is a
+// virtual one that only accumulates W' of the last pass. The slabs of one matrix share
+// blockIdx % 8 (one XCD, whose L2 serves V and T).
+template <typename T, int NB>
+__global__ __launch_bounds__(512) void lgbt_slab_kernel(const T *__restrict__ Abase, long strideA, int n,
+                                                        const T *__restrict__ Tbig, long strideTb,
+                                                        T *__restrict__ Uio, long strideU, int nslab, int batch) {
+    using M = Mf<T>;
+    using v4 = typename M::v4;
+    using CT = cx<T>;
+    constexpr int CW = 64, RC = 32, WP = CW + 1, UP = CW + 1;
+    constexpr int NKB = NB / 16;                 // k tiles of a pass
+    constexpr int NCW = 4 * NKB / 8;             // W' column tiles per wave
+    constexpr int KS = NB / 4;                   // K-steps of the update
+    constexpr int WS = RC / 4;                   // K-steps of the W' accumulation per chunk
+    constexpr int NUL = RC * CW / 512;           // U elements per thread per chunk
+    __shared__ CT wsh[NB * WP];                  // W2 of the pass being applied; W' at a pass end
+    __shared__ CT ush[2][RC * UP];               // the chunk's U rows (double buffer)
+    const long b = (blockIdx.x & 7) + 8L * (blockIdx.x / (8 * nslab));
+    const int slab = (int)((blockIdx.x >> 3) % nslab);
+    if (b >= batch) return;                      // whole workgroup
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(t / kWave);
+    const int li = lane & 15, lq = lane >> 4;
+    const int utr = w >> 2, utc = w & 3;         // update tile
+    const int wk = w % NKB, wc0 = (w / NKB) * NCW;
+    const T *A = Abase + 2 * b * strideA;
+    T *U = Uio + 2 * b * strideU;
+    const CT *Tm = reinterpret_cast<const CT *>(Tbig + 2 * b * strideTb);
+    const int c0 = slab * CW;
+    const int nbb = (n - 1 + NB - 1) / NB;
+    auto rlo = [&](int pp) { return pp >= 1 ? (pp - 1) * NB + 1 : 1; };
+    auto nchunks = [&](int pp) { return (n - rlo(pp) + RC - 1) / RC; };
+    // step (pp, cc) applies pass pp: some of its rows are at or below the pass's first row
+    auto upd = [&](int pp, int cc) { return pp < nbb && rlo(pp) + cc * RC + RC - 1 >= pp * NB + 1; };
+    const v4 z4 = {0, 0, 0, 0};
+    CT xp[KS], xw[WS], xwn[WS], un[NUL];
+    auto load_step = [&](int pp, int cc) {       // V operands and U rows of step (pp, cc)
+        const int r0 = rlo(pp) + cc * RC;
+        if (upd(pp, cc)) {
+#pragma unroll
+            for (int s = 0; s < KS; s++) xp[s] = lg_refl(A, n, pp * NB, r0 + 16 * utr + li, 4 * s + lq);
+        }
+        if (pp >= 1) {
+#pragma unroll
+            for (int s = 0; s < WS; s++) xwn[s] = lg_refl(A, n, (pp - 1) * NB, r0 + 4 * s + lq, 16 * wk + li);
+        }
+#pragma unroll
+        for (int it = 0; it < NUL; it++) {
+            const int e = t + 512 * it, rr = e / CW, cc2 = e % CW;
+            const int row = r0 + rr, col = c0 + cc2;
+            un[it] = (row < n && col < n) ? ldx(U, (long)row * n + col) : CT{(T)0, (T)0};
+        }
+    };
+    auto store_u = [&](CT *us) {
+#pragma unroll
+        for (int it = 0; it < NUL; it++) {
+            const int e = t + 512 * it;
+            us[(e / CW) * UP + e % CW] = un[it];
+        }
+    };
+    v4 wq[NCW], wr[NCW], wi[NCW];                // W' partial sums of the wave's tiles
+#pragma unroll
+    for (int c = 0; c < NCW; c++) wq[c] = wr[c] = wi[c] = z4;
+    load_step(nbb, 0);
+    store_u(ush[0]);
+#pragma unroll
+    for (int s = 0; s < WS; s++) xw[s] = xwn[s];
+    __syncthreads();
+    int p = nbb, ch = 0, cur = 0;
+    while (true) {
+        const int r0 = rlo(p) + ch * RC;
+        int pn = p, cn = ch + 1;
+        if (cn >= nchunks(p)) {
+            pn = p - 1;
+            cn = 0;
+        }
+        const bool has_next = pn >= 0;
+        CT *us = ush[cur];
+        // U -= V_p W2_p on the wave's tile (rows below the pass's first row are unchanged)
+        if (upd(p, ch)) {
+            v4 aq = z4, ar, ai;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const CT x = us[(16 * utr + M::row(lane, v)) * UP + 16 * utc + li];
+                ar[v] = x.x;
+                ai[v] = x.y;
+            }
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                const CT x = xp[s];                                          // V_p: A[row][k]
+                const CT wv = wsh[(4 * s + lq) * WP + 16 * utc + li];        // W2_p: B[k][col]
+                aq = M::mma(x.x + x.y, wv.x, aq);
+                ar = M::mma(x.y, wv.x + wv.y, ar);
+                ai = M::mma(-x.x, wv.y - wv.x, ai);
+            }
+            const int col = c0 + 16 * utc + li;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rl = 16 * utr + M::row(lane, v);
+                const CT y{ar[v] - aq[v], ai[v] - aq[v]};
+                us[rl * UP + 16 * utc + li] = y;
+                if (r0 + rl < n && col < n) stx(U, (long)(r0 + rl) * n + col, y);
+            }
+        }
+        __syncthreads();
+        if (p >= 1) {
+#pragma unroll
+            for (int s = 0; s < WS; s++) xw[s] = xwn[s];
+        }
+        if (has_next) load_step(pn, cn);
+        // W' += V_{p-1}^H U over the chunk's (updated) rows
+        if (p >= 1) {
+#pragma unroll
+            for (int s = 0; s < WS; s++) {
+                const CT x = xw[s];                                          // conj V_{p-1}: A[k][row]
+                const T xd = x.x - x.y;
+#pragma unroll
+                for (int c = 0; c < NCW; c++) {
+                    const CT u = us[(4 * s + lq) * UP + 16 * (wc0 + c) + li];   // U: B[row][col]
+                    wq[c] = M::mma(xd, u.x, wq[c]);
+                    wr[c] = M::mma(x.y, u.x + u.y, wr[c]);
+                    wi[c] = M::mma(x.x, u.y - u.x, wi[c]);
+                }
+            }
+        }
+        if (has_next) store_u(ush[cur ^ 1]);
+        if (pn != p && p >= 1) {
+            // end of pass p: W2_{p-1} = T_{p-1} W' replaces W2_p
+#pragma unroll
+            for (int c = 0; c < NCW; c++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    wsh[(16 * wk + M::row(lane, v)) * WP + 16 * (wc0 + c) + li] =
+                        CT{wq[c][v] + wr[c][v], wq[c][v] + wi[c][v]};
+            __syncthreads();
+            const CT *Tp = Tm + (long)(p - 1) * NB * NB;
+#pragma unroll
+            for (int c = 0; c < NCW; c++) wq[c] = wr[c] = wi[c] = z4;
+            for (int s = 4 * wk; s < KS; s++) {                              // T upper triangular
+                const CT tv = Tp[(long)(4 * s + lq) * NB + 16 * wk + li];    // T: A[k][k']
+                const T ts = tv.x + tv.y;
+#pragma unroll
+                for (int c = 0; c < NCW; c++) {
+                    const CT wv = wsh[(4 * s + lq) * WP + 16 * (wc0 + c) + li];   // W': B[k'][col]
+                    wq[c] = M::mma(ts, wv.x, wq[c]);
+                    wr[c] = M::mma(-tv.y, wv.x + wv.y, wr[c]);
+                    wi[c] = M::mma(tv.x, wv.y - wv.x, wi[c]);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < NCW; c++) {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    wsh[(16 * wk + M::row(lane, v)) * WP + 16 * (wc0 + c) + li] =
+                        CT{wq[c][v] + wr[c][v], wq[c][v] + wi[c][v]};
+                wq[c] = wr[c] = wi[c] = z4;
+            }
+        }
+        __syncthreads();
+        if (!has_next) break;
+        p = pn;
+        ch = cn;
+        cur ^= 1;
+    }
+}
+
 // Per-matrix workspace of the pass back-transform, in elements of T: the pass factors
 // (nbb NB x NB complex) and W2 (NB x n complex).
 template <typename T>
@@ -1258,17 +1486,19 @@ static long lgbt_work_elems(int n) {
 }
 
 // The pass back-transform over `batch` matrices, in groups that fit `cap_elems` elements of
-// T of workspace at `work`.
+// T of workspace at `work`: the pass factors, then either one launch of the slab kernel
+// (fused = true) or two launches per pass (lgbt_w_kernel, lgbt_update_kernel; W2 in the
+// workspace after the factors).
 template <typename T>
 static int launch_bt_gemm(const T *A, long strideA, int n, const T *tws, long strideT, T *U, long strideU,
-                          int batch, T *work, long cap_elems, hipStream_t s) {
+                          int batch, T *work, long cap_elems, bool fused, hipStream_t s) {
     constexpr int NB = lgbt_nb<T>(), WR = sizeof(T) == sizeof(float) ? 32 : 16, CW = 64;
-    const long per = lgbt_work_elems<T>(n);
     const int nbb = (n - 1 + NB - 1) / NB;
     const int nct = (n + CW - 1) / CW;
+    const long strideTb = (long)nbb * NB * NB, strideW2 = (long)NB * n;   // complex elements
+    const long per = fused ? 2L * strideTb : lgbt_work_elems<T>(n);
     const long mg = std::min<long>(batch, cap_elems / per);
     if (mg < 1) return -12;
-    const long strideTb = (long)nbb * NB * NB, strideW2 = (long)NB * n;   // complex elements
     for (long b0 = 0; b0 < batch; b0 += mg) {
         const int nbm = (int)std::min<long>(mg, batch - b0);
         const T *Ab = A + 2 * b0 * strideA;
@@ -1278,6 +1508,12 @@ static int launch_bt_gemm(const T *A, long strideA, int n, const T *tws, long st
         hipLaunchKernelGGL((lgbt_tbig_kernel<T, NB>), dim3((unsigned)((long)nbm * nbb)), dim3(256), 0, s, Ab, strideA,
                            n, tb, strideT, Tbig, strideTb, nbb, nbm);
         const long groups = (nbm + 7) / 8;
+        if (fused) {
+            hipLaunchKernelGGL((lgbt_slab_kernel<T, NB>), dim3((unsigned)(groups * 8 * nct)), dim3(512), 0, s, Ab,
+                               strideA, n, Tbig, strideTb, Ub, strideU, nct, nbm);
+            RLM_HIP_CHECK(hipGetLastError());
+            continue;
+        }
         for (int p = nbb - 1; p >= 0; p--) {
             const int rows = n - (p * NB + 1);
             if (rows <= 0) continue;
@@ -1292,10 +1528,13 @@ static int launch_bt_gemm(const T *A, long strideA, int n, const T *tws, long st
     return 0;
 }
 
-// RLM_LG_BT=slab selects the slab kernel for n > 512 (comparison); default: passes.
-static bool lg_bt_slab() {
+// Back-transform variant for n > 512 (RLM_LG_BT): "slab" the two-block slab kernel, "passes"
+// two launches per 128-reflector pass, default the fused slab kernel over all passes.
+static int lg_bt_mode() {
     const char *e = getenv("RLM_LG_BT");
-    return e && std::string(e) == "slab";
+    if (e && std::string(e) == "slab") return 0;
+    if (e && std::string(e) == "passes") return 1;
+    return 2;
 }
 
 // ----------------------------------------------------------------------------
@@ -1346,10 +1585,11 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
     const long strideT = backtransform_tws_complex(n);
     const int nblk = (n - 1 + 15) / 16;
     if ((rc = launch_larft(A, strideA, n, n, TAU, strideV, tws, strideT, nb, (nblk + 3) / 4, s))) return rc;
-    if (!lg_bt_slab()) {
-        // the divide & conquer workspace is free now: the pass factors and W2 live there
+    if (const int mode = lg_bt_mode()) {
+        // the divide & conquer workspace is free now: the pass factors (and W2) live there
         const long cap = (long)grid_dc * stedc_global_work_doubles(n) * (long)(sizeof(double) / sizeof(T));
-        return launch_bt_gemm<T>(A, strideA, n, tws, strideT, U, strideA, nb, reinterpret_cast<T *>(work), cap, s);
+        return launch_bt_gemm<T>(A, strideA, n, tws, strideT, U, strideA, nb, reinterpret_cast<T *>(work), cap,
+                                 mode == 2, s);
     }
     // 32 columns per slab for complex float, 16 for complex double (the W reduction
     // buffers then stay at 32 KB of LDS)
