@@ -27,7 +27,7 @@ using namespace rlm;
 namespace {
 
 constexpr int kGridTrd = 1024;   // persistent workgroups for the tridiagonalisation
-constexpr int kGridTail = 512;   // persistent workgroups for D&C + back-transform
+constexpr int kGridTail = 768;   // persistent workgroups for D&C (three per CU) + back-transform
 
 // ---------------------------------------------------------------- small kernels
 // Elementwise kernels over whole matrices take nch workgroups per matrix

@@ -2364,8 +2364,11 @@ __device__ inline void dc_transpose_out(int n, const T *Z, T *U, double *tile) {
 // (n > 512, where they exceed the LDS); the LDS then holds only the transpose tile.
 // lev_stop > 0 (GSH only): stop before merge level lev_stop's GEMM (see stedc_block);
 // workgroup b then handles matrix b (grid = batch) and keeps its state in workspace slot b.
+// n <= 512: registers capped for three waves per SIMD (three workgroups per CU, a few
+// registers spilled): n_phi = 256, 32768 matrices 58 -> 51 ms (complex float output),
+// 78 -> 69 ms (complex double); two waves per SIMD measured no faster than one.
 template <typename T, int NT, bool GSH>
-__global__ __launch_bounds__(NT) void stedc_kernel(int n, const double *__restrict__ Din,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GSH ? 1 : 3))) void stedc_kernel(int n, const double *__restrict__ Din,
                                                    const double *__restrict__ Ein, long strideV,
                                                    double *__restrict__ Wout, long strideW, T *__restrict__ Uout,
                                                    long strideU, double *__restrict__ work, int batch,
