@@ -2233,8 +2233,12 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
             // with S[i][q] = what_i / (dl_i - lambda_q) * inrm_q generated per K-step,
             // S never stored. Below that, S goes through memory and the VALU path.
             const bool mfma = merging && G >= kWave && half >= 32;
+            // 16-row halves, two nodes per wave: each wave runs its nodes' products on the
+            // matrix cores one after the other (uniform over the workgroup); at n = 256 this
+            // level's product went from 11 % of the divide & conquer (VALU) to 1.4 %
+            const bool mfma_pair = G == kWave / 2 && half == 16;
             // P7: U (K x K) into S rows lo.., scaled by the inverse norms
-            if (merging && !mfma) {
+            if (merging && !mfma && !mfma_pair) {
                 const double *dl = sh.dl + lo;
                 for (int idx = lane; idx < K * K; idx += G) {
                     const int i = idx / K, q = idx % K;
@@ -2255,6 +2259,15 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                     merge_gemm_mfma<QS, 4, 2>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh, Uo);
                 else
                     merge_gemm_mfma<QS, 2, 4>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh, Uo);
+            } else if (mfma_pair) {
+                const int wv = t / kWave;
+                for (int kk = 0; kk < 2; kk++) {
+                    const int kn = nb0 + 2 * wv + kk;              // wave-uniform
+                    const int lo2 = kn * blk, mid2 = lo2 + half;
+                    if (kn >= nnodes || mid2 >= n) continue;       // no merge (copied below)
+                    merge_gemm_mfma<QS, 1, 4>(t & (kWave - 1), kWave, lo2, mid2, min(lo2 + blk, n), half,
+                                              sh.cnt[5 * kn], sh.cnt[5 * kn + 3], sh.cnt[5 * kn + 4], n, Qa, Qb, sh);
+                }
             } else if (merging) {
                 // VALU path: each task is one row and 4 consecutive output columns,
                 // consecutive lanes on consecutive rows
