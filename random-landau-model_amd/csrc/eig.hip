@@ -3202,7 +3202,8 @@ static bool variant_valid(int v, int n) {
         case kTrdSmall: return dbl && n <= 72;
         case kTrdFused: return n <= 512;
         case kTrdResident: return n <= resident_max_n(sizeof(T));
-        case kTrdHybrid: return !dbl && n > resident_max_n(sizeof(T)) && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024;
+        case kTrdHybrid: return n > resident_max_n(sizeof(T)) && n <= 512 &&
+                                lower_lds_bytes(n, dbl ? 8 : 16, sizeof(T)) <= 160 * 1024;
         case kTrdLower: return n <= 512 && lower_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024;
         case kTrdLower8: return n <= 512 && lower_lds_bytes(n, 8, sizeof(T)) <= 160 * 1024;
         case kTrdBlocked: return n <= 512 && blocked_lds_bytes(n, 16, sizeof(T)) <= 160 * 1024;
@@ -3240,8 +3241,13 @@ static int launch_hetrd_any(T *A, long strideA, int lda, int n, double *D, doubl
         // columns 0..ncols-1 by panels of 16 through memory; the updated trailing block
         // A(ncols:n, ncols:n) (at most rmax) is then a Hermitian matrix whose
         // reduction continues on chip, writing d, e, tau and the reflectors in place
+        // (complex double: panels of 8, as its lower variant, and a trailing 128 block: the
+        // autotuner takes it over lower8 for n = 192..512; n = 256, 32768 matrices: 584 ->
+        // 529 ms)
         const int ncols = ((n - rmax) + 15) / 16 * 16;
-        int rc = launch_zhetrd_lower_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream, ncols);
+        int rc = sizeof(T) == sizeof(double)
+                     ? launch_zhetrd_lower_t<T, 8>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream, ncols)
+                     : launch_zhetrd_lower_t<T, 16>(A, strideA, lda, n, D, E, TAU, strideV, batch, grid, stream, ncols);
         if (rc) return rc;
         return launch_resident_any<T>(A + 2 * ((long)ncols * lda + ncols), strideA, lda, n - ncols, D + ncols,
                                       E + ncols, TAU + 2 * ncols, strideV, batch, stream, nullptr, 0);
