@@ -14,7 +14,7 @@
 //     lg_symv_lower_kernel + lg_symv_reduce_kernel
 //                      y = A22 v - V (W^H v) - W (V^H v) from the lower triangle
 //                      in 32 x 32 tiles (row and column sums from one read),
-//                      blocks of 4 x 16 tiles per workgroup: the trailing lower
+//                      blocks of 4 x 16 (late columns 4 x 8) tiles per workgroup: the trailing lower
 //                      triangle is streamed once per column; one extra workgroup
 //                      per matrix forms W^H v, V^H v beside the tiles, and the
 //                      reduction's idle half-wave applies the older pending terms
@@ -99,10 +99,12 @@ int symv_timing(int enable, double *total_ms, long *launches) {
 // ns = strips of the first column.
 constexpr int kLgTS = 32;   // tile edge
 constexpr int kLgTR = 4;    // strips per row block (one per wave)
-constexpr int kLgTC = 16;   // tiles per column block
+constexpr int kLgTC = 16;   // tiles per column block (at most)
+constexpr int kLgTCmin = 8; // tiles per column block where the trailing matrix is small
 static long lg_strips(int n) { return (n + kLgTS - 1) / kLgTS; }
 __host__ __device__ inline long lg_row_blocks(long ns) { return (ns + kLgTR - 1) / kLgTR; }
-__host__ __device__ inline long lg_col_blocks(long ns) { return (ns + kLgTC - 1) / kLgTC; }
+// column blocks of a strip's row sums as allocated (the narrowest blocks)
+__host__ __device__ inline long lg_col_blocks(long ns) { return (ns + kLgTCmin - 1) / kLgTCmin; }
 long heevd_large_aux_complex(int n) {
     const long ns = lg_strips(n);
     return 2L * kLgNB * n + 2L * kLgNB + ns * (lg_row_blocks(ns) + lg_col_blocks(ns)) * kLgTS;
@@ -283,8 +285,9 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
 // in 32 x 32 tiles (I, J), I >= J, of the strips of rows base + 32 I .. (rows and columns
 // below s masked; base = s rounded down to a multiple of 8). Workgroup (b, ib, jb) takes
 // the tiles of strips 4 ib .. 4 ib + 3 (wave w: strip 4 ib + w) and tile columns
-// 16 jb .. 16 jb + 15, so the four waves read neighbouring 256-byte segments of the same
-// columns together. Lane (lrg, lcg) = (lane & 7, lane >> 3) holds the 4 x 4 sub-block at
+// tc jb .. tc jb + tc - 1 (tc = 16, or 8 for the small trailing matrices of the late
+// columns, where more, smaller workgroups balance the chip), so the four waves read
+// neighbouring 256-byte segments of the same columns together. Lane (lrg, lcg) = (lane & 7, lane >> 3) holds the 4 x 4 sub-block at
 // tile rows lrg + 8 ru (the 8 lanes of a column group read 8 consecutive rows of the
 // column-major matrix per load), columns 4 lcg.., read once for both y_I += A_IJ v_J
 // (row sums: kept in registers over the wave's tiles, reduced over lcg once, into
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(kLgColNT) void lg_col_kernel(T *__restrict__ Abase,
 template <typename T>
 __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict__ Abase, long strideA, int n,
                                                             T *__restrict__ aux, long strideAux, int k0, int i,
-                                                            int per, int ns0, int nb) {
+                                                            int per, int ns0, int nb, int tc) {
     using C = cx<T>;
     // complex float issues the next tile's loads before this tile's sums (complex double
     // has no registers to spare for it and gains nothing)
@@ -341,13 +344,13 @@ __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict_
     const long b = (blockIdx.x - nb) / per;
     int q = (int)((blockIdx.x - nb) % per), ib = 0;
     for (; ib < nib - 1; ib++) {
-        const int nj = min(kLgTR * ib + kLgTR - 1, ns - 1) / kLgTC + 1;
+        const int nj = min(kLgTR * ib + kLgTR - 1, ns - 1) / tc + 1;
         if (q < nj) break;
         q -= nj;
     }
     const int jb = q;
     const int I = kLgTR * ib + w;
-    const int J0 = jb * kLgTC, J1 = min(J0 + kLgTC, min(kLgTR * ib + kLgTR - 1, ns - 1) + 1);
+    const int J0 = jb * tc, J1 = min(J0 + tc, min(kLgTR * ib + kLgTR - 1, ns - 1) + 1);
     const T *A = Abase + 2 * b * strideA;
     C *Vp = reinterpret_cast<C *>(aux + 2 * b * strideAux);
     const C *v = Vp + (long)i * n;                             // v(r) for rows r > c
@@ -456,7 +459,8 @@ __global__ __launch_bounds__(256) void lg_symv_lower_kernel(const T *__restrict_
 // order lg_col_kernel would apply them; lg_col_kernel then adds only the term l = i.
 template <typename T>
 __global__ __launch_bounds__(64) void lg_symv_reduce_kernel(T *__restrict__ Abase, long strideA, T *__restrict__ aux,
-                                                            long strideAux, int n, int k0, int i, int nbk, int ns0) {
+                                                            long strideAux, int n, int k0, int i, int nbk, int ns0,
+                                                            int tc) {
     using C = cx<T>;
     const int c = k0 + i, s0 = c + 1;
     const int base = s0 & ~7, mb = n - base;
@@ -483,7 +487,7 @@ __global__ __launch_bounds__(64) void lg_symv_reduce_kernel(T *__restrict__ Abas
         return;
     }
     // loads issued ahead of the fixed-order sums
-    const int njb = K / kLgTC + 1, ib0 = K / kLgTR;
+    const int njb = K / tc + 1, ib0 = K / kLgTR;
     C y{(T)0, (T)0};
     for (int jb = 0; jb < njb; jb++) y = y + Rp[((long)K * njbm + jb) * kLgTS + t];
     int ib = ib0;
@@ -1559,15 +1563,23 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
             const int mb = n - ((k0 + i + 1) & ~7);
             const int nsm = (mb + kLgTS - 1) / kLgTS, nib = (nsm + kLgTR - 1) / kLgTR;
             int per = 0;
-            for (int ib = 0; ib < nib; ib++) per += std::min(kLgTR * ib + kLgTR - 1, nsm - 1) / kLgTC + 1;
+            // column blocks of 16 tiles, or of 8 where 16 would leave fewer than about three
+            // rounds of workgroups on the chip (the last, partly empty round then costs less)
+            int tc = kLgTC;
+            for (int ib = 0; ib < nib; ib++) per += std::min(kLgTR * ib + kLgTR - 1, nsm - 1) / tc + 1;
+            if ((long)nb * per < 3072) {
+                tc = kLgTCmin;
+                per = 0;
+                for (int ib = 0; ib < nib; ib++) per += std::min(kLgTR * ib + kLgTR - 1, nsm - 1) / tc + 1;
+            }
             hipEvent_t te0, te1;
             kt_pair(s, te0, te1);
             if (te0) RLM_HIP_CHECK(hipEventRecord(te0, s));
             hipLaunchKernelGGL(lg_symv_lower_kernel<T>, dim3((unsigned)((long)nb * per + nb)), dim3(256), 0, s, A,
-                               strideA, n, aux, strideAux, k0, i, per, ns0, nb);
+                               strideA, n, aux, strideAux, k0, i, per, ns0, nb, tc);
             if (te1) RLM_HIP_CHECK(hipEventRecord(te1, s));
             hipLaunchKernelGGL(lg_symv_reduce_kernel<T>, dim3((unsigned)((long)nb * nsm)), dim3(64), 0, s, A, strideA,
-                               aux, strideAux, n, k0, i, nbk, ns0);
+                               aux, strideAux, n, k0, i, nbk, ns0, tc);
         }
         const int s0 = k0 + nbk;
         if (s0 < n - 1) {
