@@ -22,9 +22,11 @@
 //     lg_her2k_kernel  after each panel A22 -= V W^H + W V^H on the lower tiles, on
 //                      the matrix cores.
 //   Divide & conquer: stedc_kernel with its node arrays in global memory.
-//   Back-transform: larft_kernel with several workgroups per matrix, then
-//     lg_backtransform2_kernel: a slab of 32 eigenvector columns per workgroup,
-//     one read and one write of the slab per pair of 16-reflector blocks.
+//   Back-transform: larft_kernel with several workgroups per matrix, the factors of
+//     128-reflector passes (lgbt_tbig_kernel), then lgbt_slab_kernel: a slab of 64
+//     eigenvector columns per workgroup through every pass, one read and one write
+//     of the slab per pass (lg_backtransform2_kernel, two 16-reflector blocks per
+//     read and write, remains for complex double 256 < n <= 512).
 #include "common.hpp"
 #include "eig.hpp"
 
@@ -930,11 +932,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
 // Back-transform in passes of NB reflectors (128 for complex float, 64 for complex double):
 // Q_P = H_j0 ... H_{j0+NB-1} = I - V T V^H with T (NB x NB, upper triangular) combined from
 // the 16-reflector factors of larft_kernel by the product rule of two compact-WY blocks,
-// T_{0:l, l} = -T_{0:l,0:l} (V_{0:l}^H V_l) T_ll over the 16-column blocks l. Per pass,
-// last first: W2 = T (V^H U) (lgbt_w_kernel, one workgroup per 64 eigenvector columns,
-// the K sum over every row below j0 staged through LDS), then U -= V W2
-// (lgbt_update_kernel, tiles of 4 WR rows x 64 columns). U is read twice and written
-// once per NB reflectors (the slab kernel above: read and written once per 32).
+// T_{0:l, l} = -T_{0:l,0:l} (V_{0:l}^H V_l) T_ll over the 16-column blocks l
+// (lgbt_tbig_kernel); then every pass, last first, applies U -= V T (V^H U) to one slab of
+// U per workgroup (lgbt_slab_kernel), so U is read and written once per NB reflectors (the
+// two-block slab kernel above: once per 32). n_phi = 3000, 64 matrices: 248 ms (two-block
+// slabs) -> 192 ms (two launches per pass: W2 = T V^H U by column tiles, then U -= V W2
+// by 128 x 64 tiles; U read twice and written once per pass) -> 168 ms (one slab kernel)
+// in complex float, 433 -> 315 -> 293 ms in complex double.
 
 // T of pass p of matrix b, column-major (Tbig[k' NB + k] = T(k, k')); one workgroup per
 // (matrix, pass). G_ml = V_m^H V_l (m < l) on the matrix cores over the rows j0 + 1 .. n - 1
@@ -1060,246 +1064,6 @@ __global__ __launch_bounds__(256) void lgbt_tbig_kernel(const T *__restrict__ Ab
         const int kc = e / NB, kr = e % NB, i = kr >> 4, l = kc >> 4;
         Tb[e] = (i <= l) ? tsh[(l * (l + 1) / 2 + i) * 16 * LP + (kr & 15) * LP + (kc & 15)] : CT{(T)0, (T)0};
     }
-}
-
-// W2 = T (V^H U) for pass p: workgroup (matrix b, column tile ct) forms W' = V^H U(:, 64 ct ..)
-// over the rows j0 + 1 .. n - 1 in chunks of 16 rows (V and U staged through LDS, double
-// buffered: the next chunk's global loads issue before the current chunk's products), wave
-// w the rows k = w NB/4 .. of W'; then W2 = T W' from W' in LDS. The column tiles of one
-// matrix share blockIdx % 8 (one XCD, whose L2 serves V and T).
-template <typename T, int NB>
-__global__ __launch_bounds__(256, 2) void lgbt_w_kernel(const T *__restrict__ Abase, long strideA, int n,
-                                                        const T *__restrict__ Tbig, long strideTb,
-                                                        const T *__restrict__ Uin, long strideU, T *__restrict__ W2,
-                                                        long strideW2, int p, int nct, int batch) {
-    using M = Mf<T>;
-    using v4 = typename M::v4;
-    using CT = cx<T>;
-    constexpr int KW = NB / 4, NKT = KW / 16, CW = 64, NCT = CW / 16, KC = 16;
-    constexpr int VP = NB + 1, UP = CW + 1, WP = CW + 1;        // LDS row strides (complex)
-    constexpr int STG = KC * (VP + UP);
-    constexpr int LDSN = (2 * STG > NB * WP) ? 2 * STG : NB * WP;
-    constexpr int NVL = KC * NB / 256, NUL = KC * CW / 256;     // staged elements per thread
-    __shared__ CT lds[LDSN];
-    const long b = (blockIdx.x & 7) + 8L * (blockIdx.x / (8 * nct));
-    const int ct = (int)((blockIdx.x >> 3) % nct);
-    if (b >= batch) return;                  // whole workgroup
-    const int t = threadIdx.x, lane = t & (kWave - 1);
-    const int w = __builtin_amdgcn_readfirstlane(t / kWave);
-    const int li = lane & 15, lq = lane >> 4;
-    const T *A = Abase + 2 * b * strideA;
-    const T *U = Uin + 2 * b * strideU;
-    const int j0 = p * NB, rs = j0 + 1, c0 = ct * CW;
-    const int nch = (n - rs + KC - 1) / KC;
-    CT vreg[NVL], ureg[NUL];
-    auto gload = [&](int ch) {
-        const int r0 = rs + ch * KC;
-#pragma unroll
-        for (int it = 0; it < NVL; it++) {
-            const int e = t + 256 * it, k = e / KC, rr = e % KC;
-            vreg[it] = lg_refl(A, n, j0, r0 + rr, k);
-        }
-#pragma unroll
-        for (int it = 0; it < NUL; it++) {
-            const int e = t + 256 * it, rr = e / CW, cc = e % CW;
-            const int row = r0 + rr, col = c0 + cc;
-            ureg[it] = (row < n && col < n) ? ldx(U, (long)row * n + col) : CT{(T)0, (T)0};
-        }
-    };
-    auto sstore = [&](int buf) {
-        CT *vs = lds + buf * STG, *us = vs + KC * VP;
-#pragma unroll
-        for (int it = 0; it < NVL; it++) {
-            const int e = t + 256 * it, k = e / KC, rr = e % KC;
-            vs[rr * VP + k] = vreg[it];
-        }
-#pragma unroll
-        for (int it = 0; it < NUL; it++) {
-            const int e = t + 256 * it, rr = e / CW, cc = e % CW;
-            us[rr * UP + cc] = ureg[it];
-        }
-    };
-    const v4 z4 = {0, 0, 0, 0};
-    v4 aq[NKT][NCT], ar[NKT][NCT], ai[NKT][NCT];
-#pragma unroll
-    for (int kt = 0; kt < NKT; kt++)
-#pragma unroll
-        for (int c = 0; c < NCT; c++) aq[kt][c] = ar[kt][c] = ai[kt][c] = z4;
-    if (nch > 0) {
-        gload(0);
-        sstore(0);
-    }
-    __syncthreads();
-    for (int ch = 0; ch < nch; ch++) {
-        const int cur = ch & 1;
-        if (ch + 1 < nch) gload(ch + 1);
-        const CT *vs = lds + cur * STG, *us = vs + KC * VP;
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const int rr = 4 * s + lq;
-            CT x[NKT], u[NCT];
-#pragma unroll
-            for (int kt = 0; kt < NKT; kt++) x[kt] = vs[rr * VP + w * KW + 16 * kt + li];   // conj V: A[k][row]
-#pragma unroll
-            for (int c = 0; c < NCT; c++) u[c] = us[rr * UP + 16 * c + li];                  // U: B[row][col]
-#pragma unroll
-            for (int kt = 0; kt < NKT; kt++) {
-                const T xd = x[kt].x - x[kt].y;
-#pragma unroll
-                for (int c = 0; c < NCT; c++) {
-                    aq[kt][c] = M::mma(xd, u[c].x, aq[kt][c]);
-                    ar[kt][c] = M::mma(x[kt].y, u[c].x + u[c].y, ar[kt][c]);
-                    ai[kt][c] = M::mma(x[kt].x, u[c].y - u[c].x, ai[kt][c]);
-                }
-            }
-        }
-        if (ch + 1 < nch) sstore(cur ^ 1);
-        __syncthreads();
-    }
-    // W' (NB x 64) to LDS, then W2 = T W' (T upper triangular: k' >= the wave's first row)
-    CT *ws = lds;
-#pragma unroll
-    for (int kt = 0; kt < NKT; kt++)
-#pragma unroll
-        for (int c = 0; c < NCT; c++)
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                ws[(w * KW + 16 * kt + M::row(lane, v)) * WP + 16 * c + li] =
-                    CT{aq[kt][c][v] + ar[kt][c][v], aq[kt][c][v] + ai[kt][c][v]};
-    __syncthreads();
-    const CT *Tp = reinterpret_cast<const CT *>(Tbig + 2 * b * strideTb) + (long)p * NB * NB;
-#pragma unroll
-    for (int kt = 0; kt < NKT; kt++)
-#pragma unroll
-        for (int c = 0; c < NCT; c++) aq[kt][c] = ar[kt][c] = ai[kt][c] = z4;
-    for (int k4 = w * KW; k4 < NB; k4 += 4) {
-        const int kk = k4 + lq;
-        CT tv[NKT], wv[NCT];
-#pragma unroll
-        for (int kt = 0; kt < NKT; kt++) tv[kt] = Tp[(long)kk * NB + w * KW + 16 * kt + li];    // T: A[k][k']
-#pragma unroll
-        for (int c = 0; c < NCT; c++) wv[c] = ws[kk * WP + 16 * c + li];                       // W': B[k'][col]
-#pragma unroll
-        for (int kt = 0; kt < NKT; kt++) {
-            const T ts = tv[kt].x + tv[kt].y;
-#pragma unroll
-            for (int c = 0; c < NCT; c++) {
-                aq[kt][c] = M::mma(ts, wv[c].x, aq[kt][c]);
-                ar[kt][c] = M::mma(-tv[kt].y, wv[c].x + wv[c].y, ar[kt][c]);
-                ai[kt][c] = M::mma(tv[kt].x, wv[c].y - wv[c].x, ai[kt][c]);
-            }
-        }
-    }
-    T *W2b = W2 + 2 * b * strideW2;
-#pragma unroll
-    for (int kt = 0; kt < NKT; kt++)
-#pragma unroll
-        for (int c = 0; c < NCT; c++) {
-            const int col = c0 + 16 * c + li;
-            if (col < n) {
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    stx(W2b, (long)(w * KW + 16 * kt + M::row(lane, v)) * n + col,
-                        CT{aq[kt][c][v] + ar[kt][c][v], aq[kt][c][v] + ai[kt][c][v]});
-            }
-        }
-}
-
-// U -= V W2 for pass p on the rows j0 + 1 .. n - 1: workgroup (matrix b, row tile rt, column
-// tile ct) updates rows j0 + 1 + 4 WR rt .. (wave w: WR rows) x 64 columns, W2's 64-column
-// slice in LDS, V read directly (16 consecutive rows of a reflector per load), the next
-// four K-steps' V loaded ahead. U enters and leaves in accumulator layout.
-template <typename T, int NB, int WR>
-__global__ __launch_bounds__(256, 2) void lgbt_update_kernel(const T *__restrict__ Abase, long strideA, int n,
-                                                             const T *__restrict__ W2, long strideW2,
-                                                             T *__restrict__ Uout, long strideU, int p, int nrt,
-                                                             int nct, int batch) {
-    using M = Mf<T>;
-    using v4 = typename M::v4;
-    using CT = cx<T>;
-    constexpr int CW = 64, NCT = CW / 16, NRT = WR / 16, RT = 4 * WR, WP = CW + 1, NG = NB / 16;
-    __shared__ CT ws[NB * WP];
-    const int per = nrt * nct;
-    const long b = (blockIdx.x & 7) + 8L * (blockIdx.x / (8 * per));
-    const int q = (int)((blockIdx.x >> 3) % per), rt = q / nct, ct = q % nct;
-    if (b >= batch) return;                  // whole workgroup
-    const int t = threadIdx.x, lane = t & (kWave - 1);
-    const int w = __builtin_amdgcn_readfirstlane(t / kWave);
-    const int li = lane & 15, lq = lane >> 4;
-    const T *A = Abase + 2 * b * strideA;
-    T *U = Uout + 2 * b * strideU;
-    const int j0 = p * NB, c0 = ct * CW;
-    const int rb = j0 + 1 + rt * RT + w * WR;
-    const CT *W2b = reinterpret_cast<const CT *>(W2 + 2 * b * strideW2);
-    for (int e = t; e < NB * CW; e += 256) {
-        const int k = e / CW, cc = e % CW;
-        ws[k * WP + cc] = (c0 + cc < n) ? W2b[(long)k * n + c0 + cc] : CT{(T)0, (T)0};
-    }
-    const v4 z4 = {0, 0, 0, 0};
-    v4 uq[NRT][NCT], ur[NRT][NCT], ui[NRT][NCT];
-#pragma unroll
-    for (int r2 = 0; r2 < NRT; r2++)
-#pragma unroll
-        for (int c = 0; c < NCT; c++) {
-            uq[r2][c] = z4;
-            const int col = c0 + 16 * c + li;
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int row = rb + 16 * r2 + M::row(lane, v);
-                const CT x = (row < n && col < n) ? ldx(U, (long)row * n + col) : CT{(T)0, (T)0};
-                ur[r2][c][v] = x.x;
-                ui[r2][c][v] = x.y;
-            }
-        }
-    CT xn[NRT][4];
-#pragma unroll
-    for (int r2 = 0; r2 < NRT; r2++)
-#pragma unroll
-        for (int s = 0; s < 4; s++) xn[r2][s] = lg_refl(A, n, j0, rb + 16 * r2 + li, 4 * s + lq);
-    __syncthreads();
-    for (int g = 0; g < NG; g++) {
-        CT xc[NRT][4];
-#pragma unroll
-        for (int r2 = 0; r2 < NRT; r2++)
-#pragma unroll
-            for (int s = 0; s < 4; s++) xc[r2][s] = xn[r2][s];
-        if (g + 1 < NG) {
-#pragma unroll
-            for (int r2 = 0; r2 < NRT; r2++)
-#pragma unroll
-                for (int s = 0; s < 4; s++) xn[r2][s] = lg_refl(A, n, j0, rb + 16 * r2 + li, 16 * (g + 1) + 4 * s + lq);
-        }
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const int kk = 16 * g + 4 * s + lq;
-            CT wv[NCT];
-#pragma unroll
-            for (int c = 0; c < NCT; c++) wv[c] = ws[kk * WP + 16 * c + li];             // W2: B[k][col]
-#pragma unroll
-            for (int r2 = 0; r2 < NRT; r2++) {
-                const CT x = xc[r2][s];                                                  // V: A[row][k]
-                const T xs = x.x + x.y;
-#pragma unroll
-                for (int c = 0; c < NCT; c++) {
-                    uq[r2][c] = M::mma(xs, wv[c].x, uq[r2][c]);
-                    ur[r2][c] = M::mma(x.y, wv[c].x + wv[c].y, ur[r2][c]);
-                    ui[r2][c] = M::mma(-x.x, wv[c].y - wv[c].x, ui[r2][c]);
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int r2 = 0; r2 < NRT; r2++)
-#pragma unroll
-        for (int c = 0; c < NCT; c++) {
-            const int col = c0 + 16 * c + li;
-            if (col >= n) continue;
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int row = rb + 16 * r2 + M::row(lane, v);
-                if (row < n) stx(U, (long)row * n + col, CT{ur[r2][c][v] - uq[r2][c][v], ui[r2][c][v] - uq[r2][c][v]});
-            }
-        }
 }
 
 // All passes for one slab of 64 eigenvector columns per workgroup (512 threads): the slab
@@ -1478,67 +1242,38 @@ __global__ __launch_bounds__(512) void lgbt_slab_kernel(const T *__restrict__ Ab
     }
 }
 
-// Per-matrix workspace of the pass back-transform, in elements of T: the pass factors
-// (nbb NB x NB complex) and W2 (NB x n complex).
 template <typename T>
 constexpr int lgbt_nb() { return sizeof(T) == sizeof(float) ? 128 : 64; }
-template <typename T>
-static long lgbt_work_elems(int n) {
-    constexpr int NB = lgbt_nb<T>();
-    const long nbb = (n - 1 + NB - 1) / NB;
-    return 2L * nbb * NB * NB + 2L * NB * n;
-}
 
-// The pass back-transform over `batch` matrices, in groups that fit `cap_elems` elements of
-// T of workspace at `work`: the pass factors, then either one launch of the slab kernel
-// (fused = true) or two launches per pass (lgbt_w_kernel, lgbt_update_kernel; W2 in the
-// workspace after the factors).
+// The pass back-transform over `batch` matrices, in groups whose pass factors fit
+// `cap_elems` elements of T of workspace at `work`: the factors (lgbt_tbig_kernel), then
+// one launch of the slab kernel.
 template <typename T>
-static int launch_bt_gemm(const T *A, long strideA, int n, const T *tws, long strideT, T *U, long strideU,
-                          int batch, T *work, long cap_elems, bool fused, hipStream_t s) {
-    constexpr int NB = lgbt_nb<T>(), WR = sizeof(T) == sizeof(float) ? 32 : 16, CW = 64;
+static int launch_bt_passes(const T *A, long strideA, int n, const T *tws, long strideT, T *U, long strideU,
+                            int batch, T *work, long cap_elems, hipStream_t s) {
+    constexpr int NB = lgbt_nb<T>(), CW = 64;
     const int nbb = (n - 1 + NB - 1) / NB;
-    const int nct = (n + CW - 1) / CW;
-    const long strideTb = (long)nbb * NB * NB, strideW2 = (long)NB * n;   // complex elements
-    const long per = fused ? 2L * strideTb : lgbt_work_elems<T>(n);
-    const long mg = std::min<long>(batch, cap_elems / per);
+    const int nslab = (n + CW - 1) / CW;
+    const long strideTb = (long)nbb * NB * NB;   // complex elements per matrix
+    const long mg = std::min<long>(batch, cap_elems / (2 * strideTb));
     if (mg < 1) return -12;
     for (long b0 = 0; b0 < batch; b0 += mg) {
         const int nbm = (int)std::min<long>(mg, batch - b0);
         const T *Ab = A + 2 * b0 * strideA;
-        const T *tb = tws + 2 * b0 * strideT;
-        T *Ub = U + 2 * b0 * strideU;
-        T *Tbig = work, *W2 = work + 2L * nbm * strideTb;
         hipLaunchKernelGGL((lgbt_tbig_kernel<T, NB>), dim3((unsigned)((long)nbm * nbb)), dim3(256), 0, s, Ab, strideA,
-                           n, tb, strideT, Tbig, strideTb, nbb, nbm);
+                           n, tws + 2 * b0 * strideT, strideT, work, strideTb, nbb, nbm);
         const long groups = (nbm + 7) / 8;
-        if (fused) {
-            hipLaunchKernelGGL((lgbt_slab_kernel<T, NB>), dim3((unsigned)(groups * 8 * nct)), dim3(512), 0, s, Ab,
-                               strideA, n, Tbig, strideTb, Ub, strideU, nct, nbm);
-            RLM_HIP_CHECK(hipGetLastError());
-            continue;
-        }
-        for (int p = nbb - 1; p >= 0; p--) {
-            const int rows = n - (p * NB + 1);
-            if (rows <= 0) continue;
-            hipLaunchKernelGGL((lgbt_w_kernel<T, NB>), dim3((unsigned)(groups * 8 * nct)), dim3(256), 0, s, Ab, strideA,
-                               n, Tbig, strideTb, Ub, strideU, W2, strideW2, p, nct, nbm);
-            const int nrt = (rows + 4 * WR - 1) / (4 * WR);
-            hipLaunchKernelGGL((lgbt_update_kernel<T, NB, WR>), dim3((unsigned)(groups * 8 * nrt * nct)), dim3(256), 0,
-                               s, Ab, strideA, n, W2, strideW2, Ub, strideU, p, nrt, nct, nbm);
-        }
+        hipLaunchKernelGGL((lgbt_slab_kernel<T, NB>), dim3((unsigned)(groups * 8 * nslab)), dim3(512), 0, s, Ab,
+                           strideA, n, work, strideTb, U + 2 * b0 * strideU, strideU, nslab, nbm);
         RLM_HIP_CHECK(hipGetLastError());
     }
     return 0;
 }
 
-// Back-transform variant for n > 512 (RLM_LG_BT): "slab" the two-block slab kernel, "passes"
-// two launches per 128-reflector pass, default the fused slab kernel over all passes.
-static int lg_bt_mode() {
+// RLM_LG_BT=slab selects the two-block slab kernel for n > 512 (comparison); default: passes.
+static bool lg_bt_slab() {
     const char *e = getenv("RLM_LG_BT");
-    if (e && std::string(e) == "slab") return 0;
-    if (e && std::string(e) == "passes") return 1;
-    return 2;
+    return e && std::string(e) == "slab";
 }
 
 // ----------------------------------------------------------------------------
@@ -1546,7 +1281,6 @@ static int lg_bt_mode() {
 template <typename T>
 static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, long strideV, T *aux, double *W,
                          T *U, double *work, int grid_dc, T *tws, hipStream_t s, hipEvent_t mid, hipEvent_t mid2) {
-    using C = cx<T>;
     if (n <= 512 || n > heevd_large_max_n() || batch < 1) return batch < 1 ? 0 : -22;
     const long strideA = (long)n * n;
     const long strideAux = heevd_large_aux_complex(n);
@@ -1597,11 +1331,10 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
     const long strideT = backtransform_tws_complex(n);
     const int nblk = (n - 1 + 15) / 16;
     if ((rc = launch_larft(A, strideA, n, n, TAU, strideV, tws, strideT, nb, (nblk + 3) / 4, s))) return rc;
-    if (const int mode = lg_bt_mode()) {
-        // the divide & conquer workspace is free now: the pass factors (and W2) live there
+    if (!lg_bt_slab()) {
+        // the divide & conquer workspace is free now: the pass factors live there
         const long cap = (long)grid_dc * stedc_global_work_doubles(n) * (long)(sizeof(double) / sizeof(T));
-        return launch_bt_gemm<T>(A, strideA, n, tws, strideT, U, strideA, nb, reinterpret_cast<T *>(work), cap,
-                                 mode == 2, s);
+        return launch_bt_passes<T>(A, strideA, n, tws, strideT, U, strideA, nb, reinterpret_cast<T *>(work), cap, s);
     }
     // 32 columns per slab for complex float, 16 for complex double (the W reduction
     // buffers then stay at 32 KB of LDS)
