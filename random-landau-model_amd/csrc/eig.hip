@@ -1783,6 +1783,73 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
     tau_out = tau;
 }
 
+// Per-item steps of a merge (node at lo with K secular roots, m = K + nd values), shared
+// by stedc_block and the multi-workgroup secular phase (stedc_secular_kernel).
+// Root j: origin pole index, tau and the origin pole rc (root j = rc[j] + tau[j]).
+template <typename QS>
+__device__ inline void dc_root(int j, int K, int lo, double rho, DcShared &sh) {
+    int o;
+    double tu;
+    secular_root(j, K, sh.dl + lo, sh.z2 + lo, rho, o, tu, sizeof(QS) == sizeof(float) ? 0x1p-32 : dc_eps());
+    sh.orig[lo + j] = o;
+    sh.tau[lo + j] = tu;
+    sh.rc[lo + j] = sh.dl[lo + o];
+}
+// Gu-Eisenstat recomputed weight i (two partial products; the j == i factor is 1; the pole
+// distances are (dl_i - rc_j) - tau_j).
+template <typename QS>
+__device__ inline void dc_weight(int i, int K, int lo, DcShared &sh) {
+    const double *dl = sh.dl + lo, *ro = sh.rc + lo, *ta = sh.tau + lo;
+    const double di = dl[i];
+    double pa = (di - ro[i]) - ta[i], pb = 1.0;
+    int j = 0;
+    for (; j + 1 < K; j += 2) {
+        const bool sa = j == i, sb = j + 1 == i;
+        const double fa = ((di - ro[j]) - ta[j]) * rcp_q<QS>(sa ? 1.0 : di - dl[j]);
+        const double fb = ((di - ro[j + 1]) - ta[j + 1]) * rcp_q<QS>(sb ? 1.0 : di - dl[j + 1]);
+        pa *= sa ? 1.0 : fa;
+        pb *= sb ? 1.0 : fb;
+    }
+    if (j < K && j != i) pa *= ((di - ro[j]) - ta[j]) * rcp_q<QS>(di - dl[j]);
+    const double p = pa * pb;
+    sh.what[lo + i] = copysign(sqrt(fmax(-p, 0.0)), sh.wz[lo + i]);
+}
+// Inverse norm of eigenvector j (two partial sums).
+template <typename QS>
+__device__ inline void dc_norm(int j, int K, int lo, DcShared &sh) {
+    const double *dl = sh.dl + lo, *wh = sh.what + lo;
+    const double oj = sh.rc[lo + j];
+    const double tj = sh.tau[lo + j];
+    double sa = 0.0, sb = 0.0;
+    int i = 0;
+    for (; i + 1 < K; i += 2) {
+        const double ua = wh[i] * rcp_q<QS>((dl[i] - oj) - tj);
+        const double ub = wh[i + 1] * rcp_q<QS>((dl[i + 1] - oj) - tj);
+        sa += ua * ua;
+        sb += ub * ub;
+    }
+    if (i < K) {
+        const double ua = wh[i] * rcp_q<QS>((dl[i] - oj) - tj);
+        sa += ua * ua;
+    }
+    sh.inrm[lo + j] = 1.0 / sqrt(sa + sb);
+}
+// Value q of the merged node: root q < K, deflated value q - K after them.
+__device__ inline void dc_value(int q, int K, int lo, DcShared &sh) {
+    sh.vals[lo + q] = q < K ? sh.rc[lo + q] + sh.tau[lo + q] : sh.ds[lo + sh.dfx[lo + q - K]];
+}
+// Rank of value q among the node's m values (ties by index) and its place in dqn.
+__device__ inline void dc_rank(int q, int m, int lo, DcShared &sh, double *dqn) {
+    const double v = sh.vals[lo + q];
+    int rk = 0;
+    for (int p = 0; p < m; p++) {
+        const double u = sh.vals[lo + p];
+        rk += (u < v) || (u == v && p < q);
+    }
+    sh.rank[lo + q] = rk;
+    dqn[lo + rk] = v;
+}
+
 // Number of elements of the ascending array a[0..len) that are < v (strict) or <= v.
 __device__ inline int count_below(const double *a, int len, double v, bool inclusive) {
     int lo = 0, hi = len;
@@ -1900,10 +1967,11 @@ __device__ __forceinline__ void merge_gemm_mfma(int lane, int G, int lo, int mid
 // pointer swap) and eigenvectors of T in the returned buffer (row-major n x n).
 // QS: storage type of the eigenvector matrices Qa, Qb and of S (see merge_gemm_mfma);
 // every other quantity is FP64.
-// lev_stop > 0: stop at merge level lev_stop once its secular roots, weights, norms and
-// ranks are known (nullptr returned, the scale left in sh.red[0]); its eigenvector GEMM
-// then runs on several workgroups per matrix (stedc_level_gemm_kernel, or
-// stedc_top_gemm_kernel for the top level, which writes the outputs).
+// lev_stop > 0: stop at merge level lev_stop once its nodes are merged, deflated and
+// rotated and the secular inputs are set (nullptr returned, the scale left in sh.red[0]);
+// its roots, weights, norms and ranks (stedc_secular_kernel) and its eigenvector GEMM
+// (stedc_level_gemm_kernel, or stedc_top_gemm_kernel for the top level, which writes the
+// outputs) then run on several workgroups per matrix.
 // lev_begin > 1: resume at that level from the state a stopped call and the GEMM kernel
 // left in sh, Qa / Qb (the caller passes the workspace's first and second matrix; the
 // buffers and eigenvalue arrays swap after every level) and the scale in sh.red[0].
@@ -2158,78 +2226,27 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
                 }
             }
             __syncthreads();
+            // a stopped level's roots, weights, norms and ranks and its GEMM run on several
+            // workgroups per matrix (stedc_secular_kernel, stedc_level_gemm_kernel or
+            // stedc_top_gemm_kernel); uniform over the workgroup
+            if (L == lev_stop) continue;
             // P4: secular roots
+            if (merging)
+                for (int j = lane; j < K; j += G) dc_root<QS>(j, K, lo, rho, sh);
+            __syncthreads();
+            // P5: Gu-Eisenstat recomputed weights
+            if (merging)
+                for (int i = lane; i < K; i += G) dc_weight<QS>(i, K, lo, sh);
+            __syncthreads();
+            // P6: eigenvector norms; output values and ranks
             if (merging) {
-                for (int j = lane; j < K; j += G) {
-                    int o;
-                    double tu;
-                    secular_root(j, K, sh.dl + lo, sh.z2 + lo, rho, o, tu, sizeof(QS) == sizeof(float) ? 0x1p-32 : dc_eps());
-                    sh.orig[lo + j] = o;
-                    sh.tau[lo + j] = tu;
-                    sh.rc[lo + j] = sh.dl[lo + o];      // origin pole of root j (rc is free after P3)
-                }
+                for (int j = lane; j < K; j += G) dc_norm<QS>(j, K, lo, sh);
+                for (int q = lane; q < m; q += G) dc_value(q, K, lo, sh);
             }
             __syncthreads();
-            // (root j = rc[j] + tau[j]; the pole distances below are (dl_i - rc_j) - tau_j)
-            // P5: Gu-Eisenstat recomputed weights (two partial products; the j == i
-            // factor is 1)
-            if (merging) {
-                const double *dl = sh.dl + lo, *ro = sh.rc + lo, *ta = sh.tau + lo;
-                for (int i = lane; i < K; i += G) {
-                    const double di = dl[i];
-                    double pa = (di - ro[i]) - ta[i], pb = 1.0;
-                    int j = 0;
-                    for (; j + 1 < K; j += 2) {
-                        const bool sa = j == i, sb = j + 1 == i;
-                        const double fa = ((di - ro[j]) - ta[j]) * rcp_q<QS>(sa ? 1.0 : di - dl[j]);
-                        const double fb = ((di - ro[j + 1]) - ta[j + 1]) * rcp_q<QS>(sb ? 1.0 : di - dl[j + 1]);
-                        pa *= sa ? 1.0 : fa;
-                        pb *= sb ? 1.0 : fb;
-                    }
-                    if (j < K && j != i) pa *= ((di - ro[j]) - ta[j]) * rcp_q<QS>(di - dl[j]);
-                    const double p = pa * pb;
-                    sh.what[lo + i] = copysign(sqrt(fmax(-p, 0.0)), sh.wz[lo + i]);
-                }
-            }
+            if (merging)
+                for (int q = lane; q < m; q += G) dc_rank(q, m, lo, sh, dqn);
             __syncthreads();
-            // P6: eigenvector norms (two partial sums); output values and ranks
-            if (merging) {
-                const double *dl = sh.dl + lo, *wh = sh.what + lo;
-                for (int j = lane; j < K; j += G) {
-                    const double oj = sh.rc[lo + j];
-                    const double tj = sh.tau[lo + j];
-                    double sa = 0.0, sb = 0.0;
-                    int i = 0;
-                    for (; i + 1 < K; i += 2) {
-                        const double ua = wh[i] * rcp_q<QS>((dl[i] - oj) - tj);
-                        const double ub = wh[i + 1] * rcp_q<QS>((dl[i + 1] - oj) - tj);
-                        sa += ua * ua;
-                        sb += ub * ub;
-                    }
-                    if (i < K) {
-                        const double ua = wh[i] * rcp_q<QS>((dl[i] - oj) - tj);
-                        sa += ua * ua;
-                    }
-                    sh.inrm[lo + j] = 1.0 / sqrt(sa + sb);
-                    sh.vals[lo + j] = oj + tj;
-                }
-                for (int q = lane; q < nd; q += G) sh.vals[lo + K + q] = sh.ds[lo + sh.dfx[lo + q]];
-            }
-            __syncthreads();
-            if (merging) {
-                for (int q = lane; q < m; q += G) {
-                    const double v = sh.vals[lo + q];
-                    int rk = 0;
-                    for (int p = 0; p < m; p++) {
-                        const double u = sh.vals[lo + p];
-                        rk += (u < v) || (u == v && p < q);
-                    }
-                    sh.rank[lo + q] = rk;
-                    dqn[lo + rk] = v;
-                }
-            }
-            __syncthreads();
-            if (L == lev_stop) continue;            // the level's GEMM runs elsewhere (uniform)
             // Matrix-core path for halves of >= 32 rows (whole waves per node): Qb = Qa S
             // with S[i][q] = what_i / (dl_i - lambda_q) * inrm_q generated per K-step,
             // S never stored. Below that, S goes through memory and the VALU path.
@@ -2319,7 +2336,7 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
             if (top_direct) *direct = true;
             __syncthreads();
         }
-        if (L == lev_stop) {                        // every round's secular phase done: stop
+        if (L == lev_stop) {                        // every round's deflation done: stop
             if (t == 0) sh.red[0] = scale;
             *dq_final = dqn;
             return nullptr;
@@ -2439,6 +2456,49 @@ __global__ __launch_bounds__(NT) void stedc_resume_kernel(int n, const double *_
     double *dqf;
     (void)stedc_block<NT, T>(n, Din + b * strideV, Ein + b * strideV, Qa0, Qb0, S, sh, &dqf, lev, nullptr, nullptr,
                              lev);
+}
+
+// The secular phase of a stopped merge level L, nwg workgroups per matrix, the items of
+// all merging nodes dealt over them: phase 0 the roots (dc_root), phase 1 the weights and
+// the output values (dc_weight, dc_value), phase 2 the norms and the ranks (dc_norm,
+// dc_rank), one launch each. Same arithmetic as stedc_block's single-workgroup phases.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void stedc_secular_kernel(int n, const double *__restrict__ Ein, long strideV,
+                                                           double *__restrict__ work, int nwg, int L, int phase) {
+    const long b = blockIdx.x / nwg;
+    const int part = blockIdx.x % nwg;
+    double *wg = work + b * stedc_global_work_doubles(n);
+    DcShared sh;
+    dc_shared_at(wg + 3L * n * n, n, sh);
+    const double *eIn = Ein + b * strideV;
+    double *dqn = (L & 1) ? sh.dqn : sh.dq;
+    const double iscale = 1.0 / sh.red[0];
+    const int half = 1 << (L - 1), blk = half << 1, nnodes = (n + blk - 1) / blk;
+    const int G = nwg * NT;
+    // item idx of the level: per merging node K items (phase 0) or m items (phases 1, 2)
+    for (int idx = part * NT + (int)threadIdx.x;; idx += G) {
+        int k = 0, it = idx, lo = 0, mid = 0, m = 0, K = 0;
+        for (; k < nnodes; k++) {
+            lo = k * blk;
+            mid = lo + half;
+            if (mid >= n) continue;
+            m = min(lo + blk, n) - lo;
+            K = sh.cnt[5 * k];
+            const int cnt = phase == 0 ? K : m;
+            if (it < cnt) break;
+            it -= cnt;
+        }
+        if (k >= nnodes) break;
+        if (phase == 0) {
+            dc_root<T>(it, K, lo, 2.0 * fabs(eIn[mid - 1] * iscale), sh);
+        } else if (phase == 1) {
+            if (it < K) dc_weight<T>(it, K, lo, sh);
+            dc_value(it, K, lo, sh);
+        } else {
+            if (it < K) dc_norm<T>(it, K, lo, sh);
+            dc_rank(it, m, lo, sh, dqn);
+        }
+    }
 }
 
 // The eigenvector products Qb = Qa S of every node of merge level L (below the top), nwg
@@ -3480,7 +3540,11 @@ static int launch_stedc_global_t(int n, const double *D, const double *E, long s
         const int lev0 = std::min(first, top);
         hipLaunchKernelGGL((stedc_kernel<T, NT, true>), dim3(cnt), dim3(NT), lds, stream, n, Db, Eb, strideV,
                            W + (long)b0 * strideW, strideW, U + 2L * b0 * strideU, strideU, work, cnt, lev0);
-        for (int L = lev0; L < top; L++) {
+        for (int L = lev0; L <= top; L++) {
+            for (int phase = 0; phase < 3; phase++)
+                hipLaunchKernelGGL((stedc_secular_kernel<T, NT>), dim3((unsigned)((long)cnt * nwg)), dim3(NT), 0,
+                                   stream, n, Eb, strideV, work, nwg, L, phase);
+            if (L == top) break;
             hipLaunchKernelGGL((stedc_level_gemm_kernel<T, NT>), dim3((unsigned)((long)cnt * nwg)), dim3(NT), 0,
                                stream, n, work, nwg, L);
             hipLaunchKernelGGL((stedc_resume_kernel<T, NT>), dim3(cnt), dim3(NT), 0, stream, n, Db, Eb, strideV, work,
