@@ -3526,9 +3526,10 @@ static int launch_stedc_global_t(int n, const double *D, const double *E, long s
     // of the first split level, then GEMM and resume kernels alternate up to the top
     int top = 1;
     while ((1 << top) < n) top++;                  // level whose block covers n
-    // first split level: halves of 256 rows (n_phi = 3000: 78 -> 72 ms against splitting
-    // from 512 rows on, 71 ms from 128; RLM_DC_SPLIT_LEVEL overrides, >= 6)
-    int first = 9;
+    // first split level: halves of 128 rows (with the secular phase on several workgroups
+    // as well, n_phi = 3000, 64 matrices: 55.4 ms against 57.7 ms from 256-row halves and
+    // 55.5 ms from 64; RLM_DC_SPLIT_LEVEL overrides, >= 6)
+    int first = 8;
     if (const char *e = getenv("RLM_DC_SPLIT_LEVEL")) first = std::max(6, atoi(e));
     int dev = 0, cus = 0;
     RLM_HIP_CHECK(hipGetDevice(&dev));
