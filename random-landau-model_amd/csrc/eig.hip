@@ -3519,11 +3519,12 @@ static int launch_stedc_global_t(int n, const double *D, const double *E, long s
         RLM_HIP_CHECK(hipGetLastError());
         return 0;
     }
-    // large n: the GEMMs of the merge levels with halves of >= 256 rows (the top one
-    // alone is about 70 % of the divide & conquer's matrix-core work) on several
-    // workgroups per matrix instead of one, in rounds of `grid` matrices (one workspace
-    // slot each): the single-workgroup kernel runs the levels below and the secular phase
-    // of the first split level, then GEMM and resume kernels alternate up to the top
+    // large n: the secular phase and the GEMMs of the merge levels with halves of >= 128
+    // rows (the top GEMM alone is about 70 % of the divide & conquer's matrix-core work)
+    // on several workgroups per matrix instead of one, in rounds of `grid` matrices (one
+    // workspace slot each): the single-workgroup kernel runs the levels below and merges
+    // and deflates the first split level, then secular, GEMM and resume kernels alternate
+    // up to the top
     int top = 1;
     while ((1 << top) < n) top++;                  // level whose block covers n
     // first split level: halves of 128 rows (with the secular phase on several workgroups

@@ -1270,10 +1270,11 @@ static int launch_bt_passes(const T *A, long strideA, int n, const T *tws, long 
     return 0;
 }
 
-// RLM_LG_BT=slab selects the two-block slab kernel for n > 512 (comparison); default: passes.
-static bool lg_bt_slab() {
+// RLM_LG_BT=pairs selects lg_backtransform2_kernel (two blocks of 16 reflectors per pass) for
+// n > 512, for comparison; default: passes of lgbt_nb<T>() reflectors (tbig + slab kernels).
+static bool lg_bt_pairs() {
     const char *e = getenv("RLM_LG_BT");
-    return e && std::string(e) == "slab";
+    return e && std::string(e) == "pairs";
 }
 
 // ----------------------------------------------------------------------------
@@ -1331,7 +1332,7 @@ static int heevd_large_t(T *A, int n, long batch, double *D, double *E, T *TAU, 
     const long strideT = backtransform_tws_complex(n);
     const int nblk = (n - 1 + 15) / 16;
     if ((rc = launch_larft(A, strideA, n, n, TAU, strideV, tws, strideT, nb, (nblk + 3) / 4, s))) return rc;
-    if (!lg_bt_slab()) {
+    if (!lg_bt_pairs()) {
         // the divide & conquer workspace is free now: the pass factors live there
         const long cap = (long)grid_dc * stedc_global_work_doubles(n) * (long)(sizeof(double) / sizeof(T));
         return launch_bt_passes<T>(A, strideA, n, tws, strideT, U, strideA, nb, reinterpret_cast<T *>(work), cap, s);
