@@ -123,6 +123,116 @@ __global__ __launch_bounds__(256) void links_derived_kernel(const TU *__restrict
     }
 }
 
+// The same links for a tile of B1 x B2 points per workgroup: the tile's points and its
+// neighbours below and to the right, (B1 + 1)(B2 + 1) - 1 vector sets, are read once per
+// component row and every link of the tile is accumulated from registers, so a vector set
+// is fetched (B1 + 1)(B2 + 1) - 1 times per B1 B2 points instead of 3 times per point.
+// Grid (n_samples * nt1 * nt2, state chunks of 256), block 256, ns complex doubles of
+// dynamic LDS; nt1 = ceil((g + 1) / B1), nt2 = ceil((g + 1) / B2). Each link is summed over
+// the components in the same order as links_derived_kernel, so the results are identical.
+template <typename TU, int B1, int B2>
+__global__ __launch_bounds__(256) void links_tile_kernel(const TU *__restrict__ U, long strideU, int ns, int g,
+                                                         int nt1, int nt2, double *__restrict__ Lk1,
+                                                         double *__restrict__ Lk2) {
+    extern __shared__ cd phs[];
+    const int g1 = g + 1;
+    const int tile = (int)(blockIdx.x % (unsigned)(nt1 * nt2));
+    const long s = blockIdx.x / (unsigned)(nt1 * nt2);
+    const int a0 = (tile / nt2) * B1, b0 = (tile % nt2) * B2;
+    const double TWO_PI = 6.283185307179586232;
+    if (a0 + B1 >= g) {                                      // tiles that touch theta_1 = 2 pi
+        for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+            double sn, cs;
+            sincos(TWO_PI * (double)i / (double)ns, &sn, &cs);
+            phs[i] = cd{cs, sn};
+        }
+    }
+    __syncthreads();
+    constexpr int NS1 = B1 + 1, NS2 = B2 + 1;
+    // slot (da, db): point (a0 + da, b0 + db) of the (g + 1)^2 mesh, mapped to its solved
+    // set (theta = 2 pi rows and columns from theta = 0 with a component phase / shift);
+    // the row part of its offset, validity and phase depend on da only, the column part and
+    // the shift on db only
+    const TU *base = U + 2 * (s * g * g) * strideU;
+    long roff[NS1], coff[NS2];
+    bool rok[NS1], rph[NS1], cok[NS2], csh[NS2];
+    // rows and columns beyond the mesh keep the offset of the last one inside it (a0, b0 <=
+    // g), so every load below reads a valid set and needs no branch; their values are unused
+#pragma unroll
+    for (int da = 0; da < NS1; da++) {
+        const int a = a0 + da;
+        rok[da] = a <= g;
+        rph[da] = a == g;
+        roff[da] = rok[da] ? 2 * (long)(a == g ? 0 : a) * g * strideU : roff[da > 0 ? da - 1 : 0];
+    }
+#pragma unroll
+    for (int db = 0; db < NS2; db++) {
+        const int b = b0 + db;
+        cok[db] = b <= g;
+        csh[db] = b == g;
+        coff[db] = cok[db] ? 2 * (long)(b == g ? 0 : b) * strideU : coff[db > 0 ? db - 1 : 0];
+    }
+    auto ok = [&](int da, int db) { return rok[da] && cok[db] && !(da == B1 && db == B2); };
+    for (int j = blockIdx.y * blockDim.x + threadIdx.x; j < ns; j += gridDim.y * blockDim.x) {
+        cd a1[B1][B2], a2[B1][B2];
+#pragma unroll
+        for (int da = 0; da < B1; da++)
+#pragma unroll
+            for (int db = 0; db < B2; db++) {
+                a1[da][db] = cd{0.0, 0.0};
+                a2[da][db] = cd{0.0, 0.0};
+            }
+        for (int i = 0; i < ns; i++) {
+            const long o0 = (long)i * ns + j;
+            const long o1 = (long)(i + 1 >= ns ? i + 1 - ns : i + 1) * ns + j;
+            // every load of the component row first (no branches between them), then the
+            // links row by row: those to the row below use the previous row's values
+            cx<TU> raw[NS1][NS2];
+#pragma unroll
+            for (int da = 0; da < NS1; da++)
+#pragma unroll
+                for (int db = 0; db < NS2; db++)
+                    if (!(da == B1 && db == B2)) raw[da][db] = ldx(base + roff[da] + coff[db], csh[db] ? o1 : o0);
+            cd xp[NS2];
+#pragma unroll
+            for (int da = 0; da < NS1; da++) {
+                cd xr[NS2];
+#pragma unroll
+                for (int db = 0; db < NS2; db++) {
+                    if (da == B1 && db == B2) {
+                        xr[db] = cd{0.0, 0.0};
+                        continue;
+                    }
+                    xr[db] = cd{(double)raw[da][db].x, (double)raw[da][db].y};
+                    if (rph[da]) xr[db] = cmul(phs[i], xr[db]);
+                }
+                if (da > 0) {
+#pragma unroll
+                    for (int db = 0; db < B2; db++)
+                        if (ok(da, db)) cfmac(a1[da - 1][db], xp[db], xr[db]);
+                }
+                if (da < B1) {
+#pragma unroll
+                    for (int db = 0; db < B2; db++)
+                        if (ok(da, db + 1)) cfmac(a2[da][db], xr[db], xr[db + 1]);
+                }
+#pragma unroll
+                for (int db = 0; db < NS2; db++) xp[db] = xr[db];
+            }
+        }
+#pragma unroll
+        for (int da = 0; da < B1; da++)
+#pragma unroll
+            for (int db = 0; db < B2; db++) {
+                const int a = a0 + da, b = b0 + db;
+                if (a > g || b > g) continue;
+                const long kp = (s * g1 + a) * g1 + b;
+                st(Lk1, kp * ns + j, a1[da][db]);
+                st(Lk2, kp * ns + j, a2[da][db]);
+            }
+    }
+}
+
 // plaquettes + energies: grid (S), block 256 (states).
 // mode 0 = rlm (+1/n_phi, chern += t), mode 1 = hlm (chern -= t), mode 2 = rlm with
 // eigenvalues stored for the solved points only (theta < g; the theta = 2 pi points
@@ -216,8 +326,38 @@ int launch_links_derived(const void *U, bool fp32, long strideU, int ns, int g, 
     const long nkp = (long)n_samples * (g + 1) * (g + 1);
     if (nkp <= 0) return 0;
     if (ns > 4096 || g < 1) return -22;   // phase table: ns complex doubles of LDS
-    const dim3 grid((unsigned)(8 * ((nkp + 7) / 8)), (ns + 255) / 256);
     const size_t lds = sizeof(cd) * ns;
+    // tiles of 3 x 3 points; FP64 vectors take 4 x 4 where that still gives at least four
+    // workgroups per CU (n_phi = 256, g = 32, 32 samples: FP32 7.2 -> 4.3 ms, FP64 11.6 ->
+    // 9.4 / 8.0 ms for 3 x 3 / 4 x 4; n_phi = 3000, g = 4: 6.3 -> 3.6 ms for 3 x 3, 5.0 ms
+    // for 4 x 4, which leaves 192 workgroups). RLM_LINKS_TILE=0 keeps one point per
+    // workgroup (links_derived_kernel), for comparison.
+    const char *te = getenv("RLM_LINKS_TILE");
+    if (!(te && atoi(te) == 0)) {
+        const unsigned chunks = (unsigned)((ns + 255) / 256);
+        int B = 3;
+        if (!fp32) {
+            int dev = 0, cus = 0;
+            RLM_HIP_CHECK(hipGetDevice(&dev));
+            RLM_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            const long nt4 = (g + 1 + 3) / 4;
+            if ((long)n_samples * nt4 * nt4 * chunks >= 4L * cus) B = 4;
+        }
+        const int nt = (g + 1 + B - 1) / B;
+        const dim3 tgrid((unsigned)((long)n_samples * nt * nt), chunks);
+        if (fp32)
+            hipLaunchKernelGGL((links_tile_kernel<float, 3, 3>), tgrid, dim3(256), lds, stream,
+                               static_cast<const float *>(U), strideU, ns, g, nt, nt, Lk1, Lk2);
+        else if (B == 4)
+            hipLaunchKernelGGL((links_tile_kernel<double, 4, 4>), tgrid, dim3(256), lds, stream,
+                               static_cast<const double *>(U), strideU, ns, g, nt, nt, Lk1, Lk2);
+        else
+            hipLaunchKernelGGL((links_tile_kernel<double, 3, 3>), tgrid, dim3(256), lds, stream,
+                               static_cast<const double *>(U), strideU, ns, g, nt, nt, Lk1, Lk2);
+        RLM_HIP_CHECK(hipGetLastError());
+        return 0;
+    }
+    const dim3 grid((unsigned)(8 * ((nkp + 7) / 8)), (ns + 255) / 256);
     if (fp32)
         hipLaunchKernelGGL(links_derived_kernel<float>, grid, dim3(256), lds, stream, static_cast<const float *>(U),
                            strideU, ns, g, nkp, Lk1, Lk2);

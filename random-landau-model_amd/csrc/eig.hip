@@ -2814,6 +2814,16 @@ __global__ __launch_bounds__(NWV > 4 ? 512 : 256) __attribute__((amdgpu_waves_pe
         }
     };
     if (PREFETCH) load_v(nblk - 1);
+    // complex float: T[k1 = li][k2 = row(l, s)] of a block is loaded one block ahead (n_phi =
+    // 256, 32768 matrices: 60.5 -> 58.7 ms; complex double, whose extra registers cost it a
+    // wave per SIMD, 159 -> 199 ms)
+    constexpr bool TPF = sizeof(T) == sizeof(float);
+    CT tnext[4];
+    auto load_t = [&](int bk) {
+#pragma unroll
+        for (int s = 0; s < 4; s++) tnext[s] = Tb[bk * 256 + M::row(lane, s) * 16 + li];
+    };
+    if (TPF) load_t(nblk - 1);
     for (int bk = nblk - 1; bk >= 0; bk--) {
         const int j0 = bk * 16, r0 = j0 + 1;
         if (STAGE_ALL) {
@@ -2869,6 +2879,12 @@ __global__ __launch_bounds__(NWV > 4 ? 512 : 256) __attribute__((amdgpu_waves_pe
                 wb[((1 * NCB + cb) * 4 + v) * kWave + lane] = q1[cb][v] + wi[cb][v];
             }
         __syncthreads();
+        CT tcur[4];
+        if (TPF) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) tcur[s] = tnext[s];
+            if (bk > 0) load_t(bk - 1);
+        }
         if (PREFETCH && bk > 0) load_v(bk - 1);           // in flight during T W and the update
 #pragma unroll
         for (int cb = 0; cb < NCB; cb++)
@@ -2893,7 +2909,7 @@ __global__ __launch_bounds__(NWV > 4 ? 512 : 256) __attribute__((amdgpu_waves_pe
         }
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            const CT tv = Tb[bk * 256 + M::row(lane, s) * 16 + li];   // T[k1 = li][k2 = row(l, s)]
+            const CT tv = TPF ? tcur[s] : Tb[bk * 256 + M::row(lane, s) * 16 + li];   // T[k1 = li][k2 = row(l, s)]
             const T ts = tv.x + tv.y;
 #pragma unroll
             for (int cb = 0; cb < NCB; cb++) {
