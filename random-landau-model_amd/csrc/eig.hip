@@ -1706,8 +1706,10 @@ __device__ inline double rcp_q(double x) {
 // Rayleigh quotients: 4.3 iterations per root at 2^-53, 3.9 at 2^-40, 3.6 at 2^-32 for
 // n = 256 random Hermitian matrices; the plans' mixed energies stay within 2.2e-12 of FP64).
 // Returns origin index o and tau with lambda = dl[o] + tau.
+// [jlo, jhi] must contain j; the callers pass the range of roots their wave solves at once,
+// so the pole loops split there uniformly (see below).
 __device__ void secular_root(int j, int K, const double *dl, const double *z2, double rho, int &o_out,
-                             double &tau_out, double teps) {
+                             double &tau_out, double teps, int jlo, int jhi) {
     const bool one_nr = teps > dc_eps();      // see rcp_nr1
     int o;
     double lo, hi, tau;
@@ -1729,24 +1731,41 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
     const double dlo = dl[o];
     const double Dj = dl[j] - dlo;
     const double Dj1 = (j < K - 1) ? (dl[j + 1] - dlo) : 0.0;
+    const int jm = min(jhi, K - 1);
     for (int it = 0; it < 120; it++) {
         double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, err;
-        // poles at or below j feed psi, the others phi (all terms of one sign each);
-        // the roots of a wave are consecutive, so its lanes' split points differ by
-        // at most 63 and the two loops together run about K + 64 iterations
+        // poles at or below j feed psi, the others phi (all terms of one sign each). The
+        // wave's roots lie in [jlo, jhi]: poles below jlo feed psi and poles above jhi phi
+        // for every lane; in between (at most 64 poles) the sign of the pole distance
+        // decides, which is exact: tau stays strictly inside (0, D/2] (origin j) or
+        // [-D/2, 0) (origin j + 1), so (dl_i - dl_o) - tau < 0 exactly for i <= j and > 0
+        // for i > j. The other sum then gains +0, so psi, phi and their derivatives are
+        // bit-identical to splitting each lane at its own j, and the wave runs K pole
+        // iterations instead of K + (jhi - jlo).
+        int i = 0;
 #pragma unroll 4
-        for (int i = 0; i <= j; i++) {
+        for (; i < jlo; i++) {
             const double inv = one_nr ? rcp_nr1((dl[i] - dlo) - tau) : rcp_nr((dl[i] - dlo) - tau);
             const double tt = z2[i] * inv;
             psi += tt;
-            dpsi += tt * inv;
+            dpsi = fma(tt, inv, dpsi);
+        }
+#pragma unroll 2
+        for (; i <= jm; i++) {
+            const double inv = one_nr ? rcp_nr1((dl[i] - dlo) - tau) : rcp_nr((dl[i] - dlo) - tau);
+            const double tt = z2[i] * inv;
+            const double tn = fmin(tt, 0.0), tp = fmax(tt, 0.0);
+            psi += tn;
+            phi += tp;
+            dpsi = fma(tn, inv, dpsi);
+            dphi = fma(tp, inv, dphi);
         }
 #pragma unroll 4
-        for (int i = j + 1; i < K; i++) {
+        for (; i < K; i++) {
             const double inv = one_nr ? rcp_nr1((dl[i] - dlo) - tau) : rcp_nr((dl[i] - dlo) - tau);
             const double tt = z2[i] * inv;
             phi += tt;
-            dphi += tt * inv;
+            dphi = fma(tt, inv, dphi);
         }
         // sum of |terms|: the psi terms are all <= 0 (poles below the root), the phi terms
         // all >= 0, so it is phi - psi without a third accumulation
@@ -1786,11 +1805,13 @@ __device__ void secular_root(int j, int K, const double *dl, const double *z2, d
 // Per-item steps of a merge (node at lo with K secular roots, m = K + nd values), shared
 // by stedc_block and the multi-workgroup secular phase (stedc_secular_kernel).
 // Root j: origin pole index, tau and the origin pole rc (root j = rc[j] + tau[j]).
+// [jlo, jhi]: a range that contains j (see secular_root).
 template <typename QS>
-__device__ inline void dc_root(int j, int K, int lo, double rho, DcShared &sh) {
+__device__ inline void dc_root(int j, int K, int lo, double rho, DcShared &sh, int jlo, int jhi) {
     int o;
     double tu;
-    secular_root(j, K, sh.dl + lo, sh.z2 + lo, rho, o, tu, sizeof(QS) == sizeof(float) ? 0x1p-32 : dc_eps());
+    secular_root(j, K, sh.dl + lo, sh.z2 + lo, rho, o, tu, sizeof(QS) == sizeof(float) ? 0x1p-32 : dc_eps(), jlo,
+                 jhi);
     sh.orig[lo + j] = o;
     sh.tau[lo + j] = tu;
     sh.rc[lo + j] = sh.dl[lo + o];
@@ -2231,8 +2252,12 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
             // stedc_top_gemm_kernel); uniform over the workgroup
             if (L == lev_stop) continue;
             // P4: secular roots
-            if (merging)
-                for (int j = lane; j < K; j += G) dc_root<QS>(j, K, lo, rho, sh);
+            // a wave solves the roots j - lane % W .. + W - 1 together (W = min(G, wave):
+            // node groups are aligned to G threads)
+            if (merging) {
+                const int W = min(G, kWave);
+                for (int j = lane; j < K; j += G) dc_root<QS>(j, K, lo, rho, sh, j - lane % W, j - lane % W + W - 1);
+            }
             __syncthreads();
             // P5: Gu-Eisenstat recomputed weights
             if (merging)
@@ -2490,7 +2515,10 @@ __global__ __launch_bounds__(NT) void stedc_secular_kernel(int n, const double *
         }
         if (k >= nnodes) break;
         if (phase == 0) {
-            dc_root<T>(it, K, lo, 2.0 * fabs(eIn[mid - 1] * iscale), sh);
+            // the wave's 64 consecutive items lie in it - lane .. it - lane + 63 when the wave
+            // does not cross a node boundary (a valid range for each lane either way)
+            const int wl = (int)(threadIdx.x & (kWave - 1));
+            dc_root<T>(it, K, lo, 2.0 * fabs(eIn[mid - 1] * iscale), sh, it - wl, it - wl + kWave - 1);
         } else if (phase == 1) {
             if (it < K) dc_weight<T>(it, K, lo, sh);
             dc_value(it, K, lo, sh);
