@@ -2784,8 +2784,8 @@ __global__ __launch_bounds__(256) void larft_kernel(const T *__restrict__ Abase,
 // W = V^H U (each wave over its rows, summed through LDS in a fixed order),
 // W2 = T W, U -= V W2. Workgroups are mapped so that the slabs of one matrix share
 // blockIdx % 8, i.e. one XCD and its L2, which then serves V and T to all of them.
-template <typename T, int CW, int NWV>
-__global__ __launch_bounds__(NWV > 4 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(2))) void backtransform_mfma_kernel(const T *__restrict__ Abase, long strideA, int lda,
+template <typename T, int CW, int NWV, int WPE = 2>
+__global__ __launch_bounds__(NWV > 4 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(WPE))) void backtransform_mfma_kernel(const T *__restrict__ Abase, long strideA, int lda,
                                                                  int n, const T *__restrict__ Tin, long strideT,
                                                                  T *__restrict__ Uout, long strideU, int nslab,
                                                                  int batch) {
@@ -3140,23 +3140,29 @@ int launch_tail_t(const T *A, long strideA, int lda, int n, const double *D, con
     if (mid) RLM_HIP_CHECK(hipEventRecord(mid, stream));
     if (n == 1) return 0;
     if (n <= 256 && tws) {
-        // matrix-core path: triangular factors, then slabs of 32 eigenvector columns
+        // matrix-core path: triangular factors, then slabs of eigenvector columns
         const long strideT = backtransform_tws_complex(n);
         hipLaunchKernelGGL((larft_kernel<T>), dim3(batch), dim3(256), 0, stream, A, strideA, lda, n, TAU, strideV, tws,
                            strideT, batch, 1);
         RLM_HIP_CHECK(hipGetLastError());
-        // slab width: 32 columns for complex float; 16 for complex double, which keeps the
-        // accumulators within two waves per SIMD
-        constexpr int CW = sizeof(T) == sizeof(float) ? 32 : 16;
-        const int nslab = (n + CW - 1) / CW;
-        const long groups = (batch + 7) / 8;
-        // n <= 128: two waves per slab (twice the workgroups per CU, a 2-way W reduction)
-        if (n <= 128)
+        // n <= 128: two waves per slab (twice the workgroups per CU, a 2-way W reduction),
+        // slabs of 32 columns for complex float and 16 for complex double. Above: slabs of
+        // 16 columns at three waves per SIMD (166 VGPRs complex float, 148 complex double;
+        // three 51 KB workgroups per CU). Complex float n = 256, 32768 matrices: 58.4 ms with
+        // 32-column slabs at two waves per SIMD (241 VGPRs, 67.5 KB of LDS: a cap at three
+        // waves leaves it at one), 70.0 ms with 16-column slabs at two, 57.2 ms at three.
+        if (n <= 128) {
+            constexpr int CW = sizeof(T) == sizeof(float) ? 32 : 16;
+            const int nslab = (n + CW - 1) / CW;
+            const long groups = (batch + 7) / 8;
             hipLaunchKernelGGL((backtransform_mfma_kernel<T, CW, 2>), dim3((unsigned)(groups * 8 * nslab)), dim3(128),
                                0, stream, A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
-        else
-            hipLaunchKernelGGL((backtransform_mfma_kernel<T, CW, 4>), dim3((unsigned)(groups * 8 * nslab)), dim3(256),
-                               0, stream, A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
+        } else {
+            const int nslab = (n + 15) / 16;
+            const long groups = (batch + 7) / 8;
+            hipLaunchKernelGGL((backtransform_mfma_kernel<T, 16, 4, 3>), dim3((unsigned)(groups * 8 * nslab)),
+                               dim3(256), 0, stream, A, strideA, lda, n, tws, strideT, U, strideU, nslab, batch);
+        }
         RLM_HIP_CHECK(hipGetLastError());
         return 0;
     }
