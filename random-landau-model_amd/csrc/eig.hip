@@ -1871,6 +1871,25 @@ __device__ inline void dc_rank(int q, int m, int lo, DcShared &sh, double *dqn) 
     dqn[lo + rk] = v;
 }
 
+// Visits (a, b), a < na, b < nb, in the order of the flat index a * nb + b = lane + k G
+// (the same element per lane as that loop), carrying (a, b) from one element to the next
+// instead of dividing the flat index: two divisions per lane instead of one per element.
+template <typename F>
+__device__ __forceinline__ void for_2d(int lane, int G, int na, int nb, F f) {
+    if (na <= 0 || nb <= 0) return;
+    int a = lane / nb, b = lane - a * nb;
+    const int da = G / nb, db = G - da * nb;
+    while (a < na) {
+        f(a, b);
+        a += da;
+        b += db;
+        if (b >= nb) {
+            b -= nb;
+            a++;
+        }
+    }
+}
+
 // Number of elements of the ascending array a[0..len) that are < v (strict) or <= v.
 __device__ inline int count_below(const double *a, int len, double v, bool inclusive) {
     int lo = 0, hi = len;
@@ -2037,11 +2056,11 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
     }
     // Qa, Qb: eigenvector matrices of the current level, column-major (column c =
     // eigenvector c, Q[c*n + r])
-    for (long idx = t; idx < (long)n * n; idx += NT) {
-        const long c = idx / n, r = idx % n;
+    for_2d(t, NT, n, n, [&](int c, int r) {
+        const long idx = (long)c * n + r;
         Qa[idx] = (r == c) ? (QS)1 : (QS)0;
         Qb[idx] = (QS)0;
-    }
+    });
     __syncthreads();
     }
     const double iscale = 1.0 / scale;
@@ -2283,11 +2302,10 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
             // P7: U (K x K) into S rows lo.., scaled by the inverse norms
             if (merging && !mfma && !mfma_pair) {
                 const double *dl = sh.dl + lo;
-                for (int idx = lane; idx < K * K; idx += G) {
-                    const int i = idx / K, q = idx % K;
+                for_2d(lane, G, K, K, [&](int i, int q) {
                     const double del = (dl[i] - sh.rc[lo + q]) - sh.tau[lo + q];
                     S[(long)(lo + i) * n + q] = (QS)(sh.what[lo + i] * rcp_q<QS>(del) * sh.inrm[lo + q]);
-                }
+                });
             }
             __syncthreads();
             // P8: Qb block = Qa block (rotated, gathered) * U; deflated columns copied.
@@ -2341,21 +2359,19 @@ __device__ QS *stedc_block(int n, const double *dIn, const double *eIn, QS *Qa, 
             }
             if (Uo) {
                 // deflated columns, consecutive lanes on consecutive output columns of a row
-                for (int idx = lane; idx < m * nd; idx += G) {
-                    const int r = idx / nd, qq = idx - r * nd;
+                for_2d(lane, G, m, nd, [&](int r, int qq) {
                     stx(Uo, (long)r * n + sh.rank[K + qq], cx<QS>{Qa[(long)sh.perm[sh.dfx[qq]] * n + r], (QS)0});
-                }
+                });
             } else if (merging) {
-                for (int idx = lane; idx < m * nd; idx += G) {
-                    const int qq = idx / m, rloc = idx - qq * m;
+                for_2d(lane, G, nd, m, [&](int qq, int rloc) {
                     const int r = lo + rloc;
                     Qb[(long)(lo + sh.rank[lo + K + qq]) * n + r] = Qa[(long)(lo + sh.perm[lo + sh.dfx[lo + qq]]) * n + r];
-                }
+                });
             } else if (active) {
-                for (int idx = lane; idx < m * m; idx += G) {
-                    const int c = lo + idx / m, r = lo + idx % m;
-                    Qb[(long)c * n + r] = Qa[(long)c * n + r];
-                }
+                for_2d(lane, G, m, m, [&](int cl, int rl) {
+                    const long e = (long)(lo + cl) * n + lo + rl;
+                    Qb[e] = Qa[e];
+                });
                 for (int q = lane; q < m; q += G) dqn[lo + q] = dq[lo + q];
             }
             if (top_direct) *direct = true;
@@ -2554,16 +2570,15 @@ __global__ __launch_bounds__(NT) void stedc_level_gemm_kernel(int n, double *__r
                 merge_gemm_mfma<T, 4, 2>(lane, G, lo, mid, hi, half, K, KL, KR, n, Qa, Qb, sh);
             else
                 merge_gemm_mfma<T, 2, 4>(lane, G, lo, mid, hi, half, K, KL, KR, n, Qa, Qb, sh);
-            for (long idx = lane; idx < (long)m * nd; idx += G) {
-                const int qq = (int)(idx / m), rloc = (int)(idx - (long)qq * m);
+            for_2d(lane, G, nd, m, [&](int qq, int rloc) {
                 const int r = lo + rloc;
                 Qb[(long)(lo + sh.rank[lo + K + qq]) * n + r] = Qa[(long)(lo + sh.perm[lo + sh.dfx[lo + qq]]) * n + r];
-            }
+            });
         } else {
-            for (long idx = lane; idx < (long)m * m; idx += G) {
-                const long c = lo + idx / m, r = lo + idx % m;
-                Qb[c * n + r] = Qa[c * n + r];
-            }
+            for_2d(lane, G, m, m, [&](int cl, int rl) {
+                const long e = (long)(lo + cl) * n + lo + rl;
+                Qb[e] = Qa[e];
+            });
             for (int q = lane; q < m; q += G) dqn[lo + q] = dq[lo + q];
         }
     }
@@ -2593,10 +2608,9 @@ __global__ __launch_bounds__(NT) void stedc_top_gemm_kernel(int n, double *__res
         merge_gemm_mfma<T, 4, 2>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh, U);
     else
         merge_gemm_mfma<T, 2, 4>(lane, G, lo, mid, hi, n1, K, KL, KR, n, Qa, Qb, sh, U);
-    for (long idx = lane; idx < (long)m * nd; idx += G) {
-        const int r = (int)(idx / nd), qq = (int)(idx - (long)r * nd);
+    for_2d(lane, G, m, nd, [&](int r, int qq) {
         stx(U, (long)r * n + sh.rank[K + qq], cx<T>{Qa[(long)sh.perm[sh.dfx[qq]] * n + r], (T)0});
-    }
+    });
     if (part == 0) {
         const double *dqn = (L & 1) ? sh.dqn : sh.dq;
         const double scale = sh.red[0];
