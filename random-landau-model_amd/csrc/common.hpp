@@ -145,6 +145,26 @@ __host__ __device__ inline void fmacx(cx<T> &acc, cx<T> a, cx<T> b) {
 }
 template <typename T>
 __device__ inline cx<T> ldx(const T *p, long i) { return reinterpret_cast<const cx<T> *>(p)[i]; }
+
+// Visits (a, b), a < na, b < nb, in the order of the flat index a * nb + b = lane + k G
+// (the same element per lane as that loop), carrying (a, b) from one element to the next
+// instead of dividing the flat index: two divisions per lane instead of one per element.
+template <typename F>
+__device__ __forceinline__ void for_2d(int lane, int G, int na, int nb, F f) {
+    if (na <= 0 || nb <= 0) return;
+    int a = lane / nb, b = lane - a * nb;
+    const int da = G / nb, db = G - da * nb;
+    while (a < na) {
+        f(a, b);
+        a += da;
+        b += db;
+        if (b >= nb) {
+            b -= nb;
+            a++;
+        }
+    }
+}
+
 template <typename T>
 __device__ inline void stx(T *p, long i, cx<T> v) { reinterpret_cast<cx<T> *>(p)[i] = v; }
 

@@ -1871,25 +1871,6 @@ __device__ inline void dc_rank(int q, int m, int lo, DcShared &sh, double *dqn) 
     dqn[lo + rk] = v;
 }
 
-// Visits (a, b), a < na, b < nb, in the order of the flat index a * nb + b = lane + k G
-// (the same element per lane as that loop), carrying (a, b) from one element to the next
-// instead of dividing the flat index: two divisions per lane instead of one per element.
-template <typename F>
-__device__ __forceinline__ void for_2d(int lane, int G, int na, int nb, F f) {
-    if (na <= 0 || nb <= 0) return;
-    int a = lane / nb, b = lane - a * nb;
-    const int da = G / nb, db = G - da * nb;
-    while (a < na) {
-        f(a, b);
-        a += da;
-        b += db;
-        if (b >= nb) {
-            b -= nb;
-            a++;
-        }
-    }
-}
-
 // Number of elements of the ascending array a[0..len) that are < v (strict) or <= v.
 __device__ inline int count_below(const double *a, int len, double v, bool inclusive) {
     int lo = 0, hi = len;

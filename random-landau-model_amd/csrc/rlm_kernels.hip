@@ -254,9 +254,8 @@ __global__ __launch_bounds__(256) void expand_band_kernel(const double *__restri
     const long kp = blockIdx.x / nch;
     const double *Hk = H + 2L * kp * N * nb;
     double *Ak = A + 2L * kp * strideA;
-    const long step = (long)nch * blockDim.x;
-    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)N * N; idx += step) {
-        const int c = (int)(idx / N), r = (int)(idx % N);
+    const int step = nch * (int)blockDim.x;
+    for_2d((int)(blockIdx.x % nch) * (int)blockDim.x + (int)threadIdx.x, step, N, N, [&](int c, int r) {
         cd v{0.0, 0.0};
         if (r <= c) {
             const int b = band_b(c - r, N, off);
@@ -266,7 +265,7 @@ __global__ __launch_bounds__(256) void expand_band_kernel(const double *__restri
             if (b >= 0) v = ld(Hk, (long)c * nb + b);
         }
         st(Ak, (long)c * N + r, v);
-    }
+    });
 }
 
 int launch_wband(const double *V, const double *cmt, int n_phi, int d, int off, int n_mesh, int n_samples, double *Wb,
