@@ -199,6 +199,19 @@ __global__ void wband_kernel(const double *__restrict__ V, const double *__restr
     st(Wb, ((long)z * d + m) * nb + b, sum);
 }
 
+// Row tiles of 16 that band column tile cb of H needs: the readers take A(r, c), r > c, as
+// h_{c, r} = H[c][b(r - c)] and A(r, c), r <= c, as conj(H[r][b(c - r)]) (band_entry,
+// the register-resident reduction, the Rayleigh kernels), so they read H[j][b] only where
+// j + D(b) < N, never the entries whose column j + D(b) wraps past N (h's lower triangle,
+// about half of the band). D increases with b, so tile (j0, b0) holds such an entry iff
+// j0 + D(b0) < N.
+__host__ __device__ inline int hband_rows(int N, int off, int nrb, int cb) {
+    const int b0 = cb * 16;
+    const int D = (2 * off + 1 >= N || b0 <= off) ? b0 : b0 + N - 2 * off - 1;
+    const int r = (N - D + 15) / 16;
+    return r < 0 ? 0 : (r > nrb ? nrb : r);
+}
+
 // H[kp][j][b] = sum_m E[t2][m][j] Wb[s,t1][m][b] on the FP64 matrix cores: nsplit
 // workgroups per k-point (several where the k-points are too few to fill the chip),
 // 16x16 output tiles dealt to their waves, K = m in steps of 4. Output kp is local
@@ -207,7 +220,7 @@ __global__ void wband_kernel(const double *__restrict__ V, const double *__restr
 // points, g+1 for the whole mesh). Wb is laid out per (s, theta_1) with g+1 lines per
 // sample.
 __global__ __launch_bounds__(256) void hband_kernel(const double *__restrict__ E, const double *__restrict__ Wb, int N,
-                                                    int d, int nb, int n_mesh, int pl, long kp0,
+                                                    int d, int nb, int off, int n_mesh, int pl, long kp0,
                                                     double *__restrict__ H, int nsplit) {
     using M = Mf<double>;
     using v4 = M::v4;
@@ -225,8 +238,12 @@ __global__ __launch_bounds__(256) void hband_kernel(const double *__restrict__ E
     const double *Wz = Wb + 2L * (s * g1 + t1) * d * nb;
     double *Hk = H + 2L * kpl * N * nb;
     const int nrb = (N + 15) / 16, ncb = (nb + 15) / 16;
-    for (int tile = part * 4 + w; tile < nrb * ncb; tile += 4 * nsplit) {
-        const int j0 = (tile / ncb) * 16, b0 = (tile % ncb) * 16;
+    int ntile = 0;
+    for (int cb = 0; cb < ncb; cb++) ntile += hband_rows(N, off, nrb, cb);
+    for (int tile = part * 4 + w; tile < ntile; tile += 4 * nsplit) {
+        int cb = 0, rt = tile;                              // the needed tiles, column tile by column tile
+        while (rt >= hband_rows(N, off, nrb, cb)) rt -= hband_rows(N, off, nrb, cb++);
+        const int j0 = rt * 16, b0 = cb * 16;
         // three real products per complex one: P1 = (er + ei) xr, P2 = er (xi - xr),
         // P3 = ei (xr + xi); Re = P1 - P3, Im = P1 + P2
         v4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
@@ -285,11 +302,13 @@ int launch_hband(const double *E, const double *Wb, int n_phi, int d, int off, i
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 256;
-    const int ntile = ((n_phi + 15) / 16) * ((band_width(n_phi, off) + 15) / 16);
+    const int nrb = (n_phi + 15) / 16, ncb = (band_width(n_phi, off) + 15) / 16;
+    int ntile = 0;
+    for (int cb = 0; cb < ncb; cb++) ntile += hband_rows(n_phi, off, nrb, cb);
     const long want = (2L * cus + nkp - 1) / nkp;
     const int nsplit = (int)std::max(1L, std::min<long>(want, (ntile + 3) / 4));
     hipLaunchKernelGGL(hband_kernel, dim3((unsigned)(nkp * nsplit)), dim3(256), 0, stream, E, Wb, n_phi, d,
-                       band_width(n_phi, off), n_mesh, pl, kp0, H, nsplit);
+                       band_width(n_phi, off), off, n_mesh, pl, kp0, H, nsplit);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
