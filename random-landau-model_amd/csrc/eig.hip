@@ -2701,12 +2701,34 @@ __global__ __launch_bounds__(NT) void backtransform_kernel(const T *__restrict__
 // common.hpp.
 // Element (row, k) of the reflector block starting at reflector j0 (zhetrd lower
 // storage: unit at row j+1, A(j+2:n, j) below, zero above; zero for j >= n-1).
+// Branch-free: every lane loads from a clamped in-bounds address (a 32-bit offset: the
+// callers have n, lda <= 4096) and selects zero above the reflector, the unit entry on it;
+// per-lane branches here cost an exec-mask save, branch and restore per load.
+// refl_raw: the (clamped) stored entry of reflector j at row; refl_sel: V(row, j) from it.
+template <typename T>
+__device__ inline cx<T> refl_raw(const T *A, int lda, int n, int j, int row) {
+    const int jc = min(j, max(n - 2, 0)), rc = min(max(row, 0), n - 1);
+    return ldx(A, jc * lda + rc);
+}
+template <typename T>
+__device__ inline cx<T> refl_sel(cx<T> x, int n, int j, int row) {
+    const bool live = j < n - 1;
+    const T re = (live && row > j + 1 && row < n) ? x.x : (live && row == j + 1 ? (T)1 : (T)0);
+    const T im = (live && row > j + 1 && row < n) ? x.y : (T)0;
+    return cx<T>{re, im};
+}
 template <typename T>
 __device__ inline cx<T> refl_get(const T *A, int lda, int n, int j0, int row, int k) {
+    return refl_sel(refl_raw(A, lda, n, j0 + k, row), n, j0 + k, row);
+}
+// The same by per-lane branches (exec-masked loads): for a register prefetch, whose value
+// is not used until later and which a select after the load would make wait for it.
+template <typename T>
+__device__ inline cx<T> refl_get_masked(const T *A, int lda, int n, int j0, int row, int k) {
     const int j = j0 + k;
     if (j >= n - 1 || row <= j || row >= n) return cx<T>{(T)0, (T)0};
     if (row == j + 1) return cx<T>{(T)1, (T)0};
-    return ldx(A, (long)j * lda + row);
+    return ldx(A, j * lda + row);
 }
 
 // Triangular factors of the compact-WY blocks (zlarft, forward, columnwise):
@@ -2832,7 +2854,7 @@ __global__ __launch_bounds__(NWV > 4 ? 512 : 256) __attribute__((amdgpu_waves_pe
             const int R = (w + NWV * i) * 16;
 #pragma unroll
             for (int kk = 0; kk < 4; kk++)
-                vpre[i][kk] = (R + 15 >= bk * 16 + 1) ? refl_get(A, lda, n, bk * 16, R + li, lq + 4 * kk)
+                vpre[i][kk] = (R + 15 >= bk * 16 + 1) ? refl_get_masked(A, lda, n, bk * 16, R + li, lq + 4 * kk)
                                                       : CT{(T)0, (T)0};
         }
     };
