@@ -20,7 +20,10 @@
 namespace rlm {
 
 // V[s][m][n] = coeff_mn[m][n] * sum_i s_i e^{-i 2pi((m-o) x_i + (n-o) y_i)}
-// grid (ceil(d/64), rows, S), block 256 = 64 n-values x 4 impurity slices.
+// grid (ceil(npair / 64), S), block 256 = 64 (m, n) pairs x 4 impurity slices; the pairs
+// computed are numbered p = m d + n (rows m < o in full, row o up to n = o with mirror,
+// every row without), so that every lane of a wave has a pair (64-column blocks of the
+// d = 73 columns at n_phi = 256 left 43 % of the lanes idle: 3.4 -> 2.0 ms).
 // mirror (d = 2o + 1): the phases of (m, n) and (2o - m, 2o - n) are exact negatives
 // (both products and their sum negate exactly), so the second sum is the conjugate of the
 // first (to the last bit of sincos's odd / even symmetry): rows m <= o are computed (row
@@ -29,16 +32,17 @@ namespace rlm {
 __global__ __launch_bounds__(256) void potential_kernel(const double *__restrict__ X, const double *__restrict__ Y,
                                                         const double *__restrict__ Sg, long n_imp,
                                                         const double *__restrict__ cmn, int d, int off,
-                                                        double *__restrict__ V, int mirror) {
+                                                        double *__restrict__ V, int mirror, int npair) {
     __shared__ double red[2 * 256];
-    const int s = blockIdx.z, m = blockIdx.y;
+    const int s = blockIdx.y;
     const int nl = threadIdx.x & 63, slice = threadIdx.x >> 6;
-    const int n = blockIdx.x * 64 + nl;
+    const int pr = blockIdx.x * 64 + nl;
+    const int m = pr / d, n = pr - m * d;
     const double PI2 = 6.283185307179586232;   // (M_PI*2.0) as a double
     const double kx = PI2 * (double)(m - off);
     const double ky = PI2 * (double)(n - off);
     const double *x = X + s * n_imp, *y = Y + s * n_imp, *sg = Sg + s * n_imp;
-    const bool act = n < d && (!mirror || m < off || n <= off);
+    const bool act = pr < npair;
     double re = 0.0, im = 0.0;
     if (act) {
         for (long i = slice; i < n_imp; i += 4) {
@@ -139,8 +143,9 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double *__restrict_
 int launch_potential(const double *X, const double *Y, const double *S, long n_imp, const double *cmn, int d, int off,
                      double *V, int n_samples, hipStream_t stream) {
     const int mirror = d == 2 * off + 1 ? 1 : 0;
-    dim3 grid((d + 63) / 64, mirror ? off + 1 : d, n_samples);
-    hipLaunchKernelGGL(potential_kernel, grid, dim3(256), 0, stream, X, Y, S, n_imp, cmn, d, off, V, mirror);
+    const int npair = mirror ? off * d + off + 1 : d * d;
+    dim3 grid((npair + 63) / 64, n_samples);
+    hipLaunchKernelGGL(potential_kernel, grid, dim3(256), 0, stream, X, Y, S, n_imp, cmn, d, off, V, mirror, npair);
     RLM_HIP_CHECK(hipGetLastError());
     return 0;
 }
