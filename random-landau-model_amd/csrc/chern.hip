@@ -266,8 +266,9 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ 
             }
         }
         e /= (double)(g1 * g1);
-        // plaquettes (a = theta_1 - 1, b = m) in the reference's order; the four links of
-        // the next plaquette load while the current one's phase is taken
+        // plaquettes (a = theta_1 - 1, b = m) in the reference's order, four at a time: the
+        // sixteen links of four plaquettes load together, then their phases are taken in
+        // order (one load latency per four plaquettes)
         auto links = [&](int a, int b, cd &l1, cd &l2, cd &l3, cd &l4) {
             const long k00 = kbase + (long)a * L2n + (b % L2n);
             const long k10 = k00 + L2n;
@@ -278,22 +279,25 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ 
             l4 = ld(Lk2, k00 * ns + j);     // conj: <01|00>
         };
         double c = 0.0, mg = 4.0;
-        cd n1, n2, n3, n4;
-        if (g > 0) links(0, 0, n1, n2, n3, n4);
         for (int a = 0; a < g; a++) {
-            for (int b = 0; b < g; b++) {
-                const cd l1 = n1, l2 = n2, l3 = cconj(n3), l4 = cconj(n4);
-                if (b + 1 < g) links(a, b + 1, n1, n2, n3, n4);
-                else if (a + 1 < g) links(a + 1, 0, n1, n2, n3, n4);
-                cd prod{1.0, 0.0};
-                prod = cmul(prod, l1);
-                prod = cmul(prod, l2);
-                prod = cmul(prod, l3);
-                prod = cmul(prod, l4);
-                const double ph = atan2(prod.y, prod.x);
-                const double tth = -ph / TWO_PI;
-                if (mode != 1) c += tth; else c += -tth;
-                mg = fmin(mg, PI - fabs(ph));
+            for (int b0 = 0; b0 < g; b0 += 4) {
+                cd L[4][4];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (b0 + u < g) links(a, b0 + u, L[u][0], L[u][1], L[u][2], L[u][3]);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (b0 + u >= g) break;
+                    cd prod{1.0, 0.0};
+                    prod = cmul(prod, L[u][0]);
+                    prod = cmul(prod, L[u][1]);
+                    prod = cmul(prod, cconj(L[u][2]));
+                    prod = cmul(prod, cconj(L[u][3]));
+                    const double ph = atan2(prod.y, prod.x);
+                    const double tth = -ph / TWO_PI;
+                    if (mode != 1) c += tth; else c += -tth;
+                    mg = fmin(mg, PI - fabs(ph));
+                }
             }
         }
         if (mode != 1) c += 1.0 / n_phi_d;

@@ -2759,14 +2759,20 @@ __global__ __launch_bounds__(256) void larft_kernel(const T *__restrict__ Abase,
     for (int bk = 4 * wq + w; bk < nblk; bk += 4 * wpm) {
         const int j0 = bk * 16;
         v4 gr = {0, 0, 0, 0}, gi = {0, 0, 0, 0};
-        // G[k1][k2] = sum_r conj(V[r][k1]) V[r][k2]; K-step over rows 4s.. (rows >= j0+1)
-        for (int r4 = j0 + 1 - ((j0 + 1) & 3); r4 < n; r4 += 4) {
-            const int r = r4 + lq;
-            const CT x = refl_get(A, lda, n, j0, r, li);   // A operand: conj(V)[k1=li][row]; B: V[row][k2=li]
-            gr = M::mma(x.x, x.x, gr);
-            gr = M::mma(x.y, x.y, gr);
-            gi = M::mma(x.x, x.y, gi);
-            gi = M::mma(-x.y, x.x, gi);
+        // G[k1][k2] = sum_r conj(V[r][k1]) V[r][k2]; K-steps over rows 4s.. (rows >= j0+1),
+        // four at a time so that their four (branch-free) loads are in flight together
+        for (int r16 = j0 + 1 - ((j0 + 1) & 3); r16 < n; r16 += 16) {
+            CT x[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                x[u] = refl_get(A, lda, n, j0, r16 + 4 * u + lq, li);   // zero at rows >= n
+#pragma unroll
+            for (int u = 0; u < 4; u++) {   // A operand: conj(V)[k1=li][row]; B: V[row][k2=li]
+                gr = M::mma(x[u].x, x[u].x, gr);
+                gr = M::mma(x[u].y, x[u].y, gr);
+                gi = M::mma(x[u].x, x[u].y, gi);
+                gi = M::mma(-x[u].y, x[u].x, gi);
+            }
         }
         // G to LDS (D layout: element v of lane l is G[row(l, v)][l%16])
 #pragma unroll
