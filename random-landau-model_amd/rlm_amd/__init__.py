@@ -323,6 +323,9 @@ class RlmMulti:
         self.loaded = 0
 
     def load(self, first_sample: int, count: int):
+        # a load that fails part-way leaves no count behind, so fetch() refuses rather than
+        # sizing its buffers from a stale one
+        self.loaded = 0
         _check(lib().rlm_multi_load(self._h, first_sample, count), "rlm_multi_load")
         self.loaded = count
 
@@ -330,6 +333,8 @@ class RlmMulti:
         _check(lib().rlm_multi_execute(self._h), "rlm_multi_execute")
 
     def fetch(self, k: int):
+        if self.loaded < 1:
+            raise HipDiagError("rlm_multi_fetch: no batch loaded")
         n = self.loaded * self.sizes[k]
         e, c, m = np.zeros(n), np.zeros(n), np.zeros(n)
         _check(lib().rlm_multi_fetch(self._h, k, _ptr(e), _ptr(c), _ptr(m)), "rlm_multi_fetch")
