@@ -173,6 +173,30 @@ def test_derived_edges_match_full_mesh(monkeypatch, n_phi, g, seed, precision):
           f"{np.abs(c - np.rint(c)).max():.3e}, full {np.abs(cf - np.rint(cf)).max():.3e}", flush=True)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n_phi,g,seed,S", [(64, 5, 11, 2), (300, 7, 3, 1), (64, 30, 11, 16)])
+def test_tiled_links_identical_to_per_point(monkeypatch, n_phi, g, seed, S, precision):
+    """links_tile_kernel reads the vector sets of a tile of B x B twist points (3 x 3; 4 x 4
+    for FP64 launches of at least four workgroups per CU, which n_phi = 64, g = 30, 16
+    samples reaches) and of its neighbours once per component row. Each link sums the
+    components in the same order as the one-point-per-workgroup kernel (RLM_LINKS_TILE=0),
+    so energies, Chern values and margins are bit-identical. The meshes of g + 1 = 6, 8 and
+    31 points leave partial tiles at the theta = 2 pi edges; n_phi = 300 takes two chunks
+    of 256 states."""
+    from rlm_amd import RlmPlan, HIP_DIAG_FP64, HIP_DIAG_MIXED
+    print(f"[test_tiled_links_identical_to_per_point] n={n_phi} g={g} seed={seed} S={S} {precision}", flush=True)
+    prec = HIP_DIAG_MIXED if precision == "mixed" else HIP_DIAG_FP64
+    out = []
+    for tile in ("", "0"):
+        if tile:
+            monkeypatch.setenv("RLM_LINKS_TILE", tile)
+        plan = RlmPlan(n_phi, 1.0, 16.0, g, seed, samples_per_batch=S, precision=prec)
+        out.append(plan.run(0, S))
+        plan.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b), np.abs(a - b).max()
+
+
 def test_mixed_pair_info_reports_runs():
     """rlm_plan_pair_info after a mixed execute at n_phi = 256 (seed 11 holds a
     near-degenerate pair at every twist point): runs found and solved, none dropped or
