@@ -42,18 +42,16 @@ static int chunks_per_matrix(int n) {
 // Fill the triangle opposite to `uplo` by conjugate mirroring; real diagonal.
 __global__ void hermitize_kernel(double *A, long lda, long strideA, int n, int from_lower, int nch) {
     double *M = A + 2 * (blockIdx.x / nch) * strideA;
-    const long step = (long)nch * blockDim.x;
-    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)n * n; idx += step) {
-        const int c = (int)(idx / n), r = (int)(idx % n);
+    for_2d((int)(blockIdx.x % nch) * (int)blockDim.x + (int)threadIdx.x, nch * (int)blockDim.x, n, n, [&](int c, int r) {
         if (r == c) {
             M[2 * ((long)c * lda + r) + 1] = 0.0;
-            continue;
+            return;
         }
         const bool dst = from_lower ? (r < c) : (r > c);
-        if (!dst) continue;
+        if (!dst) return;
         const cd v = ld(M, (long)r * lda + c);  // (c, r) element
         st(M, (long)c * lda + r, cconj(v));
-    }
+    });
 }
 
 // A(r, j) = U[r*n + j] (component-major -> column-major eigenvectors)
@@ -62,11 +60,8 @@ __global__ void transpose_out_kernel(const double *U, long strideU, int n, doubl
     const long b = blockIdx.x / nch;
     const double *Ub = U + 2 * b * strideU;
     double *M = A + 2 * b * strideA;
-    const long step = (long)nch * blockDim.x;
-    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)n * n; idx += step) {
-        const int j = (int)(idx / n), r = (int)(idx % n);
-        st(M, (long)j * lda + r, ld(Ub, (long)r * n + j));
-    }
+    for_2d((int)(blockIdx.x % nch) * (int)blockDim.x + (int)threadIdx.x, nch * (int)blockDim.x, n, n,
+           [&](int j, int r) { st(M, (long)j * lda + r, ld(Ub, (long)r * n + j)); });
 }
 
 // Full-matrix copy with lda (column-major) -> packed lda = n
@@ -74,11 +69,8 @@ __global__ void pack_kernel(const double *A, long lda, long strideA, int n, doub
     const long b = blockIdx.x / nch;
     const double *M = A + 2 * b * strideA;
     double *Q = P + 2 * b * n * n;
-    const long step = (long)nch * blockDim.x;
-    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)n * n; idx += step) {
-        const int c = (int)(idx / n), r = (int)(idx % n);
-        st(Q, idx, ld(M, (long)c * lda + r));
-    }
+    for_2d((int)(blockIdx.x % nch) * (int)blockDim.x + (int)threadIdx.x, nch * (int)blockDim.x, n, n,
+           [&](int c, int r) { st(Q, (long)c * n + r, ld(M, (long)c * lda + r)); });
 }
 
 // FP64 -> FP32 copy (count values)
@@ -153,12 +145,10 @@ __global__ void transpose_out_f32_kernel(const float *U, long strideU, int n, do
     const long b = blockIdx.x / nch;
     const float *Ub = U + 2 * b * strideU;
     double *M = A + 2 * b * strideA;
-    const long step = (long)nch * blockDim.x;
-    for (long idx = (blockIdx.x % nch) * (long)blockDim.x + threadIdx.x; idx < (long)n * n; idx += step) {
-        const int j = (int)(idx / n), r = (int)(idx % n);
+    for_2d((int)(blockIdx.x % nch) * (int)blockDim.x + (int)threadIdx.x, nch * (int)blockDim.x, n, n, [&](int j, int r) {
         const cx<float> u = ldx(Ub, (long)r * n + j);
         st(M, (long)j * lda + r, cd{(double)u.x, (double)u.y});
-    }
+    });
 }
 
 // ---------------------------------------------------------------- device buffer
