@@ -521,7 +521,25 @@ __device__ void pair_rr_two(const double *Ak, const double *Hk, float *Uk, doubl
             y[a] = cd{0.0, 0.0};
         }
         const int nc = dense ? n - r : bw + 1;
-        for (int k = 0; k < nc; k++) {
+        // band form: entries (r, r + k) with r + k < n are conj(H[r][k]), a contiguous run
+        // of this row, read without band_entry's branches so that the loads batch; the same
+        // terms in the same order as band_entry gives them
+        const int k1 = Hk ? min(nc, n - r) : 0;
+        const cd *hr = Hk ? reinterpret_cast<const cd *>(Hk) + (long)r * nb : nullptr;
+        int k = 0;
+        if (k1 > 0) {
+            const cd arc{0.5 * hr[0].x, 0.0};
+#pragma unroll
+            for (int a = 0; a < 2; a++) cfma(y[a], arc, cd{ust[a][r].x, ust[a][r].y});
+            k = 1;
+        }
+#pragma unroll 4
+        for (; k < k1; k++) {
+            const cd arc = cconj(hr[k]);
+#pragma unroll
+            for (int a = 0; a < 2; a++) cfma(y[a], arc, cd{ust[a][r + k].x, ust[a][r + k].y});
+        }
+        for (; k < nc; k++) {
             const int c = r + k >= n ? r + k - n : r + k;
             cd arc = band_entry(Ak, Hk, n, bw, nb, r, c);
             if (k == 0) arc = cd{0.5 * arc.x, 0.0};
