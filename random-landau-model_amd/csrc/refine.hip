@@ -73,17 +73,36 @@ __global__ __launch_bounds__(256) void ref_apply_kernel(const double *__restrict
         v4 yq = {0.0, 0.0, 0.0, 0.0}, yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
         const int cs = dense ? 0 : r0 - bw;
         const int r = r0 + li;
-        for (int k0 = 0; k0 < kw; k0 += 4) {
-            const int kk = k0 + lq;
-            int c = cs + kk;
-            c = c < 0 ? c + n : (c >= n ? c - n : c);
-            const bool kin = kk < kw;
-            const cd a = (kin && r < n) ? band_entry(nullptr, Hk, n, bw, nb, r, c) : cd{0.0, 0.0};
-            const cx<TX> u = (kin && jin) ? ldx(Xk, (long)c * n + j) : cx<TX>{(TX)0, (TX)0};
-            const double ur = u.x, ui = u.y;
-            yq = M::mma(a.x + a.y, ur, yq);
-            yr = M::mma(-a.y, ur + ui, yr);
-            yi = M::mma(a.x, ui - ur, yi);
+        // four K-steps at a time: the band entry's offset and mode (0: outside the band,
+        // 1: as stored, 2: conjugate; band_entry's formulas) and every load first, the
+        // modes applied as each step's products issue (a gather per step, used at once,
+        // left each step waiting for its loads)
+        for (int k0 = 0; k0 < kw; k0 += 16) {
+            cd araw[4];
+            int md[4];
+            cx<TX> u[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++) {
+                const int kk = k0 + 4 * s4 + lq;
+                int c = cs + kk;
+                c = c < 0 ? c + n : (c >= n ? c - n : c);
+                const bool kin = kk < kw;
+                const bool up = r <= c;
+                const int D = up ? c - r : r - c;
+                const int b = (2 * bw + 1 >= n || D <= bw) ? D : (D >= n - bw ? D - n + 2 * bw + 1 : -1);
+                const bool ok = kin && r < n && b >= 0;
+                araw[s4] = ld(Hk, ok ? (long)(up ? r : c) * nb + b : 0);
+                md[s4] = ok ? (up ? 2 : 1) : 0;
+                u[s4] = (kin && jin) ? ldx(Xk, (long)c * n + j) : cx<TX>{(TX)0, (TX)0};
+            }
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++) {
+                const cd a = md[s4] == 0 ? cd{0.0, 0.0} : (md[s4] == 2 ? cconj(araw[s4]) : araw[s4]);
+                const double ur = u[s4].x, ui = u[s4].y;
+                yq = M::mma(a.x + a.y, ur, yq);
+                yr = M::mma(-a.y, ur + ui, yr);
+                yi = M::mma(a.x, ui - ur, yi);
+            }
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
