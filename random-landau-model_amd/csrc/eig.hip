@@ -1191,9 +1191,10 @@ struct ResidentMfma {
         float *yf = reinterpret_cast<float *>(yw);
         const int e = 2 * (R * 32 + 16 * (li >> 3) + 4 * lq + ((li >> 1) & 3)) + (li & 1);
         const float yr0 = DG ? 0.f : yf[e];
-        CT yc0[2];
-#pragma unroll
-        for (int q = Q0; q < 2; q++) yc0[q] = yw[C * 32 + 16 * q + li];
+        // the column sums end reduce-scattered: lane row lq holds component lq & 1 of column
+        // half Q0 + (lq >> 1) (Q0 = 1: rows 2 and 3 repeat rows 0 and 1, entries and values)
+        const int ec = 2 * (C * 32 + 16 * (Q0 + ((lq >> 1) & (1 - Q0))) + li) + (lq & 1);
+        const float yc0 = yf[ec];
         CT vr[2][4];
 #pragma unroll
         for (int p = 0; p < 2; p++)
@@ -1219,16 +1220,35 @@ struct ResidentMfma {
                 }
             yf[e] = yr0 + reduce_scatter16(x, li);
         }
+        CT cs[2];
 #pragma unroll
         for (int q = Q0; q < 2; q++) {
-            CT cs{0.f, 0.f};
+            cs[q] = CT{0.f, 0.f};
 #pragma unroll
             for (int p = DG ? Q0 : 0; p < 2; p++)
 #pragma unroll
-                for (int u = 0; u < 4; u++) fmacx(cs, CT{t.re[p][q][u], t.im[p][q][u]}, vr[p][u]);
-            cs.x = sum_xor32(sum_xor16(cs.x));
-            cs.y = sum_xor32(sum_xor16(cs.y));
-            yw[C * 32 + 16 * q + li] = yc0[q] + cs;   // the 4 lanes of a column hold the same sum
+                for (int u = 0; u < 4; u++) fmacx(cs[q], CT{t.re[p][q][u], t.im[p][q][u]}, vr[p][u]);
+        }
+        // sums over the four 16-lane rows, two values per swap: permlane16_swap(x, y) leaves
+        // row pairs (x0 + x1, y0 + y1, x2 + x3, y2 + y3), permlane32_swap(a, b) then
+        // (a0 + a2, a1 + a3, b0 + b2, b1 + b3): the additions of sum_xor32(sum_xor16(.)) in the
+        // same order, with three swaps for the four components instead of eight
+        auto pair16 = [](float x, float y) {
+            const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+            return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        };
+        auto pair32 = [](float a, float b) {
+            const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+            return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        };
+        if constexpr (Q0 == 0) {
+            yf[ec] = yc0 + pair32(pair16(cs[0].x, cs[0].y), pair16(cs[1].x, cs[1].y));
+        } else {
+            // every lane stores (rows 2 and 3 the same values to the same entries as rows 0
+            // and 1): a store under a lane condition would let the swap sink into a branch
+            // whose exec mask hides the lanes it reads
+            const float s = pair16(cs[1].x, cs[1].y);
+            yf[ec] = yc0 + pair32(s, s);
         }
     }
     static __device__ __forceinline__ void product(Tile &t, bool diag, int R, int C, int lane, int i1, const CT *vn,
