@@ -1048,13 +1048,49 @@ struct ResidentSub4 {
 // partner's copy of that set: bit 3 by a row rotation of 8 (xor 8), bit 2 by the
 // half-row mirror (xor 7), bits 1 and 0 by quad permutations (xor 2, xor 1). The four
 // partner sets together reach all 16 lanes of the row.
+// The first two steps split the row at 4-lane bank boundaries (bit 3: lanes 0-7 keep
+// x[k], lanes 8-15 x[8 + k]; bit 2: banks 0 and 2 keep y[k], banks 1 and 3 y[4 + k]), so
+// each kept value is one DPP add restricted by bank_mask (both halves of y[k] from two
+// adds) instead of two selects and an add: own + partner in the same order as the select
+// form, all lanes active. One s_nop 1 first covers the DPP read-after-VALU-write hazard on
+// x; the second step's DPP reads of y come at least eight instructions after their writes.
+__device__ __forceinline__ void reduce_scatter16_b32(const float (&x)[16], float (&z)[4]) {
+    float y[8];
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_add_f32_dpp %0, %12, %12 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+        "v_add_f32_dpp %0, %20, %20 row_ror:8 row_mask:0xf bank_mask:0xc\n\t"
+        "v_add_f32_dpp %1, %13, %13 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+        "v_add_f32_dpp %1, %21, %21 row_ror:8 row_mask:0xf bank_mask:0xc\n\t"
+        "v_add_f32_dpp %2, %14, %14 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+        "v_add_f32_dpp %2, %22, %22 row_ror:8 row_mask:0xf bank_mask:0xc\n\t"
+        "v_add_f32_dpp %3, %15, %15 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+        "v_add_f32_dpp %3, %23, %23 row_ror:8 row_mask:0xf bank_mask:0xc\n\t"
+        "v_add_f32_dpp %4, %16, %16 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+        "v_add_f32_dpp %4, %24, %24 row_ror:8 row_mask:0xf bank_mask:0xc\n\t"
+        "v_add_f32_dpp %5, %17, %17 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+        "v_add_f32_dpp %5, %25, %25 row_ror:8 row_mask:0xf bank_mask:0xc\n\t"
+        "v_add_f32_dpp %6, %18, %18 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+        "v_add_f32_dpp %6, %26, %26 row_ror:8 row_mask:0xf bank_mask:0xc\n\t"
+        "v_add_f32_dpp %7, %19, %19 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+        "v_add_f32_dpp %7, %27, %27 row_ror:8 row_mask:0xf bank_mask:0xc\n\t"
+        "v_add_f32_dpp %8, %0, %0 row_half_mirror row_mask:0xf bank_mask:0x5\n\t"
+        "v_add_f32_dpp %8, %4, %4 row_half_mirror row_mask:0xf bank_mask:0xa\n\t"
+        "v_add_f32_dpp %9, %1, %1 row_half_mirror row_mask:0xf bank_mask:0x5\n\t"
+        "v_add_f32_dpp %9, %5, %5 row_half_mirror row_mask:0xf bank_mask:0xa\n\t"
+        "v_add_f32_dpp %10, %2, %2 row_half_mirror row_mask:0xf bank_mask:0x5\n\t"
+        "v_add_f32_dpp %10, %6, %6 row_half_mirror row_mask:0xf bank_mask:0xa\n\t"
+        "v_add_f32_dpp %11, %3, %3 row_half_mirror row_mask:0xf bank_mask:0x5\n\t"
+        "v_add_f32_dpp %11, %7, %7 row_half_mirror row_mask:0xf bank_mask:0xa"
+        : "=&v"(y[0]), "=&v"(y[1]), "=&v"(y[2]), "=&v"(y[3]), "=&v"(y[4]), "=&v"(y[5]), "=&v"(y[6]),
+          "=&v"(y[7]), "=&v"(z[0]), "=&v"(z[1]), "=&v"(z[2]), "=&v"(z[3])
+        : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]), "v"(x[8]),
+          "v"(x[9]), "v"(x[10]), "v"(x[11]), "v"(x[12]), "v"(x[13]), "v"(x[14]), "v"(x[15]));
+}
 __device__ __forceinline__ float reduce_scatter16(const float (&x)[16], int li) {
-    const bool b3 = (li & 8) != 0, b2 = (li & 4) != 0, b1 = (li & 2) != 0, b0 = (li & 1) != 0;
-    float y[8], z[4], w[2];
-#pragma unroll
-    for (int k = 0; k < 8; k++) y[k] = (b3 ? x[8 + k] : x[k]) + dpp_mov<0x128>(b3 ? x[k] : x[8 + k]);
-#pragma unroll
-    for (int k = 0; k < 4; k++) z[k] = (b2 ? y[4 + k] : y[k]) + dpp_mov<0x141>(b2 ? y[k] : y[4 + k]);
+    const bool b1 = (li & 2) != 0, b0 = (li & 1) != 0;
+    float z[4], w[2];
+    reduce_scatter16_b32(x, z);
 #pragma unroll
     for (int k = 0; k < 2; k++) w[k] = (b1 ? z[2 + k] : z[k]) + dpp_mov<0x4E>(b1 ? z[k] : z[2 + k]);
     return (b0 ? w[1] : w[0]) + dpp_mov<0xB1>(b0 ? w[0] : w[1]);
